@@ -1,0 +1,157 @@
+/*
+ * vcphip.h -- C ABI of libvcphip, the MI355X (gfx950) verified dense matrix
+ * product behind the vcp::pidhip / vcp::pddhip policies.
+ *
+ * Plain pointers and sizes only.  Matrices are column-major as in vcp::mats
+ * (vcp/mats.hpp:411-417).  An interval element is the AoS pair {inf, sup}
+ * (kv::interval<double>, the order of vcp/vcp_fio.hpp:269-272); a
+ * double-double element is the AoS pair {hi, lo} (kv::dd, vcp/pddblas.hpp:97).
+ * Leading dimensions count ELEMENTS (an interval or dd element is two
+ * doubles).  Every result is an enclosure: lower bounds are rounded toward
+ * -inf and upper bounds toward +inf on the GPU; the caller's floating-point
+ * environment is never touched (the reference restores round-to-nearest
+ * with vcp::hwround_guard, vcp/hwround_guard.hpp:39-48; this library never
+ * changes the host rounding mode at all).
+ *
+ * Entry points without a suffix take HOST pointers and block until the
+ * result is in host memory.  The *_dev variants take DEVICE pointers, are
+ * asynchronous on the context's stream and allocate nothing but the
+ * context's cached workspace.
+ *
+ * Every function returns VCPHIP_OK (0) or a negative VCPHIP_ERR_* code;
+ * vcphip_strerror() names it.  vcp/error.hpp:42-114 is the exception
+ * hierarchy the C++ policy headers map these codes onto.
+ */
+#ifndef VCPHIP_H
+#define VCPHIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vcphip_ctx vcphip_ctx;
+
+/* status codes (mapped to vcp::dimension_error, invalid_argument,
+ * backend_error, verification_error ... by include/vcp/pidhip.hpp) */
+enum {
+    VCPHIP_OK = 0,
+    VCPHIP_ERR_DIMENSION = -1,    /* vcp::dimension_error  (pdblas.hpp:113-115)  */
+    VCPHIP_ERR_ARGUMENT = -2,     /* vcp::invalid_argument                       */
+    VCPHIP_ERR_DEVICE = -3,       /* vcp::backend_error("hip", ...)              */
+    VCPHIP_ERR_OOM = -4,          /* vcp::backend_error("hip", ...)              */
+    VCPHIP_ERR_ROUNDING = -5,     /* directed rounding not in effect             */
+    VCPHIP_ERR_VERIFICATION = -6, /* vcp::verification_error (imats.hpp:333-336) */
+    VCPHIP_ERR_UNSUPPORTED = -7,  /* size or form outside what the kernels cover */
+    VCPHIP_ERR_RANGE = -8         /* exponent range outside the supported one    */
+};
+
+/* forms of the interval x interval product */
+enum {
+    VCPHIP_FORM_PLAIN = 0, /* FP64 directed-rounding chains (pidblas op order) */
+    VCPHIP_FORM_OZAKI = 1  /* Ozaki scheme II on int8 MFMA + rigorous bounds   */
+};
+
+/* context parameters */
+enum {
+    VCPHIP_PARAM_OZ_NMOD = 1,      /* moduli of the Ozaki form (0 = tuned/default) */
+    VCPHIP_PARAM_OZ_VARIANT = 2,   /* int8 GEMM tile variant (0 = tuned/default)  */
+    VCPHIP_PARAM_STAGE_TIMING = 3, /* 1: record per-stage times with HIP events  */
+    VCPHIP_PARAM_DD_NMOD = 4       /* moduli of the dd product (0 = default)      */
+};
+
+/* ---- context ------------------------------------------------------------ */
+/* One context per (host thread, device); it owns a stream and a workspace.
+ * Replaces the implicit per-thread state of the reference (MXCSR rounding
+ * mode + BLAS thread pool), vcp/pidblas.hpp:116-134. */
+int vcphip_ctx_create(int device, vcphip_ctx** out);
+int vcphip_ctx_destroy(vcphip_ctx* ctx);
+/* run *_dev work on a caller stream (a hipStream_t); NULL restores the
+ * context's own stream */
+int vcphip_ctx_set_stream(vcphip_ctx* ctx, void* stream);
+int vcphip_ctx_synchronize(vcphip_ctx* ctx);
+int vcphip_ctx_set_param(vcphip_ctx* ctx, int key, long long value);
+int vcphip_ctx_get_param(vcphip_ctx* ctx, int key, long long* value);
+/* average time (ms) per call of each stage, over the calls made since
+ * STAGE_TIMING was last set (events are read at vcphip_ctx_synchronize);
+ * stage names are written as a ';'-separated list into names (if given). */
+int vcphip_ctx_stage_times(vcphip_ctx* ctx, double* ms, int max, int* count, char* names, int names_len);
+const char* vcphip_strerror(int code);
+int vcphip_version(void);
+/* number of moduli and scale exponent the Ozaki form uses for (k, nmod) */
+int vcphip_ozaki_config(int k, int nmod, int* beta, int* moduli, int max_moduli);
+
+/* ---- interval x interval: C = IA * IB ----------------------------------- */
+/* Replaces vcp::pidblas::mulmm (vcp/pidblas.hpp:239-294), i.e. the four
+ * dgemm_ calls at :266,271,279,281 under kv::hwround and the n^2 fix-ups. */
+int vcphip_igemm(vcphip_ctx* ctx, int form, int m, int n, int k,
+                 const double* IA, int lda, const double* IB, int ldb, double* C, int ldc);
+int vcphip_igemm_dev(vcphip_ctx* ctx, int form, int m, int n, int k,
+                     const double* IA, int lda, const double* IB, int ldb, double* C, int ldc);
+
+/* ---- interval x point: C = IA * B ---------------------------------------- */
+/* Replaces vcp::pidblas::mul_im_m (vcp/pidblas.hpp:90-140). */
+int vcphip_igemm_ip(vcphip_ctx* ctx, int m, int n, int k,
+                    const double* IA, int lda, const double* B, int ldb, double* C, int ldc);
+int vcphip_igemm_ip_dev(vcphip_ctx* ctx, int m, int n, int k,
+                        const double* IA, int lda, const double* B, int ldb, double* C, int ldc);
+
+/* ---- point x interval: C = B * IA ---------------------------------------- */
+/* Replaces vcp::pidblas::mul_m_im (vcp/pidblas.hpp:142-196); B is m x k. */
+int vcphip_igemm_pi(vcphip_ctx* ctx, int m, int n, int k,
+                    const double* B, int ldb, const double* IA, int lda, double* C, int ldc);
+int vcphip_igemm_pi_dev(vcphip_ctx* ctx, int m, int n, int k,
+                        const double* B, int ldb, const double* IA, int lda, double* C, int ldc);
+
+/* ---- interval A^T A: C = IA^T IA (IA is k x n) --------------------------- */
+/* Replaces vcp::pidblas::mulltmm (vcp/pidblas.hpp:296-358). */
+int vcphip_igemm_tn(vcphip_ctx* ctx, int k, int n, const double* IA, int lda, double* C, int ldc);
+int vcphip_igemm_tn_dev(vcphip_ctx* ctx, int k, int n, const double* IA, int lda, double* C, int ldc);
+
+/* ---- verified point A^T A: C = [RD(A^T A), RU(A^T A)] -------------------- */
+/* Replaces vcp::pidblas::vmulmm (vcp/pidblas.hpp:198-236). */
+int vcphip_vgemm_tn(vcphip_ctx* ctx, int k, int n, const double* A, int lda, double* C, int ldc);
+int vcphip_vgemm_tn_dev(vcphip_ctx* ctx, int k, int n, const double* A, int lda, double* C, int ldc);
+
+/* ---- point GEMM rounded up and down in one pass -------------------------- */
+/* Cup = RU(A*B), Cdn = RD(A*B) as k-ascending rounded FMA chains.  Replaces
+ * rmatmul(m,k,n,A,B,C,+1) and (...,-1) (vblas/rmatmul.hpp:100-127; note the
+ * reference names the inner dimension n) and, with accumulate = 1, the
+ * accumulating udmatmul (vblas/udmatmul.hpp:738-760); also the pdblas::mulmm
+ * dgemm_ under roundup()/rounddown() (vcp/pdblas.hpp:55-117). */
+int vcphip_gemm_rurd(vcphip_ctx* ctx, int m, int n, int k, const double* A, int lda,
+                     const double* B, int ldb, double* Cup, double* Cdn, int ldc, int accumulate);
+int vcphip_gemm_rurd_dev(vcphip_ctx* ctx, int m, int n, int k, const double* A, int lda,
+                         const double* B, int ldb, double* Cup, double* Cdn, int ldc, int accumulate);
+
+/* ---- double-double GEMM: C = A * B --------------------------------------- */
+/* Replaces vcp::pddblas::mulmm (vcp/pddblas.hpp:318-376), the Ozaki-scheme
+ * kv::dd product (split_row/split_col/ozaki_acc, :82-193), computed here by
+ * an Ozaki scheme II on int8 MFMA; the result is the nearest dd of a
+ * product accurate to ~2^-106 relative to the row/column scales. */
+int vcphip_ddgemm(vcphip_ctx* ctx, int m, int n, int k, const double* A, int lda,
+                  const double* B, int ldb, double* C, int ldc);
+int vcphip_ddgemm_dev(vcphip_ctx* ctx, int m, int n, int k, const double* A, int lda,
+                      const double* B, int ldb, double* C, int ldc);
+/* C = A^T A for dd A (k x n), exactly symmetric.  Replaces
+ * vcp::pddblas::mulltmm (vcp/pddblas.hpp:379-409). */
+int vcphip_ddgemm_tn(vcphip_ctx* ctx, int k, int n, const double* A, int lda, double* C, int ldc);
+
+/* ---- batched verified residual: C_i = R_i * IA_i - I ---------------------- */
+/* For each problem i (square, size n[i]): C_i = R_i*IA_i - I as intervals and
+ * normub[i] >= ||C_i||_inf.  Replaces the R*A - I segment of
+ * imats::linearsolve (vcp/imats.hpp:318-336: mul_m_im, the diagonal
+ * subtraction and mats::norminf, vcp/mats.hpp:1583-1600).  Pointer arrays
+ * are host arrays of device pointers for the _dev form. */
+int vcphip_residual_batched(vcphip_ctx* ctx, int count, const int* n,
+                            const double* const* R, const int* ldr,
+                            const double* const* IA, const int* lda,
+                            double* const* C, const int* ldc, double* normub);
+int vcphip_residual_batched_dev(vcphip_ctx* ctx, int count, const int* n,
+                                const double* const* R, const int* ldr,
+                                const double* const* IA, const int* lda,
+                                double* const* C, const int* ldc, double* normub_dev);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VCPHIP_H */

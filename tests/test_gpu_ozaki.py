@@ -1,0 +1,85 @@
+"""Parity of the Ozaki-II int8-MFMA interval GEMM (HIP, through the C ABI)
+with the oracle's step-by-step restatement: the exact scaled-integer
+product rebuilt from the residues must give bit-identical enclosures.
+Containment is checked against the exact hull; radii against the plain
+FP64 directed-rounding product."""
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+import vcp_amd
+from vcp_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _iv(rng, r, c, kind):
+    if kind == "uniform":
+        return W.uniform_intervals(rng, r, c, -40)
+    if kind == "wide":
+        return W.wide_exponent_intervals(rng, r, c)
+    if kind == "point":
+        x = W.uniform_intervals(rng, r, c, -40)
+        x[..., 1] = x[..., 0]
+        return x
+    if kind == "zerorows":
+        x = W.wide_exponent_intervals(rng, r, c)
+        x[::3, :, :] = 0.0
+        x[:, ::4, :] = 0.0
+        return x
+    if kind == "straddle":
+        mid = rng.uniform(-1, 1, (r, c))
+        return W.interval_from_midrad(mid, np.abs(rng.uniform(-1, 1, (r, c))))
+    raise ValueError(kind)
+
+
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (16, 16, 64), (70, 33, 100), (256, 256, 256), (300, 257, 513)])
+@pytest.mark.parametrize("kind", ["uniform", "wide", "point", "zerorows", "straddle"])
+def test_ozaki_bitexact(ctx, m, n, k, kind):
+    rng = np.random.default_rng(hash((m, n, k, kind, "oz")) % 2 ** 32)
+    IA, IB = _iv(rng, m, k, kind), _iv(rng, k, n, kind)
+    C = ctx.igemm(IA, IB, vcp_amd.FORM_OZAKI)
+    R = O.ozaki_igemm(IA, IB, 16)
+    bad = np.argwhere(~((C == R).all(axis=2)))
+    assert len(bad) == 0, "first mismatches %s gpu %s oracle %s" % (bad[:3].tolist(), C[tuple(bad[0])], R[tuple(bad[0])])
+
+
+@pytest.mark.parametrize("nmod", [12, 14, 18, 20])
+def test_ozaki_nmod_bitexact(ctx, nmod):
+    rng = np.random.default_rng(nmod)
+    IA, IB = _iv(rng, 128, 200, "wide"), _iv(rng, 200, 96, "wide")
+    ctx.set_param(vcp_amd.PARAM_OZ_NMOD, nmod)
+    try:
+        C = ctx.igemm(IA, IB, vcp_amd.FORM_OZAKI)
+    finally:
+        ctx.set_param(vcp_amd.PARAM_OZ_NMOD, 0)
+    assert np.array_equal(C, O.ozaki_igemm(IA, IB, nmod))
+
+
+@pytest.mark.parametrize("variant", [1, 2])
+def test_ozaki_tile_variants_agree(ctx, variant):
+    rng = np.random.default_rng(99)
+    IA, IB = _iv(rng, 300, 520, "wide"), _iv(rng, 520, 280, "wide")
+    ctx.set_param(vcp_amd.PARAM_OZ_VARIANT, variant)
+    try:
+        C = ctx.igemm(IA, IB, vcp_amd.FORM_OZAKI)
+    finally:
+        ctx.set_param(vcp_amd.PARAM_OZ_VARIANT, 0)
+    assert np.array_equal(C, O.ozaki_igemm(IA, IB, 16))
+
+
+@pytest.mark.parametrize("kind", ["wide", "straddle"])
+def test_ozaki_containment_and_radius(ctx, kind):
+    rng = np.random.default_rng(5)
+    m, n, k = 128, 96, 1024
+    IA, IB = _iv(rng, m, k, kind), _iv(rng, k, n, kind)
+    C = ctx.igemm(IA, IB, vcp_amd.FORM_OZAKI)
+    P = ctx.igemm(IA, IB, vcp_amd.FORM_PLAIN)
+    ii = rng.integers(0, m, 512)
+    jj = rng.integers(0, n, 512)
+    ok, _, _ = O.check_containment(IA, IB, C, ii, jj)
+    assert ok.all()
+    rad_o = (C[..., 1] - C[..., 0])
+    rad_p = (P[..., 1] - P[..., 0])
+    ratio = np.max(rad_o / np.maximum(rad_p, np.finfo(float).tiny))
+    assert ratio <= 2.0, ratio

@@ -1,0 +1,310 @@
+"""vcp_amd -- MI355X-native verified dense matrix product for VCP.
+
+Python host binding of libvcphip (the C ABI in include/vcphip.h).  The
+library is the product: every call below goes to hand-written gfx950 HIP
+kernels; there is no CPU fallback, and a missing or unloadable library
+raises.
+
+Host arrays follow VCP's storage (vcp/mats.hpp:411-417): column-major; an
+interval matrix is a float64 array of shape (rows, cols, 2) whose memory is
+the AoS column-major sequence {inf, sup} of kv::interval<double>; a dd
+matrix is the same with {hi, lo} (kv::dd).  Use ``pairs``/``as_pairs`` to
+make one.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["lib", "Context", "VcphipError", "pairs", "as_pairs", "FORM_PLAIN", "FORM_OZAKI"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvcphip.so")
+
+FORM_PLAIN = 0
+FORM_OZAKI = 1
+PARAM_OZ_NMOD = 1
+PARAM_OZ_VARIANT = 2
+PARAM_STAGE_TIMING = 3
+PARAM_DD_NMOD = 4
+
+ERRORS = {
+    -1: "dimension", -2: "argument", -3: "device", -4: "oom", -5: "rounding",
+    -6: "verification", -7: "unsupported", -8: "range",
+}
+
+_c_int = ctypes.c_int
+_c_ll = ctypes.c_longlong
+_vp = ctypes.c_void_p
+_p_int = ctypes.POINTER(ctypes.c_int)
+_p_dbl = ctypes.POINTER(ctypes.c_double)
+
+# exported symbols and their argument types (every symbol of include/vcphip.h)
+SIGNATURES = {
+    "vcphip_ctx_create": [_c_int, ctypes.POINTER(_vp)],
+    "vcphip_ctx_destroy": [_vp],
+    "vcphip_ctx_set_stream": [_vp, _vp],
+    "vcphip_ctx_synchronize": [_vp],
+    "vcphip_ctx_set_param": [_vp, _c_int, _c_ll],
+    "vcphip_ctx_get_param": [_vp, _c_int, ctypes.POINTER(_c_ll)],
+    "vcphip_ctx_stage_times": [_vp, _p_dbl, _c_int, _p_int, ctypes.c_char_p, _c_int],
+    "vcphip_strerror": [_c_int],
+    "vcphip_version": [],
+    "vcphip_ozaki_config": [_c_int, _c_int, _p_int, _p_int, _c_int],
+    "vcphip_igemm": [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
+    "vcphip_igemm_dev": [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
+    "vcphip_igemm_ip": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
+    "vcphip_igemm_ip_dev": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
+    "vcphip_igemm_pi": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
+    "vcphip_igemm_pi_dev": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
+    "vcphip_igemm_tn": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int],
+    "vcphip_igemm_tn_dev": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int],
+    "vcphip_vgemm_tn": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int],
+    "vcphip_vgemm_tn_dev": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int],
+    "vcphip_gemm_rurd": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _vp, _c_int, _c_int],
+    "vcphip_gemm_rurd_dev": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _vp, _c_int, _c_int],
+    "vcphip_ddgemm": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
+    "vcphip_ddgemm_dev": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
+    "vcphip_ddgemm_tn": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int],
+    "vcphip_residual_batched": [_vp, _c_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _p_dbl],
+    "vcphip_residual_batched_dev": [_vp, _c_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _vp],
+}
+
+_lib = None
+
+
+class VcphipError(RuntimeError):
+    def __init__(self, code, where):
+        self.code = code
+        msg = lib().vcphip_strerror(code).decode() if _lib is not None else str(code)
+        super().__init__("%s failed: %s (%d)" % (where, msg, code))
+
+
+def lib():
+    """Load libvcphip.so (built by vcp_amd.build); raise if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libvcphip.so not built (run `python -m vcp_amd.build`): %s" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_char_p if name == "vcphip_strerror" else ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc, where):
+    if rc != 0:
+        raise VcphipError(rc, where)
+
+
+def pairs(rows, cols):
+    """zeroed (rows, cols, 2) float64 array in AoS column-major layout."""
+    return np.zeros((cols, rows, 2), dtype=np.float64).transpose(1, 0, 2)
+
+
+def as_pairs(x):
+    out = pairs(x.shape[0], x.shape[1])
+    out[...] = x
+    return out
+
+
+def _pair_ptr(a):
+    """pointer and leading dimension of an AoS column-major pair array"""
+    assert a.dtype == np.float64 and a.ndim == 3 and a.shape[2] == 2
+    if a.size == 0:
+        return a.ctypes.data_as(_vp), max(a.shape[0], 1)
+    assert a.strides[2] == 8 and a.strides[0] == 16, "not an AoS column-major pair array"
+    ld = a.strides[1] // 16 if a.shape[1] > 1 else max(a.shape[0], 1)
+    return a.ctypes.data_as(_vp), max(ld, 1)
+
+
+def _mat_ptr(a):
+    assert a.dtype == np.float64 and a.ndim == 2 and (a.flags.f_contiguous or a.size == 0)
+    return a.ctypes.data_as(_vp), max(a.shape[0], 1)
+
+
+def ozaki_config(k, nmod=0):
+    beta = ctypes.c_int()
+    mods = (ctypes.c_int * 64)()
+    n = lib().vcphip_ozaki_config(k, nmod, ctypes.byref(beta), mods, 64)
+    if n < 0:
+        raise VcphipError(n, "vcphip_ozaki_config")
+    return beta.value, list(mods)[:n]
+
+
+class Context:
+    """A libvcphip context on one device (one stream, cached workspace)."""
+
+    def __init__(self, device=0):
+        self._h = _vp()
+        _check(lib().vcphip_ctx_create(device, ctypes.byref(self._h)), "vcphip_ctx_create")
+
+    def close(self):
+        if self._h:
+            lib().vcphip_ctx_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_param(self, key, value):
+        _check(lib().vcphip_ctx_set_param(self._h, key, int(value)), "vcphip_ctx_set_param")
+
+    def get_param(self, key):
+        v = _c_ll()
+        _check(lib().vcphip_ctx_get_param(self._h, key, ctypes.byref(v)), "vcphip_ctx_get_param")
+        return v.value
+
+    def set_stream(self, stream_ptr):
+        _check(lib().vcphip_ctx_set_stream(self._h, _vp(stream_ptr) if stream_ptr else None), "vcphip_ctx_set_stream")
+
+    def synchronize(self):
+        _check(lib().vcphip_ctx_synchronize(self._h), "vcphip_ctx_synchronize")
+
+    def stage_times(self):
+        ms = (ctypes.c_double * 32)()
+        cnt = ctypes.c_int()
+        names = ctypes.create_string_buffer(512)
+        _check(lib().vcphip_ctx_stage_times(self._h, ms, 32, ctypes.byref(cnt), names, 512), "stage_times")
+        nm = names.value.decode().split(";") if cnt.value else []
+        return dict(zip(nm, list(ms)[:cnt.value]))
+
+    # ---- host-pointer products --------------------------------------------
+    def igemm(self, IA, IB, form=FORM_PLAIN):
+        """C = IA * IB (pidblas::mulmm)."""
+        m, k = IA.shape[:2]
+        k2, n = IB.shape[:2]
+        if k != k2:
+            raise ValueError("dimension mismatch")
+        C = pairs(m, n)
+        pa, lda = _pair_ptr(IA)
+        pb, ldb = _pair_ptr(IB)
+        pc, ldc = _pair_ptr(C)
+        _check(lib().vcphip_igemm(self._h, form, m, n, k, pa, lda, pb, ldb, pc, ldc), "vcphip_igemm")
+        return C
+
+    def igemm_ip(self, IA, B):
+        """C = IA * B, B point (pidblas::mul_im_m)."""
+        m, k = IA.shape[:2]
+        n = B.shape[1]
+        C = pairs(m, n)
+        pa, lda = _pair_ptr(IA)
+        pb, ldb = _mat_ptr(B)
+        pc, ldc = _pair_ptr(C)
+        _check(lib().vcphip_igemm_ip(self._h, m, n, k, pa, lda, pb, ldb, pc, ldc), "vcphip_igemm_ip")
+        return C
+
+    def igemm_pi(self, B, IA):
+        """C = B * IA, B point (pidblas::mul_m_im)."""
+        m, k = B.shape
+        n = IA.shape[1]
+        C = pairs(m, n)
+        pb, ldb = _mat_ptr(B)
+        pa, lda = _pair_ptr(IA)
+        pc, ldc = _pair_ptr(C)
+        _check(lib().vcphip_igemm_pi(self._h, m, n, k, pb, ldb, pa, lda, pc, ldc), "vcphip_igemm_pi")
+        return C
+
+    def igemm_tn(self, IA):
+        """C = IA^T IA (pidblas::mulltmm)."""
+        k, n = IA.shape[:2]
+        C = pairs(n, n)
+        pa, lda = _pair_ptr(IA)
+        pc, ldc = _pair_ptr(C)
+        _check(lib().vcphip_igemm_tn(self._h, k, n, pa, lda, pc, ldc), "vcphip_igemm_tn")
+        return C
+
+    def vgemm_tn(self, A):
+        """verified A^T A of a point matrix (pidblas::vmulmm)."""
+        k, n = A.shape
+        C = pairs(n, n)
+        pa, lda = _mat_ptr(A)
+        pc, ldc = _pair_ptr(C)
+        _check(lib().vcphip_vgemm_tn(self._h, k, n, pa, lda, pc, ldc), "vcphip_vgemm_tn")
+        return C
+
+    def gemm_rurd(self, A, B, Cup=None, Cdn=None, accumulate=False):
+        """(RU(A*B), RD(A*B)) as k-ascending FMA chains (rmatmul +1/-1;
+        udmatmul when accumulate)."""
+        m, k = A.shape
+        n = B.shape[1]
+        if Cup is None:
+            Cup = np.zeros((m, n), order="F")
+        if Cdn is None:
+            Cdn = np.zeros((m, n), order="F")
+        pa, lda = _mat_ptr(A)
+        pb, ldb = _mat_ptr(B)
+        pu, ldc = _mat_ptr(Cup)
+        pd, _ = _mat_ptr(Cdn)
+        _check(lib().vcphip_gemm_rurd(self._h, m, n, k, pa, lda, pb, ldb, pu, pd, ldc, int(accumulate)),
+               "vcphip_gemm_rurd")
+        return Cup, Cdn
+
+    def ddgemm(self, A, B):
+        """C = A * B for double-double matrices (pddblas::mulmm)."""
+        m, k = A.shape[:2]
+        n = B.shape[1]
+        C = pairs(m, n)
+        pa, lda = _pair_ptr(A)
+        pb, ldb = _pair_ptr(B)
+        pc, ldc = _pair_ptr(C)
+        _check(lib().vcphip_ddgemm(self._h, m, n, k, pa, lda, pb, ldb, pc, ldc), "vcphip_ddgemm")
+        return C
+
+    def ddgemm_tn(self, A):
+        """C = A^T A for a double-double matrix (pddblas::mulltmm)."""
+        k, n = A.shape[:2]
+        C = pairs(n, n)
+        pa, lda = _pair_ptr(A)
+        pc, ldc = _pair_ptr(C)
+        _check(lib().vcphip_ddgemm_tn(self._h, k, n, pa, lda, pc, ldc), "vcphip_ddgemm_tn")
+        return C
+
+    def residual_batched(self, Rs, IAs):
+        """[(R_i IA_i - I, ||.||_inf upper bound)] for a list of problems."""
+        cnt = len(Rs)
+        ns = (ctypes.c_int * cnt)(*[R.shape[0] for R in Rs])
+        Cs = [pairs(R.shape[0], R.shape[0]) for R in Rs]
+        rp = (_vp * cnt)(*[_mat_ptr(R)[0] for R in Rs])
+        ap = (_vp * cnt)(*[_pair_ptr(A)[0] for A in IAs])
+        cp = (_vp * cnt)(*[_pair_ptr(C)[0] for C in Cs])
+        ldr = (ctypes.c_int * cnt)(*[_mat_ptr(R)[1] for R in Rs])
+        lda = (ctypes.c_int * cnt)(*[_pair_ptr(A)[1] for A in IAs])
+        ldc = (ctypes.c_int * cnt)(*[_pair_ptr(C)[1] for C in Cs])
+        nub = (ctypes.c_double * cnt)()
+        _check(lib().vcphip_residual_batched(self._h, cnt, ns, rp, ldr, ap, lda, cp, ldc, nub),
+               "vcphip_residual_batched")
+        return Cs, list(nub)
+
+    # ---- device-pointer products (torch tensors or raw pointers) -----------
+    def igemm_dev(self, form, m, n, k, dA, lda, dB, ldb, dC, ldc):
+        _check(lib().vcphip_igemm_dev(self._h, form, m, n, k, _vp(dA), lda, _vp(dB), ldb, _vp(dC), ldc),
+               "vcphip_igemm_dev")
+
+    def igemm_pi_dev(self, m, n, k, dB, ldb, dA, lda, dC, ldc):
+        _check(lib().vcphip_igemm_pi_dev(self._h, m, n, k, _vp(dB), ldb, _vp(dA), lda, _vp(dC), ldc),
+               "vcphip_igemm_pi_dev")
+
+    def gemm_rurd_dev(self, m, n, k, dA, lda, dB, ldb, dU, dD, ldc, accumulate=False):
+        _check(lib().vcphip_gemm_rurd_dev(self._h, m, n, k, _vp(dA), lda, _vp(dB), ldb, _vp(dU), _vp(dD), ldc,
+                                          int(accumulate)), "vcphip_gemm_rurd_dev")
+
+    def ddgemm_dev(self, m, n, k, dA, lda, dB, ldb, dC, ldc):
+        _check(lib().vcphip_ddgemm_dev(self._h, m, n, k, _vp(dA), lda, _vp(dB), ldb, _vp(dC), ldc),
+               "vcphip_ddgemm_dev")

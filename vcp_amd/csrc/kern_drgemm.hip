@@ -1,0 +1,370 @@
+// Plain FP64 directed-rounding GEMM family (vector FMA, LDS-tiled, gfx950).
+//
+// One kernel template covers the four products of the pidblas policy:
+//   II  C = IA*IB        pidblas::mulmm     vcp/pidblas.hpp:239-294
+//   IP  C = IA*B         pidblas::mul_im_m  vcp/pidblas.hpp:90-140
+//   PI  C = B*IA         pidblas::mul_m_im  vcp/pidblas.hpp:142-196
+//   PP  Cup, Cdn = A*B   rmatmul(+1/-1)     vblas/rmatmul.hpp:100-127,
+//                        udmatmul           vblas/udmatmul.hpp:738-760
+// Each output element carries the k-ascending chains of the reference's
+// BLAS calls (the rmatmul contract, vblas/README.md:167-170) in registers:
+//   up  = RU chain of mid(A)*mid(B)                       (:269-271)
+//   dn  = RU chain of (-mid(A))*mid(B) = -(RD chain)      (:264-266)
+//   r1  = RU chain of (|mid A|+rad A)*rad B               (:277-279)
+//   r2  = RU chain of rad A*|mid B|                       (:281)
+// and the epilogue applies the reference's fix-ups in its order
+//   t = RU(r1 + r2); upper = RU(up + t); lower = RD(-dn - t) = -RU(dn + t).
+// Interval operands are read as AoS {inf, sup} and converted to
+// midpoint-radius on their way into LDS (pidblas_assist::midrad, :43-60).
+//
+// The whole kernel runs in round-toward-+inf (see vcphip_device.h).
+#include "vcphip_device.h"
+#include "vcphip_kernels.h"
+
+namespace {
+
+constexpr int BM = 64, BN = 64, BK = 16;
+constexpr int TM = 4, TN = 4;
+constexpr int NT = 256;          // 16 x 16 threads, 4 waves
+constexpr int PAD = 2;           // doubles of row padding (keeps 16-byte alignment)
+constexpr int LA = BM + PAD;     // LDS row length of an A k-slice
+constexpr int LB = BN + PAD;
+constexpr int A_PER_T = BM * BK / NT;   // 4 elements of A per thread per tile
+constexpr int B_PER_T = BK * BN / NT;   // 4 elements of B per thread per tile
+
+template <bool AI, bool BI>
+struct Stage {
+  // register staging of one global tile (prefetch)
+  double a0[A_PER_T], a1[A_PER_T];
+  double b0[B_PER_T], b1[B_PER_T];
+};
+
+template <bool AI, bool BI>
+__global__ __launch_bounds__(NT, 2) void k_drgemm(int m, int n, int k,
+                                                   const double* __restrict__ A, long lda,
+                                                   const double* __restrict__ B, long ldb,
+                                                   double* __restrict__ C, long ldc,
+                                                   double* __restrict__ C2, int accumulate) {
+  vcphip_mode_round_up();
+
+  constexpr bool II = AI && BI;
+  __shared__ __attribute__((aligned(16))) double sAm[BK][LA];
+  __shared__ __attribute__((aligned(16))) double sAr[AI ? BK : 1][LA];
+  __shared__ __attribute__((aligned(16))) double sBm[BK][LB];
+  __shared__ __attribute__((aligned(16))) double sBr[BI ? BK : 1][LB];
+
+  const int tid = threadIdx.x;
+  const int ntm = (m + BM - 1) / BM, ntn = (n + BN - 1) / BN;
+  const int tile = vcphip_xcd_remap(blockIdx.x, ntm * ntn);
+  int tm, tn;
+  vcphip_tile_coords(tile, ntm, ntn, 16, tm, tn);
+  const int row0 = tm * BM, col0 = tn * BN;
+
+  // global-load mapping: A column-major, 64 consecutive rows per k column;
+  // B column-major, 16 consecutive k per column.
+  const int a_row = tid % BM, a_kc = tid / BM;     // k column a_kc + 4q
+  const int b_k = tid % BK, b_col = tid / BK;      // column b_col + 16q
+
+  Stage<AI, BI> st;
+  auto load_tile = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < A_PER_T; ++q) {
+      const int kk = k0 + a_kc + 4 * q, r = row0 + a_row;
+      const bool ok = (r < m) && (kk < k);
+      if (AI) {
+        double2 v = ok ? *reinterpret_cast<const double2*>(A + 2 * (r + lda * kk)) : make_double2(0.0, 0.0);
+        st.a0[q] = v.x;
+        st.a1[q] = v.y;
+      } else {
+        st.a0[q] = ok ? A[r + lda * kk] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < B_PER_T; ++q) {
+      const int kk = k0 + b_k, c = col0 + b_col + 16 * q;
+      const bool ok = (kk < k) && (c < n);
+      if (BI) {
+        double2 v = ok ? *reinterpret_cast<const double2*>(B + 2 * (kk + ldb * c)) : make_double2(0.0, 0.0);
+        st.b0[q] = v.x;
+        st.b1[q] = v.y;
+      } else {
+        st.b0[q] = ok ? B[kk + ldb * c] : 0.0;
+      }
+    }
+  };
+  auto store_tile = [&]() {
+#pragma unroll
+    for (int q = 0; q < A_PER_T; ++q) {
+      if (AI) {
+        double mm, rr;
+        vcphip_midrad_ru(st.a0[q], st.a1[q], mm, rr);
+        sAm[a_kc + 4 * q][a_row] = mm;
+        sAr[a_kc + 4 * q][a_row] = rr;
+      } else {
+        sAm[a_kc + 4 * q][a_row] = st.a0[q];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < B_PER_T; ++q) {
+      if (BI) {
+        double mm, rr;
+        vcphip_midrad_ru(st.b0[q], st.b1[q], mm, rr);
+        sBm[b_k][b_col + 16 * q] = mm;
+        sBr[b_k][b_col + 16 * q] = rr;
+      } else {
+        sBm[b_k][b_col + 16 * q] = st.b0[q];
+      }
+    }
+  };
+
+  const int tx = tid % 16, ty = tid / 16;
+  double up[TM][TN], dn[TM][TN], r1[TM][TN], r2[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      up[i][j] = 0.0;
+      dn[i][j] = 0.0;
+      r1[i][j] = 0.0;
+      r2[i][j] = 0.0;
+    }
+
+  if (!AI && !BI && accumulate) {
+    // udmatmul form: the chains continue from the values already in CU / CD
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int r = row0 + ty * TM + i, c = col0 + tx * TN + j;
+        if (r < m && c < n) {
+          up[i][j] = C[r + ldc * c];
+          dn[i][j] = -C2[r + ldc * c];
+        }
+      }
+  }
+
+  const int ntiles = (k + BK - 1) / BK;
+  load_tile(0);
+  store_tile();
+  __syncthreads();
+  for (int t = 0; t < ntiles; ++t) {
+    if (t + 1 < ntiles) load_tile((t + 1) * BK);
+#pragma unroll 2
+    for (int kk = 0; kk < BK; ++kk) {
+      double am[TM], ar[TM], bm[TN], br[TN];
+      {
+        const double2 x0 = *reinterpret_cast<const double2*>(&sAm[kk][ty * TM]);
+        const double2 x1 = *reinterpret_cast<const double2*>(&sAm[kk][ty * TM + 2]);
+        am[0] = x0.x; am[1] = x0.y; am[2] = x1.x; am[3] = x1.y;
+        const double2 y0 = *reinterpret_cast<const double2*>(&sBm[kk][tx * TN]);
+        const double2 y1 = *reinterpret_cast<const double2*>(&sBm[kk][tx * TN + 2]);
+        bm[0] = y0.x; bm[1] = y0.y; bm[2] = y1.x; bm[3] = y1.y;
+      }
+      if (AI) {
+        const double2 x0 = *reinterpret_cast<const double2*>(&sAr[kk][ty * TM]);
+        const double2 x1 = *reinterpret_cast<const double2*>(&sAr[kk][ty * TM + 2]);
+        ar[0] = x0.x; ar[1] = x0.y; ar[2] = x1.x; ar[3] = x1.y;
+      }
+      if (BI) {
+        const double2 y0 = *reinterpret_cast<const double2*>(&sBr[kk][tx * TN]);
+        const double2 y1 = *reinterpret_cast<const double2*>(&sBr[kk][tx * TN + 2]);
+        br[0] = y0.x; br[1] = y0.y; br[2] = y1.x; br[3] = y1.y;
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        double as = 0.0;
+        if (II) as = __builtin_fabs(am[i]) + ar[i];   // (|mA| + rA), RU (:277)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          up[i][j] = __builtin_fma(am[i], bm[j], up[i][j]);
+          dn[i][j] = __builtin_fma(-am[i], bm[j], dn[i][j]);
+          if (II) {
+            r1[i][j] = __builtin_fma(as, br[j], r1[i][j]);
+            r2[i][j] = __builtin_fma(ar[i], __builtin_fabs(bm[j]), r2[i][j]);
+          } else if (AI) {
+            r1[i][j] = __builtin_fma(ar[i], __builtin_fabs(bm[j]), r1[i][j]);   // rA*|B|  (:129)
+          } else if (BI) {
+            r1[i][j] = __builtin_fma(__builtin_fabs(am[i]), br[j], r1[i][j]);   // |B|*rA  (:185)
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (t + 1 < ntiles) {
+      store_tile();
+      __syncthreads();
+    }
+  }
+
+  // epilogue
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int c = col0 + tx * TN + j;
+    if (c >= n) continue;
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const int r = row0 + ty * TM + i;
+      if (r >= m) continue;
+      if (AI || BI) {
+        double t;
+        if (II) t = r1[i][j] + r2[i][j];    // temp = (|mA|+rA) rB + rA |mB|   (:283)
+        else t = r1[i][j];
+        const double hi = up[i][j] + t;       // uC += temp, RU                  (:284)
+        const double nlo = dn[i][j] + t;      // lC -= temp, RD = -(RU(-lC + t)) (:285-286)
+        *reinterpret_cast<double2*>(C + 2 * (r + ldc * c)) = make_double2(-nlo, hi);
+      } else {
+        C[r + ldc * c] = up[i][j];
+        C2[r + ldc * c] = -dn[i][j];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// A^T A forms.  IA is k x n; tiles (tm <= tn) of the upper triangle only.
+//   interval (pidblas::mulltmm, vcp/pidblas.hpp:296-358):
+//     up, dn  = chains of mA_i . mA_j                (dsyrk, :308-315)
+//     tij     = RU chain of |mA_i| . rA_j            (|mA|^T rA, :317-321)
+//     tji     = RU chain of |mA_j| . rA_i
+//     sij     = RU chain of rA_i . rA_j              (rA^T rA, :323)
+//     T = RU(tij + RU(tji + sij)); upper = RU(up + T); lower = -RU(dn + T)
+//   point (pidblas::vmulmm, :198-236): upper = RU chain, lower = RD chain.
+// Element (i, j), i <= j, is written to (i, j) and (j, i).
+// ---------------------------------------------------------------------------
+template <bool INTV>
+__global__ __launch_bounds__(NT, 2) void k_drsyrk(int k, int n, const double* __restrict__ A, long lda,
+                                                  double* __restrict__ C, long ldc) {
+  vcphip_mode_round_up();
+  __shared__ __attribute__((aligned(16))) double sIm[BK][LA];
+  __shared__ __attribute__((aligned(16))) double sIr[INTV ? BK : 1][LA];
+  __shared__ __attribute__((aligned(16))) double sJm[BK][LB];
+  __shared__ __attribute__((aligned(16))) double sJr[INTV ? BK : 1][LB];
+
+  const int tid = threadIdx.x;
+  const int nt = (n + BM - 1) / BM;
+  // decode upper-triangle tile index b -> (tm, tn), tm <= tn, row-major over tm
+  int b = blockIdx.x, tm = 0;
+  while (b >= nt - tm) { b -= nt - tm; ++tm; }
+  const int tn = tm + b;
+  const int i0 = tm * BM, j0 = tn * BN;
+
+  const int l_k = tid % BK, l_c = tid / BK;   // k-contiguous loads, 16 columns per pass
+  double im[A_PER_T], ir[A_PER_T], jm[B_PER_T], jr[B_PER_T];
+  auto load_tile = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < A_PER_T; ++q) {
+      const int kk = k0 + l_k;
+      const int ci = i0 + l_c + 16 * q, cj = j0 + l_c + 16 * q;
+      if (INTV) {
+        const double2 vi = (kk < k && ci < n) ? *reinterpret_cast<const double2*>(A + 2 * (kk + lda * ci)) : make_double2(0.0, 0.0);
+        const double2 vj = (kk < k && cj < n) ? *reinterpret_cast<const double2*>(A + 2 * (kk + lda * cj)) : make_double2(0.0, 0.0);
+        im[q] = vi.x; ir[q] = vi.y; jm[q] = vj.x; jr[q] = vj.y;
+      } else {
+        im[q] = (kk < k && ci < n) ? A[kk + lda * ci] : 0.0;
+        jm[q] = (kk < k && cj < n) ? A[kk + lda * cj] : 0.0;
+      }
+    }
+  };
+  auto store_tile = [&]() {
+#pragma unroll
+    for (int q = 0; q < A_PER_T; ++q) {
+      if (INTV) {
+        double mm, rr;
+        vcphip_midrad_ru(im[q], ir[q], mm, rr);
+        sIm[l_k][l_c + 16 * q] = mm;
+        sIr[l_k][l_c + 16 * q] = rr;
+        vcphip_midrad_ru(jm[q], jr[q], mm, rr);
+        sJm[l_k][l_c + 16 * q] = mm;
+        sJr[l_k][l_c + 16 * q] = rr;
+      } else {
+        sIm[l_k][l_c + 16 * q] = im[q];
+        sJm[l_k][l_c + 16 * q] = jm[q];
+      }
+    }
+  };
+
+  const int tx = tid % 16, ty = tid / 16;
+  double up[TM][TN], dn[TM][TN], tij[TM][TN], tji[TM][TN], sij[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) { up[i][j] = 0.0; dn[i][j] = 0.0; tij[i][j] = 0.0; tji[i][j] = 0.0; sij[i][j] = 0.0; }
+
+  const int ntiles = (k + BK - 1) / BK;
+  load_tile(0);
+  store_tile();
+  __syncthreads();
+  for (int t = 0; t < ntiles; ++t) {
+    if (t + 1 < ntiles) load_tile((t + 1) * BK);
+#pragma unroll 1
+    for (int kk = 0; kk < BK; ++kk) {
+      double a[TM], ar[TM], bb[TN], br[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) { a[i] = sIm[kk][ty * TM + i]; if (INTV) ar[i] = sIr[kk][ty * TM + i]; }
+#pragma unroll
+      for (int j = 0; j < TN; ++j) { bb[j] = sJm[kk][tx * TN + j]; if (INTV) br[j] = sJr[kk][tx * TN + j]; }
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          up[i][j] = __builtin_fma(a[i], bb[j], up[i][j]);
+          dn[i][j] = __builtin_fma(-a[i], bb[j], dn[i][j]);
+          if (INTV) {
+            tij[i][j] = __builtin_fma(__builtin_fabs(a[i]), br[j], tij[i][j]);
+            tji[i][j] = __builtin_fma(__builtin_fabs(bb[j]), ar[i], tji[i][j]);
+            sij[i][j] = __builtin_fma(ar[i], br[j], sij[i][j]);
+          }
+        }
+    }
+    __syncthreads();
+    if (t + 1 < ntiles) {
+      store_tile();
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int r = i0 + ty * TM + i, c = j0 + tx * TN + j;
+      if (r >= n || c >= n || r > c) continue;
+      double lo, hi;
+      if (INTV) {
+        const double T = tij[i][j] + (tji[i][j] + sij[i][j]);
+        hi = up[i][j] + T;
+        lo = -(dn[i][j] + T);
+      } else {
+        hi = up[i][j];
+        lo = -dn[i][j];
+      }
+      *reinterpret_cast<double2*>(C + 2 * (r + ldc * c)) = make_double2(lo, hi);
+      *reinterpret_cast<double2*>(C + 2 * (c + ldc * r)) = make_double2(lo, hi);
+    }
+}
+
+}  // namespace
+
+int vcphip_launch_drgemm(int kind, int m, int n, int k, const double* A, long lda, const double* B,
+                         long ldb, double* C, long ldc, double* C2, int accumulate, hipStream_t s) {
+  if (m <= 0 || n <= 0) return 0;
+  const int ntiles = ((m + BM - 1) / BM) * ((n + BN - 1) / BN);
+  dim3 grid(ntiles), block(NT);
+  switch (kind) {
+    case VCPHIP_DR_II: hipLaunchKernelGGL((k_drgemm<true, true>), grid, block, 0, s, m, n, k, A, lda, B, ldb, C, ldc, C2, accumulate); break;
+    case VCPHIP_DR_IP: hipLaunchKernelGGL((k_drgemm<true, false>), grid, block, 0, s, m, n, k, A, lda, B, ldb, C, ldc, C2, accumulate); break;
+    case VCPHIP_DR_PI: hipLaunchKernelGGL((k_drgemm<false, true>), grid, block, 0, s, m, n, k, A, lda, B, ldb, C, ldc, C2, accumulate); break;
+    case VCPHIP_DR_PP: hipLaunchKernelGGL((k_drgemm<false, false>), grid, block, 0, s, m, n, k, A, lda, B, ldb, C, ldc, C2, accumulate); break;
+    default: return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_launch_drsyrk(int interval, int k, int n, const double* A, long lda, double* C, long ldc,
+                         hipStream_t s) {
+  if (n <= 0) return 0;
+  const int nt = (n + BM - 1) / BM;
+  dim3 grid(nt * (nt + 1) / 2), block(NT);
+  if (interval) hipLaunchKernelGGL((k_drsyrk<true>), grid, block, 0, s, k, n, A, lda, C, ldc);
+  else hipLaunchKernelGGL((k_drsyrk<false>), grid, block, 0, s, k, n, A, lda, C, ldc);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}

@@ -1,0 +1,78 @@
+// Device-side helpers shared by the libvcphip kernels (gfx950 / CDNA4).
+//
+// Directed rounding.  Every kernel that needs directed rounding runs its
+// whole body in round-toward-+inf: it writes the MODE register once at entry
+// (FP_ROUND bits [3:0]: f32 and f64/f16 both := 1, toward +inf) and obtains
+// round-toward-(-inf) through RD(x) = -RU(-x), which is exact.  The write is
+// an inline-asm s_setreg so that the compiler's mode tracking (which assumes
+// the default round-to-nearest for ordinary FP instructions and would
+// otherwise re-insert a switch back to nearest before them) stays out of the
+// way; all FP work in such kernels depends on memory loads issued after the
+// asm (it carries a memory clobber), so nothing can be scheduled above it.
+// Sources are built with -ffp-contract=off: every rounding is explicit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define VCPHIP_DEVINL __device__ __forceinline__
+
+VCPHIP_DEVINL void vcphip_mode_round_up() {
+  asm volatile("s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 4), 5" ::: "memory");
+}
+
+// 2^e as a double for e in [-1022, 1023] (bit construction, exact)
+VCPHIP_DEVINL double vcphip_pow2(int e) {
+  return __longlong_as_double((long long)(e + 1023) << 52);
+}
+
+// x * 2^e under the current rounding mode; intermediate factors stay normal
+// so that only an underflowing final product rounds.
+VCPHIP_DEVINL double vcphip_mulpow2(double x, int e) {
+  while (e > 1000) { x = x * vcphip_pow2(1000); e -= 1000; }
+  while (e < -1000) { x = x * vcphip_pow2(-1000); e += 1000; }
+  return x * vcphip_pow2(e);
+}
+
+// frexp exponent of x > 0: 2^(e-1) <= x < 2^e
+VCPHIP_DEVINL int vcphip_fexp(double x) {
+  return __builtin_amdgcn_frexp_exp(x);
+}
+
+// midpoint-radius form of [lo, hi] in round-up mode: [m-r, m+r] contains
+// [lo, hi]; point intervals give (lo, 0).  Same rule as the oracle's
+// restatement of kv's midrad.
+VCPHIP_DEVINL void vcphip_midrad_ru(double lo, double hi, double& m, double& r) {
+  if (lo == hi) {
+    m = lo;
+    r = 0.0;
+  } else {
+    const double h1 = 0.5 * lo;
+    const double h2 = 0.5 * hi;
+    m = h1 + h2;
+    const double r1 = hi - m;
+    const double r2 = m - lo;
+    r = r1 > r2 ? r1 : r2;
+  }
+}
+
+// XCD-aware block remap: workgroups are dispatched round-robin over the 8
+// XCDs of an MI355X; give each XCD a contiguous range of tile ids so that
+// concurrently running tiles on one XCD share A rows / B columns in its L2.
+VCPHIP_DEVINL int vcphip_xcd_remap(int bid, int ntiles) {
+  const int q = ntiles >> 3, r = ntiles & 7;
+  const int x = bid & 7, local = bid >> 3;
+  const int start = x * q + (x < r ? x : r);
+  return start + local;
+}
+
+// grouped tile order: GH tile-rows per group, columns inside a group
+VCPHIP_DEVINL void vcphip_tile_coords(int tile, int ntm, int ntn, int GH, int& tm, int& tn) {
+  const int per_group = GH * ntn;
+  const int group = tile / per_group;
+  const int idx = tile - group * per_group;
+  int gh = ntm - group * GH;
+  gh = gh < GH ? gh : GH;
+  tm = group * GH + idx % gh;
+  tn = idx / gh;
+}

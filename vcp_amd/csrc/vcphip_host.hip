@@ -1,0 +1,606 @@
+// libvcphip host side: the C ABI declared in include/vcphip.h.
+//
+// A context owns one HIP stream (default priority) and a cached device
+// workspace; host-pointer entry points stage operands through it, device
+// entry points enqueue on the stream and return.  Parameters of the Ozaki
+// form (moduli, scale exponent, CRT constants) are derived here on the host.
+#include <fenv.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/vcphip.h"
+#include "vcphip_kernels.h"
+
+struct vcphip_ctx {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  long long oz_nmod = 0, oz_variant = 0, timing = 0, dd_nmod = 0;
+  // workspace slots (device memory, grown on demand)
+  enum { WS_PA, WS_PB, WS_U, WS_Z, WS_SA, WS_SB, WS_HA, WS_HB, WS_HC, WS_HC2, WS_BATCH, WS_N };
+  void* ws[WS_N] = {};
+  size_t ws_size[WS_N] = {};
+  // stage timing: events recorded by every call since the last synchronize,
+  // folded into per-stage totals there
+  std::vector<hipEvent_t> ev;
+  std::vector<std::string> ev_name;
+  std::vector<std::string> st_name;
+  std::vector<double> st_ms;
+  std::vector<long long> st_cnt;
+};
+
+namespace {
+
+int ws_get(vcphip_ctx* c, int slot, size_t bytes, void** out) {
+  if (bytes == 0) bytes = 256;
+  if (c->ws_size[slot] < bytes) {
+    if (c->ws[slot]) {
+      hipStreamSynchronize(c->stream);
+      hipFree(c->ws[slot]);
+      c->ws[slot] = nullptr;
+      c->ws_size[slot] = 0;
+    }
+    const size_t want = (bytes + 0xFFFFF) & ~(size_t)0xFFFFF;   // 1 MiB granularity
+    if (hipMalloc(&c->ws[slot], want) != hipSuccess) {
+      hipGetLastError();
+      return VCPHIP_ERR_OOM;
+    }
+    c->ws_size[slot] = want;
+  }
+  *out = c->ws[slot];
+  return VCPHIP_OK;
+}
+
+struct StageTimer {
+  vcphip_ctx* c;
+  bool on;
+  explicit StageTimer(vcphip_ctx* ctx) : c(ctx), on(ctx->timing != 0) {
+    if (!on) return;
+    mark("");   // call boundary: the next stage is timed from here
+  }
+  void mark(const char* name) {
+    if (!on) return;
+    hipEvent_t e;
+    hipEventCreate(&e);
+    hipEventRecord(e, c->stream);
+    c->ev.push_back(e);
+    c->ev_name.push_back(name);
+  }
+};
+
+int check_launch(int rc) { return rc == 0 ? VCPHIP_OK : (rc == -7 ? VCPHIP_ERR_UNSUPPORTED : VCPHIP_ERR_DEVICE); }
+
+int gcd_i(int a, int b) {
+  while (b) {
+    int t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+int modinv(int a, int m) {   // a^{-1} mod m, gcd(a, m) = 1
+  int t = 0, nt = 1, r = m, nr = ((a % m) + m) % m;
+  while (nr) {
+    const int q = r / nr;
+    int tmp = t - q * nt; t = nt; nt = tmp;
+    tmp = r - q * nr; r = nr; nr = tmp;
+  }
+  return ((t % m) + m) % m;
+}
+
+// greedy pairwise-coprime moduli from 256 downward
+int oz_moduli(int nmod, int* out) {
+  int cnt = 0;
+  for (int m = 256; m >= 2 && cnt < nmod; --m) {
+    bool ok = true;
+    for (int t = 0; t < cnt; ++t)
+      if (gcd_i(m, out[t]) != 1) { ok = false; break; }
+    if (ok) out[cnt++] = m;
+  }
+  return cnt;
+}
+
+int oz_beta(int k, int nmod, const int* mods) {
+  double lm = 0.0;
+  for (int t = 0; t < nmod; ++t) lm += log2((double)mods[t]);
+  int lk = 0;
+  while ((1ll << lk) < (long long)(k > 1 ? k : 1)) ++lk;
+  const int beta = (int)floor((lm - 1.0 - (double)lk - 1e-6) / 2.0);
+  return beta > 62 ? 62 : beta;
+}
+
+int oz_make_params(int k, int nmod, VcphipOzParams* P) {
+  memset(P, 0, sizeof(*P));
+  if (nmod < 8 || nmod > VCPHIP_OZ_MAXMOD) return VCPHIP_ERR_ARGUMENT;
+  if (oz_moduli(nmod, P->mod) != nmod) return VCPHIP_ERR_ARGUMENT;
+  P->nmod = nmod;
+  P->beta = oz_beta(k, nmod, P->mod);
+  if (P->beta < 8) return VCPHIP_ERR_UNSUPPORTED;
+  for (int t = 0; t < nmod; ++t) {
+    const int m = P->mod[t];
+    P->rcpf[t] = 1.0f / (float)m;
+    P->rcpd[t] = 1.0 / (double)m;
+    P->w21[t] = (int)((1ull << 21) % (unsigned)m);
+    P->w42[t] = (int)((1ull << 42) % (unsigned long long)m);
+    long long prod = 1;   // (m_0 ... m_{t-1}) mod m_t
+    for (int s = 0; s < t; ++s) {
+      P->negP[s][t] = (int)((m - prod % m) % m);
+      prod = (prod * P->mod[s]) % m;
+    }
+    P->pinv[t] = modinv((int)(prod % m), m);
+  }
+  // M and M/2 in 32-bit limbs
+  uint32_t M[VCPHIP_OZ_NLIMB] = {1, 0, 0, 0, 0};
+  for (int t = 0; t < nmod; ++t) {
+    unsigned long long carry = 0;
+    for (int l = 0; l < VCPHIP_OZ_NLIMB; ++l) {
+      const unsigned long long p = (unsigned long long)M[l] * (unsigned)P->mod[t] + carry;
+      M[l] = (uint32_t)p;
+      carry = p >> 32;
+    }
+    if (carry) return VCPHIP_ERR_UNSUPPORTED;
+  }
+  for (int l = 0; l < VCPHIP_OZ_NLIMB; ++l) {
+    P->Mlimb[l] = M[l];
+    P->halfM[l] = (M[l] >> 1) | (l + 1 < VCPHIP_OZ_NLIMB ? (M[l + 1] << 31) : 0u);
+  }
+  {
+    const int old = fegetround();
+    fesetround(FE_UPWARD);
+    volatile double one = 1.0, d = 16129.0;
+    P->c127 = one / d;
+    fesetround(old);
+  }
+  return VCPHIP_OK;
+}
+
+int default_nmod(int k) {
+  // 16 moduli: beta = 55 at k = 8192, a scaling error far below the FP64
+  // directed-rounding width of the same product
+  (void)k;
+  return 16;
+}
+
+}  // namespace
+
+// ===========================================================================
+// context
+// ===========================================================================
+extern "C" int vcphip_version(void) { return 1; }
+
+extern "C" const char* vcphip_strerror(int code) {
+  switch (code) {
+    case VCPHIP_OK: return "ok";
+    case VCPHIP_ERR_DIMENSION: return "dimension mismatch";
+    case VCPHIP_ERR_ARGUMENT: return "invalid argument";
+    case VCPHIP_ERR_DEVICE: return "HIP device error";
+    case VCPHIP_ERR_OOM: return "device memory exhausted";
+    case VCPHIP_ERR_ROUNDING: return "directed rounding not in effect";
+    case VCPHIP_ERR_VERIFICATION: return "verification failed";
+    case VCPHIP_ERR_UNSUPPORTED: return "unsupported size or form";
+    case VCPHIP_ERR_RANGE: return "exponent range not supported";
+    default: return "unknown error";
+  }
+}
+
+extern "C" int vcphip_ctx_create(int device, vcphip_ctx** out) {
+  if (!out) return VCPHIP_ERR_ARGUMENT;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    hipGetLastError();
+    return VCPHIP_ERR_DEVICE;
+  }
+  if (device < 0 || device >= ndev) return VCPHIP_ERR_ARGUMENT;
+  if (hipSetDevice(device) != hipSuccess) return VCPHIP_ERR_DEVICE;
+  vcphip_ctx* c = new vcphip_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return VCPHIP_ERR_DEVICE;
+  }
+  c->stream = c->own;
+  *out = c;
+  return VCPHIP_OK;
+}
+
+extern "C" int vcphip_ctx_destroy(vcphip_ctx* c) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  for (int s = 0; s < vcphip_ctx::WS_N; ++s)
+    if (c->ws[s]) hipFree(c->ws[s]);
+  for (auto e : c->ev) hipEventDestroy(e);
+  hipStreamDestroy(c->own);
+  delete c;
+  return VCPHIP_OK;
+}
+
+extern "C" int vcphip_ctx_set_stream(vcphip_ctx* c, void* stream) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  c->stream = stream ? (hipStream_t)stream : c->own;
+  return VCPHIP_OK;
+}
+
+extern "C" int vcphip_ctx_synchronize(vcphip_ctx* c) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  hipSetDevice(c->device);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return VCPHIP_ERR_DEVICE;
+  for (size_t i = 1; i < c->ev.size(); ++i) {
+    if (c->ev_name[i].empty()) continue;   // a call boundary ends no stage
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->ev[i - 1], c->ev[i]);
+    size_t s = 0;
+    while (s < c->st_name.size() && c->st_name[s] != c->ev_name[i]) ++s;
+    if (s == c->st_name.size()) {
+      c->st_name.push_back(c->ev_name[i]);
+      c->st_ms.push_back(0.0);
+      c->st_cnt.push_back(0);
+    }
+    c->st_ms[s] += ms;
+    c->st_cnt[s] += 1;
+  }
+  for (auto e : c->ev) hipEventDestroy(e);
+  c->ev.clear();
+  c->ev_name.clear();
+  return VCPHIP_OK;
+}
+
+extern "C" int vcphip_ctx_set_param(vcphip_ctx* c, int key, long long v) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  switch (key) {
+    case VCPHIP_PARAM_OZ_NMOD:
+      if (v != 0 && (v < 8 || v > VCPHIP_OZ_MAXMOD)) return VCPHIP_ERR_ARGUMENT;
+      c->oz_nmod = v;
+      return VCPHIP_OK;
+    case VCPHIP_PARAM_OZ_VARIANT:
+      if (v < 0 || v > 2) return VCPHIP_ERR_ARGUMENT;
+      c->oz_variant = v;
+      return VCPHIP_OK;
+    case VCPHIP_PARAM_STAGE_TIMING:   // (re)enabling clears the totals
+      c->timing = v ? 1 : 0;
+      c->st_name.clear();
+      c->st_ms.clear();
+      c->st_cnt.clear();
+      return VCPHIP_OK;
+    case VCPHIP_PARAM_DD_NMOD: c->dd_nmod = v; return VCPHIP_OK;
+    default: return VCPHIP_ERR_ARGUMENT;
+  }
+}
+
+extern "C" int vcphip_ctx_get_param(vcphip_ctx* c, int key, long long* v) {
+  if (!c || !v) return VCPHIP_ERR_ARGUMENT;
+  switch (key) {
+    case VCPHIP_PARAM_OZ_NMOD: *v = c->oz_nmod; return VCPHIP_OK;
+    case VCPHIP_PARAM_OZ_VARIANT: *v = c->oz_variant; return VCPHIP_OK;
+    case VCPHIP_PARAM_STAGE_TIMING: *v = c->timing; return VCPHIP_OK;
+    case VCPHIP_PARAM_DD_NMOD: *v = c->dd_nmod; return VCPHIP_OK;
+    default: return VCPHIP_ERR_ARGUMENT;
+  }
+}
+
+extern "C" int vcphip_ctx_stage_times(vcphip_ctx* c, double* ms, int max, int* count, char* names, int names_len) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  const int n = (int)c->st_name.size();
+  if (count) *count = n;
+  std::string all;
+  for (int i = 0; i < n; ++i) {
+    if (ms && i < max) ms[i] = c->st_cnt[i] ? c->st_ms[i] / (double)c->st_cnt[i] : 0.0;
+    if (i) all += ";";
+    all += c->st_name[i];
+  }
+  if (names && names_len > 0) {
+    strncpy(names, all.c_str(), (size_t)names_len - 1);
+    names[names_len - 1] = 0;
+  }
+  return VCPHIP_OK;
+}
+
+extern "C" int vcphip_ozaki_config(int k, int nmod, int* beta, int* moduli, int max_moduli) {
+  if (nmod <= 0) nmod = default_nmod(k);
+  VcphipOzParams P;
+  const int rc = oz_make_params(k, nmod, &P);
+  if (rc) return rc;
+  if (beta) *beta = P.beta;
+  for (int t = 0; t < nmod && t < max_moduli && moduli; ++t) moduli[t] = P.mod[t];
+  return nmod;
+}
+
+// ===========================================================================
+// device entry points
+// ===========================================================================
+static int dims_ok(int m, int n, int k) { return m >= 0 && n >= 0 && k >= 0; }
+
+static int oz_igemm_dev(vcphip_ctx* c, int m, int n, int k, const double* A, int lda, const double* B, int ldb,
+                        double* C, int ldc) {
+  if (k > 131071) return VCPHIP_ERR_UNSUPPORTED;
+  const int nmod = c->oz_nmod ? (int)c->oz_nmod : default_nmod(k);
+  VcphipOzParams P;
+  int rc = oz_make_params(k, nmod, &P);
+  if (rc) return rc;
+  const int mpad = (m + VCPHIP_OZ_PADMN - 1) / VCPHIP_OZ_PADMN * VCPHIP_OZ_PADMN;
+  const int npad = (n + VCPHIP_OZ_PADMN - 1) / VCPHIP_OZ_PADMN * VCPHIP_OZ_PADMN;
+  const int kpad = k > 0 ? (k + VCPHIP_OZ_PADK - 1) / VCPHIP_OZ_PADK * VCPHIP_OZ_PADK : VCPHIP_OZ_PADK;
+  const int nchunk = k > 0 ? (k + VCPHIP_OZ_CHUNK - 1) / VCPHIP_OZ_CHUNK : 0;
+  void *pa, *pb, *pu, *pz, *sa, *sb;
+  const size_t nplane = (size_t)nmod + 2;
+  if ((rc = ws_get(c, vcphip_ctx::WS_PA, nplane * mpad * kpad, &pa))) return rc;
+  if ((rc = ws_get(c, vcphip_ctx::WS_PB, nplane * npad * kpad, &pb))) return rc;
+  if ((rc = ws_get(c, vcphip_ctx::WS_U, (size_t)nmod * mpad * npad, &pu))) return rc;
+  if ((rc = ws_get(c, vcphip_ctx::WS_Z, 2 * sizeof(int32_t) * (size_t)mpad * npad, &pz))) return rc;
+  auto side_bytes = [&](int rows) { return (size_t)rows * (4 * sizeof(int) + sizeof(double)) + sizeof(double) * 4 * (size_t)(nchunk > 0 ? nchunk : 1) * rows + 1024; };
+  if ((rc = ws_get(c, vcphip_ctx::WS_SA, side_bytes(m), &sa))) return rc;
+  if ((rc = ws_get(c, vcphip_ctx::WS_SB, side_bytes(n), &sb))) return rc;
+  auto carve = [&](void* base, int rows) {
+    VcphipOzSide s;
+    char* p = (char*)base;
+    s.n1 = (double*)p; p += sizeof(double) * rows;
+    s.part = (double*)p; p += sizeof(double) * 4 * (size_t)(nchunk > 0 ? nchunk : 1) * rows;
+    s.e = (int*)p; p += sizeof(int) * rows;
+    s.g = (int*)p; p += sizeof(int) * rows;
+    s.h = (int*)p; p += sizeof(int) * rows;
+    s.zero = (int*)p;
+    return s;
+  };
+  VcphipOzSide SA = carve(sa, m), SB = carve(sb, n);
+  int variant = (int)c->oz_variant;
+  if (variant == 0) variant = 2;   // default tile: 128 x 128
+  StageTimer T(c);
+  if ((rc = check_launch(vcphip_oz_stats(A, 1, lda, m, k, SA, c->stream)))) return rc;
+  if ((rc = check_launch(vcphip_oz_stats(B, ldb, 1, n, k, SB, c->stream)))) return rc;
+  T.mark("stats");
+  if ((rc = check_launch(vcphip_oz_split_rows(A, lda, m, k, mpad, kpad, P, SA, (int8_t*)pa, c->stream)))) return rc;
+  if ((rc = check_launch(vcphip_oz_split_cols(B, ldb, k, n, npad, kpad, P, SB, (int8_t*)pb, c->stream)))) return rc;
+  T.mark("split");
+  if ((rc = check_launch(vcphip_oz_gemm(mpad, npad, kpad, nmod, 2, (const int8_t*)pa, (const int8_t*)pb, P,
+                                        (uint8_t*)pu, (int32_t*)pz, variant == 1 ? 1 : 0, c->stream))))
+    return rc;
+  T.mark("gemm");
+  if ((rc = check_launch(vcphip_oz_recon(m, n, k, mpad, npad, P, SA, SB, (const uint8_t*)pu, (const int32_t*)pz, C, ldc,
+                                         c->stream))))
+    return rc;
+  T.mark("recon");
+  return VCPHIP_OK;
+}
+
+extern "C" int vcphip_igemm_dev(vcphip_ctx* c, int form, int m, int n, int k, const double* IA, int lda,
+                                const double* IB, int ldb, double* C, int ldc) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (!dims_ok(m, n, k) || lda < (m > 1 ? m : 1) || ldb < (k > 1 ? k : 1) || ldc < (m > 1 ? m : 1))
+    return VCPHIP_ERR_DIMENSION;
+  if (m == 0 || n == 0) return VCPHIP_OK;
+  hipSetDevice(c->device);
+  if (form == VCPHIP_FORM_OZAKI) return oz_igemm_dev(c, m, n, k, IA, lda, IB, ldb, C, ldc);
+  if (form != VCPHIP_FORM_PLAIN) return VCPHIP_ERR_ARGUMENT;
+  StageTimer T(c);
+  const int rc = check_launch(vcphip_launch_drgemm(VCPHIP_DR_II, m, n, k, IA, lda, IB, ldb, C, ldc, nullptr, 0, c->stream));
+  T.mark("drgemm");
+  return rc;
+}
+
+extern "C" int vcphip_igemm_ip_dev(vcphip_ctx* c, int m, int n, int k, const double* IA, int lda, const double* B,
+                                   int ldb, double* C, int ldc) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (!dims_ok(m, n, k) || lda < (m > 1 ? m : 1) || ldb < (k > 1 ? k : 1) || ldc < (m > 1 ? m : 1))
+    return VCPHIP_ERR_DIMENSION;
+  if (m == 0 || n == 0) return VCPHIP_OK;
+  hipSetDevice(c->device);
+  return check_launch(vcphip_launch_drgemm(VCPHIP_DR_IP, m, n, k, IA, lda, B, ldb, C, ldc, nullptr, 0, c->stream));
+}
+
+extern "C" int vcphip_igemm_pi_dev(vcphip_ctx* c, int m, int n, int k, const double* B, int ldb, const double* IA,
+                                   int lda, double* C, int ldc) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (!dims_ok(m, n, k) || ldb < (m > 1 ? m : 1) || lda < (k > 1 ? k : 1) || ldc < (m > 1 ? m : 1))
+    return VCPHIP_ERR_DIMENSION;
+  if (m == 0 || n == 0) return VCPHIP_OK;
+  hipSetDevice(c->device);
+  return check_launch(vcphip_launch_drgemm(VCPHIP_DR_PI, m, n, k, B, ldb, IA, lda, C, ldc, nullptr, 0, c->stream));
+}
+
+extern "C" int vcphip_igemm_tn_dev(vcphip_ctx* c, int k, int n, const double* IA, int lda, double* C, int ldc) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (k < 0 || n < 0 || lda < (k > 1 ? k : 1) || ldc < (n > 1 ? n : 1)) return VCPHIP_ERR_DIMENSION;
+  if (n == 0) return VCPHIP_OK;
+  hipSetDevice(c->device);
+  return check_launch(vcphip_launch_drsyrk(1, k, n, IA, lda, C, ldc, c->stream));
+}
+
+extern "C" int vcphip_vgemm_tn_dev(vcphip_ctx* c, int k, int n, const double* A, int lda, double* C, int ldc) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (k < 0 || n < 0 || lda < (k > 1 ? k : 1) || ldc < (n > 1 ? n : 1)) return VCPHIP_ERR_DIMENSION;
+  if (n == 0) return VCPHIP_OK;
+  hipSetDevice(c->device);
+  return check_launch(vcphip_launch_drsyrk(0, k, n, A, lda, C, ldc, c->stream));
+}
+
+extern "C" int vcphip_gemm_rurd_dev(vcphip_ctx* c, int m, int n, int k, const double* A, int lda, const double* B,
+                                    int ldb, double* Cup, double* Cdn, int ldc, int accumulate) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (!dims_ok(m, n, k) || lda < (m > 1 ? m : 1) || ldb < (k > 1 ? k : 1) || ldc < (m > 1 ? m : 1))
+    return VCPHIP_ERR_DIMENSION;
+  if (m == 0 || n == 0) return VCPHIP_OK;
+  hipSetDevice(c->device);
+  return check_launch(
+      vcphip_launch_drgemm(VCPHIP_DR_PP, m, n, k, A, lda, B, ldb, Cup, ldc, Cdn, accumulate ? 1 : 0, c->stream));
+}
+
+extern "C" int vcphip_ddgemm_dev(vcphip_ctx* c, int m, int n, int k, const double* A, int lda, const double* B,
+                                 int ldb, double* C, int ldc) {
+  (void)m; (void)n; (void)k; (void)A; (void)lda; (void)B; (void)ldb; (void)C; (void)ldc;
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  return VCPHIP_ERR_UNSUPPORTED;
+}
+
+extern "C" int vcphip_residual_batched_dev(vcphip_ctx* c, int count, const int* n, const double* const* R,
+                                           const int* ldr, const double* const* IA, const int* lda, double* const* C,
+                                           const int* ldc, double* normub_dev) {
+  (void)count; (void)n; (void)R; (void)ldr; (void)IA; (void)lda; (void)C; (void)ldc; (void)normub_dev;
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  return VCPHIP_ERR_UNSUPPORTED;
+}
+
+// ===========================================================================
+// host-pointer entry points: stage through the context workspace
+// ===========================================================================
+namespace {
+
+// copy a column-major matrix of `rows` x `cols` elements of `esize` bytes
+int to_dev(vcphip_ctx* c, int slot, const double* h, int ld, int rows, int cols, size_t esize, double** d) {
+  void* p;
+  const size_t bytes = esize * (size_t)(rows > 0 ? rows : 1) * (size_t)(cols > 0 ? cols : 1);
+  int rc = ws_get(c, slot, bytes, &p);
+  if (rc) return rc;
+  *d = (double*)p;
+  if (rows > 0 && cols > 0) {
+    if (hipMemcpy2DAsync(p, esize * rows, h, esize * ld, esize * rows, cols, hipMemcpyHostToDevice, c->stream) !=
+        hipSuccess)
+      return VCPHIP_ERR_DEVICE;
+  }
+  return VCPHIP_OK;
+}
+
+int to_host(vcphip_ctx* c, const double* d, double* h, int ld, int rows, int cols, size_t esize) {
+  if (rows > 0 && cols > 0) {
+    if (hipMemcpy2DAsync(h, esize * ld, d, esize * rows, esize * rows, cols, hipMemcpyDeviceToHost, c->stream) !=
+        hipSuccess)
+      return VCPHIP_ERR_DEVICE;
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return VCPHIP_ERR_DEVICE;
+  return VCPHIP_OK;
+}
+
+}  // namespace
+
+#define VCPHIP_TRY(x)          \
+  do {                         \
+    const int _rc = (x);       \
+    if (_rc) return _rc;       \
+  } while (0)
+
+extern "C" int vcphip_igemm(vcphip_ctx* c, int form, int m, int n, int k, const double* IA, int lda, const double* IB,
+                            int ldb, double* C, int ldc) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (!dims_ok(m, n, k) || lda < (m > 1 ? m : 1) || ldb < (k > 1 ? k : 1) || ldc < (m > 1 ? m : 1))
+    return VCPHIP_ERR_DIMENSION;
+  if (m == 0 || n == 0) return VCPHIP_OK;
+  hipSetDevice(c->device);
+  double *dA, *dB, *dC;
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HA, IA, lda, m, k, 16, &dA));
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HB, IB, ldb, k, n, 16, &dB));
+  void* p;
+  VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HC, 16 * (size_t)m * n, &p));
+  dC = (double*)p;
+  VCPHIP_TRY(vcphip_igemm_dev(c, form, m, n, k, dA, m > 0 ? m : 1, dB, k > 0 ? k : 1, dC, m));
+  return to_host(c, dC, C, ldc, m, n, 16);
+}
+
+extern "C" int vcphip_igemm_ip(vcphip_ctx* c, int m, int n, int k, const double* IA, int lda, const double* B, int ldb,
+                               double* C, int ldc) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (!dims_ok(m, n, k) || lda < (m > 1 ? m : 1) || ldb < (k > 1 ? k : 1) || ldc < (m > 1 ? m : 1))
+    return VCPHIP_ERR_DIMENSION;
+  if (m == 0 || n == 0) return VCPHIP_OK;
+  hipSetDevice(c->device);
+  double *dA, *dB;
+  void* p;
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HA, IA, lda, m, k, 16, &dA));
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HB, B, ldb, k, n, 8, &dB));
+  VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HC, 16 * (size_t)m * n, &p));
+  VCPHIP_TRY(vcphip_igemm_ip_dev(c, m, n, k, dA, m, dB, k > 0 ? k : 1, (double*)p, m));
+  return to_host(c, (double*)p, C, ldc, m, n, 16);
+}
+
+extern "C" int vcphip_igemm_pi(vcphip_ctx* c, int m, int n, int k, const double* B, int ldb, const double* IA, int lda,
+                               double* C, int ldc) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (!dims_ok(m, n, k) || ldb < (m > 1 ? m : 1) || lda < (k > 1 ? k : 1) || ldc < (m > 1 ? m : 1))
+    return VCPHIP_ERR_DIMENSION;
+  if (m == 0 || n == 0) return VCPHIP_OK;
+  hipSetDevice(c->device);
+  double *dA, *dB;
+  void* p;
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HA, B, ldb, m, k, 8, &dB));
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HB, IA, lda, k, n, 16, &dA));
+  VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HC, 16 * (size_t)m * n, &p));
+  VCPHIP_TRY(vcphip_igemm_pi_dev(c, m, n, k, dB, m, dA, k > 0 ? k : 1, (double*)p, m));
+  return to_host(c, (double*)p, C, ldc, m, n, 16);
+}
+
+extern "C" int vcphip_igemm_tn(vcphip_ctx* c, int k, int n, const double* IA, int lda, double* C, int ldc) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (k < 0 || n < 0 || lda < (k > 1 ? k : 1) || ldc < (n > 1 ? n : 1)) return VCPHIP_ERR_DIMENSION;
+  if (n == 0) return VCPHIP_OK;
+  hipSetDevice(c->device);
+  double* dA;
+  void* p;
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HA, IA, lda, k, n, 16, &dA));
+  VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HC, 16 * (size_t)n * n, &p));
+  VCPHIP_TRY(vcphip_igemm_tn_dev(c, k, n, dA, k > 0 ? k : 1, (double*)p, n));
+  return to_host(c, (double*)p, C, ldc, n, n, 16);
+}
+
+extern "C" int vcphip_vgemm_tn(vcphip_ctx* c, int k, int n, const double* A, int lda, double* C, int ldc) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (k < 0 || n < 0 || lda < (k > 1 ? k : 1) || ldc < (n > 1 ? n : 1)) return VCPHIP_ERR_DIMENSION;
+  if (n == 0) return VCPHIP_OK;
+  hipSetDevice(c->device);
+  double* dA;
+  void* p;
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HA, A, lda, k, n, 8, &dA));
+  VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HC, 16 * (size_t)n * n, &p));
+  VCPHIP_TRY(vcphip_vgemm_tn_dev(c, k, n, dA, k > 0 ? k : 1, (double*)p, n));
+  return to_host(c, (double*)p, C, ldc, n, n, 16);
+}
+
+extern "C" int vcphip_gemm_rurd(vcphip_ctx* c, int m, int n, int k, const double* A, int lda, const double* B, int ldb,
+                                double* Cup, double* Cdn, int ldc, int accumulate) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (!dims_ok(m, n, k) || lda < (m > 1 ? m : 1) || ldb < (k > 1 ? k : 1) || ldc < (m > 1 ? m : 1))
+    return VCPHIP_ERR_DIMENSION;
+  if (m == 0 || n == 0) return VCPHIP_OK;
+  hipSetDevice(c->device);
+  double *dA, *dB, *dU, *dD;
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HA, A, lda, m, k, 8, &dA));
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HB, B, ldb, k, n, 8, &dB));
+  if (accumulate) {
+    VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HC, Cup, ldc, m, n, 8, &dU));
+    VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HC2, Cdn, ldc, m, n, 8, &dD));
+  } else {
+    void *p, *q;
+    VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HC, 8 * (size_t)m * n, &p));
+    VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HC2, 8 * (size_t)m * n, &q));
+    dU = (double*)p;
+    dD = (double*)q;
+  }
+  VCPHIP_TRY(vcphip_gemm_rurd_dev(c, m, n, k, dA, m, dB, k > 0 ? k : 1, dU, dD, m, accumulate));
+  VCPHIP_TRY(to_host(c, dU, Cup, ldc, m, n, 8));
+  return to_host(c, dD, Cdn, ldc, m, n, 8);
+}
+
+extern "C" int vcphip_ddgemm(vcphip_ctx* c, int m, int n, int k, const double* A, int lda, const double* B, int ldb,
+                             double* C, int ldc) {
+  (void)m; (void)n; (void)k; (void)A; (void)lda; (void)B; (void)ldb; (void)C; (void)ldc;
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  return VCPHIP_ERR_UNSUPPORTED;
+}
+
+extern "C" int vcphip_ddgemm_tn(vcphip_ctx* c, int k, int n, const double* A, int lda, double* C, int ldc) {
+  (void)k; (void)n; (void)A; (void)lda; (void)C; (void)ldc;
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  return VCPHIP_ERR_UNSUPPORTED;
+}
+
+extern "C" int vcphip_residual_batched(vcphip_ctx* c, int count, const int* n, const double* const* R, const int* ldr,
+                                       const double* const* IA, const int* lda, double* const* C, const int* ldc,
+                                       double* normub) {
+  (void)count; (void)n; (void)R; (void)ldr; (void)IA; (void)lda; (void)C; (void)ldc; (void)normub;
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  return VCPHIP_ERR_UNSUPPORTED;
+}

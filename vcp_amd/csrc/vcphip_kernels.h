@@ -1,0 +1,64 @@
+// Launchers of the libvcphip kernels (host-callable, C++ linkage inside the
+// library).  Every launcher returns 0 or a negative VCPHIP_ERR_* code and only
+// enqueues work on the given stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+enum { VCPHIP_DR_II = 0, VCPHIP_DR_IP = 1, VCPHIP_DR_PI = 2, VCPHIP_DR_PP = 3 };
+
+// plain FP64 directed-rounding GEMM family (kern_drgemm.hip)
+int vcphip_launch_drgemm(int kind, int m, int n, int k, const double* A, long lda, const double* B,
+                         long ldb, double* C, long ldc, double* C2, int accumulate, hipStream_t s);
+// C = A^T A forms (pidblas::mulltmm / vmulmm), kern_drgemm.hip
+int vcphip_launch_drsyrk(int interval, int k, int n, const double* A, long lda, double* C, long ldc,
+                         hipStream_t s);
+
+// ---- Ozaki scheme II (kern_ozaki.hip) ------------------------------------
+#define VCPHIP_OZ_MAXMOD 20
+#define VCPHIP_OZ_CHUNK 256   // k-chunk of the upward 1-norm partial sums
+#define VCPHIP_OZ_PADMN 256   // m and n are padded to multiples of this
+#define VCPHIP_OZ_PADK 64     // k is padded to a multiple of this
+#define VCPHIP_OZ_NLIMB 5     // 32-bit limbs of M (M < 2^160 for N <= 20)
+
+struct VcphipOzParams {
+  int nmod;                    // number of moduli N
+  int beta;                    // scaled integers satisfy |A'| <= 2^beta
+  int mod[VCPHIP_OZ_MAXMOD];   // pairwise coprime moduli (mod[0] = 256)
+  float rcpf[VCPHIP_OZ_MAXMOD];
+  double rcpd[VCPHIP_OZ_MAXMOD];
+  int w21[VCPHIP_OZ_MAXMOD];   // 2^21 mod m_t
+  int w42[VCPHIP_OZ_MAXMOD];   // 2^42 mod m_t
+  int pinv[VCPHIP_OZ_MAXMOD];  // (m_0 ... m_{t-1})^{-1} mod m_t
+  int negP[VCPHIP_OZ_MAXMOD][VCPHIP_OZ_MAXMOD];  // [s][t] = -(m_0 ... m_{s-1}) mod m_t
+  uint32_t halfM[VCPHIP_OZ_NLIMB];  // M/2 as little-endian 32-bit limbs
+  uint32_t Mlimb[VCPHIP_OZ_NLIMB];  // M
+  double c127;                 // RU(1/127^2)
+};
+
+// per-row (A) / per-column (B) statistics of a midpoint-radius operand
+struct VcphipOzSide {
+  int* e;       // mid exponent: max|mid| < 2^e
+  int* g;       // digit scale of |mid|+rad
+  int* h;       // digit scale of rad
+  int* zero;    // mid identically zero
+  double* n1;   // upward 1-norm of |mid|
+  double* part; // partials [4][nchunk][rows]
+};
+
+int vcphip_oz_stats(const double* X, long rs, long cs, int rows, int len, VcphipOzSide side,
+                    hipStream_t s);
+// planes: N residue planes followed by two radius-digit planes, each
+// [rows_pad][kpad] int8 with k contiguous
+int vcphip_oz_split_rows(const double* A, long lda, int m, int k, int mpad, int kpad,
+                         const VcphipOzParams& P, VcphipOzSide side, int8_t* planes, hipStream_t s);
+int vcphip_oz_split_cols(const double* B, long ldb, int k, int n, int npad, int kpad,
+                         const VcphipOzParams& P, VcphipOzSide side, int8_t* planes, hipStream_t s);
+// nmod modular products (u8 residues U) plus nrad radius products (int32 Z)
+int vcphip_oz_gemm(int mpad, int npad, int kpad, int nmod, int nrad, const int8_t* Aplanes,
+                   const int8_t* Bplanes, const VcphipOzParams& P, uint8_t* U, int32_t* Z, int variant,
+                   hipStream_t s);
+int vcphip_oz_recon(int m, int n, int k, int mpad, int npad, const VcphipOzParams& P, VcphipOzSide sa,
+                    VcphipOzSide sb, const uint8_t* U, const int32_t* Z, double* C, long ldc,
+                    hipStream_t s);
