@@ -611,10 +611,15 @@ static int64_t rnint_floor(double x) {
     return (int64_t)f + (r >= 0.5 ? 1 : 0);
 }
 
-/* upward digit in [0,127] of x >= 0 against scale 2^g (x <= 2^g) */
-static int digit_ru(double x, int g) {
-    const double y = mulpow2(x, -g) * 127.0;
-    return (int)ceil(y);
+/* two upward 7-bit digits of x >= 0 against 2^g (x < 2^g), under FE_UPWARD:
+ * t = RU(127 x 2^-g), h = floor(t), l = ceil(RU(127 (t - h))); then
+ * x <= (h + l/127) 2^g / 127 with h, l in [0, 127]. */
+static void digits2_ru(double x, int g, int* h, int* l) {
+    const double t = mulpow2(x, -g) * 127.0;
+    const double hf = floor(t);
+    const double r = (t - hf) * 127.0;
+    *h = (int)hf;
+    *l = (int)ceil(r);
 }
 
 /* X * 2^s rounded toward -inf (dir<0) or +inf (dir>0), exactly: keep the
@@ -647,11 +652,12 @@ static double conv_i128(i128 X, int s, int dir) {
     return neg ? -v : v;
 }
 
+enum { OZ_MIDZERO = 1, OZ_SUMZERO = 2, OZ_RADZERO = 4 };
 typedef struct {
-    int beta, nmod;
-    int *e, *g, *h;      /* rows of A: mid exponent, |m|+r digit scale, r digit scale */
+    int beta;
+    int *e, *g, *h;      /* mid exponent, |m|+r digit scale, r digit scale */
     double *n1;          /* upward 1-norm of |mid| */
-    int *zero;           /* row of mid identically zero */
+    int *flags;          /* OZ_*ZERO: mid / |mid|+rad / rad identically zero */
 } oz_side;
 
 /* statistics of the rows of a mid/rad operand: element c of row r is at
@@ -664,7 +670,7 @@ static void oz_stats(int rows, int len, const double* M, const double* R, size_t
     S->e = (int*)malloc(sizeof(int) * rows);
     S->g = (int*)malloc(sizeof(int) * rows);
     S->h = (int*)malloc(sizeof(int) * rows);
-    S->zero = (int*)malloc(sizeof(int) * rows);
+    S->flags = (int*)malloc(sizeof(int) * rows);
     S->n1 = dalloc(rows);
     const int old = fegetround();
     fesetround(FE_UPWARD);
@@ -684,7 +690,7 @@ static void oz_stats(int rows, int len, const double* M, const double* R, size_t
             }
             tot = tot + part;
         }
-        S->zero[r] = (mu == 0.0);
+        S->flags[r] = (mu == 0.0 ? OZ_MIDZERO : 0) | (smax == 0.0 ? OZ_SUMZERO : 0) | (rmax == 0.0 ? OZ_RADZERO : 0);
         S->e[r] = mu > 0.0 ? fexp(mu) : 0;
         S->g[r] = smax > 0.0 ? fexp(smax) : 0;
         S->h[r] = rmax > 0.0 ? fexp(rmax) : 0;
@@ -692,7 +698,7 @@ static void oz_stats(int rows, int len, const double* M, const double* R, size_t
     }
     fesetround(old);
 }
-static void oz_free(oz_side* S) { free(S->e); free(S->g); free(S->h); free(S->zero); free(S->n1); }
+static void oz_free(oz_side* S) { free(S->e); free(S->g); free(S->h); free(S->flags); free(S->n1); }
 
 void orc_ozaki_scaled_rows(int m, int k, const double* A, int lda, int beta, int64_t* Ap, int* e_out) {
     double *mA = dalloc((size_t)m * k), *rA = dalloc((size_t)m * k);
@@ -703,7 +709,7 @@ void orc_ozaki_scaled_rows(int m, int k, const double* A, int lda, int beta, int
     fesetround(FE_UPWARD);
     for (int j = 0; j < k; ++j)
         for (int i = 0; i < m; ++i)
-            Ap[i + (size_t)m * j] = S.zero[i] ? 0 : rnint_floor(mulpow2(mA[i + (size_t)m * j], beta - S.e[i]));
+            Ap[i + (size_t)m * j] = (S.flags[i] & OZ_MIDZERO) ? 0 : rnint_floor(mulpow2(mA[i + (size_t)m * j], beta - S.e[i]));
     fesetround(old);
     for (int i = 0; i < m; ++i) e_out[i] = S.e[i];
     oz_free(&S);
@@ -718,18 +724,22 @@ void orc_ozaki_scaled_cols(int k, int n, const double* B, int ldb, int beta, int
     fesetround(FE_UPWARD);
     for (int j = 0; j < n; ++j)
         for (int i = 0; i < k; ++i)
-            Bp[i + (size_t)k * j] = S.zero[j] ? 0 : rnint_floor(mulpow2(mB[i + (size_t)k * j], beta - S.e[j]));
+            Bp[i + (size_t)k * j] = (S.flags[j] & OZ_MIDZERO) ? 0 : rnint_floor(mulpow2(mB[i + (size_t)k * j], beta - S.e[j]));
     fesetround(old);
     for (int j = 0; j < n; ++j) f_out[j] = S.e[j];
     oz_free(&S);
     free(mB); free(rB);
 }
 
+/* The product IA*IB of pidblas::mulmm (vcp/pidblas.hpp:239-294) through the
+ * Ozaki-II route: exact integer product of the scaled midpoints (the value
+ * the CRT reconstruction recovers), one directed conversion each way, the
+ * scaling-error bound E and the upward two-digit bound of the radius term
+ * (|mA|+rA) rB + rA |mB| (:277-283). */
 int orc_ozaki_igemm(int m, int n, int k, const double* A, int lda, const double* B, int ldb,
                     double* C, int ldc, int nmod) {
     const int beta = orc_ozaki_beta(k, nmod);
     if (beta < 1) return -2;
-    if (beta > 62) return -2;
     const size_t mk = (size_t)m * k, kn = (size_t)k * n;
     double *mA = dalloc(mk), *rA = dalloc(mk), *mB = dalloc(kn), *rB = dalloc(kn);
     midrad_mat(m, k, A, lda, mA, rA);
@@ -737,36 +747,40 @@ int orc_ozaki_igemm(int m, int n, int k, const double* A, int lda, const double*
     oz_side SA, SB;
     oz_stats(m, k, mA, rA, 1, (size_t)m, beta, &SA);
     oz_stats(n, k, mB, rB, (size_t)k, 1, beta, &SB);
-    /* column digit scales of B: xi from rB, zeta from |mB| */
-    int* xi = (int*)malloc(sizeof(int) * n);
-    int* zeta = (int*)malloc(sizeof(int) * n);
-    for (int j = 0; j < n; ++j) { xi[j] = SB.h[j]; zeta[j] = SB.zero[j] ? 0 : SB.e[j]; }
-    int64_t* Ap = (int64_t*)malloc(sizeof(int64_t) * mk); /* row-major m x k */
+    int64_t* Ap = (int64_t*)malloc(sizeof(int64_t) * mk); /* row i contiguous */
     int64_t* Bp = (int64_t*)malloc(sizeof(int64_t) * kn); /* col j contiguous */
-    int8_t *Sd = (int8_t*)malloc(mk), *Rd = (int8_t*)malloc(mk);
-    int8_t *BRd = (int8_t*)malloc(kn), *BMd = (int8_t*)malloc(kn);
-    double c127;
+    /* digit planes (h, l) of |mA|+rA, rA (rows of A) and rB, |mB| (cols of B) */
+    int8_t *Sh = (int8_t*)malloc(mk), *Sl = (int8_t*)malloc(mk), *Rh = (int8_t*)malloc(mk), *Rl = (int8_t*)malloc(mk);
+    int8_t *BRh = (int8_t*)malloc(kn), *BRl = (int8_t*)malloc(kn), *BMh = (int8_t*)malloc(kn), *BMl = (int8_t*)malloc(kn);
+    double c127, c1;
     const int old = fegetround();
     fesetround(FE_UPWARD);
     {
-        volatile double one = 1.0, d = 16129.0;
+        volatile double one = 1.0, d = 16129.0, d1 = 127.0;
         c127 = one / d;
+        c1 = one / d1;
     }
     for (int i = 0; i < m; ++i)
         for (int c = 0; c < k; ++c) {
             const double am = mA[i + (size_t)m * c], ar = rA[i + (size_t)m * c];
             const double as = fabs(am) + ar;
-            Ap[(size_t)i * k + c] = SA.zero[i] ? 0 : rnint_floor(mulpow2(am, beta - SA.e[i]));
-            Sd[(size_t)i * k + c] = (int8_t)(as > 0.0 ? digit_ru(as, SA.g[i]) : 0);
-            Rd[(size_t)i * k + c] = (int8_t)(ar > 0.0 ? digit_ru(ar, SA.h[i]) : 0);
+            const size_t o = (size_t)i * k + c;
+            int h, l;
+            Ap[o] = (SA.flags[i] & OZ_MIDZERO) ? 0 : rnint_floor(mulpow2(am, beta - SA.e[i]));
+            digits2_ru(as, SA.g[i], &h, &l); Sh[o] = (int8_t)h; Sl[o] = (int8_t)l;
+            digits2_ru(ar, SA.h[i], &h, &l); Rh[o] = (int8_t)h; Rl[o] = (int8_t)l;
         }
-    for (int j = 0; j < n; ++j)
+    for (int j = 0; j < n; ++j) {
+        const int zc = (SB.flags[j] & OZ_MIDZERO) != 0;
         for (int c = 0; c < k; ++c) {
             const double bm = mB[c + (size_t)k * j], br = rB[c + (size_t)k * j];
-            Bp[(size_t)j * k + c] = SB.zero[j] ? 0 : rnint_floor(mulpow2(bm, beta - SB.e[j]));
-            BRd[(size_t)j * k + c] = (int8_t)(br > 0.0 ? digit_ru(br, xi[j]) : 0);
-            BMd[(size_t)j * k + c] = (int8_t)(bm != 0.0 ? digit_ru(fabs(bm), zeta[j]) : 0);
+            const size_t o = (size_t)j * k + c;
+            int h, l;
+            Bp[o] = zc ? 0 : rnint_floor(mulpow2(bm, beta - SB.e[j]));
+            digits2_ru(br, SB.h[j], &h, &l); BRh[o] = (int8_t)h; BRl[o] = (int8_t)l;
+            digits2_ru(fabs(bm), zc ? 0 : SB.e[j], &h, &l); BMh[o] = (int8_t)h; BMl[o] = (int8_t)l;
         }
+    }
     fesetround(old);
     const int nt = orc_get_threads();
 #pragma omp parallel num_threads(nt)
@@ -777,25 +791,31 @@ int orc_ozaki_igemm(int m, int n, int k, const double* A, int lda, const double*
         for (int j = 0; j < n; ++j) {
             for (int i = 0; i < m; ++i) {
                 i128 X = 0;
-                int64_t z1 = 0, z2 = 0;
-                const int64_t* ap = Ap + (size_t)i * k;
-                const int64_t* bp = Bp + (size_t)j * k;
-                const int8_t *sd = Sd + (size_t)i * k, *rd = Rd + (size_t)i * k;
-                const int8_t *brd = BRd + (size_t)j * k, *bmd = BMd + (size_t)j * k;
+                int64_t z1h = 0, z1m = 0, z2h = 0, z2m = 0;
+                const size_t ai = (size_t)i * k, bj = (size_t)j * k;
                 for (int c = 0; c < k; ++c) {
-                    X += (i128)ap[c] * bp[c];
-                    z1 += (int64_t)sd[c] * brd[c];
-                    z2 += (int64_t)rd[c] * bmd[c];
+                    X += (i128)Ap[ai + c] * Bp[bj + c];
+                    z1h += (int64_t)Sh[ai + c] * BRh[bj + c];
+                    z1m += (int64_t)Sh[ai + c] * BRl[bj + c] + (int64_t)Sl[ai + c] * BRh[bj + c];
+                    z2h += (int64_t)Rh[ai + c] * BMh[bj + c];
+                    z2m += (int64_t)Rh[ai + c] * BMl[bj + c] + (int64_t)Rl[ai + c] * BMh[bj + c];
                 }
+                const int fa = SA.flags[i], fb = SB.flags[j];
                 const int sc = SA.e[i] + SB.e[j] - 2 * beta;
                 const double lo_mid = conv_i128(X, sc, -1);
                 const double hi_mid = conv_i128(X, sc, 1);
-                /* radius term (|mA|+rA) rB + rA |mB| as upward digit products */
-                const double t1 = mulpow2((double)z1, SA.g[i] + xi[j]) * c127;
-                const double t2 = mulpow2((double)z2, SA.h[i] + zeta[j]) * c127;
+                double t1 = 0.0, t2 = 0.0;
+                if (!((fa & OZ_SUMZERO) || (fb & OZ_RADZERO))) {
+                    const double u = ((double)z1h + (double)z1m * c1) + (double)k * c127;
+                    t1 = mulpow2(u, SA.g[i] + SB.h[j]) * c127;
+                }
+                if (!((fa & OZ_RADZERO) || (fb & OZ_MIDZERO))) {
+                    const double u = ((double)z2h + (double)z2m * c1) + (double)k * c127;
+                    t2 = mulpow2(u, SA.h[i] + SB.e[j]) * c127;
+                }
                 const double rt = t1 + t2;
                 double E = 0.0;
-                if (!SA.zero[i] && !SB.zero[j]) {
+                if (!(fa & OZ_MIDZERO) && !(fb & OZ_MIDZERO)) {
                     const double ka = mulpow2((double)k, SA.e[i] - beta - 1);
                     const double e1 = mulpow2(SA.n1[i] + ka, SB.e[j] - beta - 1);
                     const double e2 = mulpow2(SB.n1[j], SA.e[i] - beta - 1);
@@ -809,7 +829,7 @@ int orc_ozaki_igemm(int m, int n, int k, const double* A, int lda, const double*
         }
         fesetround(o2);
     }
-    free(Ap); free(Bp); free(Sd); free(Rd); free(BRd); free(BMd); free(xi); free(zeta);
+    free(Ap); free(Bp); free(Sh); free(Sl); free(Rh); free(Rl); free(BRh); free(BRl); free(BMh); free(BMl);
     oz_free(&SA); oz_free(&SB);
     free(mA); free(rA); free(mB); free(rB);
     return 0;

@@ -1,0 +1,302 @@
+"""Pins of the CPU oracle (no GPU): every oracle function is checked against
+something it does not compute itself -- exact rational arithmetic
+(fractions.Fraction), the reference's own rmatmul/udmatmul (golden fixtures
+and, when built, oracle/_ref), the reference's rounding-check patterns, or
+closed forms of the reference (pddblas split parameters)."""
+import math
+import os
+from fractions import Fraction as F
+
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+from vcp_amd import workloads as W
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "rmatmul_ref.npz")
+
+
+# ---- exact rational helpers (independent of the oracle) --------------------
+def round_dir(q, direction):
+    """round a Fraction to a double: -1 down, +1 up, 0 nearest-even"""
+    x = float(q)                      # nearest (correctly rounded)
+    if direction == 0 or F(x) == q:
+        return x
+    if direction > 0:
+        return x if F(x) > q else math.nextafter(x, math.inf)
+    return x if F(x) < q else math.nextafter(x, -math.inf)
+
+
+def chain(a, b, direction, c0=0.0):
+    c = c0
+    for x, y in zip(a, b):
+        c = round_dir(F(x) * F(y) + F(c), direction)
+    return c
+
+
+def test_exact_dot_against_fractions():
+    rng = np.random.default_rng(1)
+    for trial in range(200):
+        k = int(rng.integers(1, 30))
+        lo, hi = (-1074, 1000) if trial % 4 == 0 else (-40, 40)
+        a = rng.uniform(-1, 1, k) * 2.0 ** rng.integers(lo // 2, hi // 2, k)
+        b = rng.uniform(-1, 1, k) * 2.0 ** rng.integers(lo // 2, hi // 2, k)
+        ex = sum(F(x) * F(y) for x, y in zip(a, b))
+        rd, ru, rn = O.exact_dot(a, b)
+        assert rd == round_dir(ex, -1) and ru == round_dir(ex, 1) and rn == round_dir(ex, 0)
+
+
+def test_exact_hull_against_fractions():
+    rng = np.random.default_rng(2)
+    for trial in range(100):
+        k = int(rng.integers(1, 12))
+        a = np.sort(rng.uniform(-1, 1, (k, 2)) * 2.0 ** rng.integers(-20, 20, (k, 1)), axis=1)
+        b = np.sort(rng.uniform(-1, 1, (k, 2)) * 2.0 ** rng.integers(-20, 20, (k, 1)), axis=1)
+        lo = sum(min(F(x) * F(y) for x in ai for y in bi) for ai, bi in zip(a, b))
+        hi = sum(max(F(x) * F(y) for x in ai for y in bi) for ai, bi in zip(a, b))
+        okl, oku, hl, hh = O.exact_hull(a, b, round_dir(lo, -1), round_dir(hi, 1))
+        assert okl and oku and hl == round_dir(lo, -1) and hh == round_dir(hi, 1)
+        # one ulp inside the hull must be rejected (a plausible off-by-one)
+        okl, oku, _, _ = O.exact_hull(a, b, math.nextafter(round_dir(lo, 1), math.inf),
+                                      math.nextafter(round_dir(hi, -1), -math.inf))
+        assert not okl and not oku
+
+
+@pytest.mark.parametrize("mode", [1, -1, 0])
+def test_rgemm_is_the_fma_chain(mode):
+    rng = np.random.default_rng(3)
+    A = np.asfortranarray(rng.uniform(-1, 1, (4, 9)) * 2.0 ** rng.integers(-30, 30, (4, 9)))
+    B = np.asfortranarray(rng.uniform(-1, 1, (9, 3)) * 2.0 ** rng.integers(-30, 30, (9, 3)))
+    C = O.rgemm(A, B, mode)
+    for i in range(4):
+        for j in range(3):
+            assert C[i, j] == chain(A[i], B[:, j], mode)
+
+
+def test_rgemm_matches_reference_golden():
+    g = np.load(GOLDEN)
+    for idx in range(int(g["ncases"])):
+        A, B = g["A%d" % idx], g["B%d" % idx]
+        assert np.array_equal(O.rgemm(A, B, 1), g["CU%d" % idx])
+        assert np.array_equal(O.rgemm(A, B, -1), g["CD%d" % idx])
+        assert np.array_equal(O.rgemm(A, B, 0), g["CN%d" % idx])
+        if "UU%d" % idx in g:
+            C0 = g["C0%d" % idx]
+            assert np.array_equal(O.rgemm(A, B, 1, C=C0.copy(order="F"), accumulate=True), g["UU%d" % idx])
+            assert np.array_equal(O.rgemm(A, B, -1, C=C0.copy(order="F"), accumulate=True), g["UD%d" % idx])
+    assert (g["KAT_CU"] == 1 + 2.0 ** -52).all() and (g["KAT_CD"] == 1.0).all()
+
+
+@pytest.mark.skipif(O.ref_lib() is None, reason="oracle/_ref not built (needs the reference sources)")
+def test_rgemm_matches_reference_rmatmul_live():
+    rng = np.random.default_rng(4)
+    for (m, n, k) in [(7, 5, 3), (100, 37, 259), (300, 200, 513)]:
+        A = np.asfortranarray(rng.uniform(-1, 1, (m, k)) * 2.0 ** rng.integers(-20, 20, (m, k)))
+        B = np.asfortranarray(rng.uniform(-1, 1, (k, n)) * 2.0 ** rng.integers(-20, 20, (k, n)))
+        for mode in (1, -1, 0):
+            assert np.array_equal(O.rgemm(A, B, mode), O.ref_rmatmul(A, B, mode))
+
+
+def test_rounding_check_patterns():
+    # test_matrix/Check_pidblas_rounding.cpp:14-71, n reduced
+    n = 40
+    for i in range(n):
+        A = np.zeros((n, n), order="F")
+        B = np.zeros((n, n), order="F")
+        A[:, i] = 1.0 / 3.0
+        B[i, :] = 1.0 / 3.0
+        assert (O.rgemm(A, B, 1) > O.rgemm(A, B, -1)).all()
+    for i in range(2, n):
+        A = np.zeros((n, n), order="F")
+        B = np.zeros((n, n), order="F")
+        A[:, 1] = 1.0
+        B[1, :] = 1.0
+        A[:, i] = 2.0 ** -27
+        B[i, :] = 2.0 ** -27
+        assert (O.rgemm(A, B, 1) == 1 + 2.0 ** -52).all() and (O.rgemm(A, B, -1) == 1.0).all()
+
+
+def test_midrad_contract():
+    rng = np.random.default_rng(5)
+    cases = [(1.0, 2.0), (-1.0, 1.0), (0.1, 0.1), (5e-324, 5e-324), (5e-324, 1e-323), (-3.0, -2.5),
+             (-1.7976931348623157e308, 1.7976931348623157e308)]
+    for _ in range(300):
+        lo, hi = sorted(rng.uniform(-1, 1, 2) * 2.0 ** rng.integers(-1070, 1000, 2))
+        cases.append((lo, hi))
+    for lo, hi in cases:
+        m, r = O.midrad(lo, hi)
+        assert r >= 0
+        if lo == hi:
+            assert m == lo and r == 0.0
+        else:
+            assert F(m) - F(r) <= F(lo) and F(m) + F(r) >= F(hi)
+
+
+def _iv(rng, r, c, kind):
+    if kind == "wide":
+        return W.wide_exponent_intervals(rng, r, c)
+    if kind == "straddle":
+        return W.interval_from_midrad(rng.uniform(-1, 1, (r, c)), np.abs(rng.uniform(-1, 1, (r, c))))
+    return W.uniform_intervals(rng, r, c, -40)
+
+
+def _contains_all(IA, IB, C):
+    m, n = C.shape[:2]
+    ii, jj = np.meshgrid(np.arange(m), np.arange(n), indexing="ij")
+    ok, _, _ = O.check_containment(IA, IB, C, ii.ravel(), jj.ravel())
+    return ok.all()
+
+
+@pytest.mark.parametrize("kind", ["uniform", "wide", "straddle"])
+def test_pidblas_mulmm_encloses_and_follows_op_order(kind):
+    rng = np.random.default_rng(6)
+    IA, IB = _iv(rng, 20, 33, kind), _iv(rng, 33, 17, kind)
+    C = O.pidblas_mulmm(IA, IB)
+    assert _contains_all(IA, IB, C)
+    # op order: lower = RD(RD-chain(mA mB) - t), t = RU(RU-chain((|mA|+rA) rB) + RU-chain(rA |mB|))
+    mA, rA = np.vectorize(O.midrad)(IA[..., 0], IA[..., 1])
+    mB, rB = np.vectorize(O.midrad)(IB[..., 0], IB[..., 1])
+    sA = np.vectorize(lambda a, r: round_dir(abs(F(a)) + F(r), 1))(mA, rA)
+    for i in range(0, 20, 7):
+        for j in range(0, 17, 5):
+            up = chain(mA[i], mB[:, j], 1)
+            dn = chain(mA[i], mB[:, j], -1)
+            t1 = chain(sA[i], rB[:, j], 1)
+            t2 = chain(rA[i], np.abs(mB[:, j]), 1)
+            t = round_dir(F(t1) + F(t2), 1)
+            assert C[i, j, 1] == round_dir(F(up) + F(t), 1)
+            assert C[i, j, 0] == round_dir(F(dn) - F(t), -1)
+
+
+def test_pidblas_mixed_and_transposed_forms_enclose():
+    rng = np.random.default_rng(7)
+    IA = _iv(rng, 15, 22, "wide")
+    B = np.asfortranarray(rng.uniform(-1, 1, (22, 9)))
+    pts = lambda X: W.interval_from_midrad(X, np.zeros_like(X))  # noqa: E731
+    assert _contains_all(IA, pts(B), O.pidblas_mul_im_m(IA, B))
+    Bl = np.asfortranarray(rng.uniform(-1, 1, (9, 15)))
+    assert _contains_all(pts(Bl), IA, O.pidblas_mul_m_im(Bl, IA))
+    IAt = np.ascontiguousarray(IA.transpose(1, 0, 2).transpose(1, 0, 2))
+    C = O.pidblas_mulltmm(IA)
+    IAT = W._pairs(22, 15)
+    IAT[...] = IA.transpose(1, 0, 2)
+    assert _contains_all(IAT, IA, C) and np.array_equal(C, C.transpose(1, 0, 2))
+    del IAt
+    A = np.asfortranarray(rng.uniform(-1, 1, (22, 15)))
+    V = O.pidblas_vmulmm(A)
+    AT = W._pairs(15, 22)
+    AT[...] = pts(A).transpose(1, 0, 2)
+    assert _contains_all(AT, pts(A), V)
+
+
+def test_residual_and_norm_bound():
+    rng = np.random.default_rng(8)
+    R, IA = W.residual_problem(rng, 40)
+    C, nub = O.residual(R, IA)
+    n = 40
+    IR = W.interval_from_midrad(R, np.zeros_like(R))
+    RA = O.pidblas_mul_m_im(R, IA)
+    assert _contains_all(IR, IA, RA)
+    # C = RA - I on the diagonal, RD / RU
+    for i in range(n):
+        assert C[i, i, 0] == round_dir(F(RA[i, i, 0]) - 1, -1)
+        assert C[i, i, 1] == round_dir(F(RA[i, i, 1]) - 1, 1)
+    mag = np.maximum(np.abs(C[..., 0]), np.abs(C[..., 1]))
+    exact = max(sum(F(x) for x in row) for row in mag)
+    assert F(nub) >= exact and nub < 1.0
+
+
+def test_ozaki_scaled_integers_within_half():
+    rng = np.random.default_rng(9)
+    IA = _iv(rng, 9, 40, "wide")
+    beta = O.ozaki_beta(40, 16)
+    Ap, e = O.ozaki_scaled(IA, beta, rows=True)
+    mA, _ = np.vectorize(O.midrad)(IA[..., 0], IA[..., 1])
+    for i in range(9):
+        mu = np.max(np.abs(mA[i]))
+        assert 2.0 ** (e[i] - 1) <= mu < 2.0 ** e[i]
+        for c in range(40):
+            x = F(mA[i, c]) * F(2) ** (beta - int(e[i]))
+            assert abs(x - int(Ap[i, c])) <= F(1, 2) and abs(int(Ap[i, c])) <= 2 ** beta
+
+
+def test_ozaki_moduli_and_beta():
+    mods = O.ozaki_moduli(20)
+    assert mods[0] == 256 and all(math.gcd(a, b) == 1 for i, a in enumerate(mods) for b in mods[i + 1:])
+    for k in (1, 64, 1000, 8192, 65536):
+        for nmod in (12, 16, 20):
+            beta = O.ozaki_beta(k, nmod)
+            M = math.prod(O.ozaki_moduli(nmod))
+            assert k * 2 ** (2 * beta + 1) < M          # no wrap-around in the CRT range
+            assert k * 2 ** (2 * beta + 5) >= M or beta == 62  # and not wastefully small
+
+
+def test_crt_garner_reconstruction_identity():
+    """the mixed-radix reconstruction the kernels use recovers any integer in
+    (-M/2, M/2) from its residues (pure integer check of the formula)"""
+    rng = np.random.default_rng(10)
+    for nmod in (12, 16, 20):
+        mods = O.ozaki_moduli(nmod)
+        M = math.prod(mods)
+        P = [math.prod(mods[:s]) for s in range(nmod)]
+        for _ in range(200):
+            X = int(rng.integers(-2 ** 62, 2 ** 62)) * int(rng.integers(1, 2 ** 40)) % M
+            X = X - M if X > M // 2 else X
+            u = [X % m for m in mods]
+            v = []
+            for t, m in enumerate(mods):
+                acc = u[t] + sum(v[s] * ((m - P[s] % m) % m) for s in range(t))
+                v.append((acc % m) * pow(P[t] % m, -1, m) % m)
+            Y = sum(v[t] * P[t] for t in range(nmod))
+            Y = Y - M if Y > M // 2 else Y
+            assert Y == X
+
+
+@pytest.mark.parametrize("kind", ["uniform", "wide", "straddle"])
+def test_ozaki_restatement_encloses_and_is_not_wider(kind):
+    rng = np.random.default_rng(11)
+    IA, IB = _iv(rng, 24, 300, kind), _iv(rng, 300, 20, kind)
+    Z = O.ozaki_igemm(IA, IB, 16)
+    P = O.pidblas_mulmm(IA, IB)
+    assert _contains_all(IA, IB, Z)
+    ratio = (Z[..., 1] - Z[..., 0]) / np.maximum(P[..., 1] - P[..., 0], np.finfo(float).tiny)
+    assert ratio.max() <= 2.0
+
+
+def test_ozaki_point_and_zero_inputs():
+    rng = np.random.default_rng(12)
+    IA = _iv(rng, 10, 50, "uniform")
+    IA[..., 1] = IA[..., 0]
+    IB = _iv(rng, 50, 8, "uniform")
+    IB[..., 1] = IB[..., 0]
+    IA[3] = 0.0
+    IB[:, 2] = 0.0
+    Z = O.ozaki_igemm(IA, IB, 16)
+    assert _contains_all(IA, IB, Z)
+    assert (Z[3] == 0).all() and (Z[:, 2] == 0).all()
+
+
+def test_pddblas_split_parameters_closed_form():
+    # vcp/pddblas.hpp:54-78; the k=4096 / 1024 values of the dd workloads
+    assert O.dd_params(4096) == (32, 20, 7)
+    assert O.dd_params(1024) == (31, 21, 7)
+    for k in (1, 2, 3, 100, 8192, 1 << 20):
+        s, b, smax = O.dd_params(k)
+        rho = math.ceil(math.log2(k)) if k > 1 else 0
+        assert s == (51 + rho + 1) // 2 and b == max(1, 52 - s) and smax == 110 // b + 2
+        assert 104 + rho - 2 * s <= 53
+
+
+def test_pddblas_mulmm_accuracy_and_symmetry():
+    rng = np.random.default_rng(13)
+    A = W.dd_matrix(rng, 12, 64)
+    B = W.dd_matrix(rng, 64, 10)
+    C, na, nb = O.pddblas_mulmm(A, B)
+    assert 2 <= na <= O.dd_params(64)[2] and 2 <= nb <= O.dd_params(64)[2]
+    for i in range(12):
+        for j in range(10):
+            _, _, err = O.exact_dd_dot(A[i], B[:, j], C[i, j, 0], C[i, j, 1])
+            scale = float(np.sum(np.abs(A[i, :, 0]) * np.abs(B[:, j, 0])))
+            assert err <= 2.0 ** -100 * scale
+    At = O.pddblas_mulltmm(A)
+    assert np.array_equal(At, At.transpose(1, 0, 2))

@@ -73,7 +73,8 @@ __global__ __launch_bounds__(256) void k_oz_stats_final(int rows, int nchunk, co
     rmax = d > rmax ? d : rmax;
     tot = tot + s;
   }
-  side.zero[r] = mu == 0.0;
+  side.flags[r] = (mu == 0.0 ? VCPHIP_OZ_MIDZERO : 0) | (smax == 0.0 ? VCPHIP_OZ_SUMZERO : 0) |
+                  (rmax == 0.0 ? VCPHIP_OZ_RADZERO : 0);
   side.e[r] = mu > 0.0 ? vcphip_fexp(mu) : 0;
   side.g[r] = smax > 0.0 ? vcphip_fexp(smax) : 0;
   side.h[r] = rmax > 0.0 ? vcphip_fexp(rmax) : 0;
@@ -90,10 +91,14 @@ VCPHIP_DEVINL long long rnint_floor(double x) {
   return (long long)f + (r >= 0.5 ? 1 : 0);
 }
 
-// upward digit in [0, 127] of x >= 0 against 2^g (x < 2^g); round-up mode
-VCPHIP_DEVINL int digit_ru(double x, int g) {
-  const double y = vcphip_mulpow2(x, -g) * 127.0;
-  return (int)__builtin_ceil(y);
+// two upward 7-bit digits of x >= 0 against 2^g (x < 2^g), round-up mode:
+//   t = RU(127 x 2^-g), h = floor(t), l = ceil(RU(127 (t - h)))
+// so that x <= (h + l/127) 2^g / 127 with h, l in [0, 127].
+VCPHIP_DEVINL unsigned digits2_ru(double x, int g) {
+  const double t = vcphip_mulpow2(x, -g) * 127.0;
+  const double hf = __builtin_floor(t);
+  const double r = (t - hf) * 127.0;   // t - hf is exact
+  return (unsigned)hf | ((unsigned)__builtin_ceil(r) << 8);
 }
 
 // residue of a signed 63-bit integer in the symmetric int8 range of m
@@ -121,14 +126,14 @@ VCPHIP_DEVINL void split_parts(long long a, unsigned& c0, unsigned& c1, unsigned
 
 constexpr int SR_ROWS = 64, SR_K = 64, SR_LD = 68;   // LDS byte tile, odd-word row stride
 
-// rows of A (m x k, column-major intervals) -> planes [N+2][mpad][kpad]:
-// N residue planes of A', then the digits of |mid|+rad and of rad.
+// rows of A (m x k, column-major intervals) -> planes [N+4][mpad][kpad]:
+// N residue planes of A', then the hi/lo digits of |mid|+rad and of rad.
 __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict__ A, long lda, int m, int k,
                                                        int mpad, int kpad, VcphipOzParams P, VcphipOzSide side,
                                                        int8_t* __restrict__ planes) {
   vcphip_mode_round_up();
-  extern __shared__ __attribute__((aligned(16))) unsigned char s_tile[];   // [N+2][64][SR_LD]
-  const int nplane = P.nmod + 2;
+  extern __shared__ __attribute__((aligned(16))) unsigned char s_tile[];   // [N+4][64][SR_LD]
+  const int nplane = P.nmod + VCPHIP_OZ_NDIGIT;
   const int row0 = blockIdx.x * SR_ROWS, k0 = blockIdx.y * SR_K;
   const int tid = threadIdx.x;
   const int lr = tid % SR_ROWS, kq = tid / SR_ROWS;
@@ -137,7 +142,7 @@ __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict_
   const int e = rowok ? side.e[r] : 0;
   const int g = rowok ? side.g[r] : 0;
   const int h = rowok ? side.h[r] : 0;
-  const bool zr = rowok ? (side.zero[r] != 0) : true;
+  const bool zr = rowok ? (side.flags[r] & VCPHIP_OZ_MIDZERO) != 0 : true;
   for (int q = 0; q < SR_K / 4; ++q) {
     const int lk = kq + 4 * q;
     const int kk = k0 + lk;
@@ -156,8 +161,11 @@ __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict_
     split_parts(ap, c0, c1, c2, neg);
     for (int t = 0; t < P.nmod; ++t)
       s_tile[(t * SR_ROWS + lr) * SR_LD + lk] = (unsigned char)(signed char)sym_residue(c0, c1, c2, neg, t, P);
-    s_tile[(P.nmod * SR_ROWS + lr) * SR_LD + lk] = (unsigned char)(as > 0.0 ? digit_ru(as, g) : 0);
-    s_tile[((P.nmod + 1) * SR_ROWS + lr) * SR_LD + lk] = (unsigned char)(ar > 0.0 ? digit_ru(ar, h) : 0);
+    const unsigned ds = digits2_ru(as, g), dr = digits2_ru(ar, h);
+    s_tile[(P.nmod * SR_ROWS + lr) * SR_LD + lk] = (unsigned char)(ds & 0xff);
+    s_tile[((P.nmod + 1) * SR_ROWS + lr) * SR_LD + lk] = (unsigned char)(ds >> 8);
+    s_tile[((P.nmod + 2) * SR_ROWS + lr) * SR_LD + lk] = (unsigned char)(dr & 0xff);
+    s_tile[((P.nmod + 3) * SR_ROWS + lr) * SR_LD + lk] = (unsigned char)(dr >> 8);
   }
   __syncthreads();
   // write out: per plane 64 rows x 64 bytes, 16 bytes per thread
@@ -170,9 +178,9 @@ __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict_
   }
 }
 
-// columns of B (k x n, column-major intervals) -> planes [N+2][npad][kpad]:
-// N residue planes of B', then the digits of rad and of |mid|.  One thread
-// handles 16 consecutive k of one column (a 256-byte contiguous read).
+// columns of B (k x n, column-major intervals) -> planes [N+4][npad][kpad]:
+// N residue planes of B', then the hi/lo digits of rad and of |mid|.  One
+// thread handles 16 consecutive k of one column (a 256-byte contiguous read).
 __global__ __launch_bounds__(256) void k_oz_split_cols(const double* __restrict__ B, long ldb, int k, int n,
                                                        int npad, int kpad, VcphipOzParams P, VcphipOzSide side,
                                                        int8_t* __restrict__ planes) {
@@ -184,11 +192,11 @@ __global__ __launch_bounds__(256) void k_oz_split_cols(const double* __restrict_
   const bool colok = j < n;
   const int f = colok ? side.e[j] : 0;
   const int xi = colok ? side.h[j] : 0;
-  const bool zc = colok ? (side.zero[j] != 0) : true;
+  const bool zc = colok ? (side.flags[j] & VCPHIP_OZ_MIDZERO) != 0 : true;
   const int zeta = zc ? 0 : f;
   unsigned c0[16], c1[16], c2[16];
   bool ng[16];
-  unsigned dr[4] = {0, 0, 0, 0}, dm[4] = {0, 0, 0, 0};
+  unsigned drh[4] = {0, 0, 0, 0}, drl[4] = {0, 0, 0, 0}, dmh[4] = {0, 0, 0, 0}, dml[4] = {0, 0, 0, 0};
 #pragma unroll
   for (int u = 0; u < 16; ++u) {
     const int kk = grp * 16 + u;
@@ -202,10 +210,12 @@ __global__ __launch_bounds__(256) void k_oz_split_cols(const double* __restrict_
     vcphip_midrad_ru(lo, hi, bm, br);
     const long long bp = zc ? 0 : rnint_floor(vcphip_mulpow2(bm, P.beta - f));
     split_parts(bp, c0[u], c1[u], c2[u], ng[u]);
-    const unsigned d1 = (unsigned)(br > 0.0 ? digit_ru(br, xi) : 0);
-    const unsigned d2 = (unsigned)(bm != 0.0 ? digit_ru(__builtin_fabs(bm), zeta) : 0);
-    dr[u >> 2] |= d1 << (8 * (u & 3));
-    dm[u >> 2] |= d2 << (8 * (u & 3));
+    const unsigned d1 = digits2_ru(br, xi);
+    const unsigned d2 = digits2_ru(__builtin_fabs(bm), zeta);
+    drh[u >> 2] |= (d1 & 0xff) << (8 * (u & 3));
+    drl[u >> 2] |= (d1 >> 8) << (8 * (u & 3));
+    dmh[u >> 2] |= (d2 & 0xff) << (8 * (u & 3));
+    dml[u >> 2] |= (d2 >> 8) << (8 * (u & 3));
   }
   const long plane_stride = (long)npad * kpad;
   int8_t* dst = planes + (long)j * kpad + grp * 16;
@@ -216,8 +226,10 @@ __global__ __launch_bounds__(256) void k_oz_split_cols(const double* __restrict_
       wv[u >> 2] |= ((unsigned)(unsigned char)(signed char)sym_residue(c0[u], c1[u], c2[u], ng[u], t, P)) << (8 * (u & 3));
     *reinterpret_cast<uint4*>(dst + t * plane_stride) = make_uint4(wv[0], wv[1], wv[2], wv[3]);
   }
-  *reinterpret_cast<uint4*>(dst + P.nmod * plane_stride) = make_uint4(dr[0], dr[1], dr[2], dr[3]);
-  *reinterpret_cast<uint4*>(dst + (P.nmod + 1) * plane_stride) = make_uint4(dm[0], dm[1], dm[2], dm[3]);
+  *reinterpret_cast<uint4*>(dst + P.nmod * plane_stride) = make_uint4(drh[0], drh[1], drh[2], drh[3]);
+  *reinterpret_cast<uint4*>(dst + (P.nmod + 1) * plane_stride) = make_uint4(drl[0], drl[1], drl[2], drl[3]);
+  *reinterpret_cast<uint4*>(dst + (P.nmod + 2) * plane_stride) = make_uint4(dmh[0], dmh[1], dmh[2], dmh[3]);
+  *reinterpret_cast<uint4*>(dst + (P.nmod + 3) * plane_stride) = make_uint4(dml[0], dml[1], dml[2], dml[3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -227,20 +239,28 @@ __global__ __launch_bounds__(256) void k_oz_split_cols(const double* __restrict_
 // 256 consecutive elements, element (4*(l/16)+v, l%16) of the block at l*4+v,
 // which is exactly the v_mfma_i32_16x16x64_i8 accumulator order, so every
 // wave stores whole contiguous blocks.
-constexpr int GK = 64;        // k bytes per stage
-constexpr int GLD = GK + 16;  // LDS row stride (bytes): conflict-free 16-byte fragment reads
+// Workgroup: 4 waves as 2 x 2, wave tile (16 FM) x (16 FN), k stage of BK
+// bytes; operands staged global -> registers -> LDS with a double-buffered
+// LDS and the next stage's global loads in flight during the MFMAs.
+template <int BK>
+struct GemmLds {
+  static constexpr int LD = BK + 16;   // row stride (bytes): conflict-free 16-byte fragment reads
+};
 
-template <int FM, int FN>
+template <int FM, int FN, int BK>
 __global__ __launch_bounds__(256) void k_oz_gemm(int mpad, int npad, int kpad, int nmod, int nz,
                                                  const int8_t* __restrict__ Aplanes,
                                                  const int8_t* __restrict__ Bplanes, VcphipOzParams P,
                                                  uint8_t* __restrict__ U, int32_t* __restrict__ Z) {
   constexpr int WM = 16 * FM, WN = 16 * FN;    // wave tile
   constexpr int TMB = 2 * WM, TNB = 2 * WN;    // workgroup tile (2 x 2 waves)
-  constexpr int ACH = TMB * GK / 16 / 256;     // 16-byte chunks of A per thread per stage
-  constexpr int BCH = TNB * GK / 16 / 256;
-  __shared__ __attribute__((aligned(16))) int8_t sA[2][TMB * GLD];
-  __shared__ __attribute__((aligned(16))) int8_t sB[2][TNB * GLD];
+  constexpr int LD = GemmLds<BK>::LD;
+  constexpr int CPR = BK / 16;                 // 16-byte chunks per row and stage
+  constexpr int ACH = TMB * CPR / 256;         // chunks of A per thread per stage
+  constexpr int BCH = TNB * CPR / 256;
+  static_assert(ACH * 256 == TMB * CPR && BCH * 256 == TNB * CPR, "tile/thread mismatch");
+  __shared__ __attribute__((aligned(16))) int8_t sA[2][TMB * LD];
+  __shared__ __attribute__((aligned(16))) int8_t sB[2][TNB * LD];
 
   const int ntm = mpad / TMB, ntn = npad / TNB;
   const int tiles = ntm * ntn;
@@ -249,37 +269,33 @@ __global__ __launch_bounds__(256) void k_oz_gemm(int mpad, int npad, int kpad, i
   int tm, tn;
   vcphip_tile_coords(local, ntm, ntn, 8, tm, tn);
   const int row0 = tm * TMB, col0 = tn * TNB;
-  const int8_t* Ab = Aplanes + (long)z * mpad * kpad + (long)row0 * kpad;
-  const int8_t* Bb = Bplanes + (long)z * npad * kpad + (long)col0 * kpad;
+  // plane pairs of product z: a residue pair, a hi*hi digit pair, or the
+  // two cross pairs hi*lo + lo*hi of a radius factor accumulated together
+  int npass = 1, pa0 = z, pb0 = z, pa1 = z, pb1 = z;
+  if (z >= nmod) {
+    const int r = z - nmod, base = nmod + 2 * (r >> 1);
+    pa0 = pb0 = base;
+    if (r & 1) {
+      npass = 2;
+      pb0 = base + 1;
+      pa1 = base + 1;
+      pb1 = base;
+    }
+  }
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w & 1, wn = w >> 1;
-
-  uint4 ra[ACH], rb[BCH];
-  auto gload = [&](int k0) {
-#pragma unroll
-    for (int q = 0; q < ACH; ++q) {
-      const int c = tid + 256 * q, row = c >> 2, part = c & 3;
-      ra[q] = *reinterpret_cast<const uint4*>(Ab + (long)row * kpad + k0 + 16 * part);
-    }
-#pragma unroll
-    for (int q = 0; q < BCH; ++q) {
-      const int c = tid + 256 * q, row = c >> 2, part = c & 3;
-      rb[q] = *reinterpret_cast<const uint4*>(Bb + (long)row * kpad + k0 + 16 * part);
-    }
-  };
-  auto sstore = [&](int buf) {
-#pragma unroll
-    for (int q = 0; q < ACH; ++q) {
-      const int c = tid + 256 * q, row = c >> 2, part = c & 3;
-      *reinterpret_cast<uint4*>(&sA[buf][row * GLD + 16 * part]) = ra[q];
-    }
-#pragma unroll
-    for (int q = 0; q < BCH; ++q) {
-      const int c = tid + 256 * q, row = c >> 2, part = c & 3;
-      *reinterpret_cast<uint4*>(&sB[buf][row * GLD + 16 * part]) = rb[q];
-    }
-  };
+  // per-thread global/LDS chunk coordinates (same for every stage)
+  // chunk q of this thread: row tid/CPR + q*RPQ, 16-byte part tid%CPR
+  constexpr int RPQ = 256 / CPR;
+  const int g_off = (tid / CPR) * kpad + 16 * (tid % CPR);
+  const int l_off = (tid / CPR) * LD + 16 * (tid % CPR);
+  const long g_step = (long)RPQ * kpad;
+#define A_OFF(q) (g_off + (q) * g_step)
+#define L_OFF(q) (l_off + (q) * (RPQ * LD))
+  const int frow = lane & 15, fk = (lane >> 4) * 16;
+  const int fa_lds = (wm * WM + frow) * LD + fk;
+  const int fb_lds = (wn * WN + frow) * LD + fk;
 
   v4i acc[FM][FN];
 #pragma unroll
@@ -287,29 +303,58 @@ __global__ __launch_bounds__(256) void k_oz_gemm(int mpad, int npad, int kpad, i
 #pragma unroll
     for (int j = 0; j < FN; ++j) acc[i][j] = v4i{0, 0, 0, 0};
 
-  const int nk = kpad / GK;
-  gload(0);
-  sstore(0);
-  __syncthreads();
-  const int frow = lane & 15, fk = (lane >> 4) * 16;
-  for (int kt = 0; kt < nk; ++kt) {
-    const int buf = kt & 1;
-    if (kt + 1 < nk) gload((kt + 1) * GK);
-    v4i fa[FM], fb[FN];
+  const int nk = kpad / BK;
+  v4i ra[ACH], rb[BCH];
+  for (int pass = 0; pass < npass; ++pass) {
+    const int8_t* Ab = Aplanes + (long)(pass ? pa1 : pa0) * mpad * kpad + (long)row0 * kpad;
+    const int8_t* Bb = Bplanes + (long)(pass ? pb1 : pb0) * npad * kpad + (long)col0 * kpad;
 #pragma unroll
-    for (int i = 0; i < FM; ++i)
-      fa[i] = *reinterpret_cast<const v4i*>(&sA[buf][(wm * WM + 16 * i + frow) * GLD + fk]);
+    for (int q = 0; q < ACH; ++q) ra[q] = *reinterpret_cast<const v4i*>(Ab + A_OFF(q));
 #pragma unroll
-    for (int j = 0; j < FN; ++j)
-      fb[j] = *reinterpret_cast<const v4i*>(&sB[buf][(wn * WN + 16 * j + frow) * GLD + fk]);
+    for (int q = 0; q < BCH; ++q) rb[q] = *reinterpret_cast<const v4i*>(Bb + A_OFF(q));
 #pragma unroll
-    for (int i = 0; i < FM; ++i)
+    for (int q = 0; q < ACH; ++q) *reinterpret_cast<v4i*>(&sA[0][L_OFF(q)]) = ra[q];
 #pragma unroll
-      for (int j = 0; j < FN; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[j], acc[i][j], 0, 0, 0);
-    if (kt + 1 < nk) sstore(buf ^ 1);
+    for (int q = 0; q < BCH; ++q) *reinterpret_cast<v4i*>(&sB[0][L_OFF(q)]) = rb[q];
     __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+      const int buf = kt & 1;
+      const bool more = kt + 1 < nk;
+      if (more) {
+        const int k1 = (kt + 1) * BK;
+#pragma unroll
+        for (int q = 0; q < ACH; ++q) ra[q] = *reinterpret_cast<const v4i*>(Ab + A_OFF(q) + k1);
+#pragma unroll
+        for (int q = 0; q < BCH; ++q) rb[q] = *reinterpret_cast<const v4i*>(Bb + A_OFF(q) + k1);
+      }
+      const int8_t* la = sA[buf];
+      const int8_t* lb = sB[buf];
+#pragma unroll
+      for (int ks = 0; ks < BK / 64; ++ks) {
+        v4i fa[FM], fb[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) fa[i] = *reinterpret_cast<const v4i*>(la + fa_lds + 16 * i * LD + 64 * ks);
+#pragma unroll
+        for (int j = 0; j < FN; ++j) fb[j] = *reinterpret_cast<const v4i*>(lb + fb_lds + 16 * j * LD + 64 * ks);
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[j], acc[i][j], 0, 0, 0);
+      }
+      if (more) {
+        int8_t* sa = sA[buf ^ 1];
+        int8_t* sb = sB[buf ^ 1];
+#pragma unroll
+        for (int q = 0; q < ACH; ++q) *reinterpret_cast<v4i*>(sa + L_OFF(q)) = ra[q];
+#pragma unroll
+        for (int q = 0; q < BCH; ++q) *reinterpret_cast<v4i*>(sb + L_OFF(q)) = rb[q];
+      }
+      __syncthreads();
+    }
   }
+#undef A_OFF
+#undef L_OFF
 
   const long blocks_m = mpad / 16;
   if (z < nmod) {
@@ -453,7 +498,8 @@ __global__ __launch_bounds__(256) void k_oz_recon(int m, int n, int k, int mpad,
       mag[l] = (unsigned)(d + (borrow << 32));
     }
   }
-  const bool zr = sa.zero[i] != 0, zc = sb.zero[j] != 0;
+  const int fa = sa.flags[i], fb = sb.flags[j];
+  const bool zr = (fa & VCPHIP_OZ_MIDZERO) != 0, zc = (fb & VCPHIP_OZ_MIDZERO) != 0;
   const int ei = sa.e[i], fj = sb.e[j];
   const int sc = ei + fj - 2 * P.beta;
   // RD and RU of the exact midpoint product
@@ -465,11 +511,18 @@ __global__ __launch_bounds__(256) void k_oz_recon(int m, int n, int k, int mpad,
     lo_mid = -conv_limbs(mag, sc, true);
     hi_mid = -conv_limbs(mag, sc, false);
   }
-  // radius term (|mA|+rA) rB + rA |mB| from the digit products (pidblas :277-283)
-  const int z1 = Z[boff], z2 = Z[plane + boff];
-  const int zeta = zc ? 0 : fj;
-  const double t1 = vcphip_mulpow2((double)z1, sa.g[i] + sb.h[j]) * P.c127;
-  const double t2 = vcphip_mulpow2((double)z2, sa.h[i] + zeta) * P.c127;
+  // radius term (|mA|+rA) rB + rA |mB| (pidblas :277-283) from the digit
+  // products: with x <= (h + l/127) 2^g/127 per factor,
+  //   sum x1 x2 <= 2^(g1+g2)/127^2 (Zhh + Zmix/127 + k),  l1 l2 <= 127^2.
+  double t1 = 0.0, t2 = 0.0;
+  if (!((fa & VCPHIP_OZ_SUMZERO) || (fb & VCPHIP_OZ_RADZERO))) {
+    const double u = ((double)Z[boff] + (double)Z[plane + boff] * P.c1) + (double)k * P.c127;
+    t1 = vcphip_mulpow2(u, sa.g[i] + sb.h[j]) * P.c127;
+  }
+  if (!((fa & VCPHIP_OZ_RADZERO) || zc)) {
+    const double u = ((double)Z[2 * plane + boff] + (double)Z[3 * plane + boff] * P.c1) + (double)k * P.c127;
+    t2 = vcphip_mulpow2(u, sa.h[i] + fj) * P.c127;
+  }
   const double rt = t1 + t2;
   // scaling error of the midpoint product
   double E = 0.0;
@@ -505,7 +558,7 @@ int vcphip_oz_stats(const double* X, long rs, long cs, int rows, int len, Vcphip
 int vcphip_oz_split_rows(const double* A, long lda, int m, int k, int mpad, int kpad,
                          const VcphipOzParams& P, VcphipOzSide side, int8_t* planes, hipStream_t s) {
   dim3 grid(mpad / SR_ROWS, kpad / SR_K);
-  const size_t lds = (size_t)(P.nmod + 2) * SR_ROWS * SR_LD;
+  const size_t lds = (size_t)(P.nmod + VCPHIP_OZ_NDIGIT) * SR_ROWS * SR_LD;
   hipLaunchKernelGGL(k_oz_split_rows, grid, dim3(256), lds, s, A, lda, m, k, mpad, kpad, P, side, planes);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
@@ -518,19 +571,28 @@ int vcphip_oz_split_cols(const double* B, long ldb, int k, int n, int npad, int 
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
+// tile variants: 1 = 256x256 (8x8 fragments per wave), 2 = 128x128 (4x4),
+// 3 = 128x256 (4x8), 4 = 256x256 with 64-byte stages
 int vcphip_oz_gemm(int mpad, int npad, int kpad, int nmod, int nrad, const int8_t* Aplanes,
                    const int8_t* Bplanes, const VcphipOzParams& P, uint8_t* U, int32_t* Z, int variant,
                    hipStream_t s) {
   const int nz = nmod + nrad;
-  if (variant == 1) {
-    const int tiles = (mpad / 256) * (npad / 256);
-    hipLaunchKernelGGL((k_oz_gemm<8, 8>), dim3(tiles * nz), dim3(256), 0, s, mpad, npad, kpad, nmod, nz, Aplanes,
-                       Bplanes, P, U, Z);
-  } else {
-    const int tiles = (mpad / 128) * (npad / 128);
-    hipLaunchKernelGGL((k_oz_gemm<4, 4>), dim3(tiles * nz), dim3(256), 0, s, mpad, npad, kpad, nmod, nz, Aplanes,
-                       Bplanes, P, U, Z);
+#define VCPHIP_GEMM_LAUNCH(FM, FN, BKK)                                                                          \
+  do {                                                                                                           \
+    if (kpad % (BKK)) return -7;                                                                                 \
+    const int tiles = (mpad / (32 * (FM))) * (npad / (32 * (FN)));                                               \
+    hipLaunchKernelGGL((k_oz_gemm<FM, FN, BKK>), dim3(tiles * nz), dim3(256), 0, s, mpad, npad, kpad, nmod, nz, \
+                       Aplanes, Bplanes, P, U, Z);                                                               \
+  } while (0)
+  switch (variant) {
+    case 1: VCPHIP_GEMM_LAUNCH(8, 8, 128); break;
+    case 2: VCPHIP_GEMM_LAUNCH(4, 4, 128); break;
+    case 3: VCPHIP_GEMM_LAUNCH(4, 8, 128); break;
+    case 4: VCPHIP_GEMM_LAUNCH(8, 8, 64); break;
+    case 5: VCPHIP_GEMM_LAUNCH(4, 4, 64); break;
+    default: return -7;
   }
+#undef VCPHIP_GEMM_LAUNCH
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

@@ -153,8 +153,9 @@ int oz_make_params(int k, int nmod, VcphipOzParams* P) {
   {
     const int old = fegetround();
     fesetround(FE_UPWARD);
-    volatile double one = 1.0, d = 16129.0;
+    volatile double one = 1.0, d = 16129.0, d1 = 127.0;
     P->c127 = one / d;
+    P->c1 = one / d1;
     fesetround(old);
   }
   return VCPHIP_OK;
@@ -260,7 +261,7 @@ extern "C" int vcphip_ctx_set_param(vcphip_ctx* c, int key, long long v) {
       c->oz_nmod = v;
       return VCPHIP_OK;
     case VCPHIP_PARAM_OZ_VARIANT:
-      if (v < 0 || v > 2) return VCPHIP_ERR_ARGUMENT;
+      if (v < 0 || v > 5) return VCPHIP_ERR_ARGUMENT;
       c->oz_variant = v;
       return VCPHIP_OK;
     case VCPHIP_PARAM_STAGE_TIMING:   // (re)enabling clears the totals
@@ -319,7 +320,7 @@ static int dims_ok(int m, int n, int k) { return m >= 0 && n >= 0 && k >= 0; }
 
 static int oz_igemm_dev(vcphip_ctx* c, int m, int n, int k, const double* A, int lda, const double* B, int ldb,
                         double* C, int ldc) {
-  if (k > 131071) return VCPHIP_ERR_UNSUPPORTED;
+  if (k > VCPHIP_OZ_MAXK) return VCPHIP_ERR_UNSUPPORTED;
   const int nmod = c->oz_nmod ? (int)c->oz_nmod : default_nmod(k);
   VcphipOzParams P;
   int rc = oz_make_params(k, nmod, &P);
@@ -329,11 +330,11 @@ static int oz_igemm_dev(vcphip_ctx* c, int m, int n, int k, const double* A, int
   const int kpad = k > 0 ? (k + VCPHIP_OZ_PADK - 1) / VCPHIP_OZ_PADK * VCPHIP_OZ_PADK : VCPHIP_OZ_PADK;
   const int nchunk = k > 0 ? (k + VCPHIP_OZ_CHUNK - 1) / VCPHIP_OZ_CHUNK : 0;
   void *pa, *pb, *pu, *pz, *sa, *sb;
-  const size_t nplane = (size_t)nmod + 2;
+  const size_t nplane = (size_t)nmod + VCPHIP_OZ_NDIGIT;
   if ((rc = ws_get(c, vcphip_ctx::WS_PA, nplane * mpad * kpad, &pa))) return rc;
   if ((rc = ws_get(c, vcphip_ctx::WS_PB, nplane * npad * kpad, &pb))) return rc;
   if ((rc = ws_get(c, vcphip_ctx::WS_U, (size_t)nmod * mpad * npad, &pu))) return rc;
-  if ((rc = ws_get(c, vcphip_ctx::WS_Z, 2 * sizeof(int32_t) * (size_t)mpad * npad, &pz))) return rc;
+  if ((rc = ws_get(c, vcphip_ctx::WS_Z, VCPHIP_OZ_NDIGIT * sizeof(int32_t) * (size_t)mpad * npad, &pz))) return rc;
   auto side_bytes = [&](int rows) { return (size_t)rows * (4 * sizeof(int) + sizeof(double)) + sizeof(double) * 4 * (size_t)(nchunk > 0 ? nchunk : 1) * rows + 1024; };
   if ((rc = ws_get(c, vcphip_ctx::WS_SA, side_bytes(m), &sa))) return rc;
   if ((rc = ws_get(c, vcphip_ctx::WS_SB, side_bytes(n), &sb))) return rc;
@@ -345,12 +346,12 @@ static int oz_igemm_dev(vcphip_ctx* c, int m, int n, int k, const double* A, int
     s.e = (int*)p; p += sizeof(int) * rows;
     s.g = (int*)p; p += sizeof(int) * rows;
     s.h = (int*)p; p += sizeof(int) * rows;
-    s.zero = (int*)p;
+    s.flags = (int*)p;
     return s;
   };
   VcphipOzSide SA = carve(sa, m), SB = carve(sb, n);
   int variant = (int)c->oz_variant;
-  if (variant == 0) variant = 2;   // default tile: 128 x 128
+  if (variant == 0) variant = 4;   // default tile: 256 x 256, 64-byte stages (fastest measured)
   StageTimer T(c);
   if ((rc = check_launch(vcphip_oz_stats(A, 1, lda, m, k, SA, c->stream)))) return rc;
   if ((rc = check_launch(vcphip_oz_stats(B, ldb, 1, n, k, SB, c->stream)))) return rc;
@@ -358,8 +359,8 @@ static int oz_igemm_dev(vcphip_ctx* c, int m, int n, int k, const double* A, int
   if ((rc = check_launch(vcphip_oz_split_rows(A, lda, m, k, mpad, kpad, P, SA, (int8_t*)pa, c->stream)))) return rc;
   if ((rc = check_launch(vcphip_oz_split_cols(B, ldb, k, n, npad, kpad, P, SB, (int8_t*)pb, c->stream)))) return rc;
   T.mark("split");
-  if ((rc = check_launch(vcphip_oz_gemm(mpad, npad, kpad, nmod, 2, (const int8_t*)pa, (const int8_t*)pb, P,
-                                        (uint8_t*)pu, (int32_t*)pz, variant == 1 ? 1 : 0, c->stream))))
+  if ((rc = check_launch(vcphip_oz_gemm(mpad, npad, kpad, nmod, VCPHIP_OZ_NDIGIT, (const int8_t*)pa, (const int8_t*)pb, P,
+                                        (uint8_t*)pu, (int32_t*)pz, variant, c->stream))))
     return rc;
   T.mark("gemm");
   if ((rc = check_launch(vcphip_oz_recon(m, n, k, mpad, npad, P, SA, SB, (const uint8_t*)pu, (const int32_t*)pz, C, ldc,

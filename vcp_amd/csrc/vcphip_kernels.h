@@ -19,7 +19,7 @@ int vcphip_launch_drsyrk(int interval, int k, int n, const double* A, long lda, 
 #define VCPHIP_OZ_MAXMOD 20
 #define VCPHIP_OZ_CHUNK 256   // k-chunk of the upward 1-norm partial sums
 #define VCPHIP_OZ_PADMN 256   // m and n are padded to multiples of this
-#define VCPHIP_OZ_PADK 64     // k is padded to a multiple of this
+#define VCPHIP_OZ_PADK 128    // k is padded to a multiple of this
 #define VCPHIP_OZ_NLIMB 5     // 32-bit limbs of M (M < 2^160 for N <= 20)
 
 struct VcphipOzParams {
@@ -34,15 +34,23 @@ struct VcphipOzParams {
   int negP[VCPHIP_OZ_MAXMOD][VCPHIP_OZ_MAXMOD];  // [s][t] = -(m_0 ... m_{s-1}) mod m_t
   uint32_t halfM[VCPHIP_OZ_NLIMB];  // M/2 as little-endian 32-bit limbs
   uint32_t Mlimb[VCPHIP_OZ_NLIMB];  // M
+  double c1;                   // RU(1/127)
   double c127;                 // RU(1/127^2)
 };
 
+// radius-digit planes that follow the N residue planes (two 7-bit upward
+// digits hi/lo per factor):  A: |mid|+rad hi, lo, rad hi, lo;
+//                            B: rad hi, lo, |mid| hi, lo
+#define VCPHIP_OZ_NDIGIT 4
+#define VCPHIP_OZ_MAXK 65536  // 2 k 127^2 < 2^31 for the mixed digit products
+
 // per-row (A) / per-column (B) statistics of a midpoint-radius operand
+enum { VCPHIP_OZ_MIDZERO = 1, VCPHIP_OZ_SUMZERO = 2, VCPHIP_OZ_RADZERO = 4 };
 struct VcphipOzSide {
   int* e;       // mid exponent: max|mid| < 2^e
   int* g;       // digit scale of |mid|+rad
   int* h;       // digit scale of rad
-  int* zero;    // mid identically zero
+  int* flags;   // VCPHIP_OZ_*ZERO: mid / |mid|+rad / rad identically zero
   double* n1;   // upward 1-norm of |mid|
   double* part; // partials [4][nchunk][rows]
 };
