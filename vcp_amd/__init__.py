@@ -86,6 +86,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError("libvcphip.so not built (run `python -m vcp_amd.build`): %s" % LIB_PATH)
+        # PyTorch-ROCm ships its own HIP runtime with the same soname
+        # (libamdhip64.so.7) but loads it by another name; loading torch first
+        # makes this library bind to that runtime, so one process never holds
+        # two HIP runtimes (device pointers from torch are then native here).
+        try:
+            import torch  # noqa: F401
+        except Exception:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, args in SIGNATURES.items():
             f = getattr(L, name)
@@ -308,3 +316,13 @@ class Context:
     def ddgemm_dev(self, m, n, k, dA, lda, dB, ldb, dC, ldc):
         _check(lib().vcphip_ddgemm_dev(self._h, m, n, k, _vp(dA), lda, _vp(dB), ldb, _vp(dC), ldc),
                "vcphip_ddgemm_dev")
+
+
+def residual_batched_dev(ctx, ns, R_ptrs, ldr, A_ptrs, lda, C_ptrs, ldc, normub_ptr):
+    """device-pointer batched residual (pointers as ints), on ctx's stream"""
+    cnt = len(ns)
+    arr = lambda T, xs: (T * cnt)(*xs)  # noqa: E731
+    _check(lib().vcphip_residual_batched_dev(ctx.handle, cnt, arr(ctypes.c_int, ns), arr(_vp, R_ptrs),
+                                             arr(ctypes.c_int, ldr), arr(_vp, A_ptrs), arr(ctypes.c_int, lda),
+                                             arr(_vp, C_ptrs), arr(ctypes.c_int, ldc), _vp(normub_ptr)),
+           "vcphip_residual_batched_dev")

@@ -39,14 +39,13 @@ struct Stage {
   double b0[B_PER_T], b1[B_PER_T];
 };
 
-template <bool AI, bool BI>
-__global__ __launch_bounds__(NT, 2) void k_drgemm(int m, int n, int k,
-                                                   const double* __restrict__ A, long lda,
-                                                   const double* __restrict__ B, long ldb,
-                                                   double* __restrict__ C, long ldc,
-                                                   double* __restrict__ C2, int accumulate) {
-  vcphip_mode_round_up();
-
+// One BM x BN output tile at (row0, col0).  RESID: the batched residual
+// form R*A - I, whose diagonal elements get the interval subtraction of 1
+// (imats.hpp:324-326).  Runs in the caller's round-up mode.
+template <bool AI, bool BI, bool RESID>
+__device__ __forceinline__ void dr_tile(int m, int n, int k, const double* __restrict__ A, long lda,
+                                        const double* __restrict__ B, long ldb, double* __restrict__ C,
+                                        long ldc, double* __restrict__ C2, int accumulate, int row0, int col0) {
   constexpr bool II = AI && BI;
   __shared__ __attribute__((aligned(16))) double sAm[BK][LA];
   __shared__ __attribute__((aligned(16))) double sAr[AI ? BK : 1][LA];
@@ -54,11 +53,6 @@ __global__ __launch_bounds__(NT, 2) void k_drgemm(int m, int n, int k,
   __shared__ __attribute__((aligned(16))) double sBr[BI ? BK : 1][LB];
 
   const int tid = threadIdx.x;
-  const int ntm = (m + BM - 1) / BM, ntn = (n + BN - 1) / BN;
-  const int tile = vcphip_xcd_remap(blockIdx.x, ntm * ntn);
-  int tm, tn;
-  vcphip_tile_coords(tile, ntm, ntn, 16, tm, tn);
-  const int row0 = tm * BM, col0 = tn * BN;
 
   // global-load mapping: A column-major, 64 consecutive rows per k column;
   // B column-major, 16 consecutive k per column.
@@ -209,8 +203,12 @@ __global__ __launch_bounds__(NT, 2) void k_drgemm(int m, int n, int k,
         double t;
         if (II) t = r1[i][j] + r2[i][j];    // temp = (|mA|+rA) rB + rA |mB|   (:283)
         else t = r1[i][j];
-        const double hi = up[i][j] + t;       // uC += temp, RU                  (:284)
-        const double nlo = dn[i][j] + t;      // lC -= temp, RD = -(RU(-lC + t)) (:285-286)
+        double hi = up[i][j] + t;             // uC += temp, RU                  (:284)
+        double nlo = dn[i][j] + t;            // lC -= temp, RD = -(RU(-lC + t)) (:285-286)
+        if (RESID && r == c) {                // b(i,i) -= 1 in interval arithmetic
+          hi = hi - 1.0;
+          nlo = nlo + 1.0;
+        }
         *reinterpret_cast<double2*>(C + 2 * (r + ldc * c)) = make_double2(-nlo, hi);
       } else {
         C[r + ldc * c] = up[i][j];
@@ -218,6 +216,68 @@ __global__ __launch_bounds__(NT, 2) void k_drgemm(int m, int n, int k,
       }
     }
   }
+}
+
+template <bool AI, bool BI>
+__global__ __launch_bounds__(NT, 2) void k_drgemm(int m, int n, int k, const double* __restrict__ A, long lda,
+                                                   const double* __restrict__ B, long ldb, double* __restrict__ C,
+                                                   long ldc, double* __restrict__ C2, int accumulate) {
+  vcphip_mode_round_up();
+  const int ntm = (m + BM - 1) / BM, ntn = (n + BN - 1) / BN;
+  const int tile = vcphip_xcd_remap(blockIdx.x, ntm * ntn);
+  int tm, tn;
+  vcphip_tile_coords(tile, ntm, ntn, 16, tm, tn);
+  dr_tile<AI, BI, false>(m, n, k, A, lda, B, ldb, C, ldc, C2, accumulate, tm * BM, tn * BN);
+}
+
+// ---------------------------------------------------------------------------
+// Batched verified residual C_p = R_p * IA_p - I (imats::linearsolve,
+// vcp/imats.hpp:318-326) for many problems of uneven size in ONE persistent
+// launch: workgroups stride over a tile list sorted by problem size
+// (largest first), each tile being the PI form above.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT, 2) void k_resid_tiles(int ntiles, const VcphipResidTile* __restrict__ tiles,
+                                                        const VcphipResidProblem* __restrict__ probs) {
+  vcphip_mode_round_up();
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const VcphipResidTile tl = tiles[t];
+    const VcphipResidProblem p = probs[tl.problem];
+    dr_tile<false, true, true>(p.n, p.n, p.n, p.R, p.ldr, p.A, p.lda, p.C, p.ldc, nullptr, 0, tl.row0, tl.col0);
+    __syncthreads();
+  }
+}
+
+// row sums of |C| (upward, in column order: mats::norminf, vcp/mats.hpp:1583-1591)
+__global__ __launch_bounds__(256) void k_resid_rowsum(int nrows, const int* __restrict__ row_problem,
+                                                      const VcphipResidProblem* __restrict__ probs,
+                                                      double* __restrict__ rowsum) {
+  vcphip_mode_round_up();
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= nrows) return;
+  const int pi = row_problem[g];
+  const VcphipResidProblem p = probs[pi];
+  const int i = g - p.row_base;
+  double s = 0.0;
+  for (int j = 0; j < p.n; ++j) {
+    const double2 v = *reinterpret_cast<const double2*>(p.C + 2 * (i + p.ldc * (long)j));
+    const double mag = fmax(__builtin_fabs(v.x), __builtin_fabs(v.y));
+    s = (j == 0) ? mag : s + mag;
+  }
+  rowsum[g] = s;
+}
+
+// ||C_p||_inf upper bound: the largest row sum (mats.hpp:1592-1598)
+__global__ __launch_bounds__(256) void k_resid_max(int count, const VcphipResidProblem* __restrict__ probs,
+                                                   const double* __restrict__ rowsum, double* __restrict__ normub) {
+  const int pi = blockIdx.x * 256 + threadIdx.x;
+  if (pi >= count) return;
+  const VcphipResidProblem p = probs[pi];
+  double best = 0.0;
+  for (int i = 0; i < p.n; ++i) {
+    const double v = rowsum[p.row_base + i];
+    best = (i == 0 || v > best) ? v : best;
+  }
+  normub[pi] = best;
 }
 
 // ---------------------------------------------------------------------------
@@ -366,5 +426,14 @@ int vcphip_launch_drsyrk(int interval, int k, int n, const double* A, long lda, 
   dim3 grid(nt * (nt + 1) / 2), block(NT);
   if (interval) hipLaunchKernelGGL((k_drsyrk<true>), grid, block, 0, s, k, n, A, lda, C, ldc);
   else hipLaunchKernelGGL((k_drsyrk<false>), grid, block, 0, s, k, n, A, lda, C, ldc);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_launch_resid(int ntiles, const VcphipResidTile* tiles, const VcphipResidProblem* probs, int nrows,
+                        const int* row_problem, double* rowsum, int count, double* normub, int grid,
+                        hipStream_t s) {
+  if (ntiles > 0) hipLaunchKernelGGL(k_resid_tiles, dim3(grid), dim3(NT), 0, s, ntiles, tiles, probs);
+  if (nrows > 0) hipLaunchKernelGGL(k_resid_rowsum, dim3((nrows + 255) / 256), dim3(256), 0, s, nrows, row_problem, probs, rowsum);
+  if (count > 0) hipLaunchKernelGGL(k_resid_max, dim3((count + 255) / 256), dim3(256), 0, s, count, probs, rowsum, normub);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }

@@ -10,6 +10,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -32,6 +33,18 @@ struct vcphip_ctx {
   std::vector<std::string> st_name;
   std::vector<double> st_ms;
   std::vector<long long> st_cnt;
+  // batched residual: descriptors of the last problem set and its graph
+  struct Resid {
+    std::vector<VcphipResidProblem> probs;
+    std::vector<VcphipResidTile> tiles;
+    std::vector<int> row_problem;
+    double* normub = nullptr;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int grid = 0;
+  } resid;
+  int num_cu = 0;
 };
 
 namespace {
@@ -207,6 +220,8 @@ extern "C" int vcphip_ctx_create(int device, vcphip_ctx** out) {
     return VCPHIP_ERR_DEVICE;
   }
   c->stream = c->own;
+  hipDeviceProp_t prop;
+  c->num_cu = (hipGetDeviceProperties(&prop, device) == hipSuccess) ? prop.multiProcessorCount : 256;
   *out = c;
   return VCPHIP_OK;
 }
@@ -218,6 +233,8 @@ extern "C" int vcphip_ctx_destroy(vcphip_ctx* c) {
   for (int s = 0; s < vcphip_ctx::WS_N; ++s)
     if (c->ws[s]) hipFree(c->ws[s]);
   for (auto e : c->ev) hipEventDestroy(e);
+  if (c->resid.exec) hipGraphExecDestroy(c->resid.exec);
+  if (c->resid.graph) hipGraphDestroy(c->resid.graph);
   hipStreamDestroy(c->own);
   delete c;
   return VCPHIP_OK;
@@ -439,12 +456,100 @@ extern "C" int vcphip_ddgemm_dev(vcphip_ctx* c, int m, int n, int k, const doubl
   return VCPHIP_ERR_UNSUPPORTED;
 }
 
+static bool same_problem(const VcphipResidProblem& a, const VcphipResidProblem& b) {
+  return a.R == b.R && a.A == b.A && a.C == b.C && a.ldr == b.ldr && a.lda == b.lda && a.ldc == b.ldc && a.n == b.n;
+}
+
 extern "C" int vcphip_residual_batched_dev(vcphip_ctx* c, int count, const int* n, const double* const* R,
                                            const int* ldr, const double* const* IA, const int* lda, double* const* C,
                                            const int* ldc, double* normub_dev) {
-  (void)count; (void)n; (void)R; (void)ldr; (void)IA; (void)lda; (void)C; (void)ldc; (void)normub_dev;
   if (!c) return VCPHIP_ERR_ARGUMENT;
-  return VCPHIP_ERR_UNSUPPORTED;
+  if (count < 0 || (count > 0 && (!n || !R || !ldr || !IA || !lda || !C || !ldc || !normub_dev)))
+    return VCPHIP_ERR_ARGUMENT;
+  if (count == 0) return VCPHIP_OK;
+  hipSetDevice(c->device);
+  std::vector<VcphipResidProblem> probs((size_t)count);
+  int nrows = 0;
+  for (int i = 0; i < count; ++i) {
+    const int ni = n[i];
+    if (ni < 0 || ldr[i] < (ni > 1 ? ni : 1) || lda[i] < (ni > 1 ? ni : 1) || ldc[i] < (ni > 1 ? ni : 1))
+      return VCPHIP_ERR_DIMENSION;
+    probs[i] = VcphipResidProblem{R[i], IA[i], C[i], ldr[i], lda[i], ldc[i], ni, nrows};
+    nrows += ni;
+  }
+  vcphip_ctx::Resid& rs = c->resid;
+  bool same = rs.exec && rs.normub == normub_dev && rs.stream == c->stream && rs.probs.size() == probs.size();
+  for (size_t i = 0; same && i < probs.size(); ++i) same = same_problem(rs.probs[i], probs[i]);
+  if (!same) {
+    // tile list, largest problems first (longest-processing-time order)
+    std::vector<int> order((size_t)count);
+    for (int i = 0; i < count; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return probs[a].n > probs[b].n; });
+    rs.tiles.clear();
+    for (int pi : order) {
+      const int nt = (probs[pi].n + VCPHIP_RESID_TILE - 1) / VCPHIP_RESID_TILE;
+      for (int a = 0; a < nt; ++a)
+        for (int b = 0; b < nt; ++b)
+          rs.tiles.push_back(VcphipResidTile{pi, a * VCPHIP_RESID_TILE, b * VCPHIP_RESID_TILE, 0});
+    }
+    rs.row_problem.assign((size_t)nrows, 0);
+    for (int i = 0; i < count; ++i)
+      for (int r = 0; r < probs[i].n; ++r) rs.row_problem[(size_t)probs[i].row_base + r] = i;
+    rs.probs = probs;
+    const size_t bp = sizeof(VcphipResidProblem) * count, bt = sizeof(VcphipResidTile) * rs.tiles.size();
+    const size_t br = sizeof(int) * (size_t)nrows, bs = sizeof(double) * (size_t)nrows;
+    const size_t o_t = (bp + 255) & ~(size_t)255, o_r = o_t + ((bt + 255) & ~(size_t)255);
+    const size_t o_s = o_r + ((br + 255) & ~(size_t)255), total = o_s + bs + 256;
+    void* base;
+    int rc = ws_get(c, vcphip_ctx::WS_BATCH, total, &base);
+    if (rc) return rc;
+    char* b8 = (char*)base;
+    if (hipMemcpyAsync(b8, rs.probs.data(), bp, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        (bt && hipMemcpyAsync(b8 + o_t, rs.tiles.data(), bt, hipMemcpyHostToDevice, c->stream) != hipSuccess) ||
+        (br && hipMemcpyAsync(b8 + o_r, rs.row_problem.data(), br, hipMemcpyHostToDevice, c->stream) != hipSuccess))
+      return VCPHIP_ERR_DEVICE;
+    if (rs.exec) { hipGraphExecDestroy(rs.exec); rs.exec = nullptr; }
+    if (rs.graph) { hipGraphDestroy(rs.graph); rs.graph = nullptr; }
+    rs.normub = normub_dev;
+    rs.stream = c->stream;
+    const int ntiles = (int)rs.tiles.size();
+    rs.grid = std::max(1, std::min(ntiles, 2 * (c->num_cu > 0 ? c->num_cu : 256)));
+    // capture the persistent launch and the norm kernels once; replays reuse it
+    const VcphipResidTile* dt = (const VcphipResidTile*)(b8 + o_t);
+    const VcphipResidProblem* dp = (const VcphipResidProblem*)b8;
+    const int* drp = (const int*)(b8 + o_r);
+    double* drs = (double*)(b8 + o_s);
+    bool captured = false;
+    if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+      const int lrc = vcphip_launch_resid(ntiles, dt, dp, nrows, drp, drs, count, normub_dev, rs.grid, c->stream);
+      hipGraph_t g = nullptr;
+      if (hipStreamEndCapture(c->stream, &g) == hipSuccess && lrc == 0 && g) {
+        if (hipGraphInstantiate(&rs.exec, g, nullptr, nullptr, 0) == hipSuccess) {
+          rs.graph = g;
+          captured = true;
+        } else {
+          hipGraphDestroy(g);
+        }
+      } else if (g) {
+        hipGraphDestroy(g);
+      }
+    }
+    if (!captured) {
+      hipGetLastError();
+      rs.exec = nullptr;
+      // a stream that cannot be captured (e.g. the legacy null stream) runs
+      // the same kernels as direct launches
+      StageTimer T(c);
+      const int lrc = vcphip_launch_resid(ntiles, dt, dp, nrows, drp, drs, count, normub_dev, rs.grid, c->stream);
+      T.mark("resid");
+      rs.probs.clear();   // nothing cached to replay
+      return check_launch(lrc);
+    }
+  }
+  StageTimer T(c);
+  if (hipGraphLaunch(rs.exec, c->stream) != hipSuccess) return VCPHIP_ERR_DEVICE;
+  T.mark("resid");
+  return VCPHIP_OK;
 }
 
 // ===========================================================================
@@ -601,7 +706,49 @@ extern "C" int vcphip_ddgemm_tn(vcphip_ctx* c, int k, int n, const double* A, in
 extern "C" int vcphip_residual_batched(vcphip_ctx* c, int count, const int* n, const double* const* R, const int* ldr,
                                        const double* const* IA, const int* lda, double* const* C, const int* ldc,
                                        double* normub) {
-  (void)count; (void)n; (void)R; (void)ldr; (void)IA; (void)lda; (void)C; (void)ldc; (void)normub;
   if (!c) return VCPHIP_ERR_ARGUMENT;
-  return VCPHIP_ERR_UNSUPPORTED;
+  if (count < 0 || (count > 0 && (!n || !R || !ldr || !IA || !lda || !C || !ldc || !normub))) return VCPHIP_ERR_ARGUMENT;
+  if (count == 0) return VCPHIP_OK;
+  hipSetDevice(c->device);
+  // one staging buffer: R_i (8 n^2), A_i (16 n^2), C_i (16 n^2) packed, then normub
+  std::vector<size_t> offR(count), offA(count), offC(count);
+  size_t tot = 0;
+  for (int i = 0; i < count; ++i) {
+    if (n[i] < 0 || ldr[i] < (n[i] > 1 ? n[i] : 1) || lda[i] < (n[i] > 1 ? n[i] : 1) || ldc[i] < (n[i] > 1 ? n[i] : 1))
+      return VCPHIP_ERR_DIMENSION;
+    const size_t nn = (size_t)n[i] * n[i];
+    offR[i] = tot; tot += ((8 * nn + 255) & ~(size_t)255);
+    offA[i] = tot; tot += ((16 * nn + 255) & ~(size_t)255);
+    offC[i] = tot; tot += ((16 * nn + 255) & ~(size_t)255);
+  }
+  const size_t offN = tot;
+  tot += sizeof(double) * count + 256;
+  void* base;
+  int rc = ws_get(c, vcphip_ctx::WS_HA, tot, &base);
+  if (rc) return rc;
+  char* b8 = (char*)base;
+  std::vector<const double*> dR(count), dA(count);
+  std::vector<double*> dC(count);
+  std::vector<int> ld(count);
+  for (int i = 0; i < count; ++i) {
+    const int ni = n[i];
+    dR[i] = (const double*)(b8 + offR[i]);
+    dA[i] = (const double*)(b8 + offA[i]);
+    dC[i] = (double*)(b8 + offC[i]);
+    ld[i] = ni > 1 ? ni : 1;
+    if (ni > 0) {
+      if (hipMemcpy2DAsync(b8 + offR[i], 8 * ni, R[i], 8 * (size_t)ldr[i], 8 * ni, ni, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+          hipMemcpy2DAsync(b8 + offA[i], 16 * ni, IA[i], 16 * (size_t)lda[i], 16 * ni, ni, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        return VCPHIP_ERR_DEVICE;
+    }
+  }
+  rc = vcphip_residual_batched_dev(c, count, n, dR.data(), ld.data(), dA.data(), ld.data(), dC.data(), ld.data(),
+                                   (double*)(b8 + offN));
+  if (rc) return rc;
+  for (int i = 0; i < count; ++i)
+    if (n[i] > 0 && hipMemcpy2DAsync(C[i], 16 * (size_t)ldc[i], dC[i], 16 * n[i], 16 * n[i], n[i], hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+      return VCPHIP_ERR_DEVICE;
+  if (hipMemcpyAsync(normub, b8 + offN, sizeof(double) * count, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+    return VCPHIP_ERR_DEVICE;
+  return hipStreamSynchronize(c->stream) == hipSuccess ? VCPHIP_OK : VCPHIP_ERR_DEVICE;
 }

@@ -15,6 +15,23 @@ int vcphip_launch_drgemm(int kind, int m, int n, int k, const double* A, long ld
 int vcphip_launch_drsyrk(int interval, int k, int n, const double* A, long lda, double* C, long ldc,
                          hipStream_t s);
 
+// batched verified residual R_p * IA_p - I (kern_drgemm.hip)
+struct VcphipResidProblem {
+  const double* R;   // point n x n, column-major
+  const double* A;   // interval n x n, AoS {inf, sup}
+  double* C;         // interval n x n
+  long ldr, lda, ldc;
+  int n;
+  int row_base;      // first global row of this problem in the row-sum buffer
+};
+struct VcphipResidTile {
+  int problem, row0, col0, pad;
+};
+#define VCPHIP_RESID_TILE 64
+int vcphip_launch_resid(int ntiles, const VcphipResidTile* tiles, const VcphipResidProblem* probs, int nrows,
+                        const int* row_problem, double* rowsum, int count, double* normub, int grid,
+                        hipStream_t s);
+
 // ---- Ozaki scheme II (kern_ozaki.hip) ------------------------------------
 #define VCPHIP_OZ_MAXMOD 20
 #define VCPHIP_OZ_CHUNK 256   // k-chunk of the upward 1-norm partial sums
