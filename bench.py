@@ -136,12 +136,14 @@ def main():
     t_gemm = stages.get("gemm")
     roof = None
     if t_gemm:
-        ops = 2.0 * n * n * n * (nmod + 2)
+        # one launch: nmod residue products + 6 radius-digit plane products
+        # (hi*hi, hi*lo + lo*hi for each of the two radius terms), 2 n^3 each
+        ops = 2.0 * n * n * n * (nmod + 6)
         ach = ops / (t_gemm * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
                 "frac": round(ach / PEAK_INT8_TOPS, 4), "traffic": None,
-                "kernel": "k_oz_gemm (int8 MFMA, %d modular + 2 radius products)" % nmod,
-                "ms_per_launch": round(t_gemm, 4)}
+                "kernel": "k_oz_gemm (int8 MFMA: %d residue + 6 radius-digit plane products)" % nmod,
+                "ops_per_launch": ops, "ms_per_launch": round(t_gemm, 4)}
 
     # ---- CPU baseline: the oracle's pidblas::mulmm restatement -------------
     cpu = None

@@ -1,0 +1,77 @@
+// pddhip.hpp -- vcp::pddhip, a drop-in kv::dd policy computing the products
+// of vcp::pddblas on an MI355X through libvcphip (an Ozaki scheme II on int8
+// MFMA in place of the reference's FP64-slice Ozaki scheme, vcp/pddblas.hpp:
+// 54-193).
+//
+//   vcp::matrix< kv::dd, vcp::pddhip > A, B, C;
+//   C = A * B;            // GPU
+//   C = ltransmul(A);     // A^T A, exactly symmetric
+//
+// pddblas::mulmm / mulltmm are NOT virtual (vcp/pddblas.hpp:318, :379), so
+// pddhip declares its own, which hide them for vcp::matrix<kv::dd, pddhip>
+// (name lookup in matrix's operator*, vcp/matrix.hpp:245-267).  Everything
+// else (linearsolve, Cholesky, eigsym) is inherited from pddblas and stays on
+// the CPU.  Scalar operands keep the reference's host semantics.
+#pragma once
+
+#ifndef VCP_PDDHIP_HPP
+#define VCP_PDDHIP_HPP
+
+#include <vcp/error.hpp>
+#include <vcp/pddblas.hpp>
+
+#include "vcphip_policy_core.hpp"
+
+namespace vcp {
+
+namespace pddhip_detail {
+struct Errors {
+  [[noreturn]] static void dimension(const std::string& m) { throw vcp::dimension_error(m); }
+  [[noreturn]] static void argument(const std::string& m) { throw vcp::invalid_argument(m); }
+  [[noreturn]] static void verification(const std::string& m) { throw vcp::verification_error(m); }
+  [[noreturn]] static void backend(const char* routine, int code, const std::string& m) {
+    throw vcp::backend_error("hip", routine, code, m);
+  }
+};
+static_assert(sizeof(kv::dd) == 2 * sizeof(double), "kv::dd must be {a1, a2}");
+inline const double* raw(const std::vector<kv::dd>& v) { return reinterpret_cast<const double*>(v.data()); }
+inline double* raw(std::vector<kv::dd>& v) { return reinterpret_cast<double*>(v.data()); }
+}  // namespace pddhip_detail
+
+class pddhip : public vcp::pddblas {
+  typedef pddhip_detail::Errors E;
+
+ public:
+  // C = A*B (pddblas.hpp:318-376)
+  void mulmm(const pddhip& B, pddhip& c) const {
+    if (vcphip_policy::is_scalar_case(this->type, B.type)) {
+      vcp::pddblas::mulmm(B, c);
+      return;
+    }
+    vcphip_policy::check<E>(vcphip_policy::check_product(this->type, this->row, this->column, B.type, B.row, B.column),
+                            "mulmm");
+    const int m = this->row, k = this->column, n = B.column;
+    c.zeros(m, n);
+    vcphip_policy::check<E>(vcphip_ddgemm(vcphip_policy::checked_context<E>(), m, n, k, pddhip_detail::raw(this->v), m,
+                                          pddhip_detail::raw(B.v), k, pddhip_detail::raw(c.v), m),
+                            "vcphip_ddgemm");
+  }
+  // C = A^T A (pddblas.hpp:379-409), exactly symmetric
+  void mulltmm(pddhip& c) const {
+    if (this->type == 'S') {
+      vcp::pddblas::mulltmm(c);
+      return;
+    }
+    if (this->type != 'C' && this->type != 'R' && this->type != 'M')
+      vcp::throw_error<vcp::state_error>("mulltmm: unsupported matrix type: ", this->type);
+    const int k = this->row, n = this->column;
+    c.zeros(n, n);
+    vcphip_policy::check<E>(vcphip_ddgemm_tn(vcphip_policy::checked_context<E>(), k, n, pddhip_detail::raw(this->v), k,
+                                             pddhip_detail::raw(c.v), n),
+                            "vcphip_ddgemm_tn");
+  }
+};
+
+}  // namespace vcp
+
+#endif  // VCP_PDDHIP_HPP

@@ -1,0 +1,132 @@
+// pidhip.hpp -- vcp::pidhip, a drop-in interval<double> policy computing the
+// verified products of vcp::pidblas on an MI355X through libvcphip.
+//
+//   #include <kv/interval.hpp>
+//   #include <kv/rdouble.hpp>
+//   #include <vcp/pidhip.hpp>          // this header (include/ of vcp_amd)
+//   vcp::matrix< kv::interval<double>, vcp::pidhip > A, B, C;
+//   C = A * B;                          // GPU, enclosure of the exact product
+//   x = lss(A, b);                      // R*A - I etc. go through mul_m_im
+//
+// pidhip derives from vcp::pidblas (vcp/pidblas.hpp:87) and overrides the
+// same hooks: the public virtual mulmm (:239) and mulltmm (:296), and the
+// protected virtual mul_im_m (:90), mul_m_im (:142) and vmulmm (:198) that
+// imats::linearsolve / eigsym / eigsymge call (vcp/imats.hpp:293-522).
+// Scalar ('S') operands keep the reference's host semantics (:241-254); every
+// other product runs on the GPU.  The host floating-point environment is
+// never changed (the reference leaves it at nearest via hwround_guard,
+// vcp/hwround_guard.hpp:39-48).  Errors map onto vcp/error.hpp.
+//
+// Requires the VCP and kv headers on the include path and linking with
+// -lvcphip.  vcp::pidhip::set_form() selects the interval x interval form
+// (VCPHIP_FORM_OZAKI, the default, or VCPHIP_FORM_PLAIN).
+#pragma once
+
+#ifndef VCP_PIDHIP_HPP
+#define VCP_PIDHIP_HPP
+
+#include <vcp/error.hpp>
+#include <vcp/pidblas.hpp>
+
+#include "vcphip_policy_core.hpp"
+
+namespace vcp {
+
+namespace pidhip_detail {
+struct Errors {
+  [[noreturn]] static void dimension(const std::string& m) { throw vcp::dimension_error(m); }
+  [[noreturn]] static void argument(const std::string& m) { throw vcp::invalid_argument(m); }
+  [[noreturn]] static void verification(const std::string& m) { throw vcp::verification_error(m); }
+  [[noreturn]] static void backend(const char* routine, int code, const std::string& m) {
+    throw vcp::backend_error("hip", routine, code, m);
+  }
+};
+inline const double* raw(const std::vector<kv::interval<double> >& v) {
+  return reinterpret_cast<const double*>(v.data());   // AoS {inf, sup}
+}
+inline double* raw(std::vector<kv::interval<double> >& v) { return reinterpret_cast<double*>(v.data()); }
+static_assert(sizeof(kv::interval<double>) == 2 * sizeof(double), "kv::interval<double> must be {inf, sup}");
+}  // namespace pidhip_detail
+
+class pidhip : public vcp::pidblas {
+  typedef pidhip_detail::Errors E;
+
+ protected:
+  // C = IA * B (pidblas.hpp:90-140)
+  void mul_im_m(const vcp::pdblas& B, vcp::imats<double, vcp::pdblas>& c) const override {
+    if (vcphip_policy::is_scalar_case(this->type, B.type)) {
+      vcp::pidblas::mul_im_m(B, c);
+      return;
+    }
+    vcphip_policy::check<E>(vcphip_policy::check_product(this->type, this->row, this->column, B.type, B.row, B.column),
+                            "mul_im_m");
+    const int m = this->row, k = this->column, n = B.column;
+    c.zeros(m, n);
+    vcphip_policy::check<E>(vcphip_igemm_ip(vcphip_policy::checked_context<E>(), m, n, k, pidhip_detail::raw(this->v), m,
+                                            B.v.data(), k, pidhip_detail::raw(c.v), m),
+                            "vcphip_igemm_ip");
+  }
+  // C = B * IA (pidblas.hpp:142-196)
+  void mul_m_im(const vcp::pdblas& B, vcp::imats<double, vcp::pdblas>& c) const override {
+    if (vcphip_policy::is_scalar_case(B.type, this->type)) {
+      vcp::pidblas::mul_m_im(B, c);
+      return;
+    }
+    vcphip_policy::check<E>(vcphip_policy::check_product(B.type, B.row, B.column, this->type, this->row, this->column),
+                            "mul_m_im");
+    const int m = B.row, k = B.column, n = this->column;
+    c.zeros(m, n);
+    vcphip_policy::check<E>(vcphip_igemm_pi(vcphip_policy::checked_context<E>(), m, n, k, B.v.data(), m,
+                                            pidhip_detail::raw(this->v), k, pidhip_detail::raw(c.v), m),
+                            "vcphip_igemm_pi");
+  }
+  // IC = C^T C with verification (pidblas.hpp:198-236)
+  void vmulmm(const vcp::pdblas& C) override {
+    if (C.type == 'S' || C.type == 'C') {
+      vcp::pidblas::vmulmm(C);
+      return;
+    }
+    const int k = C.row, n = C.column;
+    this->zeros(n, n);
+    vcphip_policy::check<E>(vcphip_vgemm_tn(vcphip_policy::checked_context<E>(), k, n, C.v.data(), k,
+                                            pidhip_detail::raw(this->v), n),
+                            "vcphip_vgemm_tn");
+  }
+
+ public:
+  static void set_form(int form) { vcphip_policy::default_form() = form; }
+
+  // C = IA * IB (pidblas.hpp:239-294)
+  void mulmm(const vcp::mats<kv::interval<double> >& B, vcp::mats<kv::interval<double> >& c) const override {
+    if (vcphip_policy::is_scalar_case(this->type, B.type)) {
+      vcp::pidblas::mulmm(B, c);
+      return;
+    }
+    vcphip_policy::check<E>(vcphip_policy::check_product(this->type, this->row, this->column, B.type, B.row, B.column),
+                            "mulmm");
+    const int m = this->row, k = this->column, n = B.column;
+    c.zeros(m, n);
+    vcphip_policy::check<E>(vcphip_igemm(vcphip_policy::checked_context<E>(), vcphip_policy::default_form(), m, n, k,
+                                         pidhip_detail::raw(this->v), m, pidhip_detail::raw(B.v), k,
+                                         pidhip_detail::raw(c.v), m),
+                            "vcphip_igemm");
+  }
+  // C = IA^T IA (pidblas.hpp:296-358)
+  void mulltmm(vcp::mats<kv::interval<double> >& c) const override {
+    if (this->type == 'S' || this->type == 'C') {
+      vcp::pidblas::mulltmm(c);
+      return;
+    }
+    if (this->type != 'M' && this->type != 'R')
+      vcp::throw_error<vcp::state_error>("mulltmm: unsupported matrix type: ", this->type);
+    const int k = this->row, n = this->column;
+    c.zeros(n, n);
+    vcphip_policy::check<E>(vcphip_igemm_tn(vcphip_policy::checked_context<E>(), k, n, pidhip_detail::raw(this->v), k,
+                                            pidhip_detail::raw(c.v), n),
+                            "vcphip_igemm_tn");
+  }
+};
+
+}  // namespace vcp
+
+#endif  // VCP_PIDHIP_HPP

@@ -970,3 +970,182 @@ int orc_pddblas_mulltmm(int k, int n, const double* A, int lda, double* C, int l
     free(DA); free(Ac); free(Cv);
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Ozaki scheme II double-double product (restatement of the GPU pipeline)  */
+/* ------------------------------------------------------------------------ */
+#define DL 9 /* 32-bit digits of the exact products (|X| < 2^256) */
+
+int orc_ozaki_dd_beta(int k, int nmod) {
+    int mods[64];
+    if (nmod > 64 || orc_ozaki_moduli(nmod, mods) != nmod) return -1;
+    double lm = 0.0;
+    for (int t = 0; t < nmod; ++t) lm += log2((double)mods[t]);
+    int lk = 0;
+    while ((1ll << lk) < (long long)(k > 1 ? k : 1)) ++lk;
+    const int beta = (int)floor((lm - 2.0 - (double)lk - 1e-6) / 2.0);
+    return beta > 115 ? 115 : beta;
+}
+
+/* nearest integer (ties away) of an exact double, as i128 */
+static i128 rn_int128(double y) {
+    const double a = fabs(y);
+    if (!(a >= 0.5)) return 0;
+    int e;
+    const double f = frexp(a, &e);
+    const uint64_t m = (uint64_t)ldexp(f, 53);
+    const int p = e - 53;
+    u128 mag = p >= 0 ? ((u128)m << p) : (u128)((m + (1ull << (-p - 1))) >> (-p));
+    return y < 0 ? -(i128)mag : (i128)mag;
+}
+
+/* e of a dd row: max (|hi| + |lo|) rounded up < 2^e; returns 0 for a zero row */
+static int dd_row_exp(const double* X, size_t rs, size_t cs, int r, int len, int* zero) {
+    const int old = fegetround();
+    fesetround(FE_UPWARD);
+    double mu = 0.0;
+    for (int c = 0; c < len; ++c) {
+        const size_t idx = (size_t)r * rs + (size_t)c * cs;
+        volatile double h = X[2 * idx], l = X[2 * idx + 1];
+        const double a = fabs(h) + fabs(l);
+        if (a > mu) mu = a;
+    }
+    fesetround(old);
+    *zero = mu == 0.0;
+    return mu > 0.0 ? fexp(mu) : 0;
+}
+
+/* digits (32-bit, little-endian) of |x| for a signed 256-bit carry-save acc */
+static int acc_to_mag(int64_t* d, uint32_t* mag) {
+    for (int i = 0; i < DL - 1; ++i) {
+        const int64_t carry = d[i] >> 32;
+        d[i] -= carry * ((int64_t)1 << 32);
+        d[i + 1] += carry;
+    }
+    const int neg = d[DL - 1] < 0;
+    if (neg) {
+        for (int i = 0; i < DL; ++i) d[i] = -d[i];
+        for (int i = 0; i < DL - 1; ++i) {
+            const int64_t carry = d[i] >> 32;
+            d[i] -= carry * ((int64_t)1 << 32);
+            d[i + 1] += carry;
+        }
+    }
+    for (int i = 0; i < DL; ++i) mag[i] = (uint32_t)d[i];
+    return neg;
+}
+
+static int dg_bit(const uint32_t* x, int b) { return b < 0 ? 0 : (int)((x[b >> 5] >> (b & 31)) & 1u); }
+static int dg_any_below(const uint32_t* x, int b) {
+    for (int i = 0; i < b; ++i)
+        if (dg_bit(x, i)) return 1;
+    return 0;
+}
+
+/* nearest double (ties to even) of |x| 2^s; *q = lowest kept bit, *up = rounded up */
+static double dg_rn(const uint32_t* x, int s, int* qo, int* upo) {
+    int top = -1;
+    for (int i = DL - 1; i >= 0 && top < 0; --i)
+        if (x[i]) top = 32 * i + 31 - __builtin_clz(x[i]);
+    *qo = 0;
+    *upo = 0;
+    if (top < 0) return 0.0;
+    int q = top - 52;
+    if (q < -1074 - s) q = -1074 - s;
+    if (q <= 0) {
+        uint64_t m = 0;
+        for (int b = top; b >= 0; --b) m = (m << 1) | (uint64_t)dg_bit(x, b);
+        return ldexp((double)m, s);
+    }
+    uint64_t m = 0;
+    for (int b = top; b >= q; --b) m = (m << 1) | (uint64_t)dg_bit(x, b);
+    const int rb = dg_bit(x, q - 1), st = dg_any_below(x, q - 1);
+    const int up = rb && (st || (m & 1));
+    m += (uint64_t)up;
+    *qo = q;
+    *upo = up;
+    if (m != 0 && q + s + 64 - __builtin_clzll(m) > 1024) return INFINITY;
+    return ldexp((double)m, q + s);
+}
+
+/* C = A*B (or A^T A when B == NULL, A k x n) for dd matrices through the
+ * exact scaled-integer product the GPU CRT pipeline recovers. */
+int orc_ozaki_ddgemm(int m, int n, int k, const double* A, int lda, const double* B, int ldb,
+                     double* C, int ldc, int nmod) {
+    const int beta = orc_ozaki_dd_beta(k, nmod);
+    if (beta < 8) return -2;
+    const int tn = (B == NULL);
+    if (tn) { B = A; ldb = lda; }
+    int *ea = (int*)malloc(sizeof(int) * m), *za = (int*)malloc(sizeof(int) * m);
+    int *eb = (int*)malloc(sizeof(int) * n), *zb = (int*)malloc(sizeof(int) * n);
+    i128 *Ap = (i128*)malloc(sizeof(i128) * (size_t)m * k), *Bp = (i128*)malloc(sizeof(i128) * (size_t)k * n);
+    for (int i = 0; i < m; ++i) {
+        if (tn) ea[i] = dd_row_exp(A, (size_t)lda, 1, i, k, &za[i]);
+        else ea[i] = dd_row_exp(A, 1, (size_t)lda, i, k, &za[i]);
+        for (int c = 0; c < k; ++c) {
+            const double* p = tn ? A + 2 * ((size_t)c + (size_t)lda * i) : A + 2 * ((size_t)i + (size_t)lda * c);
+            Ap[(size_t)i * k + c] = za[i] ? 0 : rn_int128(ldexp(p[0], beta - ea[i])) + rn_int128(ldexp(p[1], beta - ea[i]));
+        }
+    }
+    for (int j = 0; j < n; ++j) {
+        eb[j] = dd_row_exp(B, (size_t)ldb, 1, j, k, &zb[j]);
+        for (int c = 0; c < k; ++c) {
+            const double* p = B + 2 * ((size_t)c + (size_t)ldb * j);
+            Bp[(size_t)j * k + c] = zb[j] ? 0 : rn_int128(ldexp(p[0], beta - eb[j])) + rn_int128(ldexp(p[1], beta - eb[j]));
+        }
+    }
+    const int nt = orc_get_threads();
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 1)
+    for (int j = 0; j < n; ++j) {
+        for (int i = 0; i < m; ++i) {
+            double* cc = C + 2 * ((size_t)i + (size_t)ldc * j);
+            if (za[i] || zb[j]) { cc[0] = 0.0; cc[1] = 0.0; continue; }
+            int64_t d[DL];
+            memset(d, 0, sizeof(d));
+            for (int c = 0; c < k; ++c) {
+                const i128 a = Ap[(size_t)i * k + c], b = Bp[(size_t)j * k + c];
+                if (a == 0 || b == 0) continue;
+                const int neg = (a < 0) != (b < 0);
+                const u128 ua = a < 0 ? (u128)(-a) : (u128)a, ub = b < 0 ? (u128)(-b) : (u128)b;
+                uint32_t da[4], db[4];
+                for (int u = 0; u < 4; ++u) { da[u] = (uint32_t)(ua >> (32 * u)); db[u] = (uint32_t)(ub >> (32 * u)); }
+                for (int u = 0; u < 4; ++u)
+                    for (int v = 0; v < 4; ++v) {
+                        const uint64_t pr = (uint64_t)da[u] * db[v];
+                        const int64_t lo = (int64_t)(pr & 0xffffffffull), hi = (int64_t)(pr >> 32);
+                        if (neg) { d[u + v] -= lo; d[u + v + 1] -= hi; }
+                        else { d[u + v] += lo; d[u + v + 1] += hi; }
+                    }
+            }
+            uint32_t mag[DL];
+            const int neg = acc_to_mag(d, mag);
+            const int sc = ea[i] + eb[j] - 2 * beta;
+            int q, up;
+            const double hi = dg_rn(mag, sc, &q, &up);
+            uint32_t rem[DL];
+            memset(rem, 0, sizeof(rem));
+            int rneg = 0;
+            if (q > 0) {
+                for (int b = 0; b < q; ++b)
+                    if (dg_bit(mag, b)) rem[b >> 5] |= 1u << (b & 31);
+                if (up) { /* rem = 2^q - rem */
+                    rneg = 1;
+                    int64_t borrow = 0;
+                    for (int l = 0; l < DL; ++l) {
+                        const int64_t twoq = (q >= 32 * l && q < 32 * l + 32) ? ((int64_t)1 << (q - 32 * l)) : 0;
+                        int64_t dd = twoq - (int64_t)rem[l] - borrow;
+                        borrow = dd < 0;
+                        rem[l] = (uint32_t)(dd + (borrow << 32));
+                    }
+                }
+            }
+            int q2, up2;
+            double lo = dg_rn(rem, sc, &q2, &up2);
+            if (rneg) lo = -lo;
+            cc[0] = neg ? -hi : hi;
+            cc[1] = neg ? -lo : lo;
+        }
+    }
+    free(ea); free(za); free(eb); free(zb); free(Ap); free(Bp);
+    return 0;
+}

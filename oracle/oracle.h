@@ -91,6 +91,13 @@ void orc_ozaki_scaled_rows(int m, int k, const double* A, int lda, int beta, int
 void orc_ozaki_scaled_cols(int k, int n, const double* B, int ldb, int beta, int64_t* Bp /* k x n col-major */,
                            int* f_out);
 
+/* dd product through the Ozaki-II route (the GPU's pddhip pipeline): scale
+ * exponent beta for k and nmod (k 2^(2 beta + 2) < M), and C = A*B, or
+ * C = A^T A when B is NULL (A is then k x n, m = n). */
+int orc_ozaki_dd_beta(int k, int nmod);
+int orc_ozaki_ddgemm(int m, int n, int k, const double* A, int lda, const double* B, int ldb,
+                     double* C, int ldc, int nmod);
+
 /* ---- pddblas restatements (vcp/pddblas.hpp) ------------------------------ */
 int orc_dd_split_shift(int k);   /* pddblas.hpp:66-68 */
 int orc_dd_split_bits(int k);    /* pddblas.hpp:70-74 */

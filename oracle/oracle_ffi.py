@@ -343,3 +343,31 @@ def ref_udmatmul(A, B, CU, CD):
     m, k = A.shape
     n = B.shape[1]
     return L.ref_udmatmul(m, k, n, _dp(A), _dp(B), _dp(CU), _dp(CD)) == 0
+
+
+def ozaki_dd_beta(k, nmod):
+    L = lib()
+    L.orc_ozaki_dd_beta.argtypes = [_c_int, _c_int]
+    return L.orc_ozaki_dd_beta(k, nmod)
+
+
+def ozaki_ddgemm(A, B, nmod):
+    """dd C = A*B (or A^T A when B is None) through the exact scaled-integer
+    product of the Ozaki-II route."""
+    L = lib()
+    L.orc_ozaki_ddgemm.argtypes = [_c_int, _c_int, _c_int, _p_dbl, _c_int, _p_dbl, _c_int, _p_dbl, _c_int, _c_int]
+    L.orc_ozaki_ddgemm.restype = _c_int
+    if B is None:
+        k, n = A.shape[:2]
+        C = pairs(n, n)
+        rc = L.orc_ozaki_ddgemm(n, n, k, _dp(A.transpose(1, 0, 2)), _ld_pair(A), None, 0,
+                                _dp(C.transpose(1, 0, 2)), _ld_pair(C), nmod)
+    else:
+        m, k = A.shape[:2]
+        n = B.shape[1]
+        C = pairs(m, n)
+        rc = L.orc_ozaki_ddgemm(m, n, k, _dp(A.transpose(1, 0, 2)), _ld_pair(A), _dp(B.transpose(1, 0, 2)),
+                                _ld_pair(B), _dp(C.transpose(1, 0, 2)), _ld_pair(C), nmod)
+    if rc != 0:
+        raise ValueError("orc_ozaki_ddgemm failed: %d" % rc)
+    return C

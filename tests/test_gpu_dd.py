@@ -1,0 +1,77 @@
+"""The double-double product (pddhip, the vcp::pddblas product) on int8
+MFMA: bit-identical to the oracle's restatement of the Ozaki-II route,
+accurate to about one dd ulp against the exact product, and at least as
+accurate as the restatement of the reference's Ozaki-I pddblas::mulmm."""
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+import vcp_amd
+from vcp_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _dd(rng, r, c, kind="uniform"):
+    x = W.dd_matrix(rng, r, c)
+    if kind == "wide":
+        s = np.ldexp(1.0, rng.integers(-30, 30, (r, c)))
+        x[..., 0] *= s
+        x[..., 1] *= s
+    if kind == "zeros":
+        x[::4] = 0.0
+        x[:, ::5] = 0.0
+    return x
+
+
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (16, 16, 64), (70, 33, 100), (256, 256, 256), (300, 257, 513)])
+@pytest.mark.parametrize("kind", ["uniform", "wide", "zeros"])
+def test_dd_bitexact(ctx, m, n, k, kind):
+    rng = np.random.default_rng(hash((m, n, k, kind, "dd")) % 2 ** 32)
+    A, B = _dd(rng, m, k, kind), _dd(rng, k, n, kind)
+    nmod = vcp_amd.lib() and None
+    C = ctx.ddgemm(A, B)
+    beta_ok = [nm for nm in range(24, 33) if O.ozaki_dd_beta(k, nm) >= 110]
+    R = O.ozaki_ddgemm(A, B, beta_ok[0])
+    bad = np.argwhere(~((C == R).all(axis=2)))
+    assert len(bad) == 0, "mismatch at %s: gpu %s oracle %s" % (bad[:3].tolist(), C[tuple(bad[0])], R[tuple(bad[0])])
+    del nmod
+
+
+@pytest.mark.parametrize("nmod", [26, 30, 32])
+def test_dd_nmod_bitexact(ctx, nmod):
+    rng = np.random.default_rng(nmod)
+    A, B = _dd(rng, 64, 200), _dd(rng, 200, 48)
+    ctx.set_param(vcp_amd.PARAM_DD_NMOD, nmod)
+    try:
+        C = ctx.ddgemm(A, B)
+    finally:
+        ctx.set_param(vcp_amd.PARAM_DD_NMOD, 0)
+    assert np.array_equal(C, O.ozaki_ddgemm(A, B, nmod))
+
+
+def test_dd_tn_symmetric_and_bitexact(ctx):
+    rng = np.random.default_rng(3)
+    A = _dd(rng, 300, 70)
+    C = ctx.ddgemm_tn(A)
+    assert np.array_equal(C, C.transpose(1, 0, 2))
+    beta_ok = [nm for nm in range(24, 33) if O.ozaki_dd_beta(300, nm) >= 110]
+    assert np.array_equal(C, O.ozaki_ddgemm(A, None, beta_ok[0]))
+
+
+def test_dd_accuracy_against_exact_and_pddblas(ctx):
+    rng = np.random.default_rng(4)
+    m, n, k = 40, 32, 1024
+    A, B = _dd(rng, m, k), _dd(rng, k, n)
+    C = ctx.ddgemm(A, B)
+    P, _, _ = O.pddblas_mulmm(A, B)
+    eo, ep = [], []
+    for i in range(m):
+        for j in range(n):
+            h, _, e = O.exact_dd_dot(A[i], B[:, j], C[i, j, 0], C[i, j, 1])
+            _, _, e2 = O.exact_dd_dot(A[i], B[:, j], P[i, j, 0], P[i, j, 1])
+            scale = float(np.sum(np.abs(A[i, :, 0]) * np.abs(B[:, j, 0])))
+            eo.append(e / (scale * 2.0 ** -105))
+            ep.append(e2 / (scale * 2.0 ** -105))
+    # errors relative to sum |a||b| in units of 2^-105: both well below one
+    assert max(eo) <= 1.0 and max(eo) <= max(2.0 * max(ep), 0.05)

@@ -1,0 +1,394 @@
+// Double-double matrix product on int8 MFMA (gfx950): the vcp::pddblas
+// product (vcp/pddblas.hpp:318-376, :379-409) by an Ozaki scheme II.
+//
+// Each dd entry a = hi + lo of row i is scaled to the integer
+//   A' = RN(hi 2^(beta - e_i)) + RN(lo 2^(beta - e_i)),  |A'| <= 2^beta + 1,
+// where max_k |hi| + |lo| < 2^e_i (one exponent per row of A / column of B,
+// the sigma of split_row / split_col, pddblas.hpp:82-154).  A'B' is exact
+// through N int8 modular GEMMs (shared with the interval path) and the
+// Chinese remainder theorem; the integer is rounded to the nearest double
+// (hi) and its remainder to the nearest double (lo).  The CPU oracle
+// restates every step and the GPU result is bit-identical to it.
+#include "vcphip_device.h"
+#include "vcphip_kernels.h"
+
+namespace {
+
+typedef unsigned __int128 u128;
+
+// ---------------------------------------------------------------------------
+// stats: per row (or column) the exponent of max |hi| + |lo| (round-up sum)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_dd_stats_part(const double* __restrict__ X, long rs, long cs, int rows,
+                                                       int len, int nchunk, int* __restrict__ part) {
+  vcphip_mode_round_up();
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  const int c = blockIdx.y;
+  if (r >= rows) return;
+  const int c0 = c * VCPHIP_OZ_CHUNK, c1 = min(c0 + VCPHIP_OZ_CHUNK, len);
+  double mu = 0.0;
+  for (int cc = c0; cc < c1; ++cc) {
+    const double2 v = *reinterpret_cast<const double2*>(X + 2 * ((long)r * rs + (long)cc * cs));
+    const double a = __builtin_fabs(v.x) + __builtin_fabs(v.y);
+    mu = a > mu ? a : mu;
+  }
+  part[(long)c * rows + r] = mu > 0.0 ? vcphip_fexp(mu) : INT32_MIN;
+}
+
+__global__ __launch_bounds__(256) void k_dd_stats_final(int rows, int nchunk, const int* __restrict__ part,
+                                                        int* __restrict__ e, int* __restrict__ flags) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= rows) return;
+  int best = INT32_MIN;
+  for (int c = 0; c < nchunk; ++c) best = max(best, part[(long)c * rows + r]);
+  e[r] = best == INT32_MIN ? 0 : best;
+  flags[r] = best == INT32_MIN ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// split: 128-bit scaled integers -> symmetric int8 residues
+// ---------------------------------------------------------------------------
+// nearest integer (ties away) of the exact double y, as sign + magnitude
+VCPHIP_DEVINL void dbl_to_int_rn(double y, bool& neg, u128& mag) {
+  neg = y < 0.0;
+  const double a = __builtin_fabs(y);
+  if (!(a >= 0.5)) {
+    mag = 0;
+    return;
+  }
+  const int e = __builtin_amdgcn_frexp_exp(a);                              // a = f 2^e, f in [0.5, 1)
+  const unsigned long long m = (unsigned long long)__builtin_ldexp(__builtin_amdgcn_frexp_mant(a), 53);
+  const int p = e - 53;                                                     // a = m 2^p, p >= -53
+  if (p >= 0) mag = (u128)m << p;
+  else mag = (u128)((m + (1ull << (-p - 1))) >> (-p));
+}
+
+// A' = RN(hi 2^s) + RN(lo 2^s) as sign + magnitude
+VCPHIP_DEVINL void dd_scaled(double hi, double lo, int s, bool& neg, u128& mag) {
+  bool nh, nl;
+  u128 mh, ml;
+  dbl_to_int_rn(__builtin_ldexp(hi, s), nh, mh);
+  dbl_to_int_rn(__builtin_ldexp(lo, s), nl, ml);
+  if (nh == nl) {
+    neg = nh;
+    mag = mh + ml;
+  } else if (mh >= ml) {
+    neg = nh;
+    mag = mh - ml;
+  } else {
+    neg = nl;
+    mag = ml - mh;
+  }
+  if (mag == 0) neg = false;
+}
+
+VCPHIP_DEVINL int dd_residue(const unsigned (&c)[VCPHIP_DD_NCHUNK], bool neg, int t, const VcphipOzParams& P) {
+  const int m = P.mod[t];
+  unsigned y = 0;
+#pragma unroll
+  for (int i = 0; i < VCPHIP_DD_NCHUNK; ++i) y += c[i] * (unsigned)P.w20[i][t];   // < 6 * 2^28
+  const double yd = (double)y;
+  const double q = __builtin_floor(yd * P.rcpd[t]);
+  int r = (int)(yd - q * (double)m);
+  r += (r < 0) ? m : 0;
+  r -= (r >= m) ? m : 0;
+  if (neg && r != 0) r = m - r;
+  return (r >= (m + 1) / 2) ? r - m : r;
+}
+
+VCPHIP_DEVINL void chunks20(u128 mag, unsigned (&c)[VCPHIP_DD_NCHUNK]) {
+#pragma unroll
+  for (int i = 0; i < VCPHIP_DD_NCHUNK; ++i) c[i] = (unsigned)(mag >> (20 * i)) & 0xFFFFFu;
+}
+
+constexpr int SR_ROWS = 64, SR_K = 64, SR_LD = 68;
+
+// rows of A (m x k, column-major dd) -> N residue planes [mpad][kpad]
+__global__ __launch_bounds__(256) void k_dd_split_rows(const double* __restrict__ A, long lda, int m, int k, int mpad,
+                                                       int kpad, VcphipOzParams P, const int* __restrict__ ea,
+                                                       const int* __restrict__ fa, int8_t* __restrict__ planes) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char s_tile[];   // [N][64][SR_LD]
+  const int row0 = blockIdx.x * SR_ROWS, k0 = blockIdx.y * SR_K;
+  const int tid = threadIdx.x;
+  const int lr = tid % SR_ROWS, kq = tid / SR_ROWS;
+  const int r = row0 + lr;
+  const bool rowok = r < m && fa[min(r, m - 1)] == 0;
+  const int sc = rowok ? P.beta - ea[r] : 0;
+  for (int q = 0; q < SR_K / 4; ++q) {
+    const int lk = kq + 4 * q;
+    const int kk = k0 + lk;
+    double hi = 0.0, lo = 0.0;
+    if (rowok && kk < k) {
+      const double2 v = *reinterpret_cast<const double2*>(A + 2 * (r + lda * kk));
+      hi = v.x;
+      lo = v.y;
+    }
+    bool neg;
+    u128 mag;
+    dd_scaled(hi, lo, sc, neg, mag);
+    unsigned c[VCPHIP_DD_NCHUNK];
+    chunks20(mag, c);
+    for (int t = 0; t < P.nmod; ++t)
+      s_tile[(t * SR_ROWS + lr) * SR_LD + lk] = (unsigned char)(signed char)dd_residue(c, neg, t, P);
+  }
+  __syncthreads();
+  const int orow = tid / 4, oc = (tid % 4) * 16;
+  const long plane_stride = (long)mpad * kpad;
+  for (int p = 0; p < P.nmod; ++p) {
+    const unsigned* src = reinterpret_cast<const unsigned*>(s_tile + (p * SR_ROWS + orow) * SR_LD + oc);
+    *reinterpret_cast<uint4*>(planes + p * plane_stride + (long)(row0 + orow) * kpad + k0 + oc) =
+        make_uint4(src[0], src[1], src[2], src[3]);
+  }
+}
+
+// columns of B (k x n, column-major dd) -> N residue planes [npad][kpad]
+__global__ __launch_bounds__(256) void k_dd_split_cols(const double* __restrict__ B, long ldb, int k, int n, int npad,
+                                                       int kpad, VcphipOzParams P, const int* __restrict__ eb,
+                                                       const int* __restrict__ fb, int8_t* __restrict__ planes) {
+  const long gid = (long)blockIdx.x * 256 + threadIdx.x;
+  const int ngrp = kpad / 16;
+  const int j = (int)(gid / ngrp), grp = (int)(gid % ngrp);
+  if (j >= npad) return;
+  const bool colok = j < n && fb[min(j, n - 1)] == 0;
+  const int sc = colok ? P.beta - eb[j] : 0;
+  unsigned c[16][VCPHIP_DD_NCHUNK];
+  bool ng[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int kk = grp * 16 + u;
+    double hi = 0.0, lo = 0.0;
+    if (colok && kk < k) {
+      const double2 v = *reinterpret_cast<const double2*>(B + 2 * (kk + ldb * (long)j));
+      hi = v.x;
+      lo = v.y;
+    }
+    u128 mag;
+    dd_scaled(hi, lo, sc, ng[u], mag);
+    chunks20(mag, c[u]);
+  }
+  const long plane_stride = (long)npad * kpad;
+  int8_t* dst = planes + (long)j * kpad + grp * 16;
+  for (int t = 0; t < P.nmod; ++t) {
+    unsigned wv[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      wv[u >> 2] |= ((unsigned)(unsigned char)(signed char)dd_residue(c[u], ng[u], t, P)) << (8 * (u & 3));
+    *reinterpret_cast<uint4*>(dst + t * plane_stride) = make_uint4(wv[0], wv[1], wv[2], wv[3]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// reconstruction: CRT -> 256-bit integer -> nearest (hi, lo)
+// ---------------------------------------------------------------------------
+constexpr int NL = VCPHIP_OZ_NLIMB;
+
+VCPHIP_DEVINL int top_bit(const unsigned (&x)[NL]) {
+  int top = -1;
+#pragma unroll
+  for (int l = NL - 1; l >= 0; --l)
+    if (top < 0 && x[l] != 0) top = 32 * l + 31 - __builtin_clz(x[l]);
+  return top;
+}
+
+// bits [q, q+64) of x (q >= 0)
+VCPHIP_DEVINL unsigned long long bits64(const unsigned (&x)[NL], int q) {
+  const int ls = q >> 5, bs = q & 31;
+  unsigned w0 = 0, w1 = 0, w2 = 0;
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+    if (l == ls) w0 = x[l];
+    if (l == ls + 1) w1 = x[l];
+    if (l == ls + 2) w2 = x[l];
+  }
+  const unsigned long long lo = (unsigned long long)w0 | ((unsigned long long)w1 << 32);
+  return bs ? ((lo >> bs) | ((unsigned long long)w2 << (64 - bs))) : lo;
+}
+
+VCPHIP_DEVINL bool bit_at(const unsigned (&x)[NL], int b) {
+  bool v = false;
+  if (b < 0) return false;
+#pragma unroll
+  for (int l = 0; l < NL; ++l)
+    if (l == (b >> 5)) v = (x[l] >> (b & 31)) & 1u;
+  return v;
+}
+
+VCPHIP_DEVINL bool any_below(const unsigned (&x)[NL], int b) {   // any bit < b set
+  bool v = false;
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+    const int lo = 32 * l;
+    if (b >= lo + 32) v = v || x[l] != 0;
+    else if (b > lo) v = v || (x[l] & ((1u << (b - lo)) - 1u)) != 0;
+  }
+  return v;
+}
+
+// nearest double of x 2^s (ties to even); q_out = lowest kept bit, up_out = rounded up
+VCPHIP_DEVINL double rn_limbs(const unsigned (&x)[NL], int s, int& q_out, bool& up_out) {
+  const int top = top_bit(x);
+  q_out = 0;
+  up_out = false;
+  if (top < 0) return 0.0;
+  int q = top - 52;
+  if (q < -1074 - s) q = -1074 - s;
+  if (q <= 0) {
+    q_out = 0;
+    return __builtin_ldexp((double)bits64(x, 0), s);
+  }
+  unsigned long long mant = q < 32 * NL ? bits64(x, q) : 0ull;
+  const bool rb = bit_at(x, q - 1);
+  const bool st = any_below(x, q - 1);
+  const bool up = rb && (st || (mant & 1ull));
+  mant += up ? 1ull : 0ull;
+  q_out = q;
+  up_out = up;
+  const int ex = q + s;
+  if (mant != 0 && ex + (64 - __builtin_clzll(mant)) > 1024) return __builtin_inf();
+  return __builtin_ldexp((double)mant, ex);
+}
+
+template <int N>
+__global__ __launch_bounds__(256) void k_dd_recon(int m, int n, int mpad, int npad, VcphipOzParams P,
+                                                  const int* __restrict__ ea, const int* __restrict__ fa,
+                                                  const int* __restrict__ eb, const int* __restrict__ fb,
+                                                  const uint8_t* __restrict__ U, double* __restrict__ C, long ldc) {
+  const long gid = (long)blockIdx.x * 256 + threadIdx.x;
+  if (gid >= (long)m * n) return;
+  const int i = (int)(gid % m), j = (int)(gid / m);
+  if (fa[i] || fb[j]) {
+    *reinterpret_cast<double2*>(C + 2 * (i + ldc * (long)j)) = make_double2(0.0, 0.0);
+    return;
+  }
+  const long boff = ((long)(j >> 4) * (mpad / 16) + (i >> 4)) * 256 + ((j & 15) + 16 * ((i & 15) >> 2)) * 4 + (i & 3);
+  const long plane = (long)mpad * npad;
+  int v[N];
+#pragma unroll
+  for (int t = 0; t < N; ++t) {
+    const int md = P.mod[t];
+    int acc = (int)U[(long)t * plane + boff];
+#pragma unroll
+    for (int s2 = 0; s2 < t; ++s2) acc += v[s2] * (int)P.negP[s2][t];
+    int q = (int)__builtin_floorf((float)acc * P.rcpf[t]);
+    int r = acc - q * md;
+    r += (r < 0) ? md : 0;
+    r -= (r >= md) ? md : 0;
+    const int x = r * P.pinv[t];
+    q = (int)__builtin_floorf((float)x * P.rcpf[t]);
+    r = x - q * md;
+    r += (r < 0) ? md : 0;
+    r -= (r >= md) ? md : 0;
+    v[t] = r;
+  }
+  unsigned mag[NL];
+#pragma unroll
+  for (int l = 0; l < NL; ++l) mag[l] = 0;
+  mag[0] = (unsigned)v[N - 1];
+#pragma unroll
+  for (int t = N - 2; t >= 0; --t) {
+    unsigned long long carry = (unsigned)v[t];
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+      const unsigned long long p = (unsigned long long)mag[l] * (unsigned)P.mod[t] + carry;
+      mag[l] = (unsigned)p;
+      carry = p >> 32;
+    }
+  }
+  bool gt = false, decided = false;
+#pragma unroll
+  for (int l = NL - 1; l >= 0; --l)
+    if (!decided && mag[l] != P.halfM[l]) { gt = mag[l] > P.halfM[l]; decided = true; }
+  const bool neg = gt;
+  if (neg) {
+    long long borrow = 0;
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+      const long long d = (long long)P.Mlimb[l] - (long long)mag[l] - borrow;
+      borrow = d < 0 ? 1 : 0;
+      mag[l] = (unsigned)(d + (borrow << 32));
+    }
+  }
+  const int sc = ea[i] + eb[j] - 2 * P.beta;
+  int q;
+  bool up;
+  const double hi = rn_limbs(mag, sc, q, up);
+  // remainder |X| - round(|X|) at bit q: low bits, or 2^q - low bits if rounded up
+  unsigned rem[NL];
+  bool rem_neg = false;
+  if (q <= 0) {
+#pragma unroll
+    for (int l = 0; l < NL; ++l) rem[l] = 0;
+  } else {
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+      const int lo = 32 * l;
+      unsigned w = mag[l];
+      if (q <= lo) w = 0;
+      else if (q < lo + 32) w &= (1u << (q - lo)) - 1u;
+      rem[l] = w;
+    }
+    if (up) {   // rem = 2^q - rem
+      rem_neg = true;
+      long long borrow = 0;
+#pragma unroll
+      for (int l = 0; l < NL; ++l) {
+        const int lo = 32 * l;
+        const unsigned long long twoq = (q >= lo && q < lo + 32) ? (1ull << (q - lo)) : 0ull;
+        const long long d = (long long)twoq - (long long)rem[l] - borrow;
+        borrow = d < 0 ? 1 : 0;
+        rem[l] = (unsigned)(d + (borrow << 32));
+      }
+    }
+  }
+  int q2;
+  bool up2;
+  double lo = rn_limbs(rem, sc, q2, up2);
+  if (rem_neg) lo = -lo;
+  const double sgn = neg ? -1.0 : 1.0;
+  *reinterpret_cast<double2*>(C + 2 * (i + ldc * (long)j)) = make_double2(sgn * hi, sgn * lo);
+}
+
+}  // namespace
+
+int vcphip_dd_stats(const double* X, long rs, long cs, int rows, int len, int* e, int* flags, int* part,
+                    hipStream_t s) {
+  if (rows <= 0) return 0;
+  const int nchunk = len > 0 ? (len + VCPHIP_OZ_CHUNK - 1) / VCPHIP_OZ_CHUNK : 0;
+  if (nchunk > 0)
+    hipLaunchKernelGGL(k_dd_stats_part, dim3((rows + 255) / 256, nchunk), dim3(256), 0, s, X, rs, cs, rows, len,
+                       nchunk, part);
+  hipLaunchKernelGGL(k_dd_stats_final, dim3((rows + 255) / 256), dim3(256), 0, s, rows, nchunk, part, e, flags);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_dd_split_rows(const double* A, long lda, int m, int k, int mpad, int kpad, const VcphipOzParams& P,
+                         const int* e, const int* flags, int8_t* planes, hipStream_t s) {
+  const size_t lds = (size_t)P.nmod * SR_ROWS * SR_LD;
+  hipLaunchKernelGGL(k_dd_split_rows, dim3(mpad / SR_ROWS, kpad / SR_K), dim3(256), lds, s, A, lda, m, k, mpad, kpad,
+                     P, e, flags, planes);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_dd_split_cols(const double* B, long ldb, int k, int n, int npad, int kpad, const VcphipOzParams& P,
+                         const int* e, const int* flags, int8_t* planes, hipStream_t s) {
+  const long threads = (long)npad * (kpad / 16);
+  hipLaunchKernelGGL(k_dd_split_cols, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, B, ldb, k, n, npad,
+                     kpad, P, e, flags, planes);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_dd_recon(int m, int n, int mpad, int npad, const VcphipOzParams& P, const int* ea, const int* fa,
+                    const int* eb, const int* fb, const uint8_t* U, double* C, long ldc, hipStream_t s) {
+  if (m <= 0 || n <= 0) return 0;
+  const long total = (long)m * n;
+  dim3 grid((unsigned)((total + 255) / 256));
+  switch (P.nmod) {
+#define VCPHIP_DD_CASE(NN) \
+  case NN: hipLaunchKernelGGL((k_dd_recon<NN>), grid, dim3(256), 0, s, m, n, mpad, npad, P, ea, fa, eb, fb, U, C, ldc); break;
+    VCPHIP_DD_CASE(24) VCPHIP_DD_CASE(25) VCPHIP_DD_CASE(26) VCPHIP_DD_CASE(27) VCPHIP_DD_CASE(28)
+    VCPHIP_DD_CASE(29) VCPHIP_DD_CASE(30) VCPHIP_DD_CASE(31) VCPHIP_DD_CASE(32)
+#undef VCPHIP_DD_CASE
+    default: return -7;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}

@@ -23,7 +23,7 @@ struct vcphip_ctx {
   hipStream_t stream = nullptr;
   long long oz_nmod = 0, oz_variant = 0, timing = 0, dd_nmod = 0;
   // workspace slots (device memory, grown on demand)
-  enum { WS_PA, WS_PB, WS_U, WS_Z, WS_SA, WS_SB, WS_HA, WS_HB, WS_HC, WS_HC2, WS_BATCH, WS_N };
+  enum { WS_PA, WS_PB, WS_U, WS_Z, WS_SA, WS_SB, WS_HA, WS_HB, WS_HC, WS_HC2, WS_BATCH, WS_DD, WS_N };
   void* ws[WS_N] = {};
   size_t ws_size[WS_N] = {};
   // stage timing: events recorded by every call since the last synchronize,
@@ -128,12 +128,21 @@ int oz_beta(int k, int nmod, const int* mods) {
   return beta > 62 ? 62 : beta;
 }
 
-int oz_make_params(int k, int nmod, VcphipOzParams* P) {
+// CRT constants shared by the interval and dd forms; beta from the no-wrap
+// condition k (2^beta + slack)^2 < M/2, i.e. k 2^(2 beta + extra) < M
+int oz_fill_params(int k, int nmod, int extra, int beta_cap, VcphipOzParams* P) {
   memset(P, 0, sizeof(*P));
   if (nmod < 8 || nmod > VCPHIP_OZ_MAXMOD) return VCPHIP_ERR_ARGUMENT;
   if (oz_moduli(nmod, P->mod) != nmod) return VCPHIP_ERR_ARGUMENT;
   P->nmod = nmod;
-  P->beta = oz_beta(k, nmod, P->mod);
+  {
+    double lm = 0.0;
+    for (int t = 0; t < nmod; ++t) lm += log2((double)P->mod[t]);
+    int lk = 0;
+    while ((1ll << lk) < (long long)(k > 1 ? k : 1)) ++lk;
+    const int beta = (int)floor((lm - (double)extra - (double)lk - 1e-6) / 2.0);
+    P->beta = beta > beta_cap ? beta_cap : beta;
+  }
   if (P->beta < 8) return VCPHIP_ERR_UNSUPPORTED;
   for (int t = 0; t < nmod; ++t) {
     const int m = P->mod[t];
@@ -141,15 +150,19 @@ int oz_make_params(int k, int nmod, VcphipOzParams* P) {
     P->rcpd[t] = 1.0 / (double)m;
     P->w21[t] = (int)((1ull << 21) % (unsigned)m);
     P->w42[t] = (int)((1ull << 42) % (unsigned long long)m);
+    unsigned long long w = 1;
+    for (int c = 0; c < VCPHIP_DD_NCHUNK; ++c) {
+      P->w20[c][t] = (uint8_t)(w % (unsigned)m);
+      w = (w % (unsigned)m) * ((1ull << 20) % (unsigned)m) % (unsigned)m;
+    }
     long long prod = 1;   // (m_0 ... m_{t-1}) mod m_t
     for (int s = 0; s < t; ++s) {
-      P->negP[s][t] = (int)((m - prod % m) % m);
+      P->negP[s][t] = (uint8_t)((m - prod % m) % m);
       prod = (prod * P->mod[s]) % m;
     }
     P->pinv[t] = modinv((int)(prod % m), m);
   }
-  // M and M/2 in 32-bit limbs
-  uint32_t M[VCPHIP_OZ_NLIMB] = {1, 0, 0, 0, 0};
+  uint32_t M[VCPHIP_OZ_NLIMB] = {1, 0, 0, 0, 0, 0, 0, 0};
   for (int t = 0; t < nmod; ++t) {
     unsigned long long carry = 0;
     for (int l = 0; l < VCPHIP_OZ_NLIMB; ++l) {
@@ -172,6 +185,25 @@ int oz_make_params(int k, int nmod, VcphipOzParams* P) {
     fesetround(old);
   }
   return VCPHIP_OK;
+}
+
+// interval form: |A'| <= 2^beta, k 2^(2 beta + 1) < M, beta <= 62 (int64)
+int oz_make_params(int k, int nmod, VcphipOzParams* P) {
+  if (nmod > VCPHIP_OZ_MAXMOD_IV) return VCPHIP_ERR_ARGUMENT;
+  return oz_fill_params(k, nmod, 1, 62, P);
+}
+
+// dd form: |A'| <= 2^beta + 1, k 2^(2 beta + 2) < M, beta <= 115 (int128)
+int dd_make_params(int k, int nmod, VcphipOzParams* P) { return oz_fill_params(k, nmod, 2, 115, P); }
+
+int default_dd_nmod(int k) {
+  // smallest count giving beta >= 110 (a product accurate to ~2^-108 of the
+  // row / column scales, the dd precision)
+  for (int nmod = 24; nmod <= VCPHIP_OZ_MAXMOD; ++nmod) {
+    VcphipOzParams P;
+    if (dd_make_params(k, nmod, &P) == VCPHIP_OK && P.beta >= 110) return nmod;
+  }
+  return VCPHIP_OZ_MAXMOD;
 }
 
 int default_nmod(int k) {
@@ -274,7 +306,7 @@ extern "C" int vcphip_ctx_set_param(vcphip_ctx* c, int key, long long v) {
   if (!c) return VCPHIP_ERR_ARGUMENT;
   switch (key) {
     case VCPHIP_PARAM_OZ_NMOD:
-      if (v != 0 && (v < 8 || v > VCPHIP_OZ_MAXMOD)) return VCPHIP_ERR_ARGUMENT;
+      if (v != 0 && (v < 8 || v > VCPHIP_OZ_MAXMOD_IV)) return VCPHIP_ERR_ARGUMENT;
       c->oz_nmod = v;
       return VCPHIP_OK;
     case VCPHIP_PARAM_OZ_VARIANT:
@@ -287,7 +319,10 @@ extern "C" int vcphip_ctx_set_param(vcphip_ctx* c, int key, long long v) {
       c->st_ms.clear();
       c->st_cnt.clear();
       return VCPHIP_OK;
-    case VCPHIP_PARAM_DD_NMOD: c->dd_nmod = v; return VCPHIP_OK;
+    case VCPHIP_PARAM_DD_NMOD:
+      if (v != 0 && (v < 24 || v > VCPHIP_OZ_MAXMOD)) return VCPHIP_ERR_ARGUMENT;
+      c->dd_nmod = v;
+      return VCPHIP_OK;
     default: return VCPHIP_ERR_ARGUMENT;
   }
 }
@@ -449,11 +484,62 @@ extern "C" int vcphip_gemm_rurd_dev(vcphip_ctx* c, int m, int n, int k, const do
       vcphip_launch_drgemm(VCPHIP_DR_PP, m, n, k, A, lda, B, ldb, Cup, ldc, Cdn, accumulate ? 1 : 0, c->stream));
 }
 
+// dd product C = op(A) * B through the Ozaki-II pipeline.  tn: C = A^T A,
+// where both operands are the columns of A (k x n).
+static int dd_gemm_dev(vcphip_ctx* c, bool tn, int m, int n, int k, const double* A, int lda, const double* B,
+                       int ldb, double* C, int ldc) {
+  if (k > VCPHIP_OZ_MAXK) return VCPHIP_ERR_UNSUPPORTED;
+  const int nmod = c->dd_nmod ? (int)c->dd_nmod : default_dd_nmod(k);
+  VcphipOzParams P;
+  int rc = dd_make_params(k, nmod, &P);
+  if (rc) return rc;
+  const int mpad = (m + VCPHIP_OZ_PADMN - 1) / VCPHIP_OZ_PADMN * VCPHIP_OZ_PADMN;
+  const int npad = (n + VCPHIP_OZ_PADMN - 1) / VCPHIP_OZ_PADMN * VCPHIP_OZ_PADMN;
+  const int kpad = k > 0 ? (k + VCPHIP_OZ_PADK - 1) / VCPHIP_OZ_PADK * VCPHIP_OZ_PADK : VCPHIP_OZ_PADK;
+  const int nchunk = k > 0 ? (k + VCPHIP_OZ_CHUNK - 1) / VCPHIP_OZ_CHUNK : 1;
+  void *pa = nullptr, *pb, *pu, *st;
+  if (!tn && (rc = ws_get(c, vcphip_ctx::WS_PA, (size_t)nmod * mpad * kpad, &pa))) return rc;
+  if ((rc = ws_get(c, vcphip_ctx::WS_PB, (size_t)nmod * npad * kpad, &pb))) return rc;
+  if ((rc = ws_get(c, vcphip_ctx::WS_U, (size_t)nmod * mpad * npad, &pu))) return rc;
+  const size_t rows_all = (size_t)m + n;
+  if ((rc = ws_get(c, vcphip_ctx::WS_DD, sizeof(int) * (2 * rows_all + (size_t)nchunk * rows_all) + 1024, &st))) return rc;
+  int* ea = (int*)st;
+  int* fa = ea + m;
+  int* eb = fa + m;
+  int* fb = eb + n;
+  int* part = fb + n;
+  StageTimer T(c);
+  if (!tn) {
+    if ((rc = check_launch(vcphip_dd_stats(A, 1, lda, m, k, ea, fa, part, c->stream)))) return rc;
+  }
+  if ((rc = check_launch(vcphip_dd_stats(B, ldb, 1, n, k, eb, fb, part, c->stream)))) return rc;
+  T.mark("stats");
+  if (!tn && (rc = check_launch(vcphip_dd_split_rows(A, lda, m, k, mpad, kpad, P, ea, fa, (int8_t*)pa, c->stream))))
+    return rc;
+  if ((rc = check_launch(vcphip_dd_split_cols(B, ldb, k, n, npad, kpad, P, eb, fb, (int8_t*)pb, c->stream)))) return rc;
+  T.mark("split");
+  int variant = (int)c->oz_variant;
+  if (variant == 0) variant = 4;
+  const int8_t* Aplanes = tn ? (const int8_t*)pb : (const int8_t*)pa;
+  if ((rc = check_launch(vcphip_oz_gemm(mpad, npad, kpad, nmod, 0, Aplanes, (const int8_t*)pb, P, (uint8_t*)pu,
+                                        nullptr, variant, c->stream))))
+    return rc;
+  T.mark("gemm");
+  if ((rc = check_launch(vcphip_dd_recon(m, n, mpad, npad, P, tn ? eb : ea, tn ? fb : fa, eb, fb, (const uint8_t*)pu,
+                                         C, ldc, c->stream))))
+    return rc;
+  T.mark("recon");
+  return VCPHIP_OK;
+}
+
 extern "C" int vcphip_ddgemm_dev(vcphip_ctx* c, int m, int n, int k, const double* A, int lda, const double* B,
                                  int ldb, double* C, int ldc) {
-  (void)m; (void)n; (void)k; (void)A; (void)lda; (void)B; (void)ldb; (void)C; (void)ldc;
   if (!c) return VCPHIP_ERR_ARGUMENT;
-  return VCPHIP_ERR_UNSUPPORTED;
+  if (!dims_ok(m, n, k) || lda < (m > 1 ? m : 1) || ldb < (k > 1 ? k : 1) || ldc < (m > 1 ? m : 1))
+    return VCPHIP_ERR_DIMENSION;
+  if (m == 0 || n == 0) return VCPHIP_OK;
+  hipSetDevice(c->device);
+  return dd_gemm_dev(c, false, m, n, k, A, lda, B, ldb, C, ldc);
 }
 
 static bool same_problem(const VcphipResidProblem& a, const VcphipResidProblem& b) {
@@ -692,15 +778,31 @@ extern "C" int vcphip_gemm_rurd(vcphip_ctx* c, int m, int n, int k, const double
 
 extern "C" int vcphip_ddgemm(vcphip_ctx* c, int m, int n, int k, const double* A, int lda, const double* B, int ldb,
                              double* C, int ldc) {
-  (void)m; (void)n; (void)k; (void)A; (void)lda; (void)B; (void)ldb; (void)C; (void)ldc;
   if (!c) return VCPHIP_ERR_ARGUMENT;
-  return VCPHIP_ERR_UNSUPPORTED;
+  if (!dims_ok(m, n, k) || lda < (m > 1 ? m : 1) || ldb < (k > 1 ? k : 1) || ldc < (m > 1 ? m : 1))
+    return VCPHIP_ERR_DIMENSION;
+  if (m == 0 || n == 0) return VCPHIP_OK;
+  hipSetDevice(c->device);
+  double *dA, *dB;
+  void* p;
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HA, A, lda, m, k, 16, &dA));
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HB, B, ldb, k, n, 16, &dB));
+  VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HC, 16 * (size_t)m * n, &p));
+  VCPHIP_TRY(vcphip_ddgemm_dev(c, m, n, k, dA, m, dB, k > 0 ? k : 1, (double*)p, m));
+  return to_host(c, (double*)p, C, ldc, m, n, 16);
 }
 
 extern "C" int vcphip_ddgemm_tn(vcphip_ctx* c, int k, int n, const double* A, int lda, double* C, int ldc) {
-  (void)k; (void)n; (void)A; (void)lda; (void)C; (void)ldc;
   if (!c) return VCPHIP_ERR_ARGUMENT;
-  return VCPHIP_ERR_UNSUPPORTED;
+  if (k < 0 || n < 0 || lda < (k > 1 ? k : 1) || ldc < (n > 1 ? n : 1)) return VCPHIP_ERR_DIMENSION;
+  if (n == 0) return VCPHIP_OK;
+  hipSetDevice(c->device);
+  double* dA;
+  void* p;
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HA, A, lda, k, n, 16, &dA));
+  VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HC, 16 * (size_t)n * n, &p));
+  VCPHIP_TRY(dd_gemm_dev(c, true, n, n, k, dA, k > 0 ? k : 1, dA, k > 0 ? k : 1, (double*)p, n));
+  return to_host(c, (double*)p, C, ldc, n, n, 16);
 }
 
 extern "C" int vcphip_residual_batched(vcphip_ctx* c, int count, const int* n, const double* const* R, const int* ldr,

@@ -33,22 +33,25 @@ int vcphip_launch_resid(int ntiles, const VcphipResidTile* tiles, const VcphipRe
                         hipStream_t s);
 
 // ---- Ozaki scheme II (kern_ozaki.hip) ------------------------------------
-#define VCPHIP_OZ_MAXMOD 20
+#define VCPHIP_OZ_MAXMOD 32   // moduli (<= 20 for intervals, up to 32 for double-double)
+#define VCPHIP_OZ_MAXMOD_IV 20
 #define VCPHIP_OZ_CHUNK 256   // k-chunk of the upward 1-norm partial sums
 #define VCPHIP_OZ_PADMN 256   // m and n are padded to multiples of this
 #define VCPHIP_OZ_PADK 128    // k is padded to a multiple of this
-#define VCPHIP_OZ_NLIMB 5     // 32-bit limbs of M (M < 2^160 for N <= 20)
+#define VCPHIP_OZ_NLIMB 8     // 32-bit limbs of M (M < 2^256)
+#define VCPHIP_DD_NCHUNK 6    // 20-bit chunks of a scaled dd integer (< 2^120)
 
 struct VcphipOzParams {
   int nmod;                    // number of moduli N
-  int beta;                    // scaled integers satisfy |A'| <= 2^beta
+  int beta;                    // scaled integers satisfy |A'| <= 2^beta (+1 for dd)
   int mod[VCPHIP_OZ_MAXMOD];   // pairwise coprime moduli (mod[0] = 256)
   float rcpf[VCPHIP_OZ_MAXMOD];
   double rcpd[VCPHIP_OZ_MAXMOD];
   int w21[VCPHIP_OZ_MAXMOD];   // 2^21 mod m_t
   int w42[VCPHIP_OZ_MAXMOD];   // 2^42 mod m_t
   int pinv[VCPHIP_OZ_MAXMOD];  // (m_0 ... m_{t-1})^{-1} mod m_t
-  int negP[VCPHIP_OZ_MAXMOD][VCPHIP_OZ_MAXMOD];  // [s][t] = -(m_0 ... m_{s-1}) mod m_t
+  uint8_t negP[VCPHIP_OZ_MAXMOD][VCPHIP_OZ_MAXMOD];  // [s][t] = -(m_0 ... m_{s-1}) mod m_t
+  uint8_t w20[VCPHIP_DD_NCHUNK][VCPHIP_OZ_MAXMOD];   // 2^(20 c) mod m_t
   uint32_t halfM[VCPHIP_OZ_NLIMB];  // M/2 as little-endian 32-bit limbs
   uint32_t Mlimb[VCPHIP_OZ_NLIMB];  // M
   double c1;                   // RU(1/127)
@@ -87,3 +90,14 @@ int vcphip_oz_gemm(int mpad, int npad, int kpad, int nmod, int nrad, const int8_
 int vcphip_oz_recon(int m, int n, int k, int mpad, int npad, const VcphipOzParams& P, VcphipOzSide sa,
                     VcphipOzSide sb, const uint8_t* U, const int32_t* Z, double* C, long ldc,
                     hipStream_t s);
+
+// ---- double-double product, Ozaki scheme II (kern_dd.hip) -------------------
+// rows (rs, cs) of a dd operand: e[r] with max |hi|+|lo| < 2^e; flags bit0: zero
+int vcphip_dd_stats(const double* X, long rs, long cs, int rows, int len, int* e, int* flags, int* part,
+                    hipStream_t s);
+int vcphip_dd_split_rows(const double* A, long lda, int m, int k, int mpad, int kpad, const VcphipOzParams& P,
+                         const int* e, const int* flags, int8_t* planes, hipStream_t s);
+int vcphip_dd_split_cols(const double* B, long ldb, int k, int n, int npad, int kpad, const VcphipOzParams& P,
+                         const int* e, const int* flags, int8_t* planes, hipStream_t s);
+int vcphip_dd_recon(int m, int n, int mpad, int npad, const VcphipOzParams& P, const int* ea, const int* fa,
+                    const int* eb, const int* fb, const uint8_t* U, double* C, long ldc, hipStream_t s);
