@@ -662,10 +662,12 @@ typedef struct {
 
 /* statistics of the rows of a mid/rad operand: element c of row r is at
  * r*rs + c*cs (rows of A: rs=1, cs=m; columns of B: rs=k, cs=1).  Upward
- * 1-norms are summed sequentially inside k-chunks of OZ_CHUNK, then over the
- * chunks in order. */
+ * 1-norm order: lanes == 0 -- sequential inside k-chunks of OZ_CHUNK, then
+ * over the chunks in order (rows of A); lanes == 64 -- partial l sums the
+ * elements l, l+64, ... in order, then the 64 partials in order (columns of
+ * B, one GPU wave per column). */
 static void oz_stats(int rows, int len, const double* M, const double* R, size_t rs, size_t cs,
-                     int beta, oz_side* S) {
+                     int beta, oz_side* S, int lanes) {
     S->beta = beta;
     S->e = (int*)malloc(sizeof(int) * rows);
     S->g = (int*)malloc(sizeof(int) * rows);
@@ -676,10 +678,14 @@ static void oz_stats(int rows, int len, const double* M, const double* R, size_t
     fesetround(FE_UPWARD);
     for (int r = 0; r < rows; ++r) {
         double mu = 0.0, smax = 0.0, rmax = 0.0, tot = 0.0;
-        for (int c0 = 0; c0 < len; c0 += OZ_CHUNK) {
+        const int step = lanes > 0 ? 1 : OZ_CHUNK;
+        const int nblk = lanes > 0 ? lanes : (len + OZ_CHUNK - 1) / OZ_CHUNK;
+        for (int b = 0; b < nblk; ++b) {
             double part = 0.0;
-            const int c1 = (c0 + OZ_CHUNK < len) ? c0 + OZ_CHUNK : len;
-            for (int c = c0; c < c1; ++c) {
+            const int c0 = lanes > 0 ? b : b * step;
+            const int c1 = lanes > 0 ? len : ((c0 + OZ_CHUNK < len) ? c0 + OZ_CHUNK : len);
+            const int inc = lanes > 0 ? lanes : 1;
+            for (int c = c0; c < c1; c += inc) {
                 const size_t idx = (size_t)r * rs + (size_t)c * cs;
                 const double am = fabs(M[idx]), ar = R[idx];
                 const double as = am + ar;
@@ -704,7 +710,7 @@ void orc_ozaki_scaled_rows(int m, int k, const double* A, int lda, int beta, int
     double *mA = dalloc((size_t)m * k), *rA = dalloc((size_t)m * k);
     midrad_mat(m, k, A, lda, mA, rA);
     oz_side S;
-    oz_stats(m, k, mA, rA, 1, (size_t)m, beta, &S);
+    oz_stats(m, k, mA, rA, 1, (size_t)m, beta, &S, 0);
     const int old = fegetround();
     fesetround(FE_UPWARD);
     for (int j = 0; j < k; ++j)
@@ -719,7 +725,7 @@ void orc_ozaki_scaled_cols(int k, int n, const double* B, int ldb, int beta, int
     double *mB = dalloc((size_t)k * n), *rB = dalloc((size_t)k * n);
     midrad_mat(k, n, B, ldb, mB, rB);
     oz_side S;
-    oz_stats(n, k, mB, rB, (size_t)k, 1, beta, &S);
+    oz_stats(n, k, mB, rB, (size_t)k, 1, beta, &S, 64);
     const int old = fegetround();
     fesetround(FE_UPWARD);
     for (int j = 0; j < n; ++j)
@@ -745,8 +751,8 @@ int orc_ozaki_igemm(int m, int n, int k, const double* A, int lda, const double*
     midrad_mat(m, k, A, lda, mA, rA);
     midrad_mat(k, n, B, ldb, mB, rB);
     oz_side SA, SB;
-    oz_stats(m, k, mA, rA, 1, (size_t)m, beta, &SA);
-    oz_stats(n, k, mB, rB, (size_t)k, 1, beta, &SB);
+    oz_stats(m, k, mA, rA, 1, (size_t)m, beta, &SA, 0);
+    oz_stats(n, k, mB, rB, (size_t)k, 1, beta, &SB, 64);
     int64_t* Ap = (int64_t*)malloc(sizeof(int64_t) * mk); /* row i contiguous */
     int64_t* Bp = (int64_t*)malloc(sizeof(int64_t) * kn); /* col j contiguous */
     /* digit planes (h, l) of |mA|+rA, rA (rows of A) and rB, |mB| (cols of B) */

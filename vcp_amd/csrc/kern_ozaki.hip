@@ -38,6 +38,7 @@ __global__ __launch_bounds__(256) void k_oz_stats_part(const double* __restrict_
   const int c1 = min(c0 + VCPHIP_OZ_CHUNK, len);
   double mu = 0.0, smax = 0.0, rmax = 0.0, sum = 0.0;
   const double* base = X + 2 * ((long)r * rs);
+#pragma unroll 8
   for (int cc = c0; cc < c1; ++cc) {
     const double2 v = *reinterpret_cast<const double2*>(base + 2 * ((long)cc * cs));
     double m, rr;
@@ -54,6 +55,58 @@ __global__ __launch_bounds__(256) void k_oz_stats_part(const double* __restrict_
   part[1 * stride + (long)c * rows + r] = smax;
   part[2 * stride + (long)c * rows + r] = rmax;
   part[3 * stride + (long)c * rows + r] = sum;
+}
+
+// columns of B (k x n, column k-contiguous): one wave per column.  Lane l
+// accumulates the elements l, l+64, l+128, ... in ascending order (coalesced
+// loads); the upward 1-norm is then the sequential upward sum of the 64 lane
+// partials in lane order.  Maxima are order-independent.
+VCPHIP_DEVINL double wave_max(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double y = __shfl_xor(x, o, 64);
+    x = y > x ? y : x;
+  }
+  return x;
+}
+
+__global__ __launch_bounds__(256) void k_oz_stats_cols(const double* __restrict__ X, long ldx, int cols, int len,
+                                                       VcphipOzSide side) {
+  vcphip_mode_round_up();
+  __shared__ double s_sum[4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int j = blockIdx.x * 4 + w;
+  double mu = 0.0, smax = 0.0, rmax = 0.0, sum = 0.0;
+  if (j < cols) {
+    const double* col = X + 2 * ldx * (long)j;
+#pragma unroll 4
+    for (int kk = lane; kk < len; kk += 64) {
+      const double2 v = *reinterpret_cast<const double2*>(col + 2 * (long)kk);
+      double m, rr;
+      vcphip_midrad_ru(v.x, v.y, m, rr);
+      const double am = __builtin_fabs(m);
+      const double as = am + rr;
+      mu = am > mu ? am : mu;
+      smax = as > smax ? as : smax;
+      rmax = rr > rmax ? rr : rmax;
+      sum = sum + am;
+    }
+  }
+  mu = wave_max(mu);
+  smax = wave_max(smax);
+  rmax = wave_max(rmax);
+  s_sum[w][lane] = sum;
+  __syncthreads();
+  if (lane == 0 && j < cols) {
+    double tot = 0.0;
+    for (int l = 0; l < 64; ++l) tot = tot + s_sum[w][l];
+    side.flags[j] = (mu == 0.0 ? VCPHIP_OZ_MIDZERO : 0) | (smax == 0.0 ? VCPHIP_OZ_SUMZERO : 0) |
+                    (rmax == 0.0 ? VCPHIP_OZ_RADZERO : 0);
+    side.e[j] = mu > 0.0 ? vcphip_fexp(mu) : 0;
+    side.g[j] = smax > 0.0 ? vcphip_fexp(smax) : 0;
+    side.h[j] = rmax > 0.0 ? vcphip_fexp(rmax) : 0;
+    side.n1[j] = tot;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_oz_stats_final(int rows, int nchunk, const double* __restrict__ part,
@@ -124,7 +177,7 @@ VCPHIP_DEVINL void split_parts(long long a, unsigned& c0, unsigned& c1, unsigned
   c2 = (unsigned)(u >> 42);
 }
 
-constexpr int SR_ROWS = 64, SR_K = 64, SR_LD = 68;   // LDS byte tile, odd-word row stride
+constexpr int SR_ROWS = 32, SR_K = 64, SR_LD = 68;   // LDS byte tile, odd-word row stride
 
 // rows of A (m x k, column-major intervals) -> planes [N+4][mpad][kpad]:
 // N residue planes of A', then the hi/lo digits of |mid|+rad and of rad.
@@ -136,6 +189,7 @@ __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict_
   const int nplane = P.nmod + VCPHIP_OZ_NDIGIT;
   const int row0 = blockIdx.x * SR_ROWS, k0 = blockIdx.y * SR_K;
   const int tid = threadIdx.x;
+  constexpr int KQ = 256 / SR_ROWS;   // k phases per thread
   const int lr = tid % SR_ROWS, kq = tid / SR_ROWS;
   const int r = row0 + lr;
   const bool rowok = r < m;
@@ -143,8 +197,8 @@ __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict_
   const int g = rowok ? side.g[r] : 0;
   const int h = rowok ? side.h[r] : 0;
   const bool zr = rowok ? (side.flags[r] & VCPHIP_OZ_MIDZERO) != 0 : true;
-  for (int q = 0; q < SR_K / 4; ++q) {
-    const int lk = kq + 4 * q;
+  for (int q = 0; q < SR_K / KQ; ++q) {
+    const int lk = kq + KQ * q;
     const int kk = k0 + lk;
     double lo = 0.0, hi = 0.0;
     if (rowok && kk < k) {
@@ -168,13 +222,12 @@ __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict_
     s_tile[((P.nmod + 3) * SR_ROWS + lr) * SR_LD + lk] = (unsigned char)(dr >> 8);
   }
   __syncthreads();
-  // write out: per plane 64 rows x 64 bytes, 16 bytes per thread
-  const int orow = tid / 4, oc = (tid % 4) * 16;
+  // write out: per plane SR_ROWS rows x 64 bytes, 8 bytes per thread
+  const int orow = tid / 8, oc = (tid % 8) * 8;
   const long plane_stride = (long)mpad * kpad;
   for (int p = 0; p < nplane; ++p) {
     const unsigned* src = reinterpret_cast<const unsigned*>(s_tile + (p * SR_ROWS + orow) * SR_LD + oc);
-    uint4 v = make_uint4(src[0], src[1], src[2], src[3]);
-    *reinterpret_cast<uint4*>(planes + p * plane_stride + (long)(row0 + orow) * kpad + k0 + oc) = v;
+    *reinterpret_cast<uint2*>(planes + p * plane_stride + (long)(row0 + orow) * kpad + k0 + oc) = make_uint2(src[0], src[1]);
   }
 }
 
@@ -390,6 +443,159 @@ __global__ __launch_bounds__(256) void k_oz_gemm(int mpad, int npad, int kpad, i
   }
 }
 
+// LDS byte address of a __shared__ object.
+__device__ __forceinline__ uint32_t lds_addr(const void* p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
+}
+// One wave-wide 16-byte-per-lane load straight into LDS: lane l's 16 bytes
+// land at lds_base + 16 l (lds_base must be wave-uniform).
+__device__ __forceinline__ void lds_dma16(const int8_t* g, uint32_t lds_base) {
+  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g), "s"(lds_base) : "memory", "m0");
+}
+
+// Variant with direct global->LDS loads (global_load_lds_dwordx4): three LDS
+// stages, two in flight while the MFMAs run on the third.  Rows of a stage are
+// 64 bytes; the 16-byte chunks of a row are XOR-swizzled by (row >> 2) & 3 so
+// the 16-byte fragment reads are bank-conflict free.  Each wave fills whole
+// 1 KB segments (16 rows): lane l writes segment byte 16 l, i.e. row l / 4,
+// physical chunk l % 4, which holds logical chunk (l % 4) ^ ((l / 16) % 4).
+template <int FM, int FN>
+__global__ __launch_bounds__(256) void k_oz_gemm3(int mpad, int npad, int kpad, int nmod, int nz,
+                                                  const int8_t* __restrict__ Aplanes,
+                                                  const int8_t* __restrict__ Bplanes, VcphipOzParams P,
+                                                  uint8_t* __restrict__ U, int32_t* __restrict__ Z) {
+  constexpr int WM = 16 * FM, WN = 16 * FN;
+  constexpr int TMB = 2 * WM, TNB = 2 * WN;
+  constexpr int NST = 3;                  // LDS stages
+  constexpr int SEGA = TMB / 16, SEGB = TNB / 16;   // 1 KB segments per stage
+  constexpr int SEGA_W = SEGA / 4, SEGB_W = SEGB / 4;   // per wave
+  static_assert(SEGA_W * 4 == SEGA && SEGB_W * 4 == SEGB, "tile/wave mismatch");
+  __shared__ __attribute__((aligned(16))) int8_t sA[NST][TMB * 64];
+  __shared__ __attribute__((aligned(16))) int8_t sB[NST][TNB * 64];
+
+  const int ntm = mpad / TMB, ntn = npad / TNB;
+  const int tiles = ntm * ntn;
+  const int z = blockIdx.x / tiles;
+  const int local = vcphip_xcd_remap(blockIdx.x - z * tiles, tiles);
+  int tm, tn;
+  vcphip_tile_coords(local, ntm, ntn, 8, tm, tn);
+  const int row0 = tm * TMB, col0 = tn * TNB;
+  int npass = 1, pa0 = z, pb0 = z, pa1 = z, pb1 = z;
+  if (z >= nmod) {
+    const int r = z - nmod, base = nmod + 2 * (r >> 1);
+    pa0 = pb0 = base;
+    if (r & 1) {
+      npass = 2;
+      pb0 = base + 1;
+      pa1 = base + 1;
+      pb1 = base;
+    }
+  }
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w & 1, wn = w >> 1;
+  const int wu = __builtin_amdgcn_readfirstlane(w);
+  const uint32_t sA_base = lds_addr(&sA[0][0]), sB_base = lds_addr(&sB[0][0]);
+  // this lane's source in every 1 KB segment: row lane/4, logical chunk
+  const int lrow = lane >> 2, lchunk = (lane & 3) ^ ((lane >> 4) & 3);
+  const int src_lane = lrow * kpad + 16 * lchunk;
+  // fragment reads: row R, logical chunk g = lane / 16 -> physical g ^ ((R >> 2) & 3)
+  const int frow = lane & 15, g = lane >> 4;
+  int fa_off[FM], fb_off[FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+    const int R = wm * WM + 16 * i + frow;
+    fa_off[i] = R * 64 + 16 * (g ^ ((R >> 2) & 3));
+  }
+#pragma unroll
+  for (int j = 0; j < FN; ++j) {
+    const int R = wn * WN + 16 * j + frow;
+    fb_off[j] = R * 64 + 16 * (g ^ ((R >> 2) & 3));
+  }
+
+  v4i acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = v4i{0, 0, 0, 0};
+
+  const int nk = kpad / 64;
+  for (int pass = 0; pass < npass; ++pass) {
+    const int8_t* Ab = Aplanes + (long)(pass ? pa1 : pa0) * mpad * kpad + (long)row0 * kpad + src_lane;
+    const int8_t* Bb = Bplanes + (long)(pass ? pb1 : pb0) * npad * kpad + (long)col0 * kpad + src_lane;
+    // The LDS-DMA loads, their wait and the barrier are inline asm, so the
+    // compiler neither tracks the DMA (it would wait for all of it before
+    // every LDS read) nor fences the barrier; the "memory" clobbers keep the
+    // LDS reads between them.
+    auto issue = [&](int stage, int kt) {
+      const long k0 = (long)kt * 64;
+#pragma unroll
+      for (int q = 0; q < SEGA_W; ++q) {
+        const int seg = wu * SEGA_W + q;
+        lds_dma16(Ab + (long)seg * 16 * kpad + k0, sA_base + stage * (TMB * 64) + seg * 1024);
+      }
+#pragma unroll
+      for (int q = 0; q < SEGB_W; ++q) {
+        const int seg = wu * SEGB_W + q;
+        lds_dma16(Bb + (long)seg * 16 * kpad + k0, sB_base + stage * (TNB * 64) + seg * 1024);
+      }
+    };
+    issue(0, 0);
+    if (nk > 1) issue(1, 1);
+    for (int kt = 0; kt < nk; ++kt) {
+      // stage kt's loads are complete once only stage kt+1's remain in flight
+      if (kt + 1 < nk) asm volatile("s_waitcnt vmcnt(%0)" ::"n"(SEGA_W + SEGB_W) : "memory");
+      else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+      if (kt + 2 < nk) issue((kt + 2) % NST, kt + 2);
+      const int8_t* la = sA[kt % NST];
+      const int8_t* lb = sB[kt % NST];
+      v4i fa[FM];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) fa[i] = *reinterpret_cast<const v4i*>(la + fa_off[i]);
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const v4i fb = *reinterpret_cast<const v4i*>(lb + fb_off[j]);
+#pragma unroll
+        for (int i = 0; i < FM; ++i) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb, acc[i][j], 0, 0, 0);
+      }
+    }
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  }
+
+  const long blocks_m = mpad / 16;
+  if (z < nmod) {
+    const int md = P.mod[z];
+    const double rc = P.rcpd[z];
+    uint8_t* Uz = U + (long)z * mpad * npad;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const long bi = (row0 + wm * WM + 16 * i) / 16, bj = (col0 + wn * WN + 16 * j) / 16;
+        unsigned packed = 0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const double y = (double)acc[i][j][v];
+          const double qd = __builtin_floor(y * rc);
+          int rr = (int)(y - qd * (double)md);
+          rr += (rr < 0) ? md : 0;
+          rr -= (rr >= md) ? md : 0;
+          packed |= (unsigned)rr << (8 * v);
+        }
+        *reinterpret_cast<unsigned*>(Uz + (bj * blocks_m + bi) * 256 + lane * 4) = packed;
+      }
+  } else {
+    int32_t* Zz = Z + (long)(z - nmod) * mpad * npad;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const long bi = (row0 + wm * WM + 16 * i) / 16, bj = (col0 + wn * WN + 16 * j) / 16;
+        *reinterpret_cast<v4i*>(Zz + (bj * blocks_m + bi) * 256 + lane * 4) = acc[i][j];
+      }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // reconstruction
 // ---------------------------------------------------------------------------
@@ -546,6 +752,10 @@ __global__ __launch_bounds__(256) void k_oz_recon(int m, int n, int k, int mpad,
 int vcphip_oz_stats(const double* X, long rs, long cs, int rows, int len, VcphipOzSide side,
                     hipStream_t s) {
   if (rows <= 0) return 0;
+  if (cs == 1) {   // columns of a column-major operand: one wave per column
+    hipLaunchKernelGGL(k_oz_stats_cols, dim3((rows + 3) / 4), dim3(256), 0, s, X, rs, rows, len, side);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+  }
   const int nchunk = len > 0 ? (len + VCPHIP_OZ_CHUNK - 1) / VCPHIP_OZ_CHUNK : 0;
   if (nchunk > 0) {
     dim3 grid((rows + 255) / 256, nchunk);
@@ -590,6 +800,19 @@ int vcphip_oz_gemm(int mpad, int npad, int kpad, int nmod, int nrad, const int8_
     case 3: VCPHIP_GEMM_LAUNCH(4, 8, 128); break;
     case 4: VCPHIP_GEMM_LAUNCH(8, 8, 64); break;
     case 5: VCPHIP_GEMM_LAUNCH(4, 4, 64); break;
+    case 6:
+    case 7: {
+      const int fm = variant == 6 ? 8 : 4, fn = variant == 6 ? 8 : 8;
+      if (kpad % 64) return -7;
+      const int tiles = (mpad / (32 * fm)) * (npad / (32 * fn));
+      if (variant == 6)
+        hipLaunchKernelGGL((k_oz_gemm3<8, 8>), dim3(tiles * nz), dim3(256), 0, s, mpad, npad, kpad, nmod, nz, Aplanes,
+                           Bplanes, P, U, Z);
+      else
+        hipLaunchKernelGGL((k_oz_gemm3<4, 8>), dim3(tiles * nz), dim3(256), 0, s, mpad, npad, kpad, nmod, nz, Aplanes,
+                           Bplanes, P, U, Z);
+      break;
+    }
     default: return -7;
   }
 #undef VCPHIP_GEMM_LAUNCH
