@@ -40,6 +40,14 @@ def main():
     ap.add_argument("--samples", type=int, default=512)
     ap.add_argument("--no-resid", action="store_true")
     ap.add_argument("--resid-reps", type=int, default=120)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a 1 GiB elementwise copy (reads 1 GiB, writes 1 GiB): the calibration "
+                         "kernel for FETCH_SIZE / WRITE_SIZE counter runs")
+    ap.add_argument("--no-infsup", action="store_true")
+    ap.add_argument("--no-tuned", action="store_true")
+    ap.add_argument("--no-dd", action="store_true")
+    ap.add_argument("--dd-n", type=int, default=4096)
+    ap.add_argument("--dd-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this path runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -68,6 +76,14 @@ def main():
     dC = torch.empty((n, n, 2), dtype=torch.float64, device=dev)
     dP = torch.empty((n, n, 2), dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
+
+    if args.copy_probe:
+        src = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        log("copy probe: 1 GiB read + 1 GiB written by one elementwise kernel")
+        del src, dst
 
     ctx = vcp_amd.Context(0)
     beta, mods = vcp_amd.ozaki_config(n)
@@ -109,6 +125,25 @@ def main():
     tf_oz = flops / t_oz / 1e12
     tf_pl = flops / t_pl / 1e12
 
+    # ---- inf-sup form (pidblas_fma::mulmm), one timed product --------------
+    infsup = None
+    if not args.no_infsup:
+        dI = torch.empty_like(dC)
+        step(vcp_amd.FORM_INFSUP, dI)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        step(vcp_amd.FORM_INFSUP, dI)
+        ctx.synchronize()
+        t_is = time.perf_counter() - t0
+        w_is = (dI[..., 1] - dI[..., 0])
+        w_pl = torch.clamp(dP[..., 1] - dP[..., 0], min=torch.finfo(torch.float64).tiny)
+        r_is = w_is / w_pl
+        infsup = {"ms_per_step": round(t_is * 1e3, 3), "tflops_8n3": round(flops / t_is / 1e12, 3),
+                  "fp64_ops_per_step": 28.0 * n * n * n,
+                  "radius_ratio_over_plain": {"max": float(r_is.max()), "median": float(r_is.median())}}
+        log("infsup %s" % infsup)
+        del dI, w_is, w_pl, r_is
+
     # ---- radius ratio (all entries, on the GPU) and sampled containment ----
     rad_o = dC[..., 1] - dC[..., 0]
     rad_p = dP[..., 1] - dP[..., 0]
@@ -141,7 +176,7 @@ def main():
         ops = 2.0 * n * n * n * (nmod + 6)
         ach = ops / (t_gemm * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
-                "frac": round(ach / PEAK_INT8_TOPS, 4), "traffic": None,
+                "frac": round(ach / PEAK_INT8_TOPS, 4), "traffic": measured_traffic(n, nmod),
                 "kernel": "k_oz_gemm (int8 MFMA: %d residue + 6 radius-digit plane products)" % nmod,
                 "ops_per_launch": ops, "ms_per_launch": round(t_gemm, 4)}
 
@@ -169,7 +204,45 @@ def main():
         except Exception as e:
             log("cpu baseline skipped: %r" % (e,))
 
+    # ---- autotuned configuration (tile variant + moduli count for ratio <= 2)
+    tuned = None
+    if not args.no_tuned:
+        tctx = vcp_amd.Context(0)
+        nm, ratio_s = tctx.tune_igemm_moduli_dev(n, n, n, dA.data_ptr(), n, dB.data_ptr(), n, 2.0, 256)
+        v, v_ms = tctx.tune_igemm_tile(n, n, n)    # tiles timed with the tuned plane count
+        dT = torch.empty_like(dC)
+        tctx.igemm_dev(vcp_amd.FORM_OZAKI, n, n, n, dA.data_ptr(), n, dB.data_ptr(), n, dT.data_ptr(), n)
+        tctx.synchronize()
+        tctx.set_param(vcp_amd.PARAM_STAGE_TIMING, 1)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            tctx.igemm_dev(vcp_amd.FORM_OZAKI, n, n, n, dA.data_ptr(), n, dB.data_ptr(), n, dT.data_ptr(), n)
+        tctx.synchronize()
+        t_tu = (time.perf_counter() - t0) / args.steps
+        st_tu = tctx.stage_times()
+        w_t = dT[..., 1] - dT[..., 0]
+        r_t = w_t / torch.clamp(dP[..., 1] - dP[..., 0], min=torch.finfo(torch.float64).tiny)
+        tuned = {"moduli": nm, "sampled_ratio": ratio_s, "tile_variant": v, "ms_per_step": round(t_tu * 1e3, 3),
+                 "tflops": round(flops / t_tu / 1e12, 3), "speedup_vs_plain": round(t_pl / t_tu, 3),
+                 "stages_ms": {k: round(x, 4) for k, x in st_tu.items()},
+                 "radius_ratio_over_plain": {"max": float(r_t.max()), "median": float(r_t.median())}}
+        try:
+            import oracle_ffi as O
+
+            Ct = dT.cpu().numpy().transpose(1, 0, 2)
+            rng = np.random.default_rng(4242)
+            ii = rng.integers(0, n, args.samples)
+            jj = rng.integers(0, n, args.samples)
+            ok_t, _, _ = O.check_containment(IA, IB, Ct, ii, jj)
+            tuned["containment"] = float(ok_t.mean())
+        except Exception as e:
+            log("tuned containment skipped: %r" % (e,))
+        log("tuned %s" % tuned)
+        del dT, w_t, r_t
+        tctx.close()
+
     resid = None if args.no_resid else bench_resid(ctx, args.resid_reps)
+    dd = None if args.no_dd else bench_dd(ctx, args.dd_n, args.dd_steps, not args.no_cpu)
 
     line = {
         "metric": "verified interval GEMM FP64-equiv TFLOP/s n=8192: Ozaki-int8 vs FP64-RU/RD",
@@ -190,14 +263,32 @@ def main():
                             "frac_fp64_peak": round(tf_pl / PEAK_FP64_TFLOPS, 4)},
         "speedup_vs_plain": round(t_pl / t_oz, 3),
         "radius_ratio_ozaki_over_plain": {"max": ratio_max, "median": ratio_med},
+        "infsup_fp64": infsup,
+        "tuned": tuned,
         "containment": containment,
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "roofline": roof,
         "cpu_baseline": cpu,
         "batched_residual": resid,
+        "dd_gemm": dd,
     }
     print(json.dumps(line), flush=True)
     ctx.close()
+
+
+def measured_traffic(n, nmod):
+    """HBM bytes per launch of the int8 GEMM stage, from the committed
+    FETCH_SIZE / WRITE_SIZE counter runs (profiles/ozaki_gemm_traffic.json),
+    when they were taken on this workload; otherwise None."""
+    path = os.path.join(ROOT, "profiles", "ozaki_gemm_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if rec.get("n") != n or rec.get("moduli") != nmod:
+        return None
+    return rec.get("bytes_per_launch")
 
 
 def bench_resid(ctx, reps):
@@ -248,6 +339,83 @@ def bench_resid(ctx, reps):
     except Exception as e:
         log("residual oracle check skipped: %r" % (e,))
     log("batched residual %s" % out)
+    return out
+
+
+def bench_dd(ctx, n, steps, with_cpu):
+    """Double-double product C = A * B (pddblas::mulmm, vcp/pddblas.hpp:318)
+    at n x n x n: hi U[-1,1], lo = hi 2^-53 U[-1,1].  Reports time, the error
+    against the exact product on sampled entries (units of 2^-106 |C_ij|), and
+    the same error of the pddblas restatement on the same entries."""
+    import numpy as np
+    import torch
+
+    import vcp_amd
+    from vcp_amd import workloads as W
+
+    log("generating the n=%d dd workload" % n)
+    if n == 4096:
+        A, B = W.config4(n)
+    else:
+        rng = np.random.default_rng(W.SEED0 + 4)
+        A, B = W.dd_matrix(rng, n, n), W.dd_matrix(rng, n, n)
+    dev = torch.device("cuda:0")
+    dA = torch.from_numpy(np.ascontiguousarray(A.transpose(1, 0, 2))).to(dev)
+    dB = torch.from_numpy(np.ascontiguousarray(B.transpose(1, 0, 2))).to(dev)
+    dC = torch.empty((n, n, 2), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    def step():
+        ctx.ddgemm_dev(n, n, n, dA.data_ptr(), n, dB.data_ptr(), n, dC.data_ptr(), n)
+
+    step()
+    ctx.synchronize()
+    ctx.set_param(vcp_amd.PARAM_STAGE_TIMING, 1)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    ctx.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    stages = ctx.stage_times()
+    ctx.set_param(vcp_amd.PARAM_STAGE_TIMING, 0)
+    nmod = ctx.get_param(vcp_amd.PARAM_DD_NMOD) or None
+    out = {"n": n, "ms_per_step": round(t * 1e3, 3), "dd_tflops": round(2.0 * n ** 3 / t / 1e12, 3),
+           "stages_ms": {k: round(v, 4) for k, v in stages.items()},
+           "moduli": nmod if nmod else "default: fewest giving beta >= 110"}
+    log("dd gemm %s" % out)
+    try:
+        import oracle_ffi as O
+
+        C = dC.cpu().numpy().transpose(1, 0, 2)
+        rows = 4
+        rng = np.random.default_rng(777)
+        jj = rng.integers(0, n, 64)
+        err_gpu, err_ref = [], []
+        cpu_t = None
+        if with_cpu:
+            t0 = time.perf_counter()
+            Cr, _, _ = O.pddblas_mulmm(np.ascontiguousarray(A[:rows].transpose(1, 0, 2)).transpose(1, 0, 2), B)
+            cpu_t = time.perf_counter() - t0
+        for i in range(rows):
+            for j in jj:
+                h, _, e = O.exact_dd_dot(A[i], B[:, j], C[i, j, 0], C[i, j, 1])
+                scale = abs(h) * 2.0 ** -106
+                err_gpu.append(e / scale)
+                if with_cpu:
+                    _, _, e2 = O.exact_dd_dot(A[i], B[:, j], Cr[i, j, 0], Cr[i, j, 1])
+                    err_ref.append(e2 / scale)
+        out["err_vs_exact_2^-106_rel"] = {"samples": len(err_gpu), "max": float(np.max(err_gpu)),
+                                          "median": float(np.median(err_gpu))}
+        if with_cpu:
+            out["pddblas_restated_err_2^-106_rel"] = {"max": float(np.max(err_ref)),
+                                                      "median": float(np.median(err_ref))}
+            out["cpu_pddblas_restated"] = {"dd_tflops": round(2.0 * rows * n * n / cpu_t / 1e12, 6),
+                                           "threads": int(O.lib().orc_get_threads()),
+                                           "sample": "first %d rows (%d x %d x %d), %.1f s" % (rows, rows, n, n, cpu_t)}
+        log("dd accuracy %s" % {k: v for k, v in out.items() if "err" in k or "cpu" in k})
+    except Exception as e:
+        log("dd accuracy check skipped: %r" % (e,))
+    del dA, dB, dC
     return out
 
 

@@ -11,11 +11,15 @@
 // pddhip declares its own, which hide them for vcp::matrix<kv::dd, pddhip>
 // (name lookup in matrix's operator*, vcp/matrix.hpp:245-267).  Everything
 // else (linearsolve, Cholesky, eigsym) is inherited from pddblas and stays on
-// the CPU.  Scalar operands keep the reference's host semantics.
+// the CPU.  Scalar operands and products below the GPU / host crossover
+// (vcphip_policy::crossover, measured by tune_crossover()) keep the
+// reference's host semantics.
 #pragma once
 
 #ifndef VCP_PDDHIP_HPP
 #define VCP_PDDHIP_HPP
+
+#include <random>
 
 #include <vcp/error.hpp>
 #include <vcp/pddblas.hpp>
@@ -42,9 +46,39 @@ class pddhip : public vcp::pddblas {
   typedef pddhip_detail::Errors E;
 
  public:
+  // Measures where the GPU overtakes the host (pddblas's Ozaki scheme) for
+  // square dd products n = 8 ... max_n, installs that crossover and returns
+  // it (an m*n*k value).
+  static long long tune_crossover(int max_n = 256, int reps = 3) {
+    pddhip A, B, C;
+    std::mt19937_64 rng(0x5EED0000ULL);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    auto fill = [&](pddhip& X) {
+      for (size_t i = 0; i < X.v.size(); ++i) X.v[i] = kv::dd(U(rng));
+    };
+    auto setup = [&](int n) {
+      A.zeros(n, n);
+      B.zeros(n, n);
+      C.zeros(n, n);
+      fill(A);
+      fill(B);
+    };
+    auto host = [&]() { A.vcp::pddblas::mulmm(B, C); };
+    auto gpu = [&]() {
+      const int n = A.row;
+      vcphip_policy::check<E>(vcphip_ddgemm(vcphip_policy::checked_context<E>(), n, n, n, pddhip_detail::raw(A.v), n,
+                                            pddhip_detail::raw(B.v), n, pddhip_detail::raw(C.v), n),
+                              "vcphip_ddgemm");
+    };
+    const long long x = vcphip_policy::measure_crossover(setup, host, gpu, max_n, reps);
+    vcphip_policy::crossover(vcphip_policy::CROSS_DD) = x;
+    return x;
+  }
+
   // C = A*B (pddblas.hpp:318-376)
   void mulmm(const pddhip& B, pddhip& c) const {
-    if (vcphip_policy::is_scalar_case(this->type, B.type)) {
+    if (vcphip_policy::is_scalar_case(this->type, B.type) ||
+        vcphip_policy::on_host(vcphip_policy::CROSS_DD, this->row, B.column, this->column)) {
       vcp::pddblas::mulmm(B, c);
       return;
     }
@@ -58,7 +92,7 @@ class pddhip : public vcp::pddblas {
   }
   // C = A^T A (pddblas.hpp:379-409), exactly symmetric
   void mulltmm(pddhip& c) const {
-    if (this->type == 'S') {
+    if (this->type == 'S' || vcphip_policy::on_host(vcphip_policy::CROSS_DD, this->column, this->column, this->row)) {
       vcp::pddblas::mulltmm(c);
       return;
     }

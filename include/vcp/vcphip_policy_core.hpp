@@ -6,10 +6,12 @@
 // Header-only, C++11.  Links against libvcphip (include/vcphip.h).
 #pragma once
 
+#include <chrono>
 #include <cstdlib>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../vcphip.h"
 
@@ -72,6 +74,65 @@ inline vcphip_ctx* context(int* status = nullptr) {
 inline int& default_form() {
   static int form = VCPHIP_FORM_OZAKI;
   return form;
+}
+
+// ---- GPU / host crossover ----------------------------------------------------
+// Products with m*n*k below the crossover of their kind run on the host
+// through the base policy (the reference's host path): there the staging
+// over PCIe and the launch latency exceed the work.  The environment
+// variables VCPHIP_CROSSOVER (pidblas products), VCPHIP_CROSSOVER_FMA
+// (pidblas_fma's inf-sup product) and VCPHIP_CROSSOVER_DD (dd products) set
+// it as an m*n*k value; 0 sends everything to the GPU.  The policies'
+// tune_crossover() measures it with measure_crossover() below.
+enum CrossoverKind { CROSS_INTERVAL = 0, CROSS_DD = 1, CROSS_INFSUP = 2 };
+
+inline long long env_crossover(const char* name, long long dflt) {
+  const char* e = std::getenv(name);
+  return e ? std::atoll(e) : dflt;
+}
+
+inline long long& crossover(int kind) {
+  static long long v[3] = {env_crossover("VCPHIP_CROSSOVER", 64LL * 64 * 64),
+                           env_crossover("VCPHIP_CROSSOVER_DD", 32LL * 32 * 32),
+                           env_crossover("VCPHIP_CROSSOVER_FMA", 16LL * 16 * 16)};
+  return v[(kind >= 0 && kind <= 2) ? kind : 0];
+}
+
+inline bool on_host(int kind, long long m, long long n, long long k) { return m * n * k < crossover(kind); }
+
+// best wall time of `reps` runs of f()
+template <class F>
+inline double best_time(F& f, int reps) {
+  typedef std::chrono::steady_clock clk;
+  double best = 0.0;
+  for (int r = 0; r < reps; ++r) {
+    const clk::time_point t0 = clk::now();
+    f();
+    const double t = std::chrono::duration<double>(clk::now() - t0).count();
+    if (r == 0 || t < best) best = t;
+  }
+  return best;
+}
+
+// For n = 8, 16, ..., max_n: setup(n) prepares an n x n x n product, then
+// host() and gpu() each run it `reps` times (best time kept).  Returns the
+// smallest n^3 from which the GPU is faster at that and every larger n tried,
+// or max_n^3 + 1 when it is not faster at max_n.
+template <class Setup, class Host, class Gpu>
+long long measure_crossover(Setup setup, Host host, Gpu gpu, int max_n = 512, int reps = 3) {
+  std::vector<int> ns;
+  std::vector<bool> gpu_wins;
+  for (int n = 8; n <= max_n; n *= 2) {
+    setup(n);
+    const double th = best_time(host, reps), tg = best_time(gpu, reps);
+    ns.push_back(n);
+    gpu_wins.push_back(tg < th);
+  }
+  size_t first = ns.size();
+  while (first > 0 && gpu_wins[first - 1]) --first;
+  if (first == ns.size()) return (long long)max_n * max_n * max_n + 1;
+  const long long n0 = ns[first];
+  return n0 * n0 * n0;
 }
 
 // ---- status -> exception ----------------------------------------------------

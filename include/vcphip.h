@@ -47,8 +47,11 @@ enum {
 
 /* forms of the interval x interval product */
 enum {
-    VCPHIP_FORM_PLAIN = 0, /* FP64 directed-rounding chains (pidblas op order) */
-    VCPHIP_FORM_OZAKI = 1  /* Ozaki scheme II on int8 MFMA + rigorous bounds   */
+    VCPHIP_FORM_PLAIN = 0,  /* FP64 directed-rounding chains (pidblas op order) */
+    VCPHIP_FORM_OZAKI = 1,  /* Ozaki scheme II on int8 MFMA + rigorous bounds   */
+    VCPHIP_FORM_INFSUP = 2  /* inf-sup chains of interval FMAs: the product of
+                               vcp::pidblas_fma::mulmm (vcp/pidblas_fma.hpp:45-72,
+                               vcp/ifma.hpp:44-137); tighter, about 3x the work */
 };
 
 /* context parameters */
@@ -82,7 +85,8 @@ int vcphip_ozaki_config(int k, int nmod, int* beta, int* moduli, int max_moduli)
 
 /* ---- interval x interval: C = IA * IB ----------------------------------- */
 /* Replaces vcp::pidblas::mulmm (vcp/pidblas.hpp:239-294), i.e. the four
- * dgemm_ calls at :266,271,279,281 under kv::hwround and the n^2 fix-ups. */
+ * dgemm_ calls at :266,271,279,281 under kv::hwround and the n^2 fix-ups
+ * (forms PLAIN and OZAKI), and vcp::pidblas_fma::mulmm (form INFSUP). */
 int vcphip_igemm(vcphip_ctx* ctx, int form, int m, int n, int k,
                  const double* IA, int lda, const double* IB, int ldb, double* C, int ldc);
 int vcphip_igemm_dev(vcphip_ctx* ctx, int form, int m, int n, int k,
@@ -150,6 +154,40 @@ int vcphip_residual_batched_dev(vcphip_ctx* ctx, int count, const int* n,
                                 const double* const* R, const int* ldr,
                                 const double* const* IA, const int* lda,
                                 double* const* C, const int* ldc, double* normub_dev);
+
+/* ---- autotuner ------------------------------------------------------------
+ * No counterpart in the reference (its BLAS path has nothing to tune).  The
+ * choices live in the context and are keyed by product kind and size class
+ * (ceil(log2) of m, n and k); a product of a tuned class uses them unless
+ * VCPHIP_PARAM_OZ_VARIANT / VCPHIP_PARAM_OZ_NMOD are set.  No choice changes
+ * the containment guarantee: tile variants give bit-identical results, and
+ * every moduli count yields a rigorous enclosure. */
+
+/* Times every int8 MFMA tile variant of the interval Ozaki product's GEMM
+ * stage at (m, n, k) and records the fastest; *variant and *ms (per launch)
+ * receive the choice. */
+int vcphip_tune_igemm_tile(vcphip_ctx* ctx, int m, int n, int k, int* variant, double* ms);
+/* The same for the double-double product (vcphip_ddgemm). */
+int vcphip_tune_ddgemm_tile(vcphip_ctx* ctx, int m, int n, int k, int* variant, double* ms);
+/* Slice count of the interval Ozaki form for these operands (device
+ * pointers): the smallest number of moduli in [12, 20] whose radii, on an
+ * evenly spaced sample of sample_rows rows of IA (0: 256) against all of IB,
+ * are at most max_ratio times the plain form's; 20 if none is.  *nmod and
+ * *ratio (the sampled maximum radius ratio) receive the choice. */
+int vcphip_tune_igemm_moduli_dev(vcphip_ctx* ctx, int m, int n, int k, const double* IA, int lda,
+                                 const double* IB, int ldb, double max_ratio, int sample_rows,
+                                 int* nmod, double* ratio);
+/* Times 1-4 persistent workgroups per CU for the batched residual of this
+ * problem set (arguments as vcphip_residual_batched_dev, which it also runs)
+ * and keeps the fastest; the next call captures its graph with it. */
+int vcphip_tune_residual_dev(vcphip_ctx* ctx, int count, const int* n, const double* const* R,
+                             const int* ldr, const double* const* IA, const int* lda,
+                             double* const* C, const int* ldc, double* normub_dev, int* per_cu,
+                             double* ms);
+/* Writes / reads the context's tuning table (text: a "vcphip-tune 1" line,
+ * then "igemm|ddgemm lm ln lk variant nmod" and "residual per_cu" lines). */
+int vcphip_tune_save(vcphip_ctx* ctx, const char* path);
+int vcphip_tune_load(vcphip_ctx* ctx, const char* path);
 
 #ifdef __cplusplus
 }

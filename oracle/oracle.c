@@ -275,6 +275,67 @@ int orc_pidblas_vmulmm(int k, int n, const double* A, int lda, double* C, int ld
     return 0;
 }
 
+/* vcp::ifma, the branch without AVX-512 (vcp/ifma.hpp:58-137): the interval
+   FMA [cd, cu] + [ad, au] * [bd, bu], each end one directed FMA of the
+   endpoint product the signs select.  An operand is classified as
+   nonnegative (lower >= 0), straddling (lower <= 0 <= upper) or nonpositive,
+   tested in that order (ifma.hpp:67, :86, :111). */
+enum { ORC_POS = 0, ORC_MIX = 1, ORC_NEG = 2 };
+static int sign_class(double lo, double hi) { return lo >= 0 ? ORC_POS : (hi >= 0 ? ORC_MIX : ORC_NEG); }
+
+static void ifma_restated(double au, double ad, double bu, double bd, double cu, double cd, double* up, double* dn) {
+    const int ca = sign_class(ad, au), cb = sign_class(bd, bu);
+    volatile double x, y;
+    if (ca == ORC_MIX && cb == ORC_MIX) {
+        fesetround(FE_UPWARD);     /* larger of au*bu and ad*bd (:97-100) */
+        x = fma(au, bu, cu);
+        y = fma(ad, bd, cu);
+        *up = (y < x) ? x : y;
+        fesetround(FE_DOWNWARD);   /* smaller of au*bd and ad*bu (:102-105) */
+        x = fma(au, bd, cd);
+        y = fma(ad, bu, cd);
+        *dn = (y > x) ? x : y;
+        fesetround(FE_TONEAREST);
+        return;
+    }
+    /* endpoint pairs {a, b} for the upper and the lower end, per sign class
+       (rows: a; columns: b; 0 = lower end, 1 = upper end) */
+    static const int up_a[3][3] = {{1, 1, 0}, {1, -1, 0}, {1, 0, 0}};
+    static const int up_b[3][3] = {{1, 1, 1}, {1, -1, 0}, {0, 0, 0}};
+    static const int dn_a[3][3] = {{0, 1, 1}, {0, -1, 1}, {0, 0, 1}};
+    static const int dn_b[3][3] = {{0, 0, 0}, {1, -1, 0}, {1, 1, 1}};
+    const double a_end[2] = {ad, au}, b_end[2] = {bd, bu};
+    fesetround(FE_UPWARD);
+    x = fma(a_end[up_a[ca][cb]], b_end[up_b[ca][cb]], cu);
+    fesetround(FE_DOWNWARD);
+    y = fma(a_end[dn_a[ca][cb]], b_end[dn_b[ca][cb]], cd);
+    fesetround(FE_TONEAREST);
+    *up = x;
+    *dn = y;
+}
+
+/* pidblas_fma::mulmm, vcp/pidblas_fma.hpp:45-72: C = IA * IB in inf-sup
+   form, every element the chain of interval FMAs over the inner index in
+   ascending order from [0, 0] (:62-71) */
+int orc_pidblas_fma_mulmm(int m, int n, int k, const double* A, int lda, const double* B, int ldb,
+                          double* C, int ldc) {
+    if (m < 0 || n < 0 || k < 0) return -1;
+#pragma omp parallel for schedule(dynamic) num_threads(orc_get_threads())
+    for (int col = 0; col < n; ++col) {
+        for (int i = 0; i < m; ++i) {
+            double cu = 0.0, cd = 0.0;
+            for (int j = 0; j < k; ++j) {
+                const double* a = A + 2 * (i + (size_t)lda * j);
+                const double* b = B + 2 * (j + (size_t)ldb * col);
+                ifma_restated(a[1], a[0], b[1], b[0], cu, cd, &cu, &cd);
+            }
+            C[2 * (i + (size_t)ldc * col)] = cd;
+            C[2 * (i + (size_t)ldc * col) + 1] = cu;
+        }
+    }
+    return 0;
+}
+
 /* imats::linearsolve R*A - I segment (vcp/imats.hpp:318-330) and
  * mats::norminf for type 'M' (vcp/mats.hpp:1583-1600) on intervals:
  * |[l,u]| = [mig, mag] and the row sum is an interval sum, so the upper

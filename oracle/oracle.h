@@ -52,6 +52,9 @@ int orc_pidblas_vmulmm(int k, int n, const double* A, int lda, double* C, int ld
 
 /* ---- R*A - I and ||.||_inf (imats.hpp:319-336, mats.hpp:1565-1604) ------- */
 /* C = R*IA - I (interval, n x n) and the upper bound of ||C||_inf. */
+/* pidblas_fma::mulmm (inf-sup interval FMA chains), vcp/pidblas_fma.hpp:45-72 */
+int orc_pidblas_fma_mulmm(int m, int n, int k, const double* A, int lda, const double* B, int ldb,
+                          double* C, int ldc);
 int orc_residual(int n, const double* R, int ldr, const double* IA, int lda, double* C, int ldc,
                  double* normub);
 

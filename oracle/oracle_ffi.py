@@ -42,6 +42,7 @@ def lib():
             "orc_rsyrk": [_c_int, _c_int, _p_dbl, _c_int, _p_dbl, _c_int, _c_int],
             "orc_midrad": [_c_dbl, _c_dbl, _p_dbl, _p_dbl],
             "orc_pidblas_mulmm": [_c_int, _c_int, _c_int, _p_dbl, _c_int, _p_dbl, _c_int, _p_dbl, _c_int],
+            "orc_pidblas_fma_mulmm": [_c_int, _c_int, _c_int, _p_dbl, _c_int, _p_dbl, _c_int, _p_dbl, _c_int],
             "orc_pidblas_mul_im_m": [_c_int, _c_int, _c_int, _p_dbl, _c_int, _p_dbl, _c_int, _p_dbl, _c_int],
             "orc_pidblas_mul_m_im": [_c_int, _c_int, _c_int, _p_dbl, _c_int, _p_dbl, _c_int, _p_dbl, _c_int],
             "orc_pidblas_mulltmm": [_c_int, _c_int, _p_dbl, _c_int, _p_dbl, _c_int],
@@ -135,6 +136,17 @@ def pidblas_mulmm(IA, IB):
     C = pairs(m, n)
     lib().orc_pidblas_mulmm(m, n, k, _dp(IA.transpose(1, 0, 2)), _ld_pair(IA), _dp(IB.transpose(1, 0, 2)),
                             _ld_pair(IB), _dp(C.transpose(1, 0, 2)), _ld_pair(C))
+    return C
+
+
+def pidblas_fma_mulmm(IA, IB):
+    """pidblas_fma::mulmm: inf-sup interval FMA chains (vcp/pidblas_fma.hpp:45-72)"""
+    m, k = IA.shape[:2]
+    n = IB.shape[1]
+    C = pairs(m, n)
+    rc = lib().orc_pidblas_fma_mulmm(m, n, k, _dp(IA.transpose(1, 0, 2)), _ld_pair(IA), _dp(IB.transpose(1, 0, 2)),
+                                     _ld_pair(IB), _dp(C.transpose(1, 0, 2)), _ld_pair(C))
+    assert rc == 0
     return C
 
 

@@ -4,6 +4,7 @@
 // checked bit for bit against the CPU oracle.
 //   ./test_policy_core          logic only (no GPU)
 //   ./test_policy_core gpu      also the device part
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -98,8 +99,36 @@ void test_gpu() {
 }
 }  // namespace
 
+// busy-wait for `ns` nanoseconds (a stand-in for work of known cost)
+void spin_ns(double ns) {
+  const auto t0 = std::chrono::steady_clock::now();
+  while (std::chrono::duration<double, std::nano>(std::chrono::steady_clock::now() - t0).count() < ns) {
+  }
+}
+
+void test_crossover() {
+  using namespace vcphip_policy;
+  int cur = 0;
+  auto setup = [&cur](int n) { cur = n; };
+  // host work n^3 ns, GPU a constant 150 us: the GPU first wins at n = 64
+  auto host = [&cur]() { spin_ns((double)cur * cur * cur); };
+  auto gpu_const = []() { spin_ns(150e3); };
+  EXPECT(measure_crossover(setup, host, gpu_const, 128, 3) == 64LL * 64 * 64);
+  // a GPU that is always twice as slow never wins
+  auto gpu_slow = [&cur]() { spin_ns(2.0 * cur * cur * cur); };
+  EXPECT(measure_crossover(setup, host, gpu_slow, 64, 2) == 64LL * 64 * 64 + 1);
+  const long long saved = crossover(CROSS_INTERVAL);
+  crossover(CROSS_INTERVAL) = 1000;
+  EXPECT(on_host(CROSS_INTERVAL, 9, 10, 10) && !on_host(CROSS_INTERVAL, 10, 10, 10));
+  crossover(CROSS_INTERVAL) = 0;
+  EXPECT(!on_host(CROSS_INTERVAL, 1, 1, 1));
+  crossover(CROSS_INTERVAL) = saved;
+  EXPECT(crossover(CROSS_DD) >= 0);
+}
+
 int main(int argc, char** argv) {
   test_logic();
+  test_crossover();
   if (argc > 1 && std::string(argv[1]) == "gpu") test_gpu();
   if (failures) {
     std::fprintf(stderr, "%d failure(s)\n", failures);

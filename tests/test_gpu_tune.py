@@ -1,0 +1,124 @@
+"""Autotuner: each choice it records (int8 GEMM tile variant, moduli count,
+persistent grid of the batched residual) is used by later calls of the same
+size class, and the results stay bit-identical to the oracle's restatement
+for the chosen configuration (tile variants never change a bit; the moduli
+count is part of the Ozaki form's definition)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle_ffi as O
+import vcp_amd
+from vcp_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def ctx():
+    """a private context: tuned choices must not reach other tests"""
+    c = vcp_amd.Context(0)
+    yield c
+    c.close()
+
+
+def _dev_pairs(X):
+    """column-major AoS pairs on cuda:0 (the (cols, rows, 2) C-contiguous buffer)"""
+    return torch.from_numpy(np.ascontiguousarray(X.transpose(1, 0, 2))).cuda()
+
+
+def test_tune_tile_records_and_keeps_results(ctx):
+    rng = np.random.default_rng(31)
+    IA, IB = W.wide_exponent_intervals(rng, 300, 520), W.wide_exponent_intervals(rng, 520, 280)
+    v, ms = ctx.tune_igemm_tile(300, 280, 520)
+    assert 1 <= v <= 10 and ms > 0.0
+    C = ctx.igemm(IA, IB, vcp_amd.FORM_OZAKI)
+    assert np.array_equal(C, O.ozaki_igemm(IA, IB, 16))
+
+
+def test_tune_dd_tile_keeps_results(ctx):
+    rng = np.random.default_rng(32)
+    A, B = W.dd_matrix(rng, 200, 300), W.dd_matrix(rng, 300, 150)
+    v, ms = ctx.tune_ddgemm_tile(200, 150, 300)
+    assert 1 <= v <= 10 and ms > 0.0
+    C = ctx.ddgemm(A, B)
+    nmod = next(t for t in range(24, 33) if O.ozaki_dd_beta(300, t) >= 110)
+    assert np.array_equal(C, O.ozaki_ddgemm(A, B, nmod))
+
+
+@pytest.mark.parametrize("max_ratio", [2.0, 1.0])
+def test_tune_moduli_meets_ratio_on_the_sample(ctx, max_ratio):
+    rng = np.random.default_rng(33)
+    m, n, k = 384, 256, 1024
+    IA, IB = W.wide_exponent_intervals(rng, m, k), W.wide_exponent_intervals(rng, k, n)
+    dA, dB = _dev_pairs(IA), _dev_pairs(IB)
+    torch.cuda.synchronize()
+    nmod, ratio = ctx.tune_igemm_moduli_dev(m, n, k, dA.data_ptr(), m, dB.data_ptr(), k, max_ratio, 64)
+    assert 12 <= nmod <= 20
+    assert ratio <= max_ratio or nmod == 20
+    # the product of that size class now uses the tuned count, bit-exact to the oracle with it
+    C = ctx.igemm(IA, IB, vcp_amd.FORM_OZAKI)
+    assert np.array_equal(C, O.ozaki_igemm(IA, IB, nmod))
+    P = ctx.igemm(IA, IB, vcp_amd.FORM_PLAIN)
+    ii = np.arange(0, m, m // 64)
+    jj = rng.integers(0, n, ii.size)
+    ok, _, _ = O.check_containment(IA, IB, C, ii, jj)
+    assert ok.all()
+    # the sampled rows are rows floor(r m / 64): their full radius ratio is the reported one
+    rows = (np.arange(64) * m) // 64
+    wo = C[rows, :, 1] - C[rows, :, 0]
+    wp = np.maximum(P[rows, :, 1] - P[rows, :, 0], np.finfo(np.float64).tiny)
+    assert np.max(wo / wp) == pytest.approx(ratio, rel=1e-12)
+
+
+def test_tune_save_load_roundtrip(ctx, tmp_path):
+    rng = np.random.default_rng(34)
+    m, n, k = 200, 160, 700
+    IA, IB = W.wide_exponent_intervals(rng, m, k), W.wide_exponent_intervals(rng, k, n)
+    dA, dB = _dev_pairs(IA), _dev_pairs(IB)
+    torch.cuda.synchronize()
+    nmod, _ = ctx.tune_igemm_moduli_dev(m, n, k, dA.data_ptr(), m, dB.data_ptr(), k, 4.0, 32)
+    ctx.tune_igemm_tile(m, n, k)
+    path = tmp_path / "tune.txt"
+    ctx.tune_save(str(path))
+    text = path.read_text()
+    assert text.startswith("vcphip-tune 1\n") and "igemm" in text and "residual" in text
+    with vcp_amd.Context(0) as c2:
+        c2.tune_load(str(path))
+        C = c2.igemm(IA, IB, vcp_amd.FORM_OZAKI)
+    assert np.array_equal(C, O.ozaki_igemm(IA, IB, nmod))
+
+
+def test_tune_load_rejects_malformed(ctx, tmp_path):
+    bad = tmp_path / "bad.txt"
+    bad.write_text("vcphip-tune 1\nigemm 10 10 10 99 16\n")
+    with pytest.raises(vcp_amd.VcphipError):
+        ctx.tune_load(str(bad))
+    bad.write_text("not a table\n")
+    with pytest.raises(vcp_amd.VcphipError):
+        ctx.tune_load(str(bad))
+
+
+def test_tune_residual_then_replay_bitexact(ctx):
+    rng = np.random.default_rng(35)
+    sizes = [70, 130, 33, 200, 64]
+    probs = [W.residual_problem(rng, s) for s in sizes]
+    dR = [torch.from_numpy(np.ascontiguousarray(R.T)).cuda() for R, _ in probs]
+    dA = [_dev_pairs(IA) for _, IA in probs]
+    dC = [torch.empty((s, s, 2), dtype=torch.float64, device="cuda") for s in sizes]
+    nub = torch.empty(len(sizes), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    args = (sizes, [t.data_ptr() for t in dR], sizes, [t.data_ptr() for t in dA], sizes,
+            [t.data_ptr() for t in dC], sizes, nub.data_ptr())
+    per, ms = vcp_amd.tune_residual_dev(ctx, *args)
+    assert 1 <= per <= 4 and ms > 0.0
+    for _ in range(3):   # first call captures with the tuned grid, then replays
+        for t in dC:
+            t.fill_(np.nan)
+        vcp_amd.residual_batched_dev(ctx, *args)
+        ctx.synchronize()
+        nb = nub.cpu().numpy()
+        for i, (R, IA) in enumerate(probs):
+            Co, nbo = O.residual(R, IA)
+            assert np.array_equal(dC[i].cpu().numpy().transpose(1, 0, 2), Co)
+            assert nb[i] == nbo

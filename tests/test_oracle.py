@@ -231,25 +231,71 @@ def test_ozaki_moduli_and_beta():
             assert k * 2 ** (2 * beta + 5) >= M or beta == 62  # and not wastefully small
 
 
-def test_crt_garner_reconstruction_identity():
-    """the mixed-radix reconstruction the kernels use recovers any integer in
-    (-M/2, M/2) from its residues (pure integer check of the formula)"""
+def test_crt_symmetric_reconstruction_identity():
+    """the reconstruction the kernels use: X = sum_t u_t w_t - q M with
+    w_t = (M/m_t)((M/m_t)^-1 mod m_t) and q = floor(0.5 + sum_t u_t fl(w_t/M))
+    in double precision, recovers every X the splitting admits, including the
+    extremes |X| = (M/2) 2^-1e-6 (interval form) and M/4 (dd form); the
+    byte-column sums of the dot-product formulation give the same S"""
     rng = np.random.default_rng(10)
-    for nmod in (12, 16, 20):
+    for nmod in (8, 12, 16, 20, 24, 28, 32):
         mods = O.ozaki_moduli(nmod)
         M = math.prod(mods)
-        P = [math.prod(mods[:s]) for s in range(nmod)]
-        for _ in range(200):
-            X = int(rng.integers(-2 ** 62, 2 ** 62)) * int(rng.integers(1, 2 ** 40)) % M
-            X = X - M if X > M // 2 else X
+        w = []
+        for m in mods:
+            Mt = M // m
+            w.append(Mt * pow(Mt % m, -1, m))
+        f = [float(F(wt, M)) for wt in w]
+        edge = int(F(M, 2) * F(2) ** F(-1, 10 ** 6)) if nmod <= 20 else M // 4 - 1
+        cases = [0, 1, -1, edge, -edge, edge - 1, -(edge - 1)]
+        cases += [int(x) for x in rng.integers(-2 ** 62, 2 ** 62, 100)]
+        cases += [(int(rng.integers(0, 2 ** 62)) * int(rng.integers(1, 2 ** 62)) * (2 ** 64 + 1)) % (2 * edge + 1) - edge
+                  for _ in range(300)]
+        for X in cases:
+            assert abs(X) <= edge
             u = [X % m for m in mods]
-            v = []
-            for t, m in enumerate(mods):
-                acc = u[t] + sum(v[s] * ((m - P[s] % m) % m) for s in range(t))
-                v.append((acc % m) * pow(P[t] % m, -1, m) % m)
-            Y = sum(v[t] * P[t] for t in range(nmod))
-            Y = Y - M if Y > M // 2 else Y
-            assert Y == X
+            qf = 0.5
+            for ut, ft in zip(u, f):
+                qf = qf + ut * ft
+            q = math.floor(qf)
+            S = sum(ut * wt for ut, wt in zip(u, w))
+            assert S - q * M == X
+            # byte columns: col_c = sum_t u_t * byte_c(w_t) < 2^21
+            nb = (M.bit_length() + 7) // 8
+            cols = [sum(ut * ((wt >> (8 * c)) & 255) for ut, wt in zip(u, w)) for c in range(nb)]
+            assert max(cols) < 2 ** 21
+            assert sum(col << (8 * c) for c, col in enumerate(cols)) == S
+
+
+def test_pidblas_fma_is_the_interval_fma_chain():
+    """the inf-sup restatement (reference's sign-case ifma under fesetround)
+    equals the chain up <- max RU(a b + up), dn <- min RD(a b + dn) over the
+    four endpoint products, evaluated in exact rationals"""
+    rng = np.random.default_rng(21)
+    m, n, k = 6, 5, 9
+    def mixed(r, c):
+        lo = rng.uniform(-1, 1, (r, c)) * 2.0 ** rng.integers(-3, 4, (r, c))
+        hi = lo + np.abs(rng.uniform(0, 1, (r, c))) * 2.0 ** rng.integers(-3, 2, (r, c))
+        X = W._pairs(r, c)
+        X[..., 0], X[..., 1] = lo, hi
+        X[0, :, 0] = 0.0                  # zero lower ends
+        X[0, :, 1] = np.abs(X[0, :, 1])
+        X[1, :, 1] = 0.0                  # zero upper ends
+        X[1, :, 0] = -np.abs(X[1, :, 0])
+        X[2, :, 1] = np.minimum(X[2, :, 1], -2.0 ** -4)   # nonpositive rows
+        X[2, :, 0] = X[2, :, 1] - 1.0
+        assert (X[..., 0] <= X[..., 1]).all()
+        return X
+    IA, IB = mixed(m, k), mixed(k, n)
+    C = O.pidblas_fma_mulmm(IA, IB)
+    for i in range(m):
+        for j in range(n):
+            up, dn = 0.0, 0.0
+            for t in range(k):
+                prods = [F(a) * F(b) for a in IA[i, t] for b in IB[t, j]]
+                up = max(round_dir(p + F(up), 1) for p in prods)
+                dn = min(round_dir(p + F(dn), -1) for p in prods)
+            assert C[i, j, 1] == up and C[i, j, 0] == dn
 
 
 @pytest.mark.parametrize("kind", ["uniform", "wide", "straddle"])

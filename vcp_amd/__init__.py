@@ -16,13 +16,14 @@ import os
 
 import numpy as np
 
-__all__ = ["lib", "Context", "VcphipError", "pairs", "as_pairs", "FORM_PLAIN", "FORM_OZAKI"]
+__all__ = ["lib", "Context", "VcphipError", "pairs", "as_pairs", "FORM_PLAIN", "FORM_OZAKI", "FORM_INFSUP"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libvcphip.so")
 
 FORM_PLAIN = 0
 FORM_OZAKI = 1
+FORM_INFSUP = 2
 PARAM_OZ_NMOD = 1
 PARAM_OZ_VARIANT = 2
 PARAM_STAGE_TIMING = 3
@@ -68,6 +69,13 @@ SIGNATURES = {
     "vcphip_ddgemm_tn": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_residual_batched": [_vp, _c_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _p_dbl],
     "vcphip_residual_batched_dev": [_vp, _c_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _vp],
+    "vcphip_tune_igemm_tile": [_vp, _c_int, _c_int, _c_int, _p_int, _p_dbl],
+    "vcphip_tune_ddgemm_tile": [_vp, _c_int, _c_int, _c_int, _p_int, _p_dbl],
+    "vcphip_tune_igemm_moduli_dev": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, ctypes.c_double, _c_int,
+                                     _p_int, _p_dbl],
+    "vcphip_tune_residual_dev": [_vp, _c_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _p_dbl],
+    "vcphip_tune_save": [_vp, ctypes.c_char_p],
+    "vcphip_tune_load": [_vp, ctypes.c_char_p],
 }
 
 _lib = None
@@ -313,9 +321,51 @@ class Context:
         _check(lib().vcphip_gemm_rurd_dev(self._h, m, n, k, _vp(dA), lda, _vp(dB), ldb, _vp(dU), _vp(dD), ldc,
                                           int(accumulate)), "vcphip_gemm_rurd_dev")
 
+    # ---- autotuner ----
+    def tune_igemm_tile(self, m, n, k):
+        """fastest int8 GEMM tile variant of the interval Ozaki form at (m, n, k): (variant, ms)"""
+        v, ms = ctypes.c_int(), ctypes.c_double()
+        _check(lib().vcphip_tune_igemm_tile(self._h, m, n, k, ctypes.byref(v), ctypes.byref(ms)),
+               "vcphip_tune_igemm_tile")
+        return v.value, ms.value
+
+    def tune_ddgemm_tile(self, m, n, k):
+        """fastest int8 GEMM tile variant of the dd form at (m, n, k): (variant, ms)"""
+        v, ms = ctypes.c_int(), ctypes.c_double()
+        _check(lib().vcphip_tune_ddgemm_tile(self._h, m, n, k, ctypes.byref(v), ctypes.byref(ms)),
+               "vcphip_tune_ddgemm_tile")
+        return v.value, ms.value
+
+    def tune_igemm_moduli_dev(self, m, n, k, dA, lda, dB, ldb, max_ratio=2.0, sample_rows=0):
+        """fewest moduli keeping sampled radii within max_ratio of the plain form: (nmod, ratio)"""
+        nm, r = ctypes.c_int(), ctypes.c_double()
+        _check(lib().vcphip_tune_igemm_moduli_dev(self._h, m, n, k, _vp(dA), lda, _vp(dB), ldb, float(max_ratio),
+                                                  int(sample_rows), ctypes.byref(nm), ctypes.byref(r)),
+               "vcphip_tune_igemm_moduli_dev")
+        return nm.value, r.value
+
+    def tune_save(self, path):
+        _check(lib().vcphip_tune_save(self._h, os.fsencode(path)), "vcphip_tune_save")
+
+    def tune_load(self, path):
+        _check(lib().vcphip_tune_load(self._h, os.fsencode(path)), "vcphip_tune_load")
+
     def ddgemm_dev(self, m, n, k, dA, lda, dB, ldb, dC, ldc):
         _check(lib().vcphip_ddgemm_dev(self._h, m, n, k, _vp(dA), lda, _vp(dB), ldb, _vp(dC), ldc),
                "vcphip_ddgemm_dev")
+
+
+def tune_residual_dev(ctx, ns, R_ptrs, ldr, A_ptrs, lda, C_ptrs, ldc, normub_ptr):
+    """autotune the batched residual's persistent grid for this problem set;
+    returns (workgroups per CU, ms per batch)"""
+    cnt = len(ns)
+    arr = lambda T, xs: (T * cnt)(*xs)  # noqa: E731
+    per, ms = ctypes.c_int(), ctypes.c_double()
+    _check(lib().vcphip_tune_residual_dev(ctx.handle, cnt, arr(ctypes.c_int, ns), arr(_vp, R_ptrs),
+                                          arr(ctypes.c_int, ldr), arr(_vp, A_ptrs), arr(ctypes.c_int, lda),
+                                          arr(_vp, C_ptrs), arr(ctypes.c_int, ldc), _vp(normub_ptr),
+                                          ctypes.byref(per), ctypes.byref(ms)), "vcphip_tune_residual_dev")
+    return per.value, ms.value
 
 
 def residual_batched_dev(ctx, ns, R_ptrs, ldr, A_ptrs, lda, C_ptrs, ldc, normub_ptr):

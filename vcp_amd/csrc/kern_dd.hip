@@ -11,6 +11,7 @@
 // restates every step and the GPU result is bit-identical to it.
 #include "vcphip_device.h"
 #include "vcphip_kernels.h"
+#include "vcphip_crt.h"
 
 namespace {
 
@@ -253,61 +254,21 @@ __global__ __launch_bounds__(256) void k_dd_recon(int m, int n, int mpad, int np
                                                   const int* __restrict__ ea, const int* __restrict__ fa,
                                                   const int* __restrict__ eb, const int* __restrict__ fb,
                                                   const uint8_t* __restrict__ U, double* __restrict__ C, long ldc) {
-  const long gid = (long)blockIdx.x * 256 + threadIdx.x;
-  if (gid >= (long)m * n) return;
-  const int i = (int)(gid % m), j = (int)(gid / m);
+  // threads follow the blocked 16 x 16 layout of the residue planes
+  const long boff = (long)blockIdx.x * 256 + threadIdx.x;
+  const long blk = boff >> 8;
+  const int o = (int)(boff & 255);
+  const long bpm = mpad / 16;
+  const int i = (int)(blk % bpm) * 16 + 4 * (o >> 6) + (o & 3);
+  const int j = (int)(blk / bpm) * 16 + ((o >> 2) & 15);
+  if (i >= m || j >= n) return;
   if (fa[i] || fb[j]) {
     *reinterpret_cast<double2*>(C + 2 * (i + ldc * (long)j)) = make_double2(0.0, 0.0);
     return;
   }
-  const long boff = ((long)(j >> 4) * (mpad / 16) + (i >> 4)) * 256 + ((j & 15) + 16 * ((i & 15) >> 2)) * 4 + (i & 3);
   const long plane = (long)mpad * npad;
-  int v[N];
-#pragma unroll
-  for (int t = 0; t < N; ++t) {
-    const int md = P.mod[t];
-    int acc = (int)U[(long)t * plane + boff];
-#pragma unroll
-    for (int s2 = 0; s2 < t; ++s2) acc += v[s2] * (int)P.negP[s2][t];
-    int q = (int)__builtin_floorf((float)acc * P.rcpf[t]);
-    int r = acc - q * md;
-    r += (r < 0) ? md : 0;
-    r -= (r >= md) ? md : 0;
-    const int x = r * P.pinv[t];
-    q = (int)__builtin_floorf((float)x * P.rcpf[t]);
-    r = x - q * md;
-    r += (r < 0) ? md : 0;
-    r -= (r >= md) ? md : 0;
-    v[t] = r;
-  }
   unsigned mag[NL];
-#pragma unroll
-  for (int l = 0; l < NL; ++l) mag[l] = 0;
-  mag[0] = (unsigned)v[N - 1];
-#pragma unroll
-  for (int t = N - 2; t >= 0; --t) {
-    unsigned long long carry = (unsigned)v[t];
-#pragma unroll
-    for (int l = 0; l < NL; ++l) {
-      const unsigned long long p = (unsigned long long)mag[l] * (unsigned)P.mod[t] + carry;
-      mag[l] = (unsigned)p;
-      carry = p >> 32;
-    }
-  }
-  bool gt = false, decided = false;
-#pragma unroll
-  for (int l = NL - 1; l >= 0; --l)
-    if (!decided && mag[l] != P.halfM[l]) { gt = mag[l] > P.halfM[l]; decided = true; }
-  const bool neg = gt;
-  if (neg) {
-    long long borrow = 0;
-#pragma unroll
-    for (int l = 0; l < NL; ++l) {
-      const long long d = (long long)P.Mlimb[l] - (long long)mag[l] - borrow;
-      borrow = d < 0 ? 1 : 0;
-      mag[l] = (unsigned)(d + (borrow << 32));
-    }
-  }
+  const bool neg = vcphip_crt_symmetric<N, NL>(U, plane, boff, P, mag);
   const int sc = ea[i] + eb[j] - 2 * P.beta;
   int q;
   bool up;
@@ -380,7 +341,7 @@ int vcphip_dd_split_cols(const double* B, long ldb, int k, int n, int npad, int 
 int vcphip_dd_recon(int m, int n, int mpad, int npad, const VcphipOzParams& P, const int* ea, const int* fa,
                     const int* eb, const int* fb, const uint8_t* U, double* C, long ldc, hipStream_t s) {
   if (m <= 0 || n <= 0) return 0;
-  const long total = (long)m * n;
+  const long total = (long)mpad * npad;
   dim3 grid((unsigned)((total + 255) / 256));
   switch (P.nmod) {
 #define VCPHIP_DD_CASE(NN) \

@@ -402,7 +402,128 @@ __global__ __launch_bounds__(NT, 2) void k_drsyrk(int k, int n, const double* __
     }
 }
 
+// ---------------------------------------------------------------------------
+// Inf-sup interval product (pidblas_fma::mulmm, vcp/pidblas_fma.hpp:62-71,
+// with the interval FMA of vcp/ifma.hpp:44-55): per output element the
+// k-ascending chains
+//   up <- max over a in {inf A, sup A}, b in {inf B, sup B} of RU(a b + up)
+//   dn <- min over the same of RD(a b + dn),        both from 0.
+// RU is monotone, so the maximum of the four rounded FMAs is the rounded FMA
+// of the largest product: the same result as the reference's sign-case
+// selection (ifma.hpp:58-137), without its branches.  The RD chain is carried
+// negated, d = -dn, as d <- max RU((-a) b + d).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_isgemm(int m, int n, int k, const double* __restrict__ A, long lda,
+                                               const double* __restrict__ B, long ldb, double* __restrict__ C,
+                                               long ldc) {
+  vcphip_mode_round_up();
+  __shared__ __attribute__((aligned(16))) double sAi[BK][LA];
+  __shared__ __attribute__((aligned(16))) double sAs[BK][LA];
+  __shared__ __attribute__((aligned(16))) double sBi[BK][LB];
+  __shared__ __attribute__((aligned(16))) double sBs[BK][LB];
+  const int ntm = (m + BM - 1) / BM, ntn = (n + BN - 1) / BN;
+  const int tiles = ntm * ntn;
+  int tm, tn;
+  vcphip_tile_coords(vcphip_xcd_remap(blockIdx.x, tiles), ntm, ntn, 8, tm, tn);
+  const int row0 = tm * BM, col0 = tn * BN;
+  const int tid = threadIdx.x;
+  const int a_row = tid % BM, a_kc = tid / BM;
+  const int b_k = tid % BK, b_col = tid / BK;
+  double2 ra[A_PER_T], rb[B_PER_T];
+  auto load_tile = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < A_PER_T; ++q) {
+      const int kk = k0 + a_kc + 4 * q, r = row0 + a_row;
+      ra[q] = (r < m && kk < k) ? *reinterpret_cast<const double2*>(A + 2 * (r + lda * kk)) : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < B_PER_T; ++q) {
+      const int kk = k0 + b_k, c = col0 + b_col + 16 * q;
+      rb[q] = (kk < k && c < n) ? *reinterpret_cast<const double2*>(B + 2 * (kk + ldb * c)) : make_double2(0.0, 0.0);
+    }
+  };
+  auto store_tile = [&]() {
+#pragma unroll
+    for (int q = 0; q < A_PER_T; ++q) {
+      sAi[a_kc + 4 * q][a_row] = ra[q].x;
+      sAs[a_kc + 4 * q][a_row] = ra[q].y;
+    }
+#pragma unroll
+    for (int q = 0; q < B_PER_T; ++q) {
+      sBi[b_k][b_col + 16 * q] = rb[q].x;
+      sBs[b_k][b_col + 16 * q] = rb[q].y;
+    }
+  };
+  const int tx = tid % 16, ty = tid / 16;
+  double up[TM][TN], dn[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      up[i][j] = 0.0;
+      dn[i][j] = -0.0;   // -(+0): the RD chain starts from +0
+    }
+  const int ntiles = (k + BK - 1) / BK;
+  load_tile(0);
+  store_tile();
+  __syncthreads();
+  for (int t = 0; t < ntiles; ++t) {
+    if (t + 1 < ntiles) load_tile((t + 1) * BK);
+#pragma unroll 2
+    for (int kk = 0; kk < BK; ++kk) {
+      double ai[TM], as[TM], bi[TN], bs[TN];
+      {
+        const double2 x0 = *reinterpret_cast<const double2*>(&sAi[kk][ty * TM]);
+        const double2 x1 = *reinterpret_cast<const double2*>(&sAi[kk][ty * TM + 2]);
+        const double2 x2 = *reinterpret_cast<const double2*>(&sAs[kk][ty * TM]);
+        const double2 x3 = *reinterpret_cast<const double2*>(&sAs[kk][ty * TM + 2]);
+        ai[0] = x0.x; ai[1] = x0.y; ai[2] = x1.x; ai[3] = x1.y;
+        as[0] = x2.x; as[1] = x2.y; as[2] = x3.x; as[3] = x3.y;
+        const double2 y0 = *reinterpret_cast<const double2*>(&sBi[kk][tx * TN]);
+        const double2 y1 = *reinterpret_cast<const double2*>(&sBi[kk][tx * TN + 2]);
+        const double2 y2 = *reinterpret_cast<const double2*>(&sBs[kk][tx * TN]);
+        const double2 y3 = *reinterpret_cast<const double2*>(&sBs[kk][tx * TN + 2]);
+        bi[0] = y0.x; bi[1] = y0.y; bi[2] = y1.x; bi[3] = y1.y;
+        bs[0] = y2.x; bs[1] = y2.y; bs[2] = y3.x; bs[3] = y3.y;
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          const double u = up[i][j], d = dn[i][j];
+          up[i][j] = __builtin_fmax(__builtin_fmax(__builtin_fma(as[i], bs[j], u), __builtin_fma(ai[i], bs[j], u)),
+                                    __builtin_fmax(__builtin_fma(as[i], bi[j], u), __builtin_fma(ai[i], bi[j], u)));
+          dn[i][j] = __builtin_fmax(__builtin_fmax(__builtin_fma(-as[i], bs[j], d), __builtin_fma(-ai[i], bs[j], d)),
+                                    __builtin_fmax(__builtin_fma(-as[i], bi[j], d), __builtin_fma(-ai[i], bi[j], d)));
+        }
+    }
+    __syncthreads();
+    if (t + 1 < ntiles) {
+      store_tile();
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int c = col0 + tx * TN + j;
+    if (c >= n) continue;
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const int r = row0 + ty * TM + i;
+      if (r < m) *reinterpret_cast<double2*>(C + 2 * (r + ldc * (long)c)) = make_double2(-dn[i][j], up[i][j]);
+    }
+  }
+}
+
 }  // namespace
+
+int vcphip_launch_isgemm(int m, int n, int k, const double* A, long lda, const double* B, long ldb, double* C,
+                         long ldc, hipStream_t s) {
+  if (m <= 0 || n <= 0) return 0;
+  const int tiles = ((m + BM - 1) / BM) * ((n + BN - 1) / BN);
+  hipLaunchKernelGGL(k_isgemm, dim3(tiles), dim3(NT), 0, s, m, n, k, A, lda, B, ldb, C, ldc);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
 
 int vcphip_launch_drgemm(int kind, int m, int n, int k, const double* A, long lda, const double* B,
                          long ldb, double* C, long ldc, double* C2, int accumulate, hipStream_t s) {

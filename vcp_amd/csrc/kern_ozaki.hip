@@ -19,6 +19,7 @@
 // result is bit-identical to it.
 #include "vcphip_device.h"
 #include "vcphip_kernels.h"
+#include "vcphip_crt.h"
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
@@ -596,18 +597,183 @@ __global__ __launch_bounds__(256) void k_oz_gemm3(int mpad, int npad, int kpad, 
   }
 }
 
+// Deeper pipeline: NST LDS stages and fragments double-buffered in registers.
+// Iteration kt waits for stage kt+1, refills stage kt (its fragments are
+// already in registers) with k-step kt+NST, reads the fragments of kt+1 and
+// runs the MFMAs of kt.  The swizzle term (R >> 2) & 3 of a fragment row
+// R = 16 i + frow (+ a multiple of 16) is (frow >> 2) & 3 for every i, so one
+// base offset per operand serves all fragments.
+template <int FM, int FN>
+struct OzFrag {
+  v4i a[FM];
+  v4i b[FN];
+};
+
+template <int FM, int FN>
+__device__ __forceinline__ void oz_load_frag(const int8_t* la, const int8_t* lb, OzFrag<FM, FN>& f) {
+#pragma unroll
+  for (int i = 0; i < FM; ++i) f.a[i] = *reinterpret_cast<const v4i*>(la + i * 1024);
+#pragma unroll
+  for (int j = 0; j < FN; ++j) f.b[j] = *reinterpret_cast<const v4i*>(lb + j * 1024);
+}
+
+template <int FM, int FN>
+__device__ __forceinline__ void oz_mfma_frag(const OzFrag<FM, FN>& f, v4i (&acc)[FM][FN]) {
+#pragma unroll
+  for (int j = 0; j < FN; ++j)
+#pragma unroll
+    for (int i = 0; i < FM; ++i) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f.a[i], f.b[j], acc[i][j], 0, 0, 0);
+}
+
+template <int FM, int FN, int NST>
+__global__ __launch_bounds__(256) void k_oz_gemm4(int mpad, int npad, int kpad, int nmod, int nz,
+                                                  const int8_t* __restrict__ Aplanes,
+                                                  const int8_t* __restrict__ Bplanes, VcphipOzParams P,
+                                                  uint8_t* __restrict__ U, int32_t* __restrict__ Z) {
+  constexpr int WM = 16 * FM, WN = 16 * FN;
+  constexpr int TMB = 2 * WM, TNB = 2 * WN;
+  constexpr int SEGA = TMB / 16, SEGB = TNB / 16;
+  constexpr int SEGA_W = SEGA / 4, SEGB_W = SEGB / 4;
+  constexpr int G = SEGA_W + SEGB_W;   // loads per wave per stage
+  static_assert(SEGA_W * 4 == SEGA && SEGB_W * 4 == SEGB, "tile/wave mismatch");
+  static_assert(NST >= 3 && (NST - 1) * G < 64, "vmcnt range");
+  __shared__ __attribute__((aligned(16))) int8_t sA[NST][TMB * 64];
+  __shared__ __attribute__((aligned(16))) int8_t sB[NST][TNB * 64];
+
+  const int ntm = mpad / TMB, ntn = npad / TNB;
+  const int tiles = ntm * ntn;
+  const int z = blockIdx.x / tiles;
+  const int local = vcphip_xcd_remap(blockIdx.x - z * tiles, tiles);
+  int tm, tn;
+  vcphip_tile_coords(local, ntm, ntn, 8, tm, tn);
+  const int row0 = tm * TMB, col0 = tn * TNB;
+  int npass = 1, pa0 = z, pb0 = z, pa1 = z, pb1 = z;
+  if (z >= nmod) {
+    const int r = z - nmod, base = nmod + 2 * (r >> 1);
+    pa0 = pb0 = base;
+    if (r & 1) {
+      npass = 2;
+      pb0 = base + 1;
+      pa1 = base + 1;
+      pb1 = base;
+    }
+  }
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w & 1, wn = w >> 1;
+  const int wu = __builtin_amdgcn_readfirstlane(w);
+  const uint32_t sA_base = lds_addr(&sA[0][0]), sB_base = lds_addr(&sB[0][0]);
+  const int lrow = lane >> 2, lchunk = (lane & 3) ^ ((lane >> 4) & 3);
+  const int src_lane = lrow * kpad + 16 * lchunk;
+  const int frow = lane & 15, g = lane >> 4;
+  const int fsw = 16 * (g ^ ((frow >> 2) & 3));
+  const int fa0 = (wm * WM + frow) * 64 + fsw, fb0 = (wn * WN + frow) * 64 + fsw;
+
+  v4i acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = v4i{0, 0, 0, 0};
+
+  const int nk = kpad / 64;
+  OzFrag<FM, FN> f0, f1;
+  for (int pass = 0; pass < npass; ++pass) {
+    const int8_t* Ab = Aplanes + (long)(pass ? pa1 : pa0) * mpad * kpad + (long)row0 * kpad + src_lane;
+    const int8_t* Bb = Bplanes + (long)(pass ? pb1 : pb0) * npad * kpad + (long)col0 * kpad + src_lane;
+    auto issue = [&](int stage, int kt) {
+      const long k0 = (long)kt * 64;
+#pragma unroll
+      for (int q = 0; q < SEGA_W; ++q) {
+        const int seg = wu * SEGA_W + q;
+        lds_dma16(Ab + (long)seg * 16 * kpad + k0, sA_base + stage * (TMB * 64) + seg * 1024);
+      }
+#pragma unroll
+      for (int q = 0; q < SEGB_W; ++q) {
+        const int seg = wu * SEGB_W + q;
+        lds_dma16(Bb + (long)seg * 16 * kpad + k0, sB_base + stage * (TNB * 64) + seg * 1024);
+      }
+    };
+    // Waits until stage kt+1 (or stage 0 when kt = -1) has landed for every
+    // wave; the loads issued after it may stay in flight.
+    auto wait_next = [&](int kt) {
+      if (kt + NST < nk) asm volatile("s_waitcnt vmcnt(%0)" ::"n"((NST - 2) * G) : "memory");
+      else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    };
+    const int npro = nk < NST ? nk : NST;
+    for (int st = 0; st < npro; ++st) issue(st, st);
+    if (nk >= NST) asm volatile("s_waitcnt vmcnt(%0)" ::"n"((NST - 1) * G) : "memory");
+    else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    oz_load_frag<FM, FN>(sA[0] + fa0, sB[0] + fb0, f0);
+    for (int kt = 0; kt < nk; kt += 2) {
+      if (kt + 1 < nk) {
+        wait_next(kt);
+        if (kt + NST < nk) issue(kt % NST, kt + NST);
+        oz_load_frag<FM, FN>(sA[(kt + 1) % NST] + fa0, sB[(kt + 1) % NST] + fb0, f1);
+      }
+      oz_mfma_frag<FM, FN>(f0, acc);
+      if (kt + 1 < nk) {
+        if (kt + 2 < nk) {
+          wait_next(kt + 1);
+          if (kt + 1 + NST < nk) issue((kt + 1) % NST, kt + 1 + NST);
+          oz_load_frag<FM, FN>(sA[(kt + 2) % NST] + fa0, sB[(kt + 2) % NST] + fb0, f0);
+        }
+        oz_mfma_frag<FM, FN>(f1, acc);
+      }
+    }
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  }
+
+  const long blocks_m = mpad / 16;
+  if (z < nmod) {
+    const int md = P.mod[z];
+    const double rc = P.rcpd[z];
+    uint8_t* Uz = U + (long)z * mpad * npad;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const long bi = (row0 + wm * WM + 16 * i) / 16, bj = (col0 + wn * WN + 16 * j) / 16;
+        unsigned packed = 0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const double y = (double)acc[i][j][v];
+          const double qd = __builtin_floor(y * rc);
+          int rr = (int)(y - qd * (double)md);
+          rr += (rr < 0) ? md : 0;
+          rr -= (rr >= md) ? md : 0;
+          packed |= (unsigned)rr << (8 * v);
+        }
+        *reinterpret_cast<unsigned*>(Uz + (bj * blocks_m + bi) * 256 + lane * 4) = packed;
+      }
+  } else {
+    int32_t* Zz = Z + (long)(z - nmod) * mpad * npad;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const long bi = (row0 + wm * WM + 16 * i) / 16, bj = (col0 + wn * WN + 16 * j) / 16;
+        *reinterpret_cast<v4i*>(Zz + (bj * blocks_m + bi) * 256 + lane * 4) = acc[i][j];
+      }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // reconstruction
 // ---------------------------------------------------------------------------
 constexpr int NL = 5;   // 32-bit limbs: M < 2^160 for N <= 20
 
-// |X| * 2^s rounded away from zero (away=true) or toward zero, exactly
-VCPHIP_DEVINL double conv_limbs(const unsigned (&mag)[NL], int s, bool away) {
+// |X| * 2^s rounded toward zero (tz) and away from zero (aw), exactly: both
+// come from one 64-bit window of the limbs and the sticky bit below it
+VCPHIP_DEVINL void conv_limbs2(const unsigned (&mag)[NL], int s, double& tz, double& aw) {
   int top = -1;
 #pragma unroll
   for (int l = NL - 1; l >= 0; --l)
     if (top < 0 && mag[l] != 0) top = 32 * l + 31 - __builtin_clz(mag[l]);
-  if (top < 0) return 0.0;
+  if (top < 0) {
+    tz = aw = 0.0;
+    return;
+  }
   int q = top - 52;
   if (q < -1074 - s) q = -1074 - s;
   unsigned long long mant;
@@ -616,7 +782,7 @@ VCPHIP_DEVINL double conv_limbs(const unsigned (&mag)[NL], int s, bool away) {
     mant = (unsigned long long)mag[0] | ((unsigned long long)mag[1] << 32);
     q = 0;
   } else {
-    // gather bits [q, q+64) and the sticky tail below q
+    // bits [q, q+64) and the sticky tail below q
     const int ls = q >> 5, bs = q & 31;
     unsigned w0 = 0, w1 = 0, w2 = 0;
     bool tail = false;
@@ -630,13 +796,17 @@ VCPHIP_DEVINL double conv_limbs(const unsigned (&mag)[NL], int s, bool away) {
     if (bs != 0 && (w0 & ((1u << bs) - 1u)) != 0) tail = true;
     const unsigned long long lo = (unsigned long long)w0 | ((unsigned long long)w1 << 32);
     mant = bs ? ((lo >> bs) | ((unsigned long long)w2 << (64 - bs))) : lo;
-    if (q >= 32 * NL) { mant = 0; tail = true; }
+    if (q >= 32 * NL) {
+      mant = 0;
+      tail = true;
+    }
     rest = tail;
   }
-  if (rest && away) mant += 1;
   const int ex = q + s;
-  if (mant != 0 && ex + (64 - __builtin_clzll(mant)) > 1024) return away ? __builtin_inf() : 1.7976931348623157e308;
-  return __builtin_ldexp((double)mant, ex);
+  const unsigned long long mant_aw = mant + (rest ? 1ull : 0ull);
+  // mant < 2^53 except in the subnormal clamp (q <= 0), where both are exact
+  tz = (mant != 0 && ex + (64 - __builtin_clzll(mant)) > 1024) ? 1.7976931348623157e308 : __builtin_ldexp((double)mant, ex);
+  aw = (mant_aw != 0 && ex + (64 - __builtin_clzll(mant_aw)) > 1024) ? __builtin_inf() : __builtin_ldexp((double)mant_aw, ex);
 }
 
 template <int N>
@@ -645,78 +815,28 @@ __global__ __launch_bounds__(256) void k_oz_recon(int m, int n, int k, int mpad,
                                                   const uint8_t* __restrict__ U, const int32_t* __restrict__ Z,
                                                   double* __restrict__ C, long ldc) {
   vcphip_mode_round_up();
-  const long gid = (long)blockIdx.x * 256 + threadIdx.x;
-  if (gid >= (long)m * n) return;
-  const int i = (int)(gid % m), j = (int)(gid / m);
-  const long boff = ((long)(j >> 4) * (mpad / 16) + (i >> 4)) * 256 + ((j & 15) + 16 * ((i & 15) >> 2)) * 4 + (i & 3);
+  // threads follow the blocked 16 x 16 layout of the planes (coalesced
+  // residue reads): offset o in a block holds row 4 (o / 64) + o % 4 and
+  // column (o / 4) % 16 of that block
+  const long boff = (long)blockIdx.x * 256 + threadIdx.x;
+  const long blk = boff >> 8;
+  const int o = (int)(boff & 255);
+  const long bpm = mpad / 16;
+  const int i = (int)(blk % bpm) * 16 + 4 * (o >> 6) + (o & 3);
+  const int j = (int)(blk / bpm) * 16 + ((o >> 2) & 15);
+  if (i >= m || j >= n) return;
   const long plane = (long)mpad * npad;
 
-  // mixed-radix digits of X = A'B' mod M (Garner):
-  //   v_t = (u_t + sum_{s<t} v_s (-(m_0..m_{s-1}) mod m_t)) (m_0..m_{t-1})^{-1} mod m_t
-  int v[N];
-#pragma unroll
-  for (int t = 0; t < N; ++t) {
-    const int md = P.mod[t];
-    int acc = (int)U[(long)t * plane + boff];
-#pragma unroll
-    for (int s2 = 0; s2 < t; ++s2) acc += v[s2] * P.negP[s2][t];
-    // acc < 2^20: float division estimate, corrected exactly
-    int q = (int)__builtin_floorf((float)acc * P.rcpf[t]);
-    int r = acc - q * md;
-    r += (r < 0) ? md : 0;
-    r -= (r >= md) ? md : 0;
-    int x = r * P.pinv[t];
-    q = (int)__builtin_floorf((float)x * P.rcpf[t]);
-    r = x - q * md;
-    r += (r < 0) ? md : 0;
-    r -= (r >= md) ? md : 0;
-    v[t] = r;
-  }
-  // X = v_0 + m_0 (v_1 + m_1 (v_2 + ...)) in 32-bit limbs
   unsigned mag[NL];
-#pragma unroll
-  for (int l = 0; l < NL; ++l) mag[l] = 0;
-  mag[0] = (unsigned)v[N - 1];
-#pragma unroll
-  for (int t = N - 2; t >= 0; --t) {
-    unsigned long long carry = (unsigned)v[t];
-#pragma unroll
-    for (int l = 0; l < NL; ++l) {
-      const unsigned long long p = (unsigned long long)mag[l] * (unsigned)P.mod[t] + carry;
-      mag[l] = (unsigned)p;
-      carry = p >> 32;
-    }
-  }
-  // symmetric range: X > M/2 means X - M < 0
-  bool gt = false, decided = false;
-#pragma unroll
-  for (int l = NL - 1; l >= 0; --l) {
-    const unsigned hm = P.halfM[l];
-    if (!decided && mag[l] != hm) { gt = mag[l] > hm; decided = true; }
-  }
-  const bool neg = gt;
-  if (neg) {   // mag = M - X
-    long long borrow = 0;
-#pragma unroll
-    for (int l = 0; l < NL; ++l) {
-      const long long d = (long long)P.Mlimb[l] - (long long)mag[l] - borrow;
-      borrow = d < 0 ? 1 : 0;
-      mag[l] = (unsigned)(d + (borrow << 32));
-    }
-  }
+  const bool neg = vcphip_crt_symmetric<N, NL>(U, plane, boff, P, mag);
   const int fa = sa.flags[i], fb = sb.flags[j];
   const bool zr = (fa & VCPHIP_OZ_MIDZERO) != 0, zc = (fb & VCPHIP_OZ_MIDZERO) != 0;
   const int ei = sa.e[i], fj = sb.e[j];
   const int sc = ei + fj - 2 * P.beta;
   // RD and RU of the exact midpoint product
-  double lo_mid, hi_mid;
-  if (!neg) {
-    lo_mid = conv_limbs(mag, sc, false);
-    hi_mid = conv_limbs(mag, sc, true);
-  } else {
-    lo_mid = -conv_limbs(mag, sc, true);
-    hi_mid = -conv_limbs(mag, sc, false);
-  }
+  double tz, aw;
+  conv_limbs2(mag, sc, tz, aw);
+  const double lo_mid = neg ? -aw : tz, hi_mid = neg ? -tz : aw;
   // radius term (|mA|+rA) rB + rA |mB| (pidblas :277-283) from the digit
   // products: with x <= (h + l/127) 2^g/127 per factor,
   //   sum x1 x2 <= 2^(g1+g2)/127^2 (Zhh + Zmix/127 + k),  l1 l2 <= 127^2.
@@ -813,6 +933,23 @@ int vcphip_oz_gemm(int mpad, int npad, int kpad, int nmod, int nrad, const int8_
                            Bplanes, P, U, Z);
       break;
     }
+    case 8:
+    case 9:
+    case 10: {
+      if (kpad % 64) return -7;
+      const int fm = variant == 10 ? 4 : 8;
+      const int tiles = (mpad / (32 * fm)) * (npad / 256);
+      if (variant == 8)
+        hipLaunchKernelGGL((k_oz_gemm4<8, 8, 4>), dim3(tiles * nz), dim3(256), 0, s, mpad, npad, kpad, nmod, nz,
+                           Aplanes, Bplanes, P, U, Z);
+      else if (variant == 9)
+        hipLaunchKernelGGL((k_oz_gemm4<8, 8, 3>), dim3(tiles * nz), dim3(256), 0, s, mpad, npad, kpad, nmod, nz,
+                           Aplanes, Bplanes, P, U, Z);
+      else
+        hipLaunchKernelGGL((k_oz_gemm4<4, 8, 4>), dim3(tiles * nz), dim3(256), 0, s, mpad, npad, kpad, nmod, nz,
+                           Aplanes, Bplanes, P, U, Z);
+      break;
+    }
     default: return -7;
   }
 #undef VCPHIP_GEMM_LAUNCH
@@ -823,7 +960,7 @@ int vcphip_oz_recon(int m, int n, int k, int mpad, int npad, const VcphipOzParam
                     VcphipOzSide sb, const uint8_t* U, const int32_t* Z, double* C, long ldc,
                     hipStream_t s) {
   if (m <= 0 || n <= 0) return 0;
-  const long total = (long)m * n;
+  const long total = (long)mpad * npad;
   dim3 grid((unsigned)((total + 255) / 256));
   switch (P.nmod) {
 #define VCPHIP_RECON_CASE(NN) \

@@ -43,8 +43,21 @@ struct vcphip_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int grid = 0;
+    // device descriptors of the cached set (for re-timing by the tuner)
+    const VcphipResidTile* dt = nullptr;
+    const VcphipResidProblem* dp = nullptr;
+    const int* drp = nullptr;
+    double* drs = nullptr;
+    int nrows = 0;
   } resid;
   int num_cu = 0;
+  // autotuner table: choices per product kind and size class (ceil(log2) of
+  // m, n, k); 0 in a field means "not tuned"
+  struct Tune {
+    int kind, lm, ln, lk, variant, nmod;
+  };
+  std::vector<Tune> tune;
+  int resid_per_cu = 2;   // persistent workgroups per CU of the batched residual
 };
 
 namespace {
@@ -87,6 +100,33 @@ struct StageTimer {
 };
 
 int check_launch(int rc) { return rc == 0 ? VCPHIP_OK : (rc == -7 ? VCPHIP_ERR_UNSUPPORTED : VCPHIP_ERR_DEVICE); }
+
+// autotuner table
+enum { TUNE_IGEMM = 1, TUNE_DDGEMM = 2 };
+constexpr int DEFAULT_VARIANT = 4;   // 256 x 256 tile, register-staged
+
+int size_class(int x) {
+  int l = 0;
+  while (l < 31 && (1 << l) < x) ++l;
+  return l;
+}
+
+const vcphip_ctx::Tune* tune_find(const vcphip_ctx* c, int kind, int m, int n, int k) {
+  const int lm = size_class(m), ln = size_class(n), lk = size_class(k);
+  for (const auto& t : c->tune)
+    if (t.kind == kind && t.lm == lm && t.ln == ln && t.lk == lk) return &t;
+  return nullptr;
+}
+
+void tune_put(vcphip_ctx* c, int kind, int lm, int ln, int lk, int variant, int nmod) {
+  for (auto& t : c->tune)
+    if (t.kind == kind && t.lm == lm && t.ln == ln && t.lk == lk) {
+      if (variant) t.variant = variant;
+      if (nmod) t.nmod = nmod;
+      return;
+    }
+  c->tune.push_back(vcphip_ctx::Tune{kind, lm, ln, lk, variant, nmod});
+}
 
 int gcd_i(int a, int b) {
   while (b) {
@@ -146,7 +186,6 @@ int oz_fill_params(int k, int nmod, int extra, int beta_cap, VcphipOzParams* P) 
   if (P->beta < 8) return VCPHIP_ERR_UNSUPPORTED;
   for (int t = 0; t < nmod; ++t) {
     const int m = P->mod[t];
-    P->rcpf[t] = 1.0f / (float)m;
     P->rcpd[t] = 1.0 / (double)m;
     P->w21[t] = (int)((1ull << 21) % (unsigned)m);
     P->w42[t] = (int)((1ull << 42) % (unsigned long long)m);
@@ -155,12 +194,6 @@ int oz_fill_params(int k, int nmod, int extra, int beta_cap, VcphipOzParams* P) 
       P->w20[c][t] = (uint8_t)(w % (unsigned)m);
       w = (w % (unsigned)m) * ((1ull << 20) % (unsigned)m) % (unsigned)m;
     }
-    long long prod = 1;   // (m_0 ... m_{t-1}) mod m_t
-    for (int s = 0; s < t; ++s) {
-      P->negP[s][t] = (uint8_t)((m - prod % m) % m);
-      prod = (prod * P->mod[s]) % m;
-    }
-    P->pinv[t] = modinv((int)(prod % m), m);
   }
   uint32_t M[VCPHIP_OZ_NLIMB] = {1, 0, 0, 0, 0, 0, 0, 0};
   for (int t = 0; t < nmod; ++t) {
@@ -172,9 +205,37 @@ int oz_fill_params(int k, int nmod, int extra, int beta_cap, VcphipOzParams* P) 
     }
     if (carry) return VCPHIP_ERR_UNSUPPORTED;
   }
-  for (int l = 0; l < VCPHIP_OZ_NLIMB; ++l) {
-    P->Mlimb[l] = M[l];
-    P->halfM[l] = (M[l] >> 1) | (l + 1 < VCPHIP_OZ_NLIMB ? (M[l + 1] << 31) : 0u);
+  for (int l = 0; l < VCPHIP_OZ_NLIMB; ++l) P->Mlimb[l] = M[l];
+  // CRT weights w_t = (M/m_t) y_t, y_t = (M/m_t)^-1 mod m_t, and w_t / M
+  long double Mval = 0.0L;
+  for (int l = VCPHIP_OZ_NLIMB - 1; l >= 0; --l) Mval = Mval * 4294967296.0L + (long double)M[l];
+  for (int t = 0; t < nmod; ++t) {
+    const unsigned m = (unsigned)P->mod[t];
+    uint32_t Mt[VCPHIP_OZ_NLIMB];
+    unsigned long long rem = 0;
+    for (int l = VCPHIP_OZ_NLIMB - 1; l >= 0; --l) {   // M / m_t (exact)
+      const unsigned long long cur = (rem << 32) | M[l];
+      Mt[l] = (uint32_t)(cur / m);
+      rem = cur % m;
+    }
+    unsigned long long mt_mod = 0;   // (M / m_t) mod m_t
+    for (int l = VCPHIP_OZ_NLIMB - 1; l >= 0; --l) mt_mod = ((mt_mod << 32) | Mt[l]) % m;
+    const unsigned y = (unsigned)modinv((int)mt_mod, (int)m);
+    uint32_t wt[VCPHIP_OZ_NLIMB];
+    unsigned long long carry = 0;
+    for (int l = 0; l < VCPHIP_OZ_NLIMB; ++l) {
+      const unsigned long long p = (unsigned long long)Mt[l] * y + carry;
+      wt[l] = (uint32_t)p;
+      carry = p >> 32;
+    }
+    if (carry) return VCPHIP_ERR_UNSUPPORTED;
+    long double wval = 0.0L;
+    for (int l = VCPHIP_OZ_NLIMB - 1; l >= 0; --l) wval = wval * 4294967296.0L + (long double)wt[l];
+    P->crt_f[t] = (double)(wval / Mval);
+    for (int c = 0; c < VCPHIP_OZ_MAXMOD; ++c) {
+      const uint32_t byte = (wt[c / 4] >> (8 * (c % 4))) & 0xFFu;
+      P->crt_w8[c][t / 4] |= byte << (8 * (t % 4));
+    }
   }
   {
     const int old = fegetround();
@@ -310,7 +371,7 @@ extern "C" int vcphip_ctx_set_param(vcphip_ctx* c, int key, long long v) {
       c->oz_nmod = v;
       return VCPHIP_OK;
     case VCPHIP_PARAM_OZ_VARIANT:
-      if (v < 0 || v > 7) return VCPHIP_ERR_ARGUMENT;
+      if (v < 0 || v > 10) return VCPHIP_ERR_ARGUMENT;
       c->oz_variant = v;
       return VCPHIP_OK;
     case VCPHIP_PARAM_STAGE_TIMING:   // (re)enabling clears the totals
@@ -373,7 +434,8 @@ static int dims_ok(int m, int n, int k) { return m >= 0 && n >= 0 && k >= 0; }
 static int oz_igemm_dev(vcphip_ctx* c, int m, int n, int k, const double* A, int lda, const double* B, int ldb,
                         double* C, int ldc) {
   if (k > VCPHIP_OZ_MAXK) return VCPHIP_ERR_UNSUPPORTED;
-  const int nmod = c->oz_nmod ? (int)c->oz_nmod : default_nmod(k);
+  const vcphip_ctx::Tune* tuned = tune_find(c, TUNE_IGEMM, m, n, k);
+  const int nmod = c->oz_nmod ? (int)c->oz_nmod : (tuned && tuned->nmod ? tuned->nmod : default_nmod(k));
   VcphipOzParams P;
   int rc = oz_make_params(k, nmod, &P);
   if (rc) return rc;
@@ -402,8 +464,7 @@ static int oz_igemm_dev(vcphip_ctx* c, int m, int n, int k, const double* A, int
     return s;
   };
   VcphipOzSide SA = carve(sa, m), SB = carve(sb, n);
-  int variant = (int)c->oz_variant;
-  if (variant == 0) variant = 4;   // default tile: 256 x 256, 64-byte stages (fastest measured)
+  const int variant = c->oz_variant ? (int)c->oz_variant : (tuned && tuned->variant ? tuned->variant : DEFAULT_VARIANT);
   StageTimer T(c);
   if ((rc = check_launch(vcphip_oz_stats(A, 1, lda, m, k, SA, c->stream)))) return rc;
   if ((rc = check_launch(vcphip_oz_stats(B, ldb, 1, n, k, SB, c->stream)))) return rc;
@@ -430,6 +491,12 @@ extern "C" int vcphip_igemm_dev(vcphip_ctx* c, int form, int m, int n, int k, co
   if (m == 0 || n == 0) return VCPHIP_OK;
   hipSetDevice(c->device);
   if (form == VCPHIP_FORM_OZAKI) return oz_igemm_dev(c, m, n, k, IA, lda, IB, ldb, C, ldc);
+  if (form == VCPHIP_FORM_INFSUP) {
+    StageTimer T(c);
+    const int rc = check_launch(vcphip_launch_isgemm(m, n, k, IA, lda, IB, ldb, C, ldc, c->stream));
+    T.mark("isgemm");
+    return rc;
+  }
   if (form != VCPHIP_FORM_PLAIN) return VCPHIP_ERR_ARGUMENT;
   StageTimer T(c);
   const int rc = check_launch(vcphip_launch_drgemm(VCPHIP_DR_II, m, n, k, IA, lda, IB, ldb, C, ldc, nullptr, 0, c->stream));
@@ -518,8 +585,8 @@ static int dd_gemm_dev(vcphip_ctx* c, bool tn, int m, int n, int k, const double
     return rc;
   if ((rc = check_launch(vcphip_dd_split_cols(B, ldb, k, n, npad, kpad, P, eb, fb, (int8_t*)pb, c->stream)))) return rc;
   T.mark("split");
-  int variant = (int)c->oz_variant;
-  if (variant == 0) variant = 4;
+  const vcphip_ctx::Tune* tuned = tune_find(c, TUNE_DDGEMM, m, n, k);
+  const int variant = c->oz_variant ? (int)c->oz_variant : (tuned && tuned->variant ? tuned->variant : DEFAULT_VARIANT);
   const int8_t* Aplanes = tn ? (const int8_t*)pb : (const int8_t*)pa;
   if ((rc = check_launch(vcphip_oz_gemm(mpad, npad, kpad, nmod, 0, Aplanes, (const int8_t*)pb, P, (uint8_t*)pu,
                                         nullptr, variant, c->stream))))
@@ -599,12 +666,17 @@ extern "C" int vcphip_residual_batched_dev(vcphip_ctx* c, int count, const int* 
     rs.normub = normub_dev;
     rs.stream = c->stream;
     const int ntiles = (int)rs.tiles.size();
-    rs.grid = std::max(1, std::min(ntiles, 2 * (c->num_cu > 0 ? c->num_cu : 256)));
+    rs.grid = std::max(1, std::min(ntiles, c->resid_per_cu * (c->num_cu > 0 ? c->num_cu : 256)));
     // capture the persistent launch and the norm kernels once; replays reuse it
     const VcphipResidTile* dt = (const VcphipResidTile*)(b8 + o_t);
     const VcphipResidProblem* dp = (const VcphipResidProblem*)b8;
     const int* drp = (const int*)(b8 + o_r);
     double* drs = (double*)(b8 + o_s);
+    rs.dt = dt;
+    rs.dp = dp;
+    rs.drp = drp;
+    rs.drs = drs;
+    rs.nrows = nrows;
     bool captured = false;
     if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
       const int lrc = vcphip_launch_resid(ntiles, dt, dp, nrows, drp, drs, count, normub_dev, rs.grid, c->stream);
@@ -853,4 +925,272 @@ extern "C" int vcphip_residual_batched(vcphip_ctx* c, int count, const int* n, c
   if (hipMemcpyAsync(normub, b8 + offN, sizeof(double) * count, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
     return VCPHIP_ERR_DEVICE;
   return hipStreamSynchronize(c->stream) == hipSuccess ? VCPHIP_OK : VCPHIP_ERR_DEVICE;
+}
+
+// ===========================================================================
+// autotuner
+// ===========================================================================
+namespace {
+
+constexpr int TUNE_VARIANTS[] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10};
+
+// time the int8 MFMA stage of one Ozaki product shape with every tile variant
+// (operand planes of arbitrary content: the integer MFMA time does not depend
+// on the values); returns the fastest
+int tune_gemm_stage(vcphip_ctx* c, bool dd, int m, int n, int k, int* best_variant, double* best_ms) {
+  if (m <= 0 || n <= 0 || k <= 0) return VCPHIP_ERR_DIMENSION;
+  if (k > VCPHIP_OZ_MAXK) return VCPHIP_ERR_UNSUPPORTED;
+  const vcphip_ctx::Tune* tuned = tune_find(c, dd ? TUNE_DDGEMM : TUNE_IGEMM, m, n, k);
+  int nmod;
+  VcphipOzParams P;
+  int rc;
+  if (dd) {
+    nmod = c->dd_nmod ? (int)c->dd_nmod : default_dd_nmod(k);
+    rc = dd_make_params(k, nmod, &P);
+  } else {
+    nmod = c->oz_nmod ? (int)c->oz_nmod : (tuned && tuned->nmod ? tuned->nmod : default_nmod(k));
+    rc = oz_make_params(k, nmod, &P);
+  }
+  if (rc) return rc;
+  const int nrad = dd ? 0 : VCPHIP_OZ_NDIGIT;
+  const int mpad = (m + VCPHIP_OZ_PADMN - 1) / VCPHIP_OZ_PADMN * VCPHIP_OZ_PADMN;
+  const int npad = (n + VCPHIP_OZ_PADMN - 1) / VCPHIP_OZ_PADMN * VCPHIP_OZ_PADMN;
+  const int kpad = (k + VCPHIP_OZ_PADK - 1) / VCPHIP_OZ_PADK * VCPHIP_OZ_PADK;
+  void *pa, *pb, *pu, *pz = nullptr;
+  const size_t nplane = (size_t)nmod + nrad;
+  if ((rc = ws_get(c, vcphip_ctx::WS_PA, nplane * mpad * kpad, &pa))) return rc;
+  if ((rc = ws_get(c, vcphip_ctx::WS_PB, nplane * npad * kpad, &pb))) return rc;
+  if ((rc = ws_get(c, vcphip_ctx::WS_U, (size_t)nmod * mpad * npad, &pu))) return rc;
+  if (!dd && (rc = ws_get(c, vcphip_ctx::WS_Z, VCPHIP_OZ_NDIGIT * sizeof(int32_t) * (size_t)mpad * npad, &pz)))
+    return rc;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return VCPHIP_ERR_DEVICE;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    hipEventDestroy(e0);
+    return VCPHIP_ERR_DEVICE;
+  }
+  const double flops = 2.0 * mpad * npad * (double)kpad * (double)(nmod + (dd ? 0 : 6));
+  const int reps = flops > 1e13 ? 2 : (flops > 1e11 ? 4 : 10);
+  int best = 0;
+  double best_t = 0.0;
+  rc = VCPHIP_OK;
+  for (int v : TUNE_VARIANTS) {
+    auto launch = [&]() {
+      return vcphip_oz_gemm(mpad, npad, kpad, nmod, nrad, (const int8_t*)pa, (const int8_t*)pb, P, (uint8_t*)pu,
+                            (int32_t*)pz, v, c->stream);
+    };
+    if (launch() != 0) {   // variant not applicable to this shape
+      hipGetLastError();
+      continue;
+    }
+    hipEventRecord(e0, c->stream);
+    for (int r = 0; r < reps; ++r) launch();
+    hipEventRecord(e1, c->stream);
+    if (hipEventSynchronize(e1) != hipSuccess) {
+      rc = VCPHIP_ERR_DEVICE;
+      break;
+    }
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    const double t = ms / reps;
+    if (!best || t < best_t) {
+      best = v;
+      best_t = t;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) return rc;
+  if (!best) return VCPHIP_ERR_UNSUPPORTED;
+  tune_put(c, dd ? TUNE_DDGEMM : TUNE_IGEMM, size_class(m), size_class(n), size_class(k), best, 0);
+  if (best_variant) *best_variant = best;
+  if (best_ms) *best_ms = best_t;
+  return VCPHIP_OK;
+}
+
+}  // namespace
+
+extern "C" int vcphip_tune_igemm_tile(vcphip_ctx* c, int m, int n, int k, int* variant, double* ms) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  hipSetDevice(c->device);
+  return tune_gemm_stage(c, false, m, n, k, variant, ms);
+}
+
+extern "C" int vcphip_tune_ddgemm_tile(vcphip_ctx* c, int m, int n, int k, int* variant, double* ms) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  hipSetDevice(c->device);
+  return tune_gemm_stage(c, true, m, n, k, variant, ms);
+}
+
+extern "C" int vcphip_tune_igemm_moduli_dev(vcphip_ctx* c, int m, int n, int k, const double* IA, int lda,
+                                            const double* IB, int ldb, double max_ratio, int sample_rows, int* nmod,
+                                            double* ratio) {
+  if (!c || !IA || !IB || !(max_ratio > 0.0)) return VCPHIP_ERR_ARGUMENT;
+  if (m <= 0 || n <= 0 || k <= 0 || lda < m || ldb < k) return VCPHIP_ERR_DIMENSION;
+  if (k > VCPHIP_OZ_MAXK) return VCPHIP_ERR_UNSUPPORTED;
+  hipSetDevice(c->device);
+  const int s = std::max(1, std::min(m, sample_rows > 0 ? sample_rows : 256));
+  // sample rows, the plain and the Ozaki products of the sample, block maxima
+  const size_t bs = sizeof(double) * 2 * (size_t)s * k, bc = sizeof(double) * 2 * (size_t)s * n;
+  const int nblocks = 1024;
+  const size_t o_p = (bs + 255) & ~(size_t)255, o_o = o_p + ((bc + 255) & ~(size_t)255);
+  const size_t o_m = o_o + ((bc + 255) & ~(size_t)255);
+  void* base;
+  int rc = ws_get(c, vcphip_ctx::WS_HC2, o_m + sizeof(double) * nblocks, &base);
+  if (rc) return rc;
+  char* b8 = (char*)base;
+  double* As = (double*)b8;
+  double* Cp = (double*)(b8 + o_p);
+  double* Co = (double*)(b8 + o_o);
+  double* part = (double*)(b8 + o_m);
+  if ((rc = check_launch(vcphip_gather_rows(IA, lda, m, k, s, As, c->stream)))) return rc;
+  if ((rc = check_launch(vcphip_launch_drgemm(VCPHIP_DR_II, s, n, k, As, s, IB, ldb, Cp, s, nullptr, 0, c->stream))))
+    return rc;
+  const long long saved = c->oz_nmod;
+  const long long saved_timing = c->timing;
+  c->timing = 0;
+  std::vector<double> hpart(nblocks);
+  int chosen = 0;
+  double chosen_ratio = 0.0;
+  for (int t = 12; t <= VCPHIP_OZ_MAXMOD_IV; ++t) {
+    VcphipOzParams P;
+    if (oz_make_params(k, t, &P)) continue;   // too few moduli for this k
+    c->oz_nmod = t;
+    rc = oz_igemm_dev(c, s, n, k, As, s, IB, ldb, Co, s);
+    if (rc) break;
+    if ((rc = check_launch(vcphip_ratio_max(Co, Cp, (long)s * n, part, nblocks, c->stream)))) break;
+    if (hipMemcpyAsync(hpart.data(), part, sizeof(double) * nblocks, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+      rc = VCPHIP_ERR_DEVICE;
+      break;
+    }
+    double mx = 0.0;
+    for (double v : hpart) mx = std::max(mx, v);
+    chosen = t;
+    chosen_ratio = mx;
+    if (mx <= max_ratio) break;
+  }
+  c->oz_nmod = saved;
+  c->timing = saved_timing;
+  if (rc) return rc;
+  if (!chosen) return VCPHIP_ERR_UNSUPPORTED;
+  tune_put(c, TUNE_IGEMM, size_class(m), size_class(n), size_class(k), 0, chosen);
+  if (nmod) *nmod = chosen;
+  if (ratio) *ratio = chosen_ratio;
+  return VCPHIP_OK;
+}
+
+extern "C" int vcphip_tune_residual_dev(vcphip_ctx* c, int count, const int* n, const double* const* R,
+                                        const int* ldr, const double* const* IA, const int* lda, double* const* C,
+                                        const int* ldc, double* normub_dev, int* per_cu, double* ms) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (count <= 0) return VCPHIP_ERR_ARGUMENT;
+  // build (or reuse) the descriptors of this problem set
+  int rc = vcphip_residual_batched_dev(c, count, n, R, ldr, IA, lda, C, ldc, normub_dev);
+  if (rc) return rc;
+  vcphip_ctx::Resid& rs = c->resid;
+  if (rs.probs.empty() || !rs.dt) return VCPHIP_ERR_UNSUPPORTED;
+  const int ntiles = (int)rs.tiles.size();
+  const int cu = c->num_cu > 0 ? c->num_cu : 256;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return VCPHIP_ERR_DEVICE;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    hipEventDestroy(e0);
+    return VCPHIP_ERR_DEVICE;
+  }
+  int best = 0;
+  double best_t = 0.0;
+  for (int per = 1; per <= 4 && rc == VCPHIP_OK; ++per) {
+    const int grid = std::max(1, std::min(ntiles, per * cu));
+    auto launch = [&]() {
+      return vcphip_launch_resid(ntiles, rs.dt, rs.dp, rs.nrows, rs.drp, rs.drs, count, normub_dev, grid, c->stream);
+    };
+    if (launch() != 0) {
+      rc = VCPHIP_ERR_DEVICE;
+      break;
+    }
+    hipEventRecord(e0, c->stream);
+    const int reps = 5;
+    for (int r = 0; r < reps; ++r) launch();
+    hipEventRecord(e1, c->stream);
+    if (hipEventSynchronize(e1) != hipSuccess) {
+      rc = VCPHIP_ERR_DEVICE;
+      break;
+    }
+    float t = 0.f;
+    hipEventElapsedTime(&t, e0, e1);
+    if (!best || t / reps < best_t) {
+      best = per;
+      best_t = t / reps;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) return rc;
+  c->resid_per_cu = best;
+  // drop the cached graph: the next call captures the launch with the tuned grid
+  if (rs.exec) {
+    hipGraphExecDestroy(rs.exec);
+    rs.exec = nullptr;
+  }
+  if (rs.graph) {
+    hipGraphDestroy(rs.graph);
+    rs.graph = nullptr;
+  }
+  rs.probs.clear();
+  if (per_cu) *per_cu = best;
+  if (ms) *ms = best_t;
+  return VCPHIP_OK;
+}
+
+extern "C" int vcphip_tune_save(vcphip_ctx* c, const char* path) {
+  if (!c || !path) return VCPHIP_ERR_ARGUMENT;
+  FILE* f = fopen(path, "w");
+  if (!f) return VCPHIP_ERR_ARGUMENT;
+  fprintf(f, "vcphip-tune 1\n");
+  for (const auto& t : c->tune)
+    fprintf(f, "%s %d %d %d %d %d\n", t.kind == TUNE_IGEMM ? "igemm" : "ddgemm", t.lm, t.ln, t.lk, t.variant, t.nmod);
+  fprintf(f, "residual %d\n", c->resid_per_cu);
+  const bool ok = fclose(f) == 0;
+  return ok ? VCPHIP_OK : VCPHIP_ERR_ARGUMENT;
+}
+
+extern "C" int vcphip_tune_load(vcphip_ctx* c, const char* path) {
+  if (!c || !path) return VCPHIP_ERR_ARGUMENT;
+  FILE* f = fopen(path, "r");
+  if (!f) return VCPHIP_ERR_ARGUMENT;
+  char line[256];
+  int version = 0;
+  if (!fgets(line, sizeof line, f) || sscanf(line, "vcphip-tune %d", &version) != 1 || version != 1) {
+    fclose(f);
+    return VCPHIP_ERR_ARGUMENT;
+  }
+  std::vector<vcphip_ctx::Tune> table;
+  int per_cu = c->resid_per_cu;
+  int rc = VCPHIP_OK;
+  while (fgets(line, sizeof line, f)) {
+    char kind[32];
+    int lm, ln, lk, v, nm, p;
+    if (sscanf(line, "%31s %d %d %d %d %d", kind, &lm, &ln, &lk, &v, &nm) == 6) {
+      const bool iv = strcmp(kind, "igemm") == 0, dd = strcmp(kind, "ddgemm") == 0;
+      const bool ok = (iv || dd) && lm >= 0 && lm <= 31 && ln >= 0 && ln <= 31 && lk >= 0 && lk <= 31 && v >= 0 &&
+                      v <= 10 && (nm == 0 || (iv && nm >= 8 && nm <= VCPHIP_OZ_MAXMOD_IV) ||
+                                  (dd && nm >= 24 && nm <= VCPHIP_OZ_MAXMOD));
+      if (!ok) {
+        rc = VCPHIP_ERR_ARGUMENT;
+        break;
+      }
+      table.push_back(vcphip_ctx::Tune{iv ? TUNE_IGEMM : TUNE_DDGEMM, lm, ln, lk, v, nm});
+    } else if (sscanf(line, "residual %d", &p) == 1 && p >= 1 && p <= 4) {
+      per_cu = p;
+    } else if (line[0] != '\n' && line[0] != '#') {
+      rc = VCPHIP_ERR_ARGUMENT;
+      break;
+    }
+  }
+  fclose(f);
+  if (rc) return rc;
+  for (const auto& t : table) tune_put(c, t.kind, t.lm, t.ln, t.lk, t.variant, t.nmod);
+  c->resid_per_cu = per_cu;
+  return VCPHIP_OK;
 }

@@ -9,6 +9,8 @@
 enum { VCPHIP_DR_II = 0, VCPHIP_DR_IP = 1, VCPHIP_DR_PI = 2, VCPHIP_DR_PP = 3 };
 
 // plain FP64 directed-rounding GEMM family (kern_drgemm.hip)
+int vcphip_launch_isgemm(int m, int n, int k, const double* A, long lda, const double* B, long ldb, double* C,
+                         long ldc, hipStream_t s);
 int vcphip_launch_drgemm(int kind, int m, int n, int k, const double* A, long lda, const double* B,
                          long ldb, double* C, long ldc, double* C2, int accumulate, hipStream_t s);
 // C = A^T A forms (pidblas::mulltmm / vmulmm), kern_drgemm.hip
@@ -45,14 +47,14 @@ struct VcphipOzParams {
   int nmod;                    // number of moduli N
   int beta;                    // scaled integers satisfy |A'| <= 2^beta (+1 for dd)
   int mod[VCPHIP_OZ_MAXMOD];   // pairwise coprime moduli (mod[0] = 256)
-  float rcpf[VCPHIP_OZ_MAXMOD];
   double rcpd[VCPHIP_OZ_MAXMOD];
   int w21[VCPHIP_OZ_MAXMOD];   // 2^21 mod m_t
   int w42[VCPHIP_OZ_MAXMOD];   // 2^42 mod m_t
-  int pinv[VCPHIP_OZ_MAXMOD];  // (m_0 ... m_{t-1})^{-1} mod m_t
-  uint8_t negP[VCPHIP_OZ_MAXMOD][VCPHIP_OZ_MAXMOD];  // [s][t] = -(m_0 ... m_{s-1}) mod m_t
+  // CRT weights w_t = (M/m_t) ((M/m_t)^-1 mod m_t) < M: [c][g] packs byte c of
+  // w_{4g}, w_{4g+1}, w_{4g+2}, w_{4g+3} (low byte first); and w_t / M
+  uint32_t crt_w8[VCPHIP_OZ_MAXMOD][VCPHIP_OZ_MAXMOD / 4];
+  double crt_f[VCPHIP_OZ_MAXMOD];
   uint8_t w20[VCPHIP_DD_NCHUNK][VCPHIP_OZ_MAXMOD];   // 2^(20 c) mod m_t
-  uint32_t halfM[VCPHIP_OZ_NLIMB];  // M/2 as little-endian 32-bit limbs
   uint32_t Mlimb[VCPHIP_OZ_NLIMB];  // M
   double c1;                   // RU(1/127)
   double c127;                 // RU(1/127^2)
@@ -101,3 +103,7 @@ int vcphip_dd_split_cols(const double* B, long ldb, int k, int n, int npad, int 
                          const int* e, const int* flags, int8_t* planes, hipStream_t s);
 int vcphip_dd_recon(int m, int n, int mpad, int npad, const VcphipOzParams& P, const int* ea, const int* fa,
                     const int* eb, const int* fb, const uint8_t* U, double* C, long ldc, hipStream_t s);
+
+// autotuner helpers (kern_tune.hip)
+int vcphip_gather_rows(const double* A, long lda, int m, int k, int s, double* out, hipStream_t st);
+int vcphip_ratio_max(const double* X, const double* Y, long total, double* part, int nblocks, hipStream_t st);
