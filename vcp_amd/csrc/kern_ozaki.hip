@@ -22,6 +22,7 @@
 #include "vcphip_crt.h"
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
 
 namespace {
 
@@ -758,6 +759,191 @@ __global__ __launch_bounds__(256) void k_oz_gemm4(int mpad, int npad, int kpad, 
   }
 }
 
+// The same pipeline on v_mfma_i32_32x32x32_i8: FM x FN fragments of 32 x 32
+// per wave (4 x 4: a 128 x 128 wave tile, 256 accumulator registers).  With
+// that many accumulators the 32x32x32 form sustains a far higher issue rate
+// than 16x16x64.  Operand lane l holds row (column) l % 32 and the 16 bytes
+// of k-quarter 2 h + l / 32 of the stage's k-half h; the accumulator's
+// register v of lane l is row 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32,
+// so registers 4q..4q+3 are one lane-quad of a 16 x 16 output block.
+template <int FM, int FN>
+struct OzFrag32 {
+  v4i a[2][FM];
+  v4i b[2][FN];
+};
+
+template <int FM, int FN>
+__device__ __forceinline__ void oz_load_frag32(const int8_t* la0, const int8_t* la1, const int8_t* lb0,
+                                               const int8_t* lb1, OzFrag32<FM, FN>& f) {
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+    f.a[0][i] = *reinterpret_cast<const v4i*>(la0 + i * 2048);
+    f.a[1][i] = *reinterpret_cast<const v4i*>(la1 + i * 2048);
+  }
+#pragma unroll
+  for (int j = 0; j < FN; ++j) {
+    f.b[0][j] = *reinterpret_cast<const v4i*>(lb0 + j * 2048);
+    f.b[1][j] = *reinterpret_cast<const v4i*>(lb1 + j * 2048);
+  }
+}
+
+template <int FM, int FN>
+__device__ __forceinline__ void oz_mfma_frag32(const OzFrag32<FM, FN>& f, v16i (&acc)[FM][FN]) {
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+        acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(f.a[h][i], f.b[h][j], acc[i][j], 0, 0, 0);
+}
+
+template <int FM, int FN, int NST>
+__global__ __launch_bounds__(256) void k_oz_gemm5(int mpad, int npad, int kpad, int nmod, int nz,
+                                                  const int8_t* __restrict__ Aplanes,
+                                                  const int8_t* __restrict__ Bplanes, VcphipOzParams P,
+                                                  uint8_t* __restrict__ U, int32_t* __restrict__ Z) {
+  constexpr int WM = 32 * FM, WN = 32 * FN;
+  constexpr int TMB = 2 * WM, TNB = 2 * WN;
+  constexpr int SEGA = TMB / 16, SEGB = TNB / 16;
+  constexpr int SEGA_W = SEGA / 4, SEGB_W = SEGB / 4;
+  constexpr int G = SEGA_W + SEGB_W;
+  static_assert(SEGA_W * 4 == SEGA && SEGB_W * 4 == SEGB, "tile/wave mismatch");
+  static_assert(NST >= 3 && (NST - 1) * G < 64, "vmcnt range");
+  __shared__ __attribute__((aligned(16))) int8_t sA[NST][TMB * 64];
+  __shared__ __attribute__((aligned(16))) int8_t sB[NST][TNB * 64];
+
+  const int ntm = mpad / TMB, ntn = npad / TNB;
+  const int tiles = ntm * ntn;
+  const int z = blockIdx.x / tiles;
+  const int local = vcphip_xcd_remap(blockIdx.x - z * tiles, tiles);
+  int tm, tn;
+  vcphip_tile_coords(local, ntm, ntn, 8, tm, tn);
+  const int row0 = tm * TMB, col0 = tn * TNB;
+  int npass = 1, pa0 = z, pb0 = z, pa1 = z, pb1 = z;
+  if (z >= nmod) {
+    const int r = z - nmod, base = nmod + 2 * (r >> 1);
+    pa0 = pb0 = base;
+    if (r & 1) {
+      npass = 2;
+      pb0 = base + 1;
+      pa1 = base + 1;
+      pb1 = base;
+    }
+  }
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w & 1, wn = w >> 1;
+  const int wu = __builtin_amdgcn_readfirstlane(w);
+  const uint32_t sA_base = lds_addr(&sA[0][0]), sB_base = lds_addr(&sB[0][0]);
+  const int lrow = lane >> 2, lchunk = (lane & 3) ^ ((lane >> 4) & 3);
+  const int src_lane = lrow * kpad + 16 * lchunk;
+  // fragment rows: R = 32 i + lane % 32 (+ a multiple of 32): (R >> 2) & 3 = ((lane % 32) >> 2) & 3
+  const int frow = lane & 31, kq = lane >> 5, sw = (frow >> 2) & 3;
+  const int fs0 = 16 * ((0 + kq) ^ sw), fs1 = 16 * ((2 + kq) ^ sw);
+  const int fa0 = (wm * WM + frow) * 64 + fs0, fa1 = (wm * WM + frow) * 64 + fs1;
+  const int fb0 = (wn * WN + frow) * 64 + fs0, fb1 = (wn * WN + frow) * 64 + fs1;
+
+  v16i acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = v16i{};
+
+  const int nk = kpad / 64;
+  OzFrag32<FM, FN> f0, f1;
+  for (int pass = 0; pass < npass; ++pass) {
+    const int8_t* Ab = Aplanes + (long)(pass ? pa1 : pa0) * mpad * kpad + (long)row0 * kpad + src_lane;
+    const int8_t* Bb = Bplanes + (long)(pass ? pb1 : pb0) * npad * kpad + (long)col0 * kpad + src_lane;
+    auto issue = [&](int stage, int kt) {
+      const long k0 = (long)kt * 64;
+#pragma unroll
+      for (int q = 0; q < SEGA_W; ++q) {
+        const int seg = wu * SEGA_W + q;
+        lds_dma16(Ab + (long)seg * 16 * kpad + k0, sA_base + stage * (TMB * 64) + seg * 1024);
+      }
+#pragma unroll
+      for (int q = 0; q < SEGB_W; ++q) {
+        const int seg = wu * SEGB_W + q;
+        lds_dma16(Bb + (long)seg * 16 * kpad + k0, sB_base + stage * (TNB * 64) + seg * 1024);
+      }
+    };
+    auto wait_next = [&](int kt) {
+      if (kt + NST < nk) asm volatile("s_waitcnt vmcnt(%0)" ::"n"((NST - 2) * G) : "memory");
+      else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    };
+    auto load = [&](int stage, OzFrag32<FM, FN>& f) {
+      oz_load_frag32<FM, FN>(sA[stage] + fa0, sA[stage] + fa1, sB[stage] + fb0, sB[stage] + fb1, f);
+    };
+    const int npro = nk < NST ? nk : NST;
+    for (int st = 0; st < npro; ++st) issue(st, st);
+    if (nk >= NST) asm volatile("s_waitcnt vmcnt(%0)" ::"n"((NST - 1) * G) : "memory");
+    else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    load(0, f0);
+    for (int kt = 0; kt < nk; kt += 2) {
+      if (kt + 1 < nk) {
+        wait_next(kt);
+        if (kt + NST < nk) issue(kt % NST, kt + NST);
+        load((kt + 1) % NST, f1);
+      }
+      oz_mfma_frag32<FM, FN>(f0, acc);
+      if (kt + 1 < nk) {
+        if (kt + 2 < nk) {
+          wait_next(kt + 1);
+          if (kt + 1 + NST < nk) issue((kt + 1) % NST, kt + 1 + NST);
+          load((kt + 2) % NST, f0);
+        }
+        oz_mfma_frag32<FM, FN>(f1, acc);
+      }
+    }
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  }
+
+  // epilogue: registers 4q..4q+3 of fragment (i, j) are lane l2 of the 16 x 16
+  // block (row half q / 2, column half (lane % 32) / 16) of that fragment
+  const long blocks_m = mpad / 16;
+  const int l2 = 16 * (kq) + (lane & 15), chalf = (lane & 31) >> 4;
+  if (z < nmod) {
+    const int md = P.mod[z];
+    const double rc = P.rcpd[z];
+    uint8_t* Uz = U + (long)z * mpad * npad;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const long bi = (row0 + wm * WM + 32 * i) / 16 + (q >> 1);
+          const long bj = (col0 + wn * WN + 32 * j) / 16 + chalf;
+          unsigned packed = 0;
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const double y = (double)acc[i][j][4 * q + t];
+            const double qd = __builtin_floor(y * rc);
+            int rr = (int)(y - qd * (double)md);
+            rr += (rr < 0) ? md : 0;
+            rr -= (rr >= md) ? md : 0;
+            packed |= (unsigned)rr << (8 * t);
+          }
+          *reinterpret_cast<unsigned*>(Uz + (bj * blocks_m + bi) * 256 + (l2 + 32 * (q & 1)) * 4) = packed;
+        }
+  } else {
+    int32_t* Zz = Z + (long)(z - nmod) * mpad * npad;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const long bi = (row0 + wm * WM + 32 * i) / 16 + (q >> 1);
+          const long bj = (col0 + wn * WN + 32 * j) / 16 + chalf;
+          const v4i val = v4i{acc[i][j][4 * q], acc[i][j][4 * q + 1], acc[i][j][4 * q + 2], acc[i][j][4 * q + 3]};
+          *reinterpret_cast<v4i*>(Zz + (bj * blocks_m + bi) * 256 + (l2 + 32 * (q & 1)) * 4) = val;
+        }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // reconstruction
 // ---------------------------------------------------------------------------
@@ -947,6 +1133,18 @@ int vcphip_oz_gemm(int mpad, int npad, int kpad, int nmod, int nrad, const int8_
                            Aplanes, Bplanes, P, U, Z);
       else
         hipLaunchKernelGGL((k_oz_gemm4<4, 8, 4>), dim3(tiles * nz), dim3(256), 0, s, mpad, npad, kpad, nmod, nz,
+                           Aplanes, Bplanes, P, U, Z);
+      break;
+    }
+    case 11:
+    case 12: {
+      if (kpad % 64) return -7;
+      const int tiles = (mpad / 256) * (npad / 256);
+      if (variant == 11)
+        hipLaunchKernelGGL((k_oz_gemm5<4, 4, 4>), dim3(tiles * nz), dim3(256), 0, s, mpad, npad, kpad, nmod, nz,
+                           Aplanes, Bplanes, P, U, Z);
+      else
+        hipLaunchKernelGGL((k_oz_gemm5<4, 4, 3>), dim3(tiles * nz), dim3(256), 0, s, mpad, npad, kpad, nmod, nz,
                            Aplanes, Bplanes, P, U, Z);
       break;
     }
