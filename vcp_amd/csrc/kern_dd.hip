@@ -137,7 +137,7 @@ __global__ __launch_bounds__(256) void k_dd_split_rows(const double* __restrict_
   const long plane_stride = (long)mpad * kpad;
   for (int p = 0; p < P.nmod; ++p) {
     const unsigned* src = reinterpret_cast<const unsigned*>(s_tile + (p * SR_ROWS + orow) * SR_LD + oc);
-    *reinterpret_cast<uint4*>(planes + p * plane_stride + (long)(row0 + orow) * kpad + k0 + oc) =
+    *reinterpret_cast<uint4*>(planes + p * plane_stride + vcphip_plane_off(row0 + orow, k0 + oc, kpad)) =
         make_uint4(src[0], src[1], src[2], src[3]);
   }
 }
@@ -146,9 +146,11 @@ __global__ __launch_bounds__(256) void k_dd_split_rows(const double* __restrict_
 __global__ __launch_bounds__(256) void k_dd_split_cols(const double* __restrict__ B, long ldb, int k, int n, int npad,
                                                        int kpad, VcphipOzParams P, const int* __restrict__ eb,
                                                        const int* __restrict__ fb, int8_t* __restrict__ planes) {
+  // 64 consecutive threads fill one 1 KB plane block: 16 columns x 64 k
   const long gid = (long)blockIdx.x * 256 + threadIdx.x;
-  const int ngrp = kpad / 16;
-  const int j = (int)(gid / ngrp), grp = (int)(gid % ngrp);
+  const long unit = gid >> 6;
+  const int kblk = kpad / 64, tq = (int)(gid & 63);
+  const int j = (int)(unit / kblk) * 16 + (tq & 15), grp = (int)(unit % kblk) * 4 + (tq >> 4);
   if (j >= npad) return;
   const bool colok = j < n && fb[min(j, n - 1)] == 0;
   const int sc = colok ? P.beta - eb[j] : 0;
@@ -168,7 +170,7 @@ __global__ __launch_bounds__(256) void k_dd_split_cols(const double* __restrict_
     chunks20(mag, c[u]);
   }
   const long plane_stride = (long)npad * kpad;
-  int8_t* dst = planes + (long)j * kpad + grp * 16;
+  int8_t* dst = planes + vcphip_plane_off(j, grp * 16, kpad);
   for (int t = 0; t < P.nmod; ++t) {
     unsigned wv[4] = {0, 0, 0, 0};
 #pragma unroll

@@ -181,7 +181,7 @@ VCPHIP_DEVINL void split_parts(long long a, unsigned& c0, unsigned& c1, unsigned
 
 constexpr int SR_ROWS = 32, SR_K = 64, SR_LD = 68;   // LDS byte tile, odd-word row stride
 
-// rows of A (m x k, column-major intervals) -> planes [N+4][mpad][kpad]:
+// rows of A (m x k, column-major intervals) -> N+4 planes of mpad x kpad:
 // N residue planes of A', then the hi/lo digits of |mid|+rad and of rad.
 __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict__ A, long lda, int m, int k,
                                                        int mpad, int kpad, VcphipOzParams P, VcphipOzSide side,
@@ -229,20 +229,23 @@ __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict_
   const long plane_stride = (long)mpad * kpad;
   for (int p = 0; p < nplane; ++p) {
     const unsigned* src = reinterpret_cast<const unsigned*>(s_tile + (p * SR_ROWS + orow) * SR_LD + oc);
-    *reinterpret_cast<uint2*>(planes + p * plane_stride + (long)(row0 + orow) * kpad + k0 + oc) = make_uint2(src[0], src[1]);
+    *reinterpret_cast<uint2*>(planes + p * plane_stride + vcphip_plane_off(row0 + orow, k0 + oc, kpad)) =
+        make_uint2(src[0], src[1]);
   }
 }
 
-// columns of B (k x n, column-major intervals) -> planes [N+4][npad][kpad]:
+// columns of B (k x n, column-major intervals) -> N+4 planes of npad x kpad:
 // N residue planes of B', then the hi/lo digits of rad and of |mid|.  One
 // thread handles 16 consecutive k of one column (a 256-byte contiguous read).
 __global__ __launch_bounds__(256) void k_oz_split_cols(const double* __restrict__ B, long ldb, int k, int n,
                                                        int npad, int kpad, VcphipOzParams P, VcphipOzSide side,
                                                        int8_t* __restrict__ planes) {
   vcphip_mode_round_up();
+  // 64 consecutive threads fill one 1 KB plane block: 16 columns x 64 k
   const long gid = (long)blockIdx.x * 256 + threadIdx.x;
-  const int ngrp = kpad / 16;
-  const int j = (int)(gid / ngrp), grp = (int)(gid % ngrp);
+  const long unit = gid >> 6;
+  const int kblk = kpad / 64, t = (int)(gid & 63);
+  const int j = (int)(unit / kblk) * 16 + (t & 15), grp = (int)(unit % kblk) * 4 + (t >> 4);
   if (j >= npad) return;
   const bool colok = j < n;
   const int f = colok ? side.e[j] : 0;
@@ -273,13 +276,13 @@ __global__ __launch_bounds__(256) void k_oz_split_cols(const double* __restrict_
     dml[u >> 2] |= (d2 >> 8) << (8 * (u & 3));
   }
   const long plane_stride = (long)npad * kpad;
-  int8_t* dst = planes + (long)j * kpad + grp * 16;
-  for (int t = 0; t < P.nmod; ++t) {
+  int8_t* dst = planes + vcphip_plane_off(j, grp * 16, kpad);
+  for (int z = 0; z < P.nmod; ++z) {
     unsigned wv[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int u = 0; u < 16; ++u)
-      wv[u >> 2] |= ((unsigned)(unsigned char)(signed char)sym_residue(c0[u], c1[u], c2[u], ng[u], t, P)) << (8 * (u & 3));
-    *reinterpret_cast<uint4*>(dst + t * plane_stride) = make_uint4(wv[0], wv[1], wv[2], wv[3]);
+      wv[u >> 2] |= ((unsigned)(unsigned char)(signed char)sym_residue(c0[u], c1[u], c2[u], ng[u], z, P)) << (8 * (u & 3));
+    *reinterpret_cast<uint4*>(dst + z * plane_stride) = make_uint4(wv[0], wv[1], wv[2], wv[3]);
   }
   *reinterpret_cast<uint4*>(dst + P.nmod * plane_stride) = make_uint4(drh[0], drh[1], drh[2], drh[3]);
   *reinterpret_cast<uint4*>(dst + (P.nmod + 1) * plane_stride) = make_uint4(drl[0], drl[1], drl[2], drl[3]);
@@ -290,161 +293,15 @@ __global__ __launch_bounds__(256) void k_oz_split_cols(const double* __restrict_
 // ---------------------------------------------------------------------------
 // int8 MFMA GEMM over the planes: C_z = A_z * B_z^T (both K-contiguous)
 // ---------------------------------------------------------------------------
+// Plane layout (both operands, written by the split kernels): 1 KB blocks of
+// 16 rows x 64 k, row-major inside the block, blocks ordered by row group and
+// then by k (vcphip_plane_off), so that one wave's load of 16 rows x 64 k is
+// one contiguous kilobyte.
 // Output layout is blocked by 16x16 MFMA tiles: block (bi, bj) occupies
 // 256 consecutive elements, element (4*(l/16)+v, l%16) of the block at l*4+v,
 // which is exactly the v_mfma_i32_16x16x64_i8 accumulator order, so every
 // wave stores whole contiguous blocks.
-// Workgroup: 4 waves as 2 x 2, wave tile (16 FM) x (16 FN), k stage of BK
-// bytes; operands staged global -> registers -> LDS with a double-buffered
-// LDS and the next stage's global loads in flight during the MFMAs.
-template <int BK>
-struct GemmLds {
-  static constexpr int LD = BK + 16;   // row stride (bytes): conflict-free 16-byte fragment reads
-};
-
-template <int FM, int FN, int BK>
-__global__ __launch_bounds__(256) void k_oz_gemm(int mpad, int npad, int kpad, int nmod, int nz,
-                                                 const int8_t* __restrict__ Aplanes,
-                                                 const int8_t* __restrict__ Bplanes, VcphipOzParams P,
-                                                 uint8_t* __restrict__ U, int32_t* __restrict__ Z) {
-  constexpr int WM = 16 * FM, WN = 16 * FN;    // wave tile
-  constexpr int TMB = 2 * WM, TNB = 2 * WN;    // workgroup tile (2 x 2 waves)
-  constexpr int LD = GemmLds<BK>::LD;
-  constexpr int CPR = BK / 16;                 // 16-byte chunks per row and stage
-  constexpr int ACH = TMB * CPR / 256;         // chunks of A per thread per stage
-  constexpr int BCH = TNB * CPR / 256;
-  static_assert(ACH * 256 == TMB * CPR && BCH * 256 == TNB * CPR, "tile/thread mismatch");
-  __shared__ __attribute__((aligned(16))) int8_t sA[2][TMB * LD];
-  __shared__ __attribute__((aligned(16))) int8_t sB[2][TNB * LD];
-
-  const int ntm = mpad / TMB, ntn = npad / TNB;
-  const int tiles = ntm * ntn;
-  const int z = blockIdx.x / tiles;
-  const int local = vcphip_xcd_remap(blockIdx.x - z * tiles, tiles);
-  int tm, tn;
-  vcphip_tile_coords(local, ntm, ntn, 8, tm, tn);
-  const int row0 = tm * TMB, col0 = tn * TNB;
-  // plane pairs of product z: a residue pair, a hi*hi digit pair, or the
-  // two cross pairs hi*lo + lo*hi of a radius factor accumulated together
-  int npass = 1, pa0 = z, pb0 = z, pa1 = z, pb1 = z;
-  if (z >= nmod) {
-    const int r = z - nmod, base = nmod + 2 * (r >> 1);
-    pa0 = pb0 = base;
-    if (r & 1) {
-      npass = 2;
-      pb0 = base + 1;
-      pa1 = base + 1;
-      pb1 = base;
-    }
-  }
-
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int wm = w & 1, wn = w >> 1;
-  // per-thread global/LDS chunk coordinates (same for every stage)
-  // chunk q of this thread: row tid/CPR + q*RPQ, 16-byte part tid%CPR
-  constexpr int RPQ = 256 / CPR;
-  const int g_off = (tid / CPR) * kpad + 16 * (tid % CPR);
-  const int l_off = (tid / CPR) * LD + 16 * (tid % CPR);
-  const long g_step = (long)RPQ * kpad;
-#define A_OFF(q) (g_off + (q) * g_step)
-#define L_OFF(q) (l_off + (q) * (RPQ * LD))
-  const int frow = lane & 15, fk = (lane >> 4) * 16;
-  const int fa_lds = (wm * WM + frow) * LD + fk;
-  const int fb_lds = (wn * WN + frow) * LD + fk;
-
-  v4i acc[FM][FN];
-#pragma unroll
-  for (int i = 0; i < FM; ++i)
-#pragma unroll
-    for (int j = 0; j < FN; ++j) acc[i][j] = v4i{0, 0, 0, 0};
-
-  const int nk = kpad / BK;
-  v4i ra[ACH], rb[BCH];
-  for (int pass = 0; pass < npass; ++pass) {
-    const int8_t* Ab = Aplanes + (long)(pass ? pa1 : pa0) * mpad * kpad + (long)row0 * kpad;
-    const int8_t* Bb = Bplanes + (long)(pass ? pb1 : pb0) * npad * kpad + (long)col0 * kpad;
-#pragma unroll
-    for (int q = 0; q < ACH; ++q) ra[q] = *reinterpret_cast<const v4i*>(Ab + A_OFF(q));
-#pragma unroll
-    for (int q = 0; q < BCH; ++q) rb[q] = *reinterpret_cast<const v4i*>(Bb + A_OFF(q));
-#pragma unroll
-    for (int q = 0; q < ACH; ++q) *reinterpret_cast<v4i*>(&sA[0][L_OFF(q)]) = ra[q];
-#pragma unroll
-    for (int q = 0; q < BCH; ++q) *reinterpret_cast<v4i*>(&sB[0][L_OFF(q)]) = rb[q];
-    __syncthreads();
-    for (int kt = 0; kt < nk; ++kt) {
-      const int buf = kt & 1;
-      const bool more = kt + 1 < nk;
-      if (more) {
-        const int k1 = (kt + 1) * BK;
-#pragma unroll
-        for (int q = 0; q < ACH; ++q) ra[q] = *reinterpret_cast<const v4i*>(Ab + A_OFF(q) + k1);
-#pragma unroll
-        for (int q = 0; q < BCH; ++q) rb[q] = *reinterpret_cast<const v4i*>(Bb + A_OFF(q) + k1);
-      }
-      const int8_t* la = sA[buf];
-      const int8_t* lb = sB[buf];
-#pragma unroll
-      for (int ks = 0; ks < BK / 64; ++ks) {
-        v4i fa[FM], fb[FN];
-#pragma unroll
-        for (int i = 0; i < FM; ++i) fa[i] = *reinterpret_cast<const v4i*>(la + fa_lds + 16 * i * LD + 64 * ks);
-#pragma unroll
-        for (int j = 0; j < FN; ++j) fb[j] = *reinterpret_cast<const v4i*>(lb + fb_lds + 16 * j * LD + 64 * ks);
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-          for (int j = 0; j < FN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[j], acc[i][j], 0, 0, 0);
-      }
-      if (more) {
-        int8_t* sa = sA[buf ^ 1];
-        int8_t* sb = sB[buf ^ 1];
-#pragma unroll
-        for (int q = 0; q < ACH; ++q) *reinterpret_cast<v4i*>(sa + L_OFF(q)) = ra[q];
-#pragma unroll
-        for (int q = 0; q < BCH; ++q) *reinterpret_cast<v4i*>(sb + L_OFF(q)) = rb[q];
-      }
-      __syncthreads();
-    }
-  }
-#undef A_OFF
-#undef L_OFF
-
-  const long blocks_m = mpad / 16;
-  if (z < nmod) {
-    const int md = P.mod[z];
-    const double rc = P.rcpd[z];
-    uint8_t* Uz = U + (long)z * mpad * npad;
-#pragma unroll
-    for (int i = 0; i < FM; ++i)
-#pragma unroll
-      for (int j = 0; j < FN; ++j) {
-        const long bi = (row0 + wm * WM + 16 * i) / 16, bj = (col0 + wn * WN + 16 * j) / 16;
-        unsigned packed = 0;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const double y = (double)acc[i][j][v];
-          const double qd = __builtin_floor(y * rc);
-          int rr = (int)(y - qd * (double)md);
-          rr += (rr < 0) ? md : 0;
-          rr -= (rr >= md) ? md : 0;
-          packed |= (unsigned)rr << (8 * v);
-        }
-        *reinterpret_cast<unsigned*>(Uz + (bj * blocks_m + bi) * 256 + lane * 4) = packed;
-      }
-  } else {
-    int32_t* Zz = Z + (long)(z - nmod) * mpad * npad;
-#pragma unroll
-    for (int i = 0; i < FM; ++i)
-#pragma unroll
-      for (int j = 0; j < FN; ++j) {
-        const long bi = (row0 + wm * WM + 16 * i) / 16, bj = (col0 + wn * WN + 16 * j) / 16;
-        *reinterpret_cast<v4i*>(Zz + (bj * blocks_m + bi) * 256 + lane * 4) = acc[i][j];
-      }
-  }
-}
-
+// Workgroup: 4 waves as 2 x 2.
 // LDS byte address of a __shared__ object.
 __device__ __forceinline__ uint32_t lds_addr(const void* p) {
   return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
@@ -499,7 +356,8 @@ __global__ __launch_bounds__(256) void k_oz_gemm3(int mpad, int npad, int kpad, 
   const uint32_t sA_base = lds_addr(&sA[0][0]), sB_base = lds_addr(&sB[0][0]);
   // this lane's source in every 1 KB segment: row lane/4, logical chunk
   const int lrow = lane >> 2, lchunk = (lane & 3) ^ ((lane >> 4) & 3);
-  const int src_lane = lrow * kpad + 16 * lchunk;
+  const int src_lane = lrow * 64 + 16 * lchunk;   // inside the 1 KB block of 16 rows x 64 k
+  const long kblk = kpad / 64;
   // fragment reads: row R, logical chunk g = lane / 16 -> physical g ^ ((R >> 2) & 3)
   const int frow = lane & 15, g = lane >> 4;
   int fa_off[FM], fb_off[FN];
@@ -522,23 +380,22 @@ __global__ __launch_bounds__(256) void k_oz_gemm3(int mpad, int npad, int kpad, 
 
   const int nk = kpad / 64;
   for (int pass = 0; pass < npass; ++pass) {
-    const int8_t* Ab = Aplanes + (long)(pass ? pa1 : pa0) * mpad * kpad + (long)row0 * kpad + src_lane;
-    const int8_t* Bb = Bplanes + (long)(pass ? pb1 : pb0) * npad * kpad + (long)col0 * kpad + src_lane;
+    const int8_t* Ab = Aplanes + (long)(pass ? pa1 : pa0) * mpad * kpad + (long)(row0 / 16) * kblk * 1024 + src_lane;
+    const int8_t* Bb = Bplanes + (long)(pass ? pb1 : pb0) * npad * kpad + (long)(col0 / 16) * kblk * 1024 + src_lane;
     // The LDS-DMA loads, their wait and the barrier are inline asm, so the
     // compiler neither tracks the DMA (it would wait for all of it before
     // every LDS read) nor fences the barrier; the "memory" clobbers keep the
     // LDS reads between them.
     auto issue = [&](int stage, int kt) {
-      const long k0 = (long)kt * 64;
 #pragma unroll
       for (int q = 0; q < SEGA_W; ++q) {
         const int seg = wu * SEGA_W + q;
-        lds_dma16(Ab + (long)seg * 16 * kpad + k0, sA_base + stage * (TMB * 64) + seg * 1024);
+        lds_dma16(Ab + ((long)seg * kblk + kt) * 1024, sA_base + stage * (TMB * 64) + seg * 1024);
       }
 #pragma unroll
       for (int q = 0; q < SEGB_W; ++q) {
         const int seg = wu * SEGB_W + q;
-        lds_dma16(Bb + (long)seg * 16 * kpad + k0, sB_base + stage * (TNB * 64) + seg * 1024);
+        lds_dma16(Bb + ((long)seg * kblk + kt) * 1024, sB_base + stage * (TNB * 64) + seg * 1024);
       }
     };
     issue(0, 0);
@@ -626,7 +483,7 @@ __device__ __forceinline__ void oz_mfma_frag(const OzFrag<FM, FN>& f, v4i (&acc)
     for (int i = 0; i < FM; ++i) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f.a[i], f.b[j], acc[i][j], 0, 0, 0);
 }
 
-template <int FM, int FN, int NST>
+template <int FM, int FN, int NST, bool IL = false>
 __global__ __launch_bounds__(256) void k_oz_gemm4(int mpad, int npad, int kpad, int nmod, int nz,
                                                   const int8_t* __restrict__ Aplanes,
                                                   const int8_t* __restrict__ Bplanes, VcphipOzParams P,
@@ -664,7 +521,8 @@ __global__ __launch_bounds__(256) void k_oz_gemm4(int mpad, int npad, int kpad, 
   const int wu = __builtin_amdgcn_readfirstlane(w);
   const uint32_t sA_base = lds_addr(&sA[0][0]), sB_base = lds_addr(&sB[0][0]);
   const int lrow = lane >> 2, lchunk = (lane & 3) ^ ((lane >> 4) & 3);
-  const int src_lane = lrow * kpad + 16 * lchunk;
+  const int src_lane = lrow * 64 + 16 * lchunk;   // inside the 1 KB block of 16 rows x 64 k
+  const long kblk = kpad / 64;
   const int frow = lane & 15, g = lane >> 4;
   const int fsw = 16 * (g ^ ((frow >> 2) & 3));
   const int fa0 = (wm * WM + frow) * 64 + fsw, fb0 = (wn * WN + frow) * 64 + fsw;
@@ -678,19 +536,18 @@ __global__ __launch_bounds__(256) void k_oz_gemm4(int mpad, int npad, int kpad, 
   const int nk = kpad / 64;
   OzFrag<FM, FN> f0, f1;
   for (int pass = 0; pass < npass; ++pass) {
-    const int8_t* Ab = Aplanes + (long)(pass ? pa1 : pa0) * mpad * kpad + (long)row0 * kpad + src_lane;
-    const int8_t* Bb = Bplanes + (long)(pass ? pb1 : pb0) * npad * kpad + (long)col0 * kpad + src_lane;
+    const int8_t* Ab = Aplanes + (long)(pass ? pa1 : pa0) * mpad * kpad + (long)(row0 / 16) * kblk * 1024 + src_lane;
+    const int8_t* Bb = Bplanes + (long)(pass ? pb1 : pb0) * npad * kpad + (long)(col0 / 16) * kblk * 1024 + src_lane;
     auto issue = [&](int stage, int kt) {
-      const long k0 = (long)kt * 64;
 #pragma unroll
       for (int q = 0; q < SEGA_W; ++q) {
         const int seg = wu * SEGA_W + q;
-        lds_dma16(Ab + (long)seg * 16 * kpad + k0, sA_base + stage * (TMB * 64) + seg * 1024);
+        lds_dma16(Ab + ((long)seg * kblk + kt) * 1024, sA_base + stage * (TMB * 64) + seg * 1024);
       }
 #pragma unroll
       for (int q = 0; q < SEGB_W; ++q) {
         const int seg = wu * SEGB_W + q;
-        lds_dma16(Bb + (long)seg * 16 * kpad + k0, sB_base + stage * (TNB * 64) + seg * 1024);
+        lds_dma16(Bb + ((long)seg * kblk + kt) * 1024, sB_base + stage * (TNB * 64) + seg * 1024);
       }
     };
     // Waits until stage kt+1 (or stage 0 when kt = -1) has landed for every
@@ -699,6 +556,36 @@ __global__ __launch_bounds__(256) void k_oz_gemm4(int mpad, int npad, int kpad, 
       if (kt + NST < nk) asm volatile("s_waitcnt vmcnt(%0)" ::"n"((NST - 2) * G) : "memory");
       else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
       asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    };
+    // one of the G loads of a stage (the first SEGA_W fill A, the rest B)
+    auto issue_one = [&](int stage, int kt, int g) {
+      if (g < SEGA_W) {
+        const int seg = wu * SEGA_W + g;
+        lds_dma16(Ab + ((long)seg * kblk + kt) * 1024, sA_base + stage * (TMB * 64) + seg * 1024);
+      } else {
+        const int seg = wu * SEGB_W + (g - SEGA_W);
+        lds_dma16(Bb + ((long)seg * kblk + kt) * 1024, sB_base + stage * (TNB * 64) + seg * 1024);
+      }
+    };
+    // MFMAs of one stage; with IL the stage's refill loads are spread over
+    // them (one per group of FM MFMAs), so that the load queue never drains
+    // while the wave issues MFMAs
+    auto compute = [&](const OzFrag<FM, FN>& f, int kt) {
+      if (!IL) {
+        if (kt + NST < nk) issue(kt % NST, kt + NST);
+        oz_mfma_frag<FM, FN>(f, acc);
+        return;
+      }
+      const bool refill = kt + NST < nk;
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+          acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f.a[i], f.b[j], acc[i][j], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        if (refill && (j * G) / FN != ((j + 1) * G) / FN) issue_one(kt % NST, kt + NST, (j * G) / FN);
+        __builtin_amdgcn_sched_barrier(0);
+      }
     };
     const int npro = nk < NST ? nk : NST;
     for (int st = 0; st < npro; ++st) issue(st, st);
@@ -709,17 +596,15 @@ __global__ __launch_bounds__(256) void k_oz_gemm4(int mpad, int npad, int kpad, 
     for (int kt = 0; kt < nk; kt += 2) {
       if (kt + 1 < nk) {
         wait_next(kt);
-        if (kt + NST < nk) issue(kt % NST, kt + NST);
         oz_load_frag<FM, FN>(sA[(kt + 1) % NST] + fa0, sB[(kt + 1) % NST] + fb0, f1);
       }
-      oz_mfma_frag<FM, FN>(f0, acc);
+      compute(f0, kt);
       if (kt + 1 < nk) {
         if (kt + 2 < nk) {
           wait_next(kt + 1);
-          if (kt + 1 + NST < nk) issue((kt + 1) % NST, kt + 1 + NST);
           oz_load_frag<FM, FN>(sA[(kt + 2) % NST] + fa0, sB[(kt + 2) % NST] + fb0, f0);
         }
-        oz_mfma_frag<FM, FN>(f1, acc);
+        compute(f1, kt + 1);
       }
     }
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
@@ -836,7 +721,8 @@ __global__ __launch_bounds__(256) void k_oz_gemm5(int mpad, int npad, int kpad, 
   const int wu = __builtin_amdgcn_readfirstlane(w);
   const uint32_t sA_base = lds_addr(&sA[0][0]), sB_base = lds_addr(&sB[0][0]);
   const int lrow = lane >> 2, lchunk = (lane & 3) ^ ((lane >> 4) & 3);
-  const int src_lane = lrow * kpad + 16 * lchunk;
+  const int src_lane = lrow * 64 + 16 * lchunk;   // inside the 1 KB block of 16 rows x 64 k
+  const long kblk = kpad / 64;
   // fragment rows: R = 32 i + lane % 32 (+ a multiple of 32): (R >> 2) & 3 = ((lane % 32) >> 2) & 3
   const int frow = lane & 31, kq = lane >> 5, sw = (frow >> 2) & 3;
   const int fs0 = 16 * ((0 + kq) ^ sw), fs1 = 16 * ((2 + kq) ^ sw);
@@ -852,19 +738,18 @@ __global__ __launch_bounds__(256) void k_oz_gemm5(int mpad, int npad, int kpad, 
   const int nk = kpad / 64;
   OzFrag32<FM, FN> f0, f1;
   for (int pass = 0; pass < npass; ++pass) {
-    const int8_t* Ab = Aplanes + (long)(pass ? pa1 : pa0) * mpad * kpad + (long)row0 * kpad + src_lane;
-    const int8_t* Bb = Bplanes + (long)(pass ? pb1 : pb0) * npad * kpad + (long)col0 * kpad + src_lane;
+    const int8_t* Ab = Aplanes + (long)(pass ? pa1 : pa0) * mpad * kpad + (long)(row0 / 16) * kblk * 1024 + src_lane;
+    const int8_t* Bb = Bplanes + (long)(pass ? pb1 : pb0) * npad * kpad + (long)(col0 / 16) * kblk * 1024 + src_lane;
     auto issue = [&](int stage, int kt) {
-      const long k0 = (long)kt * 64;
 #pragma unroll
       for (int q = 0; q < SEGA_W; ++q) {
         const int seg = wu * SEGA_W + q;
-        lds_dma16(Ab + (long)seg * 16 * kpad + k0, sA_base + stage * (TMB * 64) + seg * 1024);
+        lds_dma16(Ab + ((long)seg * kblk + kt) * 1024, sA_base + stage * (TMB * 64) + seg * 1024);
       }
 #pragma unroll
       for (int q = 0; q < SEGB_W; ++q) {
         const int seg = wu * SEGB_W + q;
-        lds_dma16(Bb + (long)seg * 16 * kpad + k0, sB_base + stage * (TNB * 64) + seg * 1024);
+        lds_dma16(Bb + ((long)seg * kblk + kt) * 1024, sB_base + stage * (TNB * 64) + seg * 1024);
       }
     };
     auto wait_next = [&](int kt) {
@@ -1093,19 +978,7 @@ int vcphip_oz_gemm(int mpad, int npad, int kpad, int nmod, int nrad, const int8_
                    const int8_t* Bplanes, const VcphipOzParams& P, uint8_t* U, int32_t* Z, int variant,
                    hipStream_t s) {
   const int nz = nmod + nrad;
-#define VCPHIP_GEMM_LAUNCH(FM, FN, BKK)                                                                          \
-  do {                                                                                                           \
-    if (kpad % (BKK)) return -7;                                                                                 \
-    const int tiles = (mpad / (32 * (FM))) * (npad / (32 * (FN)));                                               \
-    hipLaunchKernelGGL((k_oz_gemm<FM, FN, BKK>), dim3(tiles * nz), dim3(256), 0, s, mpad, npad, kpad, nmod, nz, \
-                       Aplanes, Bplanes, P, U, Z);                                                               \
-  } while (0)
   switch (variant) {
-    case 1: VCPHIP_GEMM_LAUNCH(8, 8, 128); break;
-    case 2: VCPHIP_GEMM_LAUNCH(4, 4, 128); break;
-    case 3: VCPHIP_GEMM_LAUNCH(4, 8, 128); break;
-    case 4: VCPHIP_GEMM_LAUNCH(8, 8, 64); break;
-    case 5: VCPHIP_GEMM_LAUNCH(4, 4, 64); break;
     case 6:
     case 7: {
       const int fm = variant == 6 ? 8 : 4, fn = variant == 6 ? 8 : 8;
@@ -1136,6 +1009,18 @@ int vcphip_oz_gemm(int mpad, int npad, int kpad, int nmod, int nrad, const int8_
                            Aplanes, Bplanes, P, U, Z);
       break;
     }
+    case 13:
+    case 14: {
+      if (kpad % 64) return -7;
+      const int tiles = (mpad / 256) * (npad / 256);
+      if (variant == 13)
+        hipLaunchKernelGGL((k_oz_gemm4<8, 8, 4, true>), dim3(tiles * nz), dim3(256), 0, s, mpad, npad, kpad, nmod, nz,
+                           Aplanes, Bplanes, P, U, Z);
+      else
+        hipLaunchKernelGGL((k_oz_gemm4<8, 8, 3, true>), dim3(tiles * nz), dim3(256), 0, s, mpad, npad, kpad, nmod, nz,
+                           Aplanes, Bplanes, P, U, Z);
+      break;
+    }
     case 11:
     case 12: {
       if (kpad % 64) return -7;
@@ -1150,7 +1035,6 @@ int vcphip_oz_gemm(int mpad, int npad, int kpad, int nmod, int nrad, const int8_
     }
     default: return -7;
   }
-#undef VCPHIP_GEMM_LAUNCH
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

@@ -76,3 +76,9 @@ VCPHIP_DEVINL void vcphip_tile_coords(int tile, int ntm, int ntn, int GH, int& t
   tm = group * GH + idx % gh;
   tn = idx / gh;
 }
+
+// byte offset of (row, k) in an int8 operand plane of the Ozaki GEMMs: 1 KB
+// blocks of 16 rows x 64 k (row-major inside), ordered by row group, then k
+VCPHIP_DEVINL long vcphip_plane_off(long row, int k, int kpad) {
+  return ((row >> 4) * (long)(kpad >> 6) + (k >> 6)) * 1024 + (row & 15) * 64 + (k & 63);
+}

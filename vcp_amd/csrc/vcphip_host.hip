@@ -371,7 +371,7 @@ extern "C" int vcphip_ctx_set_param(vcphip_ctx* c, int key, long long v) {
       c->oz_nmod = v;
       return VCPHIP_OK;
     case VCPHIP_PARAM_OZ_VARIANT:
-      if (v < 0 || v > 12) return VCPHIP_ERR_ARGUMENT;
+      if (v != 0 && (v < 6 || v > 14)) return VCPHIP_ERR_ARGUMENT;
       c->oz_variant = v;
       return VCPHIP_OK;
     case VCPHIP_PARAM_STAGE_TIMING:   // (re)enabling clears the totals
@@ -932,7 +932,7 @@ extern "C" int vcphip_residual_batched(vcphip_ctx* c, int count, const int* n, c
 // ===========================================================================
 namespace {
 
-constexpr int TUNE_VARIANTS[] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12};
+constexpr int TUNE_VARIANTS[] = {6, 7, 8, 9, 10, 11, 12, 13, 14};
 
 // time the int8 MFMA stage of one Ozaki product shape with every tile variant
 // (operand planes of arbitrary content: the integer MFMA time does not depend
@@ -1173,8 +1173,8 @@ extern "C" int vcphip_tune_load(vcphip_ctx* c, const char* path) {
     int lm, ln, lk, v, nm, p;
     if (sscanf(line, "%31s %d %d %d %d %d", kind, &lm, &ln, &lk, &v, &nm) == 6) {
       const bool iv = strcmp(kind, "igemm") == 0, dd = strcmp(kind, "ddgemm") == 0;
-      const bool ok = (iv || dd) && lm >= 0 && lm <= 31 && ln >= 0 && ln <= 31 && lk >= 0 && lk <= 31 && v >= 0 &&
-                      v <= 12 && (nm == 0 || (iv && nm >= 8 && nm <= VCPHIP_OZ_MAXMOD_IV) ||
+      const bool ok = (iv || dd) && lm >= 0 && lm <= 31 && ln >= 0 && ln <= 31 && lk >= 0 && lk <= 31 &&
+                      (v == 0 || (v >= 6 && v <= 14)) && (nm == 0 || (iv && nm >= 8 && nm <= VCPHIP_OZ_MAXMOD_IV) ||
                                   (dd && nm >= 24 && nm <= VCPHIP_OZ_MAXMOD));
       if (!ok) {
         rc = VCPHIP_ERR_ARGUMENT;
