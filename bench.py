@@ -153,6 +153,8 @@ def main():
     del rad_o, rad_p, ratio
     containment = None
     try:
+        if args.samples <= 0:
+            raise RuntimeError("no samples requested")
         import oracle_ffi as O
 
         C = dC.cpu().numpy().transpose(1, 0, 2)
@@ -227,6 +229,8 @@ def main():
                  "stages_ms": {k: round(x, 4) for k, x in st_tu.items()},
                  "radius_ratio_over_plain": {"max": float(r_t.max()), "median": float(r_t.median())}}
         try:
+            if args.samples <= 0:
+                raise RuntimeError("no samples requested")
             import oracle_ffi as O
 
             Ct = dT.cpu().numpy().transpose(1, 0, 2)

@@ -41,7 +41,35 @@ __global__ __launch_bounds__(256) void k_ratio_max(const double2* __restrict__ X
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// bytes of a hash of (seed, index): stand-in operand planes for timing (the
+// integer MFMA draws more power, and may clock lower, on random data than on
+// the zeros of fresh memory)
+__global__ __launch_bounds__(256) void k_fill_random(uint4* __restrict__ p, long n16, uint32_t seed) {
+  for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n16; t += (long)gridDim.x * 256) {
+    uint32_t w[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      uint32_t x = (uint32_t)(4 * t + q) * 0x9E3779B9u ^ seed;
+      x ^= x >> 16;
+      x *= 0x85EBCA6Bu;
+      x ^= x >> 13;
+      x *= 0xC2B2AE35u;
+      x ^= x >> 16;
+      w[q] = x;
+    }
+    p[t] = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+}
+
 }  // namespace
+
+int vcphip_fill_random(void* p, size_t bytes, uint32_t seed, hipStream_t st) {
+  const long n16 = (long)(bytes / 16);
+  if (n16 <= 0) return 0;
+  const int grid = (int)std::min<long>((n16 + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_fill_random, dim3(grid), dim3(256), 0, st, (uint4*)p, n16, seed);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
 
 int vcphip_gather_rows(const double* A, long lda, int m, int k, int s, double* out, hipStream_t st) {
   if (s <= 0 || k <= 0) return 0;

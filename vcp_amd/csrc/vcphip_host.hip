@@ -963,6 +963,10 @@ int tune_gemm_stage(vcphip_ctx* c, bool dd, int m, int n, int k, int* best_varia
   if ((rc = ws_get(c, vcphip_ctx::WS_U, (size_t)nmod * mpad * npad, &pu))) return rc;
   if (!dd && (rc = ws_get(c, vcphip_ctx::WS_Z, VCPHIP_OZ_NDIGIT * sizeof(int32_t) * (size_t)mpad * npad, &pz)))
     return rc;
+  // operand planes of random bytes: timings on the zeros of fresh memory do
+  // not carry over to real operands
+  if ((rc = check_launch(vcphip_fill_random(pa, nplane * mpad * kpad, 0x5EED0001u, c->stream)))) return rc;
+  if ((rc = check_launch(vcphip_fill_random(pb, nplane * npad * kpad, 0x5EED0002u, c->stream)))) return rc;
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess) return VCPHIP_ERR_DEVICE;
   if (hipEventCreate(&e1) != hipSuccess) {

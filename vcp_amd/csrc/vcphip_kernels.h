@@ -107,3 +107,4 @@ int vcphip_dd_recon(int m, int n, int mpad, int npad, const VcphipOzParams& P, c
 // autotuner helpers (kern_tune.hip)
 int vcphip_gather_rows(const double* A, long lda, int m, int k, int s, double* out, hipStream_t st);
 int vcphip_ratio_max(const double* X, const double* Y, long total, double* part, int nblocks, hipStream_t st);
+int vcphip_fill_random(void* p, size_t bytes, uint32_t seed, hipStream_t st);
