@@ -257,27 +257,49 @@ __global__ __launch_bounds__(256) void k_resid_rowsum(int nrows, const int* __re
   const int pi = row_problem[g];
   const VcphipResidProblem p = probs[pi];
   const int i = g - p.row_base;
+  const double* row = p.C + 2 * (long)i;
+  const long step = 2 * (long)p.ldc;
+  // the upward sum runs in column order, as the reference's; the loads of
+  // eight columns are issued ahead of their additions
   double s = 0.0;
-  for (int j = 0; j < p.n; ++j) {
-    const double2 v = *reinterpret_cast<const double2*>(p.C + 2 * (i + p.ldc * (long)j));
+  int j = 0;
+  for (; j + 8 <= p.n; j += 8) {
+    double2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = *reinterpret_cast<const double2*>(row + step * (j + u));
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const double mag = fmax(__builtin_fabs(v[u].x), __builtin_fabs(v[u].y));
+      s = (j + u == 0) ? mag : s + mag;
+    }
+  }
+  for (; j < p.n; ++j) {
+    const double2 v = *reinterpret_cast<const double2*>(row + step * j);
     const double mag = fmax(__builtin_fabs(v.x), __builtin_fabs(v.y));
     s = (j == 0) ? mag : s + mag;
   }
   rowsum[g] = s;
 }
 
-// ||C_p||_inf upper bound: the largest row sum (mats.hpp:1592-1598)
+// ||C_p||_inf upper bound: the largest row sum (mats.hpp:1592-1598).  One
+// workgroup per problem; the maximum is exact in any order.
 __global__ __launch_bounds__(256) void k_resid_max(int count, const VcphipResidProblem* __restrict__ probs,
                                                    const double* __restrict__ rowsum, double* __restrict__ normub) {
-  const int pi = blockIdx.x * 256 + threadIdx.x;
-  if (pi >= count) return;
+  __shared__ double red[256];
+  const int pi = blockIdx.x;
   const VcphipResidProblem p = probs[pi];
-  double best = 0.0;
-  for (int i = 0; i < p.n; ++i) {
+  double best = -1.0;   // row sums are >= 0
+  for (int i = threadIdx.x; i < p.n; i += 256) {
     const double v = rowsum[p.row_base + i];
-    best = (i == 0 || v > best) ? v : best;
+    best = v > best ? v : best;
   }
-  normub[pi] = best;
+  red[threadIdx.x] = best;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] = red[threadIdx.x + h] > red[threadIdx.x] ? red[threadIdx.x + h] : red[threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) normub[pi] = p.n > 0 ? red[0] : 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -555,6 +577,6 @@ int vcphip_launch_resid(int ntiles, const VcphipResidTile* tiles, const VcphipRe
                         hipStream_t s) {
   if (ntiles > 0) hipLaunchKernelGGL(k_resid_tiles, dim3(grid), dim3(NT), 0, s, ntiles, tiles, probs);
   if (nrows > 0) hipLaunchKernelGGL(k_resid_rowsum, dim3((nrows + 255) / 256), dim3(256), 0, s, nrows, row_problem, probs, rowsum);
-  if (count > 0) hipLaunchKernelGGL(k_resid_max, dim3((count + 255) / 256), dim3(256), 0, s, count, probs, rowsum, normub);
+  if (count > 0) hipLaunchKernelGGL(k_resid_max, dim3(count), dim3(256), 0, s, count, probs, rowsum, normub);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }

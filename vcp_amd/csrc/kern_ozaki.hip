@@ -183,12 +183,13 @@ constexpr int SR_ROWS = 32, SR_K = 64, SR_LD = 68;   // LDS byte tile, odd-word 
 
 // rows of A (m x k, column-major intervals) -> N+4 planes of mpad x kpad:
 // N residue planes of A', then the hi/lo digits of |mid|+rad and of rad.
+template <int N>
 __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict__ A, long lda, int m, int k,
                                                        int mpad, int kpad, VcphipOzParams P, VcphipOzSide side,
                                                        int8_t* __restrict__ planes) {
   vcphip_mode_round_up();
   extern __shared__ __attribute__((aligned(16))) unsigned char s_tile[];   // [N+4][64][SR_LD]
-  const int nplane = P.nmod + VCPHIP_OZ_NDIGIT;
+  constexpr int nplane = N + VCPHIP_OZ_NDIGIT;
   const int row0 = blockIdx.x * SR_ROWS, k0 = blockIdx.y * SR_K;
   const int tid = threadIdx.x;
   constexpr int KQ = 256 / SR_ROWS;   // k phases per thread
@@ -215,18 +216,20 @@ __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict_
     unsigned c0, c1, c2;
     bool neg;
     split_parts(ap, c0, c1, c2, neg);
-    for (int t = 0; t < P.nmod; ++t)
+#pragma unroll
+    for (int t = 0; t < N; ++t)
       s_tile[(t * SR_ROWS + lr) * SR_LD + lk] = (unsigned char)(signed char)sym_residue(c0, c1, c2, neg, t, P);
     const unsigned ds = digits2_ru(as, g), dr = digits2_ru(ar, h);
-    s_tile[(P.nmod * SR_ROWS + lr) * SR_LD + lk] = (unsigned char)(ds & 0xff);
-    s_tile[((P.nmod + 1) * SR_ROWS + lr) * SR_LD + lk] = (unsigned char)(ds >> 8);
-    s_tile[((P.nmod + 2) * SR_ROWS + lr) * SR_LD + lk] = (unsigned char)(dr & 0xff);
-    s_tile[((P.nmod + 3) * SR_ROWS + lr) * SR_LD + lk] = (unsigned char)(dr >> 8);
+    s_tile[(N * SR_ROWS + lr) * SR_LD + lk] = (unsigned char)(ds & 0xff);
+    s_tile[((N + 1) * SR_ROWS + lr) * SR_LD + lk] = (unsigned char)(ds >> 8);
+    s_tile[((N + 2) * SR_ROWS + lr) * SR_LD + lk] = (unsigned char)(dr & 0xff);
+    s_tile[((N + 3) * SR_ROWS + lr) * SR_LD + lk] = (unsigned char)(dr >> 8);
   }
   __syncthreads();
   // write out: per plane SR_ROWS rows x 64 bytes, 8 bytes per thread
   const int orow = tid / 8, oc = (tid % 8) * 8;
   const long plane_stride = (long)mpad * kpad;
+#pragma unroll 4
   for (int p = 0; p < nplane; ++p) {
     const unsigned* src = reinterpret_cast<const unsigned*>(s_tile + (p * SR_ROWS + orow) * SR_LD + oc);
     *reinterpret_cast<uint2*>(planes + p * plane_stride + vcphip_plane_off(row0 + orow, k0 + oc, kpad)) =
@@ -237,6 +240,7 @@ __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict_
 // columns of B (k x n, column-major intervals) -> N+4 planes of npad x kpad:
 // N residue planes of B', then the hi/lo digits of rad and of |mid|.  One
 // thread handles 16 consecutive k of one column (a 256-byte contiguous read).
+template <int N>
 __global__ __launch_bounds__(256) void k_oz_split_cols(const double* __restrict__ B, long ldb, int k, int n,
                                                        int npad, int kpad, VcphipOzParams P, VcphipOzSide side,
                                                        int8_t* __restrict__ planes) {
@@ -277,17 +281,18 @@ __global__ __launch_bounds__(256) void k_oz_split_cols(const double* __restrict_
   }
   const long plane_stride = (long)npad * kpad;
   int8_t* dst = planes + vcphip_plane_off(j, grp * 16, kpad);
-  for (int z = 0; z < P.nmod; ++z) {
+#pragma unroll
+  for (int z = 0; z < N; ++z) {
     unsigned wv[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int u = 0; u < 16; ++u)
       wv[u >> 2] |= ((unsigned)(unsigned char)(signed char)sym_residue(c0[u], c1[u], c2[u], ng[u], z, P)) << (8 * (u & 3));
     *reinterpret_cast<uint4*>(dst + z * plane_stride) = make_uint4(wv[0], wv[1], wv[2], wv[3]);
   }
-  *reinterpret_cast<uint4*>(dst + P.nmod * plane_stride) = make_uint4(drh[0], drh[1], drh[2], drh[3]);
-  *reinterpret_cast<uint4*>(dst + (P.nmod + 1) * plane_stride) = make_uint4(drl[0], drl[1], drl[2], drl[3]);
-  *reinterpret_cast<uint4*>(dst + (P.nmod + 2) * plane_stride) = make_uint4(dmh[0], dmh[1], dmh[2], dmh[3]);
-  *reinterpret_cast<uint4*>(dst + (P.nmod + 3) * plane_stride) = make_uint4(dml[0], dml[1], dml[2], dml[3]);
+  *reinterpret_cast<uint4*>(dst + N * plane_stride) = make_uint4(drh[0], drh[1], drh[2], drh[3]);
+  *reinterpret_cast<uint4*>(dst + (N + 1) * plane_stride) = make_uint4(drl[0], drl[1], drl[2], drl[3]);
+  *reinterpret_cast<uint4*>(dst + (N + 2) * plane_stride) = make_uint4(dmh[0], dmh[1], dmh[2], dmh[3]);
+  *reinterpret_cast<uint4*>(dst + (N + 3) * plane_stride) = make_uint4(dml[0], dml[1], dml[2], dml[3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -960,15 +965,31 @@ int vcphip_oz_split_rows(const double* A, long lda, int m, int k, int mpad, int 
                          const VcphipOzParams& P, VcphipOzSide side, int8_t* planes, hipStream_t s) {
   dim3 grid(mpad / SR_ROWS, kpad / SR_K);
   const size_t lds = (size_t)(P.nmod + VCPHIP_OZ_NDIGIT) * SR_ROWS * SR_LD;
-  hipLaunchKernelGGL(k_oz_split_rows, grid, dim3(256), lds, s, A, lda, m, k, mpad, kpad, P, side, planes);
+  switch (P.nmod) {
+#define VCPHIP_SPLIT_CASE(NN) \
+  case NN: hipLaunchKernelGGL((k_oz_split_rows<NN>), grid, dim3(256), lds, s, A, lda, m, k, mpad, kpad, P, side, planes); break;
+    VCPHIP_SPLIT_CASE(8) VCPHIP_SPLIT_CASE(9) VCPHIP_SPLIT_CASE(10) VCPHIP_SPLIT_CASE(11) VCPHIP_SPLIT_CASE(12)
+    VCPHIP_SPLIT_CASE(13) VCPHIP_SPLIT_CASE(14) VCPHIP_SPLIT_CASE(15) VCPHIP_SPLIT_CASE(16) VCPHIP_SPLIT_CASE(17)
+    VCPHIP_SPLIT_CASE(18) VCPHIP_SPLIT_CASE(19) VCPHIP_SPLIT_CASE(20)
+#undef VCPHIP_SPLIT_CASE
+    default: return -7;
+  }
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
 int vcphip_oz_split_cols(const double* B, long ldb, int k, int n, int npad, int kpad,
                          const VcphipOzParams& P, VcphipOzSide side, int8_t* planes, hipStream_t s) {
   const long threads = (long)npad * (kpad / 16);
-  hipLaunchKernelGGL(k_oz_split_cols, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, B, ldb, k, n, npad,
-                     kpad, P, side, planes);
+  const dim3 grid((unsigned)((threads + 255) / 256));
+  switch (P.nmod) {
+#define VCPHIP_SPLIT_CASE(NN) \
+  case NN: hipLaunchKernelGGL((k_oz_split_cols<NN>), grid, dim3(256), 0, s, B, ldb, k, n, npad, kpad, P, side, planes); break;
+    VCPHIP_SPLIT_CASE(8) VCPHIP_SPLIT_CASE(9) VCPHIP_SPLIT_CASE(10) VCPHIP_SPLIT_CASE(11) VCPHIP_SPLIT_CASE(12)
+    VCPHIP_SPLIT_CASE(13) VCPHIP_SPLIT_CASE(14) VCPHIP_SPLIT_CASE(15) VCPHIP_SPLIT_CASE(16) VCPHIP_SPLIT_CASE(17)
+    VCPHIP_SPLIT_CASE(18) VCPHIP_SPLIT_CASE(19) VCPHIP_SPLIT_CASE(20)
+#undef VCPHIP_SPLIT_CASE
+    default: return -7;
+  }
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
