@@ -160,8 +160,8 @@ VCPHIP_DEVINL unsigned digits2_ru(double x, int g) {
 VCPHIP_DEVINL int sym_residue(unsigned c0, unsigned c1, unsigned c2, bool neg, int t,
                               const VcphipOzParams& P) {
   const int m = P.mod[t];
-  // |x| = c0 + c1 2^21 + c2 2^42,  c_i < 2^21
-  const unsigned y = c0 + c1 * (unsigned)P.w21[t] + c2 * (unsigned)P.w42[t];  // < 2^30.1
+  // |x| = c0 + c1 2^21 + c2 2^42,  c_i < 2^21 and w < 2^8: 24-bit products
+  const unsigned y = c0 + __umul24(c1, (unsigned)P.w21[t]) + __umul24(c2, (unsigned)P.w42[t]);  // < 2^30.1
   const double yd = (double)y;
   const double q = __builtin_floor(yd * P.rcpd[t]);
   int r = (int)(yd - q * (double)m);   // exact; off by at most one modulus
@@ -188,42 +188,56 @@ __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict_
                                                        int mpad, int kpad, VcphipOzParams P, VcphipOzSide side,
                                                        int8_t* __restrict__ planes) {
   vcphip_mode_round_up();
-  extern __shared__ __attribute__((aligned(16))) unsigned char s_tile[];   // [N+4][64][SR_LD]
+  extern __shared__ __attribute__((aligned(16))) unsigned char s_tile[];   // [N+4][SR_ROWS][SR_LD]
   constexpr int nplane = N + VCPHIP_OZ_NDIGIT;
   const int row0 = blockIdx.x * SR_ROWS, k0 = blockIdx.y * SR_K;
   const int tid = threadIdx.x;
-  constexpr int KQ = 256 / SR_ROWS;   // k phases per thread
-  const int lr = tid % SR_ROWS, kq = tid / SR_ROWS;
+  // each thread: one row, quads of 4 consecutive k (one 32-bit LDS word per plane)
+  constexpr int NQ = 256 / SR_ROWS;   // quads per pass
+  const int lr = tid % SR_ROWS, kq4 = tid / SR_ROWS;
   const int r = row0 + lr;
   const bool rowok = r < m;
   const int e = rowok ? side.e[r] : 0;
   const int g = rowok ? side.g[r] : 0;
   const int h = rowok ? side.h[r] : 0;
   const bool zr = rowok ? (side.flags[r] & VCPHIP_OZ_MIDZERO) != 0 : true;
-  for (int q = 0; q < SR_K / KQ; ++q) {
-    const int lk = kq + KQ * q;
-    const int kk = k0 + lk;
-    double lo = 0.0, hi = 0.0;
-    if (rowok && kk < k) {
-      const double2 v = *reinterpret_cast<const double2*>(A + 2 * (r + lda * kk));
-      lo = v.x;
-      hi = v.y;
-    }
-    double am, ar;
-    vcphip_midrad_ru(lo, hi, am, ar);
-    const double as = __builtin_fabs(am) + ar;
-    const long long ap = zr ? 0 : rnint_floor(vcphip_mulpow2(am, P.beta - e));
-    unsigned c0, c1, c2;
-    bool neg;
-    split_parts(ap, c0, c1, c2, neg);
+  for (int pass = 0; pass < SR_K / (4 * NQ); ++pass) {
+    const int lk0 = 4 * (pass * NQ + kq4);
+    double2 v[4];
 #pragma unroll
-    for (int t = 0; t < N; ++t)
-      s_tile[(t * SR_ROWS + lr) * SR_LD + lk] = (unsigned char)(signed char)sym_residue(c0, c1, c2, neg, t, P);
-    const unsigned ds = digits2_ru(as, g), dr = digits2_ru(ar, h);
-    s_tile[(N * SR_ROWS + lr) * SR_LD + lk] = (unsigned char)(ds & 0xff);
-    s_tile[((N + 1) * SR_ROWS + lr) * SR_LD + lk] = (unsigned char)(ds >> 8);
-    s_tile[((N + 2) * SR_ROWS + lr) * SR_LD + lk] = (unsigned char)(dr & 0xff);
-    s_tile[((N + 3) * SR_ROWS + lr) * SR_LD + lk] = (unsigned char)(dr >> 8);
+    for (int u = 0; u < 4; ++u) {
+      const int kk = k0 + lk0 + u;
+      v[u] = (rowok && kk < k) ? *reinterpret_cast<const double2*>(A + 2 * (r + lda * kk)) : make_double2(0.0, 0.0);
+    }
+    unsigned c0[4], c1[4], c2[4];
+    bool neg[4];
+    unsigned dsh = 0, dsl = 0, drh = 0, drl = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      double am, ar;
+      vcphip_midrad_ru(v[u].x, v[u].y, am, ar);
+      const double as = __builtin_fabs(am) + ar;
+      const long long ap = zr ? 0 : rnint_floor(vcphip_mulpow2(am, P.beta - e));
+      split_parts(ap, c0[u], c1[u], c2[u], neg[u]);
+      const unsigned ds = digits2_ru(as, g), dr = digits2_ru(ar, h);
+      dsh |= (ds & 0xff) << (8 * u);
+      dsl |= (ds >> 8) << (8 * u);
+      drh |= (dr & 0xff) << (8 * u);
+      drl |= (dr >> 8) << (8 * u);
+    }
+    // modulus by modulus: only one modulus' constants are live at a time
+#pragma unroll
+    for (int t = 0; t < N; ++t) {
+      unsigned word = 0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        word |= ((unsigned)(unsigned char)(signed char)sym_residue(c0[u], c1[u], c2[u], neg[u], t, P)) << (8 * u);
+      *reinterpret_cast<unsigned*>(&s_tile[(t * SR_ROWS + lr) * SR_LD + lk0]) = word;
+    }
+    *reinterpret_cast<unsigned*>(&s_tile[(N * SR_ROWS + lr) * SR_LD + lk0]) = dsh;
+    *reinterpret_cast<unsigned*>(&s_tile[((N + 1) * SR_ROWS + lr) * SR_LD + lk0]) = dsl;
+    *reinterpret_cast<unsigned*>(&s_tile[((N + 2) * SR_ROWS + lr) * SR_LD + lk0]) = drh;
+    *reinterpret_cast<unsigned*>(&s_tile[((N + 3) * SR_ROWS + lr) * SR_LD + lk0]) = drl;
   }
   __syncthreads();
   // write out: per plane SR_ROWS rows x 64 bytes, 8 bytes per thread
@@ -246,10 +260,10 @@ __global__ __launch_bounds__(256) void k_oz_split_cols(const double* __restrict_
                                                        int8_t* __restrict__ planes) {
   vcphip_mode_round_up();
   // 64 consecutive threads fill one 1 KB plane block: 16 columns x 64 k
-  const long gid = (long)blockIdx.x * 256 + threadIdx.x;
-  const long unit = gid >> 6;
-  const int kblk = kpad / 64, t = (int)(gid & 63);
-  const int j = (int)(unit / kblk) * 16 + (t & 15), grp = (int)(unit % kblk) * 4 + (t >> 4);
+  const int gid = blockIdx.x * 256 + threadIdx.x;   // npad * kpad / 16 < 2^31
+  const int unit = gid >> 6;
+  const int kblk = kpad / 64, t = gid & 63;
+  const int j = (unit / kblk) * 16 + (t & 15), grp = (unit % kblk) * 4 + (t >> 4);
   if (j >= npad) return;
   const bool colok = j < n;
   const int f = colok ? side.e[j] : 0;
@@ -980,6 +994,7 @@ int vcphip_oz_split_rows(const double* A, long lda, int m, int k, int mpad, int 
 int vcphip_oz_split_cols(const double* B, long ldb, int k, int n, int npad, int kpad,
                          const VcphipOzParams& P, VcphipOzSide side, int8_t* planes, hipStream_t s) {
   const long threads = (long)npad * (kpad / 16);
+  if (threads > 0x7FFFFF00L) return -7;   // the kernel indexes threads in 32 bits
   const dim3 grid((unsigned)((threads + 255) / 256));
   switch (P.nmod) {
 #define VCPHIP_SPLIT_CASE(NN) \
