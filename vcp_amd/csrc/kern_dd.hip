@@ -87,7 +87,7 @@ VCPHIP_DEVINL int dd_residue(const unsigned (&c)[VCPHIP_DD_NCHUNK], bool neg, in
   const int m = P.mod[t];
   unsigned y = 0;
 #pragma unroll
-  for (int i = 0; i < VCPHIP_DD_NCHUNK; ++i) y += c[i] * (unsigned)P.w20[i][t];   // < 6 * 2^28
+  for (int i = 0; i < VCPHIP_DD_NCHUNK; ++i) y += __umul24(c[i], (unsigned)P.w20[i][t]);   // < 6 * 2^28
   const double yd = (double)y;
   const double q = __builtin_floor(yd * P.rcpd[t]);
   int r = (int)(yd - q * (double)m);
@@ -102,81 +102,88 @@ VCPHIP_DEVINL void chunks20(u128 mag, unsigned (&c)[VCPHIP_DD_NCHUNK]) {
   for (int i = 0; i < VCPHIP_DD_NCHUNK; ++i) c[i] = (unsigned)(mag >> (20 * i)) & 0xFFFFFu;
 }
 
-constexpr int SR_ROWS = 64, SR_K = 64, SR_LD = 68;
+constexpr int SR_ROWS = 16, SR_K = 64, SR_LD = 68;   // one 1 KB plane block per workgroup and plane
 
-// rows of A (m x k, column-major dd) -> N residue planes [mpad][kpad]
+// rows of A (m x k, column-major dd) -> N residue planes of mpad x kpad.
+// Thread: one row and a quad of 4 consecutive k (16 rows x 16 quads); the 4
+// residues of a plane go to LDS as one word, then each plane's 1 KB block is
+// written with one coalesced word per thread.
+template <int N>
 __global__ __launch_bounds__(256) void k_dd_split_rows(const double* __restrict__ A, long lda, int m, int k, int mpad,
                                                        int kpad, VcphipOzParams P, const int* __restrict__ ea,
                                                        const int* __restrict__ fa, int8_t* __restrict__ planes) {
-  extern __shared__ __attribute__((aligned(16))) unsigned char s_tile[];   // [N][64][SR_LD]
+  __shared__ __attribute__((aligned(16))) unsigned char s_tile[N * SR_ROWS * SR_LD];
   const int row0 = blockIdx.x * SR_ROWS, k0 = blockIdx.y * SR_K;
   const int tid = threadIdx.x;
-  const int lr = tid % SR_ROWS, kq = tid / SR_ROWS;
+  const int lr = tid % SR_ROWS, lk0 = 4 * (tid / SR_ROWS);
   const int r = row0 + lr;
   const bool rowok = r < m && fa[min(r, m - 1)] == 0;
   const int sc = rowok ? P.beta - ea[r] : 0;
-  for (int q = 0; q < SR_K / 4; ++q) {
-    const int lk = kq + 4 * q;
-    const int kk = k0 + lk;
-    double hi = 0.0, lo = 0.0;
-    if (rowok && kk < k) {
-      const double2 v = *reinterpret_cast<const double2*>(A + 2 * (r + lda * kk));
-      hi = v.x;
-      lo = v.y;
-    }
-    bool neg;
+  double2 v[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int kk = k0 + lk0 + u;
+    v[u] = (rowok && kk < k) ? *reinterpret_cast<const double2*>(A + 2 * (r + lda * kk)) : make_double2(0.0, 0.0);
+  }
+  unsigned c[4][VCPHIP_DD_NCHUNK];
+  bool neg[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
     u128 mag;
-    dd_scaled(hi, lo, sc, neg, mag);
-    unsigned c[VCPHIP_DD_NCHUNK];
-    chunks20(mag, c);
-    for (int t = 0; t < P.nmod; ++t)
-      s_tile[(t * SR_ROWS + lr) * SR_LD + lk] = (unsigned char)(signed char)dd_residue(c, neg, t, P);
+    dd_scaled(v[u].x, v[u].y, sc, neg[u], mag);
+    chunks20(mag, c[u]);
+  }
+#pragma unroll
+  for (int t = 0; t < N; ++t) {
+    unsigned word = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) word |= ((unsigned)(unsigned char)(signed char)dd_residue(c[u], neg[u], t, P)) << (8 * u);
+    *reinterpret_cast<unsigned*>(&s_tile[(t * SR_ROWS + lr) * SR_LD + lk0]) = word;
   }
   __syncthreads();
-  const int orow = tid / 4, oc = (tid % 4) * 16;
+  const int orow = tid / 16, oc = (tid % 16) * 4;
   const long plane_stride = (long)mpad * kpad;
-  for (int p = 0; p < P.nmod; ++p) {
-    const unsigned* src = reinterpret_cast<const unsigned*>(s_tile + (p * SR_ROWS + orow) * SR_LD + oc);
-    *reinterpret_cast<uint4*>(planes + p * plane_stride + vcphip_plane_off(row0 + orow, k0 + oc, kpad)) =
-        make_uint4(src[0], src[1], src[2], src[3]);
-  }
+  int8_t* dst = planes + vcphip_plane_off(row0 + orow, k0 + oc, kpad);
+#pragma unroll 4
+  for (int p = 0; p < N; ++p)
+    *reinterpret_cast<unsigned*>(dst + p * plane_stride) =
+        *reinterpret_cast<const unsigned*>(&s_tile[(p * SR_ROWS + orow) * SR_LD + oc]);
 }
 
-// columns of B (k x n, column-major dd) -> N residue planes [npad][kpad]
+// columns of B (k x n, column-major dd) -> N residue planes of npad x kpad.
+// Thread: a quad of 4 consecutive k of one column; 16 consecutive threads
+// cover 64 k of a column (1 KB of contiguous input), 256 threads one 1 KB
+// plane block (16 columns x 64 k), written as 4 contiguous 256-byte runs.
+template <int N>
 __global__ __launch_bounds__(256) void k_dd_split_cols(const double* __restrict__ B, long ldb, int k, int n, int npad,
                                                        int kpad, VcphipOzParams P, const int* __restrict__ eb,
                                                        const int* __restrict__ fb, int8_t* __restrict__ planes) {
-  // 64 consecutive threads fill one 1 KB plane block: 16 columns x 64 k
-  const long gid = (long)blockIdx.x * 256 + threadIdx.x;
-  const long unit = gid >> 6;
-  const int kblk = kpad / 64, tq = (int)(gid & 63);
-  const int j = (int)(unit / kblk) * 16 + (tq & 15), grp = (int)(unit % kblk) * 4 + (tq >> 4);
-  if (j >= npad) return;
+  const int kblk = kpad / 64, unit = blockIdx.x, t = threadIdx.x;
+  const int j = (unit / kblk) * 16 + (t >> 4), k0 = (unit % kblk) * 64 + 4 * (t & 15);
   const bool colok = j < n && fb[min(j, n - 1)] == 0;
   const int sc = colok ? P.beta - eb[j] : 0;
-  unsigned c[16][VCPHIP_DD_NCHUNK];
-  bool ng[16];
+  double2 v[4];
 #pragma unroll
-  for (int u = 0; u < 16; ++u) {
-    const int kk = grp * 16 + u;
-    double hi = 0.0, lo = 0.0;
-    if (colok && kk < k) {
-      const double2 v = *reinterpret_cast<const double2*>(B + 2 * (kk + ldb * (long)j));
-      hi = v.x;
-      lo = v.y;
-    }
+  for (int u = 0; u < 4; ++u) {
+    const int kk = k0 + u;
+    v[u] = (colok && kk < k) ? *reinterpret_cast<const double2*>(B + 2 * (kk + ldb * (long)j)) : make_double2(0.0, 0.0);
+  }
+  unsigned c[4][VCPHIP_DD_NCHUNK];
+  bool neg[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
     u128 mag;
-    dd_scaled(hi, lo, sc, ng[u], mag);
+    dd_scaled(v[u].x, v[u].y, sc, neg[u], mag);
     chunks20(mag, c[u]);
   }
   const long plane_stride = (long)npad * kpad;
-  int8_t* dst = planes + vcphip_plane_off(j, grp * 16, kpad);
-  for (int t = 0; t < P.nmod; ++t) {
-    unsigned wv[4] = {0, 0, 0, 0};
+  int8_t* dst = planes + vcphip_plane_off(j, k0, kpad);
 #pragma unroll
-    for (int u = 0; u < 16; ++u)
-      wv[u >> 2] |= ((unsigned)(unsigned char)(signed char)dd_residue(c[u], ng[u], t, P)) << (8 * (u & 3));
-    *reinterpret_cast<uint4*>(dst + t * plane_stride) = make_uint4(wv[0], wv[1], wv[2], wv[3]);
+  for (int z = 0; z < N; ++z) {
+    unsigned word = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) word |= ((unsigned)(unsigned char)(signed char)dd_residue(c[u], neg[u], z, P)) << (8 * u);
+    *reinterpret_cast<unsigned*>(dst + z * plane_stride) = word;
   }
 }
 
@@ -326,17 +333,30 @@ int vcphip_dd_stats(const double* X, long rs, long cs, int rows, int len, int* e
 
 int vcphip_dd_split_rows(const double* A, long lda, int m, int k, int mpad, int kpad, const VcphipOzParams& P,
                          const int* e, const int* flags, int8_t* planes, hipStream_t s) {
-  const size_t lds = (size_t)P.nmod * SR_ROWS * SR_LD;
-  hipLaunchKernelGGL(k_dd_split_rows, dim3(mpad / SR_ROWS, kpad / SR_K), dim3(256), lds, s, A, lda, m, k, mpad, kpad,
-                     P, e, flags, planes);
+  const dim3 grid(mpad / SR_ROWS, kpad / SR_K);
+  switch (P.nmod) {
+#define VCPHIP_DD_SPLIT(NN) \
+  case NN: hipLaunchKernelGGL((k_dd_split_rows<NN>), grid, dim3(256), 0, s, A, lda, m, k, mpad, kpad, P, e, flags, planes); break;
+    VCPHIP_DD_SPLIT(24) VCPHIP_DD_SPLIT(25) VCPHIP_DD_SPLIT(26) VCPHIP_DD_SPLIT(27) VCPHIP_DD_SPLIT(28)
+    VCPHIP_DD_SPLIT(29) VCPHIP_DD_SPLIT(30) VCPHIP_DD_SPLIT(31) VCPHIP_DD_SPLIT(32)
+#undef VCPHIP_DD_SPLIT
+    default: return -7;
+  }
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
 int vcphip_dd_split_cols(const double* B, long ldb, int k, int n, int npad, int kpad, const VcphipOzParams& P,
                          const int* e, const int* flags, int8_t* planes, hipStream_t s) {
-  const long threads = (long)npad * (kpad / 16);
-  hipLaunchKernelGGL(k_dd_split_cols, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, B, ldb, k, n, npad,
-                     kpad, P, e, flags, planes);
+  const long blocks = (long)(npad / 16) * (kpad / 64);
+  if (blocks > 0x7FFFFFFFL) return -7;
+  switch (P.nmod) {
+#define VCPHIP_DD_SPLIT(NN) \
+  case NN: hipLaunchKernelGGL((k_dd_split_cols<NN>), dim3((unsigned)blocks), dim3(256), 0, s, B, ldb, k, n, npad, kpad, P, e, flags, planes); break;
+    VCPHIP_DD_SPLIT(24) VCPHIP_DD_SPLIT(25) VCPHIP_DD_SPLIT(26) VCPHIP_DD_SPLIT(27) VCPHIP_DD_SPLIT(28)
+    VCPHIP_DD_SPLIT(29) VCPHIP_DD_SPLIT(30) VCPHIP_DD_SPLIT(31) VCPHIP_DD_SPLIT(32)
+#undef VCPHIP_DD_SPLIT
+    default: return -7;
+  }
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

@@ -252,61 +252,59 @@ __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict_
 }
 
 // columns of B (k x n, column-major intervals) -> N+4 planes of npad x kpad:
-// N residue planes of B', then the hi/lo digits of rad and of |mid|.  One
-// thread handles 16 consecutive k of one column (a 256-byte contiguous read).
+// N residue planes of B', then the hi/lo digits of rad and of |mid|.
+// Thread: a quad of 4 consecutive k of one column; 16 consecutive threads
+// cover 64 k of a column (1 KB of contiguous input), a workgroup one 1 KB
+// plane block (16 columns x 64 k) per plane, written as 4 contiguous
+// 256-byte runs.
 template <int N>
 __global__ __launch_bounds__(256) void k_oz_split_cols(const double* __restrict__ B, long ldb, int k, int n,
                                                        int npad, int kpad, VcphipOzParams P, VcphipOzSide side,
                                                        int8_t* __restrict__ planes) {
   vcphip_mode_round_up();
-  // 64 consecutive threads fill one 1 KB plane block: 16 columns x 64 k
-  const int gid = blockIdx.x * 256 + threadIdx.x;   // npad * kpad / 16 < 2^31
-  const int unit = gid >> 6;
-  const int kblk = kpad / 64, t = gid & 63;
-  const int j = (unit / kblk) * 16 + (t & 15), grp = (unit % kblk) * 4 + (t >> 4);
-  if (j >= npad) return;
+  const int kblk = kpad / 64, unit = blockIdx.x, t = threadIdx.x;
+  const int j = (unit / kblk) * 16 + (t >> 4), k0 = (unit % kblk) * 64 + 4 * (t & 15);
   const bool colok = j < n;
   const int f = colok ? side.e[j] : 0;
   const int xi = colok ? side.h[j] : 0;
   const bool zc = colok ? (side.flags[j] & VCPHIP_OZ_MIDZERO) != 0 : true;
   const int zeta = zc ? 0 : f;
-  unsigned c0[16], c1[16], c2[16];
-  bool ng[16];
-  unsigned drh[4] = {0, 0, 0, 0}, drl[4] = {0, 0, 0, 0}, dmh[4] = {0, 0, 0, 0}, dml[4] = {0, 0, 0, 0};
+  double2 v[4];
 #pragma unroll
-  for (int u = 0; u < 16; ++u) {
-    const int kk = grp * 16 + u;
-    double lo = 0.0, hi = 0.0;
-    if (colok && kk < k) {
-      const double2 v = *reinterpret_cast<const double2*>(B + 2 * (kk + ldb * (long)j));
-      lo = v.x;
-      hi = v.y;
-    }
+  for (int u = 0; u < 4; ++u) {
+    const int kk = k0 + u;
+    v[u] = (colok && kk < k) ? *reinterpret_cast<const double2*>(B + 2 * (kk + ldb * (long)j)) : make_double2(0.0, 0.0);
+  }
+  unsigned c0[4], c1[4], c2[4];
+  bool ng[4];
+  unsigned drh = 0, drl = 0, dmh = 0, dml = 0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
     double bm, br;
-    vcphip_midrad_ru(lo, hi, bm, br);
+    vcphip_midrad_ru(v[u].x, v[u].y, bm, br);
     const long long bp = zc ? 0 : rnint_floor(vcphip_mulpow2(bm, P.beta - f));
     split_parts(bp, c0[u], c1[u], c2[u], ng[u]);
     const unsigned d1 = digits2_ru(br, xi);
     const unsigned d2 = digits2_ru(__builtin_fabs(bm), zeta);
-    drh[u >> 2] |= (d1 & 0xff) << (8 * (u & 3));
-    drl[u >> 2] |= (d1 >> 8) << (8 * (u & 3));
-    dmh[u >> 2] |= (d2 & 0xff) << (8 * (u & 3));
-    dml[u >> 2] |= (d2 >> 8) << (8 * (u & 3));
+    drh |= (d1 & 0xff) << (8 * u);
+    drl |= (d1 >> 8) << (8 * u);
+    dmh |= (d2 & 0xff) << (8 * u);
+    dml |= (d2 >> 8) << (8 * u);
   }
   const long plane_stride = (long)npad * kpad;
-  int8_t* dst = planes + vcphip_plane_off(j, grp * 16, kpad);
+  int8_t* dst = planes + vcphip_plane_off(j, k0, kpad);
 #pragma unroll
   for (int z = 0; z < N; ++z) {
-    unsigned wv[4] = {0, 0, 0, 0};
+    unsigned word = 0;
 #pragma unroll
-    for (int u = 0; u < 16; ++u)
-      wv[u >> 2] |= ((unsigned)(unsigned char)(signed char)sym_residue(c0[u], c1[u], c2[u], ng[u], z, P)) << (8 * (u & 3));
-    *reinterpret_cast<uint4*>(dst + z * plane_stride) = make_uint4(wv[0], wv[1], wv[2], wv[3]);
+    for (int u = 0; u < 4; ++u)
+      word |= ((unsigned)(unsigned char)(signed char)sym_residue(c0[u], c1[u], c2[u], ng[u], z, P)) << (8 * u);
+    *reinterpret_cast<unsigned*>(dst + z * plane_stride) = word;
   }
-  *reinterpret_cast<uint4*>(dst + N * plane_stride) = make_uint4(drh[0], drh[1], drh[2], drh[3]);
-  *reinterpret_cast<uint4*>(dst + (N + 1) * plane_stride) = make_uint4(drl[0], drl[1], drl[2], drl[3]);
-  *reinterpret_cast<uint4*>(dst + (N + 2) * plane_stride) = make_uint4(dmh[0], dmh[1], dmh[2], dmh[3]);
-  *reinterpret_cast<uint4*>(dst + (N + 3) * plane_stride) = make_uint4(dml[0], dml[1], dml[2], dml[3]);
+  *reinterpret_cast<unsigned*>(dst + N * plane_stride) = drh;
+  *reinterpret_cast<unsigned*>(dst + (N + 1) * plane_stride) = drl;
+  *reinterpret_cast<unsigned*>(dst + (N + 2) * plane_stride) = dmh;
+  *reinterpret_cast<unsigned*>(dst + (N + 3) * plane_stride) = dml;
 }
 
 // ---------------------------------------------------------------------------
@@ -993,9 +991,9 @@ int vcphip_oz_split_rows(const double* A, long lda, int m, int k, int mpad, int 
 
 int vcphip_oz_split_cols(const double* B, long ldb, int k, int n, int npad, int kpad,
                          const VcphipOzParams& P, VcphipOzSide side, int8_t* planes, hipStream_t s) {
-  const long threads = (long)npad * (kpad / 16);
-  if (threads > 0x7FFFFF00L) return -7;   // the kernel indexes threads in 32 bits
-  const dim3 grid((unsigned)((threads + 255) / 256));
+  const long blocks = (long)(npad / 16) * (kpad / 64);   // one 1 KB plane block each
+  if (blocks > 0x7FFFFFFFL) return -7;
+  const dim3 grid((unsigned)blocks);
   switch (P.nmod) {
 #define VCPHIP_SPLIT_CASE(NN) \
   case NN: hipLaunchKernelGGL((k_oz_split_cols<NN>), grid, dim3(256), 0, s, B, ldb, k, n, npad, kpad, P, side, planes); break;
