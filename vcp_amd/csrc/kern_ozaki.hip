@@ -179,7 +179,7 @@ VCPHIP_DEVINL void split_parts(long long a, unsigned& c0, unsigned& c1, unsigned
   c2 = (unsigned)(u >> 42);
 }
 
-constexpr int SR_ROWS = 32, SR_K = 64, SR_LD = 68;   // LDS byte tile, odd-word row stride
+constexpr int SR_ROWS = 16, SR_K = 64, SR_LD = 68;   // LDS byte tile (one plane block), odd-word row stride
 
 // rows of A (m x k, column-major intervals) -> N+4 planes of mpad x kpad:
 // N residue planes of A', then the hi/lo digits of |mid|+rad and of rad.
@@ -240,15 +240,15 @@ __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict_
     *reinterpret_cast<unsigned*>(&s_tile[((N + 3) * SR_ROWS + lr) * SR_LD + lk0]) = drl;
   }
   __syncthreads();
-  // write out: per plane SR_ROWS rows x 64 bytes, 8 bytes per thread
-  const int orow = tid / 8, oc = (tid % 8) * 8;
+  // write out: per plane one 1 KB block (16 rows x 64 k), one word per thread
+  static_assert(SR_ROWS * SR_K == 4 * 256, "one word per thread and plane");
+  const int orow = tid / 16, oc = (tid % 16) * 4;
   const long plane_stride = (long)mpad * kpad;
+  int8_t* dst = planes + vcphip_plane_off(row0 + orow, k0 + oc, kpad);
 #pragma unroll 4
-  for (int p = 0; p < nplane; ++p) {
-    const unsigned* src = reinterpret_cast<const unsigned*>(s_tile + (p * SR_ROWS + orow) * SR_LD + oc);
-    *reinterpret_cast<uint2*>(planes + p * plane_stride + vcphip_plane_off(row0 + orow, k0 + oc, kpad)) =
-        make_uint2(src[0], src[1]);
-  }
+  for (int p = 0; p < nplane; ++p)
+    *reinterpret_cast<unsigned*>(dst + p * plane_stride) =
+        *reinterpret_cast<const unsigned*>(s_tile + (p * SR_ROWS + orow) * SR_LD + oc);
 }
 
 // columns of B (k x n, column-major intervals) -> N+4 planes of npad x kpad:
