@@ -803,8 +803,8 @@ void orc_ozaki_scaled_cols(int k, int n, const double* B, int ldb, int beta, int
  * the CRT reconstruction recovers), one directed conversion each way, the
  * scaling-error bound E and the upward two-digit bound of the radius term
  * (|mA|+rA) rB + rA |mB| (:277-283). */
-int orc_ozaki_igemm(int m, int n, int k, const double* A, int lda, const double* B, int ldb,
-                    double* C, int ldc, int nmod) {
+static int oz_igemm_core(int m, int n, int k, const double* A, int lda, const double* B, int ldb,
+                         double* C, int ldc, int nmod) {
     const int beta = orc_ozaki_beta(k, nmod);
     if (beta < 1) return -2;
     const size_t mk = (size_t)m * k, kn = (size_t)k * n;
@@ -1137,8 +1137,8 @@ static double dg_rn(const uint32_t* x, int s, int* qo, int* upo) {
 
 /* C = A*B (or A^T A when B == NULL, A k x n) for dd matrices through the
  * exact scaled-integer product the GPU CRT pipeline recovers. */
-int orc_ozaki_ddgemm(int m, int n, int k, const double* A, int lda, const double* B, int ldb,
-                     double* C, int ldc, int nmod) {
+static int oz_ddgemm_core(int m, int n, int k, const double* A, int lda, const double* B, int ldb,
+                          double* C, int ldc, int nmod) {
     const int beta = orc_ozaki_dd_beta(k, nmod);
     if (beta < 8) return -2;
     const int tn = (B == NULL);
@@ -1215,4 +1215,103 @@ int orc_ozaki_ddgemm(int m, int n, int k, const double* A, int lda, const double
     }
     free(ea); free(za); free(eb); free(zb); free(Ap); free(Bp);
     return 0;
+}
+
+/* ---- non-finite operands of the Ozaki forms ----------------------------
+   A row of A or a column of B holding a +-inf or NaN end is split as zeros;
+   every product entry it takes part in is then [-inf, +inf] ([NaN, NaN] if
+   a NaN is involved) for the interval form, and (NaN, NaN) for the dd form.
+   The exact product of such operands is not a finite interval; this is the
+   widest enclosure, or NaN. */
+enum { OZ_NONFINITE = 8, OZ_HASNAN = 16 };
+static int nf_bits(double x, double y) {
+    const int nan = isnan(x) || isnan(y), inf = isinf(x) || isinf(y);
+    return ((nan || inf) ? OZ_NONFINITE : 0) | (nan ? OZ_HASNAN : 0);
+}
+
+/* flags of `count` vectors of `len` pairs; pair c of vector v at X + 2 (v vs + c cs) */
+static int nf_scan(int count, int len, const double* X, size_t vs, size_t cs, int* f) {
+    int any = 0;
+    for (int v = 0; v < count; ++v) {
+        int b = 0;
+        for (int c = 0; c < len; ++c) {
+            const double* x = X + 2 * ((size_t)v * vs + (size_t)c * cs);
+            b |= nf_bits(x[0], x[1]);
+        }
+        f[v] = b;
+        any |= b;
+    }
+    return any;
+}
+
+/* copy of a rows x cols pair matrix (leading dimension ld) with the flagged
+   rows (by_row) or columns zeroed, leading dimension rows */
+static double* nf_clean(int rows, int cols, const double* X, int ld, const int* f, int by_row) {
+    double* Y = dalloc((size_t)2 * (rows > 0 ? rows : 1) * (cols > 0 ? cols : 1));
+    for (int c = 0; c < cols; ++c)
+        for (int r = 0; r < rows; ++r) {
+            const int bad = by_row ? f[r] : f[c];
+            const double* x = X + 2 * ((size_t)r + (size_t)ld * c);
+            double* y = Y + 2 * ((size_t)r + (size_t)rows * c);
+            y[0] = bad ? 0.0 : x[0];
+            y[1] = bad ? 0.0 : x[1];
+        }
+    return Y;
+}
+
+int orc_ozaki_igemm(int m, int n, int k, const double* A, int lda, const double* B, int ldb,
+                    double* C, int ldc, int nmod) {
+    int* fa = (int*)calloc(m > 0 ? m : 1, sizeof(int));
+    int* fb = (int*)calloc(n > 0 ? n : 1, sizeof(int));
+    const int any = nf_scan(m, k, A, 1, (size_t)lda, fa) | nf_scan(n, k, B, (size_t)ldb, 1, fb);
+    int rc;
+    if (!any) {
+        rc = oz_igemm_core(m, n, k, A, lda, B, ldb, C, ldc, nmod);
+    } else {
+        double* A2 = nf_clean(m, k, A, lda, fa, 1);
+        double* B2 = nf_clean(k, n, B, ldb, fb, 0);
+        rc = oz_igemm_core(m, n, k, A2, m > 0 ? m : 1, B2, k > 0 ? k : 1, C, ldc, nmod);
+        for (int j = 0; j < n && rc == 0; ++j)
+            for (int i = 0; i < m; ++i) {
+                const int f = fa[i] | fb[j];
+                if (!(f & OZ_NONFINITE)) continue;
+                double* c = C + 2 * ((size_t)i + (size_t)ldc * j);
+                c[0] = (f & OZ_HASNAN) ? NAN : -INFINITY;
+                c[1] = (f & OZ_HASNAN) ? NAN : INFINITY;
+            }
+        free(A2);
+        free(B2);
+    }
+    free(fa);
+    free(fb);
+    return rc;
+}
+
+int orc_ozaki_ddgemm(int m, int n, int k, const double* A, int lda, const double* B, int ldb,
+                     double* C, int ldc, int nmod) {
+    const int tn = (B == NULL);
+    int* fa = (int*)calloc(m > 0 ? m : 1, sizeof(int));
+    int* fb = (int*)calloc(n > 0 ? n : 1, sizeof(int));
+    /* tn: C = A^T A with A k x m: operand rows are the columns of A */
+    int any = tn ? nf_scan(m, k, A, (size_t)lda, 1, fa) : nf_scan(m, k, A, 1, (size_t)lda, fa);
+    any |= tn ? nf_scan(n, k, A, (size_t)lda, 1, fb) : nf_scan(n, k, B, (size_t)ldb, 1, fb);
+    int rc;
+    if (!any) {
+        rc = oz_ddgemm_core(m, n, k, A, lda, B, ldb, C, ldc, nmod);
+    } else {
+        double* A2 = tn ? nf_clean(k, m, A, lda, fa, 0) : nf_clean(m, k, A, lda, fa, 1);
+        double* B2 = tn ? NULL : nf_clean(k, n, B, ldb, fb, 0);
+        rc = oz_ddgemm_core(m, n, k, A2, tn ? (k > 0 ? k : 1) : (m > 0 ? m : 1), B2, k > 0 ? k : 1, C, ldc, nmod);
+        for (int j = 0; j < n && rc == 0; ++j)
+            for (int i = 0; i < m; ++i)
+                if ((fa[i] | fb[j]) & OZ_NONFINITE) {
+                    double* c = C + 2 * ((size_t)i + (size_t)ldc * j);
+                    c[0] = c[1] = NAN;
+                }
+        free(A2);
+        free(B2);
+    }
+    free(fa);
+    free(fb);
+    return rc;
 }

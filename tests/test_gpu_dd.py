@@ -75,3 +75,22 @@ def test_dd_accuracy_against_exact_and_pddblas(ctx):
             ep.append(e2 / (scale * 2.0 ** -105))
     # errors relative to sum |a||b| in units of 2^-105: both well below one
     assert max(eo) <= 1.0 and max(eo) <= max(2.0 * max(ep), 0.05)
+
+
+def test_dd_nonfinite_operands_match_oracle(ctx):
+    """a row / column with an infinite or NaN part gives NaN entries, as the
+    oracle; the rest bit-exact"""
+    rng = np.random.default_rng(42)
+    A, B = W.dd_matrix(rng, 130, 260), W.dd_matrix(rng, 260, 70)
+    A[3, 100, 0] = np.inf
+    A[90, 1, 1] = np.nan
+    B[259, 40, 0] = -np.inf
+    C = ctx.ddgemm(A, B)
+    nmod = next(t for t in range(24, 33) if O.ozaki_dd_beta(260, t) >= 110)
+    assert np.array_equal(C, O.ozaki_ddgemm(A, B, nmod), equal_nan=True)
+    assert np.isnan(C[3, 0]).all() and np.isnan(C[0, 40]).all() and np.isfinite(C[0, 0]).all()
+    T = W.dd_matrix(rng, 200, 60)
+    T[10, 7, 1] = np.inf
+    S = ctx.ddgemm_tn(T)
+    nmod = next(t for t in range(24, 33) if O.ozaki_dd_beta(200, t) >= 110)
+    assert np.array_equal(S, O.ozaki_ddgemm(T, None, nmod), equal_nan=True)

@@ -83,3 +83,20 @@ def test_ozaki_containment_and_radius(ctx, kind):
     rad_p = (P[..., 1] - P[..., 0])
     ratio = np.max(rad_o / np.maximum(rad_p, np.finfo(float).tiny))
     assert ratio <= 2.0, ratio
+
+
+def test_ozaki_nonfinite_operands_match_oracle(ctx):
+    """rows / columns with infinite or NaN ends: [-inf, +inf] or [NaN, NaN]
+    entries, exactly where the oracle puts them; the rest bit-exact"""
+    rng = np.random.default_rng(41)
+    IA, IB = _iv(rng, 150, 300, "wide"), _iv(rng, 300, 90, "wide")
+    IA[7, 20, 1] = np.inf
+    IA[100, 299, 0] = -np.inf
+    IA[33, 5, :] = np.nan
+    IB[0, 17, 1] = np.inf
+    IB[150, 60, 0] = np.nan
+    C = ctx.igemm(IA, IB, vcp_amd.FORM_OZAKI)
+    assert np.array_equal(C, O.ozaki_igemm(IA, IB, 16), equal_nan=True)
+    assert C[7, 0, 0] == -np.inf and C[7, 0, 1] == np.inf and np.isnan(C[33, 0]).all() and np.isnan(C[0, 60]).all()
+    P = ctx.igemm(IA, IB, vcp_amd.FORM_PLAIN)
+    assert np.array_equal(P, O.pidblas_mulmm(IA, IB), equal_nan=True)

@@ -346,3 +346,40 @@ def test_pddblas_mulmm_accuracy_and_symmetry():
             assert err <= 2.0 ** -100 * scale
     At = O.pddblas_mulltmm(A)
     assert np.array_equal(At, At.transpose(1, 0, 2))
+
+
+def test_ozaki_nonfinite_operands():
+    """a row of A / column of B with an infinite end makes its product entries
+    [-inf, +inf]; with a NaN end [NaN, NaN]; every other entry is unchanged"""
+    rng = np.random.default_rng(40)
+    IA, IB = _iv(rng, 6, 9, "wide"), _iv(rng, 9, 5, "wide")
+    C0 = O.ozaki_igemm(IA, IB, 16)
+    IA[2, 4, 1] = np.inf
+    IA[4, 0, 0] = -np.inf
+    IB[3, 1, 0] = np.nan
+    C = O.ozaki_igemm(IA, IB, 16)
+    for i in range(6):
+        for j in range(5):
+            if j == 1:
+                assert np.isnan(C[i, j]).all()
+            elif i in (2, 4):
+                assert C[i, j, 0] == -np.inf and C[i, j, 1] == np.inf
+            else:
+                assert np.array_equal(C[i, j], C0[i, j])
+    A, B = W.dd_matrix(rng, 5, 8), W.dd_matrix(rng, 8, 4)
+    D0 = O.ozaki_ddgemm(A, B, 31)
+    A[1, 5, 0] = np.inf
+    B[2, 3, 1] = np.nan
+    D = O.ozaki_ddgemm(A, B, 31)
+    for i in range(5):
+        for j in range(4):
+            if i == 1 or j == 3:
+                assert np.isnan(D[i, j]).all()
+            else:
+                assert np.array_equal(D[i, j], D0[i, j])
+    # A^T A form: a column of A is both an operand row and column
+    T = W.dd_matrix(rng, 8, 5)
+    T[0, 2, 1] = np.inf
+    S = O.ozaki_ddgemm(T, None, 31)
+    assert np.isnan(S[2, :]).all() and np.isnan(S[:, 2]).all()
+    assert np.isfinite(np.delete(np.delete(S, 2, 0), 2, 1)).all()

@@ -28,12 +28,15 @@ __global__ __launch_bounds__(256) void k_dd_stats_part(const double* __restrict_
   if (r >= rows) return;
   const int c0 = c * VCPHIP_OZ_CHUNK, c1 = min(c0 + VCPHIP_OZ_CHUNK, len);
   double mu = 0.0;
+  bool nonfinite = false;
   for (int cc = c0; cc < c1; ++cc) {
     const double2 v = *reinterpret_cast<const double2*>(X + 2 * ((long)r * rs + (long)cc * cs));
+    nonfinite |= !(__builtin_isfinite(v.x) && __builtin_isfinite(v.y));
     const double a = __builtin_fabs(v.x) + __builtin_fabs(v.y);
     mu = a > mu ? a : mu;
   }
-  part[(long)c * rows + r] = mu > 0.0 ? vcphip_fexp(mu) : INT32_MIN;
+  // INT32_MAX marks a non-finite element (it wins the maximum over chunks)
+  part[(long)c * rows + r] = nonfinite ? INT32_MAX : (mu > 0.0 ? vcphip_fexp(mu) : INT32_MIN);
 }
 
 __global__ __launch_bounds__(256) void k_dd_stats_final(int rows, int nchunk, const int* __restrict__ part,
@@ -42,8 +45,9 @@ __global__ __launch_bounds__(256) void k_dd_stats_final(int rows, int nchunk, co
   if (r >= rows) return;
   int best = INT32_MIN;
   for (int c = 0; c < nchunk; ++c) best = max(best, part[(long)c * rows + r]);
-  e[r] = best == INT32_MIN ? 0 : best;
-  flags[r] = best == INT32_MIN ? 1 : 0;
+  // flags: 1 = zero row / column, 2 = a non-finite element
+  e[r] = (best == INT32_MIN || best == INT32_MAX) ? 0 : best;
+  flags[r] = best == INT32_MIN ? 1 : (best == INT32_MAX ? 2 : 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -271,8 +275,9 @@ __global__ __launch_bounds__(256) void k_dd_recon(int m, int n, int mpad, int np
   const int i = (int)(blk % bpm) * 16 + 4 * (o >> 6) + (o & 3);
   const int j = (int)(blk / bpm) * 16 + ((o >> 2) & 15);
   if (i >= m || j >= n) return;
-  if (fa[i] || fb[j]) {
-    *reinterpret_cast<double2*>(C + 2 * (i + ldc * (long)j)) = make_double2(0.0, 0.0);
+  if (fa[i] || fb[j]) {   // a zero row / column gives 0, a non-finite one NaN
+    const double v = ((fa[i] | fb[j]) & 2) ? __builtin_nan("") : 0.0;
+    *reinterpret_cast<double2*>(C + 2 * (i + ldc * (long)j)) = make_double2(v, v);
     return;
   }
   const long plane = (long)mpad * npad;

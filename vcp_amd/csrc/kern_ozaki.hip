@@ -29,6 +29,20 @@ namespace {
 // ---------------------------------------------------------------------------
 // statistics: per row (or column) of a midpoint-radius operand
 // ---------------------------------------------------------------------------
+// NONFINITE / HASNAN bits of one interval [lo, hi]
+VCPHIP_DEVINL int nonfinite_bits(double lo, double hi) {
+  const bool nan = __builtin_isnan(lo) || __builtin_isnan(hi);
+  const bool inf = __builtin_isinf(lo) || __builtin_isinf(hi);
+  return (nan || inf ? VCPHIP_OZ_NONFINITE : 0) | (nan ? VCPHIP_OZ_HASNAN : 0);
+}
+
+// flags of a row / column from its maxima and non-finite bits
+VCPHIP_DEVINL int side_flags(double mu, double smax, double rmax, int nf) {
+  if (nf) return nf | VCPHIP_OZ_MIDZERO | VCPHIP_OZ_SUMZERO | VCPHIP_OZ_RADZERO;
+  return (mu == 0.0 ? VCPHIP_OZ_MIDZERO : 0) | (smax == 0.0 ? VCPHIP_OZ_SUMZERO : 0) |
+         (rmax == 0.0 ? VCPHIP_OZ_RADZERO : 0);
+}
+
 __global__ __launch_bounds__(256) void k_oz_stats_part(const double* __restrict__ X, long rs, long cs,
                                                        int rows, int len, int nchunk,
                                                        double* __restrict__ part) {
@@ -39,10 +53,12 @@ __global__ __launch_bounds__(256) void k_oz_stats_part(const double* __restrict_
   const int c0 = c * VCPHIP_OZ_CHUNK;
   const int c1 = min(c0 + VCPHIP_OZ_CHUNK, len);
   double mu = 0.0, smax = 0.0, rmax = 0.0, sum = 0.0;
+  int nf = 0;
   const double* base = X + 2 * ((long)r * rs);
 #pragma unroll 8
   for (int cc = c0; cc < c1; ++cc) {
     const double2 v = *reinterpret_cast<const double2*>(base + 2 * ((long)cc * cs));
+    nf |= nonfinite_bits(v.x, v.y);
     double m, rr;
     vcphip_midrad_ru(v.x, v.y, m, rr);
     const double am = __builtin_fabs(m);
@@ -57,6 +73,7 @@ __global__ __launch_bounds__(256) void k_oz_stats_part(const double* __restrict_
   part[1 * stride + (long)c * rows + r] = smax;
   part[2 * stride + (long)c * rows + r] = rmax;
   part[3 * stride + (long)c * rows + r] = sum;
+  part[4 * stride + (long)c * rows + r] = (double)nf;
 }
 
 // columns of B (k x n, column k-contiguous): one wave per column.  Lane l
@@ -79,11 +96,13 @@ __global__ __launch_bounds__(256) void k_oz_stats_cols(const double* __restrict_
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int j = blockIdx.x * 4 + w;
   double mu = 0.0, smax = 0.0, rmax = 0.0, sum = 0.0;
+  int nf = 0;
   if (j < cols) {
     const double* col = X + 2 * ldx * (long)j;
 #pragma unroll 4
     for (int kk = lane; kk < len; kk += 64) {
       const double2 v = *reinterpret_cast<const double2*>(col + 2 * (long)kk);
+      nf |= nonfinite_bits(v.x, v.y);
       double m, rr;
       vcphip_midrad_ru(v.x, v.y, m, rr);
       const double am = __builtin_fabs(m);
@@ -97,17 +116,18 @@ __global__ __launch_bounds__(256) void k_oz_stats_cols(const double* __restrict_
   mu = wave_max(mu);
   smax = wave_max(smax);
   rmax = wave_max(rmax);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) nf |= __shfl_xor(nf, o, 64);
   s_sum[w][lane] = sum;
   __syncthreads();
   if (lane == 0 && j < cols) {
     double tot = 0.0;
     for (int l = 0; l < 64; ++l) tot = tot + s_sum[w][l];
-    side.flags[j] = (mu == 0.0 ? VCPHIP_OZ_MIDZERO : 0) | (smax == 0.0 ? VCPHIP_OZ_SUMZERO : 0) |
-                    (rmax == 0.0 ? VCPHIP_OZ_RADZERO : 0);
-    side.e[j] = mu > 0.0 ? vcphip_fexp(mu) : 0;
-    side.g[j] = smax > 0.0 ? vcphip_fexp(smax) : 0;
-    side.h[j] = rmax > 0.0 ? vcphip_fexp(rmax) : 0;
-    side.n1[j] = tot;
+    side.flags[j] = side_flags(mu, smax, rmax, nf);
+    side.e[j] = (!nf && mu > 0.0) ? vcphip_fexp(mu) : 0;
+    side.g[j] = (!nf && smax > 0.0) ? vcphip_fexp(smax) : 0;
+    side.h[j] = (!nf && rmax > 0.0) ? vcphip_fexp(rmax) : 0;
+    side.n1[j] = nf ? 0.0 : tot;
   }
 }
 
@@ -118,22 +138,23 @@ __global__ __launch_bounds__(256) void k_oz_stats_final(int rows, int nchunk, co
   if (r >= rows) return;
   const long stride = (long)nchunk * rows;
   double mu = 0.0, smax = 0.0, rmax = 0.0, tot = 0.0;
+  int nf = 0;
   for (int c = 0; c < nchunk; ++c) {
     const double a = part[0 * stride + (long)c * rows + r];
     const double b = part[1 * stride + (long)c * rows + r];
     const double d = part[2 * stride + (long)c * rows + r];
     const double s = part[3 * stride + (long)c * rows + r];
+    nf |= (int)part[4 * stride + (long)c * rows + r];
     mu = a > mu ? a : mu;
     smax = b > smax ? b : smax;
     rmax = d > rmax ? d : rmax;
     tot = tot + s;
   }
-  side.flags[r] = (mu == 0.0 ? VCPHIP_OZ_MIDZERO : 0) | (smax == 0.0 ? VCPHIP_OZ_SUMZERO : 0) |
-                  (rmax == 0.0 ? VCPHIP_OZ_RADZERO : 0);
-  side.e[r] = mu > 0.0 ? vcphip_fexp(mu) : 0;
-  side.g[r] = smax > 0.0 ? vcphip_fexp(smax) : 0;
-  side.h[r] = rmax > 0.0 ? vcphip_fexp(rmax) : 0;
-  side.n1[r] = tot;
+  side.flags[r] = side_flags(mu, smax, rmax, nf);
+  side.e[r] = (!nf && mu > 0.0) ? vcphip_fexp(mu) : 0;
+  side.g[r] = (!nf && smax > 0.0) ? vcphip_fexp(smax) : 0;
+  side.h[r] = (!nf && rmax > 0.0) ? vcphip_fexp(rmax) : 0;
+  side.n1[r] = nf ? 0.0 : tot;
 }
 
 // ---------------------------------------------------------------------------
@@ -196,7 +217,7 @@ __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict_
   constexpr int NQ = 256 / SR_ROWS;   // quads per pass
   const int lr = tid % SR_ROWS, kq4 = tid / SR_ROWS;
   const int r = row0 + lr;
-  const bool rowok = r < m;
+  const bool rowok = r < m && !(side.flags[r < m ? r : 0] & VCPHIP_OZ_NONFINITE);
   const int e = rowok ? side.e[r] : 0;
   const int g = rowok ? side.g[r] : 0;
   const int h = rowok ? side.h[r] : 0;
@@ -264,7 +285,7 @@ __global__ __launch_bounds__(256) void k_oz_split_cols(const double* __restrict_
   vcphip_mode_round_up();
   const int kblk = kpad / 64, unit = blockIdx.x, t = threadIdx.x;
   const int j = (unit / kblk) * 16 + (t >> 4), k0 = (unit % kblk) * 64 + 4 * (t & 15);
-  const bool colok = j < n;
+  const bool colok = j < n && !(side.flags[j < n ? j : 0] & VCPHIP_OZ_NONFINITE);
   const int f = colok ? side.e[j] : 0;
   const int xi = colok ? side.h[j] : 0;
   const bool zc = colok ? (side.flags[j] & VCPHIP_OZ_MIDZERO) != 0 : true;
@@ -947,9 +968,14 @@ __global__ __launch_bounds__(256) void k_oz_recon(int m, int n, int k, int mpad,
     E = e1 + e2;
   }
   const double rad = rt + E;
-  const double hi = hi_mid + rad;
-  const double nlo = rad - lo_mid;
-  *reinterpret_cast<double2*>(C + 2 * (i + ldc * (long)j)) = make_double2(-nlo, hi);
+  double hi = hi_mid + rad;
+  double lo = -(rad - lo_mid);
+  if ((fa | fb) & VCPHIP_OZ_NONFINITE) {   // a non-finite operand end in the row or column
+    const bool nan = ((fa | fb) & VCPHIP_OZ_HASNAN) != 0;
+    lo = nan ? __builtin_nan("") : -__builtin_inf();
+    hi = nan ? __builtin_nan("") : __builtin_inf();
+  }
+  *reinterpret_cast<double2*>(C + 2 * (i + ldc * (long)j)) = make_double2(lo, hi);
 }
 
 }  // namespace

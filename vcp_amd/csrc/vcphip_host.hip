@@ -449,14 +449,14 @@ static int oz_igemm_dev(vcphip_ctx* c, int m, int n, int k, const double* A, int
   if ((rc = ws_get(c, vcphip_ctx::WS_PB, nplane * npad * kpad, &pb))) return rc;
   if ((rc = ws_get(c, vcphip_ctx::WS_U, (size_t)nmod * mpad * npad, &pu))) return rc;
   if ((rc = ws_get(c, vcphip_ctx::WS_Z, VCPHIP_OZ_NDIGIT * sizeof(int32_t) * (size_t)mpad * npad, &pz))) return rc;
-  auto side_bytes = [&](int rows) { return (size_t)rows * (4 * sizeof(int) + sizeof(double)) + sizeof(double) * 4 * (size_t)(nchunk > 0 ? nchunk : 1) * rows + 1024; };
+  auto side_bytes = [&](int rows) { return (size_t)rows * (4 * sizeof(int) + sizeof(double)) + sizeof(double) * 5 * (size_t)(nchunk > 0 ? nchunk : 1) * rows + 1024; };
   if ((rc = ws_get(c, vcphip_ctx::WS_SA, side_bytes(m), &sa))) return rc;
   if ((rc = ws_get(c, vcphip_ctx::WS_SB, side_bytes(n), &sb))) return rc;
   auto carve = [&](void* base, int rows) {
     VcphipOzSide s;
     char* p = (char*)base;
     s.n1 = (double*)p; p += sizeof(double) * rows;
-    s.part = (double*)p; p += sizeof(double) * 4 * (size_t)(nchunk > 0 ? nchunk : 1) * rows;
+    s.part = (double*)p; p += sizeof(double) * 5 * (size_t)(nchunk > 0 ? nchunk : 1) * rows;
     s.e = (int*)p; p += sizeof(int) * rows;
     s.g = (int*)p; p += sizeof(int) * rows;
     s.h = (int*)p; p += sizeof(int) * rows;

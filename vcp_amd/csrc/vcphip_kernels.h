@@ -67,7 +67,11 @@ struct VcphipOzParams {
 #define VCPHIP_OZ_MAXK 65536  // 2 k 127^2 < 2^31 for the mixed digit products
 
 // per-row (A) / per-column (B) statistics of a midpoint-radius operand
-enum { VCPHIP_OZ_MIDZERO = 1, VCPHIP_OZ_SUMZERO = 2, VCPHIP_OZ_RADZERO = 4 };
+// NONFINITE: an end of some element of the row / column is +-inf or NaN
+// (the row / column is then split as zeros and every product entry it takes
+// part in is [-inf, +inf], or [NaN, NaN] with HASNAN)
+enum { VCPHIP_OZ_MIDZERO = 1, VCPHIP_OZ_SUMZERO = 2, VCPHIP_OZ_RADZERO = 4, VCPHIP_OZ_NONFINITE = 8,
+       VCPHIP_OZ_HASNAN = 16 };
 struct VcphipOzSide {
   int* e;       // mid exponent: max|mid| < 2^e
   int* g;       // digit scale of |mid|+rad
