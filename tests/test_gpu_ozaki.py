@@ -100,3 +100,28 @@ def test_ozaki_nonfinite_operands_match_oracle(ctx):
     assert C[7, 0, 0] == -np.inf and C[7, 0, 1] == np.inf and np.isnan(C[33, 0]).all() and np.isnan(C[0, 60]).all()
     P = ctx.igemm(IA, IB, vcp_amd.FORM_PLAIN)
     assert np.array_equal(P, O.pidblas_mulmm(IA, IB), equal_nan=True)
+
+
+def test_ozaki_extreme_exponents(ctx):
+    """rows near the overflow threshold and columns of subnormal values: the
+    scaling, the upward digits and the conversion of tiny and huge products
+    stay bit-identical to the oracle and contain the exact hull"""
+    rng = np.random.default_rng(43)
+    m, n, k = 40, 36, 300
+    IA, IB = _iv(rng, m, k, "wide"), _iv(rng, k, n, "wide")
+    IA[0] = np.ldexp(IA[0], 960)            # products near 2^1000
+    IA[1] = np.ldexp(IA[1], -1000)          # tiny rows
+    IB[:, 0] = np.ldexp(IB[:, 0], -1060)    # subnormal column (values down to 2^-1080)
+    IB[:, 1] = np.ldexp(IB[:, 1], 40)       # row 0 x column 1 overflows to +-inf ends
+    C = ctx.igemm(IA, IB, vcp_amd.FORM_OZAKI)
+    assert np.array_equal(C, O.ozaki_igemm(IA, IB, 16), equal_nan=True)
+    P = ctx.igemm(IA, IB, vcp_amd.FORM_PLAIN)
+    assert np.array_equal(P, O.pidblas_mulmm(IA, IB), equal_nan=True)
+    ii = np.array([0, 0, 1, 1, 2, 5])
+    jj = np.array([0, 2, 0, 3, 0, 7])
+    for X in (C, P):
+        ok, _, _ = O.check_containment(IA, IB, X, ii, jj)
+        assert ok.all()
+    # row 0 x column 1 overflows: an infinite end encloses it (the exact-hull
+    # checker works on finite bounds only)
+    assert C[0, 1, 0] == -np.inf or C[0, 1, 1] == np.inf
