@@ -45,6 +45,7 @@ def main():
                          "kernel for FETCH_SIZE / WRITE_SIZE counter runs")
     ap.add_argument("--no-infsup", action="store_true")
     ap.add_argument("--no-tuned", action="store_true")
+    ap.add_argument("--no-config2", action="store_true")
     ap.add_argument("--no-dd", action="store_true")
     ap.add_argument("--dd-n", type=int, default=4096)
     ap.add_argument("--dd-steps", type=int, default=5)
@@ -247,6 +248,7 @@ def main():
 
     resid = None if args.no_resid else bench_resid(ctx, args.resid_reps)
     dd = None if args.no_dd else bench_dd(ctx, args.dd_n, args.dd_steps, not args.no_cpu)
+    cfg2 = None if args.no_config2 else bench_config2(ctx)
 
     line = {
         "metric": "verified interval GEMM FP64-equiv TFLOP/s n=8192: Ozaki-int8 vs FP64-RU/RD",
@@ -275,6 +277,7 @@ def main():
         "cpu_baseline": cpu,
         "batched_residual": resid,
         "dd_gemm": dd,
+        "plain_n4096": cfg2,
     }
     print(json.dumps(line), flush=True)
     ctx.close()
@@ -343,6 +346,66 @@ def bench_resid(ctx, reps):
     except Exception as e:
         log("residual oracle check skipped: %r" % (e,))
     log("batched residual %s" % out)
+    return out
+
+
+def bench_config2(ctx, n=4096, steps=3):
+    """The plain FP64 directed-rounding forms at n=4096: point RU/RD product
+    (rmatmul +1/-1 in one pass, 4 n^3 flops) on A, B U[-1,1], and the
+    interval product (pidblas::mulmm, 8 n^3) with radii 2^-40 |mid|.  Checks
+    C_down <= exact <= C_up on sampled entries."""
+    import numpy as np
+    import torch
+
+    import vcp_amd
+    from vcp_amd import workloads as W
+
+    A, B = W.config2(n)
+    rng = np.random.default_rng(W.SEED0 + 21)
+    IA, IB = W.uniform_intervals(rng, n, n, -40), W.uniform_intervals(rng, n, n, -40)
+    dev = torch.device("cuda:0")
+    dA = torch.from_numpy(np.ascontiguousarray(A.T)).to(dev)
+    dB = torch.from_numpy(np.ascontiguousarray(B.T)).to(dev)
+    dU = torch.empty((n, n), dtype=torch.float64, device=dev)
+    dD = torch.empty((n, n), dtype=torch.float64, device=dev)
+    dIA = torch.from_numpy(np.ascontiguousarray(IA.transpose(1, 0, 2))).to(dev)
+    dIB = torch.from_numpy(np.ascontiguousarray(IB.transpose(1, 0, 2))).to(dev)
+    dC = torch.empty((n, n, 2), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    def timed(fn):
+        fn()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        ctx.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    t_pp = timed(lambda: ctx.gemm_rurd_dev(n, n, n, dA.data_ptr(), n, dB.data_ptr(), n, dU.data_ptr(),
+                                           dD.data_ptr(), n))
+    t_ii = timed(lambda: ctx.igemm_dev(vcp_amd.FORM_PLAIN, n, n, n, dIA.data_ptr(), n, dIB.data_ptr(), n,
+                                       dC.data_ptr(), n))
+    out = {"n": n,
+           "rurd_point": {"ms": round(t_pp * 1e3, 3), "tflops_4n3": round(4.0 * n ** 3 / t_pp / 1e12, 3),
+                          "frac_fp64_peak": round(4.0 * n ** 3 / t_pp / 1e12 / PEAK_FP64_TFLOPS, 4)},
+           "interval_plain": {"ms": round(t_ii * 1e3, 3), "tflops_8n3": round(8.0 * n ** 3 / t_ii / 1e12, 3),
+                              "frac_fp64_peak": round(8.0 * n ** 3 / t_ii / 1e12 / PEAK_FP64_TFLOPS, 4)}}
+    try:
+        import oracle_ffi as O
+
+        U = dU.cpu().numpy().T
+        D = dD.cpu().numpy().T
+        ii = rng.integers(0, n, 256)
+        jj = rng.integers(0, n, 256)
+        ok = 0
+        for i, j in zip(ii, jj):
+            lo, hi, _ = O.exact_dot(A[i], B[:, j])
+            ok += int(D[i, j] <= lo and hi <= U[i, j])
+        out["rurd_sampled_enclosure"] = {"samples": 256, "ok": ok}
+    except Exception as e:
+        log("config2 check skipped: %r" % (e,))
+    log("plain n=4096 %s" % out)
     return out
 
 

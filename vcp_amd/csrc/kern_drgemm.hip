@@ -219,7 +219,9 @@ __device__ __forceinline__ void dr_tile(int m, int n, int k, const double* __res
 }
 
 template <bool AI, bool BI>
-__global__ __launch_bounds__(NT, 2) void k_drgemm(int m, int n, int k, const double* __restrict__ A, long lda,
+// interval x interval tiles hold four chains per output (2 waves per SIMD);
+// the other forms fit in 168 VGPRs (3 waves per SIMD)
+__global__ __launch_bounds__(NT, (AI && BI) ? 2 : 3) void k_drgemm(int m, int n, int k, const double* __restrict__ A, long lda,
                                                    const double* __restrict__ B, long ldb, double* __restrict__ C,
                                                    long ldc, double* __restrict__ C2, int accumulate) {
   vcphip_mode_round_up();
@@ -236,7 +238,7 @@ __global__ __launch_bounds__(NT, 2) void k_drgemm(int m, int n, int k, const dou
 // launch: workgroups stride over a tile list sorted by problem size
 // (largest first), each tile being the PI form above.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(NT, 2) void k_resid_tiles(int ntiles, const VcphipResidTile* __restrict__ tiles,
+__global__ __launch_bounds__(NT, 3) void k_resid_tiles(int ntiles, const VcphipResidTile* __restrict__ tiles,
                                                         const VcphipResidProblem* __restrict__ probs) {
   vcphip_mode_round_up();
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {

@@ -57,7 +57,7 @@ struct vcphip_ctx {
     int kind, lm, ln, lk, variant, nmod;
   };
   std::vector<Tune> tune;
-  int resid_per_cu = 2;   // persistent workgroups per CU of the batched residual
+  int resid_per_cu = 3;   // persistent workgroups per CU of the batched residual (its occupancy)
 };
 
 namespace {
