@@ -155,6 +155,24 @@ int vcphip_residual_batched_dev(vcphip_ctx* ctx, int count, const int* n,
                                 const double* const* IA, const int* lda,
                                 double* const* C, const int* ldc, double* normub_dev);
 
+/* ---- verified linear solve: X contains A^-1 b ----------------------------- */
+/* Replaces imats::linearsolve (vcp/imats.hpp:292-362) after its LAPACK step:
+ * R is the caller's point approximate inverse of mid(IA) (the reference
+ * forms it with imid + pdblas::inv at :298-301).  With mx = R*mid(Ib)
+ * (nearest rounding), y = R*(IA*mx - Ib) and g >= ||R*IA - I||_inf, it
+ * writes X = mx + [-u, u] with u = |y| + |R*IA - I| * (||y||_inf / (1 - g)),
+ * every step in the reference's order and rounding, so X contains the
+ * solution of A x = b for every A in IA and b in Ib.  IA is n x n, Ib and X
+ * are n x nrhs intervals (AoS {inf, sup}), R is n x n.  *normub (may be NULL)
+ * receives g.  Returns VCPHIP_ERR_VERIFICATION when g >= 1 or is NaN
+ * (imats.hpp:330-333; X is then unspecified) and VCPHIP_ERR_DIMENSION for
+ * n < 2 (1 x 1 systems are the reference's scalar case, left to the host).
+ * Both forms synchronise the context's stream before returning. */
+int vcphip_lss(vcphip_ctx* ctx, int n, int nrhs, const double* IA, int lda, const double* Ib, int ldb,
+               const double* R, int ldr, double* X, int ldx, double* normub);
+int vcphip_lss_dev(vcphip_ctx* ctx, int n, int nrhs, const double* IA, int lda, const double* Ib, int ldb,
+                   const double* R, int ldr, double* X, int ldx, double* normub);
+
 /* ---- autotuner ------------------------------------------------------------
  * No counterpart in the reference (its BLAS path has nothing to tune).  The
  * choices live in the context and are keyed by product kind and size class

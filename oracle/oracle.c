@@ -372,6 +372,124 @@ int orc_residual(int n, const double* R, int ldr, const double* IA, int lda, dou
 }
 
 /* ------------------------------------------------------------------------ */
+/* verified linear solve (imats::linearsolve, vcp/imats.hpp:292-362)         */
+/* ------------------------------------------------------------------------ */
+/* kv's mid() as restated here: the nearest-rounded (lo + hi) / 2, or
+ * lo/2 + hi/2 where lo + hi overflows; point intervals give lo.  (kv's own
+ * rounding choice is unpinned: kv is not vendored.) */
+static double kv_mid(double lo, double hi) {
+    if (lo == hi) return lo;
+    volatile double s = lo + hi;
+    if (isinf(s)) return 0.5 * lo + 0.5 * hi;
+    return s * 0.5;
+}
+
+/* upper end of mats::norminf of an interval matrix (vcp/mats.hpp:1565-1604):
+ * a column ('C') gives the largest magnitude, a matrix ('M') the largest
+ * upward row sum of magnitudes (each sum from column 0 up); rows >= 2.
+ * The maximum keeps the first row's value and replaces it only by larger
+ * ones (so a NaN in row 0 stays, later NaNs are passed over). */
+static double norminf_ub(int rows, int cols, const double* X, int ldx) {
+    const int old = fegetround();
+    fesetround(FE_UPWARD);
+    double best = 0.0;
+    for (int i = 0; i < rows; ++i) {
+        volatile double s = 0.0;
+        for (int j = 0; j < cols; ++j) {
+            const double lo = X[2 * ((size_t)i + (size_t)ldx * j)], hi = X[2 * ((size_t)i + (size_t)ldx * j) + 1];
+            const double mag = fabs(lo) > fabs(hi) ? fabs(lo) : fabs(hi);
+            s = (j == 0) ? mag : s + mag;
+        }
+        if (i == 0 || s > best) best = s;
+    }
+    fesetround(old);
+    return best;
+}
+
+static void interval_abs(size_t cnt, double* X) {   /* kv abs(), in place */
+    for (size_t t = 0; t < cnt; ++t) {
+        const double lo = X[2 * t], hi = X[2 * t + 1];
+        if (lo >= 0.0) continue;
+        if (hi <= 0.0) { X[2 * t] = -hi; X[2 * t + 1] = -lo; continue; }
+        X[2 * t] = 0.0;
+        X[2 * t + 1] = -lo > hi ? -lo : hi;
+    }
+}
+
+int orc_lss(int n, int nrhs, const double* IA, int lda, const double* Ib, int ldb, const double* R, int ldr,
+            double* X, int ldx, double* normub) {
+    if (n < 2 || nrhs < 1 || lda < n || ldb < n || ldr < n || ldx < n) return -1;
+    const size_t nr = (size_t)n * nrhs;
+    double *mb = dalloc(nr), *mx = dalloc(nr), *Xi = dalloc(2 * nr), *Y = dalloc(2 * nr), *Tm = dalloc(2 * nr);
+    double *W = dalloc(2 * nr), *C = dalloc(2 * (size_t)n * n);
+    for (int j = 0; j < nrhs; ++j)                                  /* mb = mid(b), :299 */
+        for (int i = 0; i < n; ++i)
+            mb[i + (size_t)n * j] = kv_mid(Ib[2 * (i + (size_t)ldb * j)], Ib[2 * (i + (size_t)ldb * j) + 1]);
+    orc_rgemm(n, nrhs, n, R, ldr, mb, n, mx, n, 0, 0);               /* mx = R mb, :302 */
+    orc_pidblas_mul_im_m(n, nrhs, n, IA, lda, mx, n, Xi, n);         /* x = A mx, :306 */
+    {                                                               /* x = x - b, :308 */
+        const int old = fegetround();
+        for (int j = 0; j < nrhs; ++j)
+            for (int i = 0; i < n; ++i) {
+                double* x = Xi + 2 * (i + (size_t)n * j);
+                const double* b = Ib + 2 * (i + (size_t)ldb * j);
+                volatile double xl = x[0], xu = x[1], bl = b[0], bu = b[1];
+                fesetround(FE_DOWNWARD);
+                x[0] = xl - bu;
+                fesetround(FE_UPWARD);
+                x[1] = xu - bl;
+            }
+        fesetround(old);
+    }
+    orc_pidblas_mul_m_im(n, nrhs, n, R, ldr, Xi, n, Y, n);          /* b = R x, :310 */
+    interval_abs(nr, Y);                                            /* x = |b|, :312-315 */
+    double g = 0.0;
+    orc_residual(n, R, ldr, IA, lda, C, n, &g);                     /* R A - I, ||.||, :318-329 */
+    *normub = g;
+    int rc = 0;
+    if (!(g < 1.0)) {                                               /* :330-333 */
+        rc = 1;
+        goto done;
+    }
+    {
+        const double t = norminf_ub(n, nrhs, Y, n);                  /* T = ||x||, :335-336 */
+        interval_abs((size_t)n * n, C);                             /* b = |RA - I|, :338 */
+        double glo, ghi;
+        {                                                           /* G = T / (1 - G), :341 */
+            const int old = fegetround();
+            volatile double one = 1.0, gv = g, tv = t;
+            fesetround(FE_DOWNWARD);
+            volatile double ddn = one - gv;
+            fesetround(FE_UPWARD);
+            volatile double dup = one - gv;
+            fesetround(FE_DOWNWARD);
+            glo = tv / dup;
+            fesetround(FE_UPWARD);
+            ghi = tv / ddn;
+            fesetround(old);
+        }
+        for (size_t q = 0; q < nr; ++q) { Tm[2 * q] = glo; Tm[2 * q + 1] = ghi; }   /* :343-348 */
+        orc_pidblas_mulmm(n, nrhs, n, C, n, Tm, n, W, n);            /* G = |RA - I| T, :350 */
+        const int old = fegetround();
+        for (int j = 0; j < nrhs; ++j)                              /* :352-361 */
+            for (int i = 0; i < n; ++i) {
+                const size_t q = i + (size_t)n * j;
+                fesetround(FE_UPWARD);
+                volatile double u = Y[2 * q + 1] + W[2 * q + 1];
+                volatile double m = mx[q];
+                fesetround(FE_DOWNWARD);
+                X[2 * (i + (size_t)ldx * j)] = -u + m;
+                fesetround(FE_UPWARD);
+                X[2 * (i + (size_t)ldx * j) + 1] = u + m;
+            }
+        fesetround(old);
+    }
+done:
+    free(mb); free(mx); free(Xi); free(Y); free(Tm); free(W); free(C);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Exact arithmetic: superaccumulator                                       */
 /* ------------------------------------------------------------------------ */
 #define SA_EMIN (-2208) /* bit weight of digit 0's least significant bit   */

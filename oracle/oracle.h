@@ -58,6 +58,15 @@ int orc_pidblas_fma_mulmm(int m, int n, int k, const double* A, int lda, const d
 int orc_residual(int n, const double* R, int ldr, const double* IA, int lda, double* C, int ldc,
                  double* normub);
 
+/* ---- verified linear solve (imats::linearsolve, vcp/imats.hpp:292-362) --- */
+/* After the reference's LAPACK step: R is the approximate inverse of
+ * mid(IA).  X (n x nrhs intervals) = mx + [-u, u], u = |y| + |RA - I| G,
+ * with mx = R mid(b) (nearest), y = R (IA mx - b), G = ||y|| / (1 - g),
+ * g = ||R IA - I||_inf (written to *normub).  Returns 0, 1 when g >= 1 or
+ * is NaN (the reference throws, :330-333; X is untouched), -1 for n < 2. */
+int orc_lss(int n, int nrhs, const double* IA, int lda, const double* Ib, int ldb, const double* R, int ldr,
+            double* X, int ldx, double* normub);
+
 /* ---- exact evaluation (superaccumulator; no rounding anywhere) ---------- */
 /* For entry (i,j) of IA*IB: the exact hull [lo*, hi*] of { sum_k a_k b_k }.
  * Writes sign(lo* - C_lower) and sign(C_upper - hi*) (each >= 0 means the

@@ -48,6 +48,7 @@ def lib():
             "orc_pidblas_mulltmm": [_c_int, _c_int, _p_dbl, _c_int, _p_dbl, _c_int],
             "orc_pidblas_vmulmm": [_c_int, _c_int, _p_dbl, _c_int, _p_dbl, _c_int],
             "orc_residual": [_c_int, _p_dbl, _c_int, _p_dbl, _c_int, _p_dbl, _c_int, _p_dbl],
+            "orc_lss": [_c_int, _c_int, _p_dbl, _c_int, _p_dbl, _c_int, _p_dbl, _c_int, _p_dbl, _c_int, _p_dbl],
             "orc_exact_hull_entry": [_c_int, _p_dbl, _c_int, _p_dbl, _c_int, _c_dbl, _c_dbl,
                                      _p_int, _p_int, _p_dbl, _p_dbl],
             "orc_exact_dot": [_c_int, _p_dbl, _c_int, _p_dbl, _c_int, _p_dbl, _p_dbl, _p_dbl],
@@ -194,6 +195,21 @@ def residual(R, IA):
     lib().orc_residual(n, _dp(R), max(n, 1), _dp(IA.transpose(1, 0, 2)), _ld_pair(IA),
                        _dp(C.transpose(1, 0, 2)), _ld_pair(C), ctypes.byref(nub))
     return C, nub.value
+
+
+def lss(IA, Ib, R):
+    """imats::linearsolve after its LAPACK step (R ~ inv(mid IA)): returns
+    (X, g, ok) with X the enclosure, g >= ||R IA - I||_inf and ok False
+    when g >= 1 (the reference's verification failure; X is then zero)."""
+    R = np.asfortranarray(R, dtype=np.float64)
+    n, nrhs = Ib.shape[:2]
+    X = pairs(n, nrhs)
+    g = ctypes.c_double()
+    rc = lib().orc_lss(n, nrhs, _dp(IA.transpose(1, 0, 2)), _ld_pair(IA), _dp(Ib.transpose(1, 0, 2)), _ld_pair(Ib),
+                       _dp(R), max(n, 1), _dp(X.transpose(1, 0, 2)), _ld_pair(X), ctypes.byref(g))
+    if rc < 0:
+        raise ValueError("orc_lss: bad dimensions")
+    return X, g.value, rc == 0
 
 
 def exact_dot(a, b):

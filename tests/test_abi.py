@@ -53,6 +53,8 @@ def test_null_context_is_rejected_without_device_work():
     assert L.vcphip_igemm(None, 0, 4, 4, 4, None, 4, None, 4, None, 4) == -2
     assert L.vcphip_gemm_rurd_dev(None, 4, 4, 4, None, 4, None, 4, None, None, 4, 0) == -2
     assert L.vcphip_ctx_synchronize(None) == -2
+    assert L.vcphip_lss(None, 4, 1, None, 4, None, 4, None, 4, None, 4, None) == -2
+    assert L.vcphip_lss_dev(None, 4, 1, None, 4, None, 4, None, 4, None, 4, None) == -2
     assert L.vcphip_ctx_create(0, None) == -2
 
 

@@ -67,6 +67,8 @@ SIGNATURES = {
     "vcphip_ddgemm": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_ddgemm_dev": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_ddgemm_tn": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int],
+    "vcphip_lss": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _p_dbl],
+    "vcphip_lss_dev": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _p_dbl],
     "vcphip_residual_batched": [_vp, _c_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _p_dbl],
     "vcphip_residual_batched_dev": [_vp, _c_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _vp],
     "vcphip_tune_igemm_tile": [_vp, _c_int, _c_int, _c_int, _p_int, _p_dbl],
@@ -308,7 +310,37 @@ class Context:
                "vcphip_residual_batched")
         return Cs, list(nub)
 
+    def lss(self, IA, Ib, R=None):
+        """Verified solution of A x = b (imats::linearsolve, vcp/imats.hpp:
+        292-362): returns (X, g), X (n, nrhs, 2) containing A^-1 b for every
+        A in IA and b in Ib, g >= ||R IA - I||_inf.  R defaults to the host
+        LAPACK inverse of mid(IA) (numpy), the reference's step at :298-301.
+        Raises VcphipError (VCPHIP_ERR_VERIFICATION) when g >= 1, as the
+        reference throws vcp::verification_error."""
+        n = IA.shape[0]
+        if IA.shape[1] != n or Ib.shape[0] != n:
+            raise ValueError("dimension mismatch")
+        if R is None:
+            R = np.linalg.inv(0.5 * IA[..., 0] + 0.5 * IA[..., 1])
+        R = np.asfortranarray(R, dtype=np.float64)
+        nrhs = Ib.shape[1]
+        X = pairs(n, nrhs)
+        pa, lda = _pair_ptr(IA)
+        pb, ldb = _pair_ptr(Ib)
+        pr, ldr = _mat_ptr(R)
+        px, ldx = _pair_ptr(X)
+        g = ctypes.c_double()
+        _check(lib().vcphip_lss(self._h, n, nrhs, pa, lda, pb, ldb, pr, ldr, px, ldx, ctypes.byref(g)), "vcphip_lss")
+        return X, g.value
+
     # ---- device-pointer products (torch tensors or raw pointers) -----------
+    def lss_dev(self, n, nrhs, dA, lda, dB, ldb, dR, ldr, dX, ldx):
+        """device-pointer form of lss(); returns g (raises when g >= 1)"""
+        g = ctypes.c_double()
+        _check(lib().vcphip_lss_dev(self._h, n, nrhs, _vp(dA), lda, _vp(dB), ldb, _vp(dR), ldr, _vp(dX), ldx,
+                                    ctypes.byref(g)), "vcphip_lss_dev")
+        return g.value
+
     def igemm_dev(self, form, m, n, k, dA, lda, dB, ldb, dC, ldc):
         _check(lib().vcphip_igemm_dev(self._h, form, m, n, k, _vp(dA), lda, _vp(dB), ldb, _vp(dC), ldc),
                "vcphip_igemm_dev")

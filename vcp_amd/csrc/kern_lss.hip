@@ -1,0 +1,280 @@
+// Verified linear solve around R*A - I (imats::linearsolve,
+// vcp/imats.hpp:292-362): the elementwise and scalar steps between the
+// products.  The products themselves are the k_drgemm forms (IP for A*mx,
+// PI for R*x and R*A, II for |RA - I| * T) and the nearest-rounded point
+// product R*mid(b) below; the host drives the sequence
+// (vcphip_lss_dev in vcphip_host.hip).
+//
+// Kernels that need directed rounding run in round-toward-+inf (see
+// vcphip_device.h) and form RD(x) = -RU(-x).  k_lss_mid, k_lss_gemv_rn and
+// k_lss_fill_g run in the default round-to-nearest mode: the midpoint and
+// R*mid(b) are nearest-rounded in the reference (no rounding mode is set
+// around them), and the scalar G = T / (1 - ||RA - I||) gets its directed
+// roundings from exact error terms of nearest-rounded operations.
+#include "vcphip_device.h"
+#include "vcphip_kernels.h"
+
+namespace {
+
+constexpr int NT = 256;
+
+// kv's mid(): the nearest-rounded average, (lo + hi) / 2, or lo/2 + hi/2
+// where lo + hi overflows; point intervals give lo.
+__global__ __launch_bounds__(NT) void k_lss_mid(int n, int ncol, const double* __restrict__ b, long ldb,
+                                                double* __restrict__ mb) {
+  const long idx = (long)blockIdx.x * NT + threadIdx.x;
+  if (idx >= (long)n * ncol) return;
+  const int i = (int)(idx % n), j = (int)(idx / n);
+  const double2 v = *reinterpret_cast<const double2*>(b + 2 * (i + ldb * j));
+  double m;
+  if (v.x == v.y) {
+    m = v.x;
+  } else {
+    const double s = v.x + v.y;
+    m = __builtin_isinf(s) ? 0.5 * v.x + 0.5 * v.y : s * 0.5;
+  }
+  mb[idx] = m;
+}
+
+// C = A * B in round-to-nearest, each element the k-ascending FMA chain
+// from 0 (the rmatmul contract with nearest rounding, vblas/README.md:
+// 167-170).  One thread per output row and column; lanes of a wave take
+// consecutive rows, so the column-major A is read coalesced.
+__global__ __launch_bounds__(NT) void k_lss_gemv_rn(int m, int k, const double* __restrict__ A, long lda,
+                                                    const double* __restrict__ B, long ldb,
+                                                    double* __restrict__ C, long ldc) {
+  const int i = blockIdx.x * NT + threadIdx.x, j = blockIdx.y;
+  if (i >= m) return;
+  const double* a = A + i;
+  const double* bj = B + ldb * (long)j;
+  double acc = 0.0;
+  int kk = 0;
+  for (; kk + 4 <= k; kk += 4) {
+    const double a0 = a[lda * kk], a1 = a[lda * (kk + 1)], a2 = a[lda * (kk + 2)], a3 = a[lda * (kk + 3)];
+    acc = __builtin_fma(a0, bj[kk], acc);
+    acc = __builtin_fma(a1, bj[kk + 1], acc);
+    acc = __builtin_fma(a2, bj[kk + 2], acc);
+    acc = __builtin_fma(a3, bj[kk + 3], acc);
+  }
+  for (; kk < k; ++kk) acc = __builtin_fma(a[lda * kk], bj[kk], acc);
+  C[i + ldc * (long)j] = acc;
+}
+
+// X = X - b (interval subtraction, imats.hpp:308): [RD(xl - bu), RU(xu - bl)]
+__global__ __launch_bounds__(NT) void k_lss_sub(int n, int ncol, double* __restrict__ X, long ldx,
+                                                const double* __restrict__ b, long ldb) {
+  vcphip_mode_round_up();
+  const long idx = (long)blockIdx.x * NT + threadIdx.x;
+  if (idx >= (long)n * ncol) return;
+  const int i = (int)(idx % n), j = (int)(idx / n);
+  double2* x = reinterpret_cast<double2*>(X + 2 * (i + ldx * j));
+  const double2 bv = *reinterpret_cast<const double2*>(b + 2 * (i + ldb * j));
+  const double2 xv = *x;
+  *x = make_double2(-(bv.y - xv.x), xv.y - bv.x);
+}
+
+// kv's abs() of an interval, in place (imats.hpp:315, :338): exact
+__global__ __launch_bounds__(NT) void k_lss_abs(int rows, int cols, double* __restrict__ X, long ldx) {
+  const long idx = (long)blockIdx.x * NT + threadIdx.x;
+  if (idx >= (long)rows * cols) return;
+  const int i = (int)(idx % rows), j = (int)(idx / rows);
+  double2* x = reinterpret_cast<double2*>(X + 2 * (i + ldx * j));
+  const double2 v = *x;
+  double lo, hi;
+  if (v.x >= 0.0) {
+    lo = v.x;
+    hi = v.y;
+  } else if (v.y <= 0.0) {
+    lo = -v.y;
+    hi = -v.x;
+  } else {
+    lo = 0.0;
+    hi = -v.x > v.y ? -v.x : v.y;
+  }
+  *x = make_double2(lo, hi);
+}
+
+// C = C - I on the diagonal (imats.hpp:324-326): [RD(lo - 1), RU(hi - 1)]
+__global__ __launch_bounds__(NT) void k_lss_diag(int n, double* __restrict__ C, long ldc) {
+  vcphip_mode_round_up();
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i >= n) return;
+  double2* c = reinterpret_cast<double2*>(C + 2 * (i + ldc * (long)i));
+  const double2 v = *c;
+  *c = make_double2(-(1.0 - v.x), v.y - 1.0);
+}
+
+// upper end of the row sums of |X| (mats::norminf, vcp/mats.hpp:1587-1592):
+// s = mag(x_i0), then s = RU(s + mag(x_ij)) for j = 1, 2, ...
+__global__ __launch_bounds__(NT) void k_lss_rowsum(int rows, int cols, const double* __restrict__ X, long ldx,
+                                                   double* __restrict__ out) {
+  vcphip_mode_round_up();
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i >= rows) return;
+  double s = 0.0;
+  for (int j = 0; j < cols; ++j) {
+    const double2 v = *reinterpret_cast<const double2*>(X + 2 * (i + ldx * (long)j));
+    const double a = __builtin_fabs(v.x), b = __builtin_fabs(v.y);
+    const double mag = a > b ? a : b;
+    s = (j == 0) ? mag : s + mag;
+  }
+  out[i] = s;
+}
+
+// maximum of in[0..rows) into *out with the reference's rule (mats.hpp:
+// 1593-1598): best = in[0], then best = in[i] where in[i] > best.  So the
+// result is NaN exactly when in[0] is, and otherwise the largest non-NaN.
+__global__ __launch_bounds__(NT) void k_lss_max(int rows, const double* __restrict__ in, double* __restrict__ out) {
+  __shared__ double red[NT];
+  double best = -__builtin_inf();
+  for (int i = threadIdx.x; i < rows; i += NT) {
+    const double v = in[i];
+    best = v > best ? v : best;
+  }
+  red[threadIdx.x] = best;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      const double o = red[threadIdx.x + s];
+      red[threadIdx.x] = o > red[threadIdx.x] ? o : red[threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = __builtin_isnan(in[0]) ? in[0] : red[0];
+}
+
+// ---- directed roundings from nearest-rounded operations --------------------
+// next double above / below x for finite x >= 0 (x > 0 for below)
+VCPHIP_DEVINL double next_up_nn(double x) { return __longlong_as_double(__double_as_longlong(x) + 1); }
+VCPHIP_DEVINL double next_dn_pos(double x) { return __longlong_as_double(__double_as_longlong(x) - 1); }
+
+// RD and RU of t / d for t >= 0 and 2^-960 <= d (here d is 1 - g with
+// 0 <= g < 1, so d >= 2^-53).  q = RN(t / d); when t >= 2^-960 the
+// remainder t - q d is a double and the FMA gives it exactly, and its sign
+// says on which side of q the quotient lies.  Smaller t is scaled by 2^128
+// first; scaling the directed results back by 2^-128 is then corrected to
+// the same direction (the scaled-back product is exact unless it is
+// subnormal, and multiplying it by 2^128 again is always exact).
+VCPHIP_DEVINL void div_dir_core(double t, double d, double& dn, double& up) {
+  const double q = t / d;
+  const double r = __builtin_fma(-q, d, t);
+  up = r > 0.0 ? next_up_nn(q) : q;
+  dn = (r < 0.0 && q > 0.0) ? next_dn_pos(q) : q;
+}
+
+VCPHIP_DEVINL void div_dir(double t, double d, double& dn, double& up) {
+  const double tiny = vcphip_pow2(-960);
+  if (!(t < tiny)) {   // also t = inf or NaN: q carries it through
+    div_dir_core(t, d, dn, up);
+    return;
+  }
+  double sdn, sup;
+  div_dir_core(t * vcphip_pow2(128), d, sdn, sup);
+  const double s = vcphip_pow2(-128), S = vcphip_pow2(128);
+  up = sup * s;
+  if (up * S < sup) up = next_up_nn(up);
+  dn = sdn * s;
+  if (dn * S > sdn) dn = next_dn_pos(dn);
+}
+
+// 1 - g rounded down and up (TwoSum: the error of s = RN(1 - g) is exact)
+VCPHIP_DEVINL void one_minus_dir(double g, double& dn, double& up) {
+  const double a = 1.0, b = -g;
+  const double s = a + b;
+  const double bb = s - a;
+  const double e = (a - (s - bb)) + (b - bb);
+  up = e > 0.0 ? next_up_nn(s) : s;
+  dn = e < 0.0 ? next_dn_pos(s) : s;
+}
+
+// G = T / (1 - g) as an interval (imats.hpp:341; g is ||RA - I|| with its
+// lower end set to its upper end, T likewise), written to every element of
+// the n x ncol interval matrix Tm (imats.hpp:343-348) and to Gout.  When
+// g >= 1 (verification failed) or g is NaN, G is [NaN, NaN].
+__global__ __launch_bounds__(NT) void k_lss_fill_g(int n, int ncol, const double* __restrict__ norms,
+                                                   double* __restrict__ Tm, long ldt, double* __restrict__ Gout) {
+  const double g = norms[0], t = norms[1];
+  double glo, ghi;
+  if (g < 1.0) {
+    double ddn, dup;
+    one_minus_dir(g, ddn, dup);
+    double lo_dn, lo_up, hi_dn, hi_up;
+    div_dir(t, dup, lo_dn, lo_up);   // lower end: RD(t / RU(1 - g))
+    div_dir(t, ddn, hi_dn, hi_up);   // upper end: RU(t / RD(1 - g))
+    glo = lo_dn;
+    ghi = hi_up;
+  } else {
+    glo = ghi = __builtin_nan("");
+  }
+  const long idx = (long)blockIdx.x * NT + threadIdx.x;
+  if (idx == 0) {
+    Gout[0] = glo;
+    Gout[1] = ghi;
+  }
+  if (idx >= (long)n * ncol) return;
+  const int i = (int)(idx % n), j = (int)(idx / n);
+  *reinterpret_cast<double2*>(Tm + 2 * (i + ldt * j)) = make_double2(glo, ghi);
+}
+
+// X = mx + [-u, u] with u = RU(|y|u + Wu) (imats.hpp:352-361):
+// [RD(-u + mx), RU(u + mx)]
+__global__ __launch_bounds__(NT) void k_lss_final(int n, int ncol, const double* __restrict__ Y,
+                                                  const double* __restrict__ W, const double* __restrict__ mx,
+                                                  double* __restrict__ X, long ldx) {
+  vcphip_mode_round_up();
+  const long idx = (long)blockIdx.x * NT + threadIdx.x;
+  if (idx >= (long)n * ncol) return;
+  const int i = (int)(idx % n), j = (int)(idx / n);
+  const double yu = Y[2 * idx + 1], wu = W[2 * idx + 1], m = mx[idx];
+  const double u = yu + wu;
+  *reinterpret_cast<double2*>(X + 2 * (i + ldx * j)) = make_double2(-(u - m), u + m);
+}
+
+inline dim3 grid_of(long count) { return dim3((unsigned)((count + NT - 1) / NT)); }
+
+}  // namespace
+
+int vcphip_lss_launch_mid(int n, int ncol, const double* b, long ldb, double* mb, hipStream_t s) {
+  hipLaunchKernelGGL(k_lss_mid, grid_of((long)n * ncol), dim3(NT), 0, s, n, ncol, b, ldb, mb);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_lss_launch_gemv_rn(int m, int ncol, int k, const double* A, long lda, const double* B, long ldb, double* C,
+                              long ldc, hipStream_t s) {
+  hipLaunchKernelGGL(k_lss_gemv_rn, dim3((m + NT - 1) / NT, ncol), dim3(NT), 0, s, m, k, A, lda, B, ldb, C, ldc);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_lss_launch_sub(int n, int ncol, double* X, long ldx, const double* b, long ldb, hipStream_t s) {
+  hipLaunchKernelGGL(k_lss_sub, grid_of((long)n * ncol), dim3(NT), 0, s, n, ncol, X, ldx, b, ldb);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_lss_launch_abs(int rows, int cols, double* X, long ldx, hipStream_t s) {
+  hipLaunchKernelGGL(k_lss_abs, grid_of((long)rows * cols), dim3(NT), 0, s, rows, cols, X, ldx);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_lss_launch_diag(int n, double* C, long ldc, hipStream_t s) {
+  hipLaunchKernelGGL(k_lss_diag, grid_of(n), dim3(NT), 0, s, n, C, ldc);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_lss_launch_norm(int rows, int cols, const double* X, long ldx, double* rowbuf, double* out,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_lss_rowsum, grid_of(rows), dim3(NT), 0, s, rows, cols, X, ldx, rowbuf);
+  hipLaunchKernelGGL(k_lss_max, dim3(1), dim3(NT), 0, s, rows, rowbuf, out);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_lss_launch_fill_g(int n, int ncol, const double* norms, double* Tm, long ldt, double* Gout,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_lss_fill_g, grid_of((long)n * ncol), dim3(NT), 0, s, n, ncol, norms, Tm, ldt, Gout);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_lss_launch_final(int n, int ncol, const double* Y, const double* W, const double* mx, double* X, long ldx,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(k_lss_final, grid_of((long)n * ncol), dim3(NT), 0, s, n, ncol, Y, W, mx, X, ldx);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}

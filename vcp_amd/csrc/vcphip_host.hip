@@ -23,7 +23,7 @@ struct vcphip_ctx {
   hipStream_t stream = nullptr;
   long long oz_nmod = 0, oz_variant = 0, timing = 0, dd_nmod = 0;
   // workspace slots (device memory, grown on demand)
-  enum { WS_PA, WS_PB, WS_U, WS_Z, WS_SA, WS_SB, WS_HA, WS_HB, WS_HC, WS_HC2, WS_BATCH, WS_DD, WS_N };
+  enum { WS_PA, WS_PB, WS_U, WS_Z, WS_SA, WS_SB, WS_HA, WS_HB, WS_HC, WS_HC2, WS_BATCH, WS_DD, WS_LSS, WS_HR, WS_N };
   void* ws[WS_N] = {};
   size_t ws_size[WS_N] = {};
   // stage timing: events recorded by every call since the last synchronize,
@@ -711,6 +711,59 @@ extern "C" int vcphip_residual_batched_dev(vcphip_ctx* c, int count, const int* 
 }
 
 // ===========================================================================
+// verified linear solve (imats::linearsolve, vcp/imats.hpp:292-362)
+// ===========================================================================
+extern "C" int vcphip_lss_dev(vcphip_ctx* c, int n, int nrhs, const double* IA, int lda, const double* Ib, int ldb,
+                              const double* R, int ldr, double* X, int ldx, double* normub) {
+  if (!c || !IA || !Ib || !R || !X) return VCPHIP_ERR_ARGUMENT;
+  if (n < 2 || nrhs < 1 || lda < n || ldb < n || ldr < n || ldx < n) return VCPHIP_ERR_DIMENSION;
+  hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  const size_t nr = (size_t)n * nrhs, al = 32;   // 256-byte alignment in doubles
+  auto up = [&](size_t x) { return (x + al - 1) / al * al; };
+  const size_t o_mb = 0, o_mx = o_mb + up(nr), o_x = o_mx + up(nr), o_y = o_x + up(2 * nr), o_t = o_y + up(2 * nr);
+  const size_t o_w = o_t + up(2 * nr), o_c = o_w + up(2 * nr), o_r = o_c + up(2 * (size_t)n * n);
+  const size_t o_s = o_r + up((size_t)n), total = o_s + al;
+  void* base;
+  const int wrc = ws_get(c, vcphip_ctx::WS_LSS, total * sizeof(double), &base);
+  if (wrc) return wrc;
+  double* w = (double*)base;
+  double *mb = w + o_mb, *mx = w + o_mx, *Xi = w + o_x, *Y = w + o_y, *Tm = w + o_t, *W = w + o_w, *C = w + o_c;
+  double *rows = w + o_r, *norms = w + o_s;   // norms: g, T, G lower, G upper
+  StageTimer T(c);
+  int rc = 0;
+  // mx = R * mid(b), nearest rounding (:299-302)
+  rc |= vcphip_lss_launch_mid(n, nrhs, Ib, ldb, mb, s);
+  rc |= vcphip_lss_launch_gemv_rn(n, nrhs, n, R, ldr, mb, n, mx, n, s);
+  // y = R * (IA*mx - b) and |y| (:305-315)
+  rc |= vcphip_launch_drgemm(VCPHIP_DR_IP, n, nrhs, n, IA, lda, mx, n, Xi, n, nullptr, 0, s);
+  rc |= vcphip_lss_launch_sub(n, nrhs, Xi, n, Ib, ldb, s);
+  rc |= vcphip_launch_drgemm(VCPHIP_DR_PI, n, nrhs, n, R, ldr, Xi, n, Y, n, nullptr, 0, s);
+  rc |= vcphip_lss_launch_abs(n, nrhs, Y, n, s);
+  T.mark("lss_rhs");
+  // R*IA - I and g = ||R*IA - I||_inf (:318-329)
+  rc |= vcphip_launch_drgemm(VCPHIP_DR_PI, n, n, n, R, ldr, IA, lda, C, n, nullptr, 0, s);
+  rc |= vcphip_lss_launch_diag(n, C, n, s);
+  rc |= vcphip_lss_launch_norm(n, n, C, n, rows, norms, s);
+  T.mark("lss_resid");
+  // T = ||y||_inf, G = T / (1 - g), |R*IA - I| * G (:334-350)
+  rc |= vcphip_lss_launch_norm(n, nrhs, Y, n, rows, norms + 1, s);
+  rc |= vcphip_lss_launch_abs(n, n, C, n, s);
+  rc |= vcphip_lss_launch_fill_g(n, nrhs, norms, Tm, n, norms + 2, s);
+  rc |= vcphip_launch_drgemm(VCPHIP_DR_II, n, nrhs, n, C, n, Tm, n, W, n, nullptr, 0, s);
+  // X = mx + [-u, u], u = |y| + W (:352-361)
+  rc |= vcphip_lss_launch_final(n, nrhs, Y, W, mx, X, ldx, s);
+  T.mark("lss_bound");
+  if (rc) return VCPHIP_ERR_DEVICE;
+  double g = 0.0;
+  if (hipMemcpyAsync(&g, norms, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return VCPHIP_ERR_DEVICE;
+  if (normub) *normub = g;
+  return (g < 1.0) ? VCPHIP_OK : VCPHIP_ERR_VERIFICATION;
+}
+
+// ===========================================================================
 // host-pointer entry points: stage through the context workspace
 // ===========================================================================
 namespace {
@@ -875,6 +928,24 @@ extern "C" int vcphip_ddgemm_tn(vcphip_ctx* c, int k, int n, const double* A, in
   VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HC, 16 * (size_t)n * n, &p));
   VCPHIP_TRY(dd_gemm_dev(c, true, n, n, k, dA, k > 0 ? k : 1, dA, k > 0 ? k : 1, (double*)p, n));
   return to_host(c, (double*)p, C, ldc, n, n, 16);
+}
+
+extern "C" int vcphip_lss(vcphip_ctx* c, int n, int nrhs, const double* IA, int lda, const double* Ib, int ldb,
+                          const double* R, int ldr, double* X, int ldx, double* normub) {
+  if (!c || !IA || !Ib || !R || !X) return VCPHIP_ERR_ARGUMENT;
+  if (n < 2 || nrhs < 1 || lda < n || ldb < n || ldr < n || ldx < n) return VCPHIP_ERR_DIMENSION;
+  hipSetDevice(c->device);
+  double *dA, *dB, *dR, *dX;
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HA, IA, lda, n, n, 16, &dA));
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HB, Ib, ldb, n, nrhs, 16, &dB));
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HR, R, ldr, n, n, 8, &dR));
+  void* px;
+  VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HC2, 16 * (size_t)n * nrhs, &px));
+  dX = (double*)px;
+  const int rc = vcphip_lss_dev(c, n, nrhs, dA, n, dB, n, dR, n, dX, n, normub);
+  if (rc != VCPHIP_OK && rc != VCPHIP_ERR_VERIFICATION) return rc;
+  VCPHIP_TRY(to_host(c, dX, X, ldx, n, nrhs, 16));
+  return rc;
 }
 
 extern "C" int vcphip_residual_batched(vcphip_ctx* c, int count, const int* n, const double* const* R, const int* ldr,

@@ -112,3 +112,17 @@ int vcphip_dd_recon(int m, int n, int mpad, int npad, const VcphipOzParams& P, c
 int vcphip_gather_rows(const double* A, long lda, int m, int k, int s, double* out, hipStream_t st);
 int vcphip_ratio_max(const double* X, const double* Y, long total, double* part, int nblocks, hipStream_t st);
 int vcphip_fill_random(void* p, size_t bytes, uint32_t seed, hipStream_t st);
+
+// verified linear solve: elementwise and scalar steps (kern_lss.hip)
+int vcphip_lss_launch_mid(int n, int ncol, const double* b, long ldb, double* mb, hipStream_t s);
+int vcphip_lss_launch_gemv_rn(int m, int ncol, int k, const double* A, long lda, const double* B, long ldb, double* C,
+                              long ldc, hipStream_t s);
+int vcphip_lss_launch_sub(int n, int ncol, double* X, long ldx, const double* b, long ldb, hipStream_t s);
+int vcphip_lss_launch_abs(int rows, int cols, double* X, long ldx, hipStream_t s);
+int vcphip_lss_launch_diag(int n, double* C, long ldc, hipStream_t s);
+int vcphip_lss_launch_norm(int rows, int cols, const double* X, long ldx, double* rowbuf, double* out,
+                           hipStream_t s);
+int vcphip_lss_launch_fill_g(int n, int ncol, const double* norms, double* Tm, long ldt, double* Gout,
+                             hipStream_t s);
+int vcphip_lss_launch_final(int n, int ncol, const double* Y, const double* W, const double* mx, double* X, long ldx,
+                            hipStream_t s);
