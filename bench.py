@@ -346,6 +346,42 @@ def bench_resid(ctx, reps):
     except Exception as e:
         log("residual oracle check skipped: %r" % (e,))
     log("batched residual %s" % out)
+    # the whole verified solve on the same problems (imats::linearsolve after
+    # its LAPACK step): one right-hand side each, radii 2^-50 |b|
+    rng = np.random.default_rng(W.SEED0 + 51)
+    Ibs = []
+    for n in ns:
+        b = rng.uniform(-1.0, 1.0, (n, 1))
+        Ibs.append(W.interval_from_midrad(b, np.ldexp(np.abs(b), -50)))
+    dB = [torch.from_numpy(np.ascontiguousarray(b.transpose(1, 0, 2))).to(dev) for b in Ibs]
+    dX = [torch.empty((1, n, 2), dtype=torch.float64, device=dev) for n in ns]
+    torch.cuda.synchronize()
+    one = [1] * len(ns)
+    largs = (ns, one, [t.data_ptr() for t in dA], ns, [t.data_ptr() for t in dB], ns, [t.data_ptr() for t in dR], ns,
+             [t.data_ptr() for t in dX], ns)
+    gs = ctx.lss_batched_dev(*largs)
+    lat = []
+    for _ in range(max(10, reps // 4)):
+        t0 = time.perf_counter()
+        ctx.lss_batched_dev(*largs)   # returns after its stream synchronise
+        lat.append(time.perf_counter() - t0)
+    lat = np.array(lat) * 1e3
+    p50 = float(np.percentile(lat, 50))
+    lss = {"problems": len(ns), "nrhs": 1, "calls": len(lat), "p50_ms": round(p50, 4),
+           "p99_ms": round(float(np.percentile(lat, 99)), 4), "tflops_p50": round(flops / (p50 * 1e-3) / 1e12, 3),
+           "all_verified": bool(max(gs) < 1.0)}
+    try:
+        import oracle_ffi as O
+
+        ok = True
+        for i in sorted(range(len(ns)), key=lambda i: ns[i])[:3]:
+            Xo, go, vok = O.lss(probs[i][1], Ibs[i], probs[i][0])
+            ok = ok and vok and go == gs[i] and np.array_equal(dX[i].cpu().numpy().transpose(1, 0, 2), Xo)
+        lss["oracle_bitexact_smallest3"] = bool(ok)
+    except Exception as e:
+        log("lss oracle check skipped: %r" % (e,))
+    log("batched verified solve %s" % lss)
+    out["lss_batched"] = lss
     return out
 
 

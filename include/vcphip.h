@@ -172,6 +172,22 @@ int vcphip_lss(vcphip_ctx* ctx, int n, int nrhs, const double* IA, int lda, cons
                const double* R, int ldr, double* X, int ldx, double* normub);
 int vcphip_lss_dev(vcphip_ctx* ctx, int n, int nrhs, const double* IA, int lda, const double* Ib, int ldb,
                    const double* R, int ldr, double* X, int ldx, double* normub);
+/* Many systems of uneven size (the Galerkin blocks of a verification run):
+ * per problem p the vcphip_lss result, bit-identical to it.  R_p*IA_p - I
+ * and its norm run in the batched residual's persistent launch (its graph
+ * is reused while the problem set is unchanged); every other step is one
+ * launch over all problems.  Pointer arrays are host arrays (of device
+ * pointers for the _dev form); normub is a host array of count doubles.
+ * Returns VCPHIP_ERR_VERIFICATION when any normub[p] >= 1 (the other
+ * problems' X are valid; a failed problem's X is NaN). */
+int vcphip_lss_batched(vcphip_ctx* ctx, int count, const int* n, const int* nrhs,
+                       const double* const* IA, const int* lda, const double* const* Ib, const int* ldb,
+                       const double* const* R, const int* ldr, double* const* X, const int* ldx,
+                       double* normub);
+int vcphip_lss_batched_dev(vcphip_ctx* ctx, int count, const int* n, const int* nrhs,
+                           const double* const* IA, const int* lda, const double* const* Ib, const int* ldb,
+                           const double* const* R, const int* ldr, double* const* X, const int* ldx,
+                           double* normub);
 
 /* ---- autotuner ------------------------------------------------------------
  * No counterpart in the reference (its BLAS path has nothing to tune).  The

@@ -117,3 +117,64 @@ def test_lss_on_residual_workload_problems(ctx, idx):
     assert ok
     X, g = ctx.lss(IA, Ib, R)
     assert g == go and _eq(X, Xo)
+
+
+# ---- batched form -----------------------------------------------------------
+def _mixed_set(seed, sizes, nrhss):
+    rng = np.random.default_rng(seed)
+    sys = [_system(rng, n, r) for n, r in zip(sizes, nrhss)]
+    return [s[0] for s in sys], [s[1] for s in sys], [s[2] for s in sys]
+
+
+def test_lss_batched_bitexact_against_oracle_and_single(ctx):
+    sizes, nrhss = [2, 3, 17, 64, 100, 257, 31], [1, 2, 1, 3, 1, 2, 1]
+    print("lss_batched", sizes, flush=True)
+    IAs, Ibs, Rs = _mixed_set(11, sizes, nrhss)
+    for rep in range(2):   # the second call replays the residual's graph
+        Xs, gs = ctx.lss_batched(IAs, Ibs, Rs)
+        for p in range(len(sizes)):
+            Xo, go, ok = O.lss(IAs[p], Ibs[p], Rs[p])
+            assert ok and gs[p] == go, (rep, p)
+            assert _eq(Xs[p], Xo), (rep, p)
+            Xsingle, gsingle = ctx.lss(IAs[p], Ibs[p], Rs[p])
+            assert _eq(Xs[p], Xsingle) and gsingle == gs[p]
+
+
+def test_lss_batched_residual_workload(ctx):
+    """48 problems of the batched residual workload's size distribution,
+    each against the single-problem form (bit-identical), a few against
+    the oracle"""
+    sizes = W.config5_sizes(48)
+    rng = np.random.default_rng(W.SEED0 + 70)
+    probs = [W.residual_problem(rng, n) for n in sizes]
+    Rs, IAs = [p[0] for p in probs], [p[1] for p in probs]
+    Ibs = []
+    for n in sizes:
+        b = rng.uniform(-1, 1, (n, 1))
+        Ibs.append(W.interval_from_midrad(b, np.ldexp(np.abs(b), -50)))
+    Xs, gs = ctx.lss_batched(IAs, Ibs, Rs)
+    for p in range(len(sizes)):
+        Xsingle, gsingle = ctx.lss(IAs[p], Ibs[p], Rs[p])
+        assert gs[p] == gsingle and _eq(Xs[p], Xsingle), p
+    for p in np.argsort(sizes)[:3]:
+        Xo, go, ok = O.lss(IAs[p], Ibs[p], Rs[p])
+        assert ok and go == gs[p] and _eq(Xs[p], Xo)
+
+
+def test_lss_batched_one_failing_problem(ctx):
+    sizes, nrhss = [20, 33, 40], [1, 1, 2]
+    IAs, Ibs, Rs = _mixed_set(12, sizes, nrhss)
+    Rs[1] = np.asfortranarray(2.5 * Rs[1])
+    dA = [torch.from_numpy(np.ascontiguousarray(A.transpose(1, 0, 2))).cuda() for A in IAs]
+    dB = [torch.from_numpy(np.ascontiguousarray(b.transpose(1, 0, 2))).cuda() for b in Ibs]
+    dR = [torch.from_numpy(np.ascontiguousarray(R.T)).cuda() for R in Rs]
+    dX = [torch.zeros((r, n, 2), dtype=torch.float64, device="cuda") for n, r in zip(sizes, nrhss)]
+    torch.cuda.synchronize()
+    with pytest.raises(vcp_amd.VcphipError) as e:
+        ctx.lss_batched_dev(sizes, nrhss, [t.data_ptr() for t in dA], sizes, [t.data_ptr() for t in dB], sizes,
+                            [t.data_ptr() for t in dR], sizes, [t.data_ptr() for t in dX], sizes)
+    assert e.value.code == -6
+    for p in (0, 2):
+        Xo, go, ok = O.lss(IAs[p], Ibs[p], Rs[p])
+        assert _eq(dX[p].cpu().numpy().transpose(1, 0, 2), Xo)
+    assert np.isnan(dX[1].cpu().numpy()).all()

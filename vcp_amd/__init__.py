@@ -69,6 +69,9 @@ SIGNATURES = {
     "vcphip_ddgemm_tn": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_lss": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _p_dbl],
     "vcphip_lss_dev": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _p_dbl],
+    "vcphip_lss_batched": [_vp, _c_int, _p_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _p_dbl],
+    "vcphip_lss_batched_dev": [_vp, _c_int, _p_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int,
+                               _p_dbl],
     "vcphip_residual_batched": [_vp, _c_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _p_dbl],
     "vcphip_residual_batched_dev": [_vp, _c_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _vp],
     "vcphip_tune_igemm_tile": [_vp, _c_int, _c_int, _c_int, _p_int, _p_dbl],
@@ -332,6 +335,37 @@ class Context:
         g = ctypes.c_double()
         _check(lib().vcphip_lss(self._h, n, nrhs, pa, lda, pb, ldb, pr, ldr, px, ldx, ctypes.byref(g)), "vcphip_lss")
         return X, g.value
+
+    def lss_batched(self, IAs, Ibs, Rs=None):
+        """vcphip_lss for many systems of uneven size in one call: returns
+        ([X_p], [g_p]); raises VcphipError (VCPHIP_ERR_VERIFICATION) when
+        any g_p >= 1."""
+        cnt = len(IAs)
+        if Rs is None:
+            Rs = [np.linalg.inv(0.5 * A[..., 0] + 0.5 * A[..., 1]) for A in IAs]
+        Rs = [np.asfortranarray(R, dtype=np.float64) for R in Rs]
+        Xs = [pairs(b.shape[0], b.shape[1]) for b in Ibs]
+        ia = lambda xs: (ctypes.c_int * cnt)(*xs)
+        vp = lambda xs: (_vp * cnt)(*xs)
+        g = (ctypes.c_double * cnt)()
+        _check(lib().vcphip_lss_batched(self._h, cnt, ia([A.shape[0] for A in IAs]), ia([b.shape[1] for b in Ibs]),
+                                        vp([_pair_ptr(A)[0] for A in IAs]), ia([_pair_ptr(A)[1] for A in IAs]),
+                                        vp([_pair_ptr(b)[0] for b in Ibs]), ia([_pair_ptr(b)[1] for b in Ibs]),
+                                        vp([_mat_ptr(R)[0] for R in Rs]), ia([_mat_ptr(R)[1] for R in Rs]),
+                                        vp([_pair_ptr(X)[0] for X in Xs]), ia([_pair_ptr(X)[1] for X in Xs]), g),
+               "vcphip_lss_batched")
+        return Xs, list(g)
+
+    def lss_batched_dev(self, n, nrhs, dA, lda, dB, ldb, dR, ldr, dX, ldx):
+        """device-pointer form: lists of sizes, pointers and leading
+        dimensions; returns [g_p] (raises when any g_p >= 1)"""
+        cnt = len(n)
+        ia = lambda xs: (ctypes.c_int * cnt)(*xs)
+        vp = lambda xs: (_vp * cnt)(*[_vp(x) for x in xs])
+        g = (ctypes.c_double * cnt)()
+        _check(lib().vcphip_lss_batched_dev(self._h, cnt, ia(n), ia(nrhs), vp(dA), ia(lda), vp(dB), ia(ldb), vp(dR),
+                                            ia(ldr), vp(dX), ia(ldx), g), "vcphip_lss_batched_dev")
+        return list(g)
 
     # ---- device-pointer products (torch tensors or raw pointers) -----------
     def lss_dev(self, n, nrhs, dA, lda, dB, ldb, dR, ldr, dX, ldx):

@@ -232,6 +232,222 @@ __global__ __launch_bounds__(NT) void k_lss_final(int n, int ncol, const double*
 
 inline dim3 grid_of(long count) { return dim3((unsigned)((count + NT - 1) / NT)); }
 
+// ===========================================================================
+// Batched form: many systems of uneven size, each stage one launch over the
+// (problem, row, column) items of all problems.  The products are evaluated
+// one output per thread with exactly the chains and epilogue of the k_drgemm
+// tiles (k ascending, midrad on load, the same fused operations in the same
+// order), so every result is bit-identical to the single-problem form.
+// ===========================================================================
+
+// problem p's items are [pb[p], pb[p+1]) of the flat item space
+VCPHIP_DEVINL int find_problem(const long* __restrict__ pb, int count, long item) {
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pb[mid] <= item) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// one output of the k_drgemm forms: A row r (stride lda over k), B column c
+// (stride 1 over k); AI / BI: operand is an interval (AoS pairs).
+// Round-up mode.  Returns {lower, upper}.
+template <bool AI, bool BI>
+VCPHIP_DEVINL double2 dr_dot(int k, const double* __restrict__ A, long lda, const double* __restrict__ B) {
+  constexpr bool II = AI && BI;
+  double up = 0.0, dn = 0.0, r1 = 0.0, r2 = 0.0;
+  // unrolled so that each thread keeps several loads in flight (the chains
+  // themselves stay sequential in k)
+#pragma unroll 8
+  for (int kk = 0; kk < k; ++kk) {
+    double am, ar = 0.0, bm, br = 0.0;
+    if (AI) {
+      const double2 v = *reinterpret_cast<const double2*>(A + 2 * lda * kk);
+      vcphip_midrad_ru(v.x, v.y, am, ar);
+    } else {
+      am = A[lda * kk];
+    }
+    if (BI) {
+      const double2 v = *reinterpret_cast<const double2*>(B + 2 * kk);
+      vcphip_midrad_ru(v.x, v.y, bm, br);
+    } else {
+      bm = B[kk];
+    }
+    up = __builtin_fma(am, bm, up);
+    dn = __builtin_fma(-am, bm, dn);
+    if (II) {
+      const double as = __builtin_fabs(am) + ar;
+      r1 = __builtin_fma(as, br, r1);
+      r2 = __builtin_fma(ar, __builtin_fabs(bm), r2);
+    } else if (AI) {
+      r1 = __builtin_fma(ar, __builtin_fabs(bm), r1);
+    } else if (BI) {
+      r1 = __builtin_fma(__builtin_fabs(am), br, r1);
+    }
+  }
+  const double t = II ? r1 + r2 : r1;
+  const double hi = up + t;
+  const double nlo = dn + t;
+  return make_double2(-nlo, hi);
+}
+
+// mb = mid(b) and mx = R mb need all of mb: two launches.
+__global__ __launch_bounds__(NT) void k_lssb_mid(int count, const long* __restrict__ pb,
+                                                 const VcphipLssProblem* __restrict__ probs) {
+  const long item = (long)blockIdx.x * NT + threadIdx.x;
+  if (item >= pb[count]) return;
+  const int p = find_problem(pb, count, item);
+  const VcphipLssProblem P = probs[p];
+  const long q = item - pb[p];
+  const int i = (int)(q % P.n), j = (int)(q / P.n);
+  const double2 v = *reinterpret_cast<const double2*>(P.b + 2 * (i + P.ldb * (long)j));
+  double m;
+  if (v.x == v.y) {
+    m = v.x;
+  } else {
+    const double s = v.x + v.y;
+    m = __builtin_isinf(s) ? 0.5 * v.x + 0.5 * v.y : s * 0.5;
+  }
+  P.mb[q] = m;
+}
+
+__global__ __launch_bounds__(NT) void k_lssb_mx(int count, const long* __restrict__ pb,
+                                                const VcphipLssProblem* __restrict__ probs) {
+  const long item = (long)blockIdx.x * NT + threadIdx.x;
+  if (item >= pb[count]) return;
+  const int p = find_problem(pb, count, item);
+  const VcphipLssProblem P = probs[p];
+  const long q = item - pb[p];
+  const int i = (int)(q % P.n), j = (int)(q / P.n);
+  const double* a = P.R + i;
+  const double* bj = P.mb + (long)P.n * j;
+  double acc = 0.0;
+#pragma unroll 8
+  for (int kk = 0; kk < P.n; ++kk) acc = __builtin_fma(a[P.ldr * (long)kk], bj[kk], acc);
+  P.mx[q] = acc;
+}
+
+// x = IA mx - b (IP product, then the interval subtraction)
+__global__ __launch_bounds__(NT) void k_lssb_ax(int count, const long* __restrict__ pb,
+                                                const VcphipLssProblem* __restrict__ probs) {
+  vcphip_mode_round_up();
+  const long item = (long)blockIdx.x * NT + threadIdx.x;
+  if (item >= pb[count]) return;
+  const int p = find_problem(pb, count, item);
+  const VcphipLssProblem P = probs[p];
+  const long q = item - pb[p];
+  const int i = (int)(q % P.n), j = (int)(q / P.n);
+  const double2 x = dr_dot<true, false>(P.n, P.A + 2 * i, P.lda, P.mx + (long)P.n * j);
+  const double2 bv = *reinterpret_cast<const double2*>(P.b + 2 * (i + P.ldb * (long)j));
+  *reinterpret_cast<double2*>(P.xi + 2 * q) = make_double2(-(bv.y - x.x), x.y - bv.x);
+}
+
+// y = |R x| (PI product, then kv's abs)
+__global__ __launch_bounds__(NT) void k_lssb_ry(int count, const long* __restrict__ pb,
+                                                const VcphipLssProblem* __restrict__ probs) {
+  vcphip_mode_round_up();
+  const long item = (long)blockIdx.x * NT + threadIdx.x;
+  if (item >= pb[count]) return;
+  const int p = find_problem(pb, count, item);
+  const VcphipLssProblem P = probs[p];
+  const long q = item - pb[p];
+  const int i = (int)(q % P.n), j = (int)(q / P.n);
+  double2 v = dr_dot<false, true>(P.n, P.R + i, P.ldr, P.xi + 2 * (long)P.n * j);
+  if (v.x < 0.0) {
+    if (v.y <= 0.0) v = make_double2(-v.y, -v.x);
+    else v = make_double2(0.0, -v.x > v.y ? -v.x : v.y);
+  }
+  *reinterpret_cast<double2*>(P.y + 2 * q) = v;
+}
+
+// T = ||y||_inf per problem (one workgroup each): upward row sums over the
+// columns, then the maximum with the reference's rule (as k_lss_max)
+__global__ __launch_bounds__(NT) void k_lssb_ynorm(const VcphipLssProblem* __restrict__ probs) {
+  vcphip_mode_round_up();
+  __shared__ double red[NT];
+  __shared__ double first;
+  const VcphipLssProblem P = probs[blockIdx.x];
+  double best = -__builtin_inf();
+  for (int i = threadIdx.x; i < P.n; i += NT) {
+    double s = 0.0;
+    for (int j = 0; j < P.nrhs; ++j) {
+      const double2 v = *reinterpret_cast<const double2*>(P.y + 2 * (i + (long)P.n * j));
+      const double a = __builtin_fabs(v.x), b = __builtin_fabs(v.y);
+      const double mag = a > b ? a : b;
+      s = (j == 0) ? mag : s + mag;
+    }
+    if (i == 0) first = s;
+    best = s > best ? s : best;
+  }
+  red[threadIdx.x] = best;
+  __syncthreads();
+  for (int st = NT / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      const double o = red[threadIdx.x + st];
+      red[threadIdx.x] = o > red[threadIdx.x] ? o : red[threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) P.norms[1] = __builtin_isnan(first) ? first : red[0];
+}
+
+// G = T / (1 - g) per problem (one thread each, round-to-nearest mode: the
+// exact-error method of k_lss_fill_g), into norms[2], norms[3]
+__global__ __launch_bounds__(NT) void k_lssb_g(int count, const VcphipLssProblem* __restrict__ probs) {
+  const int p = blockIdx.x * NT + threadIdx.x;
+  if (p >= count) return;
+  const VcphipLssProblem P = probs[p];
+  const double g = P.normub[0], t = P.norms[1];
+  double glo, ghi;
+  if (g < 1.0) {
+    double ddn, dup, a, b;
+    one_minus_dir(g, ddn, dup);
+    div_dir(t, dup, glo, a);
+    div_dir(t, ddn, b, ghi);
+  } else {
+    glo = ghi = __builtin_nan("");
+  }
+  P.norms[2] = glo;
+  P.norms[3] = ghi;
+}
+
+// W = |R IA - I| G (the II product with the constant column G; kv's abs()
+// taken on load) and X = mx + [-u, u], u = RU(|y|u + Wu).  Round-up mode.
+__global__ __launch_bounds__(NT) void k_lssb_final(int count, const long* __restrict__ pb,
+                                                   const VcphipLssProblem* __restrict__ probs) {
+  vcphip_mode_round_up();
+  const long item = (long)blockIdx.x * NT + threadIdx.x;
+  if (item >= pb[count]) return;
+  const int p = find_problem(pb, count, item);
+  const VcphipLssProblem P = probs[p];
+  const long q = item - pb[p];
+  const int i = (int)(q % P.n), j = (int)(q / P.n);
+  const double2 G = make_double2(P.norms[2], P.norms[3]);
+  double bm, br;
+  vcphip_midrad_ru(G.x, G.y, bm, br);
+  double up = 0.0, r1 = 0.0, r2 = 0.0;   // the lower chain of W is not used
+  const double* crow = P.C + 2 * i;
+#pragma unroll 8
+  for (int kk = 0; kk < P.n; ++kk) {
+    double2 v = *reinterpret_cast<const double2*>(crow + 2 * (long)P.n * kk);
+    if (v.x < 0.0) {
+      if (v.y <= 0.0) v = make_double2(-v.y, -v.x);
+      else v = make_double2(0.0, -v.x > v.y ? -v.x : v.y);
+    }
+    double am, ar;
+    vcphip_midrad_ru(v.x, v.y, am, ar);
+    up = __builtin_fma(am, bm, up);
+    const double as = __builtin_fabs(am) + ar;
+    r1 = __builtin_fma(as, br, r1);
+    r2 = __builtin_fma(ar, __builtin_fabs(bm), r2);
+  }
+  const double wu = up + (r1 + r2);
+  const double yu = P.y[2 * q + 1], m = P.mx[q];
+  const double u = yu + wu;
+  *reinterpret_cast<double2*>(P.X + 2 * (i + P.ldx * (long)j)) = make_double2(-(u - m), u + m);
+}
+
 }  // namespace
 
 int vcphip_lss_launch_mid(int n, int ncol, const double* b, long ldb, double* mb, hipStream_t s) {
@@ -276,5 +492,23 @@ int vcphip_lss_launch_fill_g(int n, int ncol, const double* norms, double* Tm, l
 int vcphip_lss_launch_final(int n, int ncol, const double* Y, const double* W, const double* mx, double* X, long ldx,
                             hipStream_t s) {
   hipLaunchKernelGGL(k_lss_final, grid_of((long)n * ncol), dim3(NT), 0, s, n, ncol, Y, W, mx, X, ldx);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_lss_launch_batched_rhs(int count, long items, const long* pb, const VcphipLssProblem* probs,
+                                  hipStream_t s) {
+  const dim3 g = grid_of(items);
+  hipLaunchKernelGGL(k_lssb_mid, g, dim3(NT), 0, s, count, pb, probs);
+  hipLaunchKernelGGL(k_lssb_mx, g, dim3(NT), 0, s, count, pb, probs);
+  hipLaunchKernelGGL(k_lssb_ax, g, dim3(NT), 0, s, count, pb, probs);
+  hipLaunchKernelGGL(k_lssb_ry, g, dim3(NT), 0, s, count, pb, probs);
+  hipLaunchKernelGGL(k_lssb_ynorm, dim3(count), dim3(NT), 0, s, probs);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_lss_launch_batched_bound(int count, long items, const long* pb, const VcphipLssProblem* probs,
+                                    hipStream_t s) {
+  hipLaunchKernelGGL(k_lssb_g, grid_of(count), dim3(NT), 0, s, count, probs);
+  hipLaunchKernelGGL(k_lssb_final, grid_of(items), dim3(NT), 0, s, count, pb, probs);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }

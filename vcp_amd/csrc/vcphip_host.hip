@@ -23,7 +23,7 @@ struct vcphip_ctx {
   hipStream_t stream = nullptr;
   long long oz_nmod = 0, oz_variant = 0, timing = 0, dd_nmod = 0;
   // workspace slots (device memory, grown on demand)
-  enum { WS_PA, WS_PB, WS_U, WS_Z, WS_SA, WS_SB, WS_HA, WS_HB, WS_HC, WS_HC2, WS_BATCH, WS_DD, WS_LSS, WS_HR, WS_N };
+  enum { WS_PA, WS_PB, WS_U, WS_Z, WS_SA, WS_SB, WS_HA, WS_HB, WS_HC, WS_HC2, WS_BATCH, WS_DD, WS_LSS, WS_HR, WS_LSSB, WS_N };
   void* ws[WS_N] = {};
   size_t ws_size[WS_N] = {};
   // stage timing: events recorded by every call since the last synchronize,
@@ -58,6 +58,10 @@ struct vcphip_ctx {
   };
   std::vector<Tune> tune;
   int resid_per_cu = 3;   // persistent workgroups per CU of the batched residual (its occupancy)
+  // second stream (default priority) of the batched solve: the right-hand
+  // side steps run on it beside the residual's persistent launch
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace {
@@ -328,6 +332,12 @@ extern "C" int vcphip_ctx_destroy(vcphip_ctx* c) {
   for (auto e : c->ev) hipEventDestroy(e);
   if (c->resid.exec) hipGraphExecDestroy(c->resid.exec);
   if (c->resid.graph) hipGraphDestroy(c->resid.graph);
+  if (c->aux) {
+    hipStreamSynchronize(c->aux);
+    hipStreamDestroy(c->aux);
+    hipEventDestroy(c->ev_fork);
+    hipEventDestroy(c->ev_join);
+  }
   hipStreamDestroy(c->own);
   delete c;
   return VCPHIP_OK;
@@ -763,6 +773,111 @@ extern "C" int vcphip_lss_dev(vcphip_ctx* c, int n, int nrhs, const double* IA, 
   return (g < 1.0) ? VCPHIP_OK : VCPHIP_ERR_VERIFICATION;
 }
 
+// Many systems of uneven size: R_p IA_p - I and its norm through the batched
+// residual (its persistent launch and graph), the other steps as one launch
+// each over the (row, column) items of all problems.
+extern "C" int vcphip_lss_batched_dev(vcphip_ctx* c, int count, const int* n, const int* nrhs,
+                                      const double* const* IA, const int* lda, const double* const* Ib,
+                                      const int* ldb, const double* const* R, const int* ldr, double* const* X,
+                                      const int* ldx, double* normub) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (count < 0 || (count > 0 && (!n || !nrhs || !IA || !lda || !Ib || !ldb || !R || !ldr || !X || !ldx || !normub)))
+    return VCPHIP_ERR_ARGUMENT;
+  if (count == 0) return VCPHIP_OK;
+  for (int p = 0; p < count; ++p) {
+    const int np = n[p];
+    if (!IA[p] || !Ib[p] || !R[p] || !X[p]) return VCPHIP_ERR_ARGUMENT;
+    if (np < 2 || nrhs[p] < 1 || lda[p] < np || ldb[p] < np || ldr[p] < np || ldx[p] < np)
+      return VCPHIP_ERR_DIMENSION;
+  }
+  hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  // workspace: descriptors, item offsets, g and norms, then per problem
+  // mb, mx, xi, y (n x nrhs) and C (n x n intervals); 256-byte aligned
+  auto al = [](size_t bytes) { return (bytes + 255) & ~(size_t)255; };
+  const size_t o_pb = al(sizeof(VcphipLssProblem) * count);
+  const size_t o_g = o_pb + al(sizeof(long) * (count + 1));
+  const size_t o_nm = o_g + al(sizeof(double) * count);
+  size_t off = o_nm + al(sizeof(double) * 4 * (size_t)count);
+  std::vector<size_t> o_prob((size_t)count);
+  std::vector<long> pb((size_t)count + 1, 0);
+  for (int p = 0; p < count; ++p) {
+    const size_t nr = (size_t)n[p] * nrhs[p];
+    o_prob[p] = off;
+    off += al(8 * nr) * 2 + al(16 * nr) * 2 + al(16 * (size_t)n[p] * n[p]);
+    pb[p + 1] = pb[p] + (long)nr;
+  }
+  void* base;
+  const int wrc = ws_get(c, vcphip_ctx::WS_LSSB, off, &base);
+  if (wrc) return wrc;
+  char* b8 = (char*)base;
+  double* gdev = (double*)(b8 + o_g);
+  std::vector<VcphipLssProblem> probs((size_t)count);
+  std::vector<double*> Cp((size_t)count);
+  std::vector<int> ldc((size_t)count);
+  for (int p = 0; p < count; ++p) {
+    const size_t nr = (size_t)n[p] * nrhs[p];
+    char* q = b8 + o_prob[p];
+    VcphipLssProblem& P = probs[p];
+    P.A = IA[p];
+    P.b = Ib[p];
+    P.R = R[p];
+    P.X = X[p];
+    P.lda = lda[p];
+    P.ldb = ldb[p];
+    P.ldr = ldr[p];
+    P.ldx = ldx[p];
+    P.n = n[p];
+    P.nrhs = nrhs[p];
+    P.mb = (double*)q;
+    P.mx = (double*)(q + al(8 * nr));
+    P.xi = (double*)(q + 2 * al(8 * nr));
+    P.y = (double*)(q + 2 * al(8 * nr) + al(16 * nr));
+    Cp[p] = (double*)(q + 2 * al(8 * nr) + 2 * al(16 * nr));
+    P.C = Cp[p];
+    P.normub = gdev + p;
+    P.norms = (double*)(b8 + o_nm) + 4 * (size_t)p;
+    ldc[p] = n[p];
+  }
+  // the host vectors above are the sources of asynchronous copies: every
+  // return from here on waits for the stream first
+  auto fail = [&](int code) {
+    hipStreamSynchronize(s);
+    if (c->aux) hipStreamSynchronize(c->aux);
+    return code;
+  };
+  if (hipMemcpyAsync(b8, probs.data(), sizeof(VcphipLssProblem) * count, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(b8 + o_pb, pb.data(), sizeof(long) * (count + 1), hipMemcpyHostToDevice, s) != hipSuccess)
+    return fail(VCPHIP_ERR_DEVICE);
+  if (!c->aux) {
+    if (hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
+      return fail(VCPHIP_ERR_DEVICE);
+  }
+  const long* dpb = (const long*)(b8 + o_pb);
+  const VcphipLssProblem* dprobs = (const VcphipLssProblem*)b8;
+  // right-hand side on the second stream, after the descriptor upload
+  if (hipEventRecord(c->ev_fork, s) != hipSuccess || hipStreamWaitEvent(c->aux, c->ev_fork, 0) != hipSuccess ||
+      vcphip_lss_launch_batched_rhs(count, pb[count], dpb, dprobs, c->aux) ||
+      hipEventRecord(c->ev_join, c->aux) != hipSuccess)
+    return fail(VCPHIP_ERR_DEVICE);
+  // R_p IA_p - I and g_p (imats.hpp:318-329) meanwhile on the context stream
+  int rc = vcphip_residual_batched_dev(c, count, n, R, ldr, IA, lda, Cp.data(), ldc.data(), gdev);
+  if (rc) return fail(rc);
+  StageTimer T(c);
+  if (hipStreamWaitEvent(s, c->ev_join, 0) != hipSuccess ||
+      vcphip_lss_launch_batched_bound(count, pb[count], dpb, dprobs, s))
+    return fail(VCPHIP_ERR_DEVICE);
+  T.mark("lss_batched");
+  if (hipMemcpyAsync(normub, gdev, sizeof(double) * count, hipMemcpyDeviceToHost, s) != hipSuccess)
+    return fail(VCPHIP_ERR_DEVICE);
+  if (hipStreamSynchronize(s) != hipSuccess) return VCPHIP_ERR_DEVICE;
+  for (int p = 0; p < count; ++p)
+    if (!(normub[p] < 1.0)) return VCPHIP_ERR_VERIFICATION;
+  return VCPHIP_OK;
+}
+
 // ===========================================================================
 // host-pointer entry points: stage through the context workspace
 // ===========================================================================
@@ -945,6 +1060,63 @@ extern "C" int vcphip_lss(vcphip_ctx* c, int n, int nrhs, const double* IA, int 
   const int rc = vcphip_lss_dev(c, n, nrhs, dA, n, dB, n, dR, n, dX, n, normub);
   if (rc != VCPHIP_OK && rc != VCPHIP_ERR_VERIFICATION) return rc;
   VCPHIP_TRY(to_host(c, dX, X, ldx, n, nrhs, 16));
+  return rc;
+}
+
+extern "C" int vcphip_lss_batched(vcphip_ctx* c, int count, const int* n, const int* nrhs, const double* const* IA,
+                                  const int* lda, const double* const* Ib, const int* ldb, const double* const* R,
+                                  const int* ldr, double* const* X, const int* ldx, double* normub) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (count < 0 || (count > 0 && (!n || !nrhs || !IA || !lda || !Ib || !ldb || !R || !ldr || !X || !ldx || !normub)))
+    return VCPHIP_ERR_ARGUMENT;
+  if (count == 0) return VCPHIP_OK;
+  for (int p = 0; p < count; ++p) {
+    if (!IA[p] || !Ib[p] || !R[p] || !X[p]) return VCPHIP_ERR_ARGUMENT;
+    if (n[p] < 2 || nrhs[p] < 1 || lda[p] < n[p] || ldb[p] < n[p] || ldr[p] < n[p] || ldx[p] < n[p])
+      return VCPHIP_ERR_DIMENSION;
+  }
+  hipSetDevice(c->device);
+  // inputs packed into one staging buffer (A, b, R per problem), outputs into another
+  std::vector<size_t> oi((size_t)count), ox((size_t)count);
+  size_t in_bytes = 0, out_bytes = 0;
+  for (int p = 0; p < count; ++p) {
+    const size_t np = (size_t)n[p], nr = np * nrhs[p];
+    oi[p] = in_bytes;
+    in_bytes += ((16 * np * np + 16 * nr + 8 * np * np) + 255) & ~(size_t)255;
+    ox[p] = out_bytes;
+    out_bytes += (16 * nr + 255) & ~(size_t)255;
+  }
+  void *pin, *pout;
+  VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HA, in_bytes, &pin));
+  VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HC2, out_bytes, &pout));
+  std::vector<const double*> dA((size_t)count), dB((size_t)count), dR((size_t)count);
+  std::vector<double*> dX((size_t)count);
+  std::vector<int> pn((size_t)count);
+  for (int p = 0; p < count; ++p) {
+    const size_t np = (size_t)n[p], nr = np * nrhs[p];
+    char* q = (char*)pin + oi[p];
+    dA[p] = (const double*)q;
+    dB[p] = (const double*)(q + 16 * np * np);
+    dR[p] = (const double*)(q + 16 * np * np + 16 * nr);
+    dX[p] = (double*)((char*)pout + ox[p]);
+    pn[p] = n[p];
+    hipStream_t s = c->stream;
+    if (hipMemcpy2DAsync((void*)dA[p], 16 * np, IA[p], 16 * (size_t)lda[p], 16 * np, np, hipMemcpyHostToDevice, s) !=
+            hipSuccess ||
+        hipMemcpy2DAsync((void*)dB[p], 16 * np, Ib[p], 16 * (size_t)ldb[p], 16 * np, nrhs[p], hipMemcpyHostToDevice,
+                         s) != hipSuccess ||
+        hipMemcpy2DAsync((void*)dR[p], 8 * np, R[p], 8 * (size_t)ldr[p], 8 * np, np, hipMemcpyHostToDevice, s) !=
+            hipSuccess)
+      return VCPHIP_ERR_DEVICE;
+  }
+  const int rc = vcphip_lss_batched_dev(c, count, n, nrhs, dA.data(), pn.data(), dB.data(), pn.data(), dR.data(),
+                                        pn.data(), dX.data(), pn.data(), normub);
+  if (rc != VCPHIP_OK && rc != VCPHIP_ERR_VERIFICATION) return rc;
+  for (int p = 0; p < count; ++p)
+    if (hipMemcpy2DAsync(X[p], 16 * (size_t)ldx[p], dX[p], 16 * (size_t)n[p], 16 * (size_t)n[p], nrhs[p],
+                         hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+      return VCPHIP_ERR_DEVICE;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return VCPHIP_ERR_DEVICE;
   return rc;
 }
 

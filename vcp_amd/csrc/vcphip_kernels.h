@@ -126,3 +126,27 @@ int vcphip_lss_launch_fill_g(int n, int ncol, const double* norms, double* Tm, l
                              hipStream_t s);
 int vcphip_lss_launch_final(int n, int ncol, const double* Y, const double* W, const double* mx, double* X, long ldx,
                             hipStream_t s);
+
+// batched verified solve: one problem's operands and its workspace
+struct VcphipLssProblem {
+  const double* A;    // n x n intervals
+  const double* b;    // n x nrhs intervals
+  const double* R;    // n x n
+  double* X;          // n x nrhs intervals (output)
+  long lda, ldb, ldr, ldx;
+  int n, nrhs;
+  double* mb;         // n x nrhs, packed: mid(b)
+  double* mx;         // n x nrhs: R mid(b)
+  double* xi;         // n x nrhs intervals: IA mx - b
+  double* y;          // n x nrhs intervals: |R (IA mx - b)|
+  const double* C;    // n x n intervals, ld n: R IA - I (from the batched residual)
+  const double* normub;   // g >= ||R IA - I||_inf (from the batched residual)
+  double* norms;      // [1] T = ||y||_inf, [2] [3] G
+};
+// problem p's (row, column) items are [pb[p], pb[p + 1]), pb[count] = total.
+// rhs: mid(b), R mid(b), IA mx - b, |R (...)| and ||y||_inf (independent of
+// R IA - I); bound: G and the enclosure (needs R IA - I and its norm).
+int vcphip_lss_launch_batched_rhs(int count, long items, const long* pb, const VcphipLssProblem* probs,
+                                  hipStream_t s);
+int vcphip_lss_launch_batched_bound(int count, long items, const long* pb, const VcphipLssProblem* probs,
+                                    hipStream_t s);
