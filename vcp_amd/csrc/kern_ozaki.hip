@@ -177,27 +177,42 @@ VCPHIP_DEVINL unsigned digits2_ru(double x, int g) {
   return (unsigned)hf | ((unsigned)__builtin_ceil(r) << 8);
 }
 
-// residue of a signed 63-bit integer in the symmetric int8 range of m
-VCPHIP_DEVINL int sym_residue(unsigned c0, unsigned c1, unsigned c2, bool neg, int t,
-                              const VcphipOzParams& P) {
-  const int m = P.mod[t];
-  // |x| = c0 + c1 2^21 + c2 2^42,  c_i < 2^21 and w < 2^8: 24-bit products
-  const unsigned y = c0 + __umul24(c1, (unsigned)P.w21[t]) + __umul24(c2, (unsigned)P.w42[t]);  // < 2^30.1
-  const double yd = (double)y;
-  const double q = __builtin_floor(yd * P.rcpd[t]);
-  int r = (int)(yd - q * (double)m);   // exact; off by at most one modulus
-  r += (r < 0) ? m : 0;
-  r -= (r >= m) ? m : 0;
-  if (neg && r != 0) r = m - r;
-  return (r >= (m + 1) / 2) ? r - m : r;
+typedef unsigned short vcphip_v2u16 __attribute__((ext_vector_type(2)));
+
+// x (|x| <= 2^62) as x = d0 + d1 2^12 + d2 2^24 + d3 2^36 + d4 2^48 + (e5 - 4) 2^60
+// with unsigned 12-bit chunks d0..d4 of its two's complement and e5 in [0, 7],
+// held as the 16-bit pairs (d0, d1), (d2, d3), (d4, e5)
+struct Chunks12 {
+  unsigned p01, p23, p45;
+};
+VCPHIP_DEVINL Chunks12 chunks12(long long x) {
+  const unsigned lo = (unsigned)(unsigned long long)x, hi = (unsigned)((unsigned long long)x >> 32);
+  Chunks12 c;
+  c.p01 = (lo & 0xFFFu) | (((lo >> 12) & 0xFFFu) << 16);
+  c.p23 = ((lo >> 24) | ((hi & 0xFu) << 8)) | (((hi >> 4) & 0xFFFu) << 16);
+  c.p45 = ((hi >> 16) & 0xFFFu) | ((unsigned)((int)(x >> 60) + 4) << 16);
+  return c;
 }
 
-VCPHIP_DEVINL void split_parts(long long a, unsigned& c0, unsigned& c1, unsigned& c2, bool& neg) {
-  neg = a < 0;
-  const unsigned long long u = neg ? (unsigned long long)(-a) : (unsigned long long)a;
-  c0 = (unsigned)(u & 0x1FFFFFull);
-  c1 = (unsigned)((u >> 21) & 0x1FFFFFull);
-  c2 = (unsigned)(u >> 42);
+// the residue of x in [-floor(m/2), m - 1 - floor(m/2)] (the symmetric int8
+// range; for m = 256 that is [-128, 127]).  y = x + floor(m/2) (mod m) from
+// three 16-bit dot products of the chunks with their weights (y < 2^22,
+// exact in FP32); q = trunc(RU(y * r_rcp)) is floor(y / m) or one less
+// (r_rcp is biased low), so r = y - q m lies in [0, 2m) and one unsigned
+// minimum brings it into [0, m).  Round-up mode.
+VCPHIP_DEVINL int sym_residue12(const Chunks12& c, int t, const VcphipOzParams& P) {
+  const unsigned m = (unsigned)P.mod[t];
+  unsigned y = __builtin_amdgcn_udot2(__builtin_bit_cast(vcphip_v2u16, c.p01),
+                                      __builtin_bit_cast(vcphip_v2u16, P.r_wp[0][t]), P.r_hh[t], false);
+  y = __builtin_amdgcn_udot2(__builtin_bit_cast(vcphip_v2u16, c.p23), __builtin_bit_cast(vcphip_v2u16, P.r_wp[1][t]),
+                             y, false);
+  y = __builtin_amdgcn_udot2(__builtin_bit_cast(vcphip_v2u16, c.p45), __builtin_bit_cast(vcphip_v2u16, P.r_wp[2][t]),
+                             y, false);
+  const unsigned q = (unsigned)((float)y * P.r_rcp[t]);
+  unsigned r = y - __umul24(q, m);
+  const unsigned r2 = r - m;
+  r = r < r2 ? r : r2;
+  return (int)r - (int)(m >> 1);
 }
 
 constexpr int SR_ROWS = 16, SR_K = 64, SR_LD = 68;   // LDS byte tile (one plane block), odd-word row stride
@@ -230,8 +245,7 @@ __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict_
       const int kk = k0 + lk0 + u;
       v[u] = (rowok && kk < k) ? *reinterpret_cast<const double2*>(A + 2 * (r + lda * kk)) : make_double2(0.0, 0.0);
     }
-    unsigned c0[4], c1[4], c2[4];
-    bool neg[4];
+    Chunks12 ch[4];
     unsigned dsh = 0, dsl = 0, drh = 0, drl = 0;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -239,7 +253,7 @@ __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict_
       vcphip_midrad_ru(v[u].x, v[u].y, am, ar);
       const double as = __builtin_fabs(am) + ar;
       const long long ap = zr ? 0 : rnint_floor(vcphip_mulpow2(am, P.beta - e));
-      split_parts(ap, c0[u], c1[u], c2[u], neg[u]);
+      ch[u] = chunks12(ap);
       const unsigned ds = digits2_ru(as, g), dr = digits2_ru(ar, h);
       dsh |= (ds & 0xff) << (8 * u);
       dsl |= (ds >> 8) << (8 * u);
@@ -252,7 +266,7 @@ __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict_
       unsigned word = 0;
 #pragma unroll
       for (int u = 0; u < 4; ++u)
-        word |= ((unsigned)(unsigned char)(signed char)sym_residue(c0[u], c1[u], c2[u], neg[u], t, P)) << (8 * u);
+        word |= ((unsigned)sym_residue12(ch[u], t, P) & 0xFFu) << (8 * u);
       *reinterpret_cast<unsigned*>(&s_tile[(t * SR_ROWS + lr) * SR_LD + lk0]) = word;
     }
     *reinterpret_cast<unsigned*>(&s_tile[(N * SR_ROWS + lr) * SR_LD + lk0]) = dsh;
@@ -296,15 +310,14 @@ __global__ __launch_bounds__(256) void k_oz_split_cols(const double* __restrict_
     const int kk = k0 + u;
     v[u] = (colok && kk < k) ? *reinterpret_cast<const double2*>(B + 2 * (kk + ldb * (long)j)) : make_double2(0.0, 0.0);
   }
-  unsigned c0[4], c1[4], c2[4];
-  bool ng[4];
+  Chunks12 ch[4];
   unsigned drh = 0, drl = 0, dmh = 0, dml = 0;
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     double bm, br;
     vcphip_midrad_ru(v[u].x, v[u].y, bm, br);
     const long long bp = zc ? 0 : rnint_floor(vcphip_mulpow2(bm, P.beta - f));
-    split_parts(bp, c0[u], c1[u], c2[u], ng[u]);
+    ch[u] = chunks12(bp);
     const unsigned d1 = digits2_ru(br, xi);
     const unsigned d2 = digits2_ru(__builtin_fabs(bm), zeta);
     drh |= (d1 & 0xff) << (8 * u);
@@ -319,7 +332,7 @@ __global__ __launch_bounds__(256) void k_oz_split_cols(const double* __restrict_
     unsigned word = 0;
 #pragma unroll
     for (int u = 0; u < 4; ++u)
-      word |= ((unsigned)(unsigned char)(signed char)sym_residue(c0[u], c1[u], c2[u], ng[u], z, P)) << (8 * u);
+      word |= ((unsigned)sym_residue12(ch[u], z, P) & 0xFFu) << (8 * u);
     *reinterpret_cast<unsigned*>(dst + z * plane_stride) = word;
   }
   *reinterpret_cast<unsigned*>(dst + N * plane_stride) = drh;

@@ -41,20 +41,26 @@ int vcphip_launch_resid(int ntiles, const VcphipResidTile* tiles, const VcphipRe
 #define VCPHIP_OZ_PADMN 256   // m and n are padded to multiples of this
 #define VCPHIP_OZ_PADK 128    // k is padded to a multiple of this
 #define VCPHIP_OZ_NLIMB 8     // 32-bit limbs of M (M < 2^256)
-#define VCPHIP_DD_NCHUNK 6    // 20-bit chunks of a scaled dd integer (< 2^120)
 
 struct VcphipOzParams {
   int nmod;                    // number of moduli N
   int beta;                    // scaled integers satisfy |A'| <= 2^beta (+1 for dd)
   int mod[VCPHIP_OZ_MAXMOD];   // pairwise coprime moduli (mod[0] = 256)
   double rcpd[VCPHIP_OZ_MAXMOD];
-  int w21[VCPHIP_OZ_MAXMOD];   // 2^21 mod m_t
-  int w42[VCPHIP_OZ_MAXMOD];   // 2^42 mod m_t
   // CRT weights w_t = (M/m_t) ((M/m_t)^-1 mod m_t) < M: [c][g] packs byte c of
   // w_{4g}, w_{4g+1}, w_{4g+2}, w_{4g+3} (low byte first); and w_t / M
   uint32_t crt_w8[VCPHIP_OZ_MAXMOD][VCPHIP_OZ_MAXMOD / 4];
   double crt_f[VCPHIP_OZ_MAXMOD];
-  uint8_t w20[VCPHIP_DD_NCHUNK][VCPHIP_OZ_MAXMOD];   // 2^(20 c) mod m_t
+  // residues of dd magnitudes < 2^120 from ten 12-bit chunks: weights
+  // 2^(12 c) mod m_t as 16-bit pairs (c = 2i, 2i + 1) for v_dot2
+  uint32_t dd_wp[5][VCPHIP_OZ_MAXMOD];
+  // residues of |x| <= 2^62 from 12-bit chunks (interval split): the chunk
+  // weights 1, 2^12, 2^24, 2^36, 2^48, 2^60 mod m_t as 16-bit pairs (low
+  // half first) for v_dot2; r_hh = (floor(m/2) - 4 * 2^60) mod m_t; r_rcp an
+  // FP32 value in [(1 - 2^-21) / m, (1 - 2^-22) / m]
+  uint32_t r_wp[3][VCPHIP_OZ_MAXMOD];
+  uint32_t r_hh[VCPHIP_OZ_MAXMOD];
+  float r_rcp[VCPHIP_OZ_MAXMOD];
   uint32_t Mlimb[VCPHIP_OZ_NLIMB];  // M
   double c1;                   // RU(1/127)
   double c127;                 // RU(1/127^2)
