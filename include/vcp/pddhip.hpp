@@ -7,19 +7,25 @@
 //   C = A * B;            // GPU
 //   C = ltransmul(A);     // A^T A, exactly symmetric
 //
-// pddblas::mulmm / mulltmm are NOT virtual (vcp/pddblas.hpp:318, :379), so
-// pddhip declares its own, which hide them for vcp::matrix<kv::dd, pddhip>
-// (name lookup in matrix's operator*, vcp/matrix.hpp:245-267).  Everything
-// else (linearsolve, Cholesky, eigsym) is inherited from pddblas and stays on
-// the CPU.  Scalar operands and products below the GPU / host crossover
-// (vcphip_policy::crossover, measured by tune_crossover()) keep the
-// reference's host semantics.
+// pddblas::mulmm / mulltmm / linearsolve are NOT virtual (vcp/pddblas.hpp:
+// 318, :379, :412), so pddhip declares its own, which hide them for
+// vcp::matrix<kv::dd, pddhip> (name lookup in matrix's operator* and lss,
+// vcp/matrix.hpp:245-267, :607-634); inv() (virtual, :477) is overridden so
+// that it reaches pddhip::linearsolve.  linearsolve keeps the reference's
+// refinement loop and forms its residuals r = b - A x on the GPU
+// (vcphip_dd_residual).  Cholesky and eigsym are inherited from pddblas and
+// stay on the CPU.  Scalar operands and products below the GPU / host
+// crossover (vcphip_policy::crossover, measured by tune_crossover()) keep
+// the reference's host semantics.
 #pragma once
 
 #ifndef VCP_PDDHIP_HPP
 #define VCP_PDDHIP_HPP
 
+#include <cmath>
+#include <limits>
 #include <random>
+#include <vector>
 
 #include <vcp/error.hpp>
 #include <vcp/pddblas.hpp>
@@ -90,6 +96,65 @@ class pddhip : public vcp::pddblas {
                                           pddhip_detail::raw(B.v), k, pddhip_detail::raw(c.v), m),
                             "vcphip_ddgemm");
   }
+  // x = A \ b (pddblas.hpp:412-474): LU of the high parts by LAPACK on the
+  // host and x += the correction in dd, as the reference; the residual
+  // r = b - A x comes from the GPU (the Ozaki-II dd product and a
+  // double-double subtraction) in place of split_row/split_col/ozaki_acc.
+  void linearsolve(vcp::pddblas& b, vcp::pddblas& x) {
+    if (this->row != this->column || b.row != this->row) {
+      vcp::throw_error<vcp::dimension_error>("linearsolve: invalid dimensions: A=(", this->row, ", ", this->column,
+                                             "), b=(", b.row, ", ", b.column, ")");
+    }
+    const int an = this->row, nrhs = b.column, nb = an * nrhs;
+    if (vcphip_policy::on_host(vcphip_policy::CROSS_DD, an, nrhs, an)) {
+      vcp::pddblas::linearsolve(b, x);
+      return;
+    }
+    vcp::pdblas Ad;
+    dd_to_d(*this, Ad);
+    std::vector<int> ipiv(an);
+    int N = an, NRHS = nrhs, lda = an, ldb = an, info;
+    dgetrf_(&N, &N, &Ad.v.front(), &lda, ipiv.data(), &info);
+    if (info != 0) throw vcp::lapack_error("dgetrf", info, "linearsolve: dgetrf failed");
+    x.zeros(an, nrhs);
+    std::vector<kv::dd> r = b.v;   // r = b - A x with x = 0
+    vcp::pdblas rd;
+    rd.zeros(an, nrhs);
+    char TN[] = "N";
+    const int itmax = 10;
+    double prevnormd = std::numeric_limits<double>::infinity();
+    vcphip_ctx* ctx = vcphip_policy::checked_context<E>();
+    for (int it = 0; it < itmax; it++) {
+      for (int i = 0; i < nb; i++) rd.v[i] = r[i].a1;
+      dgetrs_(TN, &N, &NRHS, &Ad.v.front(), &lda, ipiv.data(), &rd.v.front(), &ldb, &info);
+      if (info != 0) throw vcp::lapack_error("dgetrs", info, "linearsolve: dgetrs failed");
+      double normd = 0.0, normx = 0.0;
+      for (int i = 0; i < nb; i++) {
+        x.v[i] += kv::dd(rd.v[i]);
+        const double ad = std::fabs(rd.v[i]);
+        if (ad > normd) normd = ad;
+        const double ax = std::fabs(x.v[i].a1);
+        if (ax > normx) normx = ax;
+      }
+      if (normd == 0.0 || normd <= 0.25e-32 * normx || normd >= 0.5 * prevnormd) break;
+      prevnormd = normd;
+      vcphip_policy::check<E>(vcphip_dd_residual(ctx, an, nrhs, an, pddhip_detail::raw(this->v), an,
+                                                 pddhip_detail::raw(x.v), an, pddhip_detail::raw(b.v), an,
+                                                 pddhip_detail::raw(r), an),
+                              "vcphip_dd_residual");
+    }
+  }
+
+  // A = inv(A): A X = I through pddhip::linearsolve (pddblas.hpp:477-486)
+  void inv() override {
+    if (this->row != this->column)
+      vcp::throw_error<vcp::dimension_error>("inv: matrix must be square: ", this->row, " != ", this->column);
+    pddhip b, x;
+    b.eye(this->row);
+    this->linearsolve(b, x);
+    this->v = std::move(x.v);
+  }
+
   // C = A^T A (pddblas.hpp:379-409), exactly symmetric
   void mulltmm(pddhip& c) const {
     if (this->type == 'S' || vcphip_policy::on_host(vcphip_policy::CROSS_DD, this->column, this->column, this->row)) {

@@ -140,6 +140,18 @@ int vcphip_ddgemm_dev(vcphip_ctx* ctx, int m, int n, int k, const double* A, int
  * vcp::pddblas::mulltmm (vcp/pddblas.hpp:379-409). */
 int vcphip_ddgemm_tn(vcphip_ctx* ctx, int k, int n, const double* A, int lda, double* C, int ldc);
 
+/* R = B - A*X for dd matrices (A m x k, X k x n, B and R m x n): the
+ * residual r = b - A*x of pddblas::linearsolve's refinement
+ * (vcp/pddblas.hpp:459-466), whose products ozaki_acc forms; here A*X is the
+ * Ozaki-II dd product of vcphip_ddgemm, then a double-double subtraction
+ * (TwoSum-based, round-to-nearest) per element.  pddhip::linearsolve and
+ * pddhip::inv (dd inverse: A*X = I) run the reference's refinement loop
+ * with it. */
+int vcphip_dd_residual(vcphip_ctx* ctx, int m, int n, int k, const double* A, int lda, const double* X, int ldx,
+                       const double* B, int ldb, double* R, int ldr);
+int vcphip_dd_residual_dev(vcphip_ctx* ctx, int m, int n, int k, const double* A, int lda, const double* X, int ldx,
+                           const double* B, int ldb, double* R, int ldr);
+
 /* ---- batched verified residual: C_i = R_i * IA_i - I ---------------------- */
 /* For each problem i (square, size n[i]): C_i = R_i*IA_i - I as intervals and
  * normub[i] >= ||C_i||_inf.  Replaces the R*A - I segment of

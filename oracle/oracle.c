@@ -1433,3 +1433,28 @@ int orc_ozaki_ddgemm(int m, int n, int k, const double* A, int lda, const double
     free(fb);
     return rc;
 }
+
+/* R = B - A X for dd matrices: the residual r = b - A x of pddblas::
+ * linearsolve's refinement (vcp/pddblas.hpp:459-466) with the Ozaki-II
+ * product above, then a round-to-nearest double-double subtraction per
+ * element: TwoSum of the high parts and of the low parts, the low error
+ * added, FastTwoSum, the other error added, FastTwoSum. */
+int orc_dd_residual(int m, int n, int k, const double* A, int lda, const double* X, int ldx, const double* B,
+                    int ldb, double* R, int ldr, int nmod) {
+    if (!X) return -1;
+    const int rc = orc_ozaki_ddgemm(m, n, k, A, lda, X, ldx, R, ldr, nmod);
+    if (rc < 0) return rc;
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < m; ++i) {
+            const double* b = B + 2 * ((size_t)i + (size_t)ldb * j);
+            double* r = R + 2 * ((size_t)i + (size_t)ldr * j);
+            double s, e, t, f;
+            two_sum(b[0], -r[0], &s, &e);
+            two_sum(b[1], -r[1], &t, &f);
+            e = e + t;
+            fast_two_sum(s, e, &s, &e);
+            e = e + f;
+            fast_two_sum(s, e, &r[0], &r[1]);
+        }
+    return rc;
+}

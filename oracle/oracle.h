@@ -109,6 +109,10 @@ void orc_ozaki_scaled_cols(int k, int n, const double* B, int ldb, int beta, int
 int orc_ozaki_dd_beta(int k, int nmod);
 int orc_ozaki_ddgemm(int m, int n, int k, const double* A, int lda, const double* B, int ldb,
                      double* C, int ldc, int nmod);
+/* R = B - A X (dd): pddblas::linearsolve's residual (vcp/pddblas.hpp:459-466)
+ * with the product above and a TwoSum-based double-double subtraction. */
+int orc_dd_residual(int m, int n, int k, const double* A, int lda, const double* X, int ldx, const double* B,
+                    int ldb, double* R, int ldr, int nmod);
 
 /* ---- pddblas restatements (vcp/pddblas.hpp) ------------------------------ */
 int orc_dd_split_shift(int k);   /* pddblas.hpp:66-68 */

@@ -379,6 +379,29 @@ def ozaki_dd_beta(k, nmod):
     return L.orc_ozaki_dd_beta(k, nmod)
 
 
+def dd_default_nmod(k):
+    """the dd product's default moduli count: the fewest in 24..32 with beta >= 110"""
+    return next(t for t in range(24, 33) if ozaki_dd_beta(k, t) >= 110)
+
+
+def dd_residual(A, X, B, nmod=None):
+    """R = B - A X for dd (m, k, 2), (k, n, 2), (m, n, 2) pair arrays"""
+    L = lib()
+    L.orc_dd_residual.argtypes = [_c_int, _c_int, _c_int, _p_dbl, _c_int, _p_dbl, _c_int, _p_dbl, _c_int, _p_dbl,
+                                  _c_int, _c_int]
+    L.orc_dd_residual.restype = _c_int
+    m, k = A.shape[:2]
+    n = X.shape[1]
+    if nmod is None:
+        nmod = dd_default_nmod(k)
+    R = pairs(m, n)
+    rc = L.orc_dd_residual(m, n, k, _dp(A.transpose(1, 0, 2)), _ld_pair(A), _dp(X.transpose(1, 0, 2)), _ld_pair(X),
+                           _dp(B.transpose(1, 0, 2)), _ld_pair(B), _dp(R.transpose(1, 0, 2)), _ld_pair(R), nmod)
+    if rc < 0:
+        raise ValueError("orc_dd_residual failed: %d" % rc)
+    return R
+
+
 def ozaki_ddgemm(A, B, nmod):
     """dd C = A*B (or A^T A when B is None) through the exact scaled-integer
     product of the Ozaki-II route."""

@@ -55,6 +55,7 @@ def test_null_context_is_rejected_without_device_work():
     assert L.vcphip_ctx_synchronize(None) == -2
     assert L.vcphip_lss(None, 4, 1, None, 4, None, 4, None, 4, None, 4, None) == -2
     assert L.vcphip_lss_dev(None, 4, 1, None, 4, None, 4, None, 4, None, 4, None) == -2
+    assert L.vcphip_dd_residual(None, 4, 4, 4, None, 4, None, 4, None, 4, None, 4) == -2
     assert L.vcphip_lss_batched(None, 1, None, None, None, None, None, None, None, None, None, None, None) == -2
     assert L.vcphip_lss_batched_dev(None, 1, None, None, None, None, None, None, None, None, None, None, None) == -2
     assert L.vcphip_ctx_create(0, None) == -2

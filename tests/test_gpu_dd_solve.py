@@ -1,0 +1,104 @@
+"""The dd linear solve (pddblas::linearsolve / inv, vcp/pddblas.hpp:412-486)
+with GPU residuals: vcphip_dd_residual bit-identical to the oracle, and the
+refined solutions and inverses within dd accuracy of the exact rational
+ones."""
+from fractions import Fraction as F
+
+import numpy as np
+import pytest
+import torch
+
+import oracle_ffi as O
+import vcp_amd
+
+pytestmark = pytest.mark.gpu
+
+
+def dd_rand(rng, r, c, diag=0.0):
+    hi = rng.uniform(-1, 1, (r, c))
+    if diag:
+        hi = hi + diag * np.eye(r, c)
+    lo = hi * 2.0 ** -53 * rng.uniform(-1, 1, (r, c))
+    s = hi + lo
+    X = O.pairs(r, c)
+    X[..., 0] = s
+    X[..., 1] = lo - (s - hi)
+    return X
+
+
+def val(p):
+    return F(p[0]) + F(p[1])
+
+
+def exact_solve(A, b):
+    n = len(A)
+    M = [row[:] + [b[i]] for i, row in enumerate(A)]
+    for c in range(n):
+        p = max(range(c, n), key=lambda r: abs(M[r][c]))
+        M[c], M[p] = M[p], M[c]
+        for r in range(c + 1, n):
+            f = M[r][c] / M[c][c]
+            if f:
+                M[r] = [x - f * y for x, y in zip(M[r], M[c])]
+    x = [F(0)] * n
+    for r in range(n - 1, -1, -1):
+        x[r] = (M[r][n] - sum(M[r][j] * x[j] for j in range(r + 1, n))) / M[r][r]
+    return x
+
+
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (7, 3, 17), (64, 1, 64), (130, 70, 257), (256, 256, 256)])
+def test_dd_residual_bitexact(ctx, m, n, k):
+    print("dd_residual", m, n, k, flush=True)
+    rng = np.random.default_rng(m * 7 + n * 3 + k)
+    A, X, B = dd_rand(rng, m, k), dd_rand(rng, k, n), dd_rand(rng, m, n)
+    R = ctx.dd_residual(A, X, B)
+    assert np.array_equal(R, O.dd_residual(A, X, B))
+
+
+def test_dd_residual_dev_entry(ctx):
+    rng = np.random.default_rng(5)
+    m, n, k = 40, 6, 50
+    A, X, B = dd_rand(rng, m, k), dd_rand(rng, k, n), dd_rand(rng, m, n)
+    t = lambda P: torch.from_numpy(np.ascontiguousarray(P.transpose(1, 0, 2))).cuda()
+    dA, dX, dB = t(A), t(X), t(B)
+    dR = torch.empty((n, m, 2), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    L = vcp_amd.lib()
+    assert L.vcphip_dd_residual_dev(ctx._h, m, n, k, dA.data_ptr(), m, dX.data_ptr(), k, dB.data_ptr(), m,
+                                    dR.data_ptr(), m) == 0
+    ctx.synchronize()
+    assert np.array_equal(dR.cpu().numpy().transpose(1, 0, 2), O.dd_residual(A, X, B))
+
+
+@pytest.mark.parametrize("n,nrhs", [(5, 1), (9, 2)])
+def test_dd_linearsolve_matches_exact_solution(ctx, n, nrhs):
+    rng = np.random.default_rng(10 + n)
+    A, b = dd_rand(rng, n, n, diag=3.0), dd_rand(rng, n, nrhs)
+    x, it = ctx.dd_linearsolve(A, b)
+    assert 2 <= it <= 6
+    Ae = [[val(A[i, j]) for j in range(n)] for i in range(n)]
+    for j in range(nrhs):
+        xe = exact_solve(Ae, [val(b[i, j]) for i in range(n)])
+        for i in range(n):
+            assert abs(val(x[i, j]) - xe[i]) <= F(2) ** -100 * max(abs(xe[i]), F(1, 2 ** 40))
+
+
+def test_dd_linearsolve_residual_at_n300(ctx):
+    rng = np.random.default_rng(12)
+    n = 300
+    A, b = dd_rand(rng, n, n, diag=float(n) ** 0.5 * 2), dd_rand(rng, n, 1)
+    x, it = ctx.dd_linearsolve(A, b)
+    for i in (0, 17, 150, 299):   # exact residual of sampled rows
+        r = val(b[i, 0]) - sum(val(A[i, t]) * val(x[t, 0]) for t in range(n))
+        scale = sum(abs(val(A[i, t]) * val(x[t, 0])) for t in range(n))
+        assert abs(r) <= F(2) ** -98 * scale
+
+
+def test_dd_inv(ctx):
+    rng = np.random.default_rng(13)
+    n = 48
+    A = dd_rand(rng, n, n, diag=8.0)
+    X = ctx.dd_inv(A)
+    for i, j in ((0, 0), (3, 7), (47, 47), (20, 5)):
+        s = sum(val(A[i, t]) * val(X[t, j]) for t in range(n)) - (1 if i == j else 0)
+        assert abs(s) <= F(2) ** -98 * sum(abs(val(A[i, t]) * val(X[t, j])) for t in range(n))

@@ -72,6 +72,8 @@ SIGNATURES = {
     "vcphip_lss_batched": [_vp, _c_int, _p_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _p_dbl],
     "vcphip_lss_batched_dev": [_vp, _c_int, _p_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int,
                                _p_dbl],
+    "vcphip_dd_residual": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
+    "vcphip_dd_residual_dev": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_residual_batched": [_vp, _c_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _p_dbl],
     "vcphip_residual_batched_dev": [_vp, _c_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _vp],
     "vcphip_tune_igemm_tile": [_vp, _c_int, _c_int, _c_int, _p_int, _p_dbl],
@@ -287,6 +289,61 @@ class Context:
         pc, ldc = _pair_ptr(C)
         _check(lib().vcphip_ddgemm(self._h, m, n, k, pa, lda, pb, ldb, pc, ldc), "vcphip_ddgemm")
         return C
+
+    def dd_residual(self, A, X, B):
+        """R = B - A X for dd matrices (pddblas::linearsolve's residual)."""
+        m, k = A.shape[:2]
+        n = X.shape[1]
+        R = pairs(m, n)
+        pa, lda = _pair_ptr(A)
+        px, ldx = _pair_ptr(X)
+        pb, ldb = _pair_ptr(B)
+        pr, ldr = _pair_ptr(R)
+        _check(lib().vcphip_dd_residual(self._h, m, n, k, pa, lda, px, ldx, pb, ldb, pr, ldr), "vcphip_dd_residual")
+        return R
+
+    def dd_linearsolve(self, A, b, itmax=10):
+        """x = A \\ b for dd matrices by pddblas::linearsolve's refinement
+        (vcp/pddblas.hpp:412-474): LU of the high parts on the host (LAPACK,
+        as the reference), x += the LU correction in dd, stop when the
+        correction is below 0.25e-32 |x| or stagnates, and r = b - A x on the
+        GPU (vcphip_dd_residual).  Returns (x, iterations)."""
+        import scipy.linalg
+
+        n = A.shape[0]
+        if A.shape[1] != n or b.shape[0] != n:
+            raise ValueError("dimension mismatch")
+        lu = scipy.linalg.lu_factor(np.asfortranarray(A[..., 0]), check_finite=False)
+        x = pairs(n, b.shape[1])
+        r = b
+        prev = np.inf
+        it = 0
+        while it < itmax:
+            d = scipy.linalg.lu_solve(lu, np.asfortranarray(r[..., 0]), check_finite=False)
+            # x += dd(d): TwoSum of the high parts, the low part added, FastTwoSum
+            s = x[..., 0] + d
+            bb = s - x[..., 0]
+            e = (x[..., 0] - (s - bb)) + (d - bb)
+            e = e + x[..., 1]
+            h = s + e
+            x[..., 1] = e - (h - s)
+            x[..., 0] = h
+            it += 1
+            normd = float(np.abs(d).max()) if d.size else 0.0
+            normx = float(np.abs(x[..., 0]).max()) if d.size else 0.0
+            if normd == 0.0 or normd <= 0.25e-32 * normx or normd >= 0.5 * prev:
+                break
+            prev = normd
+            r = self.dd_residual(A, x, b)
+        return x, it
+
+    def dd_inv(self, A):
+        """inv(A) for a dd matrix: A X = I by dd_linearsolve (pddblas::inv,
+        vcp/pddblas.hpp:477-486)."""
+        n = A.shape[0]
+        eye = pairs(n, n)
+        eye[..., 0] = np.eye(n)
+        return self.dd_linearsolve(A, eye)[0]
 
     def ddgemm_tn(self, A):
         """C = A^T A for a double-double matrix (pddblas::mulltmm)."""

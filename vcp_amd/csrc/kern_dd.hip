@@ -338,6 +338,41 @@ __global__ __launch_bounds__(256) void k_dd_recon(int m, int n, int mpad, int np
   *reinterpret_cast<double2*>(C + 2 * (i + ldc * (long)j)) = make_double2(sgn * hi, sgn * lo);
 }
 
+// ---------------------------------------------------------------------------
+// residual R = B - R (dd, elementwise): the r = b - A x step of
+// pddblas::linearsolve (vcp/pddblas.hpp:461-466) after the product A x has
+// been written to R.  Round-to-nearest double-double subtraction with two
+// TwoSums (Knuth) and two renormalisations (Dekker's FastTwoSum); kv::dd's
+// own operator bits are not available to pin against.
+// ---------------------------------------------------------------------------
+VCPHIP_DEVINL void two_sum(double a, double b, double& s, double& e) {
+  s = a + b;
+  const double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+VCPHIP_DEVINL void fast_two_sum(double a, double b, double& s, double& e) {
+  s = a + b;
+  e = b - (s - a);
+}
+
+__global__ __launch_bounds__(256) void k_dd_sub_from(int m, int n, const double* __restrict__ B, long ldb,
+                                                     double* __restrict__ R, long ldr) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)m * n) return;
+  const int i = (int)(idx % m), j = (int)(idx / m);
+  const double2 b = *reinterpret_cast<const double2*>(B + 2 * (i + ldb * (long)j));
+  double2* r = reinterpret_cast<double2*>(R + 2 * (i + ldr * (long)j));
+  const double2 p = *r;
+  double s, e, t, f;
+  two_sum(b.x, -p.x, s, e);
+  two_sum(b.y, -p.y, t, f);
+  e = e + t;
+  fast_two_sum(s, e, s, e);
+  e = e + f;
+  fast_two_sum(s, e, s, e);
+  *r = make_double2(s, e);
+}
+
 }  // namespace
 
 int vcphip_dd_stats(const double* X, long rs, long cs, int rows, int len, int* e, int* flags, int* part,
@@ -393,5 +428,11 @@ int vcphip_dd_recon(int m, int n, int mpad, int npad, const VcphipOzParams& P, c
 #undef VCPHIP_DD_CASE
     default: return -7;
   }
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_dd_sub_from(int m, int n, const double* B, long ldb, double* R, long ldr, hipStream_t s) {
+  if ((long)m * n == 0) return 0;
+  hipLaunchKernelGGL(k_dd_sub_from, dim3((unsigned)(((long)m * n + 255) / 256)), dim3(256), 0, s, m, n, B, ldb, R, ldr);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }

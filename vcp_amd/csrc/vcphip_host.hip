@@ -634,6 +634,21 @@ extern "C" int vcphip_ddgemm_dev(vcphip_ctx* c, int m, int n, int k, const doubl
   return dd_gemm_dev(c, false, m, n, k, A, lda, B, ldb, C, ldc);
 }
 
+// R = B - A*X (dd): the residual of pddblas::linearsolve's refinement
+// (vcp/pddblas.hpp:459-466) with the Ozaki-II dd product
+extern "C" int vcphip_dd_residual_dev(vcphip_ctx* c, int m, int n, int k, const double* A, int lda, const double* X,
+                                      int ldx, const double* B, int ldb, double* R, int ldr) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (!dims_ok(m, n, k) || lda < (m > 1 ? m : 1) || ldx < (k > 1 ? k : 1) || ldb < (m > 1 ? m : 1) ||
+      ldr < (m > 1 ? m : 1))
+    return VCPHIP_ERR_DIMENSION;
+  if (m == 0 || n == 0) return VCPHIP_OK;
+  hipSetDevice(c->device);
+  const int rc = dd_gemm_dev(c, false, m, n, k, A, lda, X, ldx, R, ldr);
+  if (rc) return rc;
+  return check_launch(vcphip_dd_sub_from(m, n, B, ldb, R, ldr, c->stream));
+}
+
 static bool same_problem(const VcphipResidProblem& a, const VcphipResidProblem& b) {
   return a.R == b.R && a.A == b.A && a.C == b.C && a.ldr == b.ldr && a.lda == b.lda && a.ldc == b.ldc && a.n == b.n;
 }
@@ -1045,6 +1060,25 @@ extern "C" int vcphip_ddgemm(vcphip_ctx* c, int m, int n, int k, const double* A
   VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HC, 16 * (size_t)m * n, &p));
   VCPHIP_TRY(vcphip_ddgemm_dev(c, m, n, k, dA, m, dB, k > 0 ? k : 1, (double*)p, m));
   return to_host(c, (double*)p, C, ldc, m, n, 16);
+}
+
+extern "C" int vcphip_dd_residual(vcphip_ctx* c, int m, int n, int k, const double* A, int lda, const double* X,
+                                  int ldx, const double* B, int ldb, double* R, int ldr) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (!dims_ok(m, n, k) || lda < (m > 1 ? m : 1) || ldx < (k > 1 ? k : 1) || ldb < (m > 1 ? m : 1) ||
+      ldr < (m > 1 ? m : 1))
+    return VCPHIP_ERR_DIMENSION;
+  if (m == 0 || n == 0) return VCPHIP_OK;
+  hipSetDevice(c->device);
+  double *dA, *dX, *dB, *dR;
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HA, A, lda, m, k, 16, &dA));
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HB, X, ldx, k, n, 16, &dX));
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HR, B, ldb, m, n, 16, &dB));
+  void* p;
+  VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HC, 16 * (size_t)m * n, &p));
+  dR = (double*)p;
+  VCPHIP_TRY(vcphip_dd_residual_dev(c, m, n, k, dA, m > 0 ? m : 1, dX, k > 0 ? k : 1, dB, m, dR, m));
+  return to_host(c, dR, R, ldr, m, n, 16);
 }
 
 extern "C" int vcphip_ddgemm_tn(vcphip_ctx* c, int k, int n, const double* A, int lda, double* C, int ldc) {

@@ -115,6 +115,8 @@ int vcphip_dd_recon(int m, int n, int mpad, int npad, const VcphipOzParams& P, c
                     const int* eb, const int* fb, const uint8_t* U, double* C, long ldc, hipStream_t s);
 
 // autotuner helpers (kern_tune.hip)
+// R = B - R for dd matrices (the residual step of the dd linear solve)
+int vcphip_dd_sub_from(int m, int n, const double* B, long ldb, double* R, long ldr, hipStream_t s);
 int vcphip_gather_rows(const double* A, long lda, int m, int k, int s, double* out, hipStream_t st);
 int vcphip_ratio_max(const double* X, const double* Y, long total, double* part, int nblocks, hipStream_t st);
 int vcphip_fill_random(void* p, size_t bytes, uint32_t seed, hipStream_t st);
