@@ -13,10 +13,11 @@
 // vcp/matrix.hpp:245-267, :607-634); inv() (virtual, :477) is overridden so
 // that it reaches pddhip::linearsolve.  linearsolve keeps the reference's
 // refinement loop and forms its residuals r = b - A x on the GPU
-// (vcphip_dd_residual).  Cholesky and eigsym are inherited from pddblas and
-// stay on the CPU.  Scalar operands and products below the GPU / host
-// crossover (vcphip_policy::crossover, measured by tune_crossover()) keep
-// the reference's host semantics.
+// (vcphip_dd_residual); Cholesky and eigsym keep the reference's
+// refinements with their dd products on the GPU (eigsymge stays on the
+// host).  Scalar operands and products below the GPU / host crossover
+// (vcphip_policy::crossover, measured by tune_crossover()) keep the
+// reference's host semantics.
 #pragma once
 
 #ifndef VCP_PDDHIP_HPP
@@ -155,6 +156,124 @@ class pddhip : public vcp::pddblas {
     this->v = std::move(x.v);
   }
 
+  // Upper Cholesky factor (pddblas.hpp:488-563): dpotrf of the high parts on
+  // the host, then two refinement steps in which E = A - U^T U and F = W U
+  // (W = triu(G, 1) + diag(G) / 2, G = Ud^-T E Ud^-1 by dtrsm) are GPU dd
+  // products; U += F in kv::dd.
+  void Cholesky() override {
+    if (!this->is_symmetric()) vcp::throw_error<vcp::domain_error>("Cholesky: matrix must be symmetric");
+    const int an = this->row, nn = an * an;
+    if (vcphip_policy::on_host(vcphip_policy::CROSS_DD, an, an, an)) {
+      vcp::pddblas::Cholesky();
+      return;
+    }
+    vcp::pdblas Ud;
+    dd_to_d(*this, Ud);
+    Ud.Cholesky();
+    std::vector<kv::dd> U(nn), P(nn), W(nn);
+    for (int q = 0; q < nn; ++q) U[q] = kv::dd(Ud.v[q]);
+    char side_l[] = "L", side_r[] = "R", upper[] = "U", trans[] = "T", notrans[] = "N";
+    double one = 1.0;
+    int N = an;
+    vcphip_ctx* ctx = vcphip_policy::checked_context<E>();
+    using pddhip_detail::raw;
+    for (int step = 0; step < 2; ++step) {
+      vcphip_policy::check<E>(vcphip_ddgemm_tn(ctx, an, an, raw(U), an, raw(P), an), "vcphip_ddgemm_tn");
+      vcp::pdblas G;
+      G.zeros(an, an);
+      for (int q = 0; q < nn; ++q) {
+        Ud.v[q] = U[q].a1;
+        G.v[q] = (this->v[q] - P[q]).a1;   // E = A - U^T U
+      }
+      dtrsm_(side_l, upper, trans, notrans, &N, &N, &one, &Ud.v.front(), &N, &G.v.front(), &N);
+      dtrsm_(side_r, upper, notrans, notrans, &N, &N, &one, &Ud.v.front(), &N, &G.v.front(), &N);
+      for (int j = 0; j < an; ++j)
+        for (int i = 0; i < an; ++i) {
+          const double g = G.v[i + an * j];
+          W[i + an * j] = kv::dd(i > j ? 0.0 : (i == j ? 0.5 * g : g));
+        }
+      vcphip_policy::check<E>(vcphip_ddgemm(ctx, an, an, an, raw(W), an, raw(U), an, raw(P), an), "vcphip_ddgemm");
+      for (int q = 0; q < nn; ++q) U[q] += P[q];   // U += W U
+    }
+    for (int j = 0; j < an; ++j)
+      for (int i = j + 1; i < an; ++i) U[i + an * j] = kv::dd(0.0);
+    this->v = std::move(U);
+  }
+
+  // Symmetric eigenproblem (pddblas.hpp:566-606): dsyev of the high parts on
+  // the host, then max(itep, 2) Ogita-Aishima steps (:219-314) whose four
+  // products (A X, X^T (A X), X^T X, X E) are GPU dd products.
+  void eigsym(int itep = 4) override {
+    vcp::pddblas V;
+    this->eigsym(V, itep);
+  }
+  void eigsym(vcp::pddblas& V, int itep = 4) {
+    if (!this->is_symmetric()) vcp::throw_error<vcp::domain_error>("eigsym: matrix must be symmetric");
+    const int an = this->row, nn = an * an;
+    if (vcphip_policy::on_host(vcphip_policy::CROSS_DD, an, an, an)) {
+      vcp::pddblas::eigsym(V, itep);
+      return;
+    }
+    vcp::pdblas Ad, Vd;
+    dd_to_d(*this, Ad);
+    Ad.eigsym(Vd);
+    std::vector<kv::dd> X(nn), lam(an);
+    for (int q = 0; q < nn; ++q) X[q] = kv::dd(Vd.v[q]);
+    for (int i = 0; i < an; ++i) lam[i] = kv::dd(Ad.v[i + an * i]);
+    const int steps = itep < 2 ? 2 : itep;
+    for (int step = 0; step < steps; ++step) oa_step_gpu(this->v, X, lam, an);
+    V.zeros(an, an);
+    V.v = std::move(X);
+    this->zeros(an, an);
+    for (int i = 0; i < an; ++i) this->v[i + an * i] = lam[i];
+  }
+
+ private:
+  // one Ogita-Aishima refinement of the eigenpairs (X columns, lam) of the
+  // symmetric n x n matrix A (pddblas.hpp:219-314 with B = I)
+  static void oa_step_gpu(const std::vector<kv::dd>& A, std::vector<kv::dd>& X, std::vector<kv::dd>& lam, int n) {
+    using pddhip_detail::raw;
+    const int nn = n * n;
+    vcphip_ctx* ctx = vcphip_policy::checked_context<E>();
+    std::vector<kv::dd> AX(nn), S(nn), R(nn), Ecorr(nn), XE(nn);
+    vcphip_policy::check<E>(vcphip_ddgemm(ctx, n, n, n, raw(A), n, raw(X), n, raw(AX), n), "vcphip_ddgemm");
+    vcphip_policy::check<E>(vcphip_ddgemm_ta(ctx, n, n, n, raw(X), n, raw(AX), n, raw(S), n), "vcphip_ddgemm_ta");
+    vcphip_policy::check<E>(vcphip_ddgemm_tn(ctx, n, n, raw(X), n, raw(R), n), "vcphip_ddgemm_tn");   // R = X^T X
+    for (int i = 0; i < n; ++i) lam[i] = S[i + n * i] / R[i + n * i];   // Rayleigh quotients
+    for (int q = 0; q < nn; ++q) R[q] = -R[q];                           // R = I - X^T X
+    for (int i = 0; i < n; ++i) R[i + n * i] += kv::dd(1.0);
+    // separation threshold delta = 2 (max off-diagonal row sum of S + ||R|| max|lam|)
+    double normR = 0.0, offS = 0.0, lmax = 0.0;
+    for (int i = 0; i < n; ++i) {
+      double rs = 0.0, ss = 0.0;
+      for (int j = 0; j < n; ++j) {
+        rs += std::fabs(R[i + n * j].a1);
+        if (j != i) ss += std::fabs(S[i + n * j].a1);
+      }
+      normR = rs > normR ? rs : normR;
+      offS = ss > offS ? ss : offS;
+      const double a = std::fabs(lam[i].a1);
+      lmax = a > lmax ? a : lmax;
+    }
+    const double delta = 2.0 * (offS + normR * lmax);
+    const kv::dd half(0.5);
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        const int q = i + n * j;
+        if (i != j) {
+          const kv::dd gap = lam[j] - lam[i];
+          if (std::fabs(gap.a1) > delta) {
+            Ecorr[q] = (S[q] + lam[j] * R[q]) / gap;
+            continue;
+          }
+        }
+        Ecorr[q] = R[q] * half;
+      }
+    vcphip_policy::check<E>(vcphip_ddgemm(ctx, n, n, n, raw(X), n, raw(Ecorr), n, raw(XE), n), "vcphip_ddgemm");
+    for (int q = 0; q < nn; ++q) X[q] += XE[q];   // X += X E
+  }
+
+ public:
   // C = A^T A (pddblas.hpp:379-409), exactly symmetric
   void mulltmm(pddhip& c) const {
     if (this->type == 'S' || vcphip_policy::on_host(vcphip_policy::CROSS_DD, this->column, this->column, this->row)) {

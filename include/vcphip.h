@@ -140,6 +140,14 @@ int vcphip_ddgemm_dev(vcphip_ctx* ctx, int m, int n, int k, const double* A, int
  * vcp::pddblas::mulltmm (vcp/pddblas.hpp:379-409). */
 int vcphip_ddgemm_tn(vcphip_ctx* ctx, int k, int n, const double* A, int lda, double* C, int ldc);
 
+/* C = A^T B for dd matrices (A k x m, B k x n, C m x n): the X^T (A X) and
+ * X^T X products of pddblas's Ogita-Aishima refinement (oa_step,
+ * vcp/pddblas.hpp:221-314), whose ozaki_acc calls take a transposed left
+ * operand; the same Ozaki-II dd product as vcphip_ddgemm. */
+int vcphip_ddgemm_ta(vcphip_ctx* ctx, int m, int n, int k, const double* A, int lda, const double* B, int ldb,
+                     double* C, int ldc);
+int vcphip_ddgemm_ta_dev(vcphip_ctx* ctx, int m, int n, int k, const double* A, int lda, const double* B, int ldb,
+                         double* C, int ldc);
 /* R = B - A*X for dd matrices (A m x k, X k x n, B and R m x n): the
  * residual r = b - A*x of pddblas::linearsolve's refinement
  * (vcp/pddblas.hpp:459-466), whose products ozaki_acc forms; here A*X is the

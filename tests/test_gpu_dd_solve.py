@@ -1,7 +1,9 @@
-"""The dd linear solve (pddblas::linearsolve / inv, vcp/pddblas.hpp:412-486)
-with GPU residuals: vcphip_dd_residual bit-identical to the oracle, and the
-refined solutions and inverses within dd accuracy of the exact rational
-ones."""
+"""pddblas's refinement algorithms with their dd products on the GPU:
+linearsolve / inv (vcp/pddblas.hpp:412-486), Cholesky (:488-563) and the
+Ogita-Aishima eigensolver (:219-314, :566-606).  vcphip_dd_residual and
+vcphip_ddgemm_ta are bit-identical to the oracle; the refined solutions,
+inverses, factors and eigenpairs are within dd accuracy of exact rational
+arithmetic."""
 from fractions import Fraction as F
 
 import numpy as np
@@ -102,3 +104,57 @@ def test_dd_inv(ctx):
     for i, j in ((0, 0), (3, 7), (47, 47), (20, 5)):
         s = sum(val(A[i, t]) * val(X[t, j]) for t in range(n)) - (1 if i == j else 0)
         assert abs(s) <= F(2) ** -98 * sum(abs(val(A[i, t]) * val(X[t, j])) for t in range(n))
+
+
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (33, 17, 70), (256, 128, 300)])
+def test_ddgemm_ta_bitexact(ctx, m, n, k):
+    rng = np.random.default_rng(m + 2 * n + 3 * k)
+    A, B = dd_rand(rng, k, m), dd_rand(rng, k, n)
+    C = ctx.ddgemm_ta(A, B)
+    At = O.pairs(m, k)   # A^T materialised for the oracle
+    At[...] = A.transpose(1, 0, 2)
+    assert np.array_equal(C, O.ozaki_ddgemm(At, B, O.dd_default_nmod(k)))
+
+
+def _spd(rng, n):
+    G = dd_rand(rng, n, n)
+    S = G[..., 0] @ G[..., 0].T + n * np.eye(n)
+    L = 2.0 ** -60 * rng.uniform(-1, 1, (n, n))
+    L = (L + L.T) * 0.5
+    A = O.pairs(n, n)
+    A[..., 0] = S
+    A[..., 1] = np.where(np.abs(L) <= 2.0 ** -53 * np.abs(S), L, 0.0)
+    assert np.array_equal(A[..., 1], A[..., 1].T)
+    return A
+
+
+def test_dd_cholesky(ctx):
+    rng = np.random.default_rng(21)
+    n = 40
+    A = _spd(rng, n)
+    U = ctx.dd_cholesky(A)
+    assert (U[np.tril_indices(n, -1)] == 0.0).all()
+    for i, j in ((0, 0), (5, 9), (39, 39), (12, 30)):
+        s = sum(val(U[t, i]) * val(U[t, j]) for t in range(n)) - val(A[i, j])
+        scale = sum(abs(val(U[t, i]) * val(U[t, j])) for t in range(n))
+        assert abs(s) <= F(2) ** -98 * scale
+
+
+def test_dd_eigsym(ctx):
+    rng = np.random.default_rng(22)
+    n = 24
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    S = Q @ np.diag(np.arange(1.0, n + 1)) @ Q.T
+    S = (S + S.T) * 0.5
+    A = O.pairs(n, n)
+    A[..., 0] = S
+    lam, X = ctx.dd_eigsym(A)
+    anorm = float(np.abs(S).sum(axis=1).max())
+    for c in (0, 7, 23):
+        for i in (0, 11, 23):
+            r = sum(val(A[i, t]) * val(X[t, c]) for t in range(n)) - val(lam[c]) * val(X[i, c])
+            assert abs(r) <= F(2) ** -96 * F(anorm)
+    for i, j in ((0, 0), (3, 4), (23, 23)):
+        g = sum(val(X[t, i]) * val(X[t, j]) for t in range(n)) - (1 if i == j else 0)
+        assert abs(g) <= F(2) ** -98 * n
+    assert np.allclose(np.sort(lam[:, 0]), np.arange(1.0, n + 1), rtol=0, atol=1e-12)

@@ -72,6 +72,8 @@ SIGNATURES = {
     "vcphip_lss_batched": [_vp, _c_int, _p_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _p_dbl],
     "vcphip_lss_batched_dev": [_vp, _c_int, _p_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int,
                                _p_dbl],
+    "vcphip_ddgemm_ta": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
+    "vcphip_ddgemm_ta_dev": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_dd_residual": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_dd_residual_dev": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_residual_batched": [_vp, _c_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _p_dbl],
@@ -290,6 +292,17 @@ class Context:
         _check(lib().vcphip_ddgemm(self._h, m, n, k, pa, lda, pb, ldb, pc, ldc), "vcphip_ddgemm")
         return C
 
+    def ddgemm_ta(self, A, B):
+        """C = A^T B for double-double matrices (A k x m, B k x n)."""
+        k, m = A.shape[:2]
+        n = B.shape[1]
+        C = pairs(m, n)
+        pa, lda = _pair_ptr(A)
+        pb, ldb = _pair_ptr(B)
+        pc, ldc = _pair_ptr(C)
+        _check(lib().vcphip_ddgemm_ta(self._h, m, n, k, pa, lda, pb, ldb, pc, ldc), "vcphip_ddgemm_ta")
+        return C
+
     def dd_residual(self, A, X, B):
         """R = B - A X for dd matrices (pddblas::linearsolve's residual)."""
         m, k = A.shape[:2]
@@ -344,6 +357,21 @@ class Context:
         eye = pairs(n, n)
         eye[..., 0] = np.eye(n)
         return self.dd_linearsolve(A, eye)[0]
+
+    def dd_cholesky(self, A, itep=2):
+        """upper U with U^T U = A to dd accuracy (pddblas::Cholesky,
+        vcp/pddblas.hpp:488-563; products on the GPU)."""
+        from . import ddrefine
+
+        return ddrefine.cholesky(self, A, itep)
+
+    def dd_eigsym(self, A, itep=4):
+        """(eigenvalues (n, 2), eigenvectors (n, n, 2)) of a symmetric dd
+        matrix (pddblas::eigsym, Ogita-Aishima refinement, vcp/pddblas.hpp:
+        219-314, 566-606; products on the GPU)."""
+        from . import ddrefine
+
+        return ddrefine.eigsym(self, A, itep)
 
     def ddgemm_tn(self, A):
         """C = A^T A for a double-double matrix (pddblas::mulltmm)."""

@@ -578,8 +578,10 @@ extern "C" int vcphip_gemm_rurd_dev(vcphip_ctx* c, int m, int n, int k, const do
 
 // dd product C = op(A) * B through the Ozaki-II pipeline.  tn: C = A^T A,
 // where both operands are the columns of A (k x n).
+// ta: C = A^T B with A k x m (A's columns are the row operand, split like
+// B's columns)
 static int dd_gemm_dev(vcphip_ctx* c, bool tn, int m, int n, int k, const double* A, int lda, const double* B,
-                       int ldb, double* C, int ldc) {
+                       int ldb, double* C, int ldc, bool ta = false) {
   if (k > VCPHIP_OZ_MAXK) return VCPHIP_ERR_UNSUPPORTED;
   const int nmod = c->dd_nmod ? (int)c->dd_nmod : default_dd_nmod(k);
   VcphipOzParams P;
@@ -602,11 +604,15 @@ static int dd_gemm_dev(vcphip_ctx* c, bool tn, int m, int n, int k, const double
   int* part = fb + n;
   StageTimer T(c);
   if (!tn) {
-    if ((rc = check_launch(vcphip_dd_stats(A, 1, lda, m, k, ea, fa, part, c->stream)))) return rc;
+    if ((rc = check_launch(ta ? vcphip_dd_stats(A, lda, 1, m, k, ea, fa, part, c->stream)
+                              : vcphip_dd_stats(A, 1, lda, m, k, ea, fa, part, c->stream))))
+      return rc;
   }
   if ((rc = check_launch(vcphip_dd_stats(B, ldb, 1, n, k, eb, fb, part, c->stream)))) return rc;
   T.mark("stats");
-  if (!tn && (rc = check_launch(vcphip_dd_split_rows(A, lda, m, k, mpad, kpad, P, ea, fa, (int8_t*)pa, c->stream))))
+  if (!tn && (rc = check_launch(ta ? vcphip_dd_split_cols(A, lda, k, m, mpad, kpad, P, ea, fa, (int8_t*)pa, c->stream)
+                                   : vcphip_dd_split_rows(A, lda, m, k, mpad, kpad, P, ea, fa, (int8_t*)pa,
+                                                          c->stream))))
     return rc;
   if ((rc = check_launch(vcphip_dd_split_cols(B, ldb, k, n, npad, kpad, P, eb, fb, (int8_t*)pb, c->stream)))) return rc;
   T.mark("split");
@@ -632,6 +638,17 @@ extern "C" int vcphip_ddgemm_dev(vcphip_ctx* c, int m, int n, int k, const doubl
   if (m == 0 || n == 0) return VCPHIP_OK;
   hipSetDevice(c->device);
   return dd_gemm_dev(c, false, m, n, k, A, lda, B, ldb, C, ldc);
+}
+
+// C = A^T B for dd matrices (the X^T Y products of the Ogita-Aishima step)
+extern "C" int vcphip_ddgemm_ta_dev(vcphip_ctx* c, int m, int n, int k, const double* A, int lda, const double* B,
+                                    int ldb, double* C, int ldc) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (!dims_ok(m, n, k) || lda < (k > 1 ? k : 1) || ldb < (k > 1 ? k : 1) || ldc < (m > 1 ? m : 1))
+    return VCPHIP_ERR_DIMENSION;
+  if (m == 0 || n == 0) return VCPHIP_OK;
+  hipSetDevice(c->device);
+  return dd_gemm_dev(c, false, m, n, k, A, lda, B, ldb, C, ldc, true);
 }
 
 // R = B - A*X (dd): the residual of pddblas::linearsolve's refinement
@@ -1079,6 +1096,22 @@ extern "C" int vcphip_dd_residual(vcphip_ctx* c, int m, int n, int k, const doub
   dR = (double*)p;
   VCPHIP_TRY(vcphip_dd_residual_dev(c, m, n, k, dA, m > 0 ? m : 1, dX, k > 0 ? k : 1, dB, m, dR, m));
   return to_host(c, dR, R, ldr, m, n, 16);
+}
+
+extern "C" int vcphip_ddgemm_ta(vcphip_ctx* c, int m, int n, int k, const double* A, int lda, const double* B,
+                                int ldb, double* C, int ldc) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (!dims_ok(m, n, k) || lda < (k > 1 ? k : 1) || ldb < (k > 1 ? k : 1) || ldc < (m > 1 ? m : 1))
+    return VCPHIP_ERR_DIMENSION;
+  if (m == 0 || n == 0) return VCPHIP_OK;
+  hipSetDevice(c->device);
+  double *dA, *dB;
+  void* p;
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HA, A, lda, k, m, 16, &dA));
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HB, B, ldb, k, n, 16, &dB));
+  VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HC, 16 * (size_t)m * n, &p));
+  VCPHIP_TRY(vcphip_ddgemm_ta_dev(c, m, n, k, dA, k > 0 ? k : 1, dB, k > 0 ? k : 1, (double*)p, m));
+  return to_host(c, (double*)p, C, ldc, m, n, 16);
 }
 
 extern "C" int vcphip_ddgemm_tn(vcphip_ctx* c, int k, int n, const double* A, int lda, double* C, int ldc) {
