@@ -25,6 +25,9 @@
 #ifndef VCPHIP_H
 #define VCPHIP_H
 
+#include <stddef.h>
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -42,7 +45,8 @@ enum {
     VCPHIP_ERR_ROUNDING = -5,     /* directed rounding not in effect             */
     VCPHIP_ERR_VERIFICATION = -6, /* vcp::verification_error (imats.hpp:333-336) */
     VCPHIP_ERR_UNSUPPORTED = -7,  /* size or form outside what the kernels cover */
-    VCPHIP_ERR_RANGE = -8         /* exponent range outside the supported one    */
+    VCPHIP_ERR_RANGE = -8,        /* exponent range outside the supported one    */
+    VCPHIP_ERR_IO = -9            /* vcp::io_error (vcp/vcp_portable_fio.hpp)    */
 };
 
 /* forms of the interval x interval product */
@@ -242,6 +246,35 @@ int vcphip_tune_residual_dev(vcphip_ctx* ctx, int count, const int* n, const dou
  * then "igemm|ddgemm lm ln lk variant nmod" and "residual per_cu" lines). */
 int vcphip_tune_save(vcphip_ctx* ctx, const char* path);
 int vcphip_tune_load(vcphip_ctx* ctx, const char* path);
+
+/* ---- matrix files ----------------------------------------------------------
+ * The portable ".vcpmat" format (vcp/vcp_portable_fio.hpp: magic "VCPMAT2",
+ * version 2, column-major, little-endian, CRC-64 over header and payload)
+ * and the native ".matrix_*" format (vcp/vcp_fio.hpp: type tag, int32 sizes,
+ * row-major elements).  path is the full file name (the reference appends
+ * ".vcpmat" / ".matrix_d" etc. to the name it is given).  Element types and
+ * their bytes: */
+enum {
+    VCPHIP_MAT_INT32 = 1,       /* int32                                  4 B */
+    VCPHIP_MAT_DOUBLE = 2,      /* binary64                               8 B */
+    VCPHIP_MAT_INTERVAL = 3,    /* interval<double> {lower, upper}       16 B */
+    VCPHIP_MAT_DD = 4,          /* kv::dd {a1, a2}                       16 B */
+    VCPHIP_MAT_INTERVAL_DD = 5  /* interval<kv::dd> {l.a1, l.a2, u.a1, u.a2} 32 B */
+};
+/* Type, size and format (*portable = 1 for .vcpmat; may be NULL) of a file.
+ * Either format is recognised by its first bytes; a bad magic, version,
+ * layout, size, payload length, CRC or trailing data gives VCPHIP_ERR_IO. */
+int vcphip_mat_info(const char* path, int* type, int* rows, int* cols, int* portable);
+/* Reads the file into a column-major host array (leading dimension ld, in
+ * elements).  The file's type must be `type` (VCPHIP_ERR_IO otherwise, the
+ * reference's data type mismatch) and its size rows x cols. */
+int vcphip_mat_read(const char* path, int type, int rows, int cols, void* data, int ld);
+/* Writes a column-major host array in the portable (portable != 0) or the
+ * native format, through path + ".tmp" and a rename, as the reference does. */
+int vcphip_mat_write(const char* path, int type, int rows, int cols, const void* data, int ld, int portable);
+/* CRC-64 of the portable format (polynomial 0x42F0E1EBA9EA3693, MSB first,
+ * initial value 0, no final xor). */
+int vcphip_crc64(const void* data, size_t bytes, uint64_t* crc);
 
 #ifdef __cplusplus
 }

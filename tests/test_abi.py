@@ -43,7 +43,7 @@ def test_every_declared_symbol_is_exported():
 
 def test_strerror_names_every_code():
     L = vcp_amd.lib()
-    for code in range(0, -9, -1):
+    for code in range(0, -10, -1):
         assert L.vcphip_strerror(code) not in (None, b"", b"unknown error")
     assert L.vcphip_strerror(-99) == b"unknown error"
 

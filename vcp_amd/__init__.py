@@ -76,6 +76,10 @@ SIGNATURES = {
     "vcphip_ddgemm_ta_dev": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_dd_residual": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_dd_residual_dev": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
+    "vcphip_mat_info": [ctypes.c_char_p, _p_int, _p_int, _p_int, _p_int],
+    "vcphip_mat_read": [ctypes.c_char_p, _c_int, _c_int, _c_int, _vp, _c_int],
+    "vcphip_mat_write": [ctypes.c_char_p, _c_int, _c_int, _c_int, _vp, _c_int, _c_int],
+    "vcphip_crc64": [_vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint64)],
     "vcphip_residual_batched": [_vp, _c_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _p_dbl],
     "vcphip_residual_batched_dev": [_vp, _c_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _vp],
     "vcphip_tune_igemm_tile": [_vp, _c_int, _c_int, _c_int, _p_int, _p_dbl],
@@ -149,6 +153,49 @@ def _pair_ptr(a):
 def _mat_ptr(a):
     assert a.dtype == np.float64 and a.ndim == 2 and (a.flags.f_contiguous or a.size == 0)
     return a.ctypes.data_as(_vp), max(a.shape[0], 1)
+
+
+# ---- matrix files (vcp/vcp_portable_fio.hpp, vcp/vcp_fio.hpp) --------------
+MAT_INT32, MAT_DOUBLE, MAT_INTERVAL, MAT_DD, MAT_INTERVAL_DD = 1, 2, 3, 4, 5
+_MAT_WORDS = {MAT_INT32: 1, MAT_DOUBLE: 1, MAT_INTERVAL: 2, MAT_DD: 2, MAT_INTERVAL_DD: 4}
+
+
+def mat_info(path):
+    """(type, rows, cols, portable) of a .vcpmat or .matrix_* file"""
+    t, r, c, p = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _check(lib().vcphip_mat_info(os.fsencode(path), ctypes.byref(t), ctypes.byref(r), ctypes.byref(c), ctypes.byref(p)),
+           "vcphip_mat_info")
+    return t.value, r.value, c.value, bool(p.value)
+
+
+def _mat_array(mtype, rows, cols):
+    if mtype == MAT_INT32:
+        return np.zeros((rows, cols), dtype=np.int32, order="F")
+    w = _MAT_WORDS[mtype]
+    if w == 1:
+        return np.zeros((rows, cols), dtype=np.float64, order="F")
+    return np.zeros((cols, rows, w), dtype=np.float64).transpose(1, 0, 2)
+
+
+def mat_read(path):
+    """(array, type): int32 / float64 (rows, cols) Fortran-order arrays, or
+    AoS column-major (rows, cols, 2) pairs (interval, dd) and (rows, cols, 4)
+    for interval<dd>"""
+    mtype, rows, cols, _ = mat_info(path)
+    X = _mat_array(mtype, rows, cols)
+    _check(lib().vcphip_mat_read(os.fsencode(path), mtype, rows, cols, X.ctypes.data_as(_vp), max(rows, 1)),
+           "vcphip_mat_read")
+    return X, mtype
+
+
+def mat_write(path, X, mtype, portable=True):
+    """write X (shaped as mat_read returns it) as a .vcpmat (portable) or
+    native .matrix_* file"""
+    rows, cols = X.shape[:2]
+    Y = _mat_array(mtype, rows, cols)
+    Y[...] = X
+    _check(lib().vcphip_mat_write(os.fsencode(path), mtype, rows, cols, Y.ctypes.data_as(_vp), max(rows, 1),
+                                  int(bool(portable))), "vcphip_mat_write")
 
 
 def ozaki_config(k, nmod=0):

@@ -311,6 +311,7 @@ extern "C" const char* vcphip_strerror(int code) {
     case VCPHIP_ERR_VERIFICATION: return "verification failed";
     case VCPHIP_ERR_UNSUPPORTED: return "unsupported size or form";
     case VCPHIP_ERR_RANGE: return "exponent range not supported";
+    case VCPHIP_ERR_IO: return "matrix file I/O or format error";
     default: return "unknown error";
   }
 }
