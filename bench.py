@@ -46,6 +46,7 @@ def main():
     ap.add_argument("--no-infsup", action="store_true")
     ap.add_argument("--no-tuned", action="store_true")
     ap.add_argument("--no-config2", action="store_true")
+    ap.add_argument("--no-galerkin", action="store_true")
     ap.add_argument("--no-dd", action="store_true")
     ap.add_argument("--dd-n", type=int, default=4096)
     ap.add_argument("--dd-steps", type=int, default=5)
@@ -249,6 +250,7 @@ def main():
     resid = None if args.no_resid else bench_resid(ctx, args.resid_reps)
     dd = None if args.no_dd else bench_dd(ctx, args.dd_n, args.dd_steps, not args.no_cpu)
     cfg2 = None if args.no_config2 else bench_config2(ctx)
+    galerkin = None if args.no_galerkin else bench_galerkin(ctx)
 
     line = {
         "metric": "verified interval GEMM FP64-equiv TFLOP/s n=8192: Ozaki-int8 vs FP64-RU/RD",
@@ -278,6 +280,7 @@ def main():
         "batched_residual": resid,
         "dd_gemm": dd,
         "plain_n4096": cfg2,
+        "galerkin_uhphiphi": galerkin,
     }
     print(json.dumps(line), flush=True)
     ctx.close()
@@ -382,6 +385,58 @@ def bench_resid(ctx, reps):
         log("lss oracle check skipped: %r" % (e,))
     log("batched verified solve %s" % lss)
     out["lss_batched"] = lss
+    return out
+
+
+def bench_galerkin(ctx, p=40, q=80, steps=5):
+    """The Galerkin matrix producer (Legendre_Bases_Generator::uhphiphi) on a
+    2D Legendre basis: m = p^2 basis functions, q^2 Gauss points, interval
+    arithmetic.  Work: m (m + 1) / 2 q^2 interval multiply-adds (8 FP64
+    products, 6 comparisons, 2 additions each)."""
+    import numpy as np
+    import torch
+
+    import vcp_amd
+    from vcp_amd import workloads as W
+
+    phi, w, u = W.galerkin(p, q)
+    m, npts = phi.shape[:2]
+    dev = torch.device("cuda:0")
+    dP = torch.from_numpy(np.ascontiguousarray(phi.transpose(1, 0, 2))).to(dev)
+    dW = torch.from_numpy(w).to(dev)
+    dU = torch.from_numpy(u).to(dev)
+    dA = torch.empty((m, m, 2), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    L = vcp_amd.lib()
+    args = (ctx._h, m, npts, dP.data_ptr(), m, dW.data_ptr(), dU.data_ptr(), dA.data_ptr(), m)
+    assert L.vcphip_uhphiphi_dev(*args) == 0
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        L.vcphip_uhphiphi_dev(*args)
+    ctx.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    ima = m * (m + 1) / 2 * npts
+    # every product and sum is separately rounded (no FMA): the bound is the
+    # FP64 vector instruction rate, half the FMA flop peak
+    instr = 16 * ima / (ms * 1e-3) / 1e12
+    out = {"basis": "2D Legendre, p=%d (m=%d), %dx%d Gauss points" % (p, m, q, q), "ms": round(ms, 3),
+           "interval_madds_per_s": round(ima / (ms * 1e-3), 1), "fp64_instr_per_s_e12": round(instr, 3),
+           "frac_fp64_issue_peak": round(instr / (PEAK_FP64_TFLOPS / 2), 4)}
+    try:
+        import oracle_ffi as O
+
+        idx = np.linspace(0, m - 1, 8).astype(int)
+        A = dA.cpu().numpy().transpose(1, 0, 2)
+        sub = O.pairs(len(idx), npts)
+        sub[...] = phi[idx]
+        t1 = time.perf_counter()
+        ref = O.uhphiphi(sub, w, u)
+        out["oracle_bitexact_8rows"] = bool(np.array_equal(A[np.ix_(idx, idx)], ref))
+        out["cpu_oracle_8rows_s"] = round(time.perf_counter() - t1, 3)
+    except Exception as e:
+        log("galerkin oracle check skipped: %r" % (e,))
+    log("galerkin %s" % out)
     return out
 
 

@@ -247,6 +247,19 @@ int vcphip_tune_residual_dev(vcphip_ctx* ctx, int count, const int* n, const dou
 int vcphip_tune_save(vcphip_ctx* ctx, const char* path);
 int vcphip_tune_load(vcphip_ctx* ctx, const char* path);
 
+/* ---- Galerkin matrix producer ---------------------------------------------- */
+/* A(i, j) = sum over points k of ((w_k * u_k) * phi(i, k)) * phi(j, k) in
+ * interval arithmetic (each product and sum separately rounded outward, k
+ * ascending from [0, 0]), for i <= j, mirrored to A(j, i): the matrix
+ * Legendre_Bases_Generator::uhphiphi builds (vcp/ldbase.hpp:955-1013) from
+ * its basis values phi_point (m x npts), quadrature weights weight_point and
+ * the coefficient function values uh_p_phi_point (1 x npts each).  phi and A
+ * are column-major interval matrices, w and u interval vectors. */
+int vcphip_uhphiphi(vcphip_ctx* ctx, int m, int npts, const double* phi, int ldp, const double* w, const double* u,
+                    double* A, int lda);
+int vcphip_uhphiphi_dev(vcphip_ctx* ctx, int m, int npts, const double* phi, int ldp, const double* w,
+                        const double* u, double* A, int lda);
+
 /* ---- matrix files ----------------------------------------------------------
  * The portable ".vcpmat" format (vcp/vcp_portable_fio.hpp: magic "VCPMAT2",
  * version 2, column-major, little-endian, CRC-64 over header and payload)

@@ -490,6 +490,57 @@ done:
 }
 
 /* ------------------------------------------------------------------------ */
+/* Galerkin matrix: Legendre_Bases_Generator::uhphiphi (vcp/ldbase.hpp:955-1013) */
+/* ------------------------------------------------------------------------ */
+/* kv's interval product: the lower end is the smallest downward-rounded
+ * endpoint product, the upper end the largest upward-rounded one. */
+void orc_imul(double al, double au, double bl, double bu, double* lo, double* hi) {
+    const int old = fegetround();
+    volatile double a0 = al, a1 = au, b0 = bl, b1 = bu;
+    fesetround(FE_DOWNWARD);
+    double p[4] = {a0 * b0, a0 * b1, a1 * b0, a1 * b1};
+    double l = p[0];
+    for (int t = 1; t < 4; ++t) if (p[t] < l) l = p[t];
+    fesetround(FE_UPWARD);
+    double q[4] = {a0 * b0, a0 * b1, a1 * b0, a1 * b1};
+    double h = q[0];
+    for (int t = 1; t < 4; ++t) if (q[t] > h) h = q[t];
+    fesetround(old);
+    *lo = l;
+    *hi = h;
+}
+
+int orc_uhphiphi(int m, int npts, const double* phi, int ldp, const double* w, const double* u, double* A,
+                 int lda) {
+    if (m < 0 || npts < 0 || ldp < (m > 1 ? m : 1) || lda < (m > 1 ? m : 1)) return -1;
+    const int nt = orc_get_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+    for (int i = 0; i < m; ++i) {
+        for (int j = i; j < m; ++j) {
+            double lo = 0.0, hi = 0.0;
+            for (int k = 0; k < npts; ++k) {
+                double vl, vh, sl, sh, pl, ph;
+                orc_imul(w[2 * k], w[2 * k + 1], u[2 * k], u[2 * k + 1], &vl, &vh);
+                orc_imul(vl, vh, phi[2 * (i + (size_t)ldp * k)], phi[2 * (i + (size_t)ldp * k) + 1], &sl, &sh);
+                orc_imul(sl, sh, phi[2 * (j + (size_t)ldp * k)], phi[2 * (j + (size_t)ldp * k) + 1], &pl, &ph);
+                const int old = fegetround();
+                volatile double x = lo, y = hi;
+                fesetround(FE_DOWNWARD);
+                lo = x + pl;
+                fesetround(FE_UPWARD);
+                hi = y + ph;
+                fesetround(old);
+            }
+            A[2 * (i + (size_t)lda * j)] = lo;
+            A[2 * (i + (size_t)lda * j) + 1] = hi;
+            A[2 * (j + (size_t)lda * i)] = lo;
+            A[2 * (j + (size_t)lda * i) + 1] = hi;
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Exact arithmetic: superaccumulator                                       */
 /* ------------------------------------------------------------------------ */
 #define SA_EMIN (-2208) /* bit weight of digit 0's least significant bit   */

@@ -67,6 +67,14 @@ int orc_residual(int n, const double* R, int ldr, const double* IA, int lda, dou
 int orc_lss(int n, int nrhs, const double* IA, int lda, const double* Ib, int ldb, const double* R, int ldr,
             double* X, int ldx, double* normub);
 
+/* ---- Galerkin matrix (Legendre_Bases_Generator::uhphiphi, vcp/ldbase.hpp:955-1013) */
+/* kv's interval product [min RD(a b), max RU(a b)] over the endpoint pairs */
+void orc_imul(double al, double au, double bl, double bu, double* lo, double* hi);
+/* A(i, j) = sum_k ((w_k u_k) phi(i, k)) phi(j, k), interval arithmetic, i <= j
+ * from [0, 0] in k order, mirrored */
+int orc_uhphiphi(int m, int npts, const double* phi, int ldp, const double* w, const double* u, double* A,
+                 int lda);
+
 /* ---- exact evaluation (superaccumulator; no rounding anywhere) ---------- */
 /* For entry (i,j) of IA*IB: the exact hull [lo*, hi*] of { sum_k a_k b_k }.
  * Writes sign(lo* - C_lower) and sign(C_upper - hi*) (each >= 0 means the

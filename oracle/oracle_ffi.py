@@ -48,6 +48,8 @@ def lib():
             "orc_pidblas_mulltmm": [_c_int, _c_int, _p_dbl, _c_int, _p_dbl, _c_int],
             "orc_pidblas_vmulmm": [_c_int, _c_int, _p_dbl, _c_int, _p_dbl, _c_int],
             "orc_residual": [_c_int, _p_dbl, _c_int, _p_dbl, _c_int, _p_dbl, _c_int, _p_dbl],
+            "orc_imul": [_c_dbl, _c_dbl, _c_dbl, _c_dbl, _p_dbl, _p_dbl],
+            "orc_uhphiphi": [_c_int, _c_int, _p_dbl, _c_int, _p_dbl, _p_dbl, _p_dbl, _c_int],
             "orc_lss": [_c_int, _c_int, _p_dbl, _c_int, _p_dbl, _c_int, _p_dbl, _c_int, _p_dbl, _c_int, _p_dbl],
             "orc_exact_hull_entry": [_c_int, _p_dbl, _c_int, _p_dbl, _c_int, _c_dbl, _c_dbl,
                                      _p_int, _p_int, _p_dbl, _p_dbl],
@@ -75,7 +77,7 @@ def lib():
             f.argtypes = args
             f.restype = None if name in ("orc_rgemm", "orc_rsyrk", "orc_midrad", "orc_exact_hull_entry",
                                          "orc_exact_dot", "orc_exact_dd_dot", "orc_check_containment",
-                                         "orc_ozaki_scaled_rows", "orc_ozaki_scaled_cols",
+                                         "orc_ozaki_scaled_rows", "orc_ozaki_scaled_cols", "orc_imul",
                                          "orc_set_threads") else _c_int
         _lib = L
     return _lib
@@ -195,6 +197,26 @@ def residual(R, IA):
     lib().orc_residual(n, _dp(R), max(n, 1), _dp(IA.transpose(1, 0, 2)), _ld_pair(IA),
                        _dp(C.transpose(1, 0, 2)), _ld_pair(C), ctypes.byref(nub))
     return C, nub.value
+
+
+def imul(al, au, bl, bu):
+    lo, hi = ctypes.c_double(), ctypes.c_double()
+    lib().orc_imul(al, au, bl, bu, ctypes.byref(lo), ctypes.byref(hi))
+    return lo.value, hi.value
+
+
+def uhphiphi(phi, w, u):
+    """Galerkin matrix of Legendre_Bases_Generator::uhphiphi: phi (m, npts, 2)
+    AoS column-major intervals, w and u (npts, 2)"""
+    m, npts = phi.shape[:2]
+    A = pairs(m, m)
+    wv = np.ascontiguousarray(w, dtype=np.float64).reshape(npts, 2)
+    uv = np.ascontiguousarray(u, dtype=np.float64).reshape(npts, 2)
+    rc = lib().orc_uhphiphi(m, npts, _dp(phi.transpose(1, 0, 2)), _ld_pair(phi), _dp(wv), _dp(uv),
+                            _dp(A.transpose(1, 0, 2)), _ld_pair(A))
+    if rc:
+        raise ValueError("orc_uhphiphi: bad dimensions")
+    return A
 
 
 def lss(IA, Ib, R):

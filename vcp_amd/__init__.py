@@ -76,6 +76,8 @@ SIGNATURES = {
     "vcphip_ddgemm_ta_dev": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_dd_residual": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_dd_residual_dev": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
+    "vcphip_uhphiphi": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _vp, _vp, _c_int],
+    "vcphip_uhphiphi_dev": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _vp, _vp, _c_int],
     "vcphip_mat_info": [ctypes.c_char_p, _p_int, _p_int, _p_int, _p_int],
     "vcphip_mat_read": [ctypes.c_char_p, _c_int, _c_int, _c_int, _vp, _c_int],
     "vcphip_mat_write": [ctypes.c_char_p, _c_int, _c_int, _c_int, _vp, _c_int, _c_int],
@@ -338,6 +340,20 @@ class Context:
         pc, ldc = _pair_ptr(C)
         _check(lib().vcphip_ddgemm(self._h, m, n, k, pa, lda, pb, ldb, pc, ldc), "vcphip_ddgemm")
         return C
+
+    def uhphiphi(self, phi, w, u):
+        """Galerkin matrix A(i, j) = sum_k ((w_k u_k) phi(i, k)) phi(j, k) in
+        interval arithmetic (Legendre_Bases_Generator::uhphiphi, vcp/ldbase.hpp:
+        955-1013): phi (m, npts, 2), w and u (npts, 2) intervals."""
+        m, npts = phi.shape[:2]
+        A = pairs(m, m)
+        pp, ldp = _pair_ptr(phi)
+        wv = np.ascontiguousarray(w, dtype=np.float64).reshape(npts, 2)
+        uv = np.ascontiguousarray(u, dtype=np.float64).reshape(npts, 2)
+        pa, lda = _pair_ptr(A)
+        _check(lib().vcphip_uhphiphi(self._h, m, npts, pp, ldp, wv.ctypes.data_as(_vp), uv.ctypes.data_as(_vp), pa,
+                                     lda), "vcphip_uhphiphi")
+        return A
 
     def ddgemm_ta(self, A, B):
         """C = A^T B for double-double matrices (A k x m, B k x n)."""

@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvcphip.so")
-SOURCES = ["kern_drgemm.hip", "kern_ozaki.hip", "kern_dd.hip", "kern_tune.hip", "kern_lss.hip", "vcphip_host.hip", "vcphip_io.cpp"]
+SOURCES = ["kern_drgemm.hip", "kern_ozaki.hip", "kern_dd.hip", "kern_tune.hip", "kern_lss.hip", "kern_galerkin.hip", "vcphip_host.hip", "vcphip_io.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -926,6 +926,20 @@ extern "C" int vcphip_lss_batched_dev(vcphip_ctx* c, int count, const int* n, co
   return VCPHIP_OK;
 }
 
+// Galerkin matrix of Legendre_Bases_Generator::uhphiphi (vcp/ldbase.hpp:955-1013)
+extern "C" int vcphip_uhphiphi_dev(vcphip_ctx* c, int m, int npts, const double* phi, int ldp, const double* w,
+                                   const double* u, double* A, int lda) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (m < 0 || npts < 0 || ldp < (m > 1 ? m : 1) || lda < (m > 1 ? m : 1)) return VCPHIP_ERR_DIMENSION;
+  if (m == 0) return VCPHIP_OK;
+  if (!phi || !w || !u || !A) return VCPHIP_ERR_ARGUMENT;
+  hipSetDevice(c->device);
+  StageTimer T(c);
+  const int rc = check_launch(vcphip_launch_uhphiphi(m, npts, phi, ldp, w, u, A, lda, c->stream));
+  T.mark("uhphiphi");
+  return rc;
+}
+
 // ===========================================================================
 // host-pointer entry points: stage through the context workspace
 // ===========================================================================
@@ -1201,6 +1215,23 @@ extern "C" int vcphip_lss_batched(vcphip_ctx* c, int count, const int* n, const 
       return VCPHIP_ERR_DEVICE;
   if (hipStreamSynchronize(c->stream) != hipSuccess) return VCPHIP_ERR_DEVICE;
   return rc;
+}
+
+extern "C" int vcphip_uhphiphi(vcphip_ctx* c, int m, int npts, const double* phi, int ldp, const double* w,
+                               const double* u, double* A, int lda) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (m < 0 || npts < 0 || ldp < (m > 1 ? m : 1) || lda < (m > 1 ? m : 1)) return VCPHIP_ERR_DIMENSION;
+  if (m == 0) return VCPHIP_OK;
+  if (!phi || !w || !u || !A) return VCPHIP_ERR_ARGUMENT;
+  hipSetDevice(c->device);
+  double *dP, *dW, *dU;
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HA, phi, ldp, m, npts, 16, &dP));
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HB, w, 1, 1, npts, 16, &dW));
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HR, u, 1, 1, npts, 16, &dU));
+  void* p;
+  VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HC, 16 * (size_t)m * m, &p));
+  VCPHIP_TRY(vcphip_uhphiphi_dev(c, m, npts, dP, m, dW, dU, (double*)p, m));
+  return to_host(c, (double*)p, A, lda, m, m, 16);
 }
 
 extern "C" int vcphip_residual_batched(vcphip_ctx* c, int count, const int* n, const double* const* R, const int* ldr,

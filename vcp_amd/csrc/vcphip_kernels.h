@@ -158,3 +158,8 @@ int vcphip_lss_launch_batched_rhs(int count, long items, const long* pb, const V
                                   hipStream_t s);
 int vcphip_lss_launch_batched_bound(int count, long items, const long* pb, const VcphipLssProblem* probs,
                                     hipStream_t s);
+
+// Galerkin matrix A(i, j) = sum_k ((w_k u_k) phi(i, k)) phi(j, k) in interval
+// arithmetic, symmetric (kern_galerkin.hip)
+int vcphip_launch_uhphiphi(int m, int npts, const double* phi, long ldp, const double* w, const double* u, double* A,
+                           long lda, hipStream_t s);

@@ -96,3 +96,35 @@ def config5(count=256):
     sizes = config5_sizes(count)
     rng = np.random.default_rng(SEED0 + 50)
     return [residual_problem(rng, n) for n in sizes]
+
+
+def galerkin(p, q, rad_exp=None):
+    """Operands of Legendre_Bases_Generator::uhphiphi for a 2D tensor basis:
+    phi(i, k) = f_{i1}(x_{k1}) f_{i2}(x_{k2}) with the Dirichlet-type Legendre
+    functions f_n = (P_{n+1} - P_{n-1}) / sqrt(2 (2n + 1)), n = 1..p, at
+    q x q Gauss-Legendre points of [-1, 1]^2; w_k the product weights and
+    u_k = sin(pi x) cos(pi y / 2) + 2 at the points.  Returns phi (p^2, q^2, 2)
+    and w, u (q^2, 2) as intervals (point intervals, or radii 2^rad_exp |x|)."""
+    from numpy.polynomial import legendre as L
+
+    x, wt = L.leggauss(q)
+    f = np.zeros((p, q))
+    for n in range(1, p + 1):
+        cp = np.zeros(n + 2)
+        cp[n + 1] = 1.0
+        cm = np.zeros(n + 2)
+        cm[n - 1] = 1.0
+        f[n - 1] = (L.legval(x, cp) - L.legval(x, cm)) / np.sqrt(2.0 * (2 * n + 1))
+    phi_v = np.einsum("ak,bl->abkl", f, f).reshape(p * p, q * q)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    w_v = np.outer(wt, wt).reshape(-1)
+    u_v = (np.sin(np.pi * X) * np.cos(0.5 * np.pi * Y) + 2.0).reshape(-1)
+
+    def iv(v):
+        rad = np.zeros_like(v) if rad_exp is None else np.ldexp(np.abs(v), rad_exp)
+        return interval_from_midrad(v.reshape(v.shape[0], -1), rad.reshape(v.shape[0], -1))
+
+    phi = iv(phi_v)
+    w = iv(w_v.reshape(-1, 1)).reshape(-1, 2)
+    u = iv(u_v.reshape(-1, 1)).reshape(-1, 2)
+    return phi, np.ascontiguousarray(w), np.ascontiguousarray(u)
