@@ -283,12 +283,11 @@ __global__ __launch_bounds__(256) void k_dd_recon(int m, int n, int mpad, int np
                                                   const int* __restrict__ eb, const int* __restrict__ fb,
                                                   const uint8_t* __restrict__ U, double* __restrict__ C, long ldc) {
   // threads follow the blocked 16 x 16 layout of the residue planes
-  const long boff = (long)blockIdx.x * 256 + threadIdx.x;
-  const long blk = boff >> 8;
-  const int o = (int)(boff & 255);
-  const long bpm = mpad / 16;
-  const int i = (int)(blk % bpm) * 16 + 4 * (o >> 6) + (o & 3);
-  const int j = (int)(blk / bpm) * 16 + ((o >> 2) & 15);
+  // grid: x = 16-row block, y = 16-column block (the plane's block order)
+  const int o = threadIdx.x;
+  const long boff = ((long)blockIdx.y * (mpad >> 4) + blockIdx.x) * 256 + o;
+  const int i = (int)blockIdx.x * 16 + 4 * (o >> 6) + (o & 3);
+  const int j = (int)blockIdx.y * 16 + ((o >> 2) & 15);
   if (i >= m || j >= n) return;
   if (fa[i] || fb[j]) {   // a zero row / column gives 0, a non-finite one NaN
     const double v = ((fa[i] | fb[j]) & 2) ? __builtin_nan("") : 0.0;
@@ -418,8 +417,7 @@ int vcphip_dd_split_cols(const double* B, long ldb, int k, int n, int npad, int 
 int vcphip_dd_recon(int m, int n, int mpad, int npad, const VcphipOzParams& P, const int* ea, const int* fa,
                     const int* eb, const int* fb, const uint8_t* U, double* C, long ldc, hipStream_t s) {
   if (m <= 0 || n <= 0) return 0;
-  const long total = (long)mpad * npad;
-  dim3 grid((unsigned)((total + 255) / 256));
+  dim3 grid((unsigned)(mpad / 16), (unsigned)(npad / 16));
   switch (P.nmod) {
 #define VCPHIP_DD_CASE(NN) \
   case NN: hipLaunchKernelGGL((k_dd_recon<NN>), grid, dim3(256), 0, s, m, n, mpad, npad, P, ea, fa, eb, fb, U, C, ldc); break;

@@ -940,12 +940,11 @@ __global__ __launch_bounds__(256) void k_oz_recon(int m, int n, int k, int mpad,
   // threads follow the blocked 16 x 16 layout of the planes (coalesced
   // residue reads): offset o in a block holds row 4 (o / 64) + o % 4 and
   // column (o / 4) % 16 of that block
-  const long boff = (long)blockIdx.x * 256 + threadIdx.x;
-  const long blk = boff >> 8;
-  const int o = (int)(boff & 255);
-  const long bpm = mpad / 16;
-  const int i = (int)(blk % bpm) * 16 + 4 * (o >> 6) + (o & 3);
-  const int j = (int)(blk / bpm) * 16 + ((o >> 2) & 15);
+  // grid: x = 16-row block, y = 16-column block (the plane's block order)
+  const int o = threadIdx.x;
+  const long boff = ((long)blockIdx.y * (mpad >> 4) + blockIdx.x) * 256 + o;
+  const int i = (int)blockIdx.x * 16 + 4 * (o >> 6) + (o & 3);
+  const int j = (int)blockIdx.y * 16 + ((o >> 2) & 15);
   if (i >= m || j >= n) return;
   const long plane = (long)mpad * npad;
 
@@ -1115,8 +1114,7 @@ int vcphip_oz_recon(int m, int n, int k, int mpad, int npad, const VcphipOzParam
                     VcphipOzSide sb, const uint8_t* U, const int32_t* Z, double* C, long ldc,
                     hipStream_t s) {
   if (m <= 0 || n <= 0) return 0;
-  const long total = (long)mpad * npad;
-  dim3 grid((unsigned)((total + 255) / 256));
+  dim3 grid((unsigned)(mpad / 16), (unsigned)(npad / 16));
   switch (P.nmod) {
 #define VCPHIP_RECON_CASE(NN) \
   case NN: hipLaunchKernelGGL((k_oz_recon<NN>), grid, dim3(256), 0, s, m, n, k, mpad, npad, P, sa, sb, U, Z, C, ldc); break;

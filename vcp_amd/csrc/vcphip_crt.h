@@ -19,16 +19,12 @@
 #include "vcphip_device.h"
 #include "vcphip_kernels.h"
 
-// Residues of element `off` from N planes of `plane` bytes; returns the sign
-// of X and its magnitude in NL little-endian 32-bit limbs.
+// From the residues u[t] (bytes) of X: returns the sign of X and its
+// magnitude in NL little-endian 32-bit limbs.
 template <int N, int NL>
-VCPHIP_DEVINL bool vcphip_crt_symmetric(const uint8_t* __restrict__ U, long plane, long off,
-                                        const VcphipOzParams& P, unsigned (&mag)[NL]) {
+VCPHIP_DEVINL bool vcphip_crt_from_residues(const unsigned (&u)[N], const VcphipOzParams& P, unsigned (&mag)[NL]) {
   constexpr int G = (N + 3) / 4;
   constexpr int NB = N < 4 * NL ? N : 4 * NL;   // byte columns below 2^(32 NL); w_t < M < 2^(8 N)
-  unsigned u[N];
-#pragma unroll
-  for (int t = 0; t < N; ++t) u[t] = U[(long)t * plane + off];
   unsigned r[G];
 #pragma unroll
   for (int g = 0; g < G; ++g) {
@@ -72,4 +68,14 @@ VCPHIP_DEVINL bool vcphip_crt_symmetric(const uint8_t* __restrict__ U, long plan
     }
   }
   return neg;
+}
+
+// Residues of element `off` from N planes of `plane` bytes
+template <int N, int NL>
+VCPHIP_DEVINL bool vcphip_crt_symmetric(const uint8_t* __restrict__ U, long plane, long off,
+                                        const VcphipOzParams& P, unsigned (&mag)[NL]) {
+  unsigned u[N];
+#pragma unroll
+  for (int t = 0; t < N; ++t) u[t] = U[(long)t * plane + off];
+  return vcphip_crt_from_residues<N, NL>(u, P, mag);
 }
