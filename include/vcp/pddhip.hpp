@@ -13,9 +13,8 @@
 // vcp/matrix.hpp:245-267, :607-634); inv() (virtual, :477) is overridden so
 // that it reaches pddhip::linearsolve.  linearsolve keeps the reference's
 // refinement loop and forms its residuals r = b - A x on the GPU
-// (vcphip_dd_residual); Cholesky and eigsym keep the reference's
-// refinements with their dd products on the GPU (eigsymge stays on the
-// host).  Scalar operands and products below the GPU / host crossover
+// (vcphip_dd_residual); Cholesky, eigsym and eigsymge keep the reference's
+// refinements with their dd products on the GPU.  Scalar operands and products below the GPU / host crossover
 // (vcphip_policy::crossover, measured by tune_crossover()) keep the
 // reference's host semantics.
 #pragma once
@@ -221,7 +220,41 @@ class pddhip : public vcp::pddblas {
     for (int q = 0; q < nn; ++q) X[q] = kv::dd(Vd.v[q]);
     for (int i = 0; i < an; ++i) lam[i] = kv::dd(Ad.v[i + an * i]);
     const int steps = itep < 2 ? 2 : itep;
-    for (int step = 0; step < steps; ++step) oa_step_gpu(this->v, X, lam, an);
+    for (int step = 0; step < steps; ++step) oa_step_gpu(this->v, nullptr, X, lam, an);
+    V.zeros(an, an);
+    V.v = std::move(X);
+    this->zeros(an, an);
+    for (int i = 0; i < an; ++i) this->v[i + an * i] = lam[i];
+  }
+
+  // Generalized problem A x = lambda B x (pddblas.hpp:611-659): dsygv of the
+  // high parts on the host, then max(itep, 2) Ogita-Aishima steps with the
+  // B-inner product; their products are GPU dd products.
+  void eigsymge(vcp::pddblas& B, int itep = 1) {
+    vcp::pddblas V;
+    this->eigsymge(B, V, itep);
+  }
+  void eigsymge(vcp::pddblas& B, vcp::pddblas& V, int itep = 1) {
+    if (!this->is_symmetric()) vcp::throw_error<vcp::domain_error>("eigsymge: matrix A must be symmetric");
+    if (!B.is_symmetric()) vcp::throw_error<vcp::domain_error>("eigsymge: matrix B must be symmetric");
+    if (this->row != B.row || this->column != B.column) {
+      vcp::throw_error<vcp::dimension_error>("eigsymge: dimension mismatch: A=(", this->row, ", ", this->column,
+                                             "), B=(", B.row, ", ", B.column, ")");
+    }
+    const int an = this->row, nn = an * an;
+    if (vcphip_policy::on_host(vcphip_policy::CROSS_DD, an, an, an)) {
+      vcp::pddblas::eigsymge(B, V, itep);
+      return;
+    }
+    vcp::pdblas Ad, Bd, Vd;
+    dd_to_d(*this, Ad);
+    dd_to_d(B, Bd);
+    Ad.eigsymge(Bd, Vd);
+    std::vector<kv::dd> X(nn), lam(an);
+    for (int q = 0; q < nn; ++q) X[q] = kv::dd(Vd.v[q]);
+    for (int i = 0; i < an; ++i) lam[i] = kv::dd(Ad.v[i + an * i]);
+    const int steps = itep < 2 ? 2 : itep;
+    for (int step = 0; step < steps; ++step) oa_step_gpu(this->v, &B.v, X, lam, an);
     V.zeros(an, an);
     V.v = std::move(X);
     this->zeros(an, an);
@@ -230,15 +263,22 @@ class pddhip : public vcp::pddblas {
 
  private:
   // one Ogita-Aishima refinement of the eigenpairs (X columns, lam) of the
-  // symmetric n x n matrix A (pddblas.hpp:219-314 with B = I)
-  static void oa_step_gpu(const std::vector<kv::dd>& A, std::vector<kv::dd>& X, std::vector<kv::dd>& lam, int n) {
+  // symmetric n x n matrix A with the B-inner product (B = I when B is
+  // null), pddblas.hpp:219-314
+  static void oa_step_gpu(const std::vector<kv::dd>& A, const std::vector<kv::dd>* B, std::vector<kv::dd>& X,
+                          std::vector<kv::dd>& lam, int n) {
     using pddhip_detail::raw;
     const int nn = n * n;
     vcphip_ctx* ctx = vcphip_policy::checked_context<E>();
     std::vector<kv::dd> AX(nn), S(nn), R(nn), Ecorr(nn), XE(nn);
     vcphip_policy::check<E>(vcphip_ddgemm(ctx, n, n, n, raw(A), n, raw(X), n, raw(AX), n), "vcphip_ddgemm");
     vcphip_policy::check<E>(vcphip_ddgemm_ta(ctx, n, n, n, raw(X), n, raw(AX), n, raw(S), n), "vcphip_ddgemm_ta");
-    vcphip_policy::check<E>(vcphip_ddgemm_tn(ctx, n, n, raw(X), n, raw(R), n), "vcphip_ddgemm_tn");   // R = X^T X
+    if (B) {   // R = X^T (B X)
+      vcphip_policy::check<E>(vcphip_ddgemm(ctx, n, n, n, raw(*B), n, raw(X), n, raw(XE), n), "vcphip_ddgemm");
+      vcphip_policy::check<E>(vcphip_ddgemm_ta(ctx, n, n, n, raw(X), n, raw(XE), n, raw(R), n), "vcphip_ddgemm_ta");
+    } else {   // R = X^T X
+      vcphip_policy::check<E>(vcphip_ddgemm_tn(ctx, n, n, raw(X), n, raw(R), n), "vcphip_ddgemm_tn");
+    }
     for (int i = 0; i < n; ++i) lam[i] = S[i + n * i] / R[i + n * i];   // Rayleigh quotients
     for (int q = 0; q < nn; ++q) R[q] = -R[q];                           // R = I - X^T X
     for (int i = 0; i < n; ++i) R[i + n * i] += kv::dd(1.0);

@@ -158,3 +158,27 @@ def test_dd_eigsym(ctx):
         g = sum(val(X[t, i]) * val(X[t, j]) for t in range(n)) - (1 if i == j else 0)
         assert abs(g) <= F(2) ** -98 * n
     assert np.allclose(np.sort(lam[:, 0]), np.arange(1.0, n + 1), rtol=0, atol=1e-12)
+
+
+def test_dd_eigsymge(ctx):
+    """A x = lambda B x: exact residuals A x - lambda B x and X^T B X - I"""
+    rng = np.random.default_rng(23)
+    n = 20
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    S = Q @ np.diag(np.linspace(-3.0, 5.0, n)) @ Q.T
+    A = O.pairs(n, n)
+    A[..., 0] = (S + S.T) * 0.5
+    G = rng.standard_normal((n, n))
+    Bm = G @ G.T / n + np.eye(n)
+    B = O.pairs(n, n)
+    B[..., 0] = (Bm + Bm.T) * 0.5
+    lam, X = ctx.dd_eigsymge(A, B)
+    anorm = float(np.abs(A[..., 0]).sum(axis=1).max()) + float(np.abs(B[..., 0]).sum(axis=1).max()) * 5
+    for c in (0, 9, 19):
+        for i in (0, 10, 19):
+            r = sum(val(A[i, t]) * val(X[t, c]) for t in range(n)) - \
+                val(lam[c]) * sum(val(B[i, t]) * val(X[t, c]) for t in range(n))
+            assert abs(r) <= F(2) ** -95 * F(anorm)
+    for i, j in ((0, 0), (2, 5), (19, 19)):
+        g = sum(val(X[s, i]) * sum(val(B[s, t]) * val(X[t, j]) for t in range(n)) for s in range(n)) - (1 if i == j else 0)
+        assert abs(g) <= F(2) ** -96 * n

@@ -436,6 +436,13 @@ class Context:
 
         return ddrefine.eigsym(self, A, itep)
 
+    def dd_eigsymge(self, A, B, itep=1):
+        """(eigenvalues, eigenvectors) of A x = lambda B x for symmetric dd A,
+        B (pddblas::eigsymge, vcp/pddblas.hpp:611-659; products on the GPU)."""
+        from . import ddrefine
+
+        return ddrefine.eigsymge(self, A, B, itep)
+
     def ddgemm_tn(self, A):
         """C = A^T A for a double-double matrix (pddblas::mulltmm)."""
         k, n = A.shape[:2]

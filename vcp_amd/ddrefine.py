@@ -1,8 +1,8 @@
 """Host side of pddblas's refinement algorithms with their dd products on the
 GPU: Cholesky (vcp/pddblas.hpp:488-563) and the Ogita-Aishima symmetric
-eigensolver (oa_step :219-314, eigsym :566-606).  The double-precision
-starting points come from LAPACK on the host (scipy / numpy), as the
-reference's dpotrf / dsyev do; every ozaki_acc product of the reference is a
+eigensolvers (oa_step :219-314, eigsym :566-606, eigsymge :611-659).  The
+double-precision starting points come from LAPACK on the host (scipy /
+numpy), as the reference's dpotrf / dsyev / dsygv do; every ozaki_acc product of the reference is a
 vcphip dd product (ddgemm, ddgemm_tn for X^T X, ddgemm_ta for X^T Y); the
 elementwise dd arithmetic between them is the round-to-nearest
 double-double arithmetic below (kv::dd's own operator bits are not
@@ -115,12 +115,13 @@ def cholesky(ctx, A, itep=2):
 
 
 # ---- Ogita-Aishima: pddblas::oa_step / eigsym (vcp/pddblas.hpp:219-314, 566-606)
-def oa_step(ctx, A, X, lam):
-    """one refinement step of the eigenpairs (X columns, lam) of symmetric A"""
+def oa_step(ctx, A, X, lam, B=None):
+    """one refinement step of the eigenpairs (X columns, lam) of symmetric A,
+    with the B-inner product when B is given (A x = lambda B x)"""
     n = A.shape[0]
     AX = ctx.ddgemm(A, X)
     S = ctx.ddgemm_ta(X, AX)
-    G = ctx.ddgemm_tn(X)
+    G = ctx.ddgemm_tn(X) if B is None else ctx.ddgemm_ta(X, ctx.ddgemm(B, X))
     dg = np.arange(n)
     lam = np.stack(dd_div(S[dg, dg, 0], S[dg, dg, 1], G[dg, dg, 0], G[dg, dg, 1]), axis=-1)
     # R = I - G
@@ -166,4 +167,25 @@ def eigsym(ctx, A, itep=4):
     lam = np.stack([w, np.zeros(n)], axis=-1)
     for _ in range(max(itep, 2)):
         X, lam = oa_step(ctx, A, X, lam)
+    return lam, X
+
+
+def eigsymge(ctx, A, B, itep=1):
+    """(lam, V) of the generalized problem A x = lambda B x (A symmetric, B
+    symmetric positive definite, dd): LAPACK's dsygv-class solver
+    (scipy.linalg.eigh) on the high parts, then max(itep, 2) Ogita-Aishima
+    steps with the B-inner product (X^T B X = I), products on the GPU."""
+    import scipy.linalg
+
+    n = A.shape[0]
+    for M, name in ((A, "A"), (B, "B")):
+        if M.shape[:2] != (n, n):
+            raise ValueError("eigsymge: dimension mismatch")
+        if not (np.array_equal(M[..., 0], M[..., 0].T) and np.array_equal(M[..., 1], M[..., 1].T)):
+            raise ValueError("eigsymge: matrix %s must be symmetric" % name)
+    w, Vd = scipy.linalg.eigh(A[..., 0], B[..., 0])
+    X = _set(_pairs(n, n), Vd, 0.0)
+    lam = np.stack([w, np.zeros(n)], axis=-1)
+    for _ in range(max(itep, 2)):
+        X, lam = oa_step(ctx, A, X, lam, B)
     return lam, X
