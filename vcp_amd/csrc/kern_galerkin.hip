@@ -57,28 +57,46 @@ __global__ __launch_bounds__(NT) void k_uhphiphi(int m, int npts, const double* 
       up[i][j] = 0.0;
       dn[i][j] = -0.0;
     }
-  for (int k0 = 0; k0 < npts; k0 += BK) {
+  // stage k0 in registers: the global loads of the next stage are issued
+  // before the current stage's arithmetic and stored to LDS after it
+  constexpr int NQ = BK / RPP;
+  double2 ga[NQ], gb[NQ], gw[NQ], gu[NQ];
+  auto load_stage = [&](int k0) {
 #pragma unroll
-    for (int q = 0; q < BK / RPP; ++q) {
-      const int kl = lk + RPP * q, k = k0 + kl;
-      double2 a = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
-      double2 s = make_double2(-0.0, 0.0);
-      if (k < npts) {
-        const double2 wk = *reinterpret_cast<const double2*>(w + 2 * k);
-        const double2 uk = *reinterpret_cast<const double2*>(u + 2 * k);
-        const double2 v = imul_neg(wk.x, wk.y, uk.x, uk.y);   // w_k * u_k
-        if (row0 + lr < m) {
-          a = *reinterpret_cast<const double2*>(phi + 2 * ((row0 + lr) + ldp * k));
-          s = imul_neg(-v.x, v.y, a.x, a.y);                   // (w_k u_k) * phi(i, k)
-        }
-        if (col0 + lr < m) b = *reinterpret_cast<const double2*>(phi + 2 * ((col0 + lr) + ldp * k));
-      }
-      sSl[kl][lr] = -s.x;
-      sSu[kl][lr] = s.y;
-      sPl[kl][lr] = b.x;
-      sPu[kl][lr] = b.y;
+    for (int q = 0; q < NQ; ++q) {
+      const int k = k0 + lk + RPP * q;
+      const bool kok = k < npts;
+      gw[q] = kok ? *reinterpret_cast<const double2*>(w + 2 * k) : make_double2(0.0, 0.0);
+      gu[q] = kok ? *reinterpret_cast<const double2*>(u + 2 * k) : make_double2(0.0, 0.0);
+      ga[q] = (kok && row0 + lr < m) ? *reinterpret_cast<const double2*>(phi + 2 * ((row0 + lr) + ldp * k))
+                                     : make_double2(0.0, 0.0);
+      gb[q] = (kok && col0 + lr < m) ? *reinterpret_cast<const double2*>(phi + 2 * ((col0 + lr) + ldp * k))
+                                     : make_double2(0.0, 0.0);
     }
-    __syncthreads();
+  };
+  auto store_stage = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int kl = lk + RPP * q, k = k0 + kl;
+      double2 sv = make_double2(-0.0, 0.0);
+      if (k < npts && row0 + lr < m) {
+        const double2 v = imul_neg(gw[q].x, gw[q].y, gu[q].x, gu[q].y);   // w_k * u_k
+        sv = imul_neg(-v.x, v.y, ga[q].x, ga[q].y);                     // (w_k u_k) * phi(i, k)
+      }
+      sSl[kl][lr] = -sv.x;
+      sSu[kl][lr] = sv.y;
+      sPl[kl][lr] = gb[q].x;
+      sPu[kl][lr] = gb[q].y;
+    }
+  };
+  if (npts > 0) {
+    load_stage(0);
+    store_stage(0);
+  }
+  __syncthreads();
+  for (int k0 = 0; k0 < npts; k0 += BK) {
+    const bool more = k0 + BK < npts;
+    if (more) load_stage(k0 + BK);
 #pragma unroll 2
     for (int kk = 0; kk < BK; ++kk) {
       double sl[TM], su[TM], pl[TM], pu[TM];
@@ -99,6 +117,10 @@ __global__ __launch_bounds__(NT) void k_uhphiphi(int m, int npts, const double* 
         }
     }
     __syncthreads();
+    if (more) {
+      store_stage(k0 + BK);
+      __syncthreads();
+    }
   }
 #pragma unroll
   for (int i = 0; i < TM; ++i)
