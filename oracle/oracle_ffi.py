@@ -369,6 +369,8 @@ def ref_lib():
         L.ref_udmatmul.argtypes = [_c_int, _c_int, _c_int, _p_dbl, _p_dbl, _p_dbl, _p_dbl]
         L.ref_udmatmul.restype = _c_int
         L.ref_backend.restype = _c_int
+        L.ref_pidblas_mulmm.argtypes = [_c_int, _c_int, _c_int, _p_dbl, _p_dbl, _p_dbl]
+        L.ref_pidblas_mulmm.restype = None
         _ref = L
     return _ref
 
@@ -382,6 +384,23 @@ def ref_rmatmul(A, B, mode):
     n = B.shape[1]
     C = fmat(m, n)
     L.ref_rmatmul(m, k, n, _dp(A), _dp(B), _dp(C), mode)
+    return C
+
+
+def ref_pidblas_mulmm(IA, IB):
+    """pidblas::mulmm with the reference's own vblas rmatmul for its four
+    products (oracle/_ref); None when the reference build is absent."""
+    L = ref_lib()
+    if L is None:
+        return None
+    m, k = IA.shape[:2]
+    n = IB.shape[1]
+    A = np.ascontiguousarray(IA.transpose(1, 0, 2))
+    B = np.ascontiguousarray(IB.transpose(1, 0, 2))
+    C = pairs(m, n)
+    Cv = np.zeros((n, m, 2))
+    L.ref_pidblas_mulmm(m, n, k, _dp(A), _dp(B), _dp(Cv))
+    C[...] = Cv.transpose(1, 0, 2)
     return C
 
 

@@ -383,3 +383,17 @@ def test_ozaki_nonfinite_operands():
     S = O.ozaki_ddgemm(T, None, 31)
     assert np.isnan(S[2, :]).all() and np.isnan(S[:, 2]).all()
     assert np.isfinite(np.delete(np.delete(S, 2, 0), 2, 1)).all()
+
+
+def test_reference_built_pidblas_product_matches_the_restatement():
+    """pidblas::mulmm built on the reference's own vblas rmatmul (oracle/_ref)
+    is bit-identical to the oracle's restatement (both are the k-ascending
+    directed chains with the reference's fix-ups)"""
+    if O.ref_lib() is None:
+        pytest.skip("oracle/_ref not built (needs the reference tree)")
+    from vcp_amd import workloads as W
+
+    rng = np.random.default_rng(77)
+    for m, n, k in ((1, 1, 1), (9, 7, 33), (70, 40, 300)):
+        IA, IB = W.wide_exponent_intervals(rng, m, k), W.wide_exponent_intervals(rng, k, n)
+        assert np.array_equal(O.ref_pidblas_mulmm(IA, IB), O.pidblas_mulmm(IA, IB))
