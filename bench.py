@@ -394,6 +394,22 @@ def bench_resid(ctx, reps):
         lss["oracle_bitexact_smallest3"] = bool(ok)
     except Exception as e:
         log("lss oracle check skipped: %r" % (e,))
+    # the same solves with every R_p = inv(mid(IA_p)) formed on the device
+    # first (imats.hpp:298-301; rocSOLVER getrf + getrs, one batched call per
+    # order rounded up to a multiple of 64), so the whole imats::linearsolve
+    # runs on the GPU
+    largs_inv = largs[:6] + (None, None) + largs[8:]
+    gi = ctx.lss_batched_dev(*largs_inv)
+    lat = []
+    for _ in range(max(3, reps // 16)):
+        t0 = time.perf_counter()
+        ctx.lss_batched_dev(*largs_inv)
+        lat.append(time.perf_counter() - t0)
+    lat = np.array(lat) * 1e3
+    lss["with_device_inverse"] = {"calls": len(lat), "orders": len(set(ns)),
+                                  "p50_ms": round(float(np.percentile(lat, 50)), 4),
+                                  "p99_ms": round(float(np.percentile(lat, 99)), 4),
+                                  "all_verified": bool(max(gi) < 1.0)}
     log("batched verified solve %s" % lss)
     out["lss_batched"] = lss
     return out

@@ -164,10 +164,10 @@ class pidhip_t : public Base {
                                          pidhip_detail::raw(c.v), m),
                             "vcphip_igemm");
   }
-  // x = A \ b with verification (imats::linearsolve, vcp/imats.hpp:292-362):
-  // R = inv(mid A) on the host (imid + pdblas::inv, LAPACK, :298-301, as in
-  // the reference), every later step on the GPU in one call (vcphip_lss,
-  // pidblas's products and roundings throughout).  Systems of order < 2,
+  // x = A \ b with verification (imats::linearsolve, vcp/imats.hpp:292-362),
+  // every step on the GPU in one call (vcphip_lss with R = NULL): R =
+  // inv(mid A) by rocSOLVER getrf/getrs in place of imid + pdblas::inv's
+  // LAPACK (:298-301), then pidblas's products and roundings throughout.  Systems of order < 2,
   // those below the crossover and pidhip_fma (whose |RA - I| * T product is
   // the inf-sup one) keep Base::linearsolve, which still reaches the GPU
   // through the hooks above.
@@ -181,13 +181,12 @@ class pidhip_t : public Base {
       Base::linearsolve(b, x);
       return;
     }
-    vcp::pdblas R;
-    this->imid(R);
-    R.inv();
     x.zeros(n, nrhs);
     double g = 0.0;
     const int rc = vcphip_lss(vcphip_policy::checked_context<E>(), n, nrhs, pidhip_detail::raw(this->v), n,
-                              pidhip_detail::raw(b.v), n, R.v.data(), n, pidhip_detail::raw(x.v), n, &g);
+                              pidhip_detail::raw(b.v), n, nullptr, n, pidhip_detail::raw(x.v), n, &g);
+    if (rc == VCPHIP_ERR_SINGULAR)   // pdblas::inv's dgetrf failure (vcp/pdblas.hpp:191-193)
+      throw vcp::lapack_error("rocsolver_dgetrf", VCPHIP_ERR_SINGULAR, "inv: zero pivot in the LU of mid(A)");
     if (rc == VCPHIP_OK || rc == VCPHIP_ERR_VERIFICATION)
       std::cout << "Linear solver Check || I - RA || < 1: " << g << " < 1 ?" << std::endl;   // as :329
     if (rc == VCPHIP_ERR_VERIFICATION)

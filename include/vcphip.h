@@ -46,7 +46,8 @@ enum {
     VCPHIP_ERR_VERIFICATION = -6, /* vcp::verification_error (imats.hpp:333-336) */
     VCPHIP_ERR_UNSUPPORTED = -7,  /* size or form outside what the kernels cover */
     VCPHIP_ERR_RANGE = -8,        /* exponent range outside the supported one    */
-    VCPHIP_ERR_IO = -9            /* vcp::io_error (vcp/vcp_portable_fio.hpp)    */
+    VCPHIP_ERR_IO = -9,           /* vcp::io_error (vcp/vcp_portable_fio.hpp)    */
+    VCPHIP_ERR_SINGULAR = -10     /* vcp::lapack_error (pdblas.hpp:191-203)      */
 };
 
 /* forms of the interval x interval product */
@@ -179,18 +180,35 @@ int vcphip_residual_batched_dev(vcphip_ctx* ctx, int count, const int* n,
                                 const double* const* IA, const int* lda,
                                 double* const* C, const int* ldc, double* normub_dev);
 
+/* ---- approximate inverse of the midpoint matrix ---------------------------- */
+/* R = inv(mid(IA)), the step of imats::linearsolve before the verified ones
+ * (vcp/imats.hpp:298-301: imid, then pdblas::inv, vcp/pdblas.hpp:180-204):
+ * each entry's midpoint as kv's mid() (round to nearest), then LU with
+ * partial pivoting and the inverse from it (rocSOLVER dgetrf, then dgetrs
+ * against the identity: the device counterpart of the reference's LAPACK
+ * dgetrf + dgetri, which on the device is the faster route to an inverse of
+ * the same accuracy).  R is only approximate; nothing verified depends on
+ * its accuracy.  IA is n x n
+ * intervals (AoS {inf, sup}), R n x n doubles.  Returns VCPHIP_ERR_SINGULAR
+ * when a pivot is exactly zero (LAPACK's info > 0; R is then unspecified).
+ * Both forms synchronise the context's stream before returning. */
+int vcphip_inv_mid(vcphip_ctx* ctx, int n, const double* IA, int lda, double* R, int ldr);
+int vcphip_inv_mid_dev(vcphip_ctx* ctx, int n, const double* IA, int lda, double* R, int ldr);
+
 /* ---- verified linear solve: X contains A^-1 b ----------------------------- */
-/* Replaces imats::linearsolve (vcp/imats.hpp:292-362) after its LAPACK step:
- * R is the caller's point approximate inverse of mid(IA) (the reference
- * forms it with imid + pdblas::inv at :298-301).  With mx = R*mid(Ib)
+/* Replaces imats::linearsolve (vcp/imats.hpp:292-362).  R is a point
+ * approximate inverse of mid(IA); R = NULL has the library form it on the
+ * device as vcphip_inv_mid does (the reference's step at :298-301), else it
+ * is the caller's (ldr is then ignored when R is NULL).  With mx = R*mid(Ib)
  * (nearest rounding), y = R*(IA*mx - Ib) and g >= ||R*IA - I||_inf, it
  * writes X = mx + [-u, u] with u = |y| + |R*IA - I| * (||y||_inf / (1 - g)),
  * every step in the reference's order and rounding, so X contains the
  * solution of A x = b for every A in IA and b in Ib.  IA is n x n, Ib and X
  * are n x nrhs intervals (AoS {inf, sup}), R is n x n.  *normub (may be NULL)
  * receives g.  Returns VCPHIP_ERR_VERIFICATION when g >= 1 or is NaN
- * (imats.hpp:330-333; X is then unspecified) and VCPHIP_ERR_DIMENSION for
- * n < 2 (1 x 1 systems are the reference's scalar case, left to the host).
+ * (imats.hpp:330-333; X is then unspecified), VCPHIP_ERR_SINGULAR when the
+ * library's own R met a zero pivot (X unspecified) and VCPHIP_ERR_DIMENSION
+ * for n < 2 (1 x 1 systems are the reference's scalar case, left to the host).
  * Both forms synchronise the context's stream before returning. */
 int vcphip_lss(vcphip_ctx* ctx, int n, int nrhs, const double* IA, int lda, const double* Ib, int ldb,
                const double* R, int ldr, double* X, int ldx, double* normub);
@@ -202,8 +220,11 @@ int vcphip_lss_dev(vcphip_ctx* ctx, int n, int nrhs, const double* IA, int lda, 
  * is reused while the problem set is unchanged); every other step is one
  * launch over all problems.  Pointer arrays are host arrays (of device
  * pointers for the _dev form); normub is a host array of count doubles.
- * Returns VCPHIP_ERR_VERIFICATION when any normub[p] >= 1 (the other
- * problems' X are valid; a failed problem's X is NaN). */
+ * R (and ldr) may be NULL, or R[p] NULL for some p: those R_p = inv(mid(IA_p))
+ * are formed on the device first, one rocSOLVER batched getrf + getrs per
+ * group of equal order (orders bordered by the identity to a multiple of 64).  Returns VCPHIP_ERR_SINGULAR when any such R_p met a
+ * zero pivot, else VCPHIP_ERR_VERIFICATION when any normub[p] >= 1 (the
+ * other problems' X are valid; a failed problem's X is NaN). */
 int vcphip_lss_batched(vcphip_ctx* ctx, int count, const int* n, const int* nrhs,
                        const double* const* IA, const int* lda, const double* const* Ib, const int* ldb,
                        const double* const* R, const int* ldr, double* const* X, const int* ldx,

@@ -43,7 +43,7 @@ def test_every_declared_symbol_is_exported():
 
 def test_strerror_names_every_code():
     L = vcp_amd.lib()
-    for code in range(0, -10, -1):
+    for code in range(0, -11, -1):
         assert L.vcphip_strerror(code) not in (None, b"", b"unknown error")
     assert L.vcphip_strerror(-99) == b"unknown error"
 
@@ -58,6 +58,8 @@ def test_null_context_is_rejected_without_device_work():
     assert L.vcphip_dd_residual(None, 4, 4, 4, None, 4, None, 4, None, 4, None, 4) == -2
     assert L.vcphip_lss_batched(None, 1, None, None, None, None, None, None, None, None, None, None, None) == -2
     assert L.vcphip_lss_batched_dev(None, 1, None, None, None, None, None, None, None, None, None, None, None) == -2
+    assert L.vcphip_inv_mid(None, 4, None, 4, None, 4) == -2
+    assert L.vcphip_inv_mid_dev(None, 4, None, 4, None, 4) == -2
     assert L.vcphip_ctx_create(0, None) == -2
 
 

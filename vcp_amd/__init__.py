@@ -31,7 +31,7 @@ PARAM_DD_NMOD = 4
 
 ERRORS = {
     -1: "dimension", -2: "argument", -3: "device", -4: "oom", -5: "rounding",
-    -6: "verification", -7: "unsupported", -8: "range",
+    -6: "verification", -7: "unsupported", -8: "range", -9: "io", -10: "singular",
 }
 
 _c_int = ctypes.c_int
@@ -67,6 +67,8 @@ SIGNATURES = {
     "vcphip_ddgemm": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_ddgemm_dev": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_ddgemm_tn": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int],
+    "vcphip_inv_mid": [_vp, _c_int, _vp, _c_int, _vp, _c_int],
+    "vcphip_inv_mid_dev": [_vp, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_lss": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _p_dbl],
     "vcphip_lss_dev": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _p_dbl],
     "vcphip_lss_batched": [_vp, _c_int, _p_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _p_dbl],
@@ -468,24 +470,47 @@ class Context:
                "vcphip_residual_batched")
         return Cs, list(nub)
 
+    def inv_mid(self, IA):
+        """R = inv(mid(IA)) formed on the device (imid + pdblas::inv,
+        vcp/imats.hpp:298-301, vcp/pdblas.hpp:180-204): kv's midpoints, then
+        rocSOLVER dgetrf and dgetrs against the identity.  Returns R (n, n), column-major; raises
+        VcphipError (VCPHIP_ERR_SINGULAR) on an exactly zero pivot."""
+        n = IA.shape[0]
+        if IA.shape[1] != n:
+            raise ValueError("inv_mid: matrix must be square")
+        R = np.zeros((n, n), dtype=np.float64, order="F")
+        if n == 0:
+            return R
+        pa, lda = _pair_ptr(IA)
+        pr, ldr = _mat_ptr(R)
+        _check(lib().vcphip_inv_mid(self._h, n, pa, lda, pr, ldr), "vcphip_inv_mid")
+        return R
+
+    def inv_mid_dev(self, n, dA, lda, dR, ldr):
+        """device-pointer form of inv_mid()"""
+        _check(lib().vcphip_inv_mid_dev(self._h, n, _vp(dA), lda, _vp(dR), ldr), "vcphip_inv_mid_dev")
+
     def lss(self, IA, Ib, R=None):
         """Verified solution of A x = b (imats::linearsolve, vcp/imats.hpp:
         292-362): returns (X, g), X (n, nrhs, 2) containing A^-1 b for every
-        A in IA and b in Ib, g >= ||R IA - I||_inf.  R defaults to the host
-        LAPACK inverse of mid(IA) (numpy), the reference's step at :298-301.
+        A in IA and b in Ib, g >= ||R IA - I||_inf.  R = None has the library
+        form R = inv(mid(IA)) on the device (as inv_mid), the reference's
+        step at :298-301; else R is the caller's approximate inverse.
         Raises VcphipError (VCPHIP_ERR_VERIFICATION) when g >= 1, as the
-        reference throws vcp::verification_error."""
+        reference throws vcp::verification_error, and VCPHIP_ERR_SINGULAR
+        when the device R meets a zero pivot (its lapack_error)."""
         n = IA.shape[0]
         if IA.shape[1] != n or Ib.shape[0] != n:
             raise ValueError("dimension mismatch")
-        if R is None:
-            R = np.linalg.inv(0.5 * IA[..., 0] + 0.5 * IA[..., 1])
-        R = np.asfortranarray(R, dtype=np.float64)
         nrhs = Ib.shape[1]
         X = pairs(n, nrhs)
         pa, lda = _pair_ptr(IA)
         pb, ldb = _pair_ptr(Ib)
-        pr, ldr = _mat_ptr(R)
+        if R is None:
+            pr, ldr = None, n
+        else:
+            R = np.asfortranarray(R, dtype=np.float64)
+            pr, ldr = _mat_ptr(R)
         px, ldx = _pair_ptr(X)
         g = ctypes.c_double()
         _check(lib().vcphip_lss(self._h, n, nrhs, pa, lda, pb, ldb, pr, ldr, px, ldx, ctypes.byref(g)), "vcphip_lss")
@@ -494,32 +519,37 @@ class Context:
     def lss_batched(self, IAs, Ibs, Rs=None):
         """vcphip_lss for many systems of uneven size in one call: returns
         ([X_p], [g_p]); raises VcphipError (VCPHIP_ERR_VERIFICATION) when
-        any g_p >= 1."""
+        any g_p >= 1.  Rs = None (or a None entry) has the library form
+        those R_p = inv(mid(IA_p)) on the device."""
         cnt = len(IAs)
         if Rs is None:
-            Rs = [np.linalg.inv(0.5 * A[..., 0] + 0.5 * A[..., 1]) for A in IAs]
-        Rs = [np.asfortranarray(R, dtype=np.float64) for R in Rs]
+            Rs = [None] * cnt
+        Rs = [None if R is None else np.asfortranarray(R, dtype=np.float64) for R in Rs]
         Xs = [pairs(b.shape[0], b.shape[1]) for b in Ibs]
         ia = lambda xs: (ctypes.c_int * cnt)(*xs)
         vp = lambda xs: (_vp * cnt)(*xs)
         g = (ctypes.c_double * cnt)()
+        rp = [(None, A.shape[0]) if R is None else _mat_ptr(R) for R, A in zip(Rs, IAs)]
         _check(lib().vcphip_lss_batched(self._h, cnt, ia([A.shape[0] for A in IAs]), ia([b.shape[1] for b in Ibs]),
                                         vp([_pair_ptr(A)[0] for A in IAs]), ia([_pair_ptr(A)[1] for A in IAs]),
                                         vp([_pair_ptr(b)[0] for b in Ibs]), ia([_pair_ptr(b)[1] for b in Ibs]),
-                                        vp([_mat_ptr(R)[0] for R in Rs]), ia([_mat_ptr(R)[1] for R in Rs]),
+                                        vp([r[0] for r in rp]), ia([r[1] for r in rp]),
                                         vp([_pair_ptr(X)[0] for X in Xs]), ia([_pair_ptr(X)[1] for X in Xs]), g),
                "vcphip_lss_batched")
         return Xs, list(g)
 
     def lss_batched_dev(self, n, nrhs, dA, lda, dB, ldb, dR, ldr, dX, ldx):
         """device-pointer form: lists of sizes, pointers and leading
-        dimensions; returns [g_p] (raises when any g_p >= 1)"""
+        dimensions; returns [g_p] (raises when any g_p >= 1).  dR (with
+        ldr) may be None, or hold None entries: those R_p are formed on the
+        device."""
         cnt = len(n)
         ia = lambda xs: (ctypes.c_int * cnt)(*xs)
         vp = lambda xs: (_vp * cnt)(*[_vp(x) for x in xs])
         g = (ctypes.c_double * cnt)()
-        _check(lib().vcphip_lss_batched_dev(self._h, cnt, ia(n), ia(nrhs), vp(dA), ia(lda), vp(dB), ia(ldb), vp(dR),
-                                            ia(ldr), vp(dX), ia(ldx), g), "vcphip_lss_batched_dev")
+        _check(lib().vcphip_lss_batched_dev(self._h, cnt, ia(n), ia(nrhs), vp(dA), ia(lda), vp(dB), ia(ldb),
+                                            None if dR is None else vp(dR), None if ldr is None else ia(ldr), vp(dX),
+                                            ia(ldx), g), "vcphip_lss_batched_dev")
         return list(g)
 
     # ---- device-pointer products (torch tensors or raw pointers) -----------

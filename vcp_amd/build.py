@@ -24,6 +24,8 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
+# the approximate inverse of the verified solve: rocSOLVER getrf/getrs
+LIBS = ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocsolver", "-lrocblas"]
 
 
 def sources():
@@ -33,7 +35,7 @@ def sources():
 def build(verbose=False):
     """Compile every HIP source of the library into vcp_amd/libvcphip.so."""
     tmp = LIB + ".tmp"
-    cmd = [HIPCC] + FLAGS + sources() + ["-o", tmp]
+    cmd = [HIPCC] + FLAGS + sources() + LIBS + ["-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

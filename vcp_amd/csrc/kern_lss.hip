@@ -19,9 +19,10 @@ namespace {
 constexpr int NT = 256;
 
 // kv's mid(): the nearest-rounded average, (lo + hi) / 2, or lo/2 + hi/2
-// where lo + hi overflows; point intervals give lo.
+// where lo + hi overflows; point intervals give lo.  mb has leading
+// dimension ldm.
 __global__ __launch_bounds__(NT) void k_lss_mid(int n, int ncol, const double* __restrict__ b, long ldb,
-                                                double* __restrict__ mb) {
+                                                double* __restrict__ mb, long ldm) {
   const long idx = (long)blockIdx.x * NT + threadIdx.x;
   if (idx >= (long)n * ncol) return;
   const int i = (int)(idx % n), j = (int)(idx / n);
@@ -33,7 +34,41 @@ __global__ __launch_bounds__(NT) void k_lss_mid(int n, int ncol, const double* _
     const double s = v.x + v.y;
     m = __builtin_isinf(s) ? 0.5 * v.x + 0.5 * v.y : s * 0.5;
   }
-  mb[idx] = m;
+  mb[i + ldm * j] = m;
+}
+
+// The midpoint matrix M of IA (n x n, as k_lss_mid) bordered by the identity
+// to order npad: P = [[M, 0], [0, I]], whose inverse is [[inv(M), 0], [0, I]].
+// In LU with partial pivoting the border rows keep zeros in M's columns
+// (their multipliers are 0), so no pivot is taken from them and a singular M
+// still meets a zero pivot.  n = 0 writes the identity (IA is not read).
+__global__ __launch_bounds__(NT) void k_inv_pad(int n, int npad, const double* __restrict__ IA, long lda,
+                                                double* __restrict__ P) {
+  const long idx = (long)blockIdx.x * NT + threadIdx.x;
+  if (idx >= (long)npad * npad) return;
+  const int i = (int)(idx % npad), j = (int)(idx / npad);
+  double m;
+  if (i < n && j < n) {
+    const double2 v = *reinterpret_cast<const double2*>(IA + 2 * (i + lda * j));
+    if (v.x == v.y) {
+      m = v.x;
+    } else {
+      const double s = v.x + v.y;
+      m = __builtin_isinf(s) ? 0.5 * v.x + 0.5 * v.y : s * 0.5;
+    }
+  } else {
+    m = (i == j) ? 1.0 : 0.0;
+  }
+  P[idx] = m;
+}
+
+// R(0:n, 0:n) = P(0:n, 0:n)
+__global__ __launch_bounds__(NT) void k_copy_block(int n, const double* __restrict__ P, long ldp,
+                                                   double* __restrict__ R, long ldr) {
+  const long idx = (long)blockIdx.x * NT + threadIdx.x;
+  if (idx >= (long)n * n) return;
+  const int i = (int)(idx % n), j = (int)(idx / n);
+  R[i + ldr * j] = P[i + ldp * j];
 }
 
 // C = A * B in round-to-nearest, each element the k-ascending FMA chain
@@ -450,8 +485,18 @@ __global__ __launch_bounds__(NT) void k_lssb_final(int count, const long* __rest
 
 }  // namespace
 
-int vcphip_lss_launch_mid(int n, int ncol, const double* b, long ldb, double* mb, hipStream_t s) {
-  hipLaunchKernelGGL(k_lss_mid, grid_of((long)n * ncol), dim3(NT), 0, s, n, ncol, b, ldb, mb);
+int vcphip_lss_launch_mid(int n, int ncol, const double* b, long ldb, double* mb, long ldm, hipStream_t s) {
+  hipLaunchKernelGGL(k_lss_mid, grid_of((long)n * ncol), dim3(NT), 0, s, n, ncol, b, ldb, mb, ldm);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_inv_launch_pad(int n, int npad, const double* IA, long lda, double* P, hipStream_t s) {
+  hipLaunchKernelGGL(k_inv_pad, grid_of((long)npad * npad), dim3(NT), 0, s, n, npad, IA, lda, P);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_inv_launch_copy(int n, const double* P, long ldp, double* R, long ldr, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_block, grid_of((long)n * n), dim3(NT), 0, s, n, P, ldp, R, ldr);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

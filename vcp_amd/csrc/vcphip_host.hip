@@ -14,8 +14,17 @@
 #include <string>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include "../../include/vcphip.h"
 #include "vcphip_kernels.h"
+
+#define VCPHIP_TRY(x)          \
+  do {                         \
+    const int _rc = (x);       \
+    if (_rc) return _rc;       \
+  } while (0)
 
 struct vcphip_ctx {
   int device = 0;
@@ -23,7 +32,8 @@ struct vcphip_ctx {
   hipStream_t stream = nullptr;
   long long oz_nmod = 0, oz_variant = 0, timing = 0, dd_nmod = 0;
   // workspace slots (device memory, grown on demand)
-  enum { WS_PA, WS_PB, WS_U, WS_Z, WS_SA, WS_SB, WS_HA, WS_HB, WS_HC, WS_HC2, WS_BATCH, WS_DD, WS_LSS, WS_HR, WS_LSSB, WS_N };
+  enum { WS_PA, WS_PB, WS_U, WS_Z, WS_SA, WS_SB, WS_HA, WS_HB, WS_HC, WS_HC2, WS_BATCH, WS_DD, WS_LSS, WS_HR, WS_LSSB,
+         WS_INV, WS_N };
   void* ws[WS_N] = {};
   size_t ws_size[WS_N] = {};
   // stage timing: events recorded by every call since the last synchronize,
@@ -62,6 +72,9 @@ struct vcphip_ctx {
   // side steps run on it beside the residual's persistent launch
   hipStream_t aux = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // rocBLAS handle of the approximate inverses (rocSOLVER getrf/getri),
+  // created on first use
+  rocblas_handle blas = nullptr;
 };
 
 namespace {
@@ -312,6 +325,7 @@ extern "C" const char* vcphip_strerror(int code) {
     case VCPHIP_ERR_UNSUPPORTED: return "unsupported size or form";
     case VCPHIP_ERR_RANGE: return "exponent range not supported";
     case VCPHIP_ERR_IO: return "matrix file I/O or format error";
+    case VCPHIP_ERR_SINGULAR: return "midpoint matrix is singular (zero pivot)";
     default: return "unknown error";
   }
 }
@@ -354,6 +368,7 @@ extern "C" int vcphip_ctx_destroy(vcphip_ctx* c) {
     hipEventDestroy(c->ev_fork);
     hipEventDestroy(c->ev_join);
   }
+  if (c->blas) rocblas_destroy_handle(c->blas);
   hipStreamDestroy(c->own);
   delete c;
   return VCPHIP_OK;
@@ -769,19 +784,147 @@ extern "C" int vcphip_residual_batched_dev(vcphip_ctx* c, int count, const int* 
 }
 
 // ===========================================================================
+// approximate inverse of the midpoint matrix: imid + pdblas::inv
+// (vcp/imats.hpp:298-301, vcp/pdblas.hpp:180-204)
+// ===========================================================================
+namespace {
+
+// orders are bordered up to a multiple of this, so that problems of nearby
+// order share one batched rocSOLVER call
+constexpr int INV_BORDER = 64;
+
+inline int inv_order(int n) { return (n + INV_BORDER - 1) / INV_BORDER * INV_BORDER; }
+
+struct InvJob {
+  int n;
+  const double* IA;
+  long lda;
+  double* P;   // 2 npad^2 doubles: the LU factors (leading dimension npad), then the inverse
+  int npad;    // inv_order(n)
+  double* R;   // if not null: receives the leading n x n block of the inverse
+  long ldr;
+};
+
+inline size_t inv_work_doubles(int n) { return 2 * (size_t)inv_order(n) * inv_order(n); }
+inline double* inv_result(const InvJob& j) { return j.P + (size_t)j.npad * j.npad; }
+
+struct InvBatch {
+  std::vector<int> ord;          // jobs sorted by npad
+  std::vector<double*> hptr;     // LU then inverse pointers in that order; the source of an async copy
+  rocblas_int* info = nullptr;   // device: getrf's info per sorted position
+};
+
+// inv(mid(IA_j)) for every job, enqueued on the context stream: the midpoint
+// matrix (kv's mid()) bordered by the identity to order npad (k_inv_pad),
+// LU with partial pivoting (rocSOLVER dgetrf), then the inverse as the
+// solution of L U X = I (dgetrs; LAPACK's dgetrf + dgetri in pdblas::inv
+// compute the same inverse, and getrs is the faster route on the device),
+// one batched call per padded order.  inv(M) is the leading n x n block of
+// inv_result(job).  The caller synchronises the stream before `b` goes out
+// of scope.
+int inv_mid_enqueue(vcphip_ctx* c, const std::vector<InvJob>& jobs, InvBatch& b) {
+  const int cnt = (int)jobs.size();
+  hipStream_t s = c->stream;
+  if (!c->blas && rocblas_create_handle(&c->blas) != rocblas_status_success) {
+    c->blas = nullptr;
+    return VCPHIP_ERR_DEVICE;
+  }
+  if (rocblas_set_stream(c->blas, s) != rocblas_status_success) return VCPHIP_ERR_DEVICE;
+  b.ord.resize(cnt);
+  for (int i = 0; i < cnt; ++i) b.ord[i] = i;
+  std::stable_sort(b.ord.begin(), b.ord.end(), [&](int x, int y) { return jobs[x].npad < jobs[y].npad; });
+  size_t npiv = 0;
+  for (const auto& j : jobs) npiv += (size_t)j.npad;
+  auto al = [](size_t bytes) { return (bytes + 255) & ~(size_t)255; };
+  const size_t o_piv = al(sizeof(double*) * 2 * cnt), o_info = o_piv + al(sizeof(rocblas_int) * npiv);
+  void* base;
+  const int wrc = ws_get(c, vcphip_ctx::WS_INV, o_info + al(sizeof(rocblas_int) * cnt), &base);
+  if (wrc) return wrc;
+  char* b8 = (char*)base;
+  double** dptr = (double**)b8;   // LU pointers, then inverse pointers
+  rocblas_int* piv = (rocblas_int*)(b8 + o_piv);
+  b.info = (rocblas_int*)(b8 + o_info);
+  b.hptr.resize(2 * (size_t)cnt);
+  for (int t = 0; t < cnt; ++t) {
+    b.hptr[t] = jobs[b.ord[t]].P;
+    b.hptr[cnt + t] = inv_result(jobs[b.ord[t]]);
+  }
+  if (hipMemcpyAsync(dptr, b.hptr.data(), sizeof(double*) * 2 * cnt, hipMemcpyHostToDevice, s) != hipSuccess)
+    return VCPHIP_ERR_DEVICE;
+  for (const auto& j : jobs)
+    if (vcphip_inv_launch_pad(j.n, j.npad, j.IA, j.lda, j.P, s) ||
+        vcphip_inv_launch_pad(0, j.npad, nullptr, 1, inv_result(j), s))   // the identity
+      return VCPHIP_ERR_DEVICE;
+  size_t poff = 0;
+  for (int t = 0; t < cnt;) {
+    const int np = jobs[b.ord[t]].npad;
+    int e = t + 1;
+    while (e < cnt && jobs[b.ord[e]].npad == np) ++e;
+    const int bc = e - t;
+    // the batched routines also for a group of one: their rounding differs
+    // from the plain ones', and a problem's R must not depend on its group
+    rocblas_status st = rocsolver_dgetrf_batched(c->blas, np, np, dptr + t, np, piv + poff, np, b.info + t, bc);
+    if (st == rocblas_status_success)
+      st = rocsolver_dgetrs_batched(c->blas, rocblas_operation_none, np, np, dptr + t, np, piv + poff, np,
+                                    dptr + cnt + t, np, bc);
+    if (st != rocblas_status_success) return VCPHIP_ERR_DEVICE;
+    poff += (size_t)np * bc;
+    t = e;
+  }
+  for (const auto& j : jobs)
+    if (j.R && vcphip_inv_launch_copy(j.n, inv_result(j), j.npad, j.R, j.ldr, s)) return VCPHIP_ERR_DEVICE;
+  return VCPHIP_OK;
+}
+
+// after the stream has been synchronised: singular[j] = 1 when job j met an
+// exactly zero pivot (LAPACK's info > 0, the reference's lapack_error)
+int inv_mid_singular(vcphip_ctx* c, const InvBatch& b, std::vector<char>& singular) {
+  const int cnt = (int)b.ord.size();
+  std::vector<rocblas_int> info((size_t)cnt);
+  if (hipMemcpy(info.data(), b.info, sizeof(rocblas_int) * cnt, hipMemcpyDeviceToHost) != hipSuccess)
+    return VCPHIP_ERR_DEVICE;
+  singular.assign(cnt, 0);
+  for (int t = 0; t < cnt; ++t) singular[b.ord[t]] = info[t] != 0 ? 1 : 0;
+  return VCPHIP_OK;
+}
+
+}  // namespace
+
+extern "C" int vcphip_inv_mid_dev(vcphip_ctx* c, int n, const double* IA, int lda, double* R, int ldr) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (n < 0 || lda < (n > 1 ? n : 1) || ldr < (n > 1 ? n : 1)) return VCPHIP_ERR_DIMENSION;
+  if (n == 0) return VCPHIP_OK;
+  if (!IA || !R) return VCPHIP_ERR_ARGUMENT;
+  hipSetDevice(c->device);
+  const int np = inv_order(n);
+  void* pw;
+  VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_LSS, sizeof(double) * inv_work_doubles(n), &pw));
+  StageTimer T(c);
+  InvBatch b;
+  const int rc = inv_mid_enqueue(c, {InvJob{n, IA, lda, (double*)pw, np, R, ldr}}, b);
+  T.mark("inv_mid");
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return VCPHIP_ERR_DEVICE;
+  if (rc) return rc;
+  std::vector<char> singular;
+  VCPHIP_TRY(inv_mid_singular(c, b, singular));
+  return singular[0] ? VCPHIP_ERR_SINGULAR : VCPHIP_OK;
+}
+
+// ===========================================================================
 // verified linear solve (imats::linearsolve, vcp/imats.hpp:292-362)
 // ===========================================================================
 extern "C" int vcphip_lss_dev(vcphip_ctx* c, int n, int nrhs, const double* IA, int lda, const double* Ib, int ldb,
                               const double* R, int ldr, double* X, int ldx, double* normub) {
-  if (!c || !IA || !Ib || !R || !X) return VCPHIP_ERR_ARGUMENT;
-  if (n < 2 || nrhs < 1 || lda < n || ldb < n || ldr < n || ldx < n) return VCPHIP_ERR_DIMENSION;
+  if (!c || !IA || !Ib || !X) return VCPHIP_ERR_ARGUMENT;
+  if (n < 2 || nrhs < 1 || lda < n || ldb < n || (R && ldr < n) || ldx < n) return VCPHIP_ERR_DIMENSION;
   hipSetDevice(c->device);
   hipStream_t s = c->stream;
   const size_t nr = (size_t)n * nrhs, al = 32;   // 256-byte alignment in doubles
   auto up = [&](size_t x) { return (x + al - 1) / al * al; };
   const size_t o_mb = 0, o_mx = o_mb + up(nr), o_x = o_mx + up(nr), o_y = o_x + up(2 * nr), o_t = o_y + up(2 * nr);
   const size_t o_w = o_t + up(2 * nr), o_c = o_w + up(2 * nr), o_r = o_c + up(2 * (size_t)n * n);
-  const size_t o_s = o_r + up((size_t)n), total = o_s + al;
+  const size_t np = (size_t)inv_order(n);
+  const size_t o_s = o_r + up((size_t)n), o_ri = o_s + al, total = o_ri + (R ? 0 : up(inv_work_doubles(n)));
   void* base;
   const int wrc = ws_get(c, vcphip_ctx::WS_LSS, total * sizeof(double), &base);
   if (wrc) return wrc;
@@ -789,9 +932,22 @@ extern "C" int vcphip_lss_dev(vcphip_ctx* c, int n, int nrhs, const double* IA, 
   double *mb = w + o_mb, *mx = w + o_mx, *Xi = w + o_x, *Y = w + o_y, *Tm = w + o_t, *W = w + o_w, *C = w + o_c;
   double *rows = w + o_r, *norms = w + o_s;   // norms: g, T, G lower, G upper
   StageTimer T(c);
+  // R = inv(mid(IA)) on the device when the caller gives none (:298-301)
+  InvBatch inv;
+  if (!R) {
+    const InvJob job{n, IA, lda, w + o_ri, (int)np, nullptr, 0};
+    const int irc = inv_mid_enqueue(c, {job}, inv);
+    if (irc) {
+      hipStreamSynchronize(s);
+      return irc;
+    }
+    R = inv_result(job);   // inv(mid(IA)) is the leading n x n block of the bordered inverse
+    ldr = (int)np;
+    T.mark("lss_inv");
+  }
   int rc = 0;
   // mx = R * mid(b), nearest rounding (:299-302)
-  rc |= vcphip_lss_launch_mid(n, nrhs, Ib, ldb, mb, s);
+  rc |= vcphip_lss_launch_mid(n, nrhs, Ib, ldb, mb, n, s);
   rc |= vcphip_lss_launch_gemv_rn(n, nrhs, n, R, ldr, mb, n, mx, n, s);
   // y = R * (IA*mx - b) and |y| (:305-315)
   rc |= vcphip_launch_drgemm(VCPHIP_DR_IP, n, nrhs, n, IA, lda, mx, n, Xi, n, nullptr, 0, s);
@@ -812,12 +968,20 @@ extern "C" int vcphip_lss_dev(vcphip_ctx* c, int n, int nrhs, const double* IA, 
   // X = mx + [-u, u], u = |y| + W (:352-361)
   rc |= vcphip_lss_launch_final(n, nrhs, Y, W, mx, X, ldx, s);
   T.mark("lss_bound");
-  if (rc) return VCPHIP_ERR_DEVICE;
+  if (rc) {
+    hipStreamSynchronize(s);
+    return VCPHIP_ERR_DEVICE;
+  }
   double g = 0.0;
   if (hipMemcpyAsync(&g, norms, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return VCPHIP_ERR_DEVICE;
   if (normub) *normub = g;
+  if (!inv.ord.empty()) {
+    std::vector<char> singular;
+    VCPHIP_TRY(inv_mid_singular(c, inv, singular));
+    if (singular[0]) return VCPHIP_ERR_SINGULAR;
+  }
   return (g < 1.0) ? VCPHIP_OK : VCPHIP_ERR_VERIFICATION;
 }
 
@@ -829,13 +993,16 @@ extern "C" int vcphip_lss_batched_dev(vcphip_ctx* c, int count, const int* n, co
                                       const int* ldb, const double* const* R, const int* ldr, double* const* X,
                                       const int* ldx, double* normub) {
   if (!c) return VCPHIP_ERR_ARGUMENT;
-  if (count < 0 || (count > 0 && (!n || !nrhs || !IA || !lda || !Ib || !ldb || !R || !ldr || !X || !ldx || !normub)))
+  if (count < 0 || (count > 0 && (!n || !nrhs || !IA || !lda || !Ib || !ldb || !X || !ldx || !normub)))
     return VCPHIP_ERR_ARGUMENT;
   if (count == 0) return VCPHIP_OK;
+  // R (the array) or R[p] may be NULL: that problem's R = inv(mid(IA_p)) is
+  // formed on the device (imats.hpp:298-301)
+  auto has_r = [&](int p) { return R && R[p]; };
   for (int p = 0; p < count; ++p) {
     const int np = n[p];
-    if (!IA[p] || !Ib[p] || !R[p] || !X[p]) return VCPHIP_ERR_ARGUMENT;
-    if (np < 2 || nrhs[p] < 1 || lda[p] < np || ldb[p] < np || ldr[p] < np || ldx[p] < np)
+    if (!IA[p] || !Ib[p] || !X[p]) return VCPHIP_ERR_ARGUMENT;
+    if (np < 2 || nrhs[p] < 1 || lda[p] < np || ldb[p] < np || (has_r(p) && (!ldr || ldr[p] < np)) || ldx[p] < np)
       return VCPHIP_ERR_DIMENSION;
   }
   hipSetDevice(c->device);
@@ -849,10 +1016,15 @@ extern "C" int vcphip_lss_batched_dev(vcphip_ctx* c, int count, const int* n, co
   size_t off = o_nm + al(sizeof(double) * 4 * (size_t)count);
   std::vector<size_t> o_prob((size_t)count);
   std::vector<long> pb((size_t)count + 1, 0);
+  std::vector<size_t> o_rinv((size_t)count, 0);
   for (int p = 0; p < count; ++p) {
     const size_t nr = (size_t)n[p] * nrhs[p];
     o_prob[p] = off;
     off += al(8 * nr) * 2 + al(16 * nr) * 2 + al(16 * (size_t)n[p] * n[p]);
+    if (!has_r(p)) {
+      o_rinv[p] = off;
+      off += al(8 * inv_work_doubles(n[p]));
+    }
     pb[p + 1] = pb[p] + (long)nr;
   }
   void* base;
@@ -862,18 +1034,31 @@ extern "C" int vcphip_lss_batched_dev(vcphip_ctx* c, int count, const int* n, co
   double* gdev = (double*)(b8 + o_g);
   std::vector<VcphipLssProblem> probs((size_t)count);
   std::vector<double*> Cp((size_t)count);
-  std::vector<int> ldc((size_t)count);
+  std::vector<int> ldc((size_t)count), ldr_use((size_t)count);
+  std::vector<const double*> R_use((size_t)count);
+  std::vector<InvJob> jobs;
+  for (int p = 0; p < count; ++p) {
+    if (has_r(p)) {
+      R_use[p] = R[p];
+      ldr_use[p] = ldr[p];
+    } else {
+      const InvJob job{n[p], IA[p], lda[p], (double*)(b8 + o_rinv[p]), inv_order(n[p]), nullptr, 0};
+      R_use[p] = inv_result(job);   // the leading n x n block of the bordered inverse
+      ldr_use[p] = job.npad;
+      jobs.push_back(job);
+    }
+  }
   for (int p = 0; p < count; ++p) {
     const size_t nr = (size_t)n[p] * nrhs[p];
     char* q = b8 + o_prob[p];
     VcphipLssProblem& P = probs[p];
     P.A = IA[p];
     P.b = Ib[p];
-    P.R = R[p];
+    P.R = R_use[p];
     P.X = X[p];
     P.lda = lda[p];
     P.ldb = ldb[p];
-    P.ldr = ldr[p];
+    P.ldr = ldr_use[p];
     P.ldx = ldx[p];
     P.n = n[p];
     P.nrhs = nrhs[p];
@@ -905,13 +1090,21 @@ extern "C" int vcphip_lss_batched_dev(vcphip_ctx* c, int count, const int* n, co
   }
   const long* dpb = (const long*)(b8 + o_pb);
   const VcphipLssProblem* dprobs = (const VcphipLssProblem*)b8;
+  // the missing approximate inverses first, on the context stream
+  InvBatch inv;
+  if (!jobs.empty()) {
+    StageTimer T(c);
+    const int irc = inv_mid_enqueue(c, jobs, inv);
+    if (irc) return fail(irc);
+    T.mark("lss_inv");
+  }
   // right-hand side on the second stream, after the descriptor upload
   if (hipEventRecord(c->ev_fork, s) != hipSuccess || hipStreamWaitEvent(c->aux, c->ev_fork, 0) != hipSuccess ||
       vcphip_lss_launch_batched_rhs(count, pb[count], dpb, dprobs, c->aux) ||
       hipEventRecord(c->ev_join, c->aux) != hipSuccess)
     return fail(VCPHIP_ERR_DEVICE);
   // R_p IA_p - I and g_p (imats.hpp:318-329) meanwhile on the context stream
-  int rc = vcphip_residual_batched_dev(c, count, n, R, ldr, IA, lda, Cp.data(), ldc.data(), gdev);
+  int rc = vcphip_residual_batched_dev(c, count, n, R_use.data(), ldr_use.data(), IA, lda, Cp.data(), ldc.data(), gdev);
   if (rc) return fail(rc);
   StageTimer T(c);
   if (hipStreamWaitEvent(s, c->ev_join, 0) != hipSuccess ||
@@ -921,6 +1114,12 @@ extern "C" int vcphip_lss_batched_dev(vcphip_ctx* c, int count, const int* n, co
   if (hipMemcpyAsync(normub, gdev, sizeof(double) * count, hipMemcpyDeviceToHost, s) != hipSuccess)
     return fail(VCPHIP_ERR_DEVICE);
   if (hipStreamSynchronize(s) != hipSuccess) return VCPHIP_ERR_DEVICE;
+  if (!jobs.empty()) {
+    std::vector<char> singular;
+    VCPHIP_TRY(inv_mid_singular(c, inv, singular));
+    for (size_t j = 0; j < jobs.size(); ++j)
+      if (singular[j]) return VCPHIP_ERR_SINGULAR;
+  }
   for (int p = 0; p < count; ++p)
     if (!(normub[p] < 1.0)) return VCPHIP_ERR_VERIFICATION;
   return VCPHIP_OK;
@@ -971,12 +1170,6 @@ int to_host(vcphip_ctx* c, const double* d, double* h, int ld, int rows, int col
 }
 
 }  // namespace
-
-#define VCPHIP_TRY(x)          \
-  do {                         \
-    const int _rc = (x);       \
-    if (_rc) return _rc;       \
-  } while (0)
 
 extern "C" int vcphip_igemm(vcphip_ctx* c, int form, int m, int n, int k, const double* IA, int lda, const double* IB,
                             int ldb, double* C, int ldc) {
@@ -1144,32 +1337,49 @@ extern "C" int vcphip_ddgemm_tn(vcphip_ctx* c, int k, int n, const double* A, in
 
 extern "C" int vcphip_lss(vcphip_ctx* c, int n, int nrhs, const double* IA, int lda, const double* Ib, int ldb,
                           const double* R, int ldr, double* X, int ldx, double* normub) {
-  if (!c || !IA || !Ib || !R || !X) return VCPHIP_ERR_ARGUMENT;
-  if (n < 2 || nrhs < 1 || lda < n || ldb < n || ldr < n || ldx < n) return VCPHIP_ERR_DIMENSION;
+  if (!c || !IA || !Ib || !X) return VCPHIP_ERR_ARGUMENT;
+  if (n < 2 || nrhs < 1 || lda < n || ldb < n || (R && ldr < n) || ldx < n) return VCPHIP_ERR_DIMENSION;
   hipSetDevice(c->device);
-  double *dA, *dB, *dR, *dX;
+  double *dA, *dB, *dR = nullptr, *dX;
   VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HA, IA, lda, n, n, 16, &dA));
   VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HB, Ib, ldb, n, nrhs, 16, &dB));
-  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HR, R, ldr, n, n, 8, &dR));
+  if (R) VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HR, R, ldr, n, n, 8, &dR));
   void* px;
   VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HC2, 16 * (size_t)n * nrhs, &px));
   dX = (double*)px;
   const int rc = vcphip_lss_dev(c, n, nrhs, dA, n, dB, n, dR, n, dX, n, normub);
-  if (rc != VCPHIP_OK && rc != VCPHIP_ERR_VERIFICATION) return rc;
+  if (rc != VCPHIP_OK && rc != VCPHIP_ERR_VERIFICATION && rc != VCPHIP_ERR_SINGULAR) return rc;
   VCPHIP_TRY(to_host(c, dX, X, ldx, n, nrhs, 16));
   return rc;
+}
+
+extern "C" int vcphip_inv_mid(vcphip_ctx* c, int n, const double* IA, int lda, double* R, int ldr) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (n < 0 || lda < (n > 1 ? n : 1) || ldr < (n > 1 ? n : 1)) return VCPHIP_ERR_DIMENSION;
+  if (n == 0) return VCPHIP_OK;
+  if (!IA || !R) return VCPHIP_ERR_ARGUMENT;
+  hipSetDevice(c->device);
+  double* dA;
+  void* p;
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HA, IA, lda, n, n, 16, &dA));
+  VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HC, 8 * (size_t)n * n, &p));
+  const int rc = vcphip_inv_mid_dev(c, n, dA, n, (double*)p, n);
+  if (rc) return rc;
+  return to_host(c, (double*)p, R, ldr, n, n, 8);
 }
 
 extern "C" int vcphip_lss_batched(vcphip_ctx* c, int count, const int* n, const int* nrhs, const double* const* IA,
                                   const int* lda, const double* const* Ib, const int* ldb, const double* const* R,
                                   const int* ldr, double* const* X, const int* ldx, double* normub) {
   if (!c) return VCPHIP_ERR_ARGUMENT;
-  if (count < 0 || (count > 0 && (!n || !nrhs || !IA || !lda || !Ib || !ldb || !R || !ldr || !X || !ldx || !normub)))
+  if (count < 0 || (count > 0 && (!n || !nrhs || !IA || !lda || !Ib || !ldb || !X || !ldx || !normub)))
     return VCPHIP_ERR_ARGUMENT;
   if (count == 0) return VCPHIP_OK;
+  auto has_r = [&](int p) { return R && R[p]; };
   for (int p = 0; p < count; ++p) {
-    if (!IA[p] || !Ib[p] || !R[p] || !X[p]) return VCPHIP_ERR_ARGUMENT;
-    if (n[p] < 2 || nrhs[p] < 1 || lda[p] < n[p] || ldb[p] < n[p] || ldr[p] < n[p] || ldx[p] < n[p])
+    if (!IA[p] || !Ib[p] || !X[p]) return VCPHIP_ERR_ARGUMENT;
+    if (n[p] < 2 || nrhs[p] < 1 || lda[p] < n[p] || ldb[p] < n[p] || (has_r(p) && (!ldr || ldr[p] < n[p])) ||
+        ldx[p] < n[p])
       return VCPHIP_ERR_DIMENSION;
   }
   hipSetDevice(c->device);
@@ -1179,7 +1389,7 @@ extern "C" int vcphip_lss_batched(vcphip_ctx* c, int count, const int* n, const 
   for (int p = 0; p < count; ++p) {
     const size_t np = (size_t)n[p], nr = np * nrhs[p];
     oi[p] = in_bytes;
-    in_bytes += ((16 * np * np + 16 * nr + 8 * np * np) + 255) & ~(size_t)255;
+    in_bytes += ((16 * np * np + 16 * nr + (has_r(p) ? 8 * np * np : 0)) + 255) & ~(size_t)255;
     ox[p] = out_bytes;
     out_bytes += (16 * nr + 255) & ~(size_t)255;
   }
@@ -1194,7 +1404,7 @@ extern "C" int vcphip_lss_batched(vcphip_ctx* c, int count, const int* n, const 
     char* q = (char*)pin + oi[p];
     dA[p] = (const double*)q;
     dB[p] = (const double*)(q + 16 * np * np);
-    dR[p] = (const double*)(q + 16 * np * np + 16 * nr);
+    dR[p] = has_r(p) ? (const double*)(q + 16 * np * np + 16 * nr) : nullptr;
     dX[p] = (double*)((char*)pout + ox[p]);
     pn[p] = n[p];
     hipStream_t s = c->stream;
@@ -1202,13 +1412,13 @@ extern "C" int vcphip_lss_batched(vcphip_ctx* c, int count, const int* n, const 
             hipSuccess ||
         hipMemcpy2DAsync((void*)dB[p], 16 * np, Ib[p], 16 * (size_t)ldb[p], 16 * np, nrhs[p], hipMemcpyHostToDevice,
                          s) != hipSuccess ||
-        hipMemcpy2DAsync((void*)dR[p], 8 * np, R[p], 8 * (size_t)ldr[p], 8 * np, np, hipMemcpyHostToDevice, s) !=
-            hipSuccess)
+        (has_r(p) && hipMemcpy2DAsync((void*)dR[p], 8 * np, R[p], 8 * (size_t)ldr[p], 8 * np, np,
+                                      hipMemcpyHostToDevice, s) != hipSuccess))
       return VCPHIP_ERR_DEVICE;
   }
   const int rc = vcphip_lss_batched_dev(c, count, n, nrhs, dA.data(), pn.data(), dB.data(), pn.data(), dR.data(),
                                         pn.data(), dX.data(), pn.data(), normub);
-  if (rc != VCPHIP_OK && rc != VCPHIP_ERR_VERIFICATION) return rc;
+  if (rc != VCPHIP_OK && rc != VCPHIP_ERR_VERIFICATION && rc != VCPHIP_ERR_SINGULAR) return rc;
   for (int p = 0; p < count; ++p)
     if (hipMemcpy2DAsync(X[p], 16 * (size_t)ldx[p], dX[p], 16 * (size_t)n[p], 16 * (size_t)n[p], nrhs[p],
                          hipMemcpyDeviceToHost, c->stream) != hipSuccess)

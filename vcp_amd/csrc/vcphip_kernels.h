@@ -122,7 +122,11 @@ int vcphip_ratio_max(const double* X, const double* Y, long total, double* part,
 int vcphip_fill_random(void* p, size_t bytes, uint32_t seed, hipStream_t st);
 
 // verified linear solve: elementwise and scalar steps (kern_lss.hip)
-int vcphip_lss_launch_mid(int n, int ncol, const double* b, long ldb, double* mb, hipStream_t s);
+int vcphip_lss_launch_mid(int n, int ncol, const double* b, long ldb, double* mb, long ldm, hipStream_t s);
+// approximate inverse: midpoint matrix bordered by the identity to order npad,
+// and the copy of its leading n x n block out (kern_lss.hip)
+int vcphip_inv_launch_pad(int n, int npad, const double* IA, long lda, double* P, hipStream_t s);
+int vcphip_inv_launch_copy(int n, const double* P, long ldp, double* R, long ldr, hipStream_t s);
 int vcphip_lss_launch_gemv_rn(int m, int ncol, int k, const double* A, long lda, const double* B, long ldb, double* C,
                               long ldc, hipStream_t s);
 int vcphip_lss_launch_sub(int n, int ncol, double* X, long ldx, const double* b, long ldb, hipStream_t s);
