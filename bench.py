@@ -600,7 +600,26 @@ def bench_dd(ctx, n, steps, with_cpu):
         log("dd accuracy %s" % {k: v for k, v in out.items() if "err" in k or "cpu" in k})
     except Exception as e:
         log("dd accuracy check skipped: %r" % (e,))
-    del dA, dB, dC
+    # pddblas::inv (A X = I by pddblas::linearsolve's refinement) on the same
+    # A, every step on the device; max |I - A X| by the dd residual product
+    eye = torch.zeros((n, n, 2), dtype=torch.float64, device=dev)
+    eye[..., 0] = torch.eye(n, dtype=torch.float64, device=dev)
+    dX = torch.empty((n, n, 2), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    ctx.dd_solve_dev(n, n, dA.data_ptr(), n, eye.data_ptr(), n, dX.data_ptr(), n)
+    ts = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        rounds = ctx.dd_solve_dev(n, n, dA.data_ptr(), n, eye.data_ptr(), n, dX.data_ptr(), n)
+        ts.append(time.perf_counter() - t0)
+    L = vcp_amd.lib()
+    rc = L.vcphip_dd_residual_dev(ctx._h, n, n, n, dA.data_ptr(), n, dX.data_ptr(), n, eye.data_ptr(), n,
+                                  dC.data_ptr(), n)
+    ctx.synchronize()
+    res = float(dC[..., 0].abs().max().item()) if rc == 0 else None
+    out["dd_inv"] = {"n": n, "ms": round(min(ts) * 1e3, 3), "rounds": int(rounds), "max_abs_I_minus_AX": res}
+    log("dd inverse %s" % out["dd_inv"])
+    del dA, dB, dC, dX, eye
     return out
 
 

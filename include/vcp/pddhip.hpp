@@ -11,9 +11,9 @@
 // 318, :379, :412), so pddhip declares its own, which hide them for
 // vcp::matrix<kv::dd, pddhip> (name lookup in matrix's operator* and lss,
 // vcp/matrix.hpp:245-267, :607-634); inv() (virtual, :477) is overridden so
-// that it reaches pddhip::linearsolve.  linearsolve keeps the reference's
-// refinement loop and forms its residuals r = b - A x on the GPU
-// (vcphip_dd_residual); Cholesky, eigsym and eigsymge keep the reference's
+// that it reaches pddhip::linearsolve.  linearsolve runs the reference's
+// refinement loop on the device (vcphip_dd_solve: LU and corrections by
+// rocSOLVER, residuals r = b - A x by the dd Ozaki product); Cholesky, eigsym and eigsymge keep the reference's
 // refinements with their dd products on the GPU.  Scalar operands and products below the GPU / host crossover
 // (vcphip_policy::crossover, measured by tune_crossover()) keep the
 // reference's host semantics.
@@ -105,44 +105,20 @@ class pddhip : public vcp::pddblas {
       vcp::throw_error<vcp::dimension_error>("linearsolve: invalid dimensions: A=(", this->row, ", ", this->column,
                                              "), b=(", b.row, ", ", b.column, ")");
     }
-    const int an = this->row, nrhs = b.column, nb = an * nrhs;
+    const int an = this->row, nrhs = b.column;
     if (vcphip_policy::on_host(vcphip_policy::CROSS_DD, an, nrhs, an)) {
       vcp::pddblas::linearsolve(b, x);
       return;
     }
-    vcp::pdblas Ad;
-    dd_to_d(*this, Ad);
-    std::vector<int> ipiv(an);
-    int N = an, NRHS = nrhs, lda = an, ldb = an, info;
-    dgetrf_(&N, &N, &Ad.v.front(), &lda, ipiv.data(), &info);
-    if (info != 0) throw vcp::lapack_error("dgetrf", info, "linearsolve: dgetrf failed");
+    // the whole refinement on the device (vcphip_dd_solve): LU and the
+    // corrections by rocSOLVER getrf/getrs for LAPACK's dgetrf/dgetrs, the
+    // residuals by the dd Ozaki product, the reference's itmax and stopping rule
     x.zeros(an, nrhs);
-    std::vector<kv::dd> r = b.v;   // r = b - A x with x = 0
-    vcp::pdblas rd;
-    rd.zeros(an, nrhs);
-    char TN[] = "N";
-    const int itmax = 10;
-    double prevnormd = std::numeric_limits<double>::infinity();
-    vcphip_ctx* ctx = vcphip_policy::checked_context<E>();
-    for (int it = 0; it < itmax; it++) {
-      for (int i = 0; i < nb; i++) rd.v[i] = r[i].a1;
-      dgetrs_(TN, &N, &NRHS, &Ad.v.front(), &lda, ipiv.data(), &rd.v.front(), &ldb, &info);
-      if (info != 0) throw vcp::lapack_error("dgetrs", info, "linearsolve: dgetrs failed");
-      double normd = 0.0, normx = 0.0;
-      for (int i = 0; i < nb; i++) {
-        x.v[i] += kv::dd(rd.v[i]);
-        const double ad = std::fabs(rd.v[i]);
-        if (ad > normd) normd = ad;
-        const double ax = std::fabs(x.v[i].a1);
-        if (ax > normx) normx = ax;
-      }
-      if (normd == 0.0 || normd <= 0.25e-32 * normx || normd >= 0.5 * prevnormd) break;
-      prevnormd = normd;
-      vcphip_policy::check<E>(vcphip_dd_residual(ctx, an, nrhs, an, pddhip_detail::raw(this->v), an,
-                                                 pddhip_detail::raw(x.v), an, pddhip_detail::raw(b.v), an,
-                                                 pddhip_detail::raw(r), an),
-                              "vcphip_dd_residual");
-    }
+    const int rc = vcphip_dd_solve(vcphip_policy::checked_context<E>(), an, nrhs, pddhip_detail::raw(this->v), an,
+                                   pddhip_detail::raw(b.v), an, pddhip_detail::raw(x.v), an, 10, nullptr);
+    if (rc == VCPHIP_ERR_SINGULAR)
+      throw vcp::lapack_error("rocsolver_dgetrf", VCPHIP_ERR_SINGULAR, "linearsolve: zero pivot in the LU of A");
+    vcphip_policy::check<E>(rc, "vcphip_dd_solve");
   }
 
   // A = inv(A): A X = I through pddhip::linearsolve (pddblas.hpp:477-486)

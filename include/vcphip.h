@@ -164,6 +164,20 @@ int vcphip_dd_residual(vcphip_ctx* ctx, int m, int n, int k, const double* A, in
                        const double* B, int ldb, double* R, int ldr);
 int vcphip_dd_residual_dev(vcphip_ctx* ctx, int m, int n, int k, const double* A, int lda, const double* X, int ldx,
                            const double* B, int ldb, double* R, int ldr);
+/* x = A \ b for dd matrices, every step on the device: pddblas::linearsolve
+ * (vcp/pddblas.hpp:412-474).  LU with partial pivoting of the high parts of
+ * A (rocSOLVER dgetrf for LAPACK's), then up to itmax rounds of d = LU \ hi(r)
+ * (dgetrs), x += dd(d), stopping when max|d| is 0, at most 0.25e-32 max|x.hi|
+ * or at least half the previous round's, else r = b - A x as
+ * vcphip_dd_residual forms it (the first round takes r = b, x = 0).  A is
+ * n x n, b and x n x nrhs, dd pairs {hi, lo}; *iters (may be NULL) receives
+ * the rounds done.  The reference's itmax is 10; pddblas::inv is this with
+ * b = I.  Returns VCPHIP_ERR_SINGULAR when a pivot is exactly zero (the
+ * reference's lapack_error("dgetrf")).  Both forms synchronise the stream. */
+int vcphip_dd_solve(vcphip_ctx* ctx, int n, int nrhs, const double* A, int lda, const double* b, int ldb, double* x,
+                    int ldx, int itmax, int* iters);
+int vcphip_dd_solve_dev(vcphip_ctx* ctx, int n, int nrhs, const double* A, int lda, const double* b, int ldb,
+                        double* x, int ldx, int itmax, int* iters);
 
 /* ---- batched verified residual: C_i = R_i * IA_i - I ---------------------- */
 /* For each problem i (square, size n[i]): C_i = R_i*IA_i - I as intervals and

@@ -182,3 +182,87 @@ def test_dd_eigsymge(ctx):
     for i, j in ((0, 0), (2, 5), (19, 19)):
         g = sum(val(X[s, i]) * sum(val(B[s, t]) * val(X[t, j]) for t in range(n)) for s in range(n)) - (1 if i == j else 0)
         assert abs(g) <= F(2) ** -96 * n
+
+
+def _host_lu_refinement(ctx, A, b, itmax=10):
+    """pddblas::linearsolve's loop with LAPACK's LU on the host (scipy) and
+    the GPU residual: the reference's arrangement, for comparison with the
+    all-device vcphip_dd_solve"""
+    import scipy.linalg
+
+    n = A.shape[0]
+    lu = scipy.linalg.lu_factor(np.asfortranarray(A[..., 0]), check_finite=False)
+    x = O.pairs(n, b.shape[1])
+    r, prev, it = b, np.inf, 0
+    while it < itmax:
+        d = scipy.linalg.lu_solve(lu, np.asfortranarray(r[..., 0]), check_finite=False)
+        s = x[..., 0] + d
+        bb = s - x[..., 0]
+        e = (x[..., 0] - (s - bb)) + (d - bb) + x[..., 1]
+        h = s + e
+        x[..., 1] = e - (h - s)
+        x[..., 0] = h
+        it += 1
+        normd, normx = float(np.abs(d).max()), float(np.abs(x[..., 0]).max())
+        if normd == 0.0 or normd <= 0.25e-32 * normx or normd >= 0.5 * prev:
+            break
+        prev = normd
+        r = ctx.dd_residual(A, x, b)
+    return x, it
+
+
+@pytest.mark.parametrize("n,nrhs", [(60, 3), (257, 1)])
+def test_dd_solve_agrees_with_host_lu_refinement(ctx, n, nrhs):
+    rng = np.random.default_rng(40 + n)
+    A, b = dd_rand(rng, n, n, diag=float(n) ** 0.5 * 2), dd_rand(rng, n, nrhs)
+    x, it = ctx.dd_linearsolve(A, b)
+    xh, ith = _host_lu_refinement(ctx, A, b)
+    assert 2 <= it <= 6 and abs(it - ith) <= 1
+    # full dd values: the high parts alone may differ by an ulp near a
+    # rounding boundary while the values agree to 2^-100
+    diff = (x[..., 0] - xh[..., 0]) + (x[..., 1] - xh[..., 1])
+    assert np.abs(diff).max() <= 2.0 ** -90 * np.abs(x[..., 0]).max()
+
+
+def test_dd_solve_at_n1000_exact_rows(ctx):
+    print("dd_solve n=1000", flush=True)
+    rng = np.random.default_rng(44)
+    n = 1000
+    A, b = dd_rand(rng, n, n, diag=float(n) ** 0.5 * 2), dd_rand(rng, n, 2)
+    x, it = ctx.dd_linearsolve(A, b)
+    assert 2 <= it <= 6
+    for i, j in ((0, 0), (499, 1), (999, 0)):
+        r = val(b[i, j]) - sum(val(A[i, t]) * val(x[t, j]) for t in range(n))
+        scale = sum(abs(val(A[i, t]) * val(x[t, j])) for t in range(n))
+        assert abs(r) <= F(2) ** -98 * scale
+
+
+def test_dd_solve_dev_entry_and_leading_dimensions(ctx):
+    rng = np.random.default_rng(45)
+    n, nrhs, lda, ldb, ldx = 70, 3, 75, 72, 80
+    A, b = dd_rand(rng, n, n, diag=20.0), dd_rand(rng, n, nrhs)
+
+    def padded(P, ld):
+        t = torch.zeros((P.shape[1], ld, 2), dtype=torch.float64, device="cuda")
+        t[:, :P.shape[0], :] = torch.from_numpy(np.ascontiguousarray(P.transpose(1, 0, 2))).cuda()
+        return t
+
+    dA, dB = padded(A, lda), padded(b, ldb)
+    dX = torch.full((nrhs, ldx, 2), 5.0, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    it = ctx.dd_solve_dev(n, nrhs, dA.data_ptr(), lda, dB.data_ptr(), ldb, dX.data_ptr(), ldx)
+    x, ith = ctx.dd_linearsolve(A, b)
+    X = dX.cpu().numpy()
+    assert it == ith
+    assert np.array_equal(X[:, :n, :].transpose(1, 0, 2), x)
+    assert (X[:, n:, :] == 5.0).all()
+
+
+def test_dd_solve_singular(ctx):
+    n = 6
+    rng = np.random.default_rng(46)
+    A, b = dd_rand(rng, n, n, diag=3.0), dd_rand(rng, n, 1)
+    A[:, 4, :] = 0.0
+    with pytest.raises(vcp_amd.VcphipError) as e:
+        ctx.dd_linearsolve(A, b)
+    assert e.value.code == -10

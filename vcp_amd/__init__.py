@@ -68,6 +68,8 @@ SIGNATURES = {
     "vcphip_ddgemm_dev": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_ddgemm_tn": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_inv_mid": [_vp, _c_int, _vp, _c_int, _vp, _c_int],
+    "vcphip_dd_solve": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _p_int],
+    "vcphip_dd_solve_dev": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _p_int],
     "vcphip_inv_mid_dev": [_vp, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_lss": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _p_dbl],
     "vcphip_lss_dev": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _p_dbl],
@@ -382,38 +384,33 @@ class Context:
 
     def dd_linearsolve(self, A, b, itmax=10):
         """x = A \\ b for dd matrices by pddblas::linearsolve's refinement
-        (vcp/pddblas.hpp:412-474): LU of the high parts on the host (LAPACK,
-        as the reference), x += the LU correction in dd, stop when the
-        correction is below 0.25e-32 |x| or stagnates, and r = b - A x on the
-        GPU (vcphip_dd_residual).  Returns (x, iterations)."""
-        import scipy.linalg
-
+        (vcp/pddblas.hpp:412-474), every step on the device (vcphip_dd_solve):
+        LU of the high parts (rocSOLVER getrf), x += the LU correction in dd
+        (getrs), stop when the correction is below 0.25e-32 |x| or
+        stagnates, else r = b - A x by the dd Ozaki product.  Returns
+        (x, iterations); raises VcphipError (VCPHIP_ERR_SINGULAR) on a zero
+        pivot."""
         n = A.shape[0]
         if A.shape[1] != n or b.shape[0] != n:
             raise ValueError("dimension mismatch")
-        lu = scipy.linalg.lu_factor(np.asfortranarray(A[..., 0]), check_finite=False)
-        x = pairs(n, b.shape[1])
-        r = b
-        prev = np.inf
-        it = 0
-        while it < itmax:
-            d = scipy.linalg.lu_solve(lu, np.asfortranarray(r[..., 0]), check_finite=False)
-            # x += dd(d): TwoSum of the high parts, the low part added, FastTwoSum
-            s = x[..., 0] + d
-            bb = s - x[..., 0]
-            e = (x[..., 0] - (s - bb)) + (d - bb)
-            e = e + x[..., 1]
-            h = s + e
-            x[..., 1] = e - (h - s)
-            x[..., 0] = h
-            it += 1
-            normd = float(np.abs(d).max()) if d.size else 0.0
-            normx = float(np.abs(x[..., 0]).max()) if d.size else 0.0
-            if normd == 0.0 or normd <= 0.25e-32 * normx or normd >= 0.5 * prev:
-                break
-            prev = normd
-            r = self.dd_residual(A, x, b)
-        return x, it
+        nrhs = b.shape[1]
+        x = pairs(n, nrhs)
+        if n == 0 or nrhs == 0:
+            return x, 0
+        pa, lda = _pair_ptr(A)
+        pb, ldb = _pair_ptr(b)
+        px, ldx = _pair_ptr(x)
+        it = ctypes.c_int()
+        _check(lib().vcphip_dd_solve(self._h, n, nrhs, pa, lda, pb, ldb, px, ldx, itmax, ctypes.byref(it)),
+               "vcphip_dd_solve")
+        return x, it.value
+
+    def dd_solve_dev(self, n, nrhs, dA, lda, dB, ldb, dX, ldx, itmax=10):
+        """device-pointer form of dd_linearsolve(); returns the iterations"""
+        it = ctypes.c_int()
+        _check(lib().vcphip_dd_solve_dev(self._h, n, nrhs, _vp(dA), lda, _vp(dB), ldb, _vp(dX), ldx, itmax,
+                                         ctypes.byref(it)), "vcphip_dd_solve_dev")
+        return it.value
 
     def dd_inv(self, A):
         """inv(A) for a dd matrix: A X = I by dd_linearsolve (pddblas::inv,

@@ -372,7 +372,97 @@ __global__ __launch_bounds__(256) void k_dd_sub_from(int m, int n, const double*
   *r = make_double2(s, e);
 }
 
+// hi parts of a dd matrix: D(i, j) = X(i, j).hi, D with leading dimension m
+__global__ __launch_bounds__(256) void k_dd_hi(int m, int n, const double* __restrict__ X, long ldx,
+                                               double* __restrict__ D) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)m * n) return;
+  const int i = (int)(idx % m), j = (int)(idx / m);
+  D[idx] = X[2 * (i + ldx * (long)j)];
+}
+
+// the refinement step of pddblas::linearsolve (vcp/pddblas.hpp:450-457):
+// x(i, j) += dd(d(i, j)) (TwoSum of the high parts, the low part added,
+// FastTwoSum), and per-block maxima of |d| and of the new |x.hi| taken as the
+// reference does (a NaN never raises the maximum).  Grid-stride; part[2 b]
+// and part[2 b + 1] receive block b's maxima.
+__global__ __launch_bounds__(256) void k_dd_add_d(int m, int n, const double* __restrict__ d,
+                                                  double* __restrict__ X, long ldx, double* __restrict__ part) {
+  __shared__ double sd[256], sx[256];
+  const long total = (long)m * n;
+  double md = 0.0, mx = 0.0;
+  for (long idx = (long)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (long)gridDim.x * 256) {
+    const int i = (int)(idx % m), j = (int)(idx / m);
+    double2* xp = reinterpret_cast<double2*>(X + 2 * (i + ldx * (long)j));
+    const double2 x = *xp;
+    const double dv = d[idx];
+    double s, e, h, l;
+    two_sum(x.x, dv, s, e);
+    e = e + x.y;
+    fast_two_sum(s, e, h, l);
+    *xp = make_double2(h, l);
+    const double ad = __builtin_fabs(dv), ax = __builtin_fabs(h);
+    if (ad > md) md = ad;
+    if (ax > mx) mx = ax;
+  }
+  sd[threadIdx.x] = md;
+  sx[threadIdx.x] = mx;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      if (sd[threadIdx.x + w] > sd[threadIdx.x]) sd[threadIdx.x] = sd[threadIdx.x + w];
+      if (sx[threadIdx.x + w] > sx[threadIdx.x]) sx[threadIdx.x] = sx[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sd[0];
+    part[2 * blockIdx.x + 1] = sx[0];
+  }
+}
+
+// out[0], out[1] = the maxima over nb blocks' partials
+__global__ __launch_bounds__(256) void k_max_pairs(int nb, const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double sd[256], sx[256];
+  double md = 0.0, mx = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) {
+    if (part[2 * b] > md) md = part[2 * b];
+    if (part[2 * b + 1] > mx) mx = part[2 * b + 1];
+  }
+  sd[threadIdx.x] = md;
+  sx[threadIdx.x] = mx;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      if (sd[threadIdx.x + w] > sd[threadIdx.x]) sd[threadIdx.x] = sd[threadIdx.x + w];
+      if (sx[threadIdx.x + w] > sx[threadIdx.x]) sx[threadIdx.x] = sx[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = sd[0];
+    out[1] = sx[0];
+  }
+}
+
 }  // namespace
+
+int vcphip_dd_hi(int m, int n, const double* X, long ldx, double* D, hipStream_t s) {
+  if ((long)m * n == 0) return 0;
+  hipLaunchKernelGGL(k_dd_hi, dim3((unsigned)(((long)m * n + 255) / 256)), dim3(256), 0, s, m, n, X, ldx, D);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+// part holds 2 * VCPHIP_DD_ADD_BLOCKS doubles; out[0] = max |d|, out[1] = max |x.hi|
+int vcphip_dd_add_d(int m, int n, const double* d, double* X, long ldx, double* part, double* out, hipStream_t s) {
+  const long total = (long)m * n;
+  long nb = (total + 255) / 256;
+  if (nb > VCPHIP_DD_ADD_BLOCKS) nb = VCPHIP_DD_ADD_BLOCKS;
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(k_dd_add_d, dim3((unsigned)nb), dim3(256), 0, s, m, n, d, X, ldx, part);
+  hipLaunchKernelGGL(k_max_pairs, dim3(1), dim3(256), 0, s, (int)nb, part, out);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
 
 int vcphip_dd_stats(const double* X, long rs, long cs, int rows, int len, int* e, int* flags, int* part,
                     hipStream_t s) {

@@ -33,7 +33,7 @@ struct vcphip_ctx {
   long long oz_nmod = 0, oz_variant = 0, timing = 0, dd_nmod = 0;
   // workspace slots (device memory, grown on demand)
   enum { WS_PA, WS_PB, WS_U, WS_Z, WS_SA, WS_SB, WS_HA, WS_HB, WS_HC, WS_HC2, WS_BATCH, WS_DD, WS_LSS, WS_HR, WS_LSSB,
-         WS_INV, WS_N };
+         WS_INV, WS_DDS, WS_N };
   void* ws[WS_N] = {};
   size_t ws_size[WS_N] = {};
   // stage timing: events recorded by every call since the last synchronize,
@@ -677,7 +677,7 @@ extern "C" int vcphip_dd_residual_dev(vcphip_ctx* c, int m, int n, int k, const 
     return VCPHIP_ERR_DIMENSION;
   if (m == 0 || n == 0) return VCPHIP_OK;
   hipSetDevice(c->device);
-  const int rc = dd_gemm_dev(c, false, m, n, k, A, lda, X, ldx, R, ldr);
+  const int rc = dd_gemm_dev(c, false, m, n, k, A, lda, X, ldx, R, ldr);  // R = A X
   if (rc) return rc;
   return check_launch(vcphip_dd_sub_from(m, n, B, ldb, R, ldr, c->stream));
 }
@@ -795,6 +795,15 @@ constexpr int INV_BORDER = 64;
 
 inline int inv_order(int n) { return (n + INV_BORDER - 1) / INV_BORDER * INV_BORDER; }
 
+// the context's rocBLAS handle (created on first use), bound to its stream
+int blas_on_stream(vcphip_ctx* c) {
+  if (!c->blas && rocblas_create_handle(&c->blas) != rocblas_status_success) {
+    c->blas = nullptr;
+    return VCPHIP_ERR_DEVICE;
+  }
+  return rocblas_set_stream(c->blas, c->stream) == rocblas_status_success ? VCPHIP_OK : VCPHIP_ERR_DEVICE;
+}
+
 struct InvJob {
   int n;
   const double* IA;
@@ -825,11 +834,7 @@ struct InvBatch {
 int inv_mid_enqueue(vcphip_ctx* c, const std::vector<InvJob>& jobs, InvBatch& b) {
   const int cnt = (int)jobs.size();
   hipStream_t s = c->stream;
-  if (!c->blas && rocblas_create_handle(&c->blas) != rocblas_status_success) {
-    c->blas = nullptr;
-    return VCPHIP_ERR_DEVICE;
-  }
-  if (rocblas_set_stream(c->blas, s) != rocblas_status_success) return VCPHIP_ERR_DEVICE;
+  VCPHIP_TRY(blas_on_stream(c));
   b.ord.resize(cnt);
   for (int i = 0; i < cnt; ++i) b.ord[i] = i;
   std::stable_sort(b.ord.begin(), b.ord.end(), [&](int x, int y) { return jobs[x].npad < jobs[y].npad; });
@@ -908,6 +913,79 @@ extern "C" int vcphip_inv_mid_dev(vcphip_ctx* c, int n, const double* IA, int ld
   std::vector<char> singular;
   VCPHIP_TRY(inv_mid_singular(c, b, singular));
   return singular[0] ? VCPHIP_ERR_SINGULAR : VCPHIP_OK;
+}
+
+// ===========================================================================
+// dd linear solve: pddblas::linearsolve (vcp/pddblas.hpp:412-474) on the device
+// ===========================================================================
+extern "C" int vcphip_dd_solve_dev(vcphip_ctx* c, int n, int nrhs, const double* A, int lda, const double* b, int ldb,
+                                   double* x, int ldx, int itmax, int* iters) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (n < 0 || nrhs < 0 || lda < (n > 1 ? n : 1) || ldb < (n > 1 ? n : 1) || ldx < (n > 1 ? n : 1))
+    return VCPHIP_ERR_DIMENSION;
+  if (iters) *iters = 0;
+  if (n == 0 || nrhs == 0) return VCPHIP_OK;
+  if (!A || !b || !x || itmax < 0) return VCPHIP_ERR_ARGUMENT;
+  hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  // workspace: LU of hi(A) (n x n), the correction d (n x nrhs), the residual
+  // r (dd, n x nrhs), the block maxima and the two norms, pivots and info
+  auto al = [](size_t bytes) { return (bytes + 255) & ~(size_t)255; };
+  const size_t nn = (size_t)n * n, nr = (size_t)n * nrhs;
+  const size_t o_d = al(8 * nn), o_r = o_d + al(8 * nr), o_part = o_r + al(16 * nr);
+  const size_t o_norm = o_part + al(16 * (size_t)VCPHIP_DD_ADD_BLOCKS), o_piv = o_norm + 256;
+  const size_t o_info = o_piv + al(sizeof(rocblas_int) * n), total = o_info + 256;
+  void* base;
+  VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_DDS, total, &base));
+  char* b8 = (char*)base;
+  double *LU = (double*)b8, *d = (double*)(b8 + o_d), *r = (double*)(b8 + o_r);
+  double *part = (double*)(b8 + o_part), *norms = (double*)(b8 + o_norm);
+  rocblas_int *piv = (rocblas_int*)(b8 + o_piv), *info = (rocblas_int*)(b8 + o_info);
+  auto fail = [&](int code) {
+    hipStreamSynchronize(s);
+    return code;
+  };
+  VCPHIP_TRY(blas_on_stream(c));
+  StageTimer T(c);
+  // LU of double(A), reused in every round (:423-432)
+  if (vcphip_dd_hi(n, n, A, lda, LU, s) || rocsolver_dgetrf(c->blas, n, n, LU, n, piv, info) != rocblas_status_success)
+    return fail(VCPHIP_ERR_DEVICE);
+  rocblas_int hinfo = 0;
+  if (hipMemcpyAsync(&hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return VCPHIP_ERR_DEVICE;
+  if (hinfo != 0) return VCPHIP_ERR_SINGULAR;   // the reference's lapack_error("dgetrf")
+  T.mark("dd_solve_lu");
+  // x = 0, r = b (:438-439)
+  if (hipMemset2DAsync(x, 16 * (size_t)ldx, 0, 16 * (size_t)n, nrhs, s) != hipSuccess) return fail(VCPHIP_ERR_DEVICE);
+  const double* rc = b;
+  long rld = ldb;
+  double prev = HUGE_VAL;
+  int it = 0;
+  while (it < itmax) {
+    // d = LU \ hi(r); x += dd(d); the norms (:446-458)
+    if (vcphip_dd_hi(n, nrhs, rc, rld, d, s) ||
+        rocsolver_dgetrs(c->blas, rocblas_operation_none, n, nrhs, LU, n, piv, d, n) != rocblas_status_success ||
+        vcphip_dd_add_d(n, nrhs, d, x, ldx, part, norms, s))
+      return fail(VCPHIP_ERR_DEVICE);
+    double h[2];
+    if (hipMemcpyAsync(h, norms, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return VCPHIP_ERR_DEVICE;
+    ++it;
+    const double normd = h[0], normx = h[1];
+    // converged to dd accuracy or stagnated (:460)
+    if (normd == 0.0 || normd <= 0.25e-32 * normx || normd >= 0.5 * prev) break;
+    prev = normd;
+    // r = b - A x with the dd Ozaki product (:463-470)
+    const int rrc = vcphip_dd_residual_dev(c, n, nrhs, n, A, lda, x, ldx, b, ldb, r, n);
+    if (rrc) return fail(rrc);
+    rc = r;
+    rld = n;
+  }
+  T.mark("dd_solve_refine");
+  if (iters) *iters = it;
+  return VCPHIP_OK;
 }
 
 // ===========================================================================
@@ -1351,6 +1429,24 @@ extern "C" int vcphip_lss(vcphip_ctx* c, int n, int nrhs, const double* IA, int 
   if (rc != VCPHIP_OK && rc != VCPHIP_ERR_VERIFICATION && rc != VCPHIP_ERR_SINGULAR) return rc;
   VCPHIP_TRY(to_host(c, dX, X, ldx, n, nrhs, 16));
   return rc;
+}
+
+extern "C" int vcphip_dd_solve(vcphip_ctx* c, int n, int nrhs, const double* A, int lda, const double* b, int ldb,
+                               double* x, int ldx, int itmax, int* iters) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (n < 0 || nrhs < 0 || lda < (n > 1 ? n : 1) || ldb < (n > 1 ? n : 1) || ldx < (n > 1 ? n : 1))
+    return VCPHIP_ERR_DIMENSION;
+  if (iters) *iters = 0;
+  if (n == 0 || nrhs == 0) return VCPHIP_OK;
+  if (!A || !b || !x || itmax < 0) return VCPHIP_ERR_ARGUMENT;
+  hipSetDevice(c->device);
+  double *dA, *dB;
+  void* px;
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HA, A, lda, n, n, 16, &dA));
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HB, b, ldb, n, nrhs, 16, &dB));
+  VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HC2, 16 * (size_t)n * nrhs, &px));
+  VCPHIP_TRY(vcphip_dd_solve_dev(c, n, nrhs, dA, n, dB, n, (double*)px, n, itmax, iters));
+  return to_host(c, (double*)px, x, ldx, n, nrhs, 16);
 }
 
 extern "C" int vcphip_inv_mid(vcphip_ctx* c, int n, const double* IA, int lda, double* R, int ldr) {

@@ -117,6 +117,10 @@ int vcphip_dd_recon(int m, int n, int mpad, int npad, const VcphipOzParams& P, c
 // autotuner helpers (kern_tune.hip)
 // R = B - R for dd matrices (the residual step of the dd linear solve)
 int vcphip_dd_sub_from(int m, int n, const double* B, long ldb, double* R, long ldr, hipStream_t s);
+// dd solve (pddblas::linearsolve): hi parts, and x += dd(d) with max |d|, max |x.hi|
+constexpr int VCPHIP_DD_ADD_BLOCKS = 1024;
+int vcphip_dd_hi(int m, int n, const double* X, long ldx, double* D, hipStream_t s);
+int vcphip_dd_add_d(int m, int n, const double* d, double* X, long ldx, double* part, double* out, hipStream_t s);
 int vcphip_gather_rows(const double* A, long lda, int m, int k, int s, double* out, hipStream_t st);
 int vcphip_ratio_max(const double* X, const double* Y, long total, double* part, int nblocks, hipStream_t st);
 int vcphip_fill_random(void* p, size_t bytes, uint32_t seed, hipStream_t st);
