@@ -142,36 +142,15 @@ class pddhip : public vcp::pddblas {
       vcp::pddblas::Cholesky();
       return;
     }
-    vcp::pdblas Ud;
-    dd_to_d(*this, Ud);
-    Ud.Cholesky();
-    std::vector<kv::dd> U(nn), P(nn), W(nn);
-    for (int q = 0; q < nn; ++q) U[q] = kv::dd(Ud.v[q]);
-    char side_l[] = "L", side_r[] = "R", upper[] = "U", trans[] = "T", notrans[] = "N";
-    double one = 1.0;
-    int N = an;
-    vcphip_ctx* ctx = vcphip_policy::checked_context<E>();
-    using pddhip_detail::raw;
-    for (int step = 0; step < 2; ++step) {
-      vcphip_policy::check<E>(vcphip_ddgemm_tn(ctx, an, an, raw(U), an, raw(P), an), "vcphip_ddgemm_tn");
-      vcp::pdblas G;
-      G.zeros(an, an);
-      for (int q = 0; q < nn; ++q) {
-        Ud.v[q] = U[q].a1;
-        G.v[q] = (this->v[q] - P[q]).a1;   // E = A - U^T U
-      }
-      dtrsm_(side_l, upper, trans, notrans, &N, &N, &one, &Ud.v.front(), &N, &G.v.front(), &N);
-      dtrsm_(side_r, upper, notrans, notrans, &N, &N, &one, &Ud.v.front(), &N, &G.v.front(), &N);
-      for (int j = 0; j < an; ++j)
-        for (int i = 0; i < an; ++i) {
-          const double g = G.v[i + an * j];
-          W[i + an * j] = kv::dd(i > j ? 0.0 : (i == j ? 0.5 * g : g));
-        }
-      vcphip_policy::check<E>(vcphip_ddgemm(ctx, an, an, an, raw(W), an, raw(U), an, raw(P), an), "vcphip_ddgemm");
-      for (int q = 0; q < nn; ++q) U[q] += P[q];   // U += W U
-    }
-    for (int j = 0; j < an; ++j)
-      for (int i = j + 1; i < an; ++i) U[i + an * j] = kv::dd(0.0);
+    // the whole refinement on the device (vcphip_dd_cholesky): dpotrf and
+    // dtrsm by rocSOLVER / rocBLAS for LAPACK / BLAS, E = A - U^T U and W U
+    // by the dd Ozaki product, the reference's two rounds
+    std::vector<kv::dd> U(nn);
+    const int rc = vcphip_dd_cholesky(vcphip_policy::checked_context<E>(), an, pddhip_detail::raw(this->v), an,
+                                      pddhip_detail::raw(U), an, 2);
+    if (rc == VCPHIP_ERR_SINGULAR)
+      throw vcp::lapack_error("rocsolver_dpotrf", VCPHIP_ERR_SINGULAR, "Cholesky: matrix not positive definite");
+    vcphip_policy::check<E>(rc, "vcphip_dd_cholesky");
     this->v = std::move(U);
   }
 

@@ -47,7 +47,8 @@ enum {
     VCPHIP_ERR_UNSUPPORTED = -7,  /* size or form outside what the kernels cover */
     VCPHIP_ERR_RANGE = -8,        /* exponent range outside the supported one    */
     VCPHIP_ERR_IO = -9,           /* vcp::io_error (vcp/vcp_portable_fio.hpp)    */
-    VCPHIP_ERR_SINGULAR = -10     /* vcp::lapack_error (pdblas.hpp:191-203)      */
+    VCPHIP_ERR_SINGULAR = -10     /* vcp::lapack_error: zero pivot (pdblas.hpp:
+                                     191-203) or not positive definite (:215-217) */
 };
 
 /* forms of the interval x interval product */
@@ -178,6 +179,17 @@ int vcphip_dd_solve(vcphip_ctx* ctx, int n, int nrhs, const double* A, int lda, 
                     int ldx, int itmax, int* iters);
 int vcphip_dd_solve_dev(vcphip_ctx* ctx, int n, int nrhs, const double* A, int lda, const double* b, int ldb,
                         double* x, int ldx, int itmax, int* iters);
+/* Upper triangular U with U^T U = A to dd accuracy, every step on the
+ * device: pddblas::Cholesky (vcp/pddblas.hpp:488-563).  The reference's
+ * symmetry check (both parts of A(i, j) and A(j, i) equal, else
+ * VCPHIP_ERR_ARGUMENT for its domain_error); dpotrf of double(A) (rocSOLVER;
+ * VCPHIP_ERR_SINGULAR when A is not positive definite, its lapack_error),
+ * the strictly lower part zeroed; then itep rounds (the reference's 2) of
+ * E = A - U^T U (dd product), G = Ud^-T E Ud^-1 (rocBLAS dtrsm, Ud = U's
+ * high parts), W = triu(G, 1) + diag(G) / 2 and U = U + W U (dd product,
+ * double-double addition).  A and U are n x n dd pairs {hi, lo}. */
+int vcphip_dd_cholesky(vcphip_ctx* ctx, int n, const double* A, int lda, double* U, int ldu, int itep);
+int vcphip_dd_cholesky_dev(vcphip_ctx* ctx, int n, const double* A, int lda, double* U, int ldu, int itep);
 
 /* ---- batched verified residual: C_i = R_i * IA_i - I ---------------------- */
 /* For each problem i (square, size n[i]): C_i = R_i*IA_i - I as intervals and

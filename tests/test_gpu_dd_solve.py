@@ -266,3 +266,61 @@ def test_dd_solve_singular(ctx):
     with pytest.raises(vcp_amd.VcphipError) as e:
         ctx.dd_linearsolve(A, b)
     assert e.value.code == -10
+
+
+def test_dd_cholesky_agrees_with_host_arrangement(ctx):
+    """the all-device Cholesky against the reference's arrangement restated
+    with LAPACK / BLAS on the host and the GPU dd products (ddrefine)"""
+    from vcp_amd import ddrefine
+
+    rng = np.random.default_rng(23)
+    n = 130
+    A = _spd(rng, n)
+    U = ctx.dd_cholesky(A)
+    Uh = ddrefine.cholesky(ctx, A)
+    diff = (U[..., 0] - Uh[..., 0]) + (U[..., 1] - Uh[..., 1])
+    assert np.abs(diff).max() <= 2.0 ** -90 * np.abs(U[..., 0]).max()
+    assert (U[np.tril_indices(n, -1)] == 0.0).all()
+
+
+def test_dd_cholesky_at_n300_exact_entries(ctx):
+    rng = np.random.default_rng(24)
+    n = 300
+    A = _spd(rng, n)
+    U = ctx.dd_cholesky(A)
+    for i, j in ((0, 0), (17, 250), (299, 299), (150, 151)):
+        s = sum(val(U[t, i]) * val(U[t, j]) for t in range(n)) - val(A[i, j])
+        scale = sum(abs(val(U[t, i]) * val(U[t, j])) for t in range(n))
+        assert abs(s) <= F(2) ** -98 * scale
+
+
+def test_dd_cholesky_errors(ctx):
+    rng = np.random.default_rng(25)
+    n = 12
+    A = _spd(rng, n)
+    B = O.pairs(n, n)
+    B[...] = A
+    B[2, 5, 1] = 2.0 ** -80   # breaks symmetry in the low part only
+    with pytest.raises(vcp_amd.VcphipError) as e:
+        ctx.dd_cholesky(B)
+    assert e.value.code == -2
+    C = O.pairs(n, n)
+    C[...] = A
+    C[..., 0] = -C[..., 0]    # negative definite
+    with pytest.raises(vcp_amd.VcphipError) as e:
+        ctx.dd_cholesky(C)
+    assert e.value.code == -10
+
+
+def test_dd_cholesky_dev_leading_dimensions(ctx):
+    rng = np.random.default_rng(26)
+    n, lda, ldu = 50, 53, 57
+    A = _spd(rng, n)
+    dA = torch.zeros((n, lda, 2), dtype=torch.float64, device="cuda")
+    dA[:, :n, :] = torch.from_numpy(np.ascontiguousarray(A.transpose(1, 0, 2))).cuda()
+    dU = torch.full((n, ldu, 2), 3.0, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    ctx.dd_cholesky_dev(n, dA.data_ptr(), lda, dU.data_ptr(), ldu)
+    Ud = dU.cpu().numpy()
+    assert np.array_equal(Ud[:, :n, :].transpose(1, 0, 2), ctx.dd_cholesky(A))
+    assert (Ud[:, n:, :] == 3.0).all()

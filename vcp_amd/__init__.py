@@ -69,6 +69,8 @@ SIGNATURES = {
     "vcphip_ddgemm_tn": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_inv_mid": [_vp, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_dd_solve": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _p_int],
+    "vcphip_dd_cholesky": [_vp, _c_int, _vp, _c_int, _vp, _c_int, _c_int],
+    "vcphip_dd_cholesky_dev": [_vp, _c_int, _vp, _c_int, _vp, _c_int, _c_int],
     "vcphip_dd_solve_dev": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _p_int],
     "vcphip_inv_mid_dev": [_vp, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_lss": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _p_dbl],
@@ -422,10 +424,24 @@ class Context:
 
     def dd_cholesky(self, A, itep=2):
         """upper U with U^T U = A to dd accuracy (pddblas::Cholesky,
-        vcp/pddblas.hpp:488-563; products on the GPU)."""
-        from . import ddrefine
+        vcp/pddblas.hpp:488-563), every step on the device
+        (vcphip_dd_cholesky).  Raises VcphipError: VCPHIP_ERR_ARGUMENT when
+        A is not symmetric, VCPHIP_ERR_SINGULAR when it is not positive
+        definite."""
+        n = A.shape[0]
+        if A.shape[1] != n:
+            raise ValueError("Cholesky: matrix must be square")
+        U = pairs(n, n)
+        if n == 0:
+            return U
+        pa, lda = _pair_ptr(A)
+        pu, ldu = _pair_ptr(U)
+        _check(lib().vcphip_dd_cholesky(self._h, n, pa, lda, pu, ldu, itep), "vcphip_dd_cholesky")
+        return U
 
-        return ddrefine.cholesky(self, A, itep)
+    def dd_cholesky_dev(self, n, dA, lda, dU, ldu, itep=2):
+        """device-pointer form of dd_cholesky()"""
+        _check(lib().vcphip_dd_cholesky_dev(self._h, n, _vp(dA), lda, _vp(dU), ldu, itep), "vcphip_dd_cholesky_dev")
 
     def dd_eigsym(self, A, itep=4):
         """(eigenvalues (n, 2), eigenvectors (n, n, 2)) of a symmetric dd

@@ -421,6 +421,63 @@ __global__ __launch_bounds__(256) void k_dd_add_d(int m, int n, const double* __
   }
 }
 
+// Cholesky refinement (pddblas::Cholesky, vcp/pddblas.hpp:490-563)
+// U = dd(Ud) on and above the diagonal, exact zeros below (:219-223 of
+// pdblas::Cholesky zero the strictly lower part of dpotrf's output)
+__global__ __launch_bounds__(256) void k_dd_from_upper(int n, const double* __restrict__ D, long ldd,
+                                                       double* __restrict__ U, long ldu) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)n * n) return;
+  const int i = (int)(idx % n), j = (int)(idx / n);
+  *reinterpret_cast<double2*>(U + 2 * (i + ldu * (long)j)) = make_double2(i <= j ? D[i + ldd * (long)j] : 0.0, 0.0);
+}
+
+// W = dd(triu(G, 1) + diag(G) / 2) (:531-545)
+__global__ __launch_bounds__(256) void k_dd_w_from_g(int n, const double* __restrict__ G, double* __restrict__ W) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)n * n) return;
+  const int i = (int)(idx % n), j = (int)(idx / n);
+  const double g = G[idx];
+  *reinterpret_cast<double2*>(W + 2 * idx) = make_double2(i < j ? g : (i == j ? g * 0.5 : 0.0), 0.0);
+}
+
+// U += F in double-double (:548-550), and U's strictly lower part set to
+// exact zeros when zero_lower (:553-558)
+__global__ __launch_bounds__(256) void k_dd_add_to(int n, const double* __restrict__ F, long ldf, double* __restrict__ U,
+                                                   long ldu, int zero_lower) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)n * n) return;
+  const int i = (int)(idx % n), j = (int)(idx / n);
+  double2* up = reinterpret_cast<double2*>(U + 2 * (i + ldu * (long)j));
+  if (zero_lower && i > j) {
+    *up = make_double2(0.0, 0.0);
+    return;
+  }
+  const double2 f = *reinterpret_cast<const double2*>(F + 2 * (i + ldf * (long)j));
+  const double2 u = *up;
+  double s, e, t, g;
+  two_sum(u.x, f.x, s, e);
+  two_sum(u.y, f.y, t, g);
+  e = e + t;
+  fast_two_sum(s, e, s, e);
+  e = e + g;
+  fast_two_sum(s, e, s, e);
+  *up = make_double2(s, e);
+}
+
+// flag[0] = 1 if A(i, j) != A(j, i) in either part for some i, j (the
+// reference's is_symmetric); the caller clears the flag first
+__global__ __launch_bounds__(256) void k_dd_symmetric(int n, const double* __restrict__ A, long lda,
+                                                      int* __restrict__ flag) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)n * n) return;
+  const int i = (int)(idx % n), j = (int)(idx / n);
+  if (i >= j) return;
+  const double2 a = *reinterpret_cast<const double2*>(A + 2 * (i + lda * (long)j));
+  const double2 b = *reinterpret_cast<const double2*>(A + 2 * (j + lda * (long)i));
+  if (!(a.x == b.x && a.y == b.y)) flag[0] = 1;
+}
+
 // out[0], out[1] = the maxima over nb blocks' partials
 __global__ __launch_bounds__(256) void k_max_pairs(int nb, const double* __restrict__ part, double* __restrict__ out) {
   __shared__ double sd[256], sx[256];
@@ -450,6 +507,32 @@ __global__ __launch_bounds__(256) void k_max_pairs(int nb, const double* __restr
 int vcphip_dd_hi(int m, int n, const double* X, long ldx, double* D, hipStream_t s) {
   if ((long)m * n == 0) return 0;
   hipLaunchKernelGGL(k_dd_hi, dim3((unsigned)(((long)m * n + 255) / 256)), dim3(256), 0, s, m, n, X, ldx, D);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+static inline dim3 grid_nn(int n) { return dim3((unsigned)(((long)n * n + 255) / 256)); }
+
+int vcphip_dd_from_upper(int n, const double* D, long ldd, double* U, long ldu, hipStream_t s) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_dd_from_upper, grid_nn(n), dim3(256), 0, s, n, D, ldd, U, ldu);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_dd_w_from_g(int n, const double* G, double* W, hipStream_t s) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_dd_w_from_g, grid_nn(n), dim3(256), 0, s, n, G, W);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_dd_add_to(int n, const double* F, long ldf, double* U, long ldu, int zero_lower, hipStream_t s) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_dd_add_to, grid_nn(n), dim3(256), 0, s, n, F, ldf, U, ldu, zero_lower);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_dd_symmetric(int n, const double* A, long lda, int* flag, hipStream_t s) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_dd_symmetric, grid_nn(n), dim3(256), 0, s, n, A, lda, flag);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

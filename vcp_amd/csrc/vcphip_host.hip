@@ -325,7 +325,7 @@ extern "C" const char* vcphip_strerror(int code) {
     case VCPHIP_ERR_UNSUPPORTED: return "unsupported size or form";
     case VCPHIP_ERR_RANGE: return "exponent range not supported";
     case VCPHIP_ERR_IO: return "matrix file I/O or format error";
-    case VCPHIP_ERR_SINGULAR: return "midpoint matrix is singular (zero pivot)";
+    case VCPHIP_ERR_SINGULAR: return "factorization failed: zero pivot or not positive definite";
     default: return "unknown error";
   }
 }
@@ -989,6 +989,70 @@ extern "C" int vcphip_dd_solve_dev(vcphip_ctx* c, int n, int nrhs, const double*
 }
 
 // ===========================================================================
+// dd Cholesky: pddblas::Cholesky (vcp/pddblas.hpp:488-563) on the device
+// ===========================================================================
+extern "C" int vcphip_dd_cholesky_dev(vcphip_ctx* c, int n, const double* A, int lda, double* U, int ldu, int itep) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (n < 0 || lda < (n > 1 ? n : 1) || ldu < (n > 1 ? n : 1)) return VCPHIP_ERR_DIMENSION;
+  if (n == 0) return VCPHIP_OK;
+  if (!A || !U || itep < 0) return VCPHIP_ERR_ARGUMENT;
+  if (n > VCPHIP_OZ_MAXK) return VCPHIP_ERR_UNSUPPORTED;
+  hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  // workspace: Ud and G (n x n doubles), P and W (n x n dd), flag / info
+  auto al = [](size_t bytes) { return (bytes + 255) & ~(size_t)255; };
+  const size_t nn = (size_t)n * n;
+  const size_t o_g = al(8 * nn), o_p = o_g + al(8 * nn), o_w = o_p + al(16 * nn), o_i = o_w + al(16 * nn);
+  void* base;
+  VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_DDS, o_i + 256, &base));
+  char* b8 = (char*)base;
+  double *Ud = (double*)b8, *G = (double*)(b8 + o_g), *P = (double*)(b8 + o_p), *W = (double*)(b8 + o_w);
+  rocblas_int* info = (rocblas_int*)(b8 + o_i);
+  int* flag = (int*)(b8 + o_i + 64);
+  auto fail = [&](int code) {
+    hipStreamSynchronize(s);
+    return code;
+  };
+  VCPHIP_TRY(blas_on_stream(c));
+  StageTimer T(c);
+  // the reference's is_symmetric (:491-493), then dpotrf of double(A) (:496-498)
+  if (hipMemsetAsync(flag, 0, sizeof(int), s) != hipSuccess || vcphip_dd_symmetric(n, A, lda, flag, s) ||
+      vcphip_dd_hi(n, n, A, lda, Ud, s) ||
+      rocsolver_dpotrf(c->blas, rocblas_fill_upper, n, Ud, n, info) != rocblas_status_success)
+    return fail(VCPHIP_ERR_DEVICE);
+  int h[2] = {0, 0};
+  if (hipMemcpyAsync(&h[0], flag, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemcpyAsync(&h[1], info, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return VCPHIP_ERR_DEVICE;
+  if (h[0]) return VCPHIP_ERR_ARGUMENT;    // the reference's domain_error
+  if (h[1]) return VCPHIP_ERR_SINGULAR;    // not positive definite: lapack_error("dpotrf")
+  if (vcphip_dd_from_upper(n, Ud, n, U, ldu, s)) return fail(VCPHIP_ERR_DEVICE);
+  T.mark("dd_chol_potrf");
+  const double one = 1.0;
+  for (int it = 0; it < itep; ++it) {
+    // E = A - U^T U with the dd product (:510-518)
+    int rc = dd_gemm_dev(c, true, n, n, n, U, ldu, U, ldu, P, n);
+    if (rc) return fail(rc);
+    // G = Ud^-T E Ud^-1 in double precision, Ud = double(U) (:520-529)
+    if (vcphip_dd_sub_from(n, n, A, lda, P, n, s) || vcphip_dd_hi(n, n, P, n, G, s) ||
+        vcphip_dd_hi(n, n, U, ldu, Ud, s) ||
+        rocblas_dtrsm(c->blas, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
+                      rocblas_diagonal_non_unit, n, n, &one, Ud, n, G, n) != rocblas_status_success ||
+        rocblas_dtrsm(c->blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                      rocblas_diagonal_non_unit, n, n, &one, Ud, n, G, n) != rocblas_status_success ||
+        vcphip_dd_w_from_g(n, G, W, s))
+      return fail(VCPHIP_ERR_DEVICE);
+    // U = U + W U (:531-550)
+    rc = dd_gemm_dev(c, false, n, n, n, W, n, U, ldu, P, n);
+    if (rc) return fail(rc);
+    if (vcphip_dd_add_to(n, P, n, U, ldu, 1, s)) return fail(VCPHIP_ERR_DEVICE);
+  }
+  T.mark("dd_chol_refine");
+  return hipStreamSynchronize(s) == hipSuccess ? VCPHIP_OK : VCPHIP_ERR_DEVICE;
+}
+
+// ===========================================================================
 // verified linear solve (imats::linearsolve, vcp/imats.hpp:292-362)
 // ===========================================================================
 extern "C" int vcphip_lss_dev(vcphip_ctx* c, int n, int nrhs, const double* IA, int lda, const double* Ib, int ldb,
@@ -1447,6 +1511,20 @@ extern "C" int vcphip_dd_solve(vcphip_ctx* c, int n, int nrhs, const double* A, 
   VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HC2, 16 * (size_t)n * nrhs, &px));
   VCPHIP_TRY(vcphip_dd_solve_dev(c, n, nrhs, dA, n, dB, n, (double*)px, n, itmax, iters));
   return to_host(c, (double*)px, x, ldx, n, nrhs, 16);
+}
+
+extern "C" int vcphip_dd_cholesky(vcphip_ctx* c, int n, const double* A, int lda, double* U, int ldu, int itep) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (n < 0 || lda < (n > 1 ? n : 1) || ldu < (n > 1 ? n : 1)) return VCPHIP_ERR_DIMENSION;
+  if (n == 0) return VCPHIP_OK;
+  if (!A || !U || itep < 0) return VCPHIP_ERR_ARGUMENT;
+  hipSetDevice(c->device);
+  double* dA;
+  void* pu;
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HA, A, lda, n, n, 16, &dA));
+  VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HC2, 16 * (size_t)n * n, &pu));
+  VCPHIP_TRY(vcphip_dd_cholesky_dev(c, n, dA, n, (double*)pu, n, itep));
+  return to_host(c, (double*)pu, U, ldu, n, n, 16);
 }
 
 extern "C" int vcphip_inv_mid(vcphip_ctx* c, int n, const double* IA, int lda, double* R, int ldr) {

@@ -121,6 +121,12 @@ int vcphip_dd_sub_from(int m, int n, const double* B, long ldb, double* R, long 
 constexpr int VCPHIP_DD_ADD_BLOCKS = 1024;
 int vcphip_dd_hi(int m, int n, const double* X, long ldx, double* D, hipStream_t s);
 int vcphip_dd_add_d(int m, int n, const double* d, double* X, long ldx, double* part, double* out, hipStream_t s);
+// dd Cholesky refinement (pddblas::Cholesky): U from dpotrf's upper part, W
+// from G, U += F (optionally zeroing U's lower part), the symmetry check
+int vcphip_dd_from_upper(int n, const double* D, long ldd, double* U, long ldu, hipStream_t s);
+int vcphip_dd_w_from_g(int n, const double* G, double* W, hipStream_t s);
+int vcphip_dd_add_to(int n, const double* F, long ldf, double* U, long ldu, int zero_lower, hipStream_t s);
+int vcphip_dd_symmetric(int n, const double* A, long lda, int* flag, hipStream_t s);
 int vcphip_gather_rows(const double* A, long lda, int m, int k, int s, double* out, hipStream_t st);
 int vcphip_ratio_max(const double* X, const double* Y, long total, double* part, int nblocks, hipStream_t st);
 int vcphip_fill_random(void* p, size_t bytes, uint32_t seed, hipStream_t st);
