@@ -50,6 +50,9 @@ def main():
     ap.add_argument("--no-dd", action="store_true")
     ap.add_argument("--dd-n", type=int, default=4096)
     ap.add_argument("--dd-steps", type=int, default=5)
+    ap.add_argument("--variant", type=int, default=0,
+                    help="int8 GEMM tile variant of the headline Ozaki product (0: the library default); "
+                         "variants are bit-identical, so only the time changes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this path runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -88,6 +91,8 @@ def main():
         del src, dst
 
     ctx = vcp_amd.Context(0)
+    if args.variant:
+        ctx.set_param(vcp_amd.PARAM_OZ_VARIANT, args.variant)
     beta, mods = vcp_amd.ozaki_config(n)
     nmod = len(mods)
 
@@ -277,7 +282,8 @@ def main():
         "dtype": "int8",
         "data": "synthetic",
         "config": {"workload": "interval GEMM n=%d, mid=+-m*2^e (m U[1,2), e U{-20..20}), rad=2^-45|mid|" % n,
-                   "n": n, "form": "ozaki-int8", "moduli": nmod, "beta": beta},
+                   "n": n, "form": "ozaki-int8", "moduli": nmod, "beta": beta,
+                   "tile_variant": args.variant or "default"},
         "plain_fp64_rurd": {"tflops": round(tf_pl, 3), "ms_per_step": round(t_pl * 1e3, 3),
                             "frac_fp64_peak": round(tf_pl / PEAK_FP64_TFLOPS, 4)},
         "speedup_vs_plain": round(t_pl / t_oz, 3),

@@ -734,17 +734,21 @@ __device__ __forceinline__ void oz_mfma_frag32(const OzFrag32<FM, FN>& f, v16i (
         acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(f.a[h][i], f.b[h][j], acc[i][j], 0, 0, 0);
 }
 
-template <int FM, int FN, int NST>
-__global__ __launch_bounds__(256) void k_oz_gemm5(int mpad, int npad, int kpad, int nmod, int nz,
-                                                  const int8_t* __restrict__ Aplanes,
-                                                  const int8_t* __restrict__ Bplanes, VcphipOzParams P,
-                                                  uint8_t* __restrict__ U, int32_t* __restrict__ Z) {
+// NW waves per workgroup in a 2 x (NW / 2) grid: with NW = 8 and 4 x 2
+// fragments (128 x 64 per wave, 128 accumulator registers) the same 256 x 256
+// workgroup tile runs two waves per SIMD, twice the loads in flight per CU.
+template <int FM, int FN, int NST, int NW = 4>
+__global__ __launch_bounds__(64 * NW) void k_oz_gemm5(int mpad, int npad, int kpad, int nmod, int nz,
+                                                      const int8_t* __restrict__ Aplanes,
+                                                      const int8_t* __restrict__ Bplanes, VcphipOzParams P,
+                                                      uint8_t* __restrict__ U, int32_t* __restrict__ Z) {
   constexpr int WM = 32 * FM, WN = 32 * FN;
-  constexpr int TMB = 2 * WM, TNB = 2 * WN;
+  constexpr int TMB = 2 * WM, TNB = (NW / 2) * WN;
   constexpr int SEGA = TMB / 16, SEGB = TNB / 16;
-  constexpr int SEGA_W = SEGA / 4, SEGB_W = SEGB / 4;
+  constexpr int SEGA_W = SEGA / NW, SEGB_W = SEGB / NW;
   constexpr int G = SEGA_W + SEGB_W;
-  static_assert(SEGA_W * 4 == SEGA && SEGB_W * 4 == SEGB, "tile/wave mismatch");
+  static_assert(NW == 4 || NW == 8, "wave count");
+  static_assert(SEGA_W * NW == SEGA && SEGB_W * NW == SEGB, "tile/wave mismatch");
   static_assert(NST >= 3 && (NST - 1) * G < 64, "vmcnt range");
   __shared__ __attribute__((aligned(16))) int8_t sA[NST][TMB * 64];
   __shared__ __attribute__((aligned(16))) int8_t sB[NST][TNB * 64];
@@ -1102,6 +1106,18 @@ int vcphip_oz_gemm(int mpad, int npad, int kpad, int nmod, int nrad, const int8_
                            Aplanes, Bplanes, P, U, Z);
       else
         hipLaunchKernelGGL((k_oz_gemm5<4, 4, 3>), dim3(tiles * nz), dim3(256), 0, s, mpad, npad, kpad, nmod, nz,
+                           Aplanes, Bplanes, P, U, Z);
+      break;
+    }
+    case 15:
+    case 16: {
+      if (kpad % 64) return -7;
+      const int tiles = (mpad / 256) * (npad / 256);
+      if (variant == 15)
+        hipLaunchKernelGGL((k_oz_gemm5<4, 2, 4, 8>), dim3(tiles * nz), dim3(512), 0, s, mpad, npad, kpad, nmod, nz,
+                           Aplanes, Bplanes, P, U, Z);
+      else
+        hipLaunchKernelGGL((k_oz_gemm5<4, 2, 3, 8>), dim3(tiles * nz), dim3(512), 0, s, mpad, npad, kpad, nmod, nz,
                            Aplanes, Bplanes, P, U, Z);
       break;
     }

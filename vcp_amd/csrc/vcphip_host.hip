@@ -412,7 +412,7 @@ extern "C" int vcphip_ctx_set_param(vcphip_ctx* c, int key, long long v) {
       c->oz_nmod = v;
       return VCPHIP_OK;
     case VCPHIP_PARAM_OZ_VARIANT:
-      if (v != 0 && (v < 6 || v > 14)) return VCPHIP_ERR_ARGUMENT;
+      if (v != 0 && (v < 6 || v > 16)) return VCPHIP_ERR_ARGUMENT;
       c->oz_variant = v;
       return VCPHIP_OK;
     case VCPHIP_PARAM_STAGE_TIMING:   // (re)enabling clears the totals
@@ -1673,7 +1673,7 @@ extern "C" int vcphip_residual_batched(vcphip_ctx* c, int count, const int* n, c
 // ===========================================================================
 namespace {
 
-constexpr int TUNE_VARIANTS[] = {6, 7, 8, 9, 10, 11, 12, 13, 14};
+constexpr int TUNE_VARIANTS[] = {6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16};
 
 // time the int8 MFMA stage of one Ozaki product shape with every tile variant
 // (operand planes of arbitrary content: the integer MFMA time does not depend
