@@ -350,10 +350,16 @@ def bench_resid(ctx, reps):
     flops = 6.0 * sum(float(n) ** 3 for n in ns)
     p50 = float(np.percentile(lat, 50))
     nb = nub.cpu().numpy()
+    # bytes the batched residual must move: R (8 n^2) and IA (16 n^2) read, C (16 n^2)
+    # written by the tiles, C read again by the row sums (16 n^2)
+    hbm = 56.0 * sum(float(n) ** 2 for n in ns)
     out = {"problems": len(ns), "n_min": int(min(ns)), "n_max": int(max(ns)), "graph_launches": reps,
            "p50_ms": round(p50, 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
-           "tflops_p50": round(flops / (p50 * 1e-3) / 1e12, 3), "all_normub_below_1": bool((nb < 1.0).all()),
-           "max_normub": float(nb.max())}
+           "tflops_p50": round(flops / (p50 * 1e-3) / 1e12, 3),
+           "frac_fp64_peak": round(flops / (p50 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4),
+           "hbm_algorithmic_bytes": hbm, "hbm_algorithmic_GBps_p50": round(hbm / (p50 * 1e-3) / 1e9, 1),
+           "frac_hbm_peak": round(hbm / (p50 * 1e-3) / 8e12, 4),
+           "all_normub_below_1": bool((nb < 1.0).all()), "max_normub": float(nb.max())}
     try:
         import oracle_ffi as O
 

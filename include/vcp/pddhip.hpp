@@ -154,9 +154,10 @@ class pddhip : public vcp::pddblas {
     this->v = std::move(U);
   }
 
-  // Symmetric eigenproblem (pddblas.hpp:566-606): dsyev of the high parts on
-  // the host, then max(itep, 2) Ogita-Aishima steps (:219-314) whose four
-  // products (A X, X^T (A X), X^T X, X E) are GPU dd products.
+  // Symmetric eigenproblem (pddblas.hpp:566-606), every step on the device
+  // (vcphip_dd_eigsym): dsyev of the high parts by rocSOLVER, then
+  // max(itep, 2) Ogita-Aishima steps (:219-314) with dd products and the
+  // elementwise double-double updates on the GPU.
   void eigsym(int itep = 4) override {
     vcp::pddblas V;
     this->eigsym(V, itep);
@@ -168,23 +169,19 @@ class pddhip : public vcp::pddblas {
       vcp::pddblas::eigsym(V, itep);
       return;
     }
-    vcp::pdblas Ad, Vd;
-    dd_to_d(*this, Ad);
-    Ad.eigsym(Vd);
     std::vector<kv::dd> X(nn), lam(an);
-    for (int q = 0; q < nn; ++q) X[q] = kv::dd(Vd.v[q]);
-    for (int i = 0; i < an; ++i) lam[i] = kv::dd(Ad.v[i + an * i]);
-    const int steps = itep < 2 ? 2 : itep;
-    for (int step = 0; step < steps; ++step) oa_step_gpu(this->v, nullptr, X, lam, an);
+    eig_result(vcphip_dd_eigsym(vcphip_policy::checked_context<E>(), an, pddhip_detail::raw(this->v), an,
+                                pddhip_detail::raw(lam), pddhip_detail::raw(X), an, itep),
+               "vcphip_dd_eigsym");
     V.zeros(an, an);
     V.v = std::move(X);
     this->zeros(an, an);
     for (int i = 0; i < an; ++i) this->v[i + an * i] = lam[i];
   }
 
-  // Generalized problem A x = lambda B x (pddblas.hpp:611-659): dsygv of the
-  // high parts on the host, then max(itep, 2) Ogita-Aishima steps with the
-  // B-inner product; their products are GPU dd products.
+  // Generalized problem A x = lambda B x (pddblas.hpp:611-659), every step on
+  // the device (vcphip_dd_eigsymge): dsygv of the high parts by rocSOLVER,
+  // then max(itep, 2) Ogita-Aishima steps with the B-inner product.
   void eigsymge(vcp::pddblas& B, int itep = 1) {
     vcp::pddblas V;
     this->eigsymge(B, V, itep);
@@ -201,15 +198,10 @@ class pddhip : public vcp::pddblas {
       vcp::pddblas::eigsymge(B, V, itep);
       return;
     }
-    vcp::pdblas Ad, Bd, Vd;
-    dd_to_d(*this, Ad);
-    dd_to_d(B, Bd);
-    Ad.eigsymge(Bd, Vd);
     std::vector<kv::dd> X(nn), lam(an);
-    for (int q = 0; q < nn; ++q) X[q] = kv::dd(Vd.v[q]);
-    for (int i = 0; i < an; ++i) lam[i] = kv::dd(Ad.v[i + an * i]);
-    const int steps = itep < 2 ? 2 : itep;
-    for (int step = 0; step < steps; ++step) oa_step_gpu(this->v, &B.v, X, lam, an);
+    eig_result(vcphip_dd_eigsymge(vcphip_policy::checked_context<E>(), an, pddhip_detail::raw(this->v), an,
+                                  pddhip_detail::raw(B.v), an, pddhip_detail::raw(lam), pddhip_detail::raw(X), an, itep),
+               "vcphip_dd_eigsymge");
     V.zeros(an, an);
     V.v = std::move(X);
     this->zeros(an, an);
@@ -217,55 +209,12 @@ class pddhip : public vcp::pddblas {
   }
 
  private:
-  // one Ogita-Aishima refinement of the eigenpairs (X columns, lam) of the
-  // symmetric n x n matrix A with the B-inner product (B = I when B is
-  // null), pddblas.hpp:219-314
-  static void oa_step_gpu(const std::vector<kv::dd>& A, const std::vector<kv::dd>* B, std::vector<kv::dd>& X,
-                          std::vector<kv::dd>& lam, int n) {
-    using pddhip_detail::raw;
-    const int nn = n * n;
-    vcphip_ctx* ctx = vcphip_policy::checked_context<E>();
-    std::vector<kv::dd> AX(nn), S(nn), R(nn), Ecorr(nn), XE(nn);
-    vcphip_policy::check<E>(vcphip_ddgemm(ctx, n, n, n, raw(A), n, raw(X), n, raw(AX), n), "vcphip_ddgemm");
-    vcphip_policy::check<E>(vcphip_ddgemm_ta(ctx, n, n, n, raw(X), n, raw(AX), n, raw(S), n), "vcphip_ddgemm_ta");
-    if (B) {   // R = X^T (B X)
-      vcphip_policy::check<E>(vcphip_ddgemm(ctx, n, n, n, raw(*B), n, raw(X), n, raw(XE), n), "vcphip_ddgemm");
-      vcphip_policy::check<E>(vcphip_ddgemm_ta(ctx, n, n, n, raw(X), n, raw(XE), n, raw(R), n), "vcphip_ddgemm_ta");
-    } else {   // R = X^T X
-      vcphip_policy::check<E>(vcphip_ddgemm_tn(ctx, n, n, raw(X), n, raw(R), n), "vcphip_ddgemm_tn");
-    }
-    for (int i = 0; i < n; ++i) lam[i] = S[i + n * i] / R[i + n * i];   // Rayleigh quotients
-    for (int q = 0; q < nn; ++q) R[q] = -R[q];                           // R = I - X^T X
-    for (int i = 0; i < n; ++i) R[i + n * i] += kv::dd(1.0);
-    // separation threshold delta = 2 (max off-diagonal row sum of S + ||R|| max|lam|)
-    double normR = 0.0, offS = 0.0, lmax = 0.0;
-    for (int i = 0; i < n; ++i) {
-      double rs = 0.0, ss = 0.0;
-      for (int j = 0; j < n; ++j) {
-        rs += std::fabs(R[i + n * j].a1);
-        if (j != i) ss += std::fabs(S[i + n * j].a1);
-      }
-      normR = rs > normR ? rs : normR;
-      offS = ss > offS ? ss : offS;
-      const double a = std::fabs(lam[i].a1);
-      lmax = a > lmax ? a : lmax;
-    }
-    const double delta = 2.0 * (offS + normR * lmax);
-    const kv::dd half(0.5);
-    for (int j = 0; j < n; ++j)
-      for (int i = 0; i < n; ++i) {
-        const int q = i + n * j;
-        if (i != j) {
-          const kv::dd gap = lam[j] - lam[i];
-          if (std::fabs(gap.a1) > delta) {
-            Ecorr[q] = (S[q] + lam[j] * R[q]) / gap;
-            continue;
-          }
-        }
-        Ecorr[q] = R[q] * half;
-      }
-    vcphip_policy::check<E>(vcphip_ddgemm(ctx, n, n, n, raw(X), n, raw(Ecorr), n, raw(XE), n), "vcphip_ddgemm");
-    for (int q = 0; q < nn; ++q) X[q] += XE[q];   // X += X E
+  // a failed dsyev / dsygv start is the reference's lapack_error
+  // (pddblas.hpp:576-579, :627-630); other codes map as everywhere else
+  static void eig_result(int rc, const char* routine) {
+    if (rc == VCPHIP_ERR_SINGULAR)
+      throw vcp::lapack_error(routine, VCPHIP_ERR_SINGULAR, "eigsym: the double-precision start failed");
+    vcphip_policy::check<E>(rc, routine);
   }
 
  public:

@@ -190,6 +190,24 @@ int vcphip_dd_solve_dev(vcphip_ctx* ctx, int n, int nrhs, const double* A, int l
  * double-double addition).  A and U are n x n dd pairs {hi, lo}. */
 int vcphip_dd_cholesky(vcphip_ctx* ctx, int n, const double* A, int lda, double* U, int ldu, int itep);
 int vcphip_dd_cholesky_dev(vcphip_ctx* ctx, int n, const double* A, int lda, double* U, int ldu, int itep);
+/* Eigenpairs of a symmetric dd matrix to dd accuracy, every step on the
+ * device: pddblas::eigsym (vcp/pddblas.hpp:566-606) and, with B symmetric
+ * positive definite, the generalized problem A x = lam B x of
+ * pddblas::eigsymge (:611-659).  The reference's symmetry checks
+ * (VCPHIP_ERR_ARGUMENT), then dsyev / dsygv of the high parts (rocSOLVER
+ * dsyevd / dsygvd; VCPHIP_ERR_SINGULAR when they fail, the reference's
+ * lapack_error), then max(itep, 2) Ogita-Aishima steps (oa_step, :219-314):
+ * A X, X^T (A X), X^T X or X^T (B X) as dd products, lam_i = S_ii / G_ii, the
+ * separation bound delta, the correction E and X += X E in double-double
+ * arithmetic.  lam receives n dd pairs (ascending order of the start), X the
+ * n x n eigenvectors (columns).  The reference's itep is 4 for eigsym and 1
+ * for eigsymge. */
+int vcphip_dd_eigsym(vcphip_ctx* ctx, int n, const double* A, int lda, double* lam, double* X, int ldx, int itep);
+int vcphip_dd_eigsym_dev(vcphip_ctx* ctx, int n, const double* A, int lda, double* lam, double* X, int ldx, int itep);
+int vcphip_dd_eigsymge(vcphip_ctx* ctx, int n, const double* A, int lda, const double* B, int ldb, double* lam,
+                       double* X, int ldx, int itep);
+int vcphip_dd_eigsymge_dev(vcphip_ctx* ctx, int n, const double* A, int lda, const double* B, int ldb, double* lam,
+                           double* X, int ldx, int itep);
 
 /* ---- batched verified residual: C_i = R_i * IA_i - I ---------------------- */
 /* For each problem i (square, size n[i]): C_i = R_i*IA_i - I as intervals and

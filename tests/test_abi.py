@@ -63,6 +63,8 @@ def test_null_context_is_rejected_without_device_work():
     assert L.vcphip_dd_solve_dev(None, 4, 1, None, 4, None, 4, None, 4, 10, None) == -2
     assert L.vcphip_dd_cholesky(None, 4, None, 4, None, 4, 2) == -2
     assert L.vcphip_dd_cholesky_dev(None, 4, None, 4, None, 4, 2) == -2
+    assert L.vcphip_dd_eigsym(None, 4, None, 4, None, None, 4, 4) == -2
+    assert L.vcphip_dd_eigsymge_dev(None, 4, None, 4, None, 4, None, None, 4, 1) == -2
     assert L.vcphip_inv_mid_dev(None, 4, None, 4, None, 4) == -2
     assert L.vcphip_ctx_create(0, None) == -2
 

@@ -324,3 +324,98 @@ def test_dd_cholesky_dev_leading_dimensions(ctx):
     Ud = dU.cpu().numpy()
     assert np.array_equal(Ud[:, :n, :].transpose(1, 0, 2), ctx.dd_cholesky(A))
     assert (Ud[:, n:, :] == 3.0).all()
+
+
+def _sym_dd(rng, n, spectrum):
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    S = Q @ np.diag(spectrum) @ Q.T
+    A = O.pairs(n, n)
+    A[..., 0] = (S + S.T) * 0.5
+    L = 2.0 ** -60 * rng.uniform(-1, 1, (n, n))
+    A[..., 1] = np.where(np.abs(L + L.T) * 0.5 <= 2.0 ** -53 * np.abs(A[..., 0]), (L + L.T) * 0.5, 0.0)
+    return A
+
+
+def _same_up_to_sign(X, Y, tol):
+    """columns of two dd eigenvector matrices agree up to a sign each"""
+    for j in range(X.shape[1]):
+        s = 1.0 if float(np.dot(X[:, j, 0], Y[:, j, 0])) >= 0 else -1.0
+        d = (X[:, j, 0] - s * Y[:, j, 0]) + (X[:, j, 1] - s * Y[:, j, 1])
+        if np.abs(d).max() > tol:
+            return False
+    return True
+
+
+def test_dd_eigsym_agrees_with_host_arrangement(ctx):
+    """the all-device eigensolver against the reference's arrangement
+    restated with host LAPACK and host double-double updates (ddrefine)"""
+    from vcp_amd import ddrefine
+
+    rng = np.random.default_rng(61)
+    n = 96
+    A = _sym_dd(rng, n, np.linspace(-4.0, 7.0, n) + 0.01 * rng.standard_normal(n))
+    lam, X = ctx.dd_eigsym(A)
+    lh, Xh = ddrefine.eigsym(ctx, A)
+    dl = (lam[:, 0] - lh[:, 0]) + (lam[:, 1] - lh[:, 1])
+    assert np.abs(dl).max() <= 2.0 ** -96 * np.abs(lh[:, 0]).max()
+    assert _same_up_to_sign(X, Xh, 2.0 ** -90)
+
+
+def test_dd_eigsym_at_n300_exact_residuals(ctx):
+    rng = np.random.default_rng(62)
+    n = 300
+    A = _sym_dd(rng, n, np.linspace(1.0, 50.0, n))
+    lam, X = ctx.dd_eigsym(A)
+    anorm = float(np.abs(A[..., 0]).sum(axis=1).max())
+    for c in (0, 151, 299):
+        for i in (0, 77, 299):
+            r = sum(val(A[i, t]) * val(X[t, c]) for t in range(n)) - val(lam[c]) * val(X[i, c])
+            assert abs(r) <= F(2) ** -94 * F(anorm)
+
+
+def test_dd_eigsymge_agrees_with_host_arrangement(ctx):
+    from vcp_amd import ddrefine
+
+    rng = np.random.default_rng(63)
+    n = 64
+    A = _sym_dd(rng, n, np.linspace(-3.0, 5.0, n))
+    G = rng.standard_normal((n, n))
+    Bm = G @ G.T / n + np.eye(n)
+    B = O.pairs(n, n)
+    B[..., 0] = (Bm + Bm.T) * 0.5
+    lam, X = ctx.dd_eigsymge(A, B)
+    lh, Xh = ddrefine.eigsymge(ctx, A, B)
+    dl = (lam[:, 0] - lh[:, 0]) + (lam[:, 1] - lh[:, 1])
+    assert np.abs(dl).max() <= 2.0 ** -94 * np.abs(lh[:, 0]).max()
+    assert _same_up_to_sign(X, Xh, 2.0 ** -88)
+
+
+def test_dd_eig_errors_and_dev_entry(ctx):
+    rng = np.random.default_rng(64)
+    n = 16
+    A = _sym_dd(rng, n, np.arange(1.0, n + 1))
+    N = O.pairs(n, n)
+    N[...] = A
+    N[3, 9, 0] += 1e-3   # not symmetric
+    with pytest.raises(vcp_amd.VcphipError) as e:
+        ctx.dd_eigsym(N)
+    assert e.value.code == -2
+    Bneg = O.pairs(n, n)
+    Bneg[..., 0] = -np.eye(n)   # not positive definite
+    with pytest.raises(vcp_amd.VcphipError) as e:
+        ctx.dd_eigsymge(A, Bneg)
+    assert e.value.code == -10
+    lda, ldx = 19, 21
+    dA = torch.zeros((n, lda, 2), dtype=torch.float64, device="cuda")
+    dA[:, :n, :] = torch.from_numpy(np.ascontiguousarray(A.transpose(1, 0, 2))).cuda()
+    dX = torch.full((n, ldx, 2), 9.0, dtype=torch.float64, device="cuda")
+    dl = torch.zeros((n, 2), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    L = vcp_amd.lib()
+    assert L.vcphip_dd_eigsym_dev(ctx._h, n, dA.data_ptr(), lda, dl.data_ptr(), dX.data_ptr(), ldx, 4) == 0
+    ctx.synchronize()
+    lam, X = ctx.dd_eigsym(A)
+    Xd = dX.cpu().numpy()
+    assert np.array_equal(dl.cpu().numpy(), lam)
+    assert np.array_equal(Xd[:, :n, :].transpose(1, 0, 2), X)
+    assert (Xd[:, n:, :] == 9.0).all()

@@ -70,6 +70,10 @@ SIGNATURES = {
     "vcphip_inv_mid": [_vp, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_dd_solve": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _p_int],
     "vcphip_dd_cholesky": [_vp, _c_int, _vp, _c_int, _vp, _c_int, _c_int],
+    "vcphip_dd_eigsym": [_vp, _c_int, _vp, _c_int, _vp, _vp, _c_int, _c_int],
+    "vcphip_dd_eigsym_dev": [_vp, _c_int, _vp, _c_int, _vp, _vp, _c_int, _c_int],
+    "vcphip_dd_eigsymge": [_vp, _c_int, _vp, _c_int, _vp, _c_int, _vp, _vp, _c_int, _c_int],
+    "vcphip_dd_eigsymge_dev": [_vp, _c_int, _vp, _c_int, _vp, _c_int, _vp, _vp, _c_int, _c_int],
     "vcphip_dd_cholesky_dev": [_vp, _c_int, _vp, _c_int, _vp, _c_int, _c_int],
     "vcphip_dd_solve_dev": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _p_int],
     "vcphip_inv_mid_dev": [_vp, _c_int, _vp, _c_int, _vp, _c_int],
@@ -446,17 +450,31 @@ class Context:
     def dd_eigsym(self, A, itep=4):
         """(eigenvalues (n, 2), eigenvectors (n, n, 2)) of a symmetric dd
         matrix (pddblas::eigsym, Ogita-Aishima refinement, vcp/pddblas.hpp:
-        219-314, 566-606; products on the GPU)."""
-        from . import ddrefine
-
-        return ddrefine.eigsym(self, A, itep)
+        219-314, 566-606), every step on the device (vcphip_dd_eigsym)."""
+        return self._dd_eig(A, None, itep)
 
     def dd_eigsymge(self, A, B, itep=1):
         """(eigenvalues, eigenvectors) of A x = lambda B x for symmetric dd A,
-        B (pddblas::eigsymge, vcp/pddblas.hpp:611-659; products on the GPU)."""
-        from . import ddrefine
+        B, B positive definite (pddblas::eigsymge, vcp/pddblas.hpp:611-659),
+        every step on the device (vcphip_dd_eigsymge)."""
+        return self._dd_eig(A, B, itep)
 
-        return ddrefine.eigsymge(self, A, B, itep)
+    def _dd_eig(self, A, B, itep):
+        n = A.shape[0]
+        if A.shape[1] != n or (B is not None and B.shape[:2] != (n, n)):
+            raise ValueError("eigsym: dimension mismatch")
+        lam, X = pairs(n, 1), pairs(n, n)
+        if n == 0:
+            return lam[:, 0, :], X
+        pa, lda = _pair_ptr(A)
+        px, ldx = _pair_ptr(X)
+        pl = lam.ctypes.data_as(_vp)
+        if B is None:
+            _check(lib().vcphip_dd_eigsym(self._h, n, pa, lda, pl, px, ldx, itep), "vcphip_dd_eigsym")
+        else:
+            pb, ldb = _pair_ptr(B)
+            _check(lib().vcphip_dd_eigsymge(self._h, n, pa, lda, pb, ldb, pl, px, ldx, itep), "vcphip_dd_eigsymge")
+        return np.ascontiguousarray(lam[:, 0, :]), X
 
     def ddgemm_tn(self, A):
         """C = A^T A for a double-double matrix (pddblas::mulltmm)."""

@@ -478,6 +478,153 @@ __global__ __launch_bounds__(256) void k_dd_symmetric(int n, const double* __res
   if (!(a.x == b.x && a.y == b.y)) flag[0] = 1;
 }
 
+// X = dd(D) for an m x n double matrix D (lo parts zero)
+__global__ __launch_bounds__(256) void k_dd_from_d(int m, int n, const double* __restrict__ D, long ldd,
+                                                   double* __restrict__ X, long ldx) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)m * n) return;
+  const int i = (int)(idx % m), j = (int)(idx / m);
+  *reinterpret_cast<double2*>(X + 2 * (i + ldx * (long)j)) = make_double2(D[i + ldd * (long)j], 0.0);
+}
+
+// ---- the elementwise part of the Ogita-Aishima step (pddblas::oa_step,
+// vcp/pddblas.hpp:219-314), in round-to-nearest double-double arithmetic:
+// TwoSum / FastTwoSum additions, the exact product error by FMA, and the
+// quotient with two correction steps
+VCPHIP_DEVINL void dd_add2(double xh, double xl, double yh, double yl, double& h, double& l) {
+  double s, e, t, f;
+  two_sum(xh, yh, s, e);
+  two_sum(xl, yl, t, f);
+  e = e + t;
+  fast_two_sum(s, e, s, e);
+  e = e + f;
+  fast_two_sum(s, e, h, l);
+}
+
+VCPHIP_DEVINL void two_prod(double a, double b, double& p, double& e) {
+  p = a * b;
+  e = __builtin_fma(a, b, -p);
+}
+
+VCPHIP_DEVINL void dd_mul2(double xh, double xl, double yh, double yl, double& h, double& l) {
+  double p, e;
+  two_prod(xh, yh, p, e);
+  e = e + (xh * yl + xl * yh);
+  fast_two_sum(p, e, h, l);
+}
+
+VCPHIP_DEVINL void dd_mul_d2(double xh, double xl, double d, double& h, double& l) {
+  double p, e;
+  two_prod(xh, d, p, e);
+  e = e + xl * d;
+  fast_two_sum(p, e, h, l);
+}
+
+VCPHIP_DEVINL void dd_div2(double xh, double xl, double yh, double yl, double& h, double& l) {
+  double ph, pl, rh, rl;
+  double q1 = xh / yh;
+  dd_mul_d2(yh, yl, q1, ph, pl);
+  dd_add2(xh, xl, -ph, -pl, rh, rl);
+  double q2 = rh / yh;
+  dd_mul_d2(yh, yl, q2, ph, pl);
+  dd_add2(rh, rl, -ph, -pl, rh, rl);
+  const double q3 = rh / yh;
+  fast_two_sum(q1, q2, q1, q2);
+  dd_add2(q1, q2, q3, 0.0, h, l);
+}
+
+// R = I - G, element (i, j)
+VCPHIP_DEVINL void oa_r(int i, int j, double2 g, double& rh, double& rl) {
+  if (i == j) dd_add2(-g.x, -g.y, 1.0, 0.0, rh, rl);
+  else {
+    rh = -g.x;
+    rl = -g.y;
+  }
+}
+
+// lam_i = S_ii / G_ii (:238-243)
+__global__ __launch_bounds__(256) void k_oa_lam(int n, const double* __restrict__ S, long lds,
+                                                const double* __restrict__ G, long ldg, double* __restrict__ lam) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double2 s = *reinterpret_cast<const double2*>(S + 2 * (i + lds * (long)i));
+  const double2 g = *reinterpret_cast<const double2*>(G + 2 * (i + ldg * (long)i));
+  double h, l;
+  dd_div2(s.x, s.y, g.x, g.y, h, l);
+  lam[2 * i] = h;
+  lam[2 * i + 1] = l;
+}
+
+// per row i: sum_j |R_ij.hi| and sum_{j != i} |S_ij.hi| (:253-263), added in
+// column order as the reference does; lanes take consecutive rows (coalesced)
+__global__ __launch_bounds__(256) void k_oa_rows(int n, const double* __restrict__ S, long lds,
+                                                 const double* __restrict__ G, long ldg, double* __restrict__ rows) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double sr = 0.0, ss = 0.0;
+  for (int j = 0; j < n; ++j) {
+    double rh, rl;
+    oa_r(i, j, *reinterpret_cast<const double2*>(G + 2 * (i + ldg * (long)j)), rh, rl);
+    sr = sr + __builtin_fabs(rh);
+    if (j != i) ss = ss + __builtin_fabs(S[2 * (i + lds * (long)j)]);
+  }
+  rows[2 * i] = sr;
+  rows[2 * i + 1] = ss;
+}
+
+// delta = 2 (offS + normR * lmax), normR = max_i sum_j |R_ij|, offS = max_i of
+// the off-diagonal row sums of |S|, lmax = max |lam| (one block)
+__global__ __launch_bounds__(256) void k_oa_delta(int n, const double* __restrict__ rows,
+                                                  const double* __restrict__ lam, double* __restrict__ delta) {
+  __shared__ double sr[256], ss[256], sl[256];
+  double mr = 0.0, ms = 0.0, ml = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    if (rows[2 * i] > mr) mr = rows[2 * i];
+    if (rows[2 * i + 1] > ms) ms = rows[2 * i + 1];
+    const double al = __builtin_fabs(lam[2 * i]);
+    if (al > ml) ml = al;
+  }
+  sr[threadIdx.x] = mr;
+  ss[threadIdx.x] = ms;
+  sl[threadIdx.x] = ml;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      if (sr[threadIdx.x + w] > sr[threadIdx.x]) sr[threadIdx.x] = sr[threadIdx.x + w];
+      if (ss[threadIdx.x + w] > ss[threadIdx.x]) ss[threadIdx.x] = ss[threadIdx.x + w];
+      if (sl[threadIdx.x + w] > sl[threadIdx.x]) sl[threadIdx.x] = sl[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) delta[0] = 2.0 * (ss[0] + sr[0] * sl[0]);
+}
+
+// E_ij = (S_ij + lam_j R_ij) / (lam_j - lam_i) where |lam_j - lam_i| > delta
+// and i != j, else R_ij / 2 (:264-300)
+__global__ __launch_bounds__(256) void k_oa_e(int n, const double* __restrict__ S, long lds,
+                                              const double* __restrict__ G, long ldg, const double* __restrict__ lam,
+                                              const double* __restrict__ delta, double* __restrict__ E) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)n * n) return;
+  const int i = (int)(idx % n), j = (int)(idx / n);
+  double rh, rl;
+  oa_r(i, j, *reinterpret_cast<const double2*>(G + 2 * (i + ldg * (long)j)), rh, rl);
+  const double ljh = lam[2 * j], ljl = lam[2 * j + 1];
+  double dh, dl;
+  dd_add2(ljh, ljl, -lam[2 * i], -lam[2 * i + 1], dh, dl);
+  double eh, el;
+  if (i != j && __builtin_fabs(dh) > delta[0]) {
+    const double2 sv = *reinterpret_cast<const double2*>(S + 2 * (i + lds * (long)j));
+    double mh, ml, nh, nl;
+    dd_mul2(ljh, ljl, rh, rl, mh, ml);
+    dd_add2(sv.x, sv.y, mh, ml, nh, nl);
+    dd_div2(nh, nl, dh, dl, eh, el);
+  } else {
+    dd_mul_d2(rh, rl, 0.5, eh, el);
+  }
+  *reinterpret_cast<double2*>(E + 2 * idx) = make_double2(eh, el);
+}
+
 // out[0], out[1] = the maxima over nb blocks' partials
 __global__ __launch_bounds__(256) void k_max_pairs(int nb, const double* __restrict__ part, double* __restrict__ out) {
   __shared__ double sd[256], sx[256];
@@ -533,6 +680,25 @@ int vcphip_dd_add_to(int n, const double* F, long ldf, double* U, long ldu, int 
 int vcphip_dd_symmetric(int n, const double* A, long lda, int* flag, hipStream_t s) {
   if (n <= 0) return 0;
   hipLaunchKernelGGL(k_dd_symmetric, grid_nn(n), dim3(256), 0, s, n, A, lda, flag);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_dd_from_d(int m, int n, const double* D, long ldd, double* X, long ldx, hipStream_t s) {
+  if ((long)m * n == 0) return 0;
+  hipLaunchKernelGGL(k_dd_from_d, dim3((unsigned)(((long)m * n + 255) / 256)), dim3(256), 0, s, m, n, D, ldd, X, ldx);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+// one Ogita-Aishima step's elementwise part: lam (2n), rows (2n) and delta (1)
+// are work arrays; E (n x n dd, leading dimension n) receives the correction
+int vcphip_oa_elementwise(int n, const double* S, long lds, const double* G, long ldg, double* lam, double* rows,
+                          double* delta, double* E, hipStream_t s) {
+  if (n <= 0) return 0;
+  const dim3 g1((unsigned)((n + 255) / 256));
+  hipLaunchKernelGGL(k_oa_lam, g1, dim3(256), 0, s, n, S, lds, G, ldg, lam);
+  hipLaunchKernelGGL(k_oa_rows, g1, dim3(256), 0, s, n, S, lds, G, ldg, rows);
+  hipLaunchKernelGGL(k_oa_delta, dim3(1), dim3(256), 0, s, n, rows, lam, delta);
+  hipLaunchKernelGGL(k_oa_e, grid_nn(n), dim3(256), 0, s, n, S, lds, G, ldg, lam, delta, E);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

@@ -1053,6 +1053,104 @@ extern "C" int vcphip_dd_cholesky_dev(vcphip_ctx* c, int n, const double* A, int
 }
 
 // ===========================================================================
+// dd symmetric eigenproblems: pddblas::eigsym / eigsymge (vcp/pddblas.hpp:
+// 219-314, 566-659) on the device
+// ===========================================================================
+namespace {
+
+// B == nullptr: A x = lam x; else A x = lam B x (B symmetric positive definite)
+int dd_eig_dev(vcphip_ctx* c, int n, const double* A, int lda, const double* B, int ldb, double* lam, double* X,
+               int ldx, int itep) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (n < 0 || lda < (n > 1 ? n : 1) || ldx < (n > 1 ? n : 1) || (B && ldb < (n > 1 ? n : 1)))
+    return VCPHIP_ERR_DIMENSION;
+  if (n == 0) return VCPHIP_OK;
+  if (!A || !lam || !X || itep < 0) return VCPHIP_ERR_ARGUMENT;
+  if (n > VCPHIP_OZ_MAXK) return VCPHIP_ERR_UNSUPPORTED;
+  hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  // workspace: AX, S, G (n x n dd each), B X for the generalized problem,
+  // then D and the tridiagonal work (n each), lam (dd), row sums, delta, flags
+  auto al = [](size_t bytes) { return (bytes + 255) & ~(size_t)255; };
+  const size_t nn = (size_t)n * n, m2 = al(16 * nn);
+  const size_t o_s = m2, o_g = 2 * m2, o_bx = 3 * m2, o_d = o_bx + (B ? m2 : 0);
+  const size_t o_e = o_d + al(8 * (size_t)n), o_lam = o_e + al(8 * (size_t)n), o_rows = o_lam + al(16 * (size_t)n);
+  const size_t o_misc = o_rows + al(16 * (size_t)n), total = o_misc + 256;
+  void* base;
+  VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_DDS, total, &base));
+  char* b8 = (char*)base;
+  double *AX = (double*)b8, *S = (double*)(b8 + o_s), *G = (double*)(b8 + o_g);
+  double* BX = B ? (double*)(b8 + o_bx) : nullptr;
+  double *D = (double*)(b8 + o_d), *Ew = (double*)(b8 + o_e), *lw = (double*)(b8 + o_lam);
+  double* rows = (double*)(b8 + o_rows);
+  double* delta = (double*)(b8 + o_misc);
+  rocblas_int* info = (rocblas_int*)(b8 + o_misc + 64);
+  int* flag = (int*)(b8 + o_misc + 128);
+  auto fail = [&](int code) {
+    hipStreamSynchronize(s);
+    return code;
+  };
+  VCPHIP_TRY(blas_on_stream(c));
+  StageTimer T(c);
+  // the reference's symmetry checks, then dsyev / dsygv of the high parts
+  // (:567-580, :612-630); Ad and Bd live in the AX and S buffers until the
+  // first step
+  double *Ad = AX, *Bd = S;
+  if (hipMemsetAsync(flag, 0, sizeof(int), s) != hipSuccess || vcphip_dd_symmetric(n, A, lda, flag, s) ||
+      (B && vcphip_dd_symmetric(n, B, ldb, flag, s)) || vcphip_dd_hi(n, n, A, lda, Ad, s) ||
+      (B && vcphip_dd_hi(n, n, B, ldb, Bd, s)))
+    return fail(VCPHIP_ERR_DEVICE);
+  const rocblas_status st =
+      B ? rocsolver_dsygvd(c->blas, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, n, Ad, n, Bd, n, D,
+                           Ew, info)
+        : rocsolver_dsyevd(c->blas, rocblas_evect_original, rocblas_fill_upper, n, Ad, n, D, Ew, info);
+  if (st != rocblas_status_success) return fail(VCPHIP_ERR_DEVICE);
+  int h[2] = {0, 0};
+  if (hipMemcpyAsync(&h[0], flag, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemcpyAsync(&h[1], info, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return VCPHIP_ERR_DEVICE;
+  if (h[0]) return VCPHIP_ERR_ARGUMENT;   // the reference's domain_error
+  if (h[1]) return VCPHIP_ERR_SINGULAR;   // no convergence, or B not positive definite: lapack_error
+  if (vcphip_dd_from_d(n, n, Ad, n, X, ldx, s) || vcphip_dd_from_d(n, 1, D, n, lam, n, s))
+    return fail(VCPHIP_ERR_DEVICE);
+  T.mark("dd_eig_start");
+  // max(itep, 2) Ogita-Aishima steps (:586-603, :640-656)
+  const int steps = itep > 2 ? itep : 2;
+  for (int it = 0; it < steps; ++it) {
+    int rc = dd_gemm_dev(c, false, n, n, n, A, lda, X, ldx, AX, n);                    // A X
+    if (!rc) rc = dd_gemm_dev(c, false, n, n, n, X, ldx, AX, n, S, n, true);           // X^T (A X)
+    if (!rc && B) rc = dd_gemm_dev(c, false, n, n, n, B, ldb, X, ldx, BX, n);          // B X
+    if (!rc) rc = B ? dd_gemm_dev(c, false, n, n, n, X, ldx, BX, n, G, n, true)         // X^T (B X)
+                    : dd_gemm_dev(c, true, n, n, n, X, ldx, X, ldx, G, n);             // X^T X
+    if (rc) return fail(rc);
+    // lam, the norms, delta and E (into AX); X += X E (X E into S)
+    if (vcphip_oa_elementwise(n, S, n, G, n, lw, rows, delta, AX, s)) return fail(VCPHIP_ERR_DEVICE);
+    rc = dd_gemm_dev(c, false, n, n, n, X, ldx, AX, n, S, n);
+    if (rc) return fail(rc);
+    if (vcphip_dd_add_to(n, S, n, X, ldx, 0, s)) return fail(VCPHIP_ERR_DEVICE);
+  }
+  // the eigenvalues of the last step (computed from X before its update)
+  if (hipMemcpyAsync(lam, lw, 16 * (size_t)n, hipMemcpyDeviceToDevice, s) != hipSuccess)
+    return fail(VCPHIP_ERR_DEVICE);
+  T.mark("dd_eig_refine");
+  return hipStreamSynchronize(s) == hipSuccess ? VCPHIP_OK : VCPHIP_ERR_DEVICE;
+}
+
+}  // namespace
+
+extern "C" int vcphip_dd_eigsym_dev(vcphip_ctx* c, int n, const double* A, int lda, double* lam, double* X, int ldx,
+                                    int itep) {
+  return dd_eig_dev(c, n, A, lda, nullptr, 0, lam, X, ldx, itep);
+}
+
+extern "C" int vcphip_dd_eigsymge_dev(vcphip_ctx* c, int n, const double* A, int lda, const double* B, int ldb,
+                                      double* lam, double* X, int ldx, int itep) {
+  if (c && n > 0 && !B) return VCPHIP_ERR_ARGUMENT;
+  return dd_eig_dev(c, n, A, lda, B, ldb, lam, X, ldx, itep);
+}
+
+// ===========================================================================
 // verified linear solve (imats::linearsolve, vcp/imats.hpp:292-362)
 // ===========================================================================
 extern "C" int vcphip_lss_dev(vcphip_ctx* c, int n, int nrhs, const double* IA, int lda, const double* Ib, int ldb,
@@ -1525,6 +1623,38 @@ extern "C" int vcphip_dd_cholesky(vcphip_ctx* c, int n, const double* A, int lda
   VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HC2, 16 * (size_t)n * n, &pu));
   VCPHIP_TRY(vcphip_dd_cholesky_dev(c, n, dA, n, (double*)pu, n, itep));
   return to_host(c, (double*)pu, U, ldu, n, n, 16);
+}
+
+namespace {
+int dd_eig_host(vcphip_ctx* c, int n, const double* A, int lda, const double* B, int ldb, double* lam, double* X,
+                int ldx, int itep) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (n < 0 || lda < (n > 1 ? n : 1) || ldx < (n > 1 ? n : 1) || (B && ldb < (n > 1 ? n : 1)))
+    return VCPHIP_ERR_DIMENSION;
+  if (n == 0) return VCPHIP_OK;
+  if (!A || !lam || !X || itep < 0) return VCPHIP_ERR_ARGUMENT;
+  hipSetDevice(c->device);
+  double *dA, *dB = nullptr;
+  void *px, *pl;
+  VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HA, A, lda, n, n, 16, &dA));
+  if (B) VCPHIP_TRY(to_dev(c, vcphip_ctx::WS_HB, B, ldb, n, n, 16, &dB));
+  VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HC2, 16 * (size_t)n * n, &px));
+  VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HR, 16 * (size_t)n, &pl));
+  VCPHIP_TRY(dd_eig_dev(c, n, dA, n, dB, n, (double*)pl, (double*)px, n, itep));
+  VCPHIP_TRY(to_host(c, (double*)pl, lam, n, n, 1, 16));
+  return to_host(c, (double*)px, X, ldx, n, n, 16);
+}
+}  // namespace
+
+extern "C" int vcphip_dd_eigsym(vcphip_ctx* c, int n, const double* A, int lda, double* lam, double* X, int ldx,
+                                int itep) {
+  return dd_eig_host(c, n, A, lda, nullptr, 0, lam, X, ldx, itep);
+}
+
+extern "C" int vcphip_dd_eigsymge(vcphip_ctx* c, int n, const double* A, int lda, const double* B, int ldb,
+                                  double* lam, double* X, int ldx, int itep) {
+  if (c && n > 0 && !B) return VCPHIP_ERR_ARGUMENT;
+  return dd_eig_host(c, n, A, lda, B, ldb, lam, X, ldx, itep);
 }
 
 extern "C" int vcphip_inv_mid(vcphip_ctx* c, int n, const double* IA, int lda, double* R, int ldr) {

@@ -127,6 +127,11 @@ int vcphip_dd_from_upper(int n, const double* D, long ldd, double* U, long ldu, 
 int vcphip_dd_w_from_g(int n, const double* G, double* W, hipStream_t s);
 int vcphip_dd_add_to(int n, const double* F, long ldf, double* U, long ldu, int zero_lower, hipStream_t s);
 int vcphip_dd_symmetric(int n, const double* A, long lda, int* flag, hipStream_t s);
+// dd eigensolvers (pddblas::eigsym / eigsymge): X = dd(D), and the elementwise
+// part of one Ogita-Aishima step (lam, row norms, delta, the correction E)
+int vcphip_dd_from_d(int m, int n, const double* D, long ldd, double* X, long ldx, hipStream_t s);
+int vcphip_oa_elementwise(int n, const double* S, long lds, const double* G, long ldg, double* lam, double* rows,
+                          double* delta, double* E, hipStream_t s);
 int vcphip_gather_rows(const double* A, long lda, int m, int k, int s, double* out, hipStream_t st);
 int vcphip_ratio_max(const double* X, const double* Y, long total, double* part, int nblocks, hipStream_t st);
 int vcphip_fill_random(void* p, size_t bytes, uint32_t seed, hipStream_t st);
