@@ -1,12 +1,14 @@
-"""Host side of pddblas's refinement algorithms with their dd products on the
-GPU: Cholesky (vcp/pddblas.hpp:488-563) and the Ogita-Aishima symmetric
-eigensolvers (oa_step :219-314, eigsym :566-606, eigsymge :611-659).  The
-double-precision starting points come from LAPACK on the host (scipy /
-numpy), as the reference's dpotrf / dsyev / dsygv do; every ozaki_acc product of the reference is a
-vcphip dd product (ddgemm, ddgemm_tn for X^T X, ddgemm_ta for X^T Y); the
-elementwise dd arithmetic between them is the round-to-nearest
-double-double arithmetic below (kv::dd's own operator bits are not
-available to pin against).
+"""pddblas's refinement algorithms in the reference's own arrangement:
+Cholesky (vcp/pddblas.hpp:488-563) and the Ogita-Aishima symmetric
+eigensolvers (oa_step :219-314, eigsym :566-606, eigsymge :611-659) with
+their LAPACK / BLAS steps on the host (scipy / numpy, as the reference's
+dpotrf / dtrsm / dsyev / dsygv) and every ozaki_acc product a vcphip dd
+product (ddgemm, ddgemm_tn for X^T X, ddgemm_ta for X^T Y).  The library's
+all-device entries (vcphip_dd_cholesky, vcphip_dd_eigsym / eigsymge) are
+checked against these; the elementwise dd arithmetic between the products
+is the round-to-nearest double-double arithmetic below, the same formulas
+as the device kernels' (kv::dd's own operator bits are not available to pin
+against).
 
 Arrays are (rows, cols, 2) AoS column-major pair arrays {hi, lo}."""
 import numpy as np
