@@ -47,6 +47,7 @@ def main():
     ap.add_argument("--no-tuned", action="store_true")
     ap.add_argument("--no-config2", action="store_true")
     ap.add_argument("--no-galerkin", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-dd", action="store_true")
     ap.add_argument("--dd-n", type=int, default=4096)
     ap.add_argument("--dd-steps", type=int, default=5)
@@ -267,6 +268,7 @@ def main():
     dd = None if args.no_dd else bench_dd(ctx, args.dd_n, args.dd_steps, not args.no_cpu)
     cfg2 = None if args.no_config2 else bench_config2(ctx)
     galerkin = None if args.no_galerkin else bench_galerkin(ctx)
+    batched = None if args.no_batched else bench_igemm_batched(ctx)
 
     line = {
         "metric": "verified interval GEMM FP64-equiv TFLOP/s n=8192: Ozaki-int8 vs FP64-RU/RD",
@@ -298,6 +300,7 @@ def main():
         "dd_gemm": dd,
         "plain_n4096": cfg2,
         "galerkin_uhphiphi": galerkin,
+        "igemm_batched": batched,
     }
     print(json.dumps(line), flush=True)
     ctx.close()
@@ -424,6 +427,59 @@ def bench_resid(ctx, reps):
                                   "all_verified": bool(max(gi) < 1.0)}
     log("batched verified solve %s" % lss)
     out["lss_batched"] = lss
+    return out
+
+
+def bench_igemm_batched(ctx, count=256, reps=20):
+    """count interval products n_p x n_p x n_p (n_p log-uniform in [64, 1024],
+    the batched-residual size distribution; midpoints U[-1,1], radii
+    2^-40 |mid|) in one graph-captured persistent launch, against the same
+    products launched one by one.  FP64 flops: 8 n^3 per product (four FMA
+    chains per output)."""
+    import numpy as np
+    import torch
+
+    from vcp_amd import workloads as W
+
+    ns = W.config5_sizes(count)
+    rng = np.random.default_rng(W.SEED0 + 61)
+    dev = torch.device("cuda:0")
+    log("generating %d batched interval products" % count)
+    dA = [torch.from_numpy(np.ascontiguousarray(W.uniform_intervals(rng, n, n, -40).transpose(1, 0, 2))).to(dev)
+          for n in ns]
+    dB = [torch.from_numpy(np.ascontiguousarray(W.uniform_intervals(rng, n, n, -40).transpose(1, 0, 2))).to(dev)
+          for n in ns]
+    dC = [torch.empty((n, n, 2), dtype=torch.float64, device=dev) for n in ns]
+    torch.cuda.synchronize()
+    args = (ns, ns, ns, [t.data_ptr() for t in dA], ns, [t.data_ptr() for t in dB], ns, [t.data_ptr() for t in dC], ns)
+    ctx.igemm_batched_dev(*args)   # builds the tile list and captures the graph
+    ctx.synchronize()
+    lat = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ctx.igemm_batched_dev(*args)
+        ctx.synchronize()
+        lat.append(time.perf_counter() - t0)
+    p50 = float(np.percentile(np.array(lat) * 1e3, 50))
+    # the same products one launch each
+    import vcp_amd
+
+    for i, n in enumerate(ns):
+        ctx.igemm_dev(vcp_amd.FORM_PLAIN, n, n, n, dA[i].data_ptr(), n, dB[i].data_ptr(), n, dC[i].data_ptr(), n)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        for i, n in enumerate(ns):
+            ctx.igemm_dev(vcp_amd.FORM_PLAIN, n, n, n, dA[i].data_ptr(), n, dB[i].data_ptr(), n, dC[i].data_ptr(), n)
+    ctx.synchronize()
+    t_single = (time.perf_counter() - t0) / 3 * 1e3
+    flops = 8.0 * sum(float(n) ** 3 for n in ns)
+    out = {"problems": count, "n_min": int(min(ns)), "n_max": int(max(ns)), "graph_launches": reps,
+           "p50_ms": round(p50, 4), "tflops_p50": round(flops / (p50 * 1e-3) / 1e12, 3),
+           "frac_fp64_peak": round(flops / (p50 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4),
+           "one_launch_per_product_ms": round(t_single, 4)}
+    log("batched interval products %s" % out)
+    del dA, dB, dC
     return out
 
 

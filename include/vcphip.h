@@ -224,6 +224,24 @@ int vcphip_residual_batched_dev(vcphip_ctx* ctx, int count, const int* n,
                                 const double* const* IA, const int* lda,
                                 double* const* C, const int* ldc, double* normub_dev);
 
+/* ---- batched interval products: C_p = IA_p * IB_p ------------------------ */
+/* Many interval products of uneven (m_p, n_p, k_p) (the Legendre-basis
+ * Galerkin blocks of a verification run) in ONE persistent launch: the plain
+ * form of pidblas::mulmm (vcp/pidblas.hpp:239-294) per problem, so each C_p
+ * is bit-identical to vcphip_igemm(VCPHIP_FORM_PLAIN).  Workgroups stride
+ * over a tile list sorted by problem size, largest first.  The _dev form
+ * captures the launch into a hipGraph on first use and replays it while the
+ * problem set (pointers, sizes, leading dimensions, stream) is unchanged.
+ * Pointer arrays are host arrays (of device pointers for the _dev form);
+ * all operands AoS {inf, sup}, column-major.  The _dev form is asynchronous;
+ * the host form synchronises. */
+int vcphip_igemm_batched(vcphip_ctx* ctx, int count, const int* m, const int* n, const int* k,
+                         const double* const* IA, const int* lda, const double* const* IB, const int* ldb,
+                         double* const* C, const int* ldc);
+int vcphip_igemm_batched_dev(vcphip_ctx* ctx, int count, const int* m, const int* n, const int* k,
+                             const double* const* IA, const int* lda, const double* const* IB, const int* ldb,
+                             double* const* C, const int* ldc);
+
 /* ---- approximate inverse of the midpoint matrix ---------------------------- */
 /* R = inv(mid(IA)), the step of imats::linearsolve before the verified ones
  * (vcp/imats.hpp:298-301: imid, then pdblas::inv, vcp/pdblas.hpp:180-204):

@@ -58,6 +58,8 @@ def test_null_context_is_rejected_without_device_work():
     assert L.vcphip_dd_residual(None, 4, 4, 4, None, 4, None, 4, None, 4, None, 4) == -2
     assert L.vcphip_lss_batched(None, 1, None, None, None, None, None, None, None, None, None, None, None) == -2
     assert L.vcphip_lss_batched_dev(None, 1, None, None, None, None, None, None, None, None, None, None, None) == -2
+    assert L.vcphip_igemm_batched(None, 1, None, None, None, None, None, None, None, None, None) == -2
+    assert L.vcphip_igemm_batched_dev(None, 1, None, None, None, None, None, None, None, None, None) == -2
     assert L.vcphip_inv_mid(None, 4, None, 4, None, 4) == -2
     assert L.vcphip_dd_solve(None, 4, 1, None, 4, None, 4, None, 4, 10, None) == -2
     assert L.vcphip_dd_solve_dev(None, 4, 1, None, 4, None, 4, None, 4, 10, None) == -2

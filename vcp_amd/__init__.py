@@ -67,6 +67,8 @@ SIGNATURES = {
     "vcphip_ddgemm": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_ddgemm_dev": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_ddgemm_tn": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int],
+    "vcphip_igemm_batched": [_vp, _c_int, _p_int, _p_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int],
+    "vcphip_igemm_batched_dev": [_vp, _c_int, _p_int, _p_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int],
     "vcphip_inv_mid": [_vp, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_dd_solve": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _p_int],
     "vcphip_dd_cholesky": [_vp, _c_int, _vp, _c_int, _vp, _c_int, _c_int],
@@ -500,6 +502,38 @@ class Context:
         _check(lib().vcphip_residual_batched(self._h, cnt, ns, rp, ldr, ap, lda, cp, ldc, nub),
                "vcphip_residual_batched")
         return Cs, list(nub)
+
+    def igemm_batched(self, IAs, IBs):
+        """[IA_p * IB_p] for interval matrices of uneven sizes in one
+        persistent launch (the plain form of pidblas::mulmm per problem,
+        bit-identical to igemm(..., FORM_PLAIN))."""
+        cnt = len(IAs)
+        if len(IBs) != cnt:
+            raise ValueError("igemm_batched: list lengths differ")
+        for A, B in zip(IAs, IBs):
+            if A.shape[1] != B.shape[0]:
+                raise ValueError("igemm_batched: dimension mismatch")
+        Cs = [pairs(A.shape[0], B.shape[1]) for A, B in zip(IAs, IBs)]
+        if cnt == 0:
+            return Cs
+        ia = lambda xs: (ctypes.c_int * cnt)(*xs)
+        vp = lambda xs: (_vp * cnt)(*xs)
+        _check(lib().vcphip_igemm_batched(self._h, cnt, ia([A.shape[0] for A in IAs]), ia([B.shape[1] for B in IBs]),
+                                          ia([A.shape[1] for A in IAs]),
+                                          vp([_pair_ptr(A)[0] for A in IAs]), ia([_pair_ptr(A)[1] for A in IAs]),
+                                          vp([_pair_ptr(B)[0] for B in IBs]), ia([_pair_ptr(B)[1] for B in IBs]),
+                                          vp([_pair_ptr(C)[0] for C in Cs]), ia([_pair_ptr(C)[1] for C in Cs])),
+               "vcphip_igemm_batched")
+        return Cs
+
+    def igemm_batched_dev(self, m, n, k, dA, lda, dB, ldb, dC, ldc):
+        """device-pointer form: lists of sizes, pointers and leading
+        dimensions; asynchronous on the context stream"""
+        cnt = len(m)
+        ia = lambda xs: (ctypes.c_int * cnt)(*xs)
+        vp = lambda xs: (_vp * cnt)(*[_vp(x) for x in xs])
+        _check(lib().vcphip_igemm_batched_dev(self._h, cnt, ia(m), ia(n), ia(k), vp(dA), ia(lda), vp(dB), ia(ldb),
+                                              vp(dC), ia(ldc)), "vcphip_igemm_batched_dev")
 
     def inv_mid(self, IA):
         """R = inv(mid(IA)) formed on the device (imid + pdblas::inv,

@@ -249,6 +249,22 @@ __global__ __launch_bounds__(NT, 3) void k_resid_tiles(int ntiles, const VcphipR
   }
 }
 
+// Many interval products C_p = IA_p * IB_p of uneven (m, n, k) in ONE
+// persistent launch (pidblas::mulmm, vcp/pidblas.hpp:239-294, per problem):
+// workgroups stride over a tile list sorted by problem size, each tile the
+// interval x interval form of k_drgemm, so every C_p is bit-identical to the
+// single product's.
+__global__ __launch_bounds__(NT, 2) void k_igemm_tiles(int ntiles, const VcphipResidTile* __restrict__ tiles,
+                                                        const VcphipIgemmProblem* __restrict__ probs) {
+  vcphip_mode_round_up();
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const VcphipResidTile tl = tiles[t];
+    const VcphipIgemmProblem p = probs[tl.problem];
+    dr_tile<true, true, false>(p.m, p.n, p.k, p.A, p.lda, p.B, p.ldb, p.C, p.ldc, nullptr, 0, tl.row0, tl.col0);
+    __syncthreads();
+  }
+}
+
 // row sums of |C| (upward, in column order: mats::norminf, vcp/mats.hpp:1583-1591)
 __global__ __launch_bounds__(256) void k_resid_rowsum(int nrows, const int* __restrict__ row_problem,
                                                       const VcphipResidProblem* __restrict__ probs,
@@ -571,6 +587,12 @@ int vcphip_launch_drsyrk(int interval, int k, int n, const double* A, long lda, 
   dim3 grid(nt * (nt + 1) / 2), block(NT);
   if (interval) hipLaunchKernelGGL((k_drsyrk<true>), grid, block, 0, s, k, n, A, lda, C, ldc);
   else hipLaunchKernelGGL((k_drsyrk<false>), grid, block, 0, s, k, n, A, lda, C, ldc);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_launch_igemm_tiles(int ntiles, const VcphipResidTile* tiles, const VcphipIgemmProblem* probs, int grid,
+                              hipStream_t s) {
+  if (ntiles > 0) hipLaunchKernelGGL(k_igemm_tiles, dim3(grid), dim3(NT), 0, s, ntiles, tiles, probs);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

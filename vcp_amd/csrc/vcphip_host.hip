@@ -33,7 +33,7 @@ struct vcphip_ctx {
   long long oz_nmod = 0, oz_variant = 0, timing = 0, dd_nmod = 0;
   // workspace slots (device memory, grown on demand)
   enum { WS_PA, WS_PB, WS_U, WS_Z, WS_SA, WS_SB, WS_HA, WS_HB, WS_HC, WS_HC2, WS_BATCH, WS_DD, WS_LSS, WS_HR, WS_LSSB,
-         WS_INV, WS_DDS, WS_N };
+         WS_INV, WS_DDS, WS_IGB, WS_N };
   void* ws[WS_N] = {};
   size_t ws_size[WS_N] = {};
   // stage timing: events recorded by every call since the last synchronize,
@@ -75,6 +75,15 @@ struct vcphip_ctx {
   // rocBLAS handle of the approximate inverses (rocSOLVER getrf/getri),
   // created on first use
   rocblas_handle blas = nullptr;
+  // batched interval products: descriptors of the last problem set and its graph
+  struct Igb {
+    std::vector<VcphipIgemmProblem> probs;
+    std::vector<VcphipResidTile> tiles;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int grid = 0;
+  } igb;
 };
 
 namespace {
@@ -369,6 +378,8 @@ extern "C" int vcphip_ctx_destroy(vcphip_ctx* c) {
     hipEventDestroy(c->ev_join);
   }
   if (c->blas) rocblas_destroy_handle(c->blas);
+  if (c->igb.exec) hipGraphExecDestroy(c->igb.exec);
+  if (c->igb.graph) hipGraphDestroy(c->igb.graph);
   hipStreamDestroy(c->own);
   delete c;
   return VCPHIP_OK;
@@ -680,6 +691,102 @@ extern "C" int vcphip_dd_residual_dev(vcphip_ctx* c, int m, int n, int k, const 
   const int rc = dd_gemm_dev(c, false, m, n, k, A, lda, X, ldx, R, ldr);  // R = A X
   if (rc) return rc;
   return check_launch(vcphip_dd_sub_from(m, n, B, ldb, R, ldr, c->stream));
+}
+
+// ===========================================================================
+// batched interval products C_p = IA_p * IB_p (pidblas::mulmm per problem, the
+// plain form): one persistent launch over a size-sorted tile list, captured
+// into a hipGraph and replayed while the problem set is unchanged
+// ===========================================================================
+static bool same_igemm(const VcphipIgemmProblem& a, const VcphipIgemmProblem& b) {
+  return a.A == b.A && a.B == b.B && a.C == b.C && a.lda == b.lda && a.ldb == b.ldb && a.ldc == b.ldc &&
+         a.m == b.m && a.n == b.n && a.k == b.k;
+}
+
+extern "C" int vcphip_igemm_batched_dev(vcphip_ctx* c, int count, const int* m, const int* n, const int* k,
+                                        const double* const* IA, const int* lda, const double* const* IB,
+                                        const int* ldb, double* const* C, const int* ldc) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (count < 0 || (count > 0 && (!m || !n || !k || !IA || !lda || !IB || !ldb || !C || !ldc)))
+    return VCPHIP_ERR_ARGUMENT;
+  if (count == 0) return VCPHIP_OK;
+  hipSetDevice(c->device);
+  std::vector<VcphipIgemmProblem> probs((size_t)count);
+  for (int i = 0; i < count; ++i) {
+    if (!dims_ok(m[i], n[i], k[i]) || lda[i] < (m[i] > 1 ? m[i] : 1) || ldb[i] < (k[i] > 1 ? k[i] : 1) ||
+        ldc[i] < (m[i] > 1 ? m[i] : 1))
+      return VCPHIP_ERR_DIMENSION;
+    if (m[i] > 0 && n[i] > 0 && (!C[i] || (k[i] > 0 && (!IA[i] || !IB[i])))) return VCPHIP_ERR_ARGUMENT;
+    probs[i] = VcphipIgemmProblem{IA[i], IB[i], C[i], lda[i], ldb[i], ldc[i], m[i], n[i], k[i], 0};
+  }
+  vcphip_ctx::Igb& g = c->igb;
+  bool same = g.exec && g.stream == c->stream && g.probs.size() == probs.size();
+  for (size_t i = 0; same && i < probs.size(); ++i) same = same_igemm(g.probs[i], probs[i]);
+  if (!same) {
+    // tile list, the most work first (longest-processing-time order)
+    std::vector<int> order((size_t)count);
+    for (int i = 0; i < count; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      return (double)probs[a].m * probs[a].n * probs[a].k > (double)probs[b].m * probs[b].n * probs[b].k;
+    });
+    g.tiles.clear();
+    for (int pi : order) {
+      const int tm = (probs[pi].m + VCPHIP_RESID_TILE - 1) / VCPHIP_RESID_TILE;
+      const int tn = (probs[pi].n + VCPHIP_RESID_TILE - 1) / VCPHIP_RESID_TILE;
+      for (int a = 0; a < tm; ++a)
+        for (int b = 0; b < tn; ++b) g.tiles.push_back(VcphipResidTile{pi, a * VCPHIP_RESID_TILE, b * VCPHIP_RESID_TILE, 0});
+    }
+    g.probs = probs;
+    const size_t bp = sizeof(VcphipIgemmProblem) * count, bt = sizeof(VcphipResidTile) * g.tiles.size();
+    const size_t o_t = (bp + 255) & ~(size_t)255;
+    void* base;
+    const int wrc = ws_get(c, vcphip_ctx::WS_IGB, o_t + bt + 256, &base);
+    if (wrc) {
+      g.probs.clear();
+      return wrc;
+    }
+    char* b8 = (char*)base;
+    if (hipMemcpyAsync(b8, g.probs.data(), bp, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        (bt && hipMemcpyAsync(b8 + o_t, g.tiles.data(), bt, hipMemcpyHostToDevice, c->stream) != hipSuccess)) {
+      g.probs.clear();
+      return VCPHIP_ERR_DEVICE;
+    }
+    if (g.exec) { hipGraphExecDestroy(g.exec); g.exec = nullptr; }
+    if (g.graph) { hipGraphDestroy(g.graph); g.graph = nullptr; }
+    g.stream = c->stream;
+    const int ntiles = (int)g.tiles.size();
+    g.grid = std::max(1, std::min(ntiles, 2 * (c->num_cu > 0 ? c->num_cu : 256)));
+    const VcphipResidTile* dt = (const VcphipResidTile*)(b8 + o_t);
+    const VcphipIgemmProblem* dp = (const VcphipIgemmProblem*)b8;
+    bool captured = false;
+    if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+      const int lrc = vcphip_launch_igemm_tiles(ntiles, dt, dp, g.grid, c->stream);
+      hipGraph_t gr = nullptr;
+      if (hipStreamEndCapture(c->stream, &gr) == hipSuccess && lrc == 0 && gr) {
+        if (hipGraphInstantiate(&g.exec, gr, nullptr, nullptr, 0) == hipSuccess) {
+          g.graph = gr;
+          captured = true;
+        } else {
+          hipGraphDestroy(gr);
+        }
+      } else if (gr) {
+        hipGraphDestroy(gr);
+      }
+    }
+    if (!captured) {
+      hipGetLastError();
+      g.exec = nullptr;
+      StageTimer T(c);
+      const int lrc = vcphip_launch_igemm_tiles(ntiles, dt, dp, g.grid, c->stream);
+      T.mark("igemm_batched");
+      g.probs.clear();
+      return check_launch(lrc);
+    }
+  }
+  StageTimer T(c);
+  if (hipGraphLaunch(g.exec, c->stream) != hipSuccess) return VCPHIP_ERR_DEVICE;
+  T.mark("igemm_batched");
+  return VCPHIP_OK;
 }
 
 static bool same_problem(const VcphipResidProblem& a, const VcphipResidProblem& b) {
@@ -1591,6 +1698,63 @@ extern "C" int vcphip_lss(vcphip_ctx* c, int n, int nrhs, const double* IA, int 
   if (rc != VCPHIP_OK && rc != VCPHIP_ERR_VERIFICATION && rc != VCPHIP_ERR_SINGULAR) return rc;
   VCPHIP_TRY(to_host(c, dX, X, ldx, n, nrhs, 16));
   return rc;
+}
+
+extern "C" int vcphip_igemm_batched(vcphip_ctx* c, int count, const int* m, const int* n, const int* k,
+                                    const double* const* IA, const int* lda, const double* const* IB, const int* ldb,
+                                    double* const* C, const int* ldc) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (count < 0 || (count > 0 && (!m || !n || !k || !IA || !lda || !IB || !ldb || !C || !ldc)))
+    return VCPHIP_ERR_ARGUMENT;
+  if (count == 0) return VCPHIP_OK;
+  for (int p = 0; p < count; ++p) {
+    if (!dims_ok(m[p], n[p], k[p]) || lda[p] < (m[p] > 1 ? m[p] : 1) || ldb[p] < (k[p] > 1 ? k[p] : 1) ||
+        ldc[p] < (m[p] > 1 ? m[p] : 1))
+      return VCPHIP_ERR_DIMENSION;
+    if (m[p] > 0 && n[p] > 0 && (!C[p] || (k[p] > 0 && (!IA[p] || !IB[p])))) return VCPHIP_ERR_ARGUMENT;
+  }
+  hipSetDevice(c->device);
+  // inputs packed into one staging buffer (A, B per problem), outputs into another
+  auto al = [](size_t bytes) { return (bytes + 255) & ~(size_t)255; };
+  std::vector<size_t> oi((size_t)count), ox((size_t)count);
+  size_t in_bytes = 0, out_bytes = 0;
+  for (int p = 0; p < count; ++p) {
+    oi[p] = in_bytes;
+    in_bytes += al(16 * (size_t)m[p] * k[p]) + al(16 * (size_t)k[p] * n[p]);
+    ox[p] = out_bytes;
+    out_bytes += al(16 * (size_t)m[p] * n[p]);
+  }
+  void *pin, *pout;
+  VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HA, in_bytes, &pin));
+  VCPHIP_TRY(ws_get(c, vcphip_ctx::WS_HC2, out_bytes, &pout));
+  std::vector<const double*> dA((size_t)count), dB((size_t)count);
+  std::vector<double*> dC((size_t)count);
+  std::vector<int> dlda((size_t)count), dldb((size_t)count), dldc((size_t)count);
+  hipStream_t s = c->stream;
+  for (int p = 0; p < count; ++p) {
+    char* q = (char*)pin + oi[p];
+    dA[p] = (const double*)q;
+    dB[p] = (const double*)(q + al(16 * (size_t)m[p] * k[p]));
+    dC[p] = (double*)((char*)pout + ox[p]);
+    dlda[p] = m[p] > 1 ? m[p] : 1;
+    dldb[p] = k[p] > 1 ? k[p] : 1;
+    dldc[p] = m[p] > 1 ? m[p] : 1;
+    if ((m[p] > 0 && k[p] > 0 &&
+         hipMemcpy2DAsync((void*)dA[p], 16 * (size_t)m[p], IA[p], 16 * (size_t)lda[p], 16 * (size_t)m[p], k[p],
+                          hipMemcpyHostToDevice, s) != hipSuccess) ||
+        (k[p] > 0 && n[p] > 0 &&
+         hipMemcpy2DAsync((void*)dB[p], 16 * (size_t)k[p], IB[p], 16 * (size_t)ldb[p], 16 * (size_t)k[p], n[p],
+                          hipMemcpyHostToDevice, s) != hipSuccess))
+      return VCPHIP_ERR_DEVICE;
+  }
+  VCPHIP_TRY(vcphip_igemm_batched_dev(c, count, m, n, k, dA.data(), dlda.data(), dB.data(), dldb.data(), dC.data(),
+                                      dldc.data()));
+  for (int p = 0; p < count; ++p)
+    if (m[p] > 0 && n[p] > 0 &&
+        hipMemcpy2DAsync(C[p], 16 * (size_t)ldc[p], dC[p], 16 * (size_t)m[p], 16 * (size_t)m[p], n[p],
+                         hipMemcpyDeviceToHost, s) != hipSuccess)
+      return VCPHIP_ERR_DEVICE;
+  return hipStreamSynchronize(s) == hipSuccess ? VCPHIP_OK : VCPHIP_ERR_DEVICE;
 }
 
 extern "C" int vcphip_dd_solve(vcphip_ctx* c, int n, int nrhs, const double* A, int lda, const double* b, int ldb,

@@ -30,6 +30,17 @@ struct VcphipResidTile {
   int problem, row0, col0, pad;
 };
 #define VCPHIP_RESID_TILE 64
+// batched interval products C_p = IA_p * IB_p (the plain form), tiles of
+// VCPHIP_RESID_TILE x VCPHIP_RESID_TILE
+struct VcphipIgemmProblem {
+  const double* A;   // interval m x k, AoS {inf, sup}, column-major
+  const double* B;   // interval k x n
+  double* C;         // interval m x n
+  long lda, ldb, ldc;
+  int m, n, k, pad;
+};
+int vcphip_launch_igemm_tiles(int ntiles, const VcphipResidTile* tiles, const VcphipIgemmProblem* probs, int grid,
+                              hipStream_t s);
 int vcphip_launch_resid(int ntiles, const VcphipResidTile* tiles, const VcphipResidProblem* probs, int nrows,
                         const int* row_problem, double* rowsum, int count, double* normub, int grid,
                         hipStream_t s);
