@@ -1,9 +1,12 @@
 // C++ test of the policy core (include/vcp/vcphip_policy_core.hpp) through the
 // C ABI: operand-type dispatch of the reference's BLAS wrappers, status ->
-// exception mapping, the per-thread context, and one interval product per form
-// checked bit for bit against the CPU oracle.
+// exception mapping, the per-thread context, one interval product per form
+// and a batched product checked bit for bit against the CPU oracle, and the
+// device-side solvers (verified solve with R formed by rocSOLVER, dd solve)
+// in a process that loads ROCm's own rocSOLVER / rocBLAS.
 //   ./test_policy_core          logic only (no GPU)
 //   ./test_policy_core gpu      also the device part
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -96,6 +99,66 @@ void test_gpu() {
     thrown = true;
   }
   EXPECT(thrown);
+
+  // batched interval products: each C_p is the single plain product
+  {
+    const int m2 = 70, k2 = 13, n2 = 40;
+    std::vector<double> A2(2 * m2 * k2), B2(2 * k2 * n2), C2(2 * m2 * n2), C1(2 * m * n), R2(2 * m2 * n2);
+    fill(A2);
+    fill(B2);
+    const int ms[2] = {m, m2}, ns[2] = {n, n2}, ks[2] = {k, k2}, las[2] = {m, m2}, lbs[2] = {k, k2};
+    const double* As[2] = {A.data(), A2.data()};
+    const double* Bs[2] = {B.data(), B2.data()};
+    double* Cs[2] = {C1.data(), C2.data()};
+    check<Errors>(vcphip_igemm_batched(ctx, 2, ms, ns, ks, As, las, Bs, lbs, Cs, las), "batched");
+    orc_pidblas_mulmm(m, n, k, A.data(), m, B.data(), k, R.data(), m);
+    orc_pidblas_mulmm(m2, n2, k2, A2.data(), m2, B2.data(), k2, R2.data(), m2);
+    EXPECT(C1 == R && C2 == R2);
+  }
+
+  // the verified solve with R formed on the device (rocSOLVER in this
+  // process's own ROCm libraries), against the oracle given that R:
+  // tridiag(-1, 2, -1) of order 15 with b = 1..15, whose solution is dyadic
+  {
+    const int N = 15;
+    std::vector<double> T(2 * N * N, 0.0), b(2 * N), X(2 * N), Xo(2 * N), Rd(N * N);
+    for (int i = 0; i < N; ++i) {
+      for (int j = 0; j < N; ++j) {
+        const double v = (i == j) ? 2.0 : ((i - j == 1 || j - i == 1) ? -1.0 : 0.0);
+        T[2 * (i + N * j)] = T[2 * (i + N * j) + 1] = v;
+      }
+      b[2 * i] = b[2 * i + 1] = i + 1.0;
+    }
+    double g = 1.0, go = 1.0;
+    check<Errors>(vcphip_lss(ctx, N, 1, T.data(), N, b.data(), N, nullptr, N, X.data(), N, &g), "lss");
+    check<Errors>(vcphip_inv_mid(ctx, N, T.data(), N, Rd.data(), N), "inv_mid");
+    EXPECT(orc_lss(N, 1, T.data(), N, b.data(), N, Rd.data(), N, Xo.data(), N, &go) == 0);
+    EXPECT(g == go && X == Xo && g < 1e-12);
+    for (int i = 0; i < N; ++i) {   // x_i = sum_j min(i, j) (16 - max(i, j)) / 16 * j, 1-based
+      double x = 0.0;
+      for (int j = 1; j <= N; ++j) x += (double)(std::min(i + 1, j) * (N + 1 - std::max(i + 1, j))) / (N + 1) * j;
+      EXPECT(X[2 * i] <= x && x <= X[2 * i + 1]);
+    }
+  }
+
+  // the dd solve: the residual of its solution, formed by the oracle
+  {
+    const int N = 9;
+    std::vector<double> Ad(2 * N * N), bd(2 * N), xd(2 * N), rd(2 * N);
+    for (int i = 0; i < N * N; ++i) {
+      Ad[2 * i] = u(rng) + ((i % (N + 1)) == 0 ? 4.0 : 0.0);
+      Ad[2 * i + 1] = 0.0;
+    }
+    for (int i = 0; i < N; ++i) {
+      bd[2 * i] = u(rng);
+      bd[2 * i + 1] = 0.0;
+    }
+    int rounds = 0;
+    check<Errors>(vcphip_dd_solve(ctx, N, 1, Ad.data(), N, bd.data(), N, xd.data(), N, 10, &rounds), "dd_solve");
+    EXPECT(rounds >= 2 && rounds <= 6);
+    orc_dd_residual(N, 1, N, Ad.data(), N, xd.data(), N, bd.data(), N, rd.data(), N, 31);
+    for (int i = 0; i < N; ++i) EXPECT(std::fabs(rd[2 * i]) <= 1e-28);
+  }
 }
 }  // namespace
 
