@@ -14,9 +14,13 @@
  * changes the host rounding mode at all).
  *
  * Entry points without a suffix take HOST pointers and block until the
- * result is in host memory.  The *_dev variants take DEVICE pointers, are
- * asynchronous on the context's stream and allocate nothing but the
- * context's cached workspace.
+ * result is in host memory.  The *_dev variants take DEVICE pointers and are
+ * asynchronous on the context's stream, except the solvers whose comments
+ * say they synchronise (they need a norm, a pivot check or a stopping test
+ * on the host).  Device memory comes from the context's cached workspace;
+ * the rocSOLVER / rocBLAS calls of the inverses and dd refinements may also
+ * allocate their handle's own.  A context is used by one host thread at a
+ * time.
  *
  * Every function returns VCPHIP_OK (0) or a negative VCPHIP_ERR_* code;
  * vcphip_strerror() names it.  vcp/error.hpp:42-114 is the exception
