@@ -1048,8 +1048,11 @@ int vcphip_oz_split_cols(const double* B, long ldb, int k, int n, int npad, int 
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-// tile variants: 1 = 256x256 (8x8 fragments per wave), 2 = 128x128 (4x4),
-// 3 = 128x256 (4x8), 4 = 256x256 with 64-byte stages
+// tile variants (all bit-identical): 6 / 7 = k_oz_gemm3 256 x 256 / 128 x 256;
+// 8 / 9 / 10 = k_oz_gemm4 256 x 256 with 4 / 3 stages, 128 x 256 with 4;
+// 13 / 14 = k_oz_gemm4 256 x 256 with the refill loads spread over the
+// MFMAs (4 / 3 stages); 11 / 12 = k_oz_gemm5 (32 x 32 x 32 MFMA, 4 waves,
+// 4 / 3 stages); 15 / 16 = k_oz_gemm5 with 8 waves of 128 x 64 (4 / 3 stages)
 int vcphip_oz_gemm(int mpad, int npad, int kpad, int nmod, int nrad, const int8_t* Aplanes,
                    const int8_t* Bplanes, const VcphipOzParams& P, uint8_t* U, int32_t* Z, int variant,
                    hipStream_t s) {
