@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include <rocblas/rocblas.h>
@@ -75,6 +76,10 @@ struct vcphip_ctx {
   // rocBLAS handle of the approximate inverses (rocSOLVER getrf/getri),
   // created on first use
   rocblas_handle blas = nullptr;
+  // two pinned host buffers (hipHostMalloc) that stage the large copies of
+  // the host-pointer entry points, and the events of their last DMA
+  void* pin[2] = {nullptr, nullptr};
+  hipEvent_t pin_ev[2] = {nullptr, nullptr};
   // batched interval products: descriptors of the last problem set and its graph
   struct Igb {
     std::vector<VcphipIgemmProblem> probs;
@@ -378,6 +383,10 @@ extern "C" int vcphip_ctx_destroy(vcphip_ctx* c) {
     hipEventDestroy(c->ev_join);
   }
   if (c->blas) rocblas_destroy_handle(c->blas);
+  for (int h = 0; h < 2; ++h) {
+    if (c->pin_ev[h]) hipEventDestroy(c->pin_ev[h]);
+    if (c->pin[h]) hipHostFree(c->pin[h]);
+  }
   if (c->igb.exec) hipGraphExecDestroy(c->igb.exec);
   if (c->igb.graph) hipGraphDestroy(c->igb.graph);
   hipStreamDestroy(c->own);
@@ -1492,15 +1501,81 @@ extern "C" int vcphip_uhphiphi_dev(vcphip_ctx* c, int m, int npts, const double*
 namespace {
 
 // copy a column-major matrix of `rows` x `cols` elements of `esize` bytes
+// Large host <-> device copies go through two pinned buffers of PIN_BYTES:
+// the host threads pack a block of columns into one buffer while the DMA of
+// the previous block runs from the other.  Copies from pageable memory run
+// at a fifth of the PCIe rate otherwise.
+constexpr size_t PIN_BYTES = (size_t)128 << 20;
+constexpr size_t PIN_MIN = (size_t)8 << 20;   // below this, one direct copy
+
+bool pin_ready(vcphip_ctx* c) {
+  for (int h = 0; h < 2; ++h) {
+    if (!c->pin[h] && hipHostMalloc(&c->pin[h], PIN_BYTES, hipHostMallocDefault) != hipSuccess) {
+      hipGetLastError();
+      c->pin[h] = nullptr;
+      return false;
+    }
+    if (!c->pin_ev[h] && hipEventCreateWithFlags(&c->pin_ev[h], hipEventDisableTiming) != hipSuccess) {
+      hipGetLastError();
+      c->pin_ev[h] = nullptr;
+      return false;
+    }
+  }
+  return true;
+}
+
+// columns [0, ncols) of `rowb` bytes each, from stride `sld` to stride `dld`
+// (bytes), split over host threads
+void copy_cols(char* dst, size_t dld, const char* src, size_t sld, size_t rowb, int ncols) {
+  const size_t total = rowb * (size_t)ncols;
+  unsigned nt = std::thread::hardware_concurrency();
+  nt = nt < 1 ? 1 : (nt > 8 ? 8 : nt);   // eight memcpy streams already outrun PCIe
+  const bool dense = (dld == rowb && sld == rowb);
+  if (total < ((size_t)4 << 20) || nt == 1 || (!dense && ncols < 2)) {
+    if (dense) memcpy(dst, src, total);
+    else
+      for (int j = 0; j < ncols; ++j) memcpy(dst + dld * j, src + sld * j, rowb);
+    return;
+  }
+  std::vector<std::thread> pool;
+  pool.reserve(nt);
+  if (dense) {   // one contiguous block, split by bytes
+    for (unsigned t = 0; t < nt; ++t) {
+      const size_t b0 = total * t / nt, b1 = total * (t + 1) / nt;
+      pool.emplace_back([=]() { memcpy(dst + b0, src + b0, b1 - b0); });
+    }
+  } else {
+    if ((int)nt > ncols) nt = (unsigned)ncols;
+    for (unsigned t = 0; t < nt; ++t) {
+      const int j0 = (int)((long)ncols * t / nt), j1 = (int)((long)ncols * (t + 1) / nt);
+      pool.emplace_back([=]() {
+        for (int j = j0; j < j1; ++j) memcpy(dst + dld * j, src + sld * j, rowb);
+      });
+    }
+  }
+  for (auto& th : pool) th.join();
+}
+
 int to_dev(vcphip_ctx* c, int slot, const double* h, int ld, int rows, int cols, size_t esize, double** d) {
   void* p;
   const size_t bytes = esize * (size_t)(rows > 0 ? rows : 1) * (size_t)(cols > 0 ? cols : 1);
   int rc = ws_get(c, slot, bytes, &p);
   if (rc) return rc;
   *d = (double*)p;
-  if (rows > 0 && cols > 0) {
-    if (hipMemcpy2DAsync(p, esize * rows, h, esize * ld, esize * rows, cols, hipMemcpyHostToDevice, c->stream) !=
-        hipSuccess)
+  if (rows <= 0 || cols <= 0) return VCPHIP_OK;
+  const size_t rowb = esize * (size_t)rows;
+  if (bytes < PIN_MIN || rowb > PIN_BYTES || !pin_ready(c)) {
+    if (hipMemcpy2DAsync(p, rowb, h, esize * ld, rowb, cols, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+      return VCPHIP_ERR_DEVICE;
+    return VCPHIP_OK;
+  }
+  const int per = (int)(PIN_BYTES / rowb);
+  for (int j0 = 0, i = 0; j0 < cols; j0 += per, ++i) {
+    const int nc = std::min(per, cols - j0), b = i & 1;
+    if (hipEventSynchronize(c->pin_ev[b]) != hipSuccess) return VCPHIP_ERR_DEVICE;   // its last DMA is done
+    copy_cols((char*)c->pin[b], rowb, (const char*)h + esize * (size_t)ld * j0, esize * (size_t)ld, rowb, nc);
+    if (hipMemcpyAsync((char*)p + rowb * j0, c->pin[b], rowb * nc, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipEventRecord(c->pin_ev[b], c->stream) != hipSuccess)
       return VCPHIP_ERR_DEVICE;
   }
   return VCPHIP_OK;
@@ -1508,9 +1583,35 @@ int to_dev(vcphip_ctx* c, int slot, const double* h, int ld, int rows, int cols,
 
 int to_host(vcphip_ctx* c, const double* d, double* h, int ld, int rows, int cols, size_t esize) {
   if (rows > 0 && cols > 0) {
-    if (hipMemcpy2DAsync(h, esize * ld, d, esize * rows, esize * rows, cols, hipMemcpyDeviceToHost, c->stream) !=
-        hipSuccess)
-      return VCPHIP_ERR_DEVICE;
+    const size_t rowb = esize * (size_t)rows, bytes = rowb * (size_t)cols;
+    if (bytes < PIN_MIN || rowb > PIN_BYTES || !pin_ready(c)) {
+      if (hipMemcpy2DAsync(h, esize * ld, d, rowb, rowb, cols, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        return VCPHIP_ERR_DEVICE;
+    } else {
+      // two blocks in flight: block i lands in buffer i % 2; the host threads
+      // copy it out, then block i + 2 is issued into the same buffer (stream
+      // order keeps it behind any earlier DMA from that buffer)
+      const int per = (int)(PIN_BYTES / rowb), nblk = (cols + per - 1) / per;
+      auto issue = [&](int i) {
+        const int j0 = i * per, nc = std::min(per, cols - j0), b = i & 1;
+        return hipMemcpyAsync(c->pin[b], (const char*)d + rowb * j0, rowb * nc, hipMemcpyDeviceToHost, c->stream) ==
+                   hipSuccess &&
+               hipEventRecord(c->pin_ev[b], c->stream) == hipSuccess;
+      };
+      if (!issue(0) || (nblk > 1 && !issue(1))) {
+        hipStreamSynchronize(c->stream);
+        return VCPHIP_ERR_DEVICE;
+      }
+      for (int i = 0; i < nblk; ++i) {
+        const int j0 = i * per, nc = std::min(per, cols - j0), b = i & 1;
+        if (hipEventSynchronize(c->pin_ev[b]) != hipSuccess) return VCPHIP_ERR_DEVICE;
+        copy_cols((char*)h + esize * (size_t)ld * j0, esize * (size_t)ld, (const char*)c->pin[b], rowb, rowb, nc);
+        if (i + 2 < nblk && !issue(i + 2)) {
+          hipStreamSynchronize(c->stream);
+          return VCPHIP_ERR_DEVICE;
+        }
+      }
+    }
   }
   if (hipStreamSynchronize(c->stream) != hipSuccess) return VCPHIP_ERR_DEVICE;
   return VCPHIP_OK;
