@@ -19,8 +19,10 @@
  * say they synchronise (they need a norm, a pivot check or a stopping test
  * on the host).  Device memory comes from the context's cached workspace;
  * the rocSOLVER / rocBLAS calls of the inverses and dd refinements may also
- * allocate their handle's own.  A context is used by one host thread at a
- * time.
+ * allocate their handle's own.  Host-pointer copies above 8 MiB are staged
+ * through two pinned host buffers of 128 MiB that the context allocates on
+ * first use and holds until it is destroyed.  A context is used by one host
+ * thread at a time.
  *
  * Every function returns VCPHIP_OK (0) or a negative VCPHIP_ERR_* code;
  * vcphip_strerror() names it.  vcp/error.hpp:42-114 is the exception
