@@ -1556,6 +1556,106 @@ void copy_cols(char* dst, size_t dld, const char* src, size_t sld, size_t rowb, 
   for (auto& th : pool) th.join();
 }
 
+// Many host matrices into one packed device region (the batched entry
+// points): piece q is ncols columns of rowb bytes at source stride sld, to be
+// stored dense at byte offset doff of `dev`.  Offsets increase with q and do
+// not overlap.  Large inputs are packed into the pinned buffers, a device
+// range at a time, one DMA per range; below PIN_MIN, one copy per piece.
+struct HostPiece {
+  const char* src;
+  size_t sld, rowb;
+  int ncols;
+  size_t doff;
+};
+
+int pieces_to_dev(vcphip_ctx* c, char* dev, const std::vector<HostPiece>& pieces) {
+  size_t total = 0;
+  for (const auto& q : pieces) total += q.rowb * (size_t)q.ncols;
+  bool pinned = total >= PIN_MIN && pin_ready(c);
+  for (const auto& q : pieces) pinned = pinned && q.rowb * (size_t)q.ncols <= PIN_BYTES;
+  if (!pinned) {
+    for (const auto& q : pieces)
+      if (q.ncols > 0 && q.rowb > 0 &&
+          hipMemcpy2DAsync(dev + q.doff, q.rowb, q.src, q.sld, q.rowb, q.ncols, hipMemcpyHostToDevice, c->stream) !=
+              hipSuccess)
+        return VCPHIP_ERR_DEVICE;
+    return VCPHIP_OK;
+  }
+  size_t i = 0;
+  int blk = 0;
+  while (i < pieces.size()) {
+    // the device range [start, end) of the pieces that fit one buffer
+    const size_t start = pieces[i].doff;
+    size_t j = i, end = start;
+    while (j < pieces.size() && pieces[j].doff + pieces[j].rowb * (size_t)pieces[j].ncols - start <= PIN_BYTES) {
+      end = pieces[j].doff + pieces[j].rowb * (size_t)pieces[j].ncols;
+      ++j;
+    }
+    const int b = blk & 1;
+    if (hipEventSynchronize(c->pin_ev[b]) != hipSuccess) return VCPHIP_ERR_DEVICE;
+    for (size_t q = i; q < j; ++q)
+      copy_cols((char*)c->pin[b] + (pieces[q].doff - start), pieces[q].rowb, pieces[q].src, pieces[q].sld,
+                pieces[q].rowb, pieces[q].ncols);
+    if (hipMemcpyAsync(dev + start, c->pin[b], end - start, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipEventRecord(c->pin_ev[b], c->stream) != hipSuccess)
+      return VCPHIP_ERR_DEVICE;
+    i = j;
+    ++blk;
+  }
+  return VCPHIP_OK;
+}
+
+// the reverse: the packed device region to the host pieces (dst, stride);
+// synchronises the stream before returning
+int pieces_to_host(vcphip_ctx* c, const char* dev, const std::vector<HostPiece>& pieces) {
+  size_t total = 0;
+  for (const auto& q : pieces) total += q.rowb * (size_t)q.ncols;
+  bool pinned = total >= PIN_MIN && pin_ready(c);
+  for (const auto& q : pieces) pinned = pinned && q.rowb * (size_t)q.ncols <= PIN_BYTES;
+  if (!pinned) {
+    for (const auto& q : pieces)
+      if (q.ncols > 0 && q.rowb > 0 &&
+          hipMemcpy2DAsync((void*)q.src, q.sld, dev + q.doff, q.rowb, q.rowb, q.ncols, hipMemcpyDeviceToHost,
+                           c->stream) != hipSuccess)
+        return VCPHIP_ERR_DEVICE;
+    return hipStreamSynchronize(c->stream) == hipSuccess ? VCPHIP_OK : VCPHIP_ERR_DEVICE;
+  }
+  // ranges as above; range r lands in buffer r % 2, two in flight
+  std::vector<std::pair<size_t, size_t>> range;   // piece index range [first, last)
+  for (size_t i = 0; i < pieces.size();) {
+    const size_t start = pieces[i].doff;
+    size_t j = i;
+    while (j < pieces.size() && pieces[j].doff + pieces[j].rowb * (size_t)pieces[j].ncols - start <= PIN_BYTES) ++j;
+    range.emplace_back(i, j);
+    i = j;
+  }
+  auto issue = [&](size_t r) {
+    const size_t start = pieces[range[r].first].doff;
+    const HostPiece& last = pieces[range[r].second - 1];
+    const size_t end = last.doff + last.rowb * (size_t)last.ncols;
+    const int b = (int)(r & 1);
+    return hipMemcpyAsync(c->pin[b], dev + start, end - start, hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
+           hipEventRecord(c->pin_ev[b], c->stream) == hipSuccess;
+  };
+  if ((range.size() > 0 && !issue(0)) || (range.size() > 1 && !issue(1))) {
+    hipStreamSynchronize(c->stream);
+    return VCPHIP_ERR_DEVICE;
+  }
+  for (size_t r = 0; r < range.size(); ++r) {
+    const int b = (int)(r & 1);
+    if (hipEventSynchronize(c->pin_ev[b]) != hipSuccess) return VCPHIP_ERR_DEVICE;
+    const size_t start = pieces[range[r].first].doff;
+    for (size_t q = range[r].first; q < range[r].second; ++q)
+      copy_cols((char*)pieces[q].src, pieces[q].sld, (const char*)c->pin[b] + (pieces[q].doff - start),
+                pieces[q].rowb, pieces[q].rowb, pieces[q].ncols);
+    if (r + 2 < range.size() && !issue(r + 2)) {
+      hipStreamSynchronize(c->stream);
+      return VCPHIP_ERR_DEVICE;
+    }
+  }
+  return hipStreamSynchronize(c->stream) == hipSuccess ? VCPHIP_OK : VCPHIP_ERR_DEVICE;
+}
+
 int to_dev(vcphip_ctx* c, int slot, const double* h, int ld, int rows, int cols, size_t esize, double** d) {
   void* p;
   const size_t bytes = esize * (size_t)(rows > 0 ? rows : 1) * (size_t)(cols > 0 ? cols : 1);
@@ -1831,7 +1931,7 @@ extern "C" int vcphip_igemm_batched(vcphip_ctx* c, int count, const int* m, cons
   std::vector<const double*> dA((size_t)count), dB((size_t)count);
   std::vector<double*> dC((size_t)count);
   std::vector<int> dlda((size_t)count), dldb((size_t)count), dldc((size_t)count);
-  hipStream_t s = c->stream;
+  std::vector<HostPiece> in, out;
   for (int p = 0; p < count; ++p) {
     char* q = (char*)pin + oi[p];
     dA[p] = (const double*)q;
@@ -1840,22 +1940,18 @@ extern "C" int vcphip_igemm_batched(vcphip_ctx* c, int count, const int* m, cons
     dlda[p] = m[p] > 1 ? m[p] : 1;
     dldb[p] = k[p] > 1 ? k[p] : 1;
     dldc[p] = m[p] > 1 ? m[p] : 1;
-    if ((m[p] > 0 && k[p] > 0 &&
-         hipMemcpy2DAsync((void*)dA[p], 16 * (size_t)m[p], IA[p], 16 * (size_t)lda[p], 16 * (size_t)m[p], k[p],
-                          hipMemcpyHostToDevice, s) != hipSuccess) ||
-        (k[p] > 0 && n[p] > 0 &&
-         hipMemcpy2DAsync((void*)dB[p], 16 * (size_t)k[p], IB[p], 16 * (size_t)ldb[p], 16 * (size_t)k[p], n[p],
-                          hipMemcpyHostToDevice, s) != hipSuccess))
-      return VCPHIP_ERR_DEVICE;
+    if (m[p] > 0 && k[p] > 0)
+      in.push_back(HostPiece{(const char*)IA[p], 16 * (size_t)lda[p], 16 * (size_t)m[p], k[p], oi[p]});
+    if (k[p] > 0 && n[p] > 0)
+      in.push_back(HostPiece{(const char*)IB[p], 16 * (size_t)ldb[p], 16 * (size_t)k[p], n[p],
+                             oi[p] + al(16 * (size_t)m[p] * k[p])});
+    if (m[p] > 0 && n[p] > 0)
+      out.push_back(HostPiece{(const char*)C[p], 16 * (size_t)ldc[p], 16 * (size_t)m[p], n[p], ox[p]});
   }
+  VCPHIP_TRY(pieces_to_dev(c, (char*)pin, in));
   VCPHIP_TRY(vcphip_igemm_batched_dev(c, count, m, n, k, dA.data(), dlda.data(), dB.data(), dldb.data(), dC.data(),
                                       dldc.data()));
-  for (int p = 0; p < count; ++p)
-    if (m[p] > 0 && n[p] > 0 &&
-        hipMemcpy2DAsync(C[p], 16 * (size_t)ldc[p], dC[p], 16 * (size_t)m[p], 16 * (size_t)m[p], n[p],
-                         hipMemcpyDeviceToHost, s) != hipSuccess)
-      return VCPHIP_ERR_DEVICE;
-  return hipStreamSynchronize(s) == hipSuccess ? VCPHIP_OK : VCPHIP_ERR_DEVICE;
+  return pieces_to_host(c, (const char*)pout, out);
 }
 
 extern "C" int vcphip_dd_solve(vcphip_ctx* c, int n, int nrhs, const double* A, int lda, const double* b, int ldb,
@@ -1968,6 +2064,7 @@ extern "C" int vcphip_lss_batched(vcphip_ctx* c, int count, const int* n, const 
   std::vector<const double*> dA((size_t)count), dB((size_t)count), dR((size_t)count);
   std::vector<double*> dX((size_t)count);
   std::vector<int> pn((size_t)count);
+  std::vector<HostPiece> in, out;
   for (int p = 0; p < count; ++p) {
     const size_t np = (size_t)n[p], nr = np * nrhs[p];
     char* q = (char*)pin + oi[p];
@@ -1976,23 +2073,17 @@ extern "C" int vcphip_lss_batched(vcphip_ctx* c, int count, const int* n, const 
     dR[p] = has_r(p) ? (const double*)(q + 16 * np * np + 16 * nr) : nullptr;
     dX[p] = (double*)((char*)pout + ox[p]);
     pn[p] = n[p];
-    hipStream_t s = c->stream;
-    if (hipMemcpy2DAsync((void*)dA[p], 16 * np, IA[p], 16 * (size_t)lda[p], 16 * np, np, hipMemcpyHostToDevice, s) !=
-            hipSuccess ||
-        hipMemcpy2DAsync((void*)dB[p], 16 * np, Ib[p], 16 * (size_t)ldb[p], 16 * np, nrhs[p], hipMemcpyHostToDevice,
-                         s) != hipSuccess ||
-        (has_r(p) && hipMemcpy2DAsync((void*)dR[p], 8 * np, R[p], 8 * (size_t)ldr[p], 8 * np, np,
-                                      hipMemcpyHostToDevice, s) != hipSuccess))
-      return VCPHIP_ERR_DEVICE;
+    in.push_back(HostPiece{(const char*)IA[p], 16 * (size_t)lda[p], 16 * np, n[p], oi[p]});
+    in.push_back(HostPiece{(const char*)Ib[p], 16 * (size_t)ldb[p], 16 * np, nrhs[p], oi[p] + 16 * np * np});
+    if (has_r(p))
+      in.push_back(HostPiece{(const char*)R[p], 8 * (size_t)ldr[p], 8 * np, n[p], oi[p] + 16 * np * np + 16 * nr});
+    out.push_back(HostPiece{(const char*)X[p], 16 * (size_t)ldx[p], 16 * np, nrhs[p], ox[p]});
   }
+  VCPHIP_TRY(pieces_to_dev(c, (char*)pin, in));
   const int rc = vcphip_lss_batched_dev(c, count, n, nrhs, dA.data(), pn.data(), dB.data(), pn.data(), dR.data(),
                                         pn.data(), dX.data(), pn.data(), normub);
   if (rc != VCPHIP_OK && rc != VCPHIP_ERR_VERIFICATION && rc != VCPHIP_ERR_SINGULAR) return rc;
-  for (int p = 0; p < count; ++p)
-    if (hipMemcpy2DAsync(X[p], 16 * (size_t)ldx[p], dX[p], 16 * (size_t)n[p], 16 * (size_t)n[p], nrhs[p],
-                         hipMemcpyDeviceToHost, c->stream) != hipSuccess)
-      return VCPHIP_ERR_DEVICE;
-  if (hipStreamSynchronize(c->stream) != hipSuccess) return VCPHIP_ERR_DEVICE;
+  VCPHIP_TRY(pieces_to_host(c, (const char*)pout, out));
   return rc;
 }
 
