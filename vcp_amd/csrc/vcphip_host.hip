@@ -1568,6 +1568,41 @@ struct HostPiece {
   size_t doff;
 };
 
+// pieces [i, j) between their host memory and a pinned block that mirrors
+// the device range from byte `start`; the pieces are spread over host
+// threads by bytes (each piece is copied by one thread)
+void copy_pieces(const std::vector<HostPiece>& pieces, size_t i, size_t j, char* block, size_t start, bool to_block) {
+  auto one = [&](size_t q) {
+    const HostPiece& p = pieces[q];
+    if (to_block) copy_cols(block + (p.doff - start), p.rowb, p.src, p.sld, p.rowb, p.ncols);
+    else copy_cols((char*)p.src, p.sld, block + (p.doff - start), p.rowb, p.rowb, p.ncols);
+  };
+  size_t bytes = 0;
+  for (size_t q = i; q < j; ++q) bytes += pieces[q].rowb * (size_t)pieces[q].ncols;
+  unsigned nt = std::thread::hardware_concurrency();
+  nt = nt < 1 ? 1 : (nt > 8 ? 8 : nt);
+  if (bytes < ((size_t)4 << 20) || nt == 1 || j - i < 2) {
+    for (size_t q = i; q < j; ++q) one(q);
+    return;
+  }
+  // thread t takes the pieces whose first byte falls in its share
+  std::vector<std::thread> pool;
+  size_t q = i, acc = 0;
+  for (unsigned t = 0; t < nt && q < j; ++t) {
+    const size_t limit = bytes * (t + 1) / nt;
+    const size_t q0 = q;
+    while (q < j && (acc < limit || t + 1 == nt)) {
+      acc += pieces[q].rowb * (size_t)pieces[q].ncols;
+      ++q;
+    }
+    if (q > q0)
+      pool.emplace_back([&one, q0, q]() {
+        for (size_t r = q0; r < q; ++r) one(r);
+      });
+  }
+  for (auto& th : pool) th.join();
+}
+
 int pieces_to_dev(vcphip_ctx* c, char* dev, const std::vector<HostPiece>& pieces) {
   size_t total = 0;
   for (const auto& q : pieces) total += q.rowb * (size_t)q.ncols;
@@ -1593,9 +1628,7 @@ int pieces_to_dev(vcphip_ctx* c, char* dev, const std::vector<HostPiece>& pieces
     }
     const int b = blk & 1;
     if (hipEventSynchronize(c->pin_ev[b]) != hipSuccess) return VCPHIP_ERR_DEVICE;
-    for (size_t q = i; q < j; ++q)
-      copy_cols((char*)c->pin[b] + (pieces[q].doff - start), pieces[q].rowb, pieces[q].src, pieces[q].sld,
-                pieces[q].rowb, pieces[q].ncols);
+    copy_pieces(pieces, i, j, (char*)c->pin[b], start, true);
     if (hipMemcpyAsync(dev + start, c->pin[b], end - start, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
         hipEventRecord(c->pin_ev[b], c->stream) != hipSuccess)
       return VCPHIP_ERR_DEVICE;
@@ -1645,9 +1678,7 @@ int pieces_to_host(vcphip_ctx* c, const char* dev, const std::vector<HostPiece>&
     const int b = (int)(r & 1);
     if (hipEventSynchronize(c->pin_ev[b]) != hipSuccess) return VCPHIP_ERR_DEVICE;
     const size_t start = pieces[range[r].first].doff;
-    for (size_t q = range[r].first; q < range[r].second; ++q)
-      copy_cols((char*)pieces[q].src, pieces[q].sld, (const char*)c->pin[b] + (pieces[q].doff - start),
-                pieces[q].rowb, pieces[q].rowb, pieces[q].ncols);
+    copy_pieces(pieces, range[r].first, range[r].second, (char*)c->pin[b], start, false);
     if (r + 2 < range.size() && !issue(r + 2)) {
       hipStreamSynchronize(c->stream);
       return VCPHIP_ERR_DEVICE;
