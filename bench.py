@@ -687,7 +687,80 @@ def bench_dd(ctx, n, steps, with_cpu):
     res = float(dC[..., 0].abs().max().item()) if rc == 0 else None
     out["dd_inv"] = {"n": n, "ms": round(min(ts) * 1e3, 3), "rounds": int(rounds), "max_abs_I_minus_AX": res}
     log("dd inverse %s" % out["dd_inv"])
-    del dA, dB, dC, dX, eye
+    del dA, dB, dX
+    out.update(bench_dd_factor(ctx, n, dC, eye))
+    del dC, eye
+    return out
+
+
+def _dd_sym(n, seed, dev):
+    """symmetric positive definite dd matrix on the device, (columns, rows, 2)
+    layout: hi = G G^T / n + I, lo = hi * 2^-54 * S with S symmetric in [-1, 1]"""
+    import torch
+
+    g = torch.Generator(device=dev).manual_seed(seed)
+    G = torch.randn((n, n), dtype=torch.float64, device=dev, generator=g)
+    H = G @ G.T / n
+    H = (H + H.T) * 0.5 + torch.eye(n, dtype=torch.float64, device=dev)
+    S = torch.rand((n, n), dtype=torch.float64, device=dev, generator=g) * 2.0 - 1.0
+    S = (S + S.T) * 0.5
+    return torch.stack([H, H * S * 2.0 ** -54], dim=2).contiguous()
+
+
+def _dd_err(P, Q):
+    """max |P - Q| of two dd matrices held as (.., .., 2) tensors"""
+    return float(((P[..., 0] - Q[..., 0]) + (P[..., 1] - Q[..., 1])).abs().max().item())
+
+
+def bench_dd_factor(ctx, n, dC, eye):
+    """pddblas::Cholesky (vcp/pddblas.hpp:488-563) at n and pddblas::eigsym
+    (Ogita-Aishima, :219-314, 566-606) at n/2, every step on the device.
+    Reports the time of one call (best of two after a warm-up) and
+    max |U^T U - A| / max |X^T X - I| from the dd product A^T B."""
+    import torch
+
+    import vcp_amd
+
+    L = vcp_amd.lib()
+    dev = torch.device("cuda:0")
+    out = {}
+    A = _dd_sym(n, 11, dev)
+    U = torch.empty_like(A)
+    torch.cuda.synchronize()
+    ts = []
+    for i in range(3):
+        t0 = time.perf_counter()
+        ctx.dd_cholesky_dev(n, A.data_ptr(), n, U.data_ptr(), n)
+        ctx.synchronize()
+        if i:
+            ts.append(time.perf_counter() - t0)
+    rc = L.vcphip_ddgemm_ta_dev(ctx._h, n, n, n, U.data_ptr(), n, U.data_ptr(), n, dC.data_ptr(), n)
+    ctx.synchronize()
+    out["dd_cholesky"] = {"n": n, "ms": round(min(ts) * 1e3, 3),
+                          "max_abs_UtU_minus_A": _dd_err(dC, A) if rc == 0 else None,
+                          "max_abs_A": float(A[..., 0].abs().max().item())}
+    log("dd Cholesky %s" % out["dd_cholesky"])
+    del A, U
+    ne = n // 2
+    A = _dd_sym(ne, 12, dev)
+    X = torch.empty_like(A)
+    lam = torch.empty((ne, 2), dtype=torch.float64, device=dev)
+    C = torch.empty_like(A)
+    torch.cuda.synchronize()
+    ts = []
+    for i in range(3):
+        t0 = time.perf_counter()
+        rc = L.vcphip_dd_eigsym_dev(ctx._h, ne, A.data_ptr(), ne, lam.data_ptr(), X.data_ptr(), ne, 4)
+        ctx.synchronize()
+        if rc:
+            raise RuntimeError("vcphip_dd_eigsym_dev: %d" % rc)
+        if i:
+            ts.append(time.perf_counter() - t0)
+    rc = L.vcphip_ddgemm_ta_dev(ctx._h, ne, ne, ne, X.data_ptr(), ne, X.data_ptr(), ne, C.data_ptr(), ne)
+    ctx.synchronize()
+    out["dd_eigsym"] = {"n": ne, "ms": round(min(ts) * 1e3, 3), "itep": 4,
+                        "max_abs_XtX_minus_I": _dd_err(C, eye[:ne, :ne]) if rc == 0 else None}
+    log("dd eigsym %s" % out["dd_eigsym"])
     return out
 
 
