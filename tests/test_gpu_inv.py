@@ -162,3 +162,21 @@ def test_lss_batched_with_device_inverses(ctx):
     for p in range(1, len(sizes), 2):
         Xo, go, ok = O.lss(IAs[p], Ibs[p], Rs[p])
         assert ok and go == gs[p] and np.array_equal(Xs[p], Xo), p
+
+
+def test_lss_batched_device_inverses_over_lanes(ctx):
+    """problems of five padded orders (64 ... 320): their getrf/getrs groups
+    run side by side on the context stream and three more, and every
+    problem's result is still its single solve's, bit for bit, on two calls
+    in a row"""
+    sizes = [5, 70, 130, 200, 260, 70, 5, 131]
+    print("lss_batched device R over lanes", sizes, flush=True)
+    rng = np.random.default_rng(78)
+    sys = [_system(rng, n, 1) for n in sizes]
+    IAs, Ibs = [s[0] for s in sys], [s[1] for s in sys]
+    singles = [ctx.lss(IA, Ib) for IA, Ib in zip(IAs, Ibs)]
+    for rep in range(2):
+        Xs, gs = ctx.lss_batched(IAs, Ibs)
+        for p in range(len(sizes)):
+            X1, g1 = singles[p]
+            assert g1 == gs[p] and np.array_equal(Xs[p], X1), (rep, p)

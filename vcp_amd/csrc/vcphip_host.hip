@@ -76,6 +76,15 @@ struct vcphip_ctx {
   // rocBLAS handle of the approximate inverses (rocSOLVER getrf/getri),
   // created on first use
   rocblas_handle blas = nullptr;
+  // three more streams (default priority), each with its own rocBLAS handle,
+  // over which a batch of approximate inverses spreads its getrf/getrs
+  // groups: rocSOLVER's panel kernels fill a few CUs each, so groups of
+  // different orders run side by side; created on first use
+  static constexpr int INV_LANES = 3;
+  hipStream_t inv_s[INV_LANES] = {};
+  rocblas_handle inv_h[INV_LANES] = {};
+  hipEvent_t inv_fork = nullptr, inv_join[INV_LANES] = {};
+  int inv_lanes = -1;   // extra lanes available; -1 before the first try
   // two pinned host buffers (hipHostMalloc) that stage the large copies of
   // the host-pointer entry points, and the events of their last DMA
   void* pin[2] = {nullptr, nullptr};
@@ -383,6 +392,13 @@ extern "C" int vcphip_ctx_destroy(vcphip_ctx* c) {
     hipEventDestroy(c->ev_join);
   }
   if (c->blas) rocblas_destroy_handle(c->blas);
+  for (int l = 0; l < vcphip_ctx::INV_LANES; ++l) {
+    if (c->inv_s[l]) hipStreamSynchronize(c->inv_s[l]);
+    if (c->inv_h[l]) rocblas_destroy_handle(c->inv_h[l]);
+    if (c->inv_s[l]) hipStreamDestroy(c->inv_s[l]);
+    if (c->inv_join[l]) hipEventDestroy(c->inv_join[l]);
+  }
+  if (c->inv_fork) hipEventDestroy(c->inv_fork);
   for (int h = 0; h < 2; ++h) {
     if (c->pin_ev[h]) hipEventDestroy(c->pin_ev[h]);
     if (c->pin[h]) hipHostFree(c->pin[h]);
@@ -920,6 +936,34 @@ int blas_on_stream(vcphip_ctx* c) {
   return rocblas_set_stream(c->blas, c->stream) == rocblas_status_success ? VCPHIP_OK : VCPHIP_ERR_DEVICE;
 }
 
+// the extra inverse lanes, created once; a context that cannot create them
+// runs every group on its own stream (the results are the same)
+int inv_lanes(vcphip_ctx* c) {
+  if (c->inv_lanes >= 0) return c->inv_lanes;
+  c->inv_lanes = 0;
+  if (hipEventCreateWithFlags(&c->inv_fork, hipEventDisableTiming) != hipSuccess) {
+    c->inv_fork = nullptr;
+    return 0;
+  }
+  for (int l = 0; l < vcphip_ctx::INV_LANES; ++l) {
+    if (hipStreamCreateWithFlags(&c->inv_s[l], hipStreamNonBlocking) != hipSuccess) {
+      c->inv_s[l] = nullptr;
+      break;
+    }
+    if (hipEventCreateWithFlags(&c->inv_join[l], hipEventDisableTiming) != hipSuccess) {
+      c->inv_join[l] = nullptr;
+      break;
+    }
+    if (rocblas_create_handle(&c->inv_h[l]) != rocblas_status_success) {
+      c->inv_h[l] = nullptr;
+      break;
+    }
+    if (rocblas_set_stream(c->inv_h[l], c->inv_s[l]) != rocblas_status_success) break;
+    c->inv_lanes = l + 1;
+  }
+  return c->inv_lanes;
+}
+
 struct InvJob {
   int n;
   const double* IA;
@@ -976,22 +1020,67 @@ int inv_mid_enqueue(vcphip_ctx* c, const std::vector<InvJob>& jobs, InvBatch& b)
     if (vcphip_inv_launch_pad(j.n, j.npad, j.IA, j.lda, j.P, s) ||
         vcphip_inv_launch_pad(0, j.npad, nullptr, 1, inv_result(j), s))   // the identity
       return VCPHIP_ERR_DEVICE;
+  // groups of equal padded order: sorted position range and pivot offset
+  struct Group {
+    int t, e;
+    size_t poff;
+    double cost;
+    int lane;
+  };
+  std::vector<Group> groups;
   size_t poff = 0;
   for (int t = 0; t < cnt;) {
     const int np = jobs[b.ord[t]].npad;
     int e = t + 1;
     while (e < cnt && jobs[b.ord[e]].npad == np) ++e;
-    const int bc = e - t;
-    // the batched routines also for a group of one: their rounding differs
-    // from the plain ones', and a problem's R must not depend on its group
-    rocblas_status st = rocsolver_dgetrf_batched(c->blas, np, np, dptr + t, np, piv + poff, np, b.info + t, bc);
-    if (st == rocblas_status_success)
-      st = rocsolver_dgetrs_batched(c->blas, rocblas_operation_none, np, np, dptr + t, np, piv + poff, np,
-                                    dptr + cnt + t, np, bc);
-    if (st != rocblas_status_success) return VCPHIP_ERR_DEVICE;
-    poff += (size_t)np * bc;
+    groups.push_back(Group{t, e, poff, (double)(e - t) * np * np * np, 0});
+    poff += (size_t)np * (e - t);
     t = e;
   }
+  // the groups over the lanes, the most work first to the least loaded lane;
+  // lane 0 is the context stream
+  const int lanes = groups.size() > 1 ? 1 + std::min((int)groups.size() - 1, inv_lanes(c)) : 1;
+  if (lanes > 1) {
+    std::vector<int> by_cost(groups.size());
+    for (size_t g = 0; g < groups.size(); ++g) by_cost[g] = (int)g;
+    std::stable_sort(by_cost.begin(), by_cost.end(), [&](int x, int y) { return groups[x].cost > groups[y].cost; });
+    double load[1 + vcphip_ctx::INV_LANES] = {};
+    for (int g : by_cost) {
+      int best = 0;
+      for (int l = 1; l < lanes; ++l)
+        if (load[l] < load[best]) best = l;
+      groups[g].lane = best;
+      load[best] += groups[g].cost;
+    }
+    // the extra lanes start after everything enqueued so far on the context stream
+    if (hipEventRecord(c->inv_fork, s) != hipSuccess) return VCPHIP_ERR_DEVICE;
+    for (int l = 1; l < lanes; ++l)
+      if (hipStreamWaitEvent(c->inv_s[l - 1], c->inv_fork, 0) != hipSuccess) return VCPHIP_ERR_DEVICE;
+  }
+  int rc = VCPHIP_OK;
+  for (const Group& g : groups) {
+    const int np = jobs[b.ord[g.t]].npad, bc = g.e - g.t;
+    rocblas_handle h = g.lane ? c->inv_h[g.lane - 1] : c->blas;
+    // the batched routines also for a group of one: their rounding differs
+    // from the plain ones', and a problem's R must not depend on its group
+    rocblas_status st = rocsolver_dgetrf_batched(h, np, np, dptr + g.t, np, piv + g.poff, np, b.info + g.t, bc);
+    if (st == rocblas_status_success)
+      st = rocsolver_dgetrs_batched(h, rocblas_operation_none, np, np, dptr + g.t, np, piv + g.poff, np,
+                                    dptr + cnt + g.t, np, bc);
+    if (st != rocblas_status_success) {
+      rc = VCPHIP_ERR_DEVICE;
+      break;
+    }
+  }
+  // the context stream waits for every lane, also after a failure, so that
+  // synchronising it covers all work on the workspace
+  for (int l = 1; l < lanes; ++l)
+    if (hipEventRecord(c->inv_join[l - 1], c->inv_s[l - 1]) != hipSuccess ||
+        hipStreamWaitEvent(s, c->inv_join[l - 1], 0) != hipSuccess) {
+      hipStreamSynchronize(c->inv_s[l - 1]);
+      rc = VCPHIP_ERR_DEVICE;
+    }
+  if (rc) return rc;
   for (const auto& j : jobs)
     if (j.R && vcphip_inv_launch_copy(j.n, inv_result(j), j.npad, j.R, j.ldr, s)) return VCPHIP_ERR_DEVICE;
   return VCPHIP_OK;
