@@ -19,7 +19,10 @@
  * say they synchronise (they need a norm, a pivot check or a stopping test
  * on the host).  Device memory comes from the context's cached workspace;
  * the rocSOLVER / rocBLAS calls of the inverses and dd refinements may also
- * allocate their handle's own.  Host-pointer copies above 8 MiB are staged
+ * allocate their handle's own.  A batch of inverses spreads its groups of
+ * equal padded order over the context stream and three more streams of its
+ * own (default priority), forked from and joined back to the context stream,
+ * so the batch stays ordered on that stream.  Host-pointer copies above 8 MiB are staged
  * through two pinned host buffers of 128 MiB that the context allocates on
  * first use and holds until it is destroyed.  A context is used by one host
  * thread at a time.
