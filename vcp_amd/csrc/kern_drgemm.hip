@@ -129,7 +129,7 @@ __device__ __forceinline__ void dr_tile(int m, int n, int k, const double* __res
     for (int i = 0; i < TM; ++i)
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
-        const int r = row0 + ty * TM + i, c = col0 + tx * TN + j;
+        const int r = row0 + ty * TM + i, c = col0 + tx + 16 * j;
         if (r < m && c < n) {
           up[i][j] = C[r + ldc * c];
           dn[i][j] = -C2[r + ldc * c];
@@ -150,9 +150,8 @@ __device__ __forceinline__ void dr_tile(int m, int n, int k, const double* __res
         const double2 x0 = *reinterpret_cast<const double2*>(&sAm[kk][ty * TM]);
         const double2 x1 = *reinterpret_cast<const double2*>(&sAm[kk][ty * TM + 2]);
         am[0] = x0.x; am[1] = x0.y; am[2] = x1.x; am[3] = x1.y;
-        const double2 y0 = *reinterpret_cast<const double2*>(&sBm[kk][tx * TN]);
-        const double2 y1 = *reinterpret_cast<const double2*>(&sBm[kk][tx * TN + 2]);
-        bm[0] = y0.x; bm[1] = y0.y; bm[2] = y1.x; bm[3] = y1.y;
+#pragma unroll
+        for (int j = 0; j < TN; ++j) bm[j] = sBm[kk][tx + 16 * j];
       }
       if (AI) {
         const double2 x0 = *reinterpret_cast<const double2*>(&sAr[kk][ty * TM]);
@@ -160,9 +159,8 @@ __device__ __forceinline__ void dr_tile(int m, int n, int k, const double* __res
         ar[0] = x0.x; ar[1] = x0.y; ar[2] = x1.x; ar[3] = x1.y;
       }
       if (BI) {
-        const double2 y0 = *reinterpret_cast<const double2*>(&sBr[kk][tx * TN]);
-        const double2 y1 = *reinterpret_cast<const double2*>(&sBr[kk][tx * TN + 2]);
-        br[0] = y0.x; br[1] = y0.y; br[2] = y1.x; br[3] = y1.y;
+#pragma unroll
+        for (int j = 0; j < TN; ++j) br[j] = sBr[kk][tx + 16 * j];
       }
 #pragma unroll
       for (int i = 0; i < TM; ++i) {
@@ -193,7 +191,7 @@ __device__ __forceinline__ void dr_tile(int m, int n, int k, const double* __res
   // epilogue
 #pragma unroll
   for (int j = 0; j < TN; ++j) {
-    const int c = col0 + tx * TN + j;
+    const int c = col0 + tx + 16 * j;
     if (c >= n) continue;
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
