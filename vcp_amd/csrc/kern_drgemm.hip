@@ -26,6 +26,9 @@ namespace {
 constexpr int BM = 64, BN = 64, BK = 16;
 constexpr int TM = 4, TN = 4;
 constexpr int NT = 256;          // 16 x 16 threads, 4 waves
+// thread (tx, ty) holds rows 4 ty .. 4 ty + 3 and columns tx + 16 j (j < 4) of
+// a tile: a wave's B reads are then 16 consecutive doubles (no LDS bank
+// conflicts) and its A reads a broadcast of four row quadruples
 constexpr int PAD = 2;           // doubles of row padding (keeps 16-byte alignment)
 constexpr int LA = BM + PAD;     // LDS row length of an A k-slice
 constexpr int LB = BN + PAD;
@@ -400,7 +403,7 @@ __global__ __launch_bounds__(NT, 2) void k_drsyrk(int k, int n, const double* __
 #pragma unroll
       for (int i = 0; i < TM; ++i) { a[i] = sIm[kk][ty * TM + i]; if (INTV) ar[i] = sIr[kk][ty * TM + i]; }
 #pragma unroll
-      for (int j = 0; j < TN; ++j) { bb[j] = sJm[kk][tx * TN + j]; if (INTV) br[j] = sJr[kk][tx * TN + j]; }
+      for (int j = 0; j < TN; ++j) { bb[j] = sJm[kk][tx + 16 * j]; if (INTV) br[j] = sJr[kk][tx + 16 * j]; }
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -424,7 +427,7 @@ __global__ __launch_bounds__(NT, 2) void k_drsyrk(int k, int n, const double* __
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
-      const int r = i0 + ty * TM + i, c = j0 + tx * TN + j;
+      const int r = i0 + ty * TM + i, c = j0 + tx + 16 * j;
       if (r >= n || c >= n || r > c) continue;
       double lo, hi;
       if (INTV) {
@@ -517,12 +520,11 @@ __global__ __launch_bounds__(NT) void k_isgemm(int m, int n, int k, const double
         const double2 x3 = *reinterpret_cast<const double2*>(&sAs[kk][ty * TM + 2]);
         ai[0] = x0.x; ai[1] = x0.y; ai[2] = x1.x; ai[3] = x1.y;
         as[0] = x2.x; as[1] = x2.y; as[2] = x3.x; as[3] = x3.y;
-        const double2 y0 = *reinterpret_cast<const double2*>(&sBi[kk][tx * TN]);
-        const double2 y1 = *reinterpret_cast<const double2*>(&sBi[kk][tx * TN + 2]);
-        const double2 y2 = *reinterpret_cast<const double2*>(&sBs[kk][tx * TN]);
-        const double2 y3 = *reinterpret_cast<const double2*>(&sBs[kk][tx * TN + 2]);
-        bi[0] = y0.x; bi[1] = y0.y; bi[2] = y1.x; bi[3] = y1.y;
-        bs[0] = y2.x; bs[1] = y2.y; bs[2] = y3.x; bs[3] = y3.y;
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          bi[j] = sBi[kk][tx + 16 * j];
+          bs[j] = sBs[kk][tx + 16 * j];
+        }
       }
 #pragma unroll
       for (int i = 0; i < TM; ++i)
@@ -543,7 +545,7 @@ __global__ __launch_bounds__(NT) void k_isgemm(int m, int n, int k, const double
   }
 #pragma unroll
   for (int j = 0; j < TN; ++j) {
-    const int c = col0 + tx * TN + j;
+    const int c = col0 + tx + 16 * j;
     if (c >= n) continue;
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
