@@ -26,9 +26,11 @@ namespace {
 constexpr int BM = 64, BN = 64, BK = 16;
 constexpr int TM = 4, TN = 4;
 constexpr int NT = 256;          // 16 x 16 threads, 4 waves
-// thread (tx, ty) holds rows 4 ty .. 4 ty + 3 and columns tx + 16 j (j < 4) of
-// a tile: a wave's B reads are then 16 consecutive doubles (no LDS bank
-// conflicts) and its A reads a broadcast of four row quadruples
+// thread (tx, ty) holds rows 4 ty .. 4 ty + 3 of a tile and four of its
+// columns: 2 tx, 2 tx + 1, 2 tx + 32, 2 tx + 33 in dr_tile (dr_col), tx + 16 j
+// in the syrk and inf-sup tiles.  Either way a wave reads B from consecutive
+// LDS addresses (no bank conflicts) and A as a broadcast of four row
+// quadruples
 constexpr int PAD = 2;           // doubles of row padding (keeps 16-byte alignment)
 constexpr int LA = BM + PAD;     // LDS row length of an A k-slice
 constexpr int LB = BN + PAD;
@@ -41,6 +43,11 @@ struct Stage {
   double a0[A_PER_T], a1[A_PER_T];
   double b0[B_PER_T], b1[B_PER_T];
 };
+
+// column of output j (< 4) of thread column tx in a 64-wide tile: pairs
+// 2 tx, 2 tx + 1 and 2 tx + 32, 2 tx + 33, so a wave reads B as 16-byte
+// pieces at consecutive addresses
+__device__ __forceinline__ int dr_col(int tx, int j) { return 2 * tx + (j & 1) + 32 * (j >> 1); }
 
 // One BM x BN output tile at (row0, col0).  RESID: the batched residual
 // form R*A - I, whose diagonal elements get the interval subtraction of 1
@@ -132,7 +139,7 @@ __device__ __forceinline__ void dr_tile(int m, int n, int k, const double* __res
     for (int i = 0; i < TM; ++i)
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
-        const int r = row0 + ty * TM + i, c = col0 + tx + 16 * j;
+        const int r = row0 + ty * TM + i, c = col0 + dr_col(tx, j);
         if (r < m && c < n) {
           up[i][j] = C[r + ldc * c];
           dn[i][j] = -C2[r + ldc * c];
@@ -153,8 +160,9 @@ __device__ __forceinline__ void dr_tile(int m, int n, int k, const double* __res
         const double2 x0 = *reinterpret_cast<const double2*>(&sAm[kk][ty * TM]);
         const double2 x1 = *reinterpret_cast<const double2*>(&sAm[kk][ty * TM + 2]);
         am[0] = x0.x; am[1] = x0.y; am[2] = x1.x; am[3] = x1.y;
-#pragma unroll
-        for (int j = 0; j < TN; ++j) bm[j] = sBm[kk][tx + 16 * j];
+        const double2 y0 = *reinterpret_cast<const double2*>(&sBm[kk][2 * tx]);
+        const double2 y1 = *reinterpret_cast<const double2*>(&sBm[kk][2 * tx + 32]);
+        bm[0] = y0.x; bm[1] = y0.y; bm[2] = y1.x; bm[3] = y1.y;
       }
       if (AI) {
         const double2 x0 = *reinterpret_cast<const double2*>(&sAr[kk][ty * TM]);
@@ -162,8 +170,9 @@ __device__ __forceinline__ void dr_tile(int m, int n, int k, const double* __res
         ar[0] = x0.x; ar[1] = x0.y; ar[2] = x1.x; ar[3] = x1.y;
       }
       if (BI) {
-#pragma unroll
-        for (int j = 0; j < TN; ++j) br[j] = sBr[kk][tx + 16 * j];
+        const double2 y0 = *reinterpret_cast<const double2*>(&sBr[kk][2 * tx]);
+        const double2 y1 = *reinterpret_cast<const double2*>(&sBr[kk][2 * tx + 32]);
+        br[0] = y0.x; br[1] = y0.y; br[2] = y1.x; br[3] = y1.y;
       }
 #pragma unroll
       for (int i = 0; i < TM; ++i) {
@@ -194,7 +203,7 @@ __device__ __forceinline__ void dr_tile(int m, int n, int k, const double* __res
   // epilogue
 #pragma unroll
   for (int j = 0; j < TN; ++j) {
-    const int c = col0 + tx + 16 * j;
+    const int c = col0 + dr_col(tx, j);
     if (c >= n) continue;
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
