@@ -577,6 +577,30 @@ def bench_config2(ctx, n=4096, steps=3):
                           "frac_fp64_peak": round(4.0 * n ** 3 / t_pp / 1e12 / PEAK_FP64_TFLOPS, 4)},
            "interval_plain": {"ms": round(t_ii * 1e3, 3), "tflops_8n3": round(8.0 * n ** 3 / t_ii / 1e12, 3),
                               "frac_fp64_peak": round(8.0 * n ** 3 / t_ii / 1e12 / PEAK_FP64_TFLOPS, 4)}}
+    # the other pidblas forms on the same operands; flops count the FMA
+    # chains each output carries (2 per chain and k): mul_im_m / mul_m_im 3
+    # chains (6 n^3), mulltmm 5 chains on the upper half (5 n^3), vmulmm 2
+    # chains on the upper half (2 n^3)
+    L = vcp_amd.lib()
+
+    def call(rc, name):
+        if rc:
+            raise RuntimeError("%s: %d" % (name, rc))
+
+    forms = [
+        ("mul_im_m", 6.0, lambda: call(L.vcphip_igemm_ip_dev(ctx._h, n, n, n, dIA.data_ptr(), n, dA.data_ptr(), n,
+                                                             dC.data_ptr(), n), "vcphip_igemm_ip_dev")),
+        ("mul_m_im", 6.0, lambda: ctx.igemm_pi_dev(n, n, n, dA.data_ptr(), n, dIB.data_ptr(), n, dC.data_ptr(), n)),
+        ("mulltmm", 5.0, lambda: call(L.vcphip_igemm_tn_dev(ctx._h, n, n, dIA.data_ptr(), n, dC.data_ptr(), n),
+                                      "vcphip_igemm_tn_dev")),
+        ("vmulmm", 2.0, lambda: call(L.vcphip_vgemm_tn_dev(ctx._h, n, n, dA.data_ptr(), n, dC.data_ptr(), n),
+                                     "vcphip_vgemm_tn_dev")),
+    ]
+    for name, f, fn in forms:
+        t = timed(fn)
+        out[name] = {"ms": round(t * 1e3, 3), "chain_flops_per_n3": f,
+                     "tflops": round(f * n ** 3 / t / 1e12, 3),
+                     "frac_fp64_peak": round(f * n ** 3 / t / 1e12 / PEAK_FP64_TFLOPS, 4)}
     try:
         import oracle_ffi as O
 
