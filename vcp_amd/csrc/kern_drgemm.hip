@@ -406,11 +406,19 @@ __global__ __launch_bounds__(NT, 2) void k_drsyrk(int k, int n, const double* __
   __syncthreads();
   for (int t = 0; t < ntiles; ++t) {
     if (t + 1 < ntiles) load_tile((t + 1) * BK);
-#pragma unroll 1
+#pragma unroll(INTV ? 1 : 2)
     for (int kk = 0; kk < BK; ++kk) {
       double a[TM], ar[TM], bb[TN], br[TN];
-#pragma unroll
-      for (int i = 0; i < TM; ++i) { a[i] = sIm[kk][ty * TM + i]; if (INTV) ar[i] = sIr[kk][ty * TM + i]; }
+      {
+        const double2 x0 = *reinterpret_cast<const double2*>(&sIm[kk][ty * TM]);
+        const double2 x1 = *reinterpret_cast<const double2*>(&sIm[kk][ty * TM + 2]);
+        a[0] = x0.x; a[1] = x0.y; a[2] = x1.x; a[3] = x1.y;
+      }
+      if (INTV) {
+        const double2 x0 = *reinterpret_cast<const double2*>(&sIr[kk][ty * TM]);
+        const double2 x1 = *reinterpret_cast<const double2*>(&sIr[kk][ty * TM + 2]);
+        ar[0] = x0.x; ar[1] = x0.y; ar[2] = x1.x; ar[3] = x1.y;
+      }
 #pragma unroll
       for (int j = 0; j < TN; ++j) { bb[j] = sJm[kk][tx + 16 * j]; if (INTV) br[j] = sJr[kk][tx + 16 * j]; }
 #pragma unroll
