@@ -96,6 +96,8 @@ def main():
         ctx.set_param(vcp_amd.PARAM_OZ_VARIANT, args.variant)
     beta, mods = vcp_amd.ozaki_config(n)
     nmod = len(mods)
+    plan = ctx.ozaki_plan(n, n, n)
+    scheme = {vcp_amd.OZ_RADIUS_FULL: "full", vcp_amd.OZ_RADIUS_FOLDED: "folded"}[plan["radius"]]
 
     def step(form, out):
         ctx.igemm_dev(form, n, n, n, dA.data_ptr(), n, dB.data_ptr(), n, out.data_ptr(), n)
@@ -181,13 +183,16 @@ def main():
     t_gemm = stages.get("gemm")
     roof = None
     if t_gemm:
-        # one launch: nmod residue products + 6 radius-digit plane products
-        # (hi*hi, hi*lo + lo*hi for each of the two radius terms), 2 n^3 each
-        ops = 2.0 * n * n * n * (nmod + 6)
+        # one launch: nmod residue products + the radius-digit plane products
+        # (hi*hi, hi*lo + lo*hi and, full scheme, lo*lo of each of the two
+        # radius terms), 2 n^3 int8 ops each
+        nprod = plan["products"]
+        ops = 2.0 * n * n * n * nprod
         ach = ops / (t_gemm * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
-                "frac": round(ach / PEAK_INT8_TOPS, 4), "traffic": measured_traffic(n, nmod),
-                "kernel": "k_oz_gemm (int8 MFMA: %d residue + 6 radius-digit plane products)" % nmod,
+                "frac": round(ach / PEAK_INT8_TOPS, 4), "traffic": measured_traffic(n, nmod, nprod),
+                "kernel": "k_oz_gemm (int8 MFMA: %d residue + %d radius-digit plane products, %s radius scheme)" %
+                          (nmod, nprod - nmod, scheme),
                 "ops_per_launch": ops, "ms_per_launch": round(t_gemm, 4)}
 
     # ---- CPU baseline: pidblas::mulmm on the reference's own vblas rmatmul
@@ -229,7 +234,7 @@ def main():
     tuned = None
     if not args.no_tuned:
         tctx = vcp_amd.Context(0)
-        nm, ratio_s = tctx.tune_igemm_moduli_dev(n, n, n, dA.data_ptr(), n, dB.data_ptr(), n, 2.0, 256)
+        nm, rad_s, ratio_s = tctx.tune_igemm_moduli_dev(n, n, n, dA.data_ptr(), n, dB.data_ptr(), n, 2.0, 256)
         v, v_ms = tctx.tune_igemm_tile(n, n, n)    # tiles timed with the tuned plane count
         dT = torch.empty_like(dC)
         tctx.igemm_dev(vcp_amd.FORM_OZAKI, n, n, n, dA.data_ptr(), n, dB.data_ptr(), n, dT.data_ptr(), n)
@@ -243,7 +248,8 @@ def main():
         st_tu = tctx.stage_times()
         w_t = dT[..., 1] - dT[..., 0]
         r_t = w_t / torch.clamp(dP[..., 1] - dP[..., 0], min=torch.finfo(torch.float64).tiny)
-        tuned = {"moduli": nm, "sampled_ratio": ratio_s, "tile_variant": v, "ms_per_step": round(t_tu * 1e3, 3),
+        tuned = {"moduli": nm, "radius_scheme": {vcp_amd.OZ_RADIUS_FULL: "full", vcp_amd.OZ_RADIUS_FOLDED: "folded"}[rad_s],
+                 "sampled_ratio": ratio_s, "tile_variant": v, "ms_per_step": round(t_tu * 1e3, 3),
                  "tflops": round(flops / t_tu / 1e12, 3), "speedup_vs_plain": round(t_pl / t_tu, 3),
                  "stages_ms": {k: round(x, 4) for k, x in st_tu.items()},
                  "radius_ratio_over_plain": {"max": float(r_t.max()), "median": float(r_t.median())}}
@@ -284,7 +290,7 @@ def main():
         "dtype": "int8",
         "data": "synthetic",
         "config": {"workload": "interval GEMM n=%d, mid=+-m*2^e (m U[1,2), e U{-20..20}), rad=2^-45|mid|" % n,
-                   "n": n, "form": "ozaki-int8", "moduli": nmod, "beta": beta,
+                   "n": n, "form": "ozaki-int8", "moduli": nmod, "beta": beta, "radius_scheme": scheme,
                    "tile_variant": args.variant or "default"},
         "plain_fp64_rurd": {"tflops": round(tf_pl, 3), "ms_per_step": round(t_pl * 1e3, 3),
                             "frac_fp64_peak": round(tf_pl / PEAK_FP64_TFLOPS, 4)},
@@ -306,17 +312,18 @@ def main():
     ctx.close()
 
 
-def measured_traffic(n, nmod):
+def measured_traffic(n, nmod, nprod):
     """HBM bytes per launch of the int8 GEMM stage, from the committed
     FETCH_SIZE / WRITE_SIZE counter runs (profiles/ozaki_gemm_traffic.json),
-    when they were taken on this workload; otherwise None."""
+    when they were taken on this workload (size, moduli and plane products);
+    otherwise None."""
     path = os.path.join(ROOT, "profiles", "ozaki_gemm_traffic.json")
     try:
         with open(path) as f:
             rec = json.load(f)
     except (OSError, ValueError):
         return None
-    if rec.get("n") != n or rec.get("moduli") != nmod:
+    if rec.get("n") != n or rec.get("moduli") != nmod or rec.get("plane_products", nmod + 6) != nprod:
         return None
     return rec.get("bytes_per_launch")
 

@@ -74,7 +74,25 @@ enum {
     VCPHIP_PARAM_OZ_NMOD = 1,      /* moduli of the Ozaki form (0 = tuned/default) */
     VCPHIP_PARAM_OZ_VARIANT = 2,   /* int8 GEMM tile variant 6-16 (0 = tuned/default) */
     VCPHIP_PARAM_STAGE_TIMING = 3, /* 1: record per-stage times with HIP events  */
-    VCPHIP_PARAM_DD_NMOD = 4       /* moduli of the dd product (0 = default)      */
+    VCPHIP_PARAM_DD_NMOD = 4,      /* moduli of the dd product (0 = default)      */
+    VCPHIP_PARAM_OZ_RADIUS = 5     /* radius-digit scheme of the Ozaki form:
+                                      VCPHIP_OZ_RADIUS_* (0 = tuned, else full for
+                                      k < 4096 and folded from k = 4096) */
+};
+
+/* radius-digit schemes of the Ozaki form.  Each factor of the radius term
+ * (|mid A| + rad A) rad B + rad A |mid B| (vcp/pidblas.hpp:277-283) is bounded
+ * above by two 7-bit digits against its row / column maximum, and the sum
+ * over k of the products of those bounds is formed on int8 MFMA:
+ *   FULL    hi x hi, hi x lo + lo x hi and lo x lo per term: the exact sum of
+ *           the digit-bound products (8 int8 plane products);
+ *   FOLDED  B's digits carry a non-positive low digit, so the lo x lo sum is
+ *           never positive and is left out (6 plane products; wider where k
+ *           is small and exponents spread, equal at k in the thousands).
+ * Both give a rigorous enclosure. */
+enum {
+    VCPHIP_OZ_RADIUS_FULL = 1,
+    VCPHIP_OZ_RADIUS_FOLDED = 2
 };
 
 /* ---- context ------------------------------------------------------------ */
@@ -319,14 +337,24 @@ int vcphip_lss_batched_dev(vcphip_ctx* ctx, int count, const int* n, const int* 
 int vcphip_tune_igemm_tile(vcphip_ctx* ctx, int m, int n, int k, int* variant, double* ms);
 /* The same for the double-double product (vcphip_ddgemm). */
 int vcphip_tune_ddgemm_tile(vcphip_ctx* ctx, int m, int n, int k, int* variant, double* ms);
-/* Slice count of the interval Ozaki form for these operands (device
- * pointers): the smallest number of moduli in [12, 20] whose radii, on an
- * evenly spaced sample of sample_rows rows of IA (0: 256) against all of IB,
- * are at most max_ratio times the plain form's; 20 if none is.  *nmod and
- * *ratio (the sampled maximum radius ratio) receive the choice. */
+/* What an interval Ozaki product of shape (m, n, k) in this context uses
+ * (parameters, then the tuned table, then the defaults): moduli count,
+ * radius-digit scheme, int8 GEMM tile variant and the number of int8 plane
+ * products of its GEMM stage (each 2 m n k int8 operations, padded).  Any
+ * output may be NULL. */
+int vcphip_ozaki_plan(vcphip_ctx* ctx, int m, int n, int k, int* nmod, int* radius, int* variant,
+                      int* products);
+/* Slice count and radius-digit scheme of the interval Ozaki form for these
+ * operands (device pointers): the cheapest pair (moduli N in [12, 20],
+ * VCPHIP_OZ_RADIUS_FOLDED at N + 6 int8 plane products or _FULL at N + 8)
+ * whose radii, on an evenly spaced sample of sample_rows rows of IA (0: 256)
+ * against all of IB, are at most max_ratio times the plain form's; at equal
+ * cost the smaller ratio wins; 20 moduli with the full scheme if none is.
+ * *nmod, *radius and *ratio (the sampled maximum radius ratio) receive the
+ * choice (any of them may be NULL). */
 int vcphip_tune_igemm_moduli_dev(vcphip_ctx* ctx, int m, int n, int k, const double* IA, int lda,
                                  const double* IB, int ldb, double max_ratio, int sample_rows,
-                                 int* nmod, double* ratio);
+                                 int* nmod, int* radius, double* ratio);
 /* Times 1-4 persistent workgroups per CU for the batched residual of this
  * problem set (arguments as vcphip_residual_batched_dev, which it also runs)
  * and keeps the fastest; the next call captures its graph with it. */
@@ -334,8 +362,9 @@ int vcphip_tune_residual_dev(vcphip_ctx* ctx, int count, const int* n, const dou
                              const int* ldr, const double* const* IA, const int* lda,
                              double* const* C, const int* ldc, double* normub_dev, int* per_cu,
                              double* ms);
-/* Writes / reads the context's tuning table (text: a "vcphip-tune 1" line,
- * then "igemm|ddgemm lm ln lk variant nmod" and "residual per_cu" lines). */
+/* Writes / reads the context's tuning table (text: a "vcphip-tune 2" line,
+ * then "igemm|ddgemm lm ln lk variant nmod radius" and "residual per_cu"
+ * lines; version 1 tables, without the radius field, still load). */
 int vcphip_tune_save(vcphip_ctx* ctx, const char* path);
 int vcphip_tune_load(vcphip_ctx* ctx, const char* path);
 

@@ -852,33 +852,53 @@ static void digits2_ru(double x, int g, int* h, int* l) {
     *l = (int)ceil(r);
 }
 
-/* X * 2^s rounded toward -inf (dir<0) or +inf (dir>0), exactly: keep the
- * bits at or above the result's ulp (the larger of the 53-bit window and the
- * subnormal grid 2^-1074), round the dropped tail away from zero if needed */
-static double conv_i128(i128 X, int s, int dir) {
-    if (X == 0) return 0.0;
-    const int neg = X < 0;
-    const u128 mag = neg ? (u128)(-X) : (u128)X;
-    const int P = u128_bitlen(mag) - 1;
-    int q = P - 52;
+/* The same bound of x written with a non-positive low digit:
+ *   x <= (H + L/127) 2^g / 127,  H = h + [l > 0] in [0, 127],  L = l - 127 [l > 0] in [-126, 0].
+ * H + L/127 = h + l/127, and H <= 127 because t <= 127 forces l = 0 when
+ * h = 127.  For a factor x1 <= (h1 + l1/127) s1 (l1 >= 0) and x2 held this way,
+ *   x1 x2 <= s1 s2 (h1 H + (h1 L + l1 H)/127 + l1 L/127^2)
+ * and the last term is never positive, so
+ *   sum_k x1 x2 <= s1 s2 (sum h1 H + (sum h1 L + l1 H)/127).            */
+static void digits2_ru_neglo(double x, int g, int* H, int* L) {
+    int h, l;
+    digits2_ru(x, g, &h, &l);
+    *H = h + (l > 0);
+    *L = l > 0 ? l - 127 : 0;
+}
+
+/* X = hi 2^64 + lo (lo unsigned; |X| < 2^190) times 2^s, rounded toward -inf
+ * (dir<0) or +inf (dir>0), exactly.  The interval form's scaled midpoints
+ * reach |A'|, |B'| = 2^62, so a dot product of k of their products needs up
+ * to 124 + log2 k + 1 bits: more than one i128 holds. */
+static double conv_x192(i128 hi, uint64_t lo, int s, int dir) {
+    const int neg = hi < 0;
+    u128 mh;
+    uint64_t ml;
+    if (!neg) { mh = (u128)hi; ml = lo; }
+    else if (lo == 0) { mh = (u128)(-hi); ml = 0; }
+    else { mh = (u128)(-(hi + 1)); ml = (uint64_t)0 - lo; } /* 2^64 - lo */
+    uint32_t x[6];
+    for (int i = 0; i < 2; ++i) x[i] = (uint32_t)(ml >> (32 * i));
+    for (int i = 0; i < 4; ++i) x[2 + i] = (uint32_t)(mh >> (32 * i));
+    int top = -1;
+    for (int i = 5; i >= 0 && top < 0; --i)
+        if (x[i]) top = 32 * i + 31 - __builtin_clz(x[i]);
+    if (top < 0) return 0.0;
+#define X192_BIT(b) ((b) < 0 || (b) >= 192 ? 0u : (x[(b) >> 5] >> ((b) & 31)) & 1u)
+    int q = top - 52;
     if (q < -1074 - s) q = -1074 - s;
-    u128 mant = mag;
+    if (q < 0) q = 0; /* exact: the value lies on the double grid */
+    uint64_t mant = 0;
+    for (int b = top; b >= q; --b) mant = (mant << 1) | X192_BIT(b);
     int rest = 0;
-    if (q <= 0) {
-        q = 0; /* exact: mag < 2^53 and mag * 2^s lies on the double grid */
-    } else if (q >= 128) {
-        mant = 0;
-        rest = 1;
-    } else {
-        mant = mag >> q;
-        rest = (mag & (((u128)1 << q) - 1)) != 0;
-    }
+    for (int b = 0; b < q && !rest; ++b) rest = (int)X192_BIT(b);
+#undef X192_BIT
     const int away = dir > 0 ? !neg : neg;
     if (rest && away) mant += 1;
     const int ex = q + s;
     double v;
-    if (mant != 0 && ex + u128_bitlen(mant) > 1024) v = away ? INFINITY : DBL_MAX;
-    else v = ldexp((double)(uint64_t)mant, ex);
+    if (mant != 0 && ex + 64 - __builtin_clzll(mant) > 1024) v = away ? INFINITY : DBL_MAX;
+    else v = ldexp((double)mant, ex);
     return neg ? -v : v;
 }
 
@@ -973,7 +993,8 @@ void orc_ozaki_scaled_cols(int k, int n, const double* B, int ldb, int beta, int
  * scaling-error bound E and the upward two-digit bound of the radius term
  * (|mA|+rA) rB + rA |mB| (:277-283). */
 static int oz_igemm_core(int m, int n, int k, const double* A, int lda, const double* B, int ldb,
-                         double* C, int ldc, int nmod) {
+                         double* C, int ldc, int nmod, int radius) {
+    const int full = radius != ORC_OZ_RADIUS_FOLDED;
     const int beta = orc_ozaki_beta(k, nmod);
     if (beta < 1) return -2;
     const size_t mk = (size_t)m * k, kn = (size_t)k * n;
@@ -985,14 +1006,18 @@ static int oz_igemm_core(int m, int n, int k, const double* A, int lda, const do
     oz_stats(n, k, mB, rB, (size_t)k, 1, beta, &SB, 64);
     int64_t* Ap = (int64_t*)malloc(sizeof(int64_t) * mk); /* row i contiguous */
     int64_t* Bp = (int64_t*)malloc(sizeof(int64_t) * kn); /* col j contiguous */
-    /* digit planes (h, l) of |mA|+rA, rA (rows of A) and rB, |mB| (cols of B) */
+    /* digit planes (h, l) of |mA|+rA, rA (rows of A); (H, L), the same bound
+     * with a non-positive low digit, of rB, |mB| (cols of B) */
     int8_t *Sh = (int8_t*)malloc(mk), *Sl = (int8_t*)malloc(mk), *Rh = (int8_t*)malloc(mk), *Rl = (int8_t*)malloc(mk);
     int8_t *BRh = (int8_t*)malloc(kn), *BRl = (int8_t*)malloc(kn), *BMh = (int8_t*)malloc(kn), *BMl = (int8_t*)malloc(kn);
-    double c127, c1;
+    double c127, c1, c1dn, c127dn;
     const int old = fegetround();
-    fesetround(FE_UPWARD);
     {
         volatile double one = 1.0, d = 16129.0, d1 = 127.0;
+        fesetround(FE_DOWNWARD);
+        c1dn = one / d1;
+        c127dn = one / d;
+        fesetround(FE_UPWARD);
         c127 = one / d;
         c1 = one / d1;
     }
@@ -1013,8 +1038,8 @@ static int oz_igemm_core(int m, int n, int k, const double* A, int lda, const do
             const size_t o = (size_t)j * k + c;
             int h, l;
             Bp[o] = zc ? 0 : rnint_floor(mulpow2(bm, beta - SB.e[j]));
-            digits2_ru(br, SB.h[j], &h, &l); BRh[o] = (int8_t)h; BRl[o] = (int8_t)l;
-            digits2_ru(fabs(bm), zc ? 0 : SB.e[j], &h, &l); BMh[o] = (int8_t)h; BMl[o] = (int8_t)l;
+            digits2_ru_neglo(br, SB.h[j], &h, &l); BRh[o] = (int8_t)h; BRl[o] = (int8_t)l;
+            digits2_ru_neglo(fabs(bm), zc ? 0 : SB.e[j], &h, &l); BMh[o] = (int8_t)h; BMl[o] = (int8_t)l;
         }
     }
     fesetround(old);
@@ -1026,27 +1051,39 @@ static int oz_igemm_core(int m, int n, int k, const double* A, int lda, const do
 #pragma omp for schedule(dynamic, 1)
         for (int j = 0; j < n; ++j) {
             for (int i = 0; i < m; ++i) {
-                i128 X = 0;
-                int64_t z1h = 0, z1m = 0, z2h = 0, z2m = 0;
+                i128 Xh = 0; /* sum of the products' high halves (arithmetic >> 64) */
+                u128 Xl = 0; /* sum of their low 64 bits */
+                int64_t z1h = 0, z1m = 0, z1l = 0, z2h = 0, z2m = 0, z2l = 0;
                 const size_t ai = (size_t)i * k, bj = (size_t)j * k;
                 for (int c = 0; c < k; ++c) {
-                    X += (i128)Ap[ai + c] * Bp[bj + c];
+                    const i128 pr = (i128)Ap[ai + c] * Bp[bj + c];
+                    Xh += pr >> 64;
+                    Xl += (uint64_t)pr;
                     z1h += (int64_t)Sh[ai + c] * BRh[bj + c];
                     z1m += (int64_t)Sh[ai + c] * BRl[bj + c] + (int64_t)Sl[ai + c] * BRh[bj + c];
                     z2h += (int64_t)Rh[ai + c] * BMh[bj + c];
                     z2m += (int64_t)Rh[ai + c] * BMl[bj + c] + (int64_t)Rl[ai + c] * BMh[bj + c];
+                    z1l += (int64_t)Sl[ai + c] * BRl[bj + c];
+                    z2l += (int64_t)Rl[ai + c] * BMl[bj + c];
                 }
                 const int fa = SA.flags[i], fb = SB.flags[j];
                 const int sc = SA.e[i] + SB.e[j] - 2 * beta;
-                const double lo_mid = conv_i128(X, sc, -1);
-                const double hi_mid = conv_i128(X, sc, 1);
+                Xh += (i128)(Xl >> 64);
+                const double lo_mid = conv_x192(Xh, (uint64_t)Xl, sc, -1);
+                const double hi_mid = conv_x192(Xh, (uint64_t)Xl, sc, 1);
                 double t1 = 0.0, t2 = 0.0;
+                /* sum x1 x2 <= 2^(g1+g2)/127^2 (zh + zm/127 + zl/127^2): the sum of
+                 * the products of the digit bounds (full scheme); the folded one
+                 * leaves out zl <= 0.  A quotient with a numerator of either sign is
+                 * bounded above by the numerator times RU or RD of the reciprocal. */
                 if (!((fa & OZ_SUMZERO) || (fb & OZ_RADZERO))) {
-                    const double u = ((double)z1h + (double)z1m * c1) + (double)k * c127;
+                    double u = (double)z1h + (double)z1m * (z1m >= 0 ? c1 : c1dn);
+                    if (full) u = u + (double)z1l * (z1l >= 0 ? c127 : c127dn);
                     t1 = mulpow2(u, SA.g[i] + SB.h[j]) * c127;
                 }
                 if (!((fa & OZ_RADZERO) || (fb & OZ_MIDZERO))) {
-                    const double u = ((double)z2h + (double)z2m * c1) + (double)k * c127;
+                    double u = (double)z2h + (double)z2m * (z2m >= 0 ? c1 : c1dn);
+                    if (full) u = u + (double)z2l * (z2l >= 0 ? c127 : c127dn);
                     t2 = mulpow2(u, SA.h[i] + SB.e[j]) * c127;
                 }
                 const double rt = t1 + t2;
@@ -1429,17 +1466,17 @@ static double* nf_clean(int rows, int cols, const double* X, int ld, const int* 
 }
 
 int orc_ozaki_igemm(int m, int n, int k, const double* A, int lda, const double* B, int ldb,
-                    double* C, int ldc, int nmod) {
+                    double* C, int ldc, int nmod, int radius) {
     int* fa = (int*)calloc(m > 0 ? m : 1, sizeof(int));
     int* fb = (int*)calloc(n > 0 ? n : 1, sizeof(int));
     const int any = nf_scan(m, k, A, 1, (size_t)lda, fa) | nf_scan(n, k, B, (size_t)ldb, 1, fb);
     int rc;
     if (!any) {
-        rc = oz_igemm_core(m, n, k, A, lda, B, ldb, C, ldc, nmod);
+        rc = oz_igemm_core(m, n, k, A, lda, B, ldb, C, ldc, nmod, radius);
     } else {
         double* A2 = nf_clean(m, k, A, lda, fa, 1);
         double* B2 = nf_clean(k, n, B, ldb, fb, 0);
-        rc = oz_igemm_core(m, n, k, A2, m > 0 ? m : 1, B2, k > 0 ? k : 1, C, ldc, nmod);
+        rc = oz_igemm_core(m, n, k, A2, m > 0 ? m : 1, B2, k > 0 ? k : 1, C, ldc, nmod, radius);
         for (int j = 0; j < n && rc == 0; ++j)
             for (int i = 0; i < m; ++i) {
                 const int f = fa[i] | fb[j];

@@ -101,9 +101,13 @@ int orc_ozaki_moduli(int nmod, int* out);
 /* Scale exponent beta for k and nmod so that k*2^(2 beta + 1) < M. */
 int orc_ozaki_beta(int k, int nmod);
 /* Interval GEMM through the exact scaled-integer product.  Mirrors the GPU
- * pipeline bit for bit (the CRT result equals the integer product). */
+ * pipeline bit for bit (the CRT result equals the integer product).
+ * radius: ORC_OZ_RADIUS_FULL (the exact sum of the products of the upward
+ * digit bounds of the radius-term factors) or ORC_OZ_RADIUS_FOLDED (the lo x lo
+ * part, never positive with B's non-positive low digits, left out). */
+enum { ORC_OZ_RADIUS_FULL = 1, ORC_OZ_RADIUS_FOLDED = 2 };
 int orc_ozaki_igemm(int m, int n, int k, const double* A, int lda, const double* B, int ldb,
-                    double* C, int ldc, int nmod);
+                    double* C, int ldc, int nmod, int radius);
 /* Integer slice work of the same pipeline, exposed for bit-exact checks:
  * the scaled integer operand A' (row scaling) of row-stats exponent e. */
 void orc_ozaki_scaled_rows(int m, int k, const double* A, int lda, int beta, int64_t* Ap /* m x k col-major */,

@@ -60,7 +60,8 @@ def lib():
                                       _p_int, _p_int, _p_int, _p_dbl, _p_dbl],
             "orc_ozaki_moduli": [_c_int, _p_int],
             "orc_ozaki_beta": [_c_int, _c_int],
-            "orc_ozaki_igemm": [_c_int, _c_int, _c_int, _p_dbl, _c_int, _p_dbl, _c_int, _p_dbl, _c_int, _c_int],
+            "orc_ozaki_igemm": [_c_int, _c_int, _c_int, _p_dbl, _c_int, _p_dbl, _c_int, _p_dbl, _c_int, _c_int,
+                                _c_int],
             "orc_ozaki_scaled_rows": [_c_int, _c_int, _p_dbl, _c_int, _c_int, _p_i64, _p_int],
             "orc_ozaki_scaled_cols": [_c_int, _c_int, _p_dbl, _c_int, _c_int, _p_i64, _p_int],
             "orc_dd_split_shift": [_c_int],
@@ -290,12 +291,21 @@ def ozaki_beta(k, nmod):
     return lib().orc_ozaki_beta(k, nmod)
 
 
-def ozaki_igemm(IA, IB, nmod):
+OZ_RADIUS_FULL = 1
+OZ_RADIUS_FOLDED = 2
+OZ_FOLDED_MINK = 4096   # the library's untuned default: folded from this k
+
+
+def ozaki_igemm(IA, IB, nmod, radius=None):
+    """radius: OZ_RADIUS_FULL / OZ_RADIUS_FOLDED; None follows the library's
+    untuned default (folded for k >= 4096, else full)"""
     m, k = IA.shape[:2]
     n = IB.shape[1]
+    if radius is None:
+        radius = OZ_RADIUS_FOLDED if k >= OZ_FOLDED_MINK else OZ_RADIUS_FULL
     C = pairs(m, n)
     rc = lib().orc_ozaki_igemm(m, n, k, _dp(IA.transpose(1, 0, 2)), _ld_pair(IA), _dp(IB.transpose(1, 0, 2)),
-                               _ld_pair(IB), _dp(C.transpose(1, 0, 2)), _ld_pair(C), nmod)
+                               _ld_pair(IB), _dp(C.transpose(1, 0, 2)), _ld_pair(C), nmod, radius)
     if rc != 0:
         raise ValueError("orc_ozaki_igemm failed: %d" % rc)
     return C

@@ -90,7 +90,7 @@ void test_gpu() {
   orc_pidblas_mulmm(m, n, k, A.data(), m, B.data(), k, R.data(), m);
   EXPECT(std::memcmp(C.data(), R.data(), sizeof(double) * C.size()) == 0 || C == R);
   check<Errors>(vcphip_igemm(ctx, VCPHIP_FORM_OZAKI, m, n, k, A.data(), m, B.data(), k, C.data(), m), "ozaki");
-  orc_ozaki_igemm(m, n, k, A.data(), m, B.data(), k, R.data(), m, 16);
+  orc_ozaki_igemm(m, n, k, A.data(), m, B.data(), k, R.data(), m, 16, k >= 4096 ? ORC_OZ_RADIUS_FOLDED : ORC_OZ_RADIUS_FULL);
   EXPECT(C == R);
   bool thrown = false;
   try {

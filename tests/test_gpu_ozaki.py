@@ -10,6 +10,8 @@ import oracle_ffi as O
 import vcp_amd
 from vcp_amd import workloads as W
 
+import ozaki_cases
+
 pytestmark = pytest.mark.gpu
 
 
@@ -125,3 +127,35 @@ def test_ozaki_extreme_exponents(ctx):
     # row 0 x column 1 overflows: an infinite end encloses it (the exact-hull
     # checker works on finite bounds only)
     assert C[0, 1, 0] == -np.inf or C[0, 1, 1] == np.inf
+
+
+@pytest.mark.parametrize("radius", [vcp_amd.OZ_RADIUS_FULL, vcp_amd.OZ_RADIUS_FOLDED])
+@pytest.mark.parametrize("nmod", [12, 16, 20])
+@pytest.mark.parametrize("case", ozaki_cases.NAMES)
+def test_ozaki_digit_stress_every_entry(ctx, case, nmod, radius):
+    """inputs whose radius-term factors mostly have a zero high digit (the
+    low-digit products carry a large share of the radius) and, at 20 moduli,
+    midpoint sums beyond 2^127: bit-exact to the oracle with the same moduli
+    count and radius scheme, and every entry contains the exact hull"""
+    IA, IB = ozaki_cases.make(case)
+    print("case", case, "nmod", nmod, "radius", radius, IA.shape, IB.shape, flush=True)
+    ctx.set_param(vcp_amd.PARAM_OZ_NMOD, nmod)
+    ctx.set_param(vcp_amd.PARAM_OZ_RADIUS, radius)
+    try:
+        C = ctx.igemm(IA, IB, vcp_amd.FORM_OZAKI)
+    finally:
+        ctx.set_param(vcp_amd.PARAM_OZ_NMOD, 0)
+        ctx.set_param(vcp_amd.PARAM_OZ_RADIUS, 0)
+    R = O.ozaki_igemm(IA, IB, nmod, radius)
+    bad = np.argwhere(~((C == R).all(axis=2)))
+    assert len(bad) == 0, "first mismatches %s gpu %s oracle %s" % (bad[:3].tolist(), C[tuple(bad[0])], R[tuple(bad[0])])
+    ii, jj = ozaki_cases.every_entry(C)
+    ok, _, _ = O.check_containment(IA, IB, C, ii, jj)
+    assert ok.all(), "%d of %d entries not enclosed" % ((~ok).sum(), ok.size)
+
+
+def test_ozaki_radius_param_checked(ctx):
+    for bad in (3, -1, 7):
+        with pytest.raises(vcp_amd.VcphipError):
+            ctx.set_param(vcp_amd.PARAM_OZ_RADIUS, bad)
+    assert ctx.get_param(vcp_amd.PARAM_OZ_RADIUS) == 0

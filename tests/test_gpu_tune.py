@@ -53,12 +53,13 @@ def test_tune_moduli_meets_ratio_on_the_sample(ctx, max_ratio):
     IA, IB = W.wide_exponent_intervals(rng, m, k), W.wide_exponent_intervals(rng, k, n)
     dA, dB = _dev_pairs(IA), _dev_pairs(IB)
     torch.cuda.synchronize()
-    nmod, ratio = ctx.tune_igemm_moduli_dev(m, n, k, dA.data_ptr(), m, dB.data_ptr(), k, max_ratio, 64)
-    assert 12 <= nmod <= 20
-    assert ratio <= max_ratio or nmod == 20
-    # the product of that size class now uses the tuned count, bit-exact to the oracle with it
+    nmod, radius, ratio = ctx.tune_igemm_moduli_dev(m, n, k, dA.data_ptr(), m, dB.data_ptr(), k, max_ratio, 64)
+    assert 12 <= nmod <= 20 and radius in (vcp_amd.OZ_RADIUS_FULL, vcp_amd.OZ_RADIUS_FOLDED)
+    assert ratio <= max_ratio or (nmod == 20 and radius == vcp_amd.OZ_RADIUS_FULL)
+    # the product of that size class now uses the tuned count and scheme,
+    # bit-exact to the oracle with them
     C = ctx.igemm(IA, IB, vcp_amd.FORM_OZAKI)
-    assert np.array_equal(C, O.ozaki_igemm(IA, IB, nmod))
+    assert np.array_equal(C, O.ozaki_igemm(IA, IB, nmod, radius))
     P = ctx.igemm(IA, IB, vcp_amd.FORM_PLAIN)
     ii = np.arange(0, m, m // 64)
     jj = rng.integers(0, n, ii.size)
@@ -77,21 +78,35 @@ def test_tune_save_load_roundtrip(ctx, tmp_path):
     IA, IB = W.wide_exponent_intervals(rng, m, k), W.wide_exponent_intervals(rng, k, n)
     dA, dB = _dev_pairs(IA), _dev_pairs(IB)
     torch.cuda.synchronize()
-    nmod, _ = ctx.tune_igemm_moduli_dev(m, n, k, dA.data_ptr(), m, dB.data_ptr(), k, 4.0, 32)
+    nmod, radius, _ = ctx.tune_igemm_moduli_dev(m, n, k, dA.data_ptr(), m, dB.data_ptr(), k, 4.0, 32)
     ctx.tune_igemm_tile(m, n, k)
     path = tmp_path / "tune.txt"
     ctx.tune_save(str(path))
     text = path.read_text()
-    assert text.startswith("vcphip-tune 1\n") and "igemm" in text and "residual" in text
+    assert text.startswith("vcphip-tune 2\n") and "igemm" in text and "residual" in text
     with vcp_amd.Context(0) as c2:
         c2.tune_load(str(path))
         C = c2.igemm(IA, IB, vcp_amd.FORM_OZAKI)
-    assert np.array_equal(C, O.ozaki_igemm(IA, IB, nmod))
+    assert np.array_equal(C, O.ozaki_igemm(IA, IB, nmod, radius))
+    # a version-1 table (no radius field; variants up to 16) still loads, with
+    # the default full scheme
+    old = tmp_path / "tune_v1.txt"
+    old.write_text("vcphip-tune 1\nigemm 8 8 10 16 %d\nresidual 2\n" % nmod)
+    with vcp_amd.Context(0) as c3:
+        c3.tune_load(str(old))
+        C = c3.igemm(IA, IB, vcp_amd.FORM_OZAKI)
+    assert np.array_equal(C, O.ozaki_igemm(IA, IB, nmod, vcp_amd.OZ_RADIUS_FULL))
 
 
 def test_tune_load_rejects_malformed(ctx, tmp_path):
     bad = tmp_path / "bad.txt"
     bad.write_text("vcphip-tune 1\nigemm 10 10 10 99 16\n")
+    with pytest.raises(vcp_amd.VcphipError):
+        ctx.tune_load(str(bad))
+    bad.write_text("vcphip-tune 2\nigemm 10 10 10 8 16 3\n")   # no radius scheme 3
+    with pytest.raises(vcp_amd.VcphipError):
+        ctx.tune_load(str(bad))
+    bad.write_text("vcphip-tune 2\nddgemm 10 10 10 8 32 1\n")   # dd products have no radius scheme
     with pytest.raises(vcp_amd.VcphipError):
         ctx.tune_load(str(bad))
     bad.write_text("not a table\n")

@@ -13,6 +13,8 @@ import pytest
 import oracle_ffi as O
 from vcp_amd import workloads as W
 
+import ozaki_cases
+
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "rmatmul_ref.npz")
 
 
@@ -302,11 +304,61 @@ def test_pidblas_fma_is_the_interval_fma_chain():
 def test_ozaki_restatement_encloses_and_is_not_wider(kind):
     rng = np.random.default_rng(11)
     IA, IB = _iv(rng, 24, 300, kind), _iv(rng, 300, 20, kind)
-    Z = O.ozaki_igemm(IA, IB, 16)
     P = O.pidblas_mulmm(IA, IB)
+    for radius in (O.OZ_RADIUS_FULL, O.OZ_RADIUS_FOLDED):
+        assert _contains_all(IA, IB, O.ozaki_igemm(IA, IB, 16, radius))
+    # The radius ratio depends on the moduli count (the scaling error E) and
+    # on the data: for k = 300 products over a 41-binade exponent span, E of
+    # 16 moduli alone takes about 0.6 of the plain width (max ratio 2.06 on
+    # "wide"); with 18 moduli the full scheme stays within 2x.
+    Z = O.ozaki_igemm(IA, IB, 18)
     assert _contains_all(IA, IB, Z)
     ratio = (Z[..., 1] - Z[..., 0]) / np.maximum(P[..., 1] - P[..., 0], np.finfo(float).tiny)
     assert ratio.max() <= 2.0
+
+
+@pytest.mark.parametrize("radius", [O.OZ_RADIUS_FULL, O.OZ_RADIUS_FOLDED])
+@pytest.mark.parametrize("nmod", [12, 16, 20])
+@pytest.mark.parametrize("case", ozaki_cases.NAMES)
+def test_ozaki_restatement_encloses_every_entry_on_digit_stress(case, nmod, radius):
+    """Every entry contains the exact hull (superaccumulator) on inputs whose
+    radius-term factors mostly have a zero high digit, and radii stay within
+    2x of the plain form's.  Dropping the low-digit product, or an int128
+    midpoint sum that wraps (beta = 62 at 20 moduli), fails here."""
+    IA, IB = ozaki_cases.make(case)
+    Z = O.ozaki_igemm(IA, IB, nmod, radius)
+    ii, jj = ozaki_cases.every_entry(Z)
+    ok, _, _ = O.check_containment(IA, IB, Z, ii, jj)
+    assert ok.all(), "%d of %d entries not enclosed" % ((~ok).sum(), ok.size)
+    P = O.pidblas_mulmm(IA, IB)
+    ratio = (Z[..., 1] - Z[..., 0]) / np.maximum(P[..., 1] - P[..., 0], np.finfo(float).tiny)
+    assert ratio.max() <= 2.0, ratio.max()
+
+
+def test_ozaki_midpoint_sum_beyond_int128():
+    """20 moduli give beta = 62: |A'|, |B'| up to 2^62, so the exact midpoint
+    sum of k = 300 products exceeds 2^127.  Point inputs of that size: the
+    enclosure's ends straddle the exact product computed with Python
+    integers from the oracle's own scaled operands and lie within a relative
+    2^-40 of it."""
+    rng = np.random.default_rng(21)
+    m, n, k = 6, 5, 300
+    IA = W.wide_exponent_intervals(rng, m, k, -1, 0, rad_exp=-60)
+    IB = W.wide_exponent_intervals(rng, k, n, -1, 0, rad_exp=-60)
+    IA[..., 1] = IA[..., 0] = np.abs(IA[..., 0])   # one sign: no cancellation
+    IB[..., 1] = IB[..., 0] = np.abs(IB[..., 0])
+    beta = O.ozaki_beta(k, 20)
+    assert beta == 62
+    Ap, e = O.ozaki_scaled(IA, beta, rows=True)
+    Bp, f = O.ozaki_scaled(IB, beta, rows=False)
+    Z = O.ozaki_igemm(IA, IB, 20)
+    for i in range(m):
+        for j in range(n):
+            X = sum(int(Ap[i, c]) * int(Bp[c, j]) for c in range(k))
+            assert abs(X) >= 2 ** 127
+            exact = F(X) * F(2) ** (int(e[i]) + int(f[j]) - 2 * beta)
+            assert F(Z[i, j, 0]) <= exact <= F(Z[i, j, 1])
+            assert F(Z[i, j, 1]) - F(Z[i, j, 0]) <= abs(exact) * F(2) ** -40
 
 
 def test_ozaki_point_and_zero_inputs():

@@ -28,6 +28,9 @@ PARAM_OZ_NMOD = 1
 PARAM_OZ_VARIANT = 2
 PARAM_STAGE_TIMING = 3
 PARAM_DD_NMOD = 4
+PARAM_OZ_RADIUS = 5
+OZ_RADIUS_FULL = 1
+OZ_RADIUS_FOLDED = 2
 
 ERRORS = {
     -1: "dimension", -2: "argument", -3: "device", -4: "oom", -5: "rounding",
@@ -99,8 +102,9 @@ SIGNATURES = {
     "vcphip_tune_igemm_tile": [_vp, _c_int, _c_int, _c_int, _p_int, _p_dbl],
     "vcphip_tune_ddgemm_tile": [_vp, _c_int, _c_int, _c_int, _p_int, _p_dbl],
     "vcphip_tune_igemm_moduli_dev": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, ctypes.c_double, _c_int,
-                                     _p_int, _p_dbl],
+                                     _p_int, _p_int, _p_dbl],
     "vcphip_tune_residual_dev": [_vp, _c_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _p_dbl],
+    "vcphip_ozaki_plan": [_vp, _c_int, _c_int, _c_int, _p_int, _p_int, _p_int, _p_int],
     "vcphip_tune_save": [_vp, ctypes.c_char_p],
     "vcphip_tune_load": [_vp, ctypes.c_char_p],
 }
@@ -653,12 +657,21 @@ class Context:
         return v.value, ms.value
 
     def tune_igemm_moduli_dev(self, m, n, k, dA, lda, dB, ldb, max_ratio=2.0, sample_rows=0):
-        """fewest moduli keeping sampled radii within max_ratio of the plain form: (nmod, ratio)"""
-        nm, r = ctypes.c_int(), ctypes.c_double()
+        """cheapest (moduli, radius scheme) keeping sampled radii within max_ratio
+        of the plain form: (nmod, radius, ratio), radius one of OZ_RADIUS_*"""
+        nm, rad, r = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         _check(lib().vcphip_tune_igemm_moduli_dev(self._h, m, n, k, _vp(dA), lda, _vp(dB), ldb, float(max_ratio),
-                                                  int(sample_rows), ctypes.byref(nm), ctypes.byref(r)),
+                                                  int(sample_rows), ctypes.byref(nm), ctypes.byref(rad),
+                                                  ctypes.byref(r)),
                "vcphip_tune_igemm_moduli_dev")
-        return nm.value, r.value
+        return nm.value, rad.value, r.value
+
+    def ozaki_plan(self, m, n, k):
+        """moduli, radius scheme, tile variant and int8 plane products an
+        interval Ozaki product of this shape uses in this context"""
+        v = [ctypes.c_int() for _ in range(4)]
+        _check(lib().vcphip_ozaki_plan(self._h, m, n, k, *[ctypes.byref(x) for x in v]), "vcphip_ozaki_plan")
+        return {"moduli": v[0].value, "radius": v[1].value, "variant": v[2].value, "products": v[3].value}
 
     def tune_save(self, path):
         _check(lib().vcphip_tune_save(self._h, os.fsencode(path)), "vcphip_tune_save")

@@ -177,6 +177,42 @@ VCPHIP_DEVINL unsigned digits2_ru(double x, int g) {
   return (unsigned)hf | ((unsigned)__builtin_ceil(r) << 8);
 }
 
+// the same bound with a non-positive low digit (B's factors):
+//   x <= (H + L/127) 2^g / 127,  H = h + [l > 0] <= 127,  L = l - 127 [l > 0] in [-126, 0]
+// (h = 127 only when t = 127, and then l = 0).  Against A's (h1, l1 >= 0)
+// the product of the two bounds is h1 H + (h1 L + l1 H)/127 + l1 L/127^2,
+// whose last term is never positive: the digit GEMMs leave it out.
+VCPHIP_DEVINL unsigned digits2_ru_neglo(double x, int g) {
+  const unsigned d = digits2_ru(x, g);
+  const unsigned h = d & 0xffu, l = d >> 8;
+  return l ? ((h + 1u) | (((l - 127u) & 0xffu) << 8)) : d;
+}
+
+// operand planes of GEMM output z.  Residue z < nmod: A plane z x B plane z.
+// Radius-digit outputs r = z - nmod (planes nmod + 0..3 per operand: term 1
+// hi, lo; term 2 hi, lo):
+//   r = 0, 2  hi x hi of term r/2                 (one pass)
+//   r = 1, 3  hi x lo + lo x hi of term (r-1)/2   (two passes into one sum)
+//   r = 4, 5  lo x lo of term r - 4               (one pass; the full radius
+//             scheme only)
+VCPHIP_DEVINL void oz_pass_planes(int z, int nmod, int& npass, int& pa0, int& pb0, int& pa1, int& pb1) {
+  npass = 1;
+  pa0 = pb0 = pa1 = pb1 = z;
+  if (z < nmod) return;
+  const int r = z - nmod;
+  if (r >= 4) {
+    pa0 = pb0 = pa1 = pb1 = nmod + 1 + 2 * (r - 4);
+    return;
+  }
+  const int base = nmod + 2 * (r >> 1);
+  pa0 = pb0 = pa1 = pb1 = base;
+  if (r & 1) {
+    npass = 2;
+    pb0 = base + 1;
+    pa1 = base + 1;
+  }
+}
+
 typedef unsigned short vcphip_v2u16 __attribute__((ext_vector_type(2)));
 
 // x (|x| <= 2^62) as x = d0 + d1 2^12 + d2 2^24 + d3 2^36 + d4 2^48 + (e5 - 4) 2^60
@@ -318,8 +354,8 @@ __global__ __launch_bounds__(256) void k_oz_split_cols(const double* __restrict_
     vcphip_midrad_ru(v[u].x, v[u].y, bm, br);
     const long long bp = zc ? 0 : rnint_floor(vcphip_mulpow2(bm, P.beta - f));
     ch[u] = chunks12(bp);
-    const unsigned d1 = digits2_ru(br, xi);
-    const unsigned d2 = digits2_ru(__builtin_fabs(bm), zeta);
+    const unsigned d1 = digits2_ru_neglo(br, xi);
+    const unsigned d2 = digits2_ru_neglo(__builtin_fabs(bm), zeta);
     drh |= (d1 & 0xff) << (8 * u);
     drl |= (d1 >> 8) << (8 * u);
     dmh |= (d2 & 0xff) << (8 * u);
@@ -390,17 +426,8 @@ __global__ __launch_bounds__(256) void k_oz_gemm3(int mpad, int npad, int kpad, 
   int tm, tn;
   vcphip_tile_coords(local, ntm, ntn, 8, tm, tn);
   const int row0 = tm * TMB, col0 = tn * TNB;
-  int npass = 1, pa0 = z, pb0 = z, pa1 = z, pb1 = z;
-  if (z >= nmod) {
-    const int r = z - nmod, base = nmod + 2 * (r >> 1);
-    pa0 = pb0 = base;
-    if (r & 1) {
-      npass = 2;
-      pb0 = base + 1;
-      pa1 = base + 1;
-      pb1 = base;
-    }
-  }
+  int npass, pa0, pb0, pa1, pb1;
+  oz_pass_planes(z, nmod, npass, pa0, pb0, pa1, pb1);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w & 1, wn = w >> 1;
   const int wu = __builtin_amdgcn_readfirstlane(w);
@@ -556,17 +583,8 @@ __global__ __launch_bounds__(256) void k_oz_gemm4(int mpad, int npad, int kpad, 
   int tm, tn;
   vcphip_tile_coords(local, ntm, ntn, 8, tm, tn);
   const int row0 = tm * TMB, col0 = tn * TNB;
-  int npass = 1, pa0 = z, pb0 = z, pa1 = z, pb1 = z;
-  if (z >= nmod) {
-    const int r = z - nmod, base = nmod + 2 * (r >> 1);
-    pa0 = pb0 = base;
-    if (r & 1) {
-      npass = 2;
-      pb0 = base + 1;
-      pa1 = base + 1;
-      pb1 = base;
-    }
-  }
+  int npass, pa0, pb0, pa1, pb1;
+  oz_pass_planes(z, nmod, npass, pa0, pb0, pa1, pb1);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w & 1, wn = w >> 1;
   const int wu = __builtin_amdgcn_readfirstlane(w);
@@ -760,17 +778,8 @@ __global__ __launch_bounds__(64 * NW) void k_oz_gemm5(int mpad, int npad, int kp
   int tm, tn;
   vcphip_tile_coords(local, ntm, ntn, 8, tm, tn);
   const int row0 = tm * TMB, col0 = tn * TNB;
-  int npass = 1, pa0 = z, pb0 = z, pa1 = z, pb1 = z;
-  if (z >= nmod) {
-    const int r = z - nmod, base = nmod + 2 * (r >> 1);
-    pa0 = pb0 = base;
-    if (r & 1) {
-      npass = 2;
-      pb0 = base + 1;
-      pa1 = base + 1;
-      pb1 = base;
-    }
-  }
+  int npass, pa0, pb0, pa1, pb1;
+  oz_pass_planes(z, nmod, npass, pa0, pb0, pa1, pb1);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w & 1, wn = w >> 1;
   const int wu = __builtin_amdgcn_readfirstlane(w);
@@ -936,7 +945,7 @@ VCPHIP_DEVINL void conv_limbs2(const unsigned (&mag)[NL], int s, double& tz, dou
 }
 
 template <int N>
-__global__ __launch_bounds__(256) void k_oz_recon(int m, int n, int k, int mpad, int npad, VcphipOzParams P,
+__global__ __launch_bounds__(256) void k_oz_recon(int m, int n, int k, int mpad, int npad, int nrad, VcphipOzParams P,
                                                   VcphipOzSide sa, VcphipOzSide sb,
                                                   const uint8_t* __restrict__ U, const int32_t* __restrict__ Z,
                                                   double* __restrict__ C, long ldc) {
@@ -963,15 +972,31 @@ __global__ __launch_bounds__(256) void k_oz_recon(int m, int n, int k, int mpad,
   conv_limbs2(mag, sc, tz, aw);
   const double lo_mid = neg ? -aw : tz, hi_mid = neg ? -tz : aw;
   // radius term (|mA|+rA) rB + rA |mB| (pidblas :277-283) from the digit
-  // products: with x <= (h + l/127) 2^g/127 per factor,
-  //   sum x1 x2 <= 2^(g1+g2)/127^2 (Zhh + Zmix/127 + k),  l1 l2 <= 127^2.
+  // products: with x1 <= (h1 + l1/127) 2^g1/127 (A) and x2 <= (H + L/127)
+  // 2^g2/127 (B, L <= 0),
+  //   sum x1 x2 <= 2^(g1+g2)/127^2 (Zhh + Zmix/127 + Zll/127^2),
+  // Zhh = sum h1 H, Zmix = sum h1 L + l1 H, Zll = sum l1 L <= 0.  The full
+  // scheme (nrad = 6) has Zll; the folded one (nrad = 4) leaves it out,
+  // which only raises the bound.  Each quotient is bounded above by the
+  // RU or, for a negative numerator, the RD reciprocal.
   double t1 = 0.0, t2 = 0.0;
+  const bool full = nrad > 4;
   if (!((fa & VCPHIP_OZ_SUMZERO) || (fb & VCPHIP_OZ_RADZERO))) {
-    const double u = ((double)Z[boff] + (double)Z[plane + boff] * P.c1) + (double)k * P.c127;
+    const int zm = Z[plane + boff];
+    double u = (double)Z[boff] + (double)zm * (zm >= 0 ? P.c1 : P.c1dn);
+    if (full) {
+      const int zl = Z[4 * plane + boff];
+      u = u + (double)zl * (zl >= 0 ? P.c127 : P.c127dn);
+    }
     t1 = vcphip_mulpow2(u, sa.g[i] + sb.h[j]) * P.c127;
   }
   if (!((fa & VCPHIP_OZ_RADZERO) || zc)) {
-    const double u = ((double)Z[2 * plane + boff] + (double)Z[3 * plane + boff] * P.c1) + (double)k * P.c127;
+    const int zm = Z[3 * plane + boff];
+    double u = (double)Z[2 * plane + boff] + (double)zm * (zm >= 0 ? P.c1 : P.c1dn);
+    if (full) {
+      const int zl = Z[5 * plane + boff];
+      u = u + (double)zl * (zl >= 0 ? P.c127 : P.c127dn);
+    }
     t2 = vcphip_mulpow2(u, sa.h[i] + fj) * P.c127;
   }
   const double rt = t1 + t2;
@@ -1129,14 +1154,15 @@ int vcphip_oz_gemm(int mpad, int npad, int kpad, int nmod, int nrad, const int8_
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int vcphip_oz_recon(int m, int n, int k, int mpad, int npad, const VcphipOzParams& P, VcphipOzSide sa,
+int vcphip_oz_recon(int m, int n, int k, int mpad, int npad, int nrad, const VcphipOzParams& P, VcphipOzSide sa,
                     VcphipOzSide sb, const uint8_t* U, const int32_t* Z, double* C, long ldc,
                     hipStream_t s) {
   if (m <= 0 || n <= 0) return 0;
+  if (nrad != VCPHIP_OZ_NZ_FOLDED && nrad != VCPHIP_OZ_NZ_FULL) return -7;
   dim3 grid((unsigned)(mpad / 16), (unsigned)(npad / 16));
   switch (P.nmod) {
 #define VCPHIP_RECON_CASE(NN) \
-  case NN: hipLaunchKernelGGL((k_oz_recon<NN>), grid, dim3(256), 0, s, m, n, k, mpad, npad, P, sa, sb, U, Z, C, ldc); break;
+  case NN: hipLaunchKernelGGL((k_oz_recon<NN>), grid, dim3(256), 0, s, m, n, k, mpad, npad, nrad, P, sa, sb, U, Z, C, ldc); break;
     VCPHIP_RECON_CASE(8) VCPHIP_RECON_CASE(9) VCPHIP_RECON_CASE(10) VCPHIP_RECON_CASE(11)
     VCPHIP_RECON_CASE(12) VCPHIP_RECON_CASE(13) VCPHIP_RECON_CASE(14) VCPHIP_RECON_CASE(15)
     VCPHIP_RECON_CASE(16) VCPHIP_RECON_CASE(17) VCPHIP_RECON_CASE(18) VCPHIP_RECON_CASE(19)

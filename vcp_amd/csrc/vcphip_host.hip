@@ -31,7 +31,7 @@ struct vcphip_ctx {
   int device = 0;
   hipStream_t own = nullptr;
   hipStream_t stream = nullptr;
-  long long oz_nmod = 0, oz_variant = 0, timing = 0, dd_nmod = 0;
+  long long oz_nmod = 0, oz_variant = 0, timing = 0, dd_nmod = 0, oz_radius = 0;
   // workspace slots (device memory, grown on demand)
   enum { WS_PA, WS_PB, WS_U, WS_Z, WS_SA, WS_SB, WS_HA, WS_HB, WS_HC, WS_HC2, WS_BATCH, WS_DD, WS_LSS, WS_HR, WS_LSSB,
          WS_INV, WS_DDS, WS_IGB, WS_N };
@@ -65,7 +65,7 @@ struct vcphip_ctx {
   // autotuner table: choices per product kind and size class (ceil(log2) of
   // m, n, k); 0 in a field means "not tuned"
   struct Tune {
-    int kind, lm, ln, lk, variant, nmod;
+    int kind, lm, ln, lk, variant, nmod, radius;
   };
   std::vector<Tune> tune;
   int resid_per_cu = 3;   // persistent workgroups per CU of the batched residual (its occupancy)
@@ -158,15 +158,29 @@ const vcphip_ctx::Tune* tune_find(const vcphip_ctx* c, int kind, int m, int n, i
   return nullptr;
 }
 
-void tune_put(vcphip_ctx* c, int kind, int lm, int ln, int lk, int variant, int nmod) {
+void tune_put(vcphip_ctx* c, int kind, int lm, int ln, int lk, int variant, int nmod, int radius = 0) {
   for (auto& t : c->tune)
     if (t.kind == kind && t.lm == lm && t.ln == ln && t.lk == lk) {
       if (variant) t.variant = variant;
       if (nmod) t.nmod = nmod;
+      if (radius) t.radius = radius;
       return;
     }
-  c->tune.push_back(vcphip_ctx::Tune{kind, lm, ln, lk, variant, nmod});
+  c->tune.push_back(vcphip_ctx::Tune{kind, lm, ln, lk, variant, nmod, radius});
 }
+
+// radius-digit scheme of an interval Ozaki product: the parameter, else the
+// tuned choice, else by k: the folded scheme's extra term (A's low digits
+// against a unit of B's digit) averages out over thousands of products, so
+// from k = 4096 it gives the full scheme's radii (max ratio to the plain form
+// 0.88 either way at n = k = 8192 on exponents spread over 41 binades) for
+// two plane products less; below that the full scheme
+int oz_radius_scheme(const vcphip_ctx* c, const vcphip_ctx::Tune* tuned, int k) {
+  if (c->oz_radius) return (int)c->oz_radius;
+  if (tuned && tuned->radius) return tuned->radius;
+  return k >= VCPHIP_OZ_FOLDED_MINK ? VCPHIP_OZ_RADIUS_FOLDED : VCPHIP_OZ_RADIUS_FULL;
+}
+int oz_nz(int radius) { return radius == VCPHIP_OZ_RADIUS_FOLDED ? VCPHIP_OZ_NZ_FOLDED : VCPHIP_OZ_NZ_FULL; }
 
 int gcd_i(int a, int b) {
   while (b) {
@@ -294,8 +308,11 @@ int oz_fill_params(int k, int nmod, int extra, int beta_cap, VcphipOzParams* P) 
   }
   {
     const int old = fegetround();
-    fesetround(FE_UPWARD);
     volatile double one = 1.0, d = 16129.0, d1 = 127.0;
+    fesetround(FE_DOWNWARD);
+    P->c1dn = one / d1;
+    P->c127dn = one / d;
+    fesetround(FE_UPWARD);
     P->c127 = one / d;
     P->c1 = one / d1;
     fesetround(old);
@@ -461,6 +478,10 @@ extern "C" int vcphip_ctx_set_param(vcphip_ctx* c, int key, long long v) {
       if (v != 0 && (v < 24 || v > VCPHIP_OZ_MAXMOD)) return VCPHIP_ERR_ARGUMENT;
       c->dd_nmod = v;
       return VCPHIP_OK;
+    case VCPHIP_PARAM_OZ_RADIUS:
+      if (v != 0 && v != VCPHIP_OZ_RADIUS_FULL && v != VCPHIP_OZ_RADIUS_FOLDED) return VCPHIP_ERR_ARGUMENT;
+      c->oz_radius = v;
+      return VCPHIP_OK;
     default: return VCPHIP_ERR_ARGUMENT;
   }
 }
@@ -472,6 +493,7 @@ extern "C" int vcphip_ctx_get_param(vcphip_ctx* c, int key, long long* v) {
     case VCPHIP_PARAM_OZ_VARIANT: *v = c->oz_variant; return VCPHIP_OK;
     case VCPHIP_PARAM_STAGE_TIMING: *v = c->timing; return VCPHIP_OK;
     case VCPHIP_PARAM_DD_NMOD: *v = c->dd_nmod; return VCPHIP_OK;
+    case VCPHIP_PARAM_OZ_RADIUS: *v = c->oz_radius; return VCPHIP_OK;
     default: return VCPHIP_ERR_ARGUMENT;
   }
 }
@@ -503,6 +525,22 @@ extern "C" int vcphip_ozaki_config(int k, int nmod, int* beta, int* moduli, int 
   return nmod;
 }
 
+extern "C" int vcphip_ozaki_plan(vcphip_ctx* c, int m, int n, int k, int* nmod, int* radius, int* variant,
+                                 int* products) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (m < 0 || n < 0 || k < 0) return VCPHIP_ERR_DIMENSION;
+  const vcphip_ctx::Tune* tuned = tune_find(c, TUNE_IGEMM, m, n, k);
+  const int nm = c->oz_nmod ? (int)c->oz_nmod : (tuned && tuned->nmod ? tuned->nmod : default_nmod(k));
+  const int rad = oz_radius_scheme(c, tuned, k);
+  const int var = c->oz_variant ? (int)c->oz_variant : (tuned && tuned->variant ? tuned->variant : DEFAULT_VARIANT);
+  if (nmod) *nmod = nm;
+  if (radius) *radius = rad;
+  if (variant) *variant = var;
+  // residue products plus hi x hi, two mixed and (full) lo x lo per radius term
+  if (products) *products = nm + oz_nz(rad) + 2;
+  return VCPHIP_OK;
+}
+
 // ===========================================================================
 // device entry points
 // ===========================================================================
@@ -522,10 +560,11 @@ static int oz_igemm_dev(vcphip_ctx* c, int m, int n, int k, const double* A, int
   const int nchunk = k > 0 ? (k + VCPHIP_OZ_CHUNK - 1) / VCPHIP_OZ_CHUNK : 0;
   void *pa, *pb, *pu, *pz, *sa, *sb;
   const size_t nplane = (size_t)nmod + VCPHIP_OZ_NDIGIT;
+  const int nz = oz_nz(oz_radius_scheme(c, tuned, k));
   if ((rc = ws_get(c, vcphip_ctx::WS_PA, nplane * mpad * kpad, &pa))) return rc;
   if ((rc = ws_get(c, vcphip_ctx::WS_PB, nplane * npad * kpad, &pb))) return rc;
   if ((rc = ws_get(c, vcphip_ctx::WS_U, (size_t)nmod * mpad * npad, &pu))) return rc;
-  if ((rc = ws_get(c, vcphip_ctx::WS_Z, VCPHIP_OZ_NDIGIT * sizeof(int32_t) * (size_t)mpad * npad, &pz))) return rc;
+  if ((rc = ws_get(c, vcphip_ctx::WS_Z, (size_t)nz * sizeof(int32_t) * (size_t)mpad * npad, &pz))) return rc;
   auto side_bytes = [&](int rows) { return (size_t)rows * (4 * sizeof(int) + sizeof(double)) + sizeof(double) * 5 * (size_t)(nchunk > 0 ? nchunk : 1) * rows + 1024; };
   if ((rc = ws_get(c, vcphip_ctx::WS_SA, side_bytes(m), &sa))) return rc;
   if ((rc = ws_get(c, vcphip_ctx::WS_SB, side_bytes(n), &sb))) return rc;
@@ -549,11 +588,11 @@ static int oz_igemm_dev(vcphip_ctx* c, int m, int n, int k, const double* A, int
   if ((rc = check_launch(vcphip_oz_split_rows(A, lda, m, k, mpad, kpad, P, SA, (int8_t*)pa, c->stream)))) return rc;
   if ((rc = check_launch(vcphip_oz_split_cols(B, ldb, k, n, npad, kpad, P, SB, (int8_t*)pb, c->stream)))) return rc;
   T.mark("split");
-  if ((rc = check_launch(vcphip_oz_gemm(mpad, npad, kpad, nmod, VCPHIP_OZ_NDIGIT, (const int8_t*)pa, (const int8_t*)pb, P,
+  if ((rc = check_launch(vcphip_oz_gemm(mpad, npad, kpad, nmod, nz, (const int8_t*)pa, (const int8_t*)pb, P,
                                         (uint8_t*)pu, (int32_t*)pz, variant, c->stream))))
     return rc;
   T.mark("gemm");
-  if ((rc = check_launch(vcphip_oz_recon(m, n, k, mpad, npad, P, SA, SB, (const uint8_t*)pu, (const int32_t*)pz, C, ldc,
+  if ((rc = check_launch(vcphip_oz_recon(m, n, k, mpad, npad, nz, P, SA, SB, (const uint8_t*)pu, (const int32_t*)pz, C, ldc,
                                          c->stream))))
     return rc;
   T.mark("recon");
@@ -2299,16 +2338,16 @@ int tune_gemm_stage(vcphip_ctx* c, bool dd, int m, int n, int k, int* best_varia
     rc = oz_make_params(k, nmod, &P);
   }
   if (rc) return rc;
-  const int nrad = dd ? 0 : VCPHIP_OZ_NDIGIT;
+  const int nrad = dd ? 0 : oz_nz(oz_radius_scheme(c, tuned, k));
   const int mpad = (m + VCPHIP_OZ_PADMN - 1) / VCPHIP_OZ_PADMN * VCPHIP_OZ_PADMN;
   const int npad = (n + VCPHIP_OZ_PADMN - 1) / VCPHIP_OZ_PADMN * VCPHIP_OZ_PADMN;
   const int kpad = (k + VCPHIP_OZ_PADK - 1) / VCPHIP_OZ_PADK * VCPHIP_OZ_PADK;
   void *pa, *pb, *pu, *pz = nullptr;
-  const size_t nplane = (size_t)nmod + nrad;
+  const size_t nplane = (size_t)nmod + (dd ? 0 : VCPHIP_OZ_NDIGIT);
   if ((rc = ws_get(c, vcphip_ctx::WS_PA, nplane * mpad * kpad, &pa))) return rc;
   if ((rc = ws_get(c, vcphip_ctx::WS_PB, nplane * npad * kpad, &pb))) return rc;
   if ((rc = ws_get(c, vcphip_ctx::WS_U, (size_t)nmod * mpad * npad, &pu))) return rc;
-  if (!dd && (rc = ws_get(c, vcphip_ctx::WS_Z, VCPHIP_OZ_NDIGIT * sizeof(int32_t) * (size_t)mpad * npad, &pz)))
+  if (!dd && (rc = ws_get(c, vcphip_ctx::WS_Z, (size_t)nrad * sizeof(int32_t) * (size_t)mpad * npad, &pz)))
     return rc;
   // operand planes of random bytes: timings on the zeros of fresh memory do
   // not carry over to real operands
@@ -2320,7 +2359,7 @@ int tune_gemm_stage(vcphip_ctx* c, bool dd, int m, int n, int k, int* best_varia
     hipEventDestroy(e0);
     return VCPHIP_ERR_DEVICE;
   }
-  const double flops = 2.0 * mpad * npad * (double)kpad * (double)(nmod + (dd ? 0 : 6));
+  const double flops = 2.0 * mpad * npad * (double)kpad * (double)(nmod + (dd ? 0 : nrad + 2));
   const int reps = flops > 1e13 ? 2 : (flops > 1e11 ? 4 : 10);
   int best = 0;
   double best_t = 0.0;
@@ -2375,7 +2414,7 @@ extern "C" int vcphip_tune_ddgemm_tile(vcphip_ctx* c, int m, int n, int k, int* 
 
 extern "C" int vcphip_tune_igemm_moduli_dev(vcphip_ctx* c, int m, int n, int k, const double* IA, int lda,
                                             const double* IB, int ldb, double max_ratio, int sample_rows, int* nmod,
-                                            double* ratio) {
+                                            int* radius, double* ratio) {
   if (!c || !IA || !IB || !(max_ratio > 0.0)) return VCPHIP_ERR_ARGUMENT;
   if (m <= 0 || n <= 0 || k <= 0 || lda < m || ldb < k) return VCPHIP_ERR_DIMENSION;
   if (k > VCPHIP_OZ_MAXK) return VCPHIP_ERR_UNSUPPORTED;
@@ -2397,36 +2436,55 @@ extern "C" int vcphip_tune_igemm_moduli_dev(vcphip_ctx* c, int m, int n, int k, 
   if ((rc = check_launch(vcphip_gather_rows(IA, lda, m, k, s, As, c->stream)))) return rc;
   if ((rc = check_launch(vcphip_launch_drgemm(VCPHIP_DR_II, s, n, k, As, s, IB, ldb, Cp, s, nullptr, 0, c->stream))))
     return rc;
-  const long long saved = c->oz_nmod;
+  const long long saved = c->oz_nmod, saved_rad = c->oz_radius;
   const long long saved_timing = c->timing;
   c->timing = 0;
   std::vector<double> hpart(nblocks);
-  int chosen = 0;
-  double chosen_ratio = 0.0;
-  for (int t = 12; t <= VCPHIP_OZ_MAXMOD_IV; ++t) {
-    VcphipOzParams P;
-    if (oz_make_params(k, t, &P)) continue;   // too few moduli for this k
-    c->oz_nmod = t;
-    rc = oz_igemm_dev(c, s, n, k, As, s, IB, ldb, Co, s);
-    if (rc) break;
-    if ((rc = check_launch(vcphip_ratio_max(Co, Cp, (long)s * n, part, nblocks, c->stream)))) break;
-    if (hipMemcpyAsync(hpart.data(), part, sizeof(double) * nblocks, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-        hipStreamSynchronize(c->stream) != hipSuccess) {
-      rc = VCPHIP_ERR_DEVICE;
-      break;
+  // candidates in order of their plane products (N + 6 folded, N + 8 full);
+  // the first cost with a candidate within max_ratio decides, the smaller
+  // sampled ratio winning a tie of cost
+  int chosen = 0, chosen_rad = 0, last = 0, last_rad = 0;
+  double chosen_ratio = 0.0, last_ratio = 0.0;
+  for (int cost = 12 + 6; cost <= VCPHIP_OZ_MAXMOD_IV + 8 && !chosen && !rc; ++cost) {
+    for (const int rad : {VCPHIP_OZ_RADIUS_FOLDED, VCPHIP_OZ_RADIUS_FULL}) {
+      const int t = cost - (rad == VCPHIP_OZ_RADIUS_FOLDED ? 6 : 8);
+      VcphipOzParams P;
+      if (t < 12 || t > VCPHIP_OZ_MAXMOD_IV || oz_make_params(k, t, &P)) continue;   // too few moduli for this k
+      c->oz_nmod = t;
+      c->oz_radius = rad;
+      rc = oz_igemm_dev(c, s, n, k, As, s, IB, ldb, Co, s);
+      if (rc) break;
+      if ((rc = check_launch(vcphip_ratio_max(Co, Cp, (long)s * n, part, nblocks, c->stream)))) break;
+      if (hipMemcpyAsync(hpart.data(), part, sizeof(double) * nblocks, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+          hipStreamSynchronize(c->stream) != hipSuccess) {
+        rc = VCPHIP_ERR_DEVICE;
+        break;
+      }
+      double mx = 0.0;
+      for (double v : hpart) mx = std::max(mx, v);
+      last = t;
+      last_rad = rad;
+      last_ratio = mx;
+      if (mx <= max_ratio && (!chosen || mx < chosen_ratio)) {
+        chosen = t;
+        chosen_rad = rad;
+        chosen_ratio = mx;
+      }
     }
-    double mx = 0.0;
-    for (double v : hpart) mx = std::max(mx, v);
-    chosen = t;
-    chosen_ratio = mx;
-    if (mx <= max_ratio) break;
   }
   c->oz_nmod = saved;
+  c->oz_radius = saved_rad;
   c->timing = saved_timing;
   if (rc) return rc;
+  if (!chosen) {   // none within max_ratio: the most moduli, full scheme
+    chosen = last;
+    chosen_rad = last_rad;
+    chosen_ratio = last_ratio;
+  }
   if (!chosen) return VCPHIP_ERR_UNSUPPORTED;
-  tune_put(c, TUNE_IGEMM, size_class(m), size_class(n), size_class(k), 0, chosen);
+  tune_put(c, TUNE_IGEMM, size_class(m), size_class(n), size_class(k), 0, chosen, chosen_rad);
   if (nmod) *nmod = chosen;
+  if (radius) *radius = chosen_rad;
   if (ratio) *ratio = chosen_ratio;
   return VCPHIP_OK;
 }
@@ -2498,9 +2556,10 @@ extern "C" int vcphip_tune_save(vcphip_ctx* c, const char* path) {
   if (!c || !path) return VCPHIP_ERR_ARGUMENT;
   FILE* f = fopen(path, "w");
   if (!f) return VCPHIP_ERR_ARGUMENT;
-  fprintf(f, "vcphip-tune 1\n");
+  fprintf(f, "vcphip-tune 2\n");
   for (const auto& t : c->tune)
-    fprintf(f, "%s %d %d %d %d %d\n", t.kind == TUNE_IGEMM ? "igemm" : "ddgemm", t.lm, t.ln, t.lk, t.variant, t.nmod);
+    fprintf(f, "%s %d %d %d %d %d %d\n", t.kind == TUNE_IGEMM ? "igemm" : "ddgemm", t.lm, t.ln, t.lk, t.variant,
+            t.nmod, t.radius);
   fprintf(f, "residual %d\n", c->resid_per_cu);
   const bool ok = fclose(f) == 0;
   return ok ? VCPHIP_OK : VCPHIP_ERR_ARGUMENT;
@@ -2512,7 +2571,7 @@ extern "C" int vcphip_tune_load(vcphip_ctx* c, const char* path) {
   if (!f) return VCPHIP_ERR_ARGUMENT;
   char line[256];
   int version = 0;
-  if (!fgets(line, sizeof line, f) || sscanf(line, "vcphip-tune %d", &version) != 1 || version != 1) {
+  if (!fgets(line, sizeof line, f) || sscanf(line, "vcphip-tune %d", &version) != 1 || version < 1 || version > 2) {
     fclose(f);
     return VCPHIP_ERR_ARGUMENT;
   }
@@ -2521,17 +2580,19 @@ extern "C" int vcphip_tune_load(vcphip_ctx* c, const char* path) {
   int rc = VCPHIP_OK;
   while (fgets(line, sizeof line, f)) {
     char kind[32];
-    int lm, ln, lk, v, nm, p;
-    if (sscanf(line, "%31s %d %d %d %d %d", kind, &lm, &ln, &lk, &v, &nm) == 6) {
+    int lm, ln, lk, v, nm, p, rad = 0;
+    const int nf = sscanf(line, "%31s %d %d %d %d %d %d", kind, &lm, &ln, &lk, &v, &nm, &rad);
+    if (nf == 6 + (version >= 2 ? 1 : 0)) {   // version 1 lines carry no radius scheme
       const bool iv = strcmp(kind, "igemm") == 0, dd = strcmp(kind, "ddgemm") == 0;
       const bool ok = (iv || dd) && lm >= 0 && lm <= 31 && ln >= 0 && ln <= 31 && lk >= 0 && lk <= 31 &&
-                      (v == 0 || (v >= 6 && v <= 14)) && (nm == 0 || (iv && nm >= 8 && nm <= VCPHIP_OZ_MAXMOD_IV) ||
-                                  (dd && nm >= 24 && nm <= VCPHIP_OZ_MAXMOD));
+                      (v == 0 || (v >= 6 && v <= 16)) && (nm == 0 || (iv && nm >= 8 && nm <= VCPHIP_OZ_MAXMOD_IV) ||
+                                  (dd && nm >= 24 && nm <= VCPHIP_OZ_MAXMOD)) &&
+                      (rad == 0 || (iv && (rad == VCPHIP_OZ_RADIUS_FULL || rad == VCPHIP_OZ_RADIUS_FOLDED)));
       if (!ok) {
         rc = VCPHIP_ERR_ARGUMENT;
         break;
       }
-      table.push_back(vcphip_ctx::Tune{iv ? TUNE_IGEMM : TUNE_DDGEMM, lm, ln, lk, v, nm});
+      table.push_back(vcphip_ctx::Tune{iv ? TUNE_IGEMM : TUNE_DDGEMM, lm, ln, lk, v, nm, rad});
     } else if (sscanf(line, "residual %d", &p) == 1 && p >= 1 && p <= 4) {
       per_cu = p;
     } else if (line[0] != '\n' && line[0] != '#') {
@@ -2541,7 +2602,7 @@ extern "C" int vcphip_tune_load(vcphip_ctx* c, const char* path) {
   }
   fclose(f);
   if (rc) return rc;
-  for (const auto& t : table) tune_put(c, t.kind, t.lm, t.ln, t.lk, t.variant, t.nmod);
+  for (const auto& t : table) tune_put(c, t.kind, t.lm, t.ln, t.lk, t.variant, t.nmod, t.radius);
   c->resid_per_cu = per_cu;
   return VCPHIP_OK;
 }

@@ -74,13 +74,24 @@ struct VcphipOzParams {
   float r_rcp[VCPHIP_OZ_MAXMOD];
   uint32_t Mlimb[VCPHIP_OZ_NLIMB];  // M
   double c1;                   // RU(1/127)
+  double c1dn;                 // RD(1/127)
   double c127;                 // RU(1/127^2)
+  double c127dn;               // RD(1/127^2)
 };
 
 // radius-digit planes that follow the N residue planes (two 7-bit upward
-// digits hi/lo per factor):  A: |mid|+rad hi, lo, rad hi, lo;
-//                            B: rad hi, lo, |mid| hi, lo
+// digits hi/lo per factor):  A: |mid|+rad hi, lo, rad hi, lo (h, l >= 0);
+//                            B: rad hi, lo, |mid| hi, lo with the low digit
+// non-positive (H = h + [l > 0], L = l - 127 [l > 0]: the same bound), so
+// that the lo*lo term left out of the digit products is never positive
 #define VCPHIP_OZ_NDIGIT 4
+// radius-digit GEMM outputs (int32 planes of Z): hi x hi and hi x lo + lo x hi
+// of each term (folded scheme, 6 plane products), plus lo x lo of each term
+// (full scheme, 8 plane products)
+#define VCPHIP_OZ_NZ_FOLDED 4
+#define VCPHIP_OZ_NZ_FULL 6
+// untuned products with k at least this use the folded scheme
+#define VCPHIP_OZ_FOLDED_MINK 4096
 #define VCPHIP_OZ_MAXK 65536  // 2 k 127^2 < 2^31 for the mixed digit products
 
 // per-row (A) / per-column (B) statistics of a midpoint-radius operand
@@ -110,7 +121,8 @@ int vcphip_oz_split_cols(const double* B, long ldb, int k, int n, int npad, int 
 int vcphip_oz_gemm(int mpad, int npad, int kpad, int nmod, int nrad, const int8_t* Aplanes,
                    const int8_t* Bplanes, const VcphipOzParams& P, uint8_t* U, int32_t* Z, int variant,
                    hipStream_t s);
-int vcphip_oz_recon(int m, int n, int k, int mpad, int npad, const VcphipOzParams& P, VcphipOzSide sa,
+// nrad: VCPHIP_OZ_NZ_FULL or VCPHIP_OZ_NZ_FOLDED radius products in Z
+int vcphip_oz_recon(int m, int n, int k, int mpad, int npad, int nrad, const VcphipOzParams& P, VcphipOzSide sa,
                     VcphipOzSide sb, const uint8_t* U, const int32_t* Z, double* C, long ldc,
                     hipStream_t s);
 
