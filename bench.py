@@ -45,6 +45,7 @@ def main():
                          "kernel for FETCH_SIZE / WRITE_SIZE counter runs")
     ap.add_argument("--no-infsup", action="store_true")
     ap.add_argument("--no-tuned", action="store_true")
+    ap.add_argument("--no-config1", action="store_true")
     ap.add_argument("--no-config2", action="store_true")
     ap.add_argument("--no-galerkin", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
@@ -272,6 +273,7 @@ def main():
 
     resid = None if args.no_resid else bench_resid(ctx, args.resid_reps)
     dd = None if args.no_dd else bench_dd(ctx, args.dd_n, args.dd_steps, not args.no_cpu)
+    cfg1 = None if args.no_config1 else bench_config1(ctx, with_cpu=not args.no_cpu)
     cfg2 = None if args.no_config2 else bench_config2(ctx)
     galerkin = None if args.no_galerkin else bench_galerkin(ctx)
     batched = None if args.no_batched else bench_igemm_batched(ctx)
@@ -304,6 +306,7 @@ def main():
         "cpu_baseline": cpu,
         "batched_residual": resid,
         "dd_gemm": dd,
+        "interval_n1024": cfg1,
         "plain_n4096": cfg2,
         "galerkin_uhphiphi": galerkin,
         "igemm_batched": batched,
@@ -542,6 +545,85 @@ def bench_galerkin(ctx, p=40, q=80, steps=5):
     return out
 
 
+def bench_config1(ctx, n=1024, steps=20, with_cpu=True):
+    """The interval product at n=1024 (mid U[-1,1], radius 2^-40 |mid|): the
+    configuration of the reference's timing demo of this path
+    (test_matrix/test_interval_matrix_mul.cpp:46-72).  Plain and Ozaki forms
+    on the GPU (8 n^3 FP64-equivalent flops each), 4096 sampled entries
+    checked for exact containment, radii against the reference's, and the
+    reference's own pidblas::mulmm (vblas rmatmul, OpenMP) on the host cores
+    for the whole product, median of three."""
+    import numpy as np
+    import torch
+
+    import vcp_amd
+    from vcp_amd import workloads as W
+
+    IA, IB = W.config1(n)
+    dev = torch.device("cuda:0")
+    dA = torch.from_numpy(np.ascontiguousarray(IA.transpose(1, 0, 2))).to(dev)
+    dB = torch.from_numpy(np.ascontiguousarray(IB.transpose(1, 0, 2))).to(dev)
+    dP = torch.empty((n, n, 2), dtype=torch.float64, device=dev)
+    dZ = torch.empty((n, n, 2), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    flops = 8.0 * n ** 3
+
+    def timed(form, out):
+        def run():
+            ctx.igemm_dev(form, n, n, n, dA.data_ptr(), n, dB.data_ptr(), n, out.data_ptr(), n)
+        run()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            run()
+        ctx.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    t_p = timed(vcp_amd.FORM_PLAIN, dP)
+    t_z = timed(vcp_amd.FORM_OZAKI, dZ)
+    plan = ctx.ozaki_plan(n, n, n)
+    out = {"workload": "interval GEMM n=%d, mid U[-1,1], rad=2^-40|mid|" % n, "n": n,
+           "plain": {"ms": round(t_p * 1e3, 4), "tflops": round(flops / t_p / 1e12, 3),
+                     "frac_fp64_peak": round(flops / t_p / 1e12 / PEAK_FP64_TFLOPS, 4)},
+           "ozaki": {"ms": round(t_z * 1e3, 4), "tflops": round(flops / t_z / 1e12, 3), "moduli": plan["moduli"],
+                     "radius_scheme": {vcp_amd.OZ_RADIUS_FULL: "full", vcp_amd.OZ_RADIUS_FOLDED: "folded"}[plan["radius"]]}}
+    P = dP.cpu().numpy().transpose(1, 0, 2)
+    Z = dZ.cpu().numpy().transpose(1, 0, 2)
+    try:
+        import oracle_ffi as O
+
+        rng = np.random.default_rng(W.SEED0 + 1)
+        ii = rng.integers(0, n, 4096)
+        jj = rng.integers(0, n, 4096)
+        ok_p, _, _ = O.check_containment(IA, IB, P, ii, jj)
+        ok_z, _, _ = O.check_containment(IA, IB, Z, ii, jj)
+        out["containment_4096"] = {"plain": float(ok_p.mean()), "ozaki": float(ok_z.mean())}
+        if with_cpu:
+            use_ref = O.ref_lib() is not None
+            product = O.ref_pidblas_mulmm if use_ref else O.pidblas_mulmm
+            ts = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                ref = product(IA, IB)
+                ts.append(time.perf_counter() - t0)
+            t_c = sorted(ts)[1]
+            w_ref = np.maximum(ref[..., 1] - ref[..., 0], np.finfo(float).tiny)
+            rz = (Z[..., 1] - Z[..., 0]) / w_ref
+            out["cpu_reference"] = {
+                "ms": round(t_c * 1e3, 2), "tflops": round(flops / t_c / 1e12, 4),
+                "cores": int(O.lib().orc_get_threads()), "kind": "reference" if use_ref else "port",
+                "what": "pidblas::mulmm with the reference's vblas rmatmul for its four products" if use_ref else
+                        "pidblas::mulmm restated (oracle)",
+                "plain_gpu_bitexact": bool(np.array_equal(P, ref))}
+            out["radius_ratio_ozaki_over_reference"] = {"max": float(rz.max()), "median": float(np.median(rz))}
+            out["gpu_plain_speedup_vs_cpu"] = round(t_c / t_p, 2)
+            out["gpu_ozaki_speedup_vs_cpu"] = round(t_c / t_z, 2)
+    except Exception as e:
+        log("config1 checks skipped: %r" % (e,))
+    log("config1 n=%d %s" % (n, out))
+    return out
+
+
 def bench_config2(ctx, n=4096, steps=3):
     """The plain FP64 directed-rounding forms at n=4096: point RU/RD product
     (rmatmul +1/-1 in one pass, 4 n^3 flops) on A, B U[-1,1], and the
@@ -629,8 +711,9 @@ def bench_config2(ctx, n=4096, steps=3):
 def bench_dd(ctx, n, steps, with_cpu):
     """Double-double product C = A * B (pddblas::mulmm, vcp/pddblas.hpp:318)
     at n x n x n: hi U[-1,1], lo = hi 2^-53 U[-1,1].  Reports time, the error
-    against the exact product on sampled entries (units of 2^-106 |C_ij|), and
-    the same error of the pddblas restatement on the same entries."""
+    against the exact product on 4096 sampled entries in dd ulps (units of
+    2^-105 |C_ij|), and the error of the pddblas restatement beside the
+    GPU's on the same entries of its first rows."""
     import numpy as np
     import torch
 
@@ -671,28 +754,35 @@ def bench_dd(ctx, n, steps, with_cpu):
         import oracle_ffi as O
 
         C = dC.cpu().numpy().transpose(1, 0, 2)
-        rows = 4
+        # 4096 sampled entries against the exact product (superaccumulator),
+        # in dd ulps: units of 2^-105 |c_ij|
         rng = np.random.default_rng(777)
-        jj = rng.integers(0, n, 64)
-        err_gpu, err_ref = [], []
-        cpu_t = None
+        ii = rng.integers(0, n, 4096)
+        jj = rng.integers(0, n, 4096)
+        err_gpu = []
+        for i, j in zip(ii, jj):
+            h, _, e = O.exact_dd_dot(A[i], B[:, j], C[i, j, 0], C[i, j, 1])
+            err_gpu.append(e / (abs(h) * 2.0 ** -105))
+        out["err_vs_exact_dd_ulps"] = {"samples": len(err_gpu), "unit": "2^-105 |c_ij|",
+                                       "max": float(np.max(err_gpu)), "median": float(np.median(err_gpu))}
         if with_cpu:
+            # the pddblas restatement (Ozaki scheme I, dd accumulation) on the
+            # first rows, compared with the GPU on the same entries
+            rows = 8
             t0 = time.perf_counter()
             Cr, _, _ = O.pddblas_mulmm(np.ascontiguousarray(A[:rows].transpose(1, 0, 2)).transpose(1, 0, 2), B)
             cpu_t = time.perf_counter() - t0
-        for i in range(rows):
-            for j in jj:
-                h, _, e = O.exact_dd_dot(A[i], B[:, j], C[i, j, 0], C[i, j, 1])
-                scale = abs(h) * 2.0 ** -106
-                err_gpu.append(e / scale)
-                if with_cpu:
+            jc = rng.integers(0, n, 64)
+            eg, er = [], []
+            for i in range(rows):
+                for j in jc:
+                    h, _, e = O.exact_dd_dot(A[i], B[:, j], C[i, j, 0], C[i, j, 1])
                     _, _, e2 = O.exact_dd_dot(A[i], B[:, j], Cr[i, j, 0], Cr[i, j, 1])
-                    err_ref.append(e2 / scale)
-        out["err_vs_exact_2^-106_rel"] = {"samples": len(err_gpu), "max": float(np.max(err_gpu)),
-                                          "median": float(np.median(err_gpu))}
-        if with_cpu:
-            out["pddblas_restated_err_2^-106_rel"] = {"max": float(np.max(err_ref)),
-                                                      "median": float(np.median(err_ref))}
+                    eg.append(e / (abs(h) * 2.0 ** -105))
+                    er.append(e2 / (abs(h) * 2.0 ** -105))
+            out["same_entries_gpu_vs_pddblas_restated_dd_ulps"] = {
+                "samples": len(eg), "gpu": {"max": float(np.max(eg)), "median": float(np.median(eg))},
+                "pddblas_restated": {"max": float(np.max(er)), "median": float(np.median(er))}}
             out["cpu_pddblas_restated"] = {"dd_tflops": round(2.0 * rows * n * n / cpu_t / 1e12, 6),
                                            "threads": int(O.lib().orc_get_threads()),
                                            "sample": "first %d rows (%d x %d x %d), %.1f s" % (rows, rows, n, n, cpu_t)}

@@ -381,6 +381,12 @@ def ref_lib():
         L.ref_backend.restype = _c_int
         L.ref_pidblas_mulmm.argtypes = [_c_int, _c_int, _c_int, _p_dbl, _p_dbl, _p_dbl]
         L.ref_pidblas_mulmm.restype = None
+        for name in ("ref_pidblas_mul_im_m", "ref_pidblas_mul_m_im"):
+            getattr(L, name).argtypes = [_c_int, _c_int, _c_int, _p_dbl, _p_dbl, _p_dbl]
+            getattr(L, name).restype = None
+        for name in ("ref_pidblas_mulltmm", "ref_pidblas_vmulmm"):
+            getattr(L, name).argtypes = [_c_int, _c_int, _p_dbl, _p_dbl]
+            getattr(L, name).restype = None
         _ref = L
     return _ref
 
@@ -412,6 +418,70 @@ def ref_pidblas_mulmm(IA, IB):
     L.ref_pidblas_mulmm(m, n, k, _dp(A), _dp(B), _dp(Cv))
     C[...] = Cv.transpose(1, 0, 2)
     return C
+
+
+def _ref_pairs_out(rows, cols):
+    return np.zeros((cols, rows, 2))
+
+
+def _ref_pairs_in(X):
+    return np.ascontiguousarray(X.transpose(1, 0, 2))
+
+
+def _ref_pairs_back(Cv):
+    C = pairs(Cv.shape[1], Cv.shape[0])
+    C[...] = Cv.transpose(1, 0, 2)
+    return C
+
+
+def ref_pidblas_mul_im_m(IA, B):
+    """pidblas::mul_im_m (C = IA B, B point) on the reference's rmatmul; None
+    when the reference build is absent."""
+    L = ref_lib()
+    if L is None:
+        return None
+    m, k = IA.shape[:2]
+    n = B.shape[1]
+    Bf = np.asfortranarray(B, dtype=np.float64)
+    Cv = _ref_pairs_out(m, n)
+    L.ref_pidblas_mul_im_m(m, n, k, _dp(_ref_pairs_in(IA)), _dp(Bf), _dp(Cv))
+    return _ref_pairs_back(Cv)
+
+
+def ref_pidblas_mul_m_im(B, IA):
+    """pidblas::mul_m_im (C = B IA, B point) on the reference's rmatmul"""
+    L = ref_lib()
+    if L is None:
+        return None
+    m, k = B.shape
+    n = IA.shape[1]
+    Bf = np.asfortranarray(B, dtype=np.float64)
+    Cv = _ref_pairs_out(m, n)
+    L.ref_pidblas_mul_m_im(m, n, k, _dp(Bf), _dp(_ref_pairs_in(IA)), _dp(Cv))
+    return _ref_pairs_back(Cv)
+
+
+def ref_pidblas_mulltmm(IA):
+    """pidblas::mulltmm (IA^T IA) on the reference's rmatmul"""
+    L = ref_lib()
+    if L is None:
+        return None
+    k, n = IA.shape[:2]
+    Cv = _ref_pairs_out(n, n)
+    L.ref_pidblas_mulltmm(k, n, _dp(_ref_pairs_in(IA)), _dp(Cv))
+    return _ref_pairs_back(Cv)
+
+
+def ref_pidblas_vmulmm(A):
+    """pidblas::vmulmm (verified A^T A of a point matrix) on the reference's rmatmul"""
+    L = ref_lib()
+    if L is None:
+        return None
+    k, n = A.shape
+    Af = np.asfortranarray(A, dtype=np.float64)
+    Cv = _ref_pairs_out(n, n)
+    L.ref_pidblas_vmulmm(k, n, _dp(Af), _dp(Cv))
+    return _ref_pairs_back(Cv)
 
 
 def ref_udmatmul(A, B, CU, CD):

@@ -1,11 +1,11 @@
-"""The host double-double arithmetic of vcp_amd.ddrefine (the elementwise
+"""The host double-double arithmetic of tests/ddrefine.py (the elementwise
 steps between the GPU products of the refinement algorithms), against exact
 rational arithmetic."""
 from fractions import Fraction as F
 
 import numpy as np
 
-from vcp_amd import ddrefine as D
+import ddrefine as D
 
 
 def _dd(rng, n, spread=30):

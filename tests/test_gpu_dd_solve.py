@@ -271,7 +271,7 @@ def test_dd_solve_singular(ctx):
 def test_dd_cholesky_agrees_with_host_arrangement(ctx):
     """the all-device Cholesky against the reference's arrangement restated
     with LAPACK / BLAS on the host and the GPU dd products (ddrefine)"""
-    from vcp_amd import ddrefine
+    import ddrefine
 
     rng = np.random.default_rng(23)
     n = 130
@@ -349,7 +349,7 @@ def _same_up_to_sign(X, Y, tol):
 def test_dd_eigsym_agrees_with_host_arrangement(ctx):
     """the all-device eigensolver against the reference's arrangement
     restated with host LAPACK and host double-double updates (ddrefine)"""
-    from vcp_amd import ddrefine
+    import ddrefine
 
     rng = np.random.default_rng(61)
     n = 96
@@ -374,7 +374,7 @@ def test_dd_eigsym_at_n300_exact_residuals(ctx):
 
 
 def test_dd_eigsymge_agrees_with_host_arrangement(ctx):
-    from vcp_amd import ddrefine
+    import ddrefine
 
     rng = np.random.default_rng(63)
     n = 64

@@ -449,3 +449,55 @@ def test_reference_built_pidblas_product_matches_the_restatement():
     for m, n, k in ((1, 1, 1), (9, 7, 33), (70, 40, 300)):
         IA, IB = W.wide_exponent_intervals(rng, m, k), W.wide_exponent_intervals(rng, k, n)
         assert np.array_equal(O.ref_pidblas_mulmm(IA, IB), O.pidblas_mulmm(IA, IB))
+
+
+@pytest.mark.parametrize("kind", ["wide", "straddle", "uniform"])
+def test_reference_built_mixed_and_transposed_forms_match_the_restatement(kind):
+    """pidblas::mul_im_m, mul_m_im, mulltmm and vmulmm built on the
+    reference's own vblas rmatmul (oracle/_ref), each following its reference
+    function's op order, are bit-identical to the oracle's restatements: an
+    op-order or rounding-mode slip in either changes bits here"""
+    if O.ref_lib() is None:
+        pytest.skip("oracle/_ref not built (needs the reference tree)")
+    rng = np.random.default_rng(78)
+    for m, n, k in ((1, 1, 1), (9, 7, 33), (40, 31, 150)):
+        IA = _iv(rng, m, k, kind)
+        B = np.asfortranarray(rng.uniform(-1, 1, (k, n)) * 2.0 ** rng.integers(-8, 8, (k, n)))
+        assert np.array_equal(O.ref_pidblas_mul_im_m(IA, B), O.pidblas_mul_im_m(IA, B))
+        Bl = np.asfortranarray(rng.uniform(-1, 1, (n, m)) * 2.0 ** rng.integers(-8, 8, (n, m)))
+        assert np.array_equal(O.ref_pidblas_mul_m_im(Bl, IA), O.pidblas_mul_m_im(Bl, IA))
+        assert np.array_equal(O.ref_pidblas_mulltmm(IA), O.pidblas_mulltmm(IA))
+        A = np.asfortranarray(rng.uniform(-1, 1, (k, m)))
+        assert np.array_equal(O.ref_pidblas_vmulmm(A), O.pidblas_vmulmm(A))
+
+
+def test_reference_built_forms_catch_a_planted_slip():
+    """the pin has teeth: a restatement that swaps the rounding of the radius
+    product of mul_im_m (RD instead of RU), planted here in numpy on the
+    oracle's own intermediate values, differs from the reference-built result"""
+    if O.ref_lib() is None:
+        pytest.skip("oracle/_ref not built (needs the reference tree)")
+    rng = np.random.default_rng(79)
+    IA = W.wide_exponent_intervals(rng, 12, 40, rad_exp=-10)
+    B = np.asfortranarray(rng.uniform(-1, 1, (40, 9)))
+    ref = O.ref_pidblas_mul_im_m(IA, B)
+    mA, rA = np.vectorize(O.midrad)(IA[..., 0], IA[..., 1])
+    slip = np.empty_like(ref)
+    for i in range(12):
+        for j in range(9):
+            lo = chain(mA[i], B[:, j], -1)
+            hi = chain(mA[i], B[:, j], 1)
+            t = chain(rA[i], np.abs(B[:, j]), -1)        # the planted slip: RD chain
+            slip[i, j, 1] = round_dir(F(hi) + F(t), 1)
+            slip[i, j, 0] = round_dir(F(lo) - F(t), -1)
+    assert not np.array_equal(slip, ref)
+    # the same construction with the right rounding reproduces it exactly
+    good = np.empty_like(ref)
+    for i in range(12):
+        for j in range(9):
+            lo = chain(mA[i], B[:, j], -1)
+            hi = chain(mA[i], B[:, j], 1)
+            t = chain(rA[i], np.abs(B[:, j]), 1)
+            good[i, j, 1] = round_dir(F(hi) + F(t), 1)
+            good[i, j, 0] = round_dir(F(lo) - F(t), -1)
+    assert np.array_equal(good, ref)

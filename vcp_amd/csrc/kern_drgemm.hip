@@ -572,6 +572,13 @@ __global__ __launch_bounds__(NT) void k_isgemm(int m, int n, int k, const double
   }
 }
 
+// p[0..n) = v (a kernel, so that it can precede graph replays without a
+// memset node)
+__global__ __launch_bounds__(256) void k_fill_f64(double* __restrict__ p, long n, double v) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 }  // namespace
 
 int vcphip_launch_isgemm(int m, int n, int k, const double* A, long lda, const double* B, long ldb, double* C,
@@ -610,6 +617,12 @@ int vcphip_launch_drsyrk(int interval, int k, int n, const double* A, long lda, 
 int vcphip_launch_igemm_tiles(int ntiles, const VcphipResidTile* tiles, const VcphipIgemmProblem* probs, int grid,
                               hipStream_t s) {
   if (ntiles > 0) hipLaunchKernelGGL(k_igemm_tiles, dim3(grid), dim3(NT), 0, s, ntiles, tiles, probs);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_fill_f64(double* p, long n, double v, hipStream_t s) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

@@ -106,7 +106,14 @@ int ws_get(vcphip_ctx* c, int slot, size_t bytes, void** out) {
   if (bytes == 0) bytes = 256;
   if (c->ws_size[slot] < bytes) {
     if (c->ws[slot]) {
+      // every stream of the context may still read or write the slot: the
+      // context stream, the second stream of the batched solve and the
+      // inverse lanes (they join the context stream before each entry point
+      // returns, but not before an early error return)
       hipStreamSynchronize(c->stream);
+      if (c->aux) hipStreamSynchronize(c->aux);
+      for (int l = 0; l < vcphip_ctx::INV_LANES; ++l)
+        if (c->inv_s[l]) hipStreamSynchronize(c->inv_s[l]);
       hipFree(c->ws[slot]);
       c->ws[slot] = nullptr;
       c->ws_size[slot] = 0;
@@ -762,6 +769,22 @@ extern "C" int vcphip_dd_residual_dev(vcphip_ctx* c, int m, int n, int k, const 
 // plain form): one persistent launch over a size-sorted tile list, captured
 // into a hipGraph and replayed while the problem set is unchanged
 // ===========================================================================
+// a captured graph is kept only when it holds exactly the `expect` kernel
+// nodes that the capture enqueued: a capture that ends successfully but
+// recorded nothing (as a tool intercepting launches can cause) would replay
+// as a no-op
+static bool graph_has_kernels(hipGraph_t g, size_t expect) {
+  size_t nn = 0;
+  if (hipGraphGetNodes(g, nullptr, &nn) != hipSuccess || nn != expect) return false;
+  std::vector<hipGraphNode_t> nodes(nn);
+  if (nn && hipGraphGetNodes(g, nodes.data(), &nn) != hipSuccess) return false;
+  for (hipGraphNode_t nd : nodes) {
+    hipGraphNodeType t;
+    if (hipGraphNodeGetType(nd, &t) != hipSuccess || t != hipGraphNodeTypeKernel) return false;
+  }
+  return true;
+}
+
 static bool same_igemm(const VcphipIgemmProblem& a, const VcphipIgemmProblem& b) {
   return a.A == b.A && a.B == b.B && a.C == b.C && a.lda == b.lda && a.ldb == b.ldb && a.ldc == b.ldc &&
          a.m == b.m && a.n == b.n && a.k == b.k;
@@ -826,7 +849,8 @@ extern "C" int vcphip_igemm_batched_dev(vcphip_ctx* c, int count, const int* m, 
     if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
       const int lrc = vcphip_launch_igemm_tiles(ntiles, dt, dp, g.grid, c->stream);
       hipGraph_t gr = nullptr;
-      if (hipStreamEndCapture(c->stream, &gr) == hipSuccess && lrc == 0 && gr) {
+      if (hipStreamEndCapture(c->stream, &gr) == hipSuccess && lrc == 0 && gr &&
+          graph_has_kernels(gr, ntiles > 0 ? 1 : 0)) {
         if (hipGraphInstantiate(&g.exec, gr, nullptr, nullptr, 0) == hipSuccess) {
           g.graph = gr;
           captured = true;
@@ -925,7 +949,8 @@ extern "C" int vcphip_residual_batched_dev(vcphip_ctx* c, int count, const int* 
     if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
       const int lrc = vcphip_launch_resid(ntiles, dt, dp, nrows, drp, drs, count, normub_dev, rs.grid, c->stream);
       hipGraph_t g = nullptr;
-      if (hipStreamEndCapture(c->stream, &g) == hipSuccess && lrc == 0 && g) {
+      if (hipStreamEndCapture(c->stream, &g) == hipSuccess && lrc == 0 && g &&
+          graph_has_kernels(g, (size_t)vcphip_resid_kernels(ntiles, nrows, count))) {
         if (hipGraphInstantiate(&rs.exec, g, nullptr, nullptr, 0) == hipSuccess) {
           rs.graph = g;
           captured = true;
@@ -949,6 +974,9 @@ extern "C" int vcphip_residual_batched_dev(vcphip_ctx* c, int count, const int* 
     }
   }
   StageTimer T(c);
+  // the bounds fail closed: +inf until the replay's norm kernel overwrites
+  // them, so a replay that runs nothing cannot pass as verified
+  if (vcphip_fill_f64(normub_dev, count, __builtin_inf(), c->stream)) return VCPHIP_ERR_DEVICE;
   if (hipGraphLaunch(rs.exec, c->stream) != hipSuccess) return VCPHIP_ERR_DEVICE;
   T.mark("resid");
   return VCPHIP_OK;

@@ -44,6 +44,12 @@ int vcphip_launch_igemm_tiles(int ntiles, const VcphipResidTile* tiles, const Vc
 int vcphip_launch_resid(int ntiles, const VcphipResidTile* tiles, const VcphipResidProblem* probs, int nrows,
                         const int* row_problem, double* rowsum, int count, double* normub, int grid,
                         hipStream_t s);
+// the number of kernels vcphip_launch_resid enqueues for these sizes
+inline int vcphip_resid_kernels(int ntiles, int nrows, int count) {
+  return (ntiles > 0) + (nrows > 0) + (count > 0);
+}
+// p[0..n) = v by a kernel
+int vcphip_fill_f64(double* p, long n, double v, hipStream_t s);
 
 // ---- Ozaki scheme II (kern_ozaki.hip) ------------------------------------
 #define VCPHIP_OZ_MAXMOD 32   // moduli (<= 20 for intervals, up to 32 for double-double)
