@@ -56,8 +56,10 @@ enum {
     VCPHIP_ERR_UNSUPPORTED = -7,  /* size or form outside what the kernels cover */
     VCPHIP_ERR_RANGE = -8,        /* exponent range outside the supported one    */
     VCPHIP_ERR_IO = -9,           /* vcp::io_error (vcp/vcp_portable_fio.hpp)    */
-    VCPHIP_ERR_SINGULAR = -10     /* vcp::lapack_error: zero pivot (pdblas.hpp:
+    VCPHIP_ERR_SINGULAR = -10,    /* vcp::lapack_error: zero pivot (pdblas.hpp:
                                      191-203) or not positive definite (:215-217) */
+    VCPHIP_ERR_SCRATCH = -11      /* the product needs more scratch than the
+                                     no-allocation entry point holds          */
 };
 
 /* forms of the interval x interval product */
@@ -124,6 +126,26 @@ int vcphip_igemm(vcphip_ctx* ctx, int form, int m, int n, int k,
                  const double* IA, int lda, const double* IB, int ldb, double* C, int ldc);
 int vcphip_igemm_dev(vcphip_ctx* ctx, int form, int m, int n, int k,
                      const double* IA, int lda, const double* IB, int ldb, double* C, int ldc);
+
+/* ---- interval x interval, Ozaki form, allocating nothing -----------------
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same product as vcphip_igemm_dev(ctx,
+ * VCPHIP_FORM_OZAKI, ...), bit for bit, with the same moduli count, radius
+ * scheme and tile variant (parameters, tuned table, defaults), on the
+ * context's stream; its workspace (int8 planes, residues, digit sums,
+ * row / column statistics) lives in one device array of
+ * VCPHIP_NOALLOC_SCRATCH_BYTES per GPU, reserved when the library's code is
+ * loaded on that GPU and held until the process ends.  It calls no
+ * hipMalloc, hipFree, memset or event routine.  That covers every shape
+ * with m, n, k <= 1024 at up to 20 moduli; a product whose padded workspace
+ * does not fit returns VCPHIP_ERR_SCRATCH before anything is enqueued.
+ * Calls on one GPU share its array: two that overlap in time on that GPU
+ * (different streams or host threads) race and corrupt each other's
+ * results; serialise them.  Device pointers only. */
+#define VCPHIP_NOALLOC_SCRATCH_BYTES ((size_t)100 << 20)
+int vcphip_igemm_ozaki_noalloc_dev(vcphip_ctx* ctx, int m, int n, int k, const double* IA, int lda,
+                                   const double* IB, int ldb, double* C, int ldc);
 
 /* ---- interval x point: C = IA * B ---------------------------------------- */
 /* Replaces vcp::pidblas::mul_im_m (vcp/pidblas.hpp:90-140). */

@@ -29,6 +29,7 @@ PARAM_OZ_VARIANT = 2
 PARAM_STAGE_TIMING = 3
 PARAM_DD_NMOD = 4
 PARAM_OZ_RADIUS = 5
+NOALLOC_SCRATCH_BYTES = 100 << 20
 OZ_RADIUS_FULL = 1
 OZ_RADIUS_FOLDED = 2
 
@@ -105,6 +106,7 @@ SIGNATURES = {
                                      _p_int, _p_int, _p_dbl],
     "vcphip_tune_residual_dev": [_vp, _c_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _p_dbl],
     "vcphip_ozaki_plan": [_vp, _c_int, _c_int, _c_int, _p_int, _p_int, _p_int, _p_int],
+    "vcphip_igemm_ozaki_noalloc_dev": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_tune_save": [_vp, ctypes.c_char_p],
     "vcphip_tune_load": [_vp, ctypes.c_char_p],
 }
@@ -665,6 +667,12 @@ class Context:
                                                   ctypes.byref(r)),
                "vcphip_tune_igemm_moduli_dev")
         return nm.value, rad.value, r.value
+
+    def igemm_ozaki_noalloc_dev(self, m, n, k, dA, lda, dB, ldb, dC, ldc):
+        """the Ozaki-form product of igemm_dev with its workspace in the
+        library's fixed device scratch (no allocation; m, n, k <= 1024 fit)"""
+        _check(lib().vcphip_igemm_ozaki_noalloc_dev(self._h, m, n, k, _vp(dA), lda, _vp(dB), ldb, _vp(dC), ldc),
+               "vcphip_igemm_ozaki_noalloc_dev")
 
     def ozaki_plan(self, m, n, k):
         """moduli, radius scheme, tile variant and int8 plane products an

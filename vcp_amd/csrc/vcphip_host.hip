@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -132,7 +133,8 @@ int ws_get(vcphip_ctx* c, int slot, size_t bytes, void** out) {
 struct StageTimer {
   vcphip_ctx* c;
   bool on;
-  explicit StageTimer(vcphip_ctx* ctx) : c(ctx), on(ctx->timing != 0) {
+  // allowed = false: no events at all (the no-allocation entry point)
+  explicit StageTimer(vcphip_ctx* ctx, bool allowed = true) : c(ctx), on(allowed && ctx->timing != 0) {
     if (!on) return;
     mark("");   // call boundary: the next stage is timed from here
   }
@@ -360,6 +362,10 @@ int default_nmod(int k) {
 // ===========================================================================
 extern "C" int vcphip_version(void) { return 1; }
 
+namespace {
+void* scratch_init(int device);   // the no-allocation entry point's scratch, below
+}  // namespace
+
 extern "C" const char* vcphip_strerror(int code) {
   switch (code) {
     case VCPHIP_OK: return "ok";
@@ -373,6 +379,7 @@ extern "C" const char* vcphip_strerror(int code) {
     case VCPHIP_ERR_RANGE: return "exponent range not supported";
     case VCPHIP_ERR_IO: return "matrix file I/O or format error";
     case VCPHIP_ERR_SINGULAR: return "factorization failed: zero pivot or not positive definite";
+    case VCPHIP_ERR_SCRATCH: return "product needs more scratch than the no-allocation entry point holds";
     default: return "unknown error";
   }
 }
@@ -396,6 +403,9 @@ extern "C" int vcphip_ctx_create(int device, vcphip_ctx** out) {
   c->stream = c->own;
   hipDeviceProp_t prop;
   c->num_cu = (hipGetDeviceProperties(&prop, device) == hipSuccess) ? prop.multiProcessorCount : 256;
+  // the no-allocation entry point's scratch address on this device (once per
+  // device; a failure only disables that entry point)
+  scratch_init(device);
   *out = c;
   return VCPHIP_OK;
 }
@@ -553,57 +563,152 @@ extern "C" int vcphip_ozaki_plan(vcphip_ctx* c, int m, int n, int k, int* nmod, 
 // ===========================================================================
 static int dims_ok(int m, int n, int k) { return m >= 0 && n >= 0 && k >= 0; }
 
-static int oz_igemm_dev(vcphip_ctx* c, int m, int n, int k, const double* A, int lda, const double* B, int ldb,
-                        double* C, int ldc) {
+// the moduli count, radius scheme, tile variant and buffer sizes of one
+// interval Ozaki product in this context
+struct OzPlan {
+  VcphipOzParams P;
+  int nmod = 0, nz = 0, variant = 0, mpad = 0, npad = 0, kpad = 0, nchunk = 0;
+  size_t b_pa = 0, b_pb = 0, b_pu = 0, b_pz = 0, b_sa = 0, b_sb = 0;   // bytes of each workspace buffer
+};
+
+static int oz_plan(vcphip_ctx* c, int m, int n, int k, OzPlan& p) {
   if (k > VCPHIP_OZ_MAXK) return VCPHIP_ERR_UNSUPPORTED;
   const vcphip_ctx::Tune* tuned = tune_find(c, TUNE_IGEMM, m, n, k);
-  const int nmod = c->oz_nmod ? (int)c->oz_nmod : (tuned && tuned->nmod ? tuned->nmod : default_nmod(k));
-  VcphipOzParams P;
-  int rc = oz_make_params(k, nmod, &P);
+  p.nmod = c->oz_nmod ? (int)c->oz_nmod : (tuned && tuned->nmod ? tuned->nmod : default_nmod(k));
+  const int rc = oz_make_params(k, p.nmod, &p.P);
   if (rc) return rc;
-  const int mpad = (m + VCPHIP_OZ_PADMN - 1) / VCPHIP_OZ_PADMN * VCPHIP_OZ_PADMN;
-  const int npad = (n + VCPHIP_OZ_PADMN - 1) / VCPHIP_OZ_PADMN * VCPHIP_OZ_PADMN;
-  const int kpad = k > 0 ? (k + VCPHIP_OZ_PADK - 1) / VCPHIP_OZ_PADK * VCPHIP_OZ_PADK : VCPHIP_OZ_PADK;
-  const int nchunk = k > 0 ? (k + VCPHIP_OZ_CHUNK - 1) / VCPHIP_OZ_CHUNK : 0;
-  void *pa, *pb, *pu, *pz, *sa, *sb;
-  const size_t nplane = (size_t)nmod + VCPHIP_OZ_NDIGIT;
-  const int nz = oz_nz(oz_radius_scheme(c, tuned, k));
-  if ((rc = ws_get(c, vcphip_ctx::WS_PA, nplane * mpad * kpad, &pa))) return rc;
-  if ((rc = ws_get(c, vcphip_ctx::WS_PB, nplane * npad * kpad, &pb))) return rc;
-  if ((rc = ws_get(c, vcphip_ctx::WS_U, (size_t)nmod * mpad * npad, &pu))) return rc;
-  if ((rc = ws_get(c, vcphip_ctx::WS_Z, (size_t)nz * sizeof(int32_t) * (size_t)mpad * npad, &pz))) return rc;
-  auto side_bytes = [&](int rows) { return (size_t)rows * (4 * sizeof(int) + sizeof(double)) + sizeof(double) * 5 * (size_t)(nchunk > 0 ? nchunk : 1) * rows + 1024; };
-  if ((rc = ws_get(c, vcphip_ctx::WS_SA, side_bytes(m), &sa))) return rc;
-  if ((rc = ws_get(c, vcphip_ctx::WS_SB, side_bytes(n), &sb))) return rc;
+  p.nz = oz_nz(oz_radius_scheme(c, tuned, k));
+  p.variant = c->oz_variant ? (int)c->oz_variant : (tuned && tuned->variant ? tuned->variant : DEFAULT_VARIANT);
+  p.mpad = (m + VCPHIP_OZ_PADMN - 1) / VCPHIP_OZ_PADMN * VCPHIP_OZ_PADMN;
+  p.npad = (n + VCPHIP_OZ_PADMN - 1) / VCPHIP_OZ_PADMN * VCPHIP_OZ_PADMN;
+  p.kpad = k > 0 ? (k + VCPHIP_OZ_PADK - 1) / VCPHIP_OZ_PADK * VCPHIP_OZ_PADK : VCPHIP_OZ_PADK;
+  p.nchunk = k > 0 ? (k + VCPHIP_OZ_CHUNK - 1) / VCPHIP_OZ_CHUNK : 0;
+  const size_t nplane = (size_t)p.nmod + VCPHIP_OZ_NDIGIT;
+  p.b_pa = nplane * p.mpad * p.kpad;
+  p.b_pb = nplane * p.npad * p.kpad;
+  p.b_pu = (size_t)p.nmod * p.mpad * p.npad;
+  p.b_pz = (size_t)p.nz * sizeof(int32_t) * (size_t)p.mpad * p.npad;
+  auto side_bytes = [&](int rows) {
+    return (size_t)rows * (4 * sizeof(int) + sizeof(double)) +
+           sizeof(double) * 5 * (size_t)(p.nchunk > 0 ? p.nchunk : 1) * rows + 1024;
+  };
+  p.b_sa = side_bytes(m);
+  p.b_sb = side_bytes(n);
+  return VCPHIP_OK;
+}
+
+// statistics, split, int8 GEMM and reconstruction of one product into the
+// given buffers (sizes from oz_plan; every kernel writes the whole padded
+// extent it or a later kernel reads, so the buffers need no clearing)
+static int oz_run(vcphip_ctx* c, const OzPlan& p, int m, int n, int k, const double* A, int lda, const double* B, int ldb,
+                  double* C, int ldc, void* pa, void* pb, void* pu, void* pz, void* sa, void* sb, bool timed) {
   auto carve = [&](void* base, int rows) {
     VcphipOzSide s;
-    char* p = (char*)base;
-    s.n1 = (double*)p; p += sizeof(double) * rows;
-    s.part = (double*)p; p += sizeof(double) * 5 * (size_t)(nchunk > 0 ? nchunk : 1) * rows;
-    s.e = (int*)p; p += sizeof(int) * rows;
-    s.g = (int*)p; p += sizeof(int) * rows;
-    s.h = (int*)p; p += sizeof(int) * rows;
-    s.flags = (int*)p;
+    char* q = (char*)base;
+    s.n1 = (double*)q; q += sizeof(double) * rows;
+    s.part = (double*)q; q += sizeof(double) * 5 * (size_t)(p.nchunk > 0 ? p.nchunk : 1) * rows;
+    s.e = (int*)q; q += sizeof(int) * rows;
+    s.g = (int*)q; q += sizeof(int) * rows;
+    s.h = (int*)q; q += sizeof(int) * rows;
+    s.flags = (int*)q;
     return s;
   };
-  VcphipOzSide SA = carve(sa, m), SB = carve(sb, n);
-  const int variant = c->oz_variant ? (int)c->oz_variant : (tuned && tuned->variant ? tuned->variant : DEFAULT_VARIANT);
-  StageTimer T(c);
+  const VcphipOzSide SA = carve(sa, m), SB = carve(sb, n);
+  const int mpad = p.mpad, npad = p.npad, kpad = p.kpad;
+  int rc;
+  StageTimer T(c, timed);
   if ((rc = check_launch(vcphip_oz_stats(A, 1, lda, m, k, SA, c->stream)))) return rc;
   if ((rc = check_launch(vcphip_oz_stats(B, ldb, 1, n, k, SB, c->stream)))) return rc;
   T.mark("stats");
-  if ((rc = check_launch(vcphip_oz_split_rows(A, lda, m, k, mpad, kpad, P, SA, (int8_t*)pa, c->stream)))) return rc;
-  if ((rc = check_launch(vcphip_oz_split_cols(B, ldb, k, n, npad, kpad, P, SB, (int8_t*)pb, c->stream)))) return rc;
+  if ((rc = check_launch(vcphip_oz_split_rows(A, lda, m, k, mpad, kpad, p.P, SA, (int8_t*)pa, c->stream)))) return rc;
+  if ((rc = check_launch(vcphip_oz_split_cols(B, ldb, k, n, npad, kpad, p.P, SB, (int8_t*)pb, c->stream)))) return rc;
   T.mark("split");
-  if ((rc = check_launch(vcphip_oz_gemm(mpad, npad, kpad, nmod, nz, (const int8_t*)pa, (const int8_t*)pb, P,
-                                        (uint8_t*)pu, (int32_t*)pz, variant, c->stream))))
+  if ((rc = check_launch(vcphip_oz_gemm(mpad, npad, kpad, p.nmod, p.nz, (const int8_t*)pa, (const int8_t*)pb, p.P,
+                                        (uint8_t*)pu, (int32_t*)pz, p.variant, c->stream))))
     return rc;
   T.mark("gemm");
-  if ((rc = check_launch(vcphip_oz_recon(m, n, k, mpad, npad, nz, P, SA, SB, (const uint8_t*)pu, (const int32_t*)pz, C, ldc,
-                                         c->stream))))
+  if ((rc = check_launch(vcphip_oz_recon(m, n, k, mpad, npad, p.nz, p.P, SA, SB, (const uint8_t*)pu,
+                                         (const int32_t*)pz, C, ldc, c->stream))))
     return rc;
   T.mark("recon");
   return VCPHIP_OK;
+}
+
+static int oz_igemm_dev(vcphip_ctx* c, int m, int n, int k, const double* A, int lda, const double* B, int ldb,
+                        double* C, int ldc) {
+  OzPlan p;
+  int rc = oz_plan(c, m, n, k, p);
+  if (rc) return rc;
+  void *pa, *pb, *pu, *pz, *sa, *sb;
+  if ((rc = ws_get(c, vcphip_ctx::WS_PA, p.b_pa, &pa))) return rc;
+  if ((rc = ws_get(c, vcphip_ctx::WS_PB, p.b_pb, &pb))) return rc;
+  if ((rc = ws_get(c, vcphip_ctx::WS_U, p.b_pu, &pu))) return rc;
+  if ((rc = ws_get(c, vcphip_ctx::WS_Z, p.b_pz, &pz))) return rc;
+  if ((rc = ws_get(c, vcphip_ctx::WS_SA, p.b_sa, &sa))) return rc;
+  if ((rc = ws_get(c, vcphip_ctx::WS_SB, p.b_sb, &sb))) return rc;
+  return oz_run(c, p, m, n, k, A, lda, B, ldb, C, ldc, pa, pb, pu, pz, sa, sb, true);
+}
+
+// ---------------------------------------------------------------------------
+// the Ozaki product without allocation: its workspace is one file-scope
+// device array of VCPHIP_NOALLOC_SCRATCH_BYTES, held by every GPU the library
+// runs on until the process ends.  Host code never uses the array's name as
+// an address (it is a host-side stand-in): vcphip_ctx_create takes each
+// device's own address with hipGetSymbolAddress, once per device, and the
+// entry point passes that address on.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[VCPHIP_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+bool g_scratch_tried[SCRATCH_MAX_DEVICES] = {};
+
+// the scratch array's address on `device`, taken with that device current;
+// nullptr if the runtime cannot give it
+void* scratch_init(int device) {
+  if (device < 0 || device >= SCRATCH_MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_tried[device]) {
+    g_scratch_tried[device] = true;
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) == hipSuccess && ptr) {
+      g_scratch_ptr[device] = ptr;
+    } else {
+      hipGetLastError();
+    }
+  }
+  return g_scratch_ptr[device];
+}
+
+void* scratch_of(int device) {
+  if (device < 0 || device >= SCRATCH_MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+
+size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+}  // namespace
+
+extern "C" int vcphip_igemm_ozaki_noalloc_dev(vcphip_ctx* c, int m, int n, int k, const double* IA, int lda,
+                                              const double* IB, int ldb, double* C, int ldc) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (!dims_ok(m, n, k) || lda < (m > 1 ? m : 1) || ldb < (k > 1 ? k : 1) || ldc < (m > 1 ? m : 1))
+    return VCPHIP_ERR_DIMENSION;
+  if (m == 0 || n == 0) return VCPHIP_OK;
+  if (!IA || !IB || !C) return VCPHIP_ERR_ARGUMENT;
+  OzPlan p;
+  int rc = oz_plan(c, m, n, k, p);
+  if (rc) return rc;
+  const size_t o_pb = al256(p.b_pa), o_pu = o_pb + al256(p.b_pb), o_pz = o_pu + al256(p.b_pu);
+  const size_t o_sa = o_pz + al256(p.b_pz), o_sb = o_sa + al256(p.b_sa), total = o_sb + al256(p.b_sb);
+  if (total > VCPHIP_NOALLOC_SCRATCH_BYTES) return VCPHIP_ERR_SCRATCH;
+  unsigned char* base = (unsigned char*)scratch_of(c->device);
+  if (!base) return VCPHIP_ERR_DEVICE;
+  hipSetDevice(c->device);
+  return oz_run(c, p, m, n, k, IA, lda, IB, ldb, C, ldc, base, base + o_pb, base + o_pu, base + o_pz, base + o_sa,
+                base + o_sb, false);
 }
 
 extern "C" int vcphip_igemm_dev(vcphip_ctx* c, int form, int m, int n, int k, const double* IA, int lda,
