@@ -95,87 +95,116 @@ def main():
     ctx = vcp_amd.Context(0)
     if args.variant:
         ctx.set_param(vcp_amd.PARAM_OZ_VARIANT, args.variant)
-    beta, mods = vcp_amd.ozaki_config(n)
-    nmod = len(mods)
-    plan = ctx.ozaki_plan(n, n, n)
-    scheme = {vcp_amd.OZ_RADIUS_FULL: "full", vcp_amd.OZ_RADIUS_FOLDED: "folded"}[plan["radius"]]
+    names = vcp_amd.OZ_RADIUS_NAMES
+    wtiny = torch.finfo(torch.float64).tiny
 
-    def step(form, out):
-        ctx.igemm_dev(form, n, n, n, dA.data_ptr(), n, dB.data_ptr(), n, out.data_ptr(), n)
+    def run(c, form, out):
+        c.igemm_dev(form, n, n, n, dA.data_ptr(), n, dB.data_ptr(), n, out.data_ptr(), n)
 
-    # ---- Ozaki form (the headline) --------------------------------------
-    log("ozaki warmup x%d" % args.warmup)
-    for _ in range(args.warmup):
-        step(vcp_amd.FORM_OZAKI, dC)
-    ctx.synchronize()
-    torch.cuda.synchronize()
-    ctx.set_param(vcp_amd.PARAM_STAGE_TIMING, 1)
-    log("ozaki timed x%d" % args.steps)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step(vcp_amd.FORM_OZAKI, dC)
-    ctx.synchronize()
-    torch.cuda.synchronize()
-    t_oz = (time.perf_counter() - t0) / args.steps
-    stages = ctx.stage_times()
-    ctx.set_param(vcp_amd.PARAM_STAGE_TIMING, 0)
-    log("ozaki %.3f ms/step, stages %s" % (t_oz * 1e3, stages))
+    def timed_ozaki(c, out, warmup, steps):
+        for _ in range(warmup):
+            run(c, vcp_amd.FORM_OZAKI, out)
+        c.synchronize()
+        torch.cuda.synchronize()
+        c.set_param(vcp_amd.PARAM_STAGE_TIMING, 1)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            run(c, vcp_amd.FORM_OZAKI, out)
+        c.synchronize()
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / steps
+        st = c.stage_times()
+        c.set_param(vcp_amd.PARAM_STAGE_TIMING, 0)
+        return t, st
 
-    # ---- plain FP64 directed-rounding form --------------------------------
+    flops = 8.0 * n * n * n
+
+    # ---- plain FP64 directed-rounding form (the radius reference) ----------
     log("plain warmup x1, timed x%d" % args.plain_steps)
-    step(vcp_amd.FORM_PLAIN, dP)
+    run(ctx, vcp_amd.FORM_PLAIN, dP)
     ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.plain_steps):
-        step(vcp_amd.FORM_PLAIN, dP)
+        run(ctx, vcp_amd.FORM_PLAIN, dP)
     ctx.synchronize()
     t_pl = (time.perf_counter() - t0) / max(args.plain_steps, 1)
-    log("plain %.3f ms/step" % (t_pl * 1e3))
-
-    flops = 8.0 * n * n * n
-    tf_oz = flops / t_oz / 1e12
     tf_pl = flops / t_pl / 1e12
+    log("plain %.3f ms/step" % (t_pl * 1e3))
+    wp = torch.clamp(dP[..., 1] - dP[..., 0], min=wtiny)
+
+    def ratio_of(out):
+        r = (out[..., 1] - out[..., 0]) / wp
+        return {"max": float(r.max()), "median": float(r.median())}
+
+    # ---- Ozaki form, untuned defaults ---------------------------------------
+    plan_d = ctx.ozaki_plan(n, n, n)
+    log("ozaki default (%d moduli, %s radius digits): warmup x1, timed x%d"
+        % (plan_d["moduli"], names[plan_d["radius"]], args.steps))
+    t_def, st_def = timed_ozaki(ctx, dC, 1, args.steps)
+    default = {"moduli": plan_d["moduli"], "radius_scheme": names[plan_d["radius"]],
+               "tile_variant": plan_d["variant"], "plane_products": plan_d["products"],
+               "ms_per_step": round(t_def * 1e3, 3), "tflops": round(flops / t_def / 1e12, 3),
+               "speedup_vs_plain": round(t_pl / t_def, 3), "stages_ms": {k: round(x, 4) for k, x in st_def.items()},
+               "radius_ratio_over_plain": ratio_of(dC)}
+    log("ozaki default %s" % default)
+
+    # ---- Ozaki form, autotuned: the headline --------------------------------
+    # The tuner (before warmup, outside the timed region) picks the cheapest
+    # moduli count and radius-digit scheme whose radii stay within 2x of the
+    # plain form's on a row sample, then the fastest int8 tile variant for
+    # that plane count.  Without tuning (--no-tuned) the headline is the
+    # default configuration.
+    hctx, tune = ctx, None
+    dH = dC
+    if not args.no_tuned:
+        hctx = vcp_amd.Context(0)
+        t0 = time.perf_counter()
+        nm_t, rad_t, ratio_s = hctx.tune_igemm_moduli_dev(n, n, n, dA.data_ptr(), n, dB.data_ptr(), n, 2.0, 0)
+        v_t, _ = hctx.tune_igemm_tile(n, n, n)
+        tune = {"moduli": nm_t, "radius_scheme": names[rad_t], "sampled_ratio": ratio_s,
+                "sample_rows": max(1024, n // 8), "tile_variant": v_t, "target_ratio": 2.0,
+                "tuning_s": round(time.perf_counter() - t0, 3)}
+        log("tuned %s" % tune)
+        dH = torch.empty_like(dC)
+    plan = hctx.ozaki_plan(n, n, n)
+    nmod, scheme = plan["moduli"], names[plan["radius"]]
+    beta, _ = vcp_amd.ozaki_config(n, nmod)
+    log("ozaki headline (%d moduli, %s radius digits, tile variant %d): warmup x%d, timed x%d"
+        % (nmod, scheme, plan["variant"], args.warmup, args.steps))
+    t_oz, stages = timed_ozaki(hctx, dH, args.warmup, args.steps)
+    tf_oz = flops / t_oz / 1e12
+    ratio_h = ratio_of(dH)
+    log("ozaki %.3f ms/step, stages %s, radius ratio %s" % (t_oz * 1e3, stages, ratio_h))
 
     # ---- inf-sup form (pidblas_fma::mulmm), one timed product --------------
     infsup = None
     if not args.no_infsup:
         dI = torch.empty_like(dC)
-        step(vcp_amd.FORM_INFSUP, dI)
+        run(ctx, vcp_amd.FORM_INFSUP, dI)
         ctx.synchronize()
         t0 = time.perf_counter()
-        step(vcp_amd.FORM_INFSUP, dI)
+        run(ctx, vcp_amd.FORM_INFSUP, dI)
         ctx.synchronize()
         t_is = time.perf_counter() - t0
-        w_is = (dI[..., 1] - dI[..., 0])
-        w_pl = torch.clamp(dP[..., 1] - dP[..., 0], min=torch.finfo(torch.float64).tiny)
-        r_is = w_is / w_pl
         infsup = {"ms_per_step": round(t_is * 1e3, 3), "tflops_8n3": round(flops / t_is / 1e12, 3),
-                  "fp64_ops_per_step": 28.0 * n * n * n,
-                  "radius_ratio_over_plain": {"max": float(r_is.max()), "median": float(r_is.median())}}
+                  "fp64_ops_per_step": 28.0 * n * n * n, "radius_ratio_over_plain": ratio_of(dI)}
         log("infsup %s" % infsup)
-        del dI, w_is, w_pl, r_is
+        del dI
 
-    # ---- radius ratio (all entries, on the GPU) and sampled containment ----
-    rad_o = dC[..., 1] - dC[..., 0]
-    rad_p = dP[..., 1] - dP[..., 0]
-    ratio = (rad_o / torch.clamp(rad_p, min=torch.finfo(torch.float64).tiny))
-    ratio_max = float(ratio.max())
-    ratio_med = float(ratio.median())
-    del rad_o, rad_p, ratio
+    # ---- sampled containment of the headline and default results -----------
     containment = None
     try:
         if args.samples <= 0:
             raise RuntimeError("no samples requested")
         import oracle_ffi as O
 
-        C = dC.cpu().numpy().transpose(1, 0, 2)
-        P = dP.cpu().numpy().transpose(1, 0, 2)
         rng = np.random.default_rng(12345)
         ii = rng.integers(0, n, args.samples)
         jj = rng.integers(0, n, args.samples)
-        ok_o, _, _ = O.check_containment(IA, IB, C, ii, jj)
-        ok_p, _, _ = O.check_containment(IA, IB, P, ii, jj)
-        containment = {"samples": int(args.samples), "ozaki": float(ok_o.mean()), "plain": float(ok_p.mean())}
+        containment = {"samples": int(args.samples)}
+        for key, X in (("ozaki", dH), ("ozaki_default", dC), ("plain", dP)):
+            ok, _, _ = O.check_containment(IA, IB, X.cpu().numpy().transpose(1, 0, 2), ii, jj)
+            containment[key] = float(ok.mean())
         log("containment %s" % containment)
     except Exception as e:  # the checker is optional for the measurement
         log("containment check skipped: %r" % (e,))
@@ -185,15 +214,15 @@ def main():
     roof = None
     if t_gemm:
         # one launch: nmod residue products + the radius-digit plane products
-        # (hi*hi, hi*lo + lo*hi and, full scheme, lo*lo of each of the two
-        # radius terms), 2 n^3 int8 ops each
+        # of the scheme (hi*hi, hi*lo + lo*hi and, full, lo*lo of each of the
+        # two radius terms; half: hi*H, lo*H; ceil: H*H), 2 n^3 int8 ops each
         nprod = plan["products"]
         ops = 2.0 * n * n * n * nprod
         ach = ops / (t_gemm * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
                 "frac": round(ach / PEAK_INT8_TOPS, 4), "traffic": measured_traffic(n, nmod, nprod),
-                "kernel": "k_oz_gemm (int8 MFMA: %d residue + %d radius-digit plane products, %s radius scheme)" %
-                          (nmod, nprod - nmod, scheme),
+                "kernel": "k_oz_gemm (int8 MFMA: %d residue + %d radius-digit plane products, %s radius scheme, "
+                          "tile variant %d)" % (nmod, nprod - nmod, scheme, plan["variant"]),
                 "ops_per_launch": ops, "ms_per_launch": round(t_gemm, 4)}
 
     # ---- CPU baseline: pidblas::mulmm on the reference's own vblas rmatmul
@@ -230,46 +259,8 @@ def main():
             cpu["plain_gpu_rows_bitexact"] = bool(np.array_equal(Cp_rows, Cref))
         except Exception as e:
             log("cpu baseline skipped: %r" % (e,))
-
-    # ---- autotuned configuration (tile variant + moduli count for ratio <= 2)
-    tuned = None
-    if not args.no_tuned:
-        tctx = vcp_amd.Context(0)
-        nm, rad_s, ratio_s = tctx.tune_igemm_moduli_dev(n, n, n, dA.data_ptr(), n, dB.data_ptr(), n, 2.0, 256)
-        v, v_ms = tctx.tune_igemm_tile(n, n, n)    # tiles timed with the tuned plane count
-        dT = torch.empty_like(dC)
-        tctx.igemm_dev(vcp_amd.FORM_OZAKI, n, n, n, dA.data_ptr(), n, dB.data_ptr(), n, dT.data_ptr(), n)
-        tctx.synchronize()
-        tctx.set_param(vcp_amd.PARAM_STAGE_TIMING, 1)
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            tctx.igemm_dev(vcp_amd.FORM_OZAKI, n, n, n, dA.data_ptr(), n, dB.data_ptr(), n, dT.data_ptr(), n)
-        tctx.synchronize()
-        t_tu = (time.perf_counter() - t0) / args.steps
-        st_tu = tctx.stage_times()
-        w_t = dT[..., 1] - dT[..., 0]
-        r_t = w_t / torch.clamp(dP[..., 1] - dP[..., 0], min=torch.finfo(torch.float64).tiny)
-        tuned = {"moduli": nm, "radius_scheme": {vcp_amd.OZ_RADIUS_FULL: "full", vcp_amd.OZ_RADIUS_FOLDED: "folded"}[rad_s],
-                 "sampled_ratio": ratio_s, "tile_variant": v, "ms_per_step": round(t_tu * 1e3, 3),
-                 "tflops": round(flops / t_tu / 1e12, 3), "speedup_vs_plain": round(t_pl / t_tu, 3),
-                 "stages_ms": {k: round(x, 4) for k, x in st_tu.items()},
-                 "radius_ratio_over_plain": {"max": float(r_t.max()), "median": float(r_t.median())}}
-        try:
-            if args.samples <= 0:
-                raise RuntimeError("no samples requested")
-            import oracle_ffi as O
-
-            Ct = dT.cpu().numpy().transpose(1, 0, 2)
-            rng = np.random.default_rng(4242)
-            ii = rng.integers(0, n, args.samples)
-            jj = rng.integers(0, n, args.samples)
-            ok_t, _, _ = O.check_containment(IA, IB, Ct, ii, jj)
-            tuned["containment"] = float(ok_t.mean())
-        except Exception as e:
-            log("tuned containment skipped: %r" % (e,))
-        log("tuned %s" % tuned)
-        del dT, w_t, r_t
-        tctx.close()
+    if hctx is not ctx:
+        hctx.close()
 
     resid = None if args.no_resid else bench_resid(ctx, args.resid_reps)
     dd = None if args.no_dd else bench_dd(ctx, args.dd_n, args.dd_steps, not args.no_cpu)
@@ -293,13 +284,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": "interval GEMM n=%d, mid=+-m*2^e (m U[1,2), e U{-20..20}), rad=2^-45|mid|" % n,
                    "n": n, "form": "ozaki-int8", "moduli": nmod, "beta": beta, "radius_scheme": scheme,
-                   "tile_variant": args.variant or "default"},
+                   "tile_variant": plan["variant"], "plane_products": plan["products"],
+                   "configuration": "autotuned (radius ratio target 2)" if tune else "untuned defaults"},
         "plain_fp64_rurd": {"tflops": round(tf_pl, 3), "ms_per_step": round(t_pl * 1e3, 3),
                             "frac_fp64_peak": round(tf_pl / PEAK_FP64_TFLOPS, 4)},
         "speedup_vs_plain": round(t_pl / t_oz, 3),
-        "radius_ratio_ozaki_over_plain": {"max": ratio_max, "median": ratio_med},
+        "radius_ratio_ozaki_over_plain": ratio_h,
+        "tuning": tune,
+        "untuned_default": default,
         "infsup_fp64": infsup,
-        "tuned": tuned,
         "containment": containment,
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "roofline": roof,

@@ -84,17 +84,23 @@ enum {
 
 /* radius-digit schemes of the Ozaki form.  Each factor of the radius term
  * (|mid A| + rad A) rad B + rad A |mid B| (vcp/pidblas.hpp:277-283) is bounded
- * above by two 7-bit digits against its row / column maximum, and the sum
- * over k of the products of those bounds is formed on int8 MFMA:
- *   FULL    hi x hi, hi x lo + lo x hi and lo x lo per term: the exact sum of
- *           the digit-bound products (8 int8 plane products);
- *   FOLDED  B's digits carry a non-positive low digit, so the lo x lo sum is
- *           never positive and is left out (6 plane products; wider where k
- *           is small and exponents spread, equal at k in the thousands).
- * Both give a rigorous enclosure. */
+ * above by 7-bit digits against its row / column maximum, and the sum over k
+ * of the products of those bounds is formed on int8 MFMA:
+ *   FULL    two digits per factor; hi x hi, hi x lo + lo x hi and lo x lo per
+ *           term: the exact sum of the digit-bound products (8 int8 plane
+ *           products);
+ *   FOLDED  the same with B's low digit non-positive, so that the lo x lo sum
+ *           is never positive and is left out (6 plane products);
+ *   HALF    A's two digits against B's rounded-up single digit (4);
+ *   CEIL    rounded-up single digits of both factors (2).
+ * Every scheme gives a rigorous enclosure; they trade radius for int8 work
+ * (each cheaper one is wider where small entries sit beside large ones in a
+ * row or column, since a digit rounds up against the row / column maximum). */
 enum {
     VCPHIP_OZ_RADIUS_FULL = 1,
-    VCPHIP_OZ_RADIUS_FOLDED = 2
+    VCPHIP_OZ_RADIUS_FOLDED = 2,
+    VCPHIP_OZ_RADIUS_HALF = 3,
+    VCPHIP_OZ_RADIUS_CEIL = 4
 };
 
 /* ---- context ------------------------------------------------------------ */
@@ -367,11 +373,12 @@ int vcphip_tune_ddgemm_tile(vcphip_ctx* ctx, int m, int n, int k, int* variant, 
 int vcphip_ozaki_plan(vcphip_ctx* ctx, int m, int n, int k, int* nmod, int* radius, int* variant,
                       int* products);
 /* Slice count and radius-digit scheme of the interval Ozaki form for these
- * operands (device pointers): the cheapest pair (moduli N in [12, 20],
- * VCPHIP_OZ_RADIUS_FOLDED at N + 6 int8 plane products or _FULL at N + 8)
- * whose radii, on an evenly spaced sample of sample_rows rows of IA (0: 256)
- * against all of IB, are at most max_ratio times the plain form's; at equal
- * cost the smaller ratio wins; 20 moduli with the full scheme if none is.
+ * operands (device pointers): the cheapest pair (moduli N in [12, 20] with
+ * VCPHIP_OZ_RADIUS_CEIL at N + 2 int8 plane products, _HALF at N + 4,
+ * _FOLDED at N + 6 or _FULL at N + 8) whose radii, on an evenly spaced sample
+ * of sample_rows rows of IA (0: max(1024, m/8)) against all of IB, are at most
+ * max_ratio times the plain form's; at equal cost the smaller ratio wins; 20
+ * moduli with the full scheme if none is.
  * *nmod, *radius and *ratio (the sampled maximum radius ratio) receive the
  * choice (any of them may be NULL). */
 int vcphip_tune_igemm_moduli_dev(vcphip_ctx* ctx, int m, int n, int k, const double* IA, int lda,

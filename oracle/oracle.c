@@ -994,7 +994,7 @@ void orc_ozaki_scaled_cols(int k, int n, const double* B, int ldb, int beta, int
  * (|mA|+rA) rB + rA |mB| (:277-283). */
 static int oz_igemm_core(int m, int n, int k, const double* A, int lda, const double* B, int ldb,
                          double* C, int ldc, int nmod, int radius) {
-    const int full = radius != ORC_OZ_RADIUS_FOLDED;
+    if (radius < ORC_OZ_RADIUS_FULL || radius > ORC_OZ_RADIUS_CEIL) return -3;
     const int beta = orc_ozaki_beta(k, nmod);
     if (beta < 1) return -2;
     const size_t mk = (size_t)m * k, kn = (size_t)k * n;
@@ -1006,9 +1006,11 @@ static int oz_igemm_core(int m, int n, int k, const double* A, int lda, const do
     oz_stats(n, k, mB, rB, (size_t)k, 1, beta, &SB, 64);
     int64_t* Ap = (int64_t*)malloc(sizeof(int64_t) * mk); /* row i contiguous */
     int64_t* Bp = (int64_t*)malloc(sizeof(int64_t) * kn); /* col j contiguous */
-    /* digit planes (h, l) of |mA|+rA, rA (rows of A); (H, L), the same bound
-     * with a non-positive low digit, of rB, |mB| (cols of B) */
+    /* digit planes (h, l) of |mA|+rA, rA (rows of A) and their rounded-up
+     * single digits h + [l > 0]; (H, L), the same bound with a non-positive
+     * low digit, of rB, |mB| (cols of B), whose H is the rounded-up digit */
     int8_t *Sh = (int8_t*)malloc(mk), *Sl = (int8_t*)malloc(mk), *Rh = (int8_t*)malloc(mk), *Rl = (int8_t*)malloc(mk);
+    int8_t *Sc = (int8_t*)malloc(mk), *Rc = (int8_t*)malloc(mk);
     int8_t *BRh = (int8_t*)malloc(kn), *BRl = (int8_t*)malloc(kn), *BMh = (int8_t*)malloc(kn), *BMl = (int8_t*)malloc(kn);
     double c127, c1, c1dn, c127dn;
     const int old = fegetround();
@@ -1028,8 +1030,8 @@ static int oz_igemm_core(int m, int n, int k, const double* A, int lda, const do
             const size_t o = (size_t)i * k + c;
             int h, l;
             Ap[o] = (SA.flags[i] & OZ_MIDZERO) ? 0 : rnint_floor(mulpow2(am, beta - SA.e[i]));
-            digits2_ru(as, SA.g[i], &h, &l); Sh[o] = (int8_t)h; Sl[o] = (int8_t)l;
-            digits2_ru(ar, SA.h[i], &h, &l); Rh[o] = (int8_t)h; Rl[o] = (int8_t)l;
+            digits2_ru(as, SA.g[i], &h, &l); Sh[o] = (int8_t)h; Sl[o] = (int8_t)l; Sc[o] = (int8_t)(h + (l > 0));
+            digits2_ru(ar, SA.h[i], &h, &l); Rh[o] = (int8_t)h; Rl[o] = (int8_t)l; Rc[o] = (int8_t)(h + (l > 0));
         }
     for (int j = 0; j < n; ++j) {
         const int zc = (SB.flags[j] & OZ_MIDZERO) != 0;
@@ -1054,6 +1056,7 @@ static int oz_igemm_core(int m, int n, int k, const double* A, int lda, const do
                 i128 Xh = 0; /* sum of the products' high halves (arithmetic >> 64) */
                 u128 Xl = 0; /* sum of their low 64 bits */
                 int64_t z1h = 0, z1m = 0, z1l = 0, z2h = 0, z2m = 0, z2l = 0;
+                int64_t z1lh = 0, z2lh = 0, z1c = 0, z2c = 0; /* half: lo x H; ceil: H x H */
                 const size_t ai = (size_t)i * k, bj = (size_t)j * k;
                 for (int c = 0; c < k; ++c) {
                     const i128 pr = (i128)Ap[ai + c] * Bp[bj + c];
@@ -1065,6 +1068,10 @@ static int oz_igemm_core(int m, int n, int k, const double* A, int lda, const do
                     z2m += (int64_t)Rh[ai + c] * BMl[bj + c] + (int64_t)Rl[ai + c] * BMh[bj + c];
                     z1l += (int64_t)Sl[ai + c] * BRl[bj + c];
                     z2l += (int64_t)Rl[ai + c] * BMl[bj + c];
+                    z1lh += (int64_t)Sl[ai + c] * BRh[bj + c];
+                    z2lh += (int64_t)Rl[ai + c] * BMh[bj + c];
+                    z1c += (int64_t)Sc[ai + c] * BRh[bj + c];
+                    z2c += (int64_t)Rc[ai + c] * BMh[bj + c];
                 }
                 const int fa = SA.flags[i], fb = SB.flags[j];
                 const int sc = SA.e[i] + SB.e[j] - 2 * beta;
@@ -1072,20 +1079,30 @@ static int oz_igemm_core(int m, int n, int k, const double* A, int lda, const do
                 const double lo_mid = conv_x192(Xh, (uint64_t)Xl, sc, -1);
                 const double hi_mid = conv_x192(Xh, (uint64_t)Xl, sc, 1);
                 double t1 = 0.0, t2 = 0.0;
-                /* sum x1 x2 <= 2^(g1+g2)/127^2 (zh + zm/127 + zl/127^2): the sum of
-                 * the products of the digit bounds (full scheme); the folded one
-                 * leaves out zl <= 0.  A quotient with a numerator of either sign is
-                 * bounded above by the numerator times RU or RD of the reciprocal. */
-                if (!((fa & OZ_SUMZERO) || (fb & OZ_RADZERO))) {
-                    double u = (double)z1h + (double)z1m * (z1m >= 0 ? c1 : c1dn);
-                    if (full) u = u + (double)z1l * (z1l >= 0 ? c127 : c127dn);
-                    t1 = mulpow2(u, SA.g[i] + SB.h[j]) * c127;
+                /* Per term, with x1 <= (h1 + l1/127) s1 <= c1 s1 (c1 = h1 + [l1 > 0]) and
+                 * x2 <= (H + L/127) s2 <= H s2, s = 2^g/127:
+                 *   full    sum x1 x2 <= s1 s2 (zh + zm/127 + zl/127^2), zl <= 0;
+                 *   folded  s1 s2 (zh + zm/127)   (zl left out);
+                 *   half    s1 s2 (zh + zlh/127), zlh = sum l1 H;
+                 *   ceil    s1 s2 zc,             zc = sum c1 H.
+                 * A quotient with a numerator of either sign is bounded above by the
+                 * numerator times RU or RD of the reciprocal. */
+                double u1, u2;
+                if (radius == ORC_OZ_RADIUS_CEIL) {
+                    u1 = (double)z1c;
+                    u2 = (double)z2c;
+                } else {
+                    const int64_t m1 = radius == ORC_OZ_RADIUS_HALF ? z1lh : z1m;
+                    const int64_t m2 = radius == ORC_OZ_RADIUS_HALF ? z2lh : z2m;
+                    u1 = (double)z1h + (double)m1 * (m1 >= 0 ? c1 : c1dn);
+                    u2 = (double)z2h + (double)m2 * (m2 >= 0 ? c1 : c1dn);
+                    if (radius == ORC_OZ_RADIUS_FULL) {
+                        u1 = u1 + (double)z1l * (z1l >= 0 ? c127 : c127dn);
+                        u2 = u2 + (double)z2l * (z2l >= 0 ? c127 : c127dn);
+                    }
                 }
-                if (!((fa & OZ_RADZERO) || (fb & OZ_MIDZERO))) {
-                    double u = (double)z2h + (double)z2m * (z2m >= 0 ? c1 : c1dn);
-                    if (full) u = u + (double)z2l * (z2l >= 0 ? c127 : c127dn);
-                    t2 = mulpow2(u, SA.h[i] + SB.e[j]) * c127;
-                }
+                if (!((fa & OZ_SUMZERO) || (fb & OZ_RADZERO))) t1 = mulpow2(u1, SA.g[i] + SB.h[j]) * c127;
+                if (!((fa & OZ_RADZERO) || (fb & OZ_MIDZERO))) t2 = mulpow2(u2, SA.h[i] + SB.e[j]) * c127;
                 const double rt = t1 + t2;
                 double E = 0.0;
                 if (!(fa & OZ_MIDZERO) && !(fb & OZ_MIDZERO)) {
@@ -1102,7 +1119,8 @@ static int oz_igemm_core(int m, int n, int k, const double* A, int lda, const do
         }
         fesetround(o2);
     }
-    free(Ap); free(Bp); free(Sh); free(Sl); free(Rh); free(Rl); free(BRh); free(BRl); free(BMh); free(BMl);
+    free(Ap); free(Bp); free(Sh); free(Sl); free(Rh); free(Rl); free(Sc); free(Rc); free(BRh); free(BRl); free(BMh);
+    free(BMl);
     oz_free(&SA); oz_free(&SB);
     free(mA); free(rA); free(mB); free(rB);
     return 0;

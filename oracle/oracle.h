@@ -103,9 +103,11 @@ int orc_ozaki_beta(int k, int nmod);
 /* Interval GEMM through the exact scaled-integer product.  Mirrors the GPU
  * pipeline bit for bit (the CRT result equals the integer product).
  * radius: ORC_OZ_RADIUS_FULL (the exact sum of the products of the upward
- * digit bounds of the radius-term factors) or ORC_OZ_RADIUS_FOLDED (the lo x lo
- * part, never positive with B's non-positive low digits, left out). */
-enum { ORC_OZ_RADIUS_FULL = 1, ORC_OZ_RADIUS_FOLDED = 2 };
+ * two-digit bounds of the radius-term factors), ORC_OZ_RADIUS_FOLDED (the
+ * lo x lo part, never positive with B's non-positive low digits, left out),
+ * ORC_OZ_RADIUS_HALF (A's two digits against B's rounded-up digit) or
+ * ORC_OZ_RADIUS_CEIL (rounded-up single digits of both factors). */
+enum { ORC_OZ_RADIUS_FULL = 1, ORC_OZ_RADIUS_FOLDED = 2, ORC_OZ_RADIUS_HALF = 3, ORC_OZ_RADIUS_CEIL = 4 };
 int orc_ozaki_igemm(int m, int n, int k, const double* A, int lda, const double* B, int ldb,
                     double* C, int ldc, int nmod, int radius);
 /* Integer slice work of the same pipeline, exposed for bit-exact checks:

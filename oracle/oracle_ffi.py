@@ -293,12 +293,15 @@ def ozaki_beta(k, nmod):
 
 OZ_RADIUS_FULL = 1
 OZ_RADIUS_FOLDED = 2
+OZ_RADIUS_HALF = 3
+OZ_RADIUS_CEIL = 4
+OZ_RADIUS_ALL = (OZ_RADIUS_FULL, OZ_RADIUS_FOLDED, OZ_RADIUS_HALF, OZ_RADIUS_CEIL)
 OZ_FOLDED_MINK = 4096   # the library's untuned default: folded from this k
 
 
 def ozaki_igemm(IA, IB, nmod, radius=None):
-    """radius: OZ_RADIUS_FULL / OZ_RADIUS_FOLDED; None follows the library's
-    untuned default (folded for k >= 4096, else full)"""
+    """radius: one of OZ_RADIUS_ALL; None follows the library's untuned
+    default (folded for k >= 4096, else full)"""
     m, k = IA.shape[:2]
     n = IB.shape[1]
     if radius is None:

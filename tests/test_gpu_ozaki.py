@@ -129,7 +129,8 @@ def test_ozaki_extreme_exponents(ctx):
     assert C[0, 1, 0] == -np.inf or C[0, 1, 1] == np.inf
 
 
-@pytest.mark.parametrize("radius", [vcp_amd.OZ_RADIUS_FULL, vcp_amd.OZ_RADIUS_FOLDED])
+@pytest.mark.parametrize("radius", [vcp_amd.OZ_RADIUS_FULL, vcp_amd.OZ_RADIUS_FOLDED, vcp_amd.OZ_RADIUS_HALF,
+                                    vcp_amd.OZ_RADIUS_CEIL])
 @pytest.mark.parametrize("nmod", [12, 16, 20])
 @pytest.mark.parametrize("case", ozaki_cases.NAMES)
 def test_ozaki_digit_stress_every_entry(ctx, case, nmod, radius):
@@ -155,7 +156,7 @@ def test_ozaki_digit_stress_every_entry(ctx, case, nmod, radius):
 
 
 def test_ozaki_radius_param_checked(ctx):
-    for bad in (3, -1, 7):
+    for bad in (5, -1, 7):
         with pytest.raises(vcp_amd.VcphipError):
             ctx.set_param(vcp_amd.PARAM_OZ_RADIUS, bad)
     assert ctx.get_param(vcp_amd.PARAM_OZ_RADIUS) == 0

@@ -103,7 +103,7 @@ def test_tune_load_rejects_malformed(ctx, tmp_path):
     bad.write_text("vcphip-tune 1\nigemm 10 10 10 99 16\n")
     with pytest.raises(vcp_amd.VcphipError):
         ctx.tune_load(str(bad))
-    bad.write_text("vcphip-tune 2\nigemm 10 10 10 8 16 3\n")   # no radius scheme 3
+    bad.write_text("vcphip-tune 2\nigemm 10 10 10 8 16 5\n")   # no radius scheme 5
     with pytest.raises(vcp_amd.VcphipError):
         ctx.tune_load(str(bad))
     bad.write_text("vcphip-tune 2\nddgemm 10 10 10 8 32 1\n")   # dd products have no radius scheme

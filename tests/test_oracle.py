@@ -305,7 +305,7 @@ def test_ozaki_restatement_encloses_and_is_not_wider(kind):
     rng = np.random.default_rng(11)
     IA, IB = _iv(rng, 24, 300, kind), _iv(rng, 300, 20, kind)
     P = O.pidblas_mulmm(IA, IB)
-    for radius in (O.OZ_RADIUS_FULL, O.OZ_RADIUS_FOLDED):
+    for radius in O.OZ_RADIUS_ALL:
         assert _contains_all(IA, IB, O.ozaki_igemm(IA, IB, 16, radius))
     # The radius ratio depends on the moduli count (the scaling error E) and
     # on the data: for k = 300 products over a 41-binade exponent span, E of
@@ -317,7 +317,7 @@ def test_ozaki_restatement_encloses_and_is_not_wider(kind):
     assert ratio.max() <= 2.0
 
 
-@pytest.mark.parametrize("radius", [O.OZ_RADIUS_FULL, O.OZ_RADIUS_FOLDED])
+@pytest.mark.parametrize("radius", O.OZ_RADIUS_ALL)
 @pytest.mark.parametrize("nmod", [12, 16, 20])
 @pytest.mark.parametrize("case", ozaki_cases.NAMES)
 def test_ozaki_restatement_encloses_every_entry_on_digit_stress(case, nmod, radius):

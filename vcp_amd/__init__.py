@@ -32,6 +32,9 @@ PARAM_OZ_RADIUS = 5
 NOALLOC_SCRATCH_BYTES = 100 << 20
 OZ_RADIUS_FULL = 1
 OZ_RADIUS_FOLDED = 2
+OZ_RADIUS_HALF = 3
+OZ_RADIUS_CEIL = 4
+OZ_RADIUS_NAMES = {1: "full", 2: "folded", 3: "half", 4: "ceil"}
 
 ERRORS = {
     -1: "dimension", -2: "argument", -3: "device", -4: "oom", -5: "rounding",

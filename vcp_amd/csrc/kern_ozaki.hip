@@ -189,28 +189,19 @@ VCPHIP_DEVINL unsigned digits2_ru_neglo(double x, int g) {
 }
 
 // operand planes of GEMM output z.  Residue z < nmod: A plane z x B plane z.
-// Radius-digit outputs r = z - nmod (planes nmod + 0..3 per operand: term 1
-// hi, lo; term 2 hi, lo):
-//   r = 0, 2  hi x hi of term r/2                 (one pass)
-//   r = 1, 3  hi x lo + lo x hi of term (r-1)/2   (two passes into one sum)
-//   r = 4, 5  lo x lo of term r - 4               (one pass; the full radius
-//             scheme only)
-VCPHIP_DEVINL void oz_pass_planes(int z, int nmod, int& npass, int& pa0, int& pb0, int& pa1, int& pb1) {
+// Radius-digit output r = z - nmod: the passes of P's table (rnp[r] of them,
+// summed into one int32 output), digit planes counted after the residues.
+VCPHIP_DEVINL void oz_pass_planes(const VcphipOzParams& P, int z, int nmod, int& npass, int& pa0, int& pb0, int& pa1,
+                                  int& pb1) {
   npass = 1;
   pa0 = pb0 = pa1 = pb1 = z;
   if (z < nmod) return;
   const int r = z - nmod;
-  if (r >= 4) {
-    pa0 = pb0 = pa1 = pb1 = nmod + 1 + 2 * (r - 4);
-    return;
-  }
-  const int base = nmod + 2 * (r >> 1);
-  pa0 = pb0 = pa1 = pb1 = base;
-  if (r & 1) {
-    npass = 2;
-    pb0 = base + 1;
-    pa1 = base + 1;
-  }
+  npass = P.rnp[r];
+  pa0 = nmod + P.rpa[r][0];
+  pb0 = nmod + P.rpb[r][0];
+  pa1 = nmod + P.rpa[r][1];
+  pb1 = nmod + P.rpb[r][1];
 }
 
 typedef unsigned short vcphip_v2u16 __attribute__((ext_vector_type(2)));
@@ -253,15 +244,16 @@ VCPHIP_DEVINL int sym_residue12(const Chunks12& c, int t, const VcphipOzParams& 
 
 constexpr int SR_ROWS = 16, SR_K = 64, SR_LD = 68;   // LDS byte tile (one plane block), odd-word row stride
 
-// rows of A (m x k, column-major intervals) -> N+4 planes of mpad x kpad:
-// N residue planes of A', then the hi/lo digits of |mid|+rad and of rad.
+// rows of A (m x k, column-major intervals) -> N+6 planes of mpad x kpad:
+// N residue planes of A', then the hi/lo digits of |mid|+rad and of rad, then
+// their rounded-up single digits H = hi + [lo > 0].
 template <int N>
 __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict__ A, long lda, int m, int k,
                                                        int mpad, int kpad, VcphipOzParams P, VcphipOzSide side,
                                                        int8_t* __restrict__ planes) {
   vcphip_mode_round_up();
-  extern __shared__ __attribute__((aligned(16))) unsigned char s_tile[];   // [N+4][SR_ROWS][SR_LD]
-  constexpr int nplane = N + VCPHIP_OZ_NDIGIT;
+  extern __shared__ __attribute__((aligned(16))) unsigned char s_tile[];   // [N+6][SR_ROWS][SR_LD]
+  constexpr int nplane = N + VCPHIP_OZ_NDIGIT_A;
   const int row0 = blockIdx.x * SR_ROWS, k0 = blockIdx.y * SR_K;
   const int tid = threadIdx.x;
   // each thread: one row, quads of 4 consecutive k (one 32-bit LDS word per plane)
@@ -282,7 +274,7 @@ __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict_
       v[u] = (rowok && kk < k) ? *reinterpret_cast<const double2*>(A + 2 * (r + lda * kk)) : make_double2(0.0, 0.0);
     }
     Chunks12 ch[4];
-    unsigned dsh = 0, dsl = 0, drh = 0, drl = 0;
+    unsigned dsh = 0, dsl = 0, drh = 0, drl = 0, dsc = 0, drc = 0;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       double am, ar;
@@ -295,6 +287,8 @@ __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict_
       dsl |= (ds >> 8) << (8 * u);
       drh |= (dr & 0xff) << (8 * u);
       drl |= (dr >> 8) << (8 * u);
+      dsc |= ((ds & 0xff) + ((ds >> 8) ? 1u : 0u)) << (8 * u);
+      drc |= ((dr & 0xff) + ((dr >> 8) ? 1u : 0u)) << (8 * u);
     }
     // modulus by modulus: only one modulus' constants are live at a time
 #pragma unroll
@@ -309,6 +303,8 @@ __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict_
     *reinterpret_cast<unsigned*>(&s_tile[((N + 1) * SR_ROWS + lr) * SR_LD + lk0]) = dsl;
     *reinterpret_cast<unsigned*>(&s_tile[((N + 2) * SR_ROWS + lr) * SR_LD + lk0]) = drh;
     *reinterpret_cast<unsigned*>(&s_tile[((N + 3) * SR_ROWS + lr) * SR_LD + lk0]) = drl;
+    *reinterpret_cast<unsigned*>(&s_tile[((N + 4) * SR_ROWS + lr) * SR_LD + lk0]) = dsc;
+    *reinterpret_cast<unsigned*>(&s_tile[((N + 5) * SR_ROWS + lr) * SR_LD + lk0]) = drc;
   }
   __syncthreads();
   // write out: per plane one 1 KB block (16 rows x 64 k), one word per thread
@@ -427,7 +423,7 @@ __global__ __launch_bounds__(256) void k_oz_gemm3(int mpad, int npad, int kpad, 
   vcphip_tile_coords(local, ntm, ntn, 8, tm, tn);
   const int row0 = tm * TMB, col0 = tn * TNB;
   int npass, pa0, pb0, pa1, pb1;
-  oz_pass_planes(z, nmod, npass, pa0, pb0, pa1, pb1);
+  oz_pass_planes(P, z, nmod, npass, pa0, pb0, pa1, pb1);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w & 1, wn = w >> 1;
   const int wu = __builtin_amdgcn_readfirstlane(w);
@@ -584,7 +580,7 @@ __global__ __launch_bounds__(256) void k_oz_gemm4(int mpad, int npad, int kpad, 
   vcphip_tile_coords(local, ntm, ntn, 8, tm, tn);
   const int row0 = tm * TMB, col0 = tn * TNB;
   int npass, pa0, pb0, pa1, pb1;
-  oz_pass_planes(z, nmod, npass, pa0, pb0, pa1, pb1);
+  oz_pass_planes(P, z, nmod, npass, pa0, pb0, pa1, pb1);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w & 1, wn = w >> 1;
   const int wu = __builtin_amdgcn_readfirstlane(w);
@@ -779,7 +775,7 @@ __global__ __launch_bounds__(64 * NW) void k_oz_gemm5(int mpad, int npad, int kp
   vcphip_tile_coords(local, ntm, ntn, 8, tm, tn);
   const int row0 = tm * TMB, col0 = tn * TNB;
   int npass, pa0, pb0, pa1, pb1;
-  oz_pass_planes(z, nmod, npass, pa0, pb0, pa1, pb1);
+  oz_pass_planes(P, z, nmod, npass, pa0, pb0, pa1, pb1);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w & 1, wn = w >> 1;
   const int wu = __builtin_amdgcn_readfirstlane(w);
@@ -945,7 +941,7 @@ VCPHIP_DEVINL void conv_limbs2(const unsigned (&mag)[NL], int s, double& tz, dou
 }
 
 template <int N>
-__global__ __launch_bounds__(256) void k_oz_recon(int m, int n, int k, int mpad, int npad, int nrad, VcphipOzParams P,
+__global__ __launch_bounds__(256) void k_oz_recon(int m, int n, int k, int mpad, int npad, VcphipOzParams P,
                                                   VcphipOzSide sa, VcphipOzSide sb,
                                                   const uint8_t* __restrict__ U, const int32_t* __restrict__ Z,
                                                   double* __restrict__ C, long ldc) {
@@ -972,33 +968,31 @@ __global__ __launch_bounds__(256) void k_oz_recon(int m, int n, int k, int mpad,
   conv_limbs2(mag, sc, tz, aw);
   const double lo_mid = neg ? -aw : tz, hi_mid = neg ? -tz : aw;
   // radius term (|mA|+rA) rB + rA |mB| (pidblas :277-283) from the digit
-  // products: with x1 <= (h1 + l1/127) 2^g1/127 (A) and x2 <= (H + L/127)
-  // 2^g2/127 (B, L <= 0),
-  //   sum x1 x2 <= 2^(g1+g2)/127^2 (Zhh + Zmix/127 + Zll/127^2),
-  // Zhh = sum h1 H, Zmix = sum h1 L + l1 H, Zll = sum l1 L <= 0.  The full
-  // scheme (nrad = 6) has Zll; the folded one (nrad = 4) leaves it out,
-  // which only raises the bound.  Each quotient is bounded above by the
-  // RU or, for a negative numerator, the RD reciprocal.
+  // products.  With x1 <= (h1 + l1/127) 2^g1/127 <= H1 2^g1/127 (A, l1 >= 0)
+  // and x2 <= (H + L/127) 2^g2/127 <= H 2^g2/127 (B, L <= 0), per term
+  //   full    sum x1 x2 <= 2^(g1+g2)/127^2 (Zhh + Zmix/127 + Zll/127^2),
+  //           Zhh = sum h1 H, Zmix = sum h1 L + l1 H, Zll = sum l1 L <= 0;
+  //   folded  the same without Zll (<= 0, so the bound only rises);
+  //   half    (Zhh + Zlh/127) with Zlh = sum l1 H;
+  //   ceil    Zcc = sum H1 H.
+  // A quotient of a numerator of either sign is bounded above by the
+  // numerator times the RU or, when negative, the RD reciprocal.
   double t1 = 0.0, t2 = 0.0;
-  const bool full = nrad > 4;
-  if (!((fa & VCPHIP_OZ_SUMZERO) || (fb & VCPHIP_OZ_RADZERO))) {
-    const int zm = Z[plane + boff];
-    double u = (double)Z[boff] + (double)zm * (zm >= 0 ? P.c1 : P.c1dn);
+  const bool ceil_only = P.rscheme == VCPHIP_OZ_RADIUS_CEIL, full = P.rscheme == VCPHIP_OZ_RADIUS_FULL;
+  auto digit_sum = [&](int zh, int zm, int zl) {
+    if (ceil_only) return (double)Z[zh * plane + boff];
+    const int mx = Z[zm * plane + boff];
+    double u = (double)Z[zh * plane + boff] + (double)mx * (mx >= 0 ? P.c1 : P.c1dn);
     if (full) {
-      const int zl = Z[4 * plane + boff];
-      u = u + (double)zl * (zl >= 0 ? P.c127 : P.c127dn);
+      const int ll = Z[zl * plane + boff];
+      u = u + (double)ll * (ll >= 0 ? P.c127 : P.c127dn);
     }
-    t1 = vcphip_mulpow2(u, sa.g[i] + sb.h[j]) * P.c127;
-  }
-  if (!((fa & VCPHIP_OZ_RADZERO) || zc)) {
-    const int zm = Z[3 * plane + boff];
-    double u = (double)Z[2 * plane + boff] + (double)zm * (zm >= 0 ? P.c1 : P.c1dn);
-    if (full) {
-      const int zl = Z[5 * plane + boff];
-      u = u + (double)zl * (zl >= 0 ? P.c127 : P.c127dn);
-    }
-    t2 = vcphip_mulpow2(u, sa.h[i] + fj) * P.c127;
-  }
+    return u;
+  };
+  if (!((fa & VCPHIP_OZ_SUMZERO) || (fb & VCPHIP_OZ_RADZERO)))
+    t1 = vcphip_mulpow2(ceil_only ? digit_sum(0, 0, 0) : digit_sum(0, 1, 4), sa.g[i] + sb.h[j]) * P.c127;
+  if (!((fa & VCPHIP_OZ_RADZERO) || zc))
+    t2 = vcphip_mulpow2(ceil_only ? digit_sum(1, 1, 1) : digit_sum(2, 3, 5), sa.h[i] + fj) * P.c127;
   const double rt = t1 + t2;
   // scaling error of the midpoint product
   double E = 0.0;
@@ -1043,7 +1037,7 @@ int vcphip_oz_stats(const double* X, long rs, long cs, int rows, int len, Vcphip
 int vcphip_oz_split_rows(const double* A, long lda, int m, int k, int mpad, int kpad,
                          const VcphipOzParams& P, VcphipOzSide side, int8_t* planes, hipStream_t s) {
   dim3 grid(mpad / SR_ROWS, kpad / SR_K);
-  const size_t lds = (size_t)(P.nmod + VCPHIP_OZ_NDIGIT) * SR_ROWS * SR_LD;
+  const size_t lds = (size_t)(P.nmod + VCPHIP_OZ_NDIGIT_A) * SR_ROWS * SR_LD;
   switch (P.nmod) {
 #define VCPHIP_SPLIT_CASE(NN) \
   case NN: hipLaunchKernelGGL((k_oz_split_rows<NN>), grid, dim3(256), lds, s, A, lda, m, k, mpad, kpad, P, side, planes); break;
@@ -1154,15 +1148,15 @@ int vcphip_oz_gemm(int mpad, int npad, int kpad, int nmod, int nrad, const int8_
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int vcphip_oz_recon(int m, int n, int k, int mpad, int npad, int nrad, const VcphipOzParams& P, VcphipOzSide sa,
+int vcphip_oz_recon(int m, int n, int k, int mpad, int npad, const VcphipOzParams& P, VcphipOzSide sa,
                     VcphipOzSide sb, const uint8_t* U, const int32_t* Z, double* C, long ldc,
                     hipStream_t s) {
   if (m <= 0 || n <= 0) return 0;
-  if (nrad != VCPHIP_OZ_NZ_FOLDED && nrad != VCPHIP_OZ_NZ_FULL) return -7;
+  if (P.rscheme < VCPHIP_OZ_RADIUS_FULL || P.rscheme > VCPHIP_OZ_RADIUS_CEIL) return -7;
   dim3 grid((unsigned)(mpad / 16), (unsigned)(npad / 16));
   switch (P.nmod) {
 #define VCPHIP_RECON_CASE(NN) \
-  case NN: hipLaunchKernelGGL((k_oz_recon<NN>), grid, dim3(256), 0, s, m, n, k, mpad, npad, nrad, P, sa, sb, U, Z, C, ldc); break;
+  case NN: hipLaunchKernelGGL((k_oz_recon<NN>), grid, dim3(256), 0, s, m, n, k, mpad, npad, P, sa, sb, U, Z, C, ldc); break;
     VCPHIP_RECON_CASE(8) VCPHIP_RECON_CASE(9) VCPHIP_RECON_CASE(10) VCPHIP_RECON_CASE(11)
     VCPHIP_RECON_CASE(12) VCPHIP_RECON_CASE(13) VCPHIP_RECON_CASE(14) VCPHIP_RECON_CASE(15)
     VCPHIP_RECON_CASE(16) VCPHIP_RECON_CASE(17) VCPHIP_RECON_CASE(18) VCPHIP_RECON_CASE(19)

@@ -189,7 +189,58 @@ int oz_radius_scheme(const vcphip_ctx* c, const vcphip_ctx::Tune* tuned, int k) 
   if (tuned && tuned->radius) return tuned->radius;
   return k >= VCPHIP_OZ_FOLDED_MINK ? VCPHIP_OZ_RADIUS_FOLDED : VCPHIP_OZ_RADIUS_FULL;
 }
-int oz_nz(int radius) { return radius == VCPHIP_OZ_RADIUS_FOLDED ? VCPHIP_OZ_NZ_FOLDED : VCPHIP_OZ_NZ_FULL; }
+bool oz_radius_valid(long long v) { return v >= VCPHIP_OZ_RADIUS_FULL && v <= VCPHIP_OZ_RADIUS_CEIL; }
+
+// int8 plane products of the radius term under each scheme
+int oz_radius_products(int radius) {
+  switch (radius) {
+    case VCPHIP_OZ_RADIUS_FULL: return 8;
+    case VCPHIP_OZ_RADIUS_FOLDED: return 6;
+    case VCPHIP_OZ_RADIUS_HALF: return 4;
+    default: return 2;
+  }
+}
+
+// the radius-digit GEMM outputs of a scheme (digit planes: A 0 |m|+r hi,
+// 1 lo, 2 r hi, 3 lo, 4 |m|+r rounded-up, 5 r rounded-up; B 0 r H, 1 L,
+// 2 |m| H, 3 L), in the order the reconstruction reads them
+void oz_set_radius(VcphipOzParams* P, int radius) {
+  P->rscheme = radius;
+  auto out = [&](int r, int a0, int b0, int a1 = -1, int b1 = -1) {
+    P->rnp[r] = a1 < 0 ? 1 : 2;
+    P->rpa[r][0] = a0;
+    P->rpb[r][0] = b0;
+    P->rpa[r][1] = a1 < 0 ? a0 : a1;
+    P->rpb[r][1] = b1 < 0 ? b0 : b1;
+  };
+  switch (radius) {
+    case VCPHIP_OZ_RADIUS_FULL:
+    case VCPHIP_OZ_RADIUS_FOLDED:
+      out(0, 0, 0);            // term 1: hi x H
+      out(1, 0, 1, 1, 0);      //         hi x L + lo x H
+      out(2, 2, 2);            // term 2
+      out(3, 2, 3, 3, 2);
+      P->rnz = 4;
+      if (radius == VCPHIP_OZ_RADIUS_FULL) {
+        out(4, 1, 1);          // lo x L of each term
+        out(5, 3, 3);
+        P->rnz = 6;
+      }
+      break;
+    case VCPHIP_OZ_RADIUS_HALF:
+      out(0, 0, 0);            // term 1: hi x H, lo x H
+      out(1, 1, 0);
+      out(2, 2, 2);            // term 2
+      out(3, 3, 2);
+      P->rnz = 4;
+      break;
+    default:                   // ceil: rounded-up x H of each term
+      out(0, 4, 0);
+      out(1, 5, 2);
+      P->rnz = 2;
+      break;
+  }
+}
 
 int gcd_i(int a, int b) {
   while (b) {
@@ -496,7 +547,7 @@ extern "C" int vcphip_ctx_set_param(vcphip_ctx* c, int key, long long v) {
       c->dd_nmod = v;
       return VCPHIP_OK;
     case VCPHIP_PARAM_OZ_RADIUS:
-      if (v != 0 && v != VCPHIP_OZ_RADIUS_FULL && v != VCPHIP_OZ_RADIUS_FOLDED) return VCPHIP_ERR_ARGUMENT;
+      if (v != 0 && !oz_radius_valid(v)) return VCPHIP_ERR_ARGUMENT;
       c->oz_radius = v;
       return VCPHIP_OK;
     default: return VCPHIP_ERR_ARGUMENT;
@@ -554,7 +605,7 @@ extern "C" int vcphip_ozaki_plan(vcphip_ctx* c, int m, int n, int k, int* nmod, 
   if (radius) *radius = rad;
   if (variant) *variant = var;
   // residue products plus hi x hi, two mixed and (full) lo x lo per radius term
-  if (products) *products = nm + oz_nz(rad) + 2;
+  if (products) *products = nm + oz_radius_products(rad);
   return VCPHIP_OK;
 }
 
@@ -577,15 +628,15 @@ static int oz_plan(vcphip_ctx* c, int m, int n, int k, OzPlan& p) {
   p.nmod = c->oz_nmod ? (int)c->oz_nmod : (tuned && tuned->nmod ? tuned->nmod : default_nmod(k));
   const int rc = oz_make_params(k, p.nmod, &p.P);
   if (rc) return rc;
-  p.nz = oz_nz(oz_radius_scheme(c, tuned, k));
+  oz_set_radius(&p.P, oz_radius_scheme(c, tuned, k));
+  p.nz = p.P.rnz;
   p.variant = c->oz_variant ? (int)c->oz_variant : (tuned && tuned->variant ? tuned->variant : DEFAULT_VARIANT);
   p.mpad = (m + VCPHIP_OZ_PADMN - 1) / VCPHIP_OZ_PADMN * VCPHIP_OZ_PADMN;
   p.npad = (n + VCPHIP_OZ_PADMN - 1) / VCPHIP_OZ_PADMN * VCPHIP_OZ_PADMN;
   p.kpad = k > 0 ? (k + VCPHIP_OZ_PADK - 1) / VCPHIP_OZ_PADK * VCPHIP_OZ_PADK : VCPHIP_OZ_PADK;
   p.nchunk = k > 0 ? (k + VCPHIP_OZ_CHUNK - 1) / VCPHIP_OZ_CHUNK : 0;
-  const size_t nplane = (size_t)p.nmod + VCPHIP_OZ_NDIGIT;
-  p.b_pa = nplane * p.mpad * p.kpad;
-  p.b_pb = nplane * p.npad * p.kpad;
+  p.b_pa = ((size_t)p.nmod + VCPHIP_OZ_NDIGIT_A) * p.mpad * p.kpad;
+  p.b_pb = ((size_t)p.nmod + VCPHIP_OZ_NDIGIT_B) * p.npad * p.kpad;
   p.b_pu = (size_t)p.nmod * p.mpad * p.npad;
   p.b_pz = (size_t)p.nz * sizeof(int32_t) * (size_t)p.mpad * p.npad;
   auto side_bytes = [&](int rows) {
@@ -627,7 +678,7 @@ static int oz_run(vcphip_ctx* c, const OzPlan& p, int m, int n, int k, const dou
                                         (uint8_t*)pu, (int32_t*)pz, p.variant, c->stream))))
     return rc;
   T.mark("gemm");
-  if ((rc = check_launch(vcphip_oz_recon(m, n, k, mpad, npad, p.nz, p.P, SA, SB, (const uint8_t*)pu,
+  if ((rc = check_launch(vcphip_oz_recon(m, n, k, mpad, npad, p.P, SA, SB, (const uint8_t*)pu,
                                          (const int32_t*)pz, C, ldc, c->stream))))
     return rc;
   T.mark("recon");
@@ -2471,28 +2522,30 @@ int tune_gemm_stage(vcphip_ctx* c, bool dd, int m, int n, int k, int* best_varia
     rc = oz_make_params(k, nmod, &P);
   }
   if (rc) return rc;
-  const int nrad = dd ? 0 : oz_nz(oz_radius_scheme(c, tuned, k));
+  if (!dd) oz_set_radius(&P, oz_radius_scheme(c, tuned, k));
+  const int nrad = dd ? 0 : P.rnz;
   const int mpad = (m + VCPHIP_OZ_PADMN - 1) / VCPHIP_OZ_PADMN * VCPHIP_OZ_PADMN;
   const int npad = (n + VCPHIP_OZ_PADMN - 1) / VCPHIP_OZ_PADMN * VCPHIP_OZ_PADMN;
   const int kpad = (k + VCPHIP_OZ_PADK - 1) / VCPHIP_OZ_PADK * VCPHIP_OZ_PADK;
   void *pa, *pb, *pu, *pz = nullptr;
-  const size_t nplane = (size_t)nmod + (dd ? 0 : VCPHIP_OZ_NDIGIT);
-  if ((rc = ws_get(c, vcphip_ctx::WS_PA, nplane * mpad * kpad, &pa))) return rc;
-  if ((rc = ws_get(c, vcphip_ctx::WS_PB, nplane * npad * kpad, &pb))) return rc;
+  const size_t nplane_a = (size_t)nmod + (dd ? 0 : VCPHIP_OZ_NDIGIT_A);
+  const size_t nplane_b = (size_t)nmod + (dd ? 0 : VCPHIP_OZ_NDIGIT_B);
+  if ((rc = ws_get(c, vcphip_ctx::WS_PA, nplane_a * mpad * kpad, &pa))) return rc;
+  if ((rc = ws_get(c, vcphip_ctx::WS_PB, nplane_b * npad * kpad, &pb))) return rc;
   if ((rc = ws_get(c, vcphip_ctx::WS_U, (size_t)nmod * mpad * npad, &pu))) return rc;
   if (!dd && (rc = ws_get(c, vcphip_ctx::WS_Z, (size_t)nrad * sizeof(int32_t) * (size_t)mpad * npad, &pz)))
     return rc;
   // operand planes of random bytes: timings on the zeros of fresh memory do
   // not carry over to real operands
-  if ((rc = check_launch(vcphip_fill_random(pa, nplane * mpad * kpad, 0x5EED0001u, c->stream)))) return rc;
-  if ((rc = check_launch(vcphip_fill_random(pb, nplane * npad * kpad, 0x5EED0002u, c->stream)))) return rc;
+  if ((rc = check_launch(vcphip_fill_random(pa, nplane_a * mpad * kpad, 0x5EED0001u, c->stream)))) return rc;
+  if ((rc = check_launch(vcphip_fill_random(pb, nplane_b * npad * kpad, 0x5EED0002u, c->stream)))) return rc;
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess) return VCPHIP_ERR_DEVICE;
   if (hipEventCreate(&e1) != hipSuccess) {
     hipEventDestroy(e0);
     return VCPHIP_ERR_DEVICE;
   }
-  const double flops = 2.0 * mpad * npad * (double)kpad * (double)(nmod + (dd ? 0 : nrad + 2));
+  const double flops = 2.0 * mpad * npad * (double)kpad * (double)(nmod + (dd ? 0 : oz_radius_products(P.rscheme)));
   const int reps = flops > 1e13 ? 2 : (flops > 1e11 ? 4 : 10);
   int best = 0;
   double best_t = 0.0;
@@ -2552,7 +2605,11 @@ extern "C" int vcphip_tune_igemm_moduli_dev(vcphip_ctx* c, int m, int n, int k, 
   if (m <= 0 || n <= 0 || k <= 0 || lda < m || ldb < k) return VCPHIP_ERR_DIMENSION;
   if (k > VCPHIP_OZ_MAXK) return VCPHIP_ERR_UNSUPPORTED;
   hipSetDevice(c->device);
-  const int s = std::max(1, std::min(m, sample_rows > 0 ? sample_rows : 256));
+  // the sampled maximum of the ratio underestimates the full one, and the
+  // more so the smaller the sample: by default max(1024, m/8) rows (at
+  // n = 8192 on exponents over 41 binades, 256 rows accepted a choice whose
+  // full-matrix maximum was 2.16 against a target of 2; 1024 rows did not)
+  const int s = std::max(1, std::min(m, sample_rows > 0 ? sample_rows : std::max(1024, m / 8)));
   // sample rows, the plain and the Ozaki products of the sample, block maxima
   const size_t bs = sizeof(double) * 2 * (size_t)s * k, bc = sizeof(double) * 2 * (size_t)s * n;
   const int nblocks = 1024;
@@ -2578,9 +2635,10 @@ extern "C" int vcphip_tune_igemm_moduli_dev(vcphip_ctx* c, int m, int n, int k, 
   // sampled ratio winning a tie of cost
   int chosen = 0, chosen_rad = 0, last = 0, last_rad = 0;
   double chosen_ratio = 0.0, last_ratio = 0.0;
-  for (int cost = 12 + 6; cost <= VCPHIP_OZ_MAXMOD_IV + 8 && !chosen && !rc; ++cost) {
-    for (const int rad : {VCPHIP_OZ_RADIUS_FOLDED, VCPHIP_OZ_RADIUS_FULL}) {
-      const int t = cost - (rad == VCPHIP_OZ_RADIUS_FOLDED ? 6 : 8);
+  for (int cost = 12 + 2; cost <= VCPHIP_OZ_MAXMOD_IV + 8 && !chosen && !rc; ++cost) {
+    for (const int rad : {VCPHIP_OZ_RADIUS_CEIL, VCPHIP_OZ_RADIUS_HALF, VCPHIP_OZ_RADIUS_FOLDED,
+                          VCPHIP_OZ_RADIUS_FULL}) {
+      const int t = cost - oz_radius_products(rad);
       VcphipOzParams P;
       if (t < 12 || t > VCPHIP_OZ_MAXMOD_IV || oz_make_params(k, t, &P)) continue;   // too few moduli for this k
       c->oz_nmod = t;
@@ -2720,7 +2778,7 @@ extern "C" int vcphip_tune_load(vcphip_ctx* c, const char* path) {
       const bool ok = (iv || dd) && lm >= 0 && lm <= 31 && ln >= 0 && ln <= 31 && lk >= 0 && lk <= 31 &&
                       (v == 0 || (v >= 6 && v <= 16)) && (nm == 0 || (iv && nm >= 8 && nm <= VCPHIP_OZ_MAXMOD_IV) ||
                                   (dd && nm >= 24 && nm <= VCPHIP_OZ_MAXMOD)) &&
-                      (rad == 0 || (iv && (rad == VCPHIP_OZ_RADIUS_FULL || rad == VCPHIP_OZ_RADIUS_FOLDED)));
+                      (rad == 0 || (iv && oz_radius_valid(rad)));
       if (!ok) {
         rc = VCPHIP_ERR_ARGUMENT;
         break;

@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/vcphip.h"
+
 enum { VCPHIP_DR_II = 0, VCPHIP_DR_IP = 1, VCPHIP_DR_PI = 2, VCPHIP_DR_PP = 3 };
 
 // plain FP64 directed-rounding GEMM family (kern_drgemm.hip)
@@ -54,6 +56,11 @@ int vcphip_fill_f64(double* p, long n, double v, hipStream_t s);
 // ---- Ozaki scheme II (kern_ozaki.hip) ------------------------------------
 #define VCPHIP_OZ_MAXMOD 32   // moduli (<= 20 for intervals, up to 32 for double-double)
 #define VCPHIP_OZ_MAXMOD_IV 20
+// radius-digit GEMM outputs (int32 planes of Z) of each scheme, for both
+// radius terms together: full 6 (hi x hi, hi x lo + lo x hi, lo x lo per
+// term: 8 plane products), folded 4 (no lo x lo: 6), half 4 (A's hi and lo
+// against B's rounded-up digit: 4), ceil 2 (rounded-up digits: 2)
+#define VCPHIP_OZ_MAXRAD 6
 #define VCPHIP_OZ_CHUNK 256   // k-chunk of the upward 1-norm partial sums
 #define VCPHIP_OZ_PADMN 256   // m and n are padded to multiples of this
 #define VCPHIP_OZ_PADK 128    // k is padded to a multiple of this
@@ -83,19 +90,23 @@ struct VcphipOzParams {
   double c1dn;                 // RD(1/127)
   double c127;                 // RU(1/127^2)
   double c127dn;               // RD(1/127^2)
+  // radius-digit GEMM outputs of the interval form: rscheme (a
+  // VCPHIP_OZ_RADIUS_* value; 0 for the dd form), rnz outputs; output r sums
+  // rnp[r] passes (1 or 2) of A digit plane rpa[r][p] times B digit plane
+  // rpb[r][p] (digit planes counted from the first one after the residues)
+  int rscheme, rnz;
+  int rnp[VCPHIP_OZ_MAXRAD], rpa[VCPHIP_OZ_MAXRAD][2], rpb[VCPHIP_OZ_MAXRAD][2];
 };
 
-// radius-digit planes that follow the N residue planes (two 7-bit upward
-// digits hi/lo per factor):  A: |mid|+rad hi, lo, rad hi, lo (h, l >= 0);
-//                            B: rad hi, lo, |mid| hi, lo with the low digit
-// non-positive (H = h + [l > 0], L = l - 127 [l > 0]: the same bound), so
-// that the lo*lo term left out of the digit products is never positive
-#define VCPHIP_OZ_NDIGIT 4
-// radius-digit GEMM outputs (int32 planes of Z): hi x hi and hi x lo + lo x hi
-// of each term (folded scheme, 6 plane products), plus lo x lo of each term
-// (full scheme, 8 plane products)
-#define VCPHIP_OZ_NZ_FOLDED 4
-#define VCPHIP_OZ_NZ_FULL 6
+// radius-digit planes that follow the N residue planes (7-bit upward digits
+// per factor of the radius term, against the row / column maximum):
+//   A (h, l >= 0):  |mid|+rad hi, lo, rad hi, lo, then the rounded-up single
+//                   digits H = h + [l > 0] of |mid|+rad and of rad;
+//   B (H, L <= 0):  rad H, L, |mid| H, L (the same bound h + l/127 written
+//                   with a non-positive low digit; H alone is the rounded-up
+//                   single digit)
+#define VCPHIP_OZ_NDIGIT_A 6
+#define VCPHIP_OZ_NDIGIT_B 4
 // untuned products with k at least this use the folded scheme
 #define VCPHIP_OZ_FOLDED_MINK 4096
 #define VCPHIP_OZ_MAXK 65536  // 2 k 127^2 < 2^31 for the mixed digit products
@@ -117,8 +128,9 @@ struct VcphipOzSide {
 
 int vcphip_oz_stats(const double* X, long rs, long cs, int rows, int len, VcphipOzSide side,
                     hipStream_t s);
-// planes: N residue planes followed by two radius-digit planes, each
-// [rows_pad][kpad] int8 with k contiguous
+// planes: N residue planes followed by the radius-digit planes
+// (VCPHIP_OZ_NDIGIT_A for A, _B for B), each [rows_pad][kpad] int8 with k
+// contiguous
 int vcphip_oz_split_rows(const double* A, long lda, int m, int k, int mpad, int kpad,
                          const VcphipOzParams& P, VcphipOzSide side, int8_t* planes, hipStream_t s);
 int vcphip_oz_split_cols(const double* B, long ldb, int k, int n, int npad, int kpad,
@@ -127,8 +139,8 @@ int vcphip_oz_split_cols(const double* B, long ldb, int k, int n, int npad, int 
 int vcphip_oz_gemm(int mpad, int npad, int kpad, int nmod, int nrad, const int8_t* Aplanes,
                    const int8_t* Bplanes, const VcphipOzParams& P, uint8_t* U, int32_t* Z, int variant,
                    hipStream_t s);
-// nrad: VCPHIP_OZ_NZ_FULL or VCPHIP_OZ_NZ_FOLDED radius products in Z
-int vcphip_oz_recon(int m, int n, int k, int mpad, int npad, int nrad, const VcphipOzParams& P, VcphipOzSide sa,
+// the radius outputs in Z follow P.rscheme / P.rnz
+int vcphip_oz_recon(int m, int n, int k, int mpad, int npad, const VcphipOzParams& P, VcphipOzSide sa,
                     VcphipOzSide sb, const uint8_t* U, const int32_t* Z, double* C, long ldc,
                     hipStream_t s);
 
