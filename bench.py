@@ -311,17 +311,18 @@ def main():
 def measured_traffic(n, nmod, nprod):
     """HBM bytes per launch of the int8 GEMM stage, from the committed
     FETCH_SIZE / WRITE_SIZE counter runs (profiles/ozaki_gemm_traffic.json),
-    when they were taken on this workload (size, moduli and plane products);
+    when one was taken on this workload (size, moduli and plane products);
     otherwise None."""
     path = os.path.join(ROOT, "profiles", "ozaki_gemm_traffic.json")
     try:
         with open(path) as f:
-            rec = json.load(f)
-    except (OSError, ValueError):
+            recs = json.load(f).get("records", [])
+    except (OSError, ValueError, AttributeError):
         return None
-    if rec.get("n") != n or rec.get("moduli") != nmod or rec.get("plane_products", nmod + 6) != nprod:
-        return None
-    return rec.get("bytes_per_launch")
+    for rec in recs:
+        if rec.get("n") == n and rec.get("moduli") == nmod and rec.get("plane_products") == nprod:
+            return rec.get("bytes_per_launch")
+    return None
 
 
 def bench_resid(ctx, reps):
