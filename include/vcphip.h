@@ -74,7 +74,7 @@ enum {
 /* context parameters */
 enum {
     VCPHIP_PARAM_OZ_NMOD = 1,      /* moduli of the Ozaki form (0 = tuned/default) */
-    VCPHIP_PARAM_OZ_VARIANT = 2,   /* int8 GEMM tile variant 6-16 (0 = tuned/default) */
+    VCPHIP_PARAM_OZ_VARIANT = 2,   /* int8 GEMM tile variant 6-18 (0 = tuned/default) */
     VCPHIP_PARAM_STAGE_TIMING = 3, /* 1: record per-stage times with HIP events  */
     VCPHIP_PARAM_DD_NMOD = 4,      /* moduli of the dd product (0 = default)      */
     VCPHIP_PARAM_OZ_RADIUS = 5     /* radius-digit scheme of the Ozaki form:

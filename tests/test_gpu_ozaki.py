@@ -58,7 +58,7 @@ def test_ozaki_nmod_bitexact(ctx, nmod):
     assert np.array_equal(C, O.ozaki_igemm(IA, IB, nmod))
 
 
-@pytest.mark.parametrize("variant", [6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16])
+@pytest.mark.parametrize("variant", [6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18])
 def test_ozaki_tile_variants_agree(ctx, variant):
     rng = np.random.default_rng(99)
     IA, IB = _iv(rng, 300, 520, "wide"), _iv(rng, 520, 280, "wide")

@@ -70,7 +70,39 @@ __device__ __forceinline__ void dr_tile(int m, int n, int k, const double* __res
   const int b_k = tid % BK, b_col = tid / BK;      // column b_col + 16q
 
   Stage<AI, BI> st;
+  // global loads of one k-slice; a slice whose rows, columns and k lie
+  // inside the operands (every slice of an interior tile but a partial last
+  // one) takes the unchecked path (the choice is uniform per workgroup).  The
+  // interval x point form keeps the checked path only: the second path makes
+  // it spill at three waves per SIMD.
   auto load_tile = [&](int k0) {
+    constexpr bool FAST = II || !AI;
+    const bool inside = FAST && row0 + BM <= m && col0 + BN <= n && k0 + BK <= k;
+    if (inside) {
+#pragma unroll
+      for (int q = 0; q < A_PER_T; ++q) {
+        const int kk = k0 + a_kc + 4 * q, r = row0 + a_row;
+        if (AI) {
+          const double2 v = *reinterpret_cast<const double2*>(A + 2 * (r + lda * kk));
+          st.a0[q] = v.x;
+          st.a1[q] = v.y;
+        } else {
+          st.a0[q] = A[r + lda * kk];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < B_PER_T; ++q) {
+        const int kk = k0 + b_k, c = col0 + b_col + 16 * q;
+        if (BI) {
+          const double2 v = *reinterpret_cast<const double2*>(B + 2 * (kk + ldb * c));
+          st.b0[q] = v.x;
+          st.b1[q] = v.y;
+        } else {
+          st.b0[q] = B[kk + ldb * c];
+        }
+      }
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < A_PER_T; ++q) {
       const int kk = k0 + a_kc + 4 * q, r = row0 + a_row;
@@ -176,14 +208,16 @@ __device__ __forceinline__ void dr_tile(int m, int n, int k, const double* __res
       }
 #pragma unroll
       for (int i = 0; i < TM; ++i) {
+        // (|mA| + rA), RU (:277): an add per k and row, not a third LDS array,
+        // since the tile's LDS reads already match its FMA time
         double as = 0.0;
-        if (II) as = __builtin_fabs(am[i]) + ar[i];   // (|mA| + rA), RU (:277)
+        if (II) as = __builtin_fabs(am[i]) + ar[i];
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
           up[i][j] = __builtin_fma(am[i], bm[j], up[i][j]);
           dn[i][j] = __builtin_fma(-am[i], bm[j], dn[i][j]);
           if (II) {
-            r1[i][j] = __builtin_fma(as, br[j], r1[i][j]);
+            r1[i][j] = __builtin_fma(as, br[j], r1[i][j]);   // (|mA| + rA) rB (:277-279)
             r2[i][j] = __builtin_fma(ar[i], __builtin_fabs(bm[j]), r2[i][j]);
           } else if (AI) {
             r1[i][j] = __builtin_fma(ar[i], __builtin_fabs(bm[j]), r1[i][j]);   // rA*|B|  (:129)

@@ -889,6 +889,164 @@ __global__ __launch_bounds__(64 * NW) void k_oz_gemm5(int mpad, int npad, int kp
   }
 }
 
+// 256 x 384 workgroup tile, 17% fewer operand bytes per int8 op than the
+// 256 x 256 tiles: eight waves in a 2 x 4 grid, each 128 x 96 (4 x 3
+// fragments of v_mfma_i32_32x32x32_i8, 192 accumulator registers).  Each
+// 64-k stage runs as two 32-k halves whose fragments are read just before
+// their MFMAs (the other wave on the SIMD covers that LDS latency), which
+// keeps a wave within 256 registers.  NST stages, NST - 1 in flight: the
+// barrier that opens stage kt also frees stage kt - 1, which is refilled
+// with k-step kt + NST - 1 before the MFMAs of stage kt.  The last column
+// tile may pass npad (a multiple of 256): its loads beyond npad re-read the
+// last 16-column segment, so every wave issues the same loads per stage, and
+// its outputs there are not stored.
+template <int NST>
+__global__ __launch_bounds__(512) void k_oz_gemm6(int mpad, int npad, int kpad, int nmod, int nz,
+                                                  const int8_t* __restrict__ Aplanes,
+                                                  const int8_t* __restrict__ Bplanes, VcphipOzParams P,
+                                                  uint8_t* __restrict__ U, int32_t* __restrict__ Z) {
+  constexpr int FM = 4, FN = 3, NW = 8;
+  constexpr int WM = 32 * FM, WN = 32 * FN;
+  constexpr int TMB = 2 * WM, TNB = (NW / 2) * WN;   // 256 x 384
+  constexpr int SEGA = TMB / 16, SEGB = TNB / 16;
+  constexpr int SEGA_W = SEGA / NW, SEGB_W = SEGB / NW;
+  constexpr int G = SEGA_W + SEGB_W;                  // loads per wave per stage
+  static_assert(SEGA_W * NW == SEGA && SEGB_W * NW == SEGB, "tile/wave mismatch");
+  static_assert(NST >= 2 && (NST - 1) * G < 64, "vmcnt range");
+  __shared__ __attribute__((aligned(16))) int8_t sA[NST][TMB * 64];
+  __shared__ __attribute__((aligned(16))) int8_t sB[NST][TNB * 64];
+
+  const int ntm = mpad / TMB, ntn = (npad + TNB - 1) / TNB;
+  const int tiles = ntm * ntn;
+  const int z = blockIdx.x / tiles;
+  const int local = vcphip_xcd_remap(blockIdx.x - z * tiles, tiles);
+  int tm, tn;
+  vcphip_tile_coords(local, ntm, ntn, 8, tm, tn);
+  const int row0 = tm * TMB, col0 = tn * TNB;
+  int npass, pa0, pb0, pa1, pb1;
+  oz_pass_planes(P, z, nmod, npass, pa0, pb0, pa1, pb1);
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w & 1, wn = w >> 1;
+  const int wu = __builtin_amdgcn_readfirstlane(w);
+  const uint32_t sA_base = lds_addr(&sA[0][0]), sB_base = lds_addr(&sB[0][0]);
+  const int lrow = lane >> 2, lchunk = (lane & 3) ^ ((lane >> 4) & 3);
+  const int src_lane = lrow * 64 + 16 * lchunk;   // inside the 1 KB block of 16 rows x 64 k
+  const long kblk = kpad / 64;
+  const int last_seg_b = npad / 16 - 1;
+  // this wave's B segments (16 columns each), clamped to the plane
+  int bseg[SEGB_W];
+#pragma unroll
+  for (int q = 0; q < SEGB_W; ++q) {
+    const int sg = col0 / 16 + wu * SEGB_W + q;
+    bseg[q] = sg < last_seg_b ? sg : last_seg_b;
+  }
+  // fragment rows: R = 32 i + lane % 32; k half h reads logical chunks 2 h + kq
+  const int frow = lane & 31, kq = lane >> 5, sw = (frow >> 2) & 3;
+  const int fs0 = 16 * ((0 + kq) ^ sw), fs1 = 16 * ((2 + kq) ^ sw);
+  const int fa0 = (wm * WM + frow) * 64 + fs0, fa1 = (wm * WM + frow) * 64 + fs1;
+  const int fb0 = (wn * WN + frow) * 64 + fs0, fb1 = (wn * WN + frow) * 64 + fs1;
+
+  v16i acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = v16i{};
+
+  const int nk = kpad / 64;
+  for (int pass = 0; pass < npass; ++pass) {
+    const int8_t* Ab = Aplanes + (long)(pass ? pa1 : pa0) * mpad * kpad + (long)(row0 / 16) * kblk * 1024 + src_lane;
+    const int8_t* Bb = Bplanes + (long)(pass ? pb1 : pb0) * npad * kpad + src_lane;
+    auto issue = [&](int stage, int kt) {
+#pragma unroll
+      for (int q = 0; q < SEGA_W; ++q) {
+        const int seg = wu * SEGA_W + q;
+        lds_dma16(Ab + ((long)seg * kblk + kt) * 1024, sA_base + stage * (TMB * 64) + seg * 1024);
+      }
+#pragma unroll
+      for (int q = 0; q < SEGB_W; ++q) {
+        const int seg = wu * SEGB_W + q;
+        lds_dma16(Bb + ((long)bseg[q] * kblk + kt) * 1024, sB_base + stage * (TNB * 64) + seg * 1024);
+      }
+    };
+    auto half = [&](int stage, int h) {
+      const int8_t* la = sA[stage] + (h ? fa1 : fa0);
+      const int8_t* lb = sB[stage] + (h ? fb1 : fb0);
+      v4i a[FM], b[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) a[i] = *reinterpret_cast<const v4i*>(la + i * 2048);
+#pragma unroll
+      for (int j = 0; j < FN; ++j) b[j] = *reinterpret_cast<const v4i*>(lb + j * 2048);
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int i = 0; i < FM; ++i) acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j], acc[i][j], 0, 0, 0);
+    };
+    const int npro = nk < NST - 1 ? nk : NST - 1;
+    for (int st = 0; st < npro; ++st) issue(st, st);
+    for (int kt = 0; kt < nk; ++kt) {
+      // stage kt landed for this wave once only the later stages' loads
+      // (those issued: up to kt + NST - 2) may still be in flight
+      const int later = (kt + NST - 2 < nk ? kt + NST - 2 : nk - 1) - kt;
+      if (later >= NST - 2 && NST > 2) asm volatile("s_waitcnt vmcnt(%0)" ::"n"((NST - 2) * G) : "memory");
+      else if (later == 1) asm volatile("s_waitcnt vmcnt(%0)" ::"n"(G) : "memory");
+      else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+      if (kt + NST - 1 < nk) issue((kt + NST - 1) % NST, kt + NST - 1);
+      half(kt % NST, 0);
+      half(kt % NST, 1);
+    }
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  }
+
+  // epilogue (as k_oz_gemm5's): registers 4q..4q+3 of fragment (i, j) are
+  // lane l2 of the 16 x 16 block (row half q / 2, column half (lane % 32) /
+  // 16) of that fragment; fragments past npad are not stored
+  const long blocks_m = mpad / 16;
+  const int l2 = 16 * (kq) + (lane & 15), chalf = (lane & 31) >> 4;
+  if (z < nmod) {
+    const int md = P.mod[z];
+    const double rc = P.rcpd[z];
+    uint8_t* Uz = U + (long)z * mpad * npad;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        if (col0 + wn * WN + 32 * j >= npad) continue;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const long bi = (row0 + wm * WM + 32 * i) / 16 + (q >> 1);
+          const long bj = (col0 + wn * WN + 32 * j) / 16 + chalf;
+          unsigned packed = 0;
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const double y = (double)acc[i][j][4 * q + t];
+            const double qd = __builtin_floor(y * rc);
+            int rr = (int)(y - qd * (double)md);
+            rr += (rr < 0) ? md : 0;
+            rr -= (rr >= md) ? md : 0;
+            packed |= (unsigned)rr << (8 * t);
+          }
+          *reinterpret_cast<unsigned*>(Uz + (bj * blocks_m + bi) * 256 + (l2 + 32 * (q & 1)) * 4) = packed;
+        }
+      }
+  } else {
+    int32_t* Zz = Z + (long)(z - nmod) * mpad * npad;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        if (col0 + wn * WN + 32 * j >= npad) continue;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const long bi = (row0 + wm * WM + 32 * i) / 16 + (q >> 1);
+          const long bj = (col0 + wn * WN + 32 * j) / 16 + chalf;
+          const v4i val = v4i{acc[i][j][4 * q], acc[i][j][4 * q + 1], acc[i][j][4 * q + 2], acc[i][j][4 * q + 3]};
+          *reinterpret_cast<v4i*>(Zz + (bj * blocks_m + bi) * 256 + (l2 + 32 * (q & 1)) * 4) = val;
+        }
+      }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // reconstruction
 // ---------------------------------------------------------------------------
@@ -1141,6 +1299,18 @@ int vcphip_oz_gemm(int mpad, int npad, int kpad, int nmod, int nrad, const int8_
       else
         hipLaunchKernelGGL((k_oz_gemm5<4, 2, 3, 8>), dim3(tiles * nz), dim3(512), 0, s, mpad, npad, kpad, nmod, nz,
                            Aplanes, Bplanes, P, U, Z);
+      break;
+    }
+    case 17:
+    case 18: {
+      if (kpad % 64 || npad % 256) return -7;
+      const int tiles = (mpad / 256) * ((npad + 383) / 384);
+      if (variant == 17)
+        hipLaunchKernelGGL((k_oz_gemm6<3>), dim3(tiles * nz), dim3(512), 0, s, mpad, npad, kpad, nmod, nz, Aplanes,
+                           Bplanes, P, U, Z);
+      else
+        hipLaunchKernelGGL((k_oz_gemm6<2>), dim3(tiles * nz), dim3(512), 0, s, mpad, npad, kpad, nmod, nz, Aplanes,
+                           Bplanes, P, U, Z);
       break;
     }
     default: return -7;

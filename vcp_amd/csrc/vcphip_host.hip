@@ -533,7 +533,7 @@ extern "C" int vcphip_ctx_set_param(vcphip_ctx* c, int key, long long v) {
       c->oz_nmod = v;
       return VCPHIP_OK;
     case VCPHIP_PARAM_OZ_VARIANT:
-      if (v != 0 && (v < 6 || v > 16)) return VCPHIP_ERR_ARGUMENT;
+      if (v != 0 && (v < 6 || v > 18)) return VCPHIP_ERR_ARGUMENT;
       c->oz_variant = v;
       return VCPHIP_OK;
     case VCPHIP_PARAM_STAGE_TIMING:   // (re)enabling clears the totals
@@ -2502,7 +2502,7 @@ extern "C" int vcphip_residual_batched(vcphip_ctx* c, int count, const int* n, c
 // ===========================================================================
 namespace {
 
-constexpr int TUNE_VARIANTS[] = {6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16};
+constexpr int TUNE_VARIANTS[] = {6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18};
 
 // time the int8 MFMA stage of one Ozaki product shape with every tile variant
 // (operand planes of arbitrary content: the integer MFMA time does not depend
@@ -2776,7 +2776,7 @@ extern "C" int vcphip_tune_load(vcphip_ctx* c, const char* path) {
     if (nf == 6 + (version >= 2 ? 1 : 0)) {   // version 1 lines carry no radius scheme
       const bool iv = strcmp(kind, "igemm") == 0, dd = strcmp(kind, "ddgemm") == 0;
       const bool ok = (iv || dd) && lm >= 0 && lm <= 31 && ln >= 0 && ln <= 31 && lk >= 0 && lk <= 31 &&
-                      (v == 0 || (v >= 6 && v <= 16)) && (nm == 0 || (iv && nm >= 8 && nm <= VCPHIP_OZ_MAXMOD_IV) ||
+                      (v == 0 || (v >= 6 && v <= 18)) && (nm == 0 || (iv && nm >= 8 && nm <= VCPHIP_OZ_MAXMOD_IV) ||
                                   (dd && nm >= 24 && nm <= VCPHIP_OZ_MAXMOD)) &&
                       (rad == 0 || (iv && oz_radius_valid(rad)));
       if (!ok) {
