@@ -325,6 +325,41 @@ def measured_traffic(n, nmod, nprod):
     return None
 
 
+def padded_fraction(ms, ns, ks, shapes):
+    """padded fraction of the multiply-adds of a batched launch's tiles:
+    rows in 64- or 32-row blocks by the shape of each problem's size class
+    (shapes: ceil(log2 rows) -> mode; missing = 64 x 64), columns in 64"""
+    import math
+
+    tile = useful = 0.0
+    for m, n, k in zip(ms, ns, ks):
+        mode = shapes.get(max(0, math.ceil(math.log2(m))) if m > 0 else 0, 1)
+        rows = 0
+        while rows < m:
+            rows += 32 if mode == 3 or (mode == 2 and m - rows <= 32) else 64
+        tile += float(rows) * 64 * math.ceil(n / 64) * k
+        useful += float(m) * n * k
+    return 1.0 - useful / tile if tile else 0.0
+
+
+def tuned_shapes(ctx, kind):
+    """the tile shape chosen per size class (from the saved tuning table)"""
+    import tempfile
+
+    import vcp_amd
+
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "tune.txt")
+        ctx.tune_save(path)
+        with open(path) as f:
+            for ln in f:
+                fs = ln.split()
+                if len(fs) == 4 and fs[0] == "tiles" and fs[1] == kind:
+                    out["n<=%d" % (1 << int(fs[2]))] = vcp_amd.BATCH_TILES_NAMES[int(fs[3])]
+    return out
+
+
 def bench_resid(ctx, reps):
     """256 verified residuals R_i A_i - I of uneven size (n log-uniform in
     [64, 1024]) in one graph-captured persistent launch; latency per batch."""
@@ -345,16 +380,28 @@ def bench_resid(ctx, reps):
     torch.cuda.synchronize()
     args = (ns, [t.data_ptr() for t in dR], ns, [t.data_ptr() for t in dA], ns, [t.data_ptr() for t in dC], ns,
             nub.data_ptr())
-    vcp_amd.residual_batched_dev(ctx, *args)      # builds the descriptors and captures the graph
-    ctx.synchronize()
-    lat = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        vcp_amd.residual_batched_dev(ctx, *args)  # graph replay
+    def replays():
+        vcp_amd.residual_batched_dev(ctx, *args)      # builds the descriptors and captures the graph
         ctx.synchronize()
-        lat.append(time.perf_counter() - t0)
-    lat = np.array(lat) * 1e3
+        lat = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            vcp_amd.residual_batched_dev(ctx, *args)  # graph replay
+            ctx.synchronize()
+            lat.append(time.perf_counter() - t0)
+        return np.array(lat) * 1e3
+
     flops = 6.0 * sum(float(n) ** 3 for n in ns)
+    lat = replays()   # untuned: 64 x 64 tiles, three workgroups per CU
+    untuned = {"p50_ms": round(float(np.percentile(lat, 50)), 4),
+               "frac_fp64_peak": round(flops / (float(np.percentile(lat, 50)) * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4),
+               "padded_fraction": round(padded_fraction(ns, ns, ns, {}), 4)}
+    t0 = time.perf_counter()
+    per_cu, _, padded = vcp_amd.tune_residual_dev(ctx, *args)
+    tuning = {"seconds": round(time.perf_counter() - t0, 3), "workgroups_per_cu": per_cu,
+              "tile_shapes": tuned_shapes(ctx, "residual"), "padded_fraction": round(padded, 4)}
+    log("batched residual untuned %s, tuning %s" % (untuned, tuning))
+    lat = replays()
     p50 = float(np.percentile(lat, 50))
     nb = nub.cpu().numpy()
     # bytes the batched residual must move: R (8 n^2) and IA (16 n^2) read, C (16 n^2)
@@ -366,7 +413,9 @@ def bench_resid(ctx, reps):
            "frac_fp64_peak": round(flops / (p50 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4),
            "hbm_algorithmic_bytes": hbm, "hbm_algorithmic_GBps_p50": round(hbm / (p50 * 1e-3) / 1e9, 1),
            "frac_hbm_peak": round(hbm / (p50 * 1e-3) / 8e12, 4),
-           "all_normub_below_1": bool((nb < 1.0).all()), "max_normub": float(nb.max())}
+           "all_normub_below_1": bool((nb < 1.0).all()), "max_normub": float(nb.max()),
+           "configuration": "autotuned tile shapes per size class and persistent grid", "tuning": tuning,
+           "untuned_64x64": untuned}
     try:
         import oracle_ffi as O
 
@@ -456,18 +505,40 @@ def bench_igemm_batched(ctx, count=256, reps=20):
     dC = [torch.empty((n, n, 2), dtype=torch.float64, device=dev) for n in ns]
     torch.cuda.synchronize()
     args = (ns, ns, ns, [t.data_ptr() for t in dA], ns, [t.data_ptr() for t in dB], ns, [t.data_ptr() for t in dC], ns)
-    ctx.igemm_batched_dev(*args)   # builds the tile list and captures the graph
-    ctx.synchronize()
-    lat = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        ctx.igemm_batched_dev(*args)
+    flops = 8.0 * sum(float(n) ** 3 for n in ns)
+
+    def replays():
+        ctx.igemm_batched_dev(*args)   # builds the tile list and captures the graph
         ctx.synchronize()
-        lat.append(time.perf_counter() - t0)
-    p50 = float(np.percentile(np.array(lat) * 1e3, 50))
-    # the same products one launch each
+        lat = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            ctx.igemm_batched_dev(*args)
+            ctx.synchronize()
+            lat.append(time.perf_counter() - t0)
+        return float(np.percentile(np.array(lat) * 1e3, 50))
+
+    p50u = replays()   # untuned: 64 x 64 tiles
+    untuned = {"p50_ms": round(p50u, 4), "frac_fp64_peak": round(flops / (p50u * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4),
+               "padded_fraction": round(padded_fraction(ns, ns, ns, {}), 4)}
+    t0 = time.perf_counter()
+    _, padded = ctx.tune_igemm_batched_dev(*args)
+    tuning = {"seconds": round(time.perf_counter() - t0, 3), "tile_shapes": tuned_shapes(ctx, "igemm"),
+              "padded_fraction": round(padded, 4)}
+    log("batched interval products untuned %s, tuning %s" % (untuned, tuning))
+    p50 = replays()
+    # every product against its single plain product, on the smallest and largest
     import vcp_amd
 
+    check = sorted(range(len(ns)), key=lambda i: ns[i])
+    same = True
+    for i in check[:4] + check[-2:]:
+        C1 = torch.empty_like(dC[i])
+        ctx.igemm_dev(vcp_amd.FORM_PLAIN, ns[i], ns[i], ns[i], dA[i].data_ptr(), ns[i], dB[i].data_ptr(), ns[i],
+                      C1.data_ptr(), ns[i])
+        ctx.synchronize()
+        same = same and bool(torch.equal(C1, dC[i]))
+    # the same products one launch each
     for i, n in enumerate(ns):
         ctx.igemm_dev(vcp_amd.FORM_PLAIN, n, n, n, dA[i].data_ptr(), n, dB[i].data_ptr(), n, dC[i].data_ptr(), n)
     ctx.synchronize()
@@ -477,11 +548,11 @@ def bench_igemm_batched(ctx, count=256, reps=20):
             ctx.igemm_dev(vcp_amd.FORM_PLAIN, n, n, n, dA[i].data_ptr(), n, dB[i].data_ptr(), n, dC[i].data_ptr(), n)
     ctx.synchronize()
     t_single = (time.perf_counter() - t0) / 3 * 1e3
-    flops = 8.0 * sum(float(n) ** 3 for n in ns)
     out = {"problems": count, "n_min": int(min(ns)), "n_max": int(max(ns)), "graph_launches": reps,
            "p50_ms": round(p50, 4), "tflops_p50": round(flops / (p50 * 1e-3) / 1e12, 3),
            "frac_fp64_peak": round(flops / (p50 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4),
-           "one_launch_per_product_ms": round(t_single, 4)}
+           "configuration": "autotuned tile shapes per size class", "tuning": tuning, "untuned_64x64": untuned,
+           "single_product_bitexact_6": same, "one_launch_per_product_ms": round(t_single, 4)}
     log("batched interval products %s" % out)
     del dA, dB, dC
     return out

@@ -77,9 +77,28 @@ enum {
     VCPHIP_PARAM_OZ_VARIANT = 2,   /* int8 GEMM tile variant 6-18 (0 = tuned/default) */
     VCPHIP_PARAM_STAGE_TIMING = 3, /* 1: record per-stage times with HIP events  */
     VCPHIP_PARAM_DD_NMOD = 4,      /* moduli of the dd product (0 = default)      */
-    VCPHIP_PARAM_OZ_RADIUS = 5     /* radius-digit scheme of the Ozaki form:
+    VCPHIP_PARAM_OZ_RADIUS = 5,    /* radius-digit scheme of the Ozaki form:
                                       VCPHIP_OZ_RADIUS_* (0 = tuned, else full for
                                       k < 4096 and folded from k = 4096) */
+    VCPHIP_PARAM_BATCH_TILES = 6   /* tile shape of the batched launches for every
+                                      problem: VCPHIP_BATCH_TILES_* (0 = tuned per
+                                      size class, else 64 x 64) */
+};
+
+/* tile shapes of the batched persistent launches (vcphip_igemm_batched*,
+ * vcphip_residual_batched_dev, vcphip_lss_batched*), per problem:
+ *   64    64 x 64 output tiles only;
+ *   EDGE  64 x 64 tiles, and 32 x 64 ones for a last row block of at most 32
+ *         rows;
+ *   32    32 x 64 tiles only.
+ * Each output's FMA chains are the same in every shape, so are its bits; the
+ * shapes trade padded work (rows rounded up to 64 or to 32) against the
+ * larger tile's efficiency.  Each shape runs as a persistent launch of its
+ * own, 64 x 64 first. */
+enum {
+    VCPHIP_BATCH_TILES_64 = 1,
+    VCPHIP_BATCH_TILES_EDGE = 2,
+    VCPHIP_BATCH_TILES_32 = 3
 };
 
 /* radius-digit schemes of the Ozaki form.  Each factor of the radius term
@@ -384,13 +403,26 @@ int vcphip_ozaki_plan(vcphip_ctx* ctx, int m, int n, int k, int* nmod, int* radi
 int vcphip_tune_igemm_moduli_dev(vcphip_ctx* ctx, int m, int n, int k, const double* IA, int lda,
                                  const double* IB, int ldb, double max_ratio, int sample_rows,
                                  int* nmod, int* radius, double* ratio);
-/* Times 1-4 persistent workgroups per CU for the batched residual of this
- * problem set (arguments as vcphip_residual_batched_dev, which it also runs)
- * and keeps the fastest; the next call captures its graph with it. */
+/* Tunes the batched residual for this problem set (arguments as
+ * vcphip_residual_batched_dev, which it also runs): first the tile shape of
+ * each size class of the set (VCPHIP_BATCH_TILES_*; classes are ceil(log2 n),
+ * taken in order of their work, each tried with the three shapes while the
+ * others keep theirs, the whole set's tile launches timed), then 1-4
+ * persistent workgroups per CU.  Keeps the fastest; the next call captures
+ * its graph with them.  *per_cu, *ms (the set's tile launches) and *padded
+ * (the fraction of the tiles' multiply-adds that is padding, under the
+ * choice) receive the result; any may be NULL. */
 int vcphip_tune_residual_dev(vcphip_ctx* ctx, int count, const int* n, const double* const* R,
                              const int* ldr, const double* const* IA, const int* lda,
                              double* const* C, const int* ldc, double* normub_dev, int* per_cu,
-                             double* ms);
+                             double* ms, double* padded);
+/* The same shape choice per size class (ceil(log2) of a problem's rows) for
+ * the batched interval products (arguments as vcphip_igemm_batched_dev, which
+ * it also runs).  *ms: the set's launch with the choice; *padded as above. */
+int vcphip_tune_igemm_batched_dev(vcphip_ctx* ctx, int count, const int* m, const int* n, const int* k,
+                                  const double* const* IA, const int* lda, const double* const* IB,
+                                  const int* ldb, double* const* C, const int* ldc, double* ms,
+                                  double* padded);
 /* Writes / reads the context's tuning table (text: a "vcphip-tune 2" line,
  * then "igemm|ddgemm lm ln lk variant nmod radius" and "residual per_cu"
  * lines; version 1 tables, without the radius field, still load). */

@@ -87,6 +87,70 @@ def test_batched_galerkin_block_sizes(ctx):
         assert np.array_equal(Cs[p], ctx.igemm(IAs[p], IBs[p], vcp_amd.FORM_PLAIN)), p
 
 
+@pytest.mark.parametrize("mode", [vcp_amd.BATCH_TILES_64, vcp_amd.BATCH_TILES_EDGE, vcp_amd.BATCH_TILES_32])
+def test_batched_every_tile_shape_bitexact(ctx, mode):
+    """the 64 x 64 and 32 x 64 tiles run the same chains for every output:
+    each shape gives the single product's bits, on row counts below, at and
+    above multiples of 32 and 64"""
+    shapes = SHAPES + [(31, 40, 50), (32, 33, 17), (33, 64, 9), (96, 70, 40), (97, 5, 130), (160, 129, 65)]
+    print("igemm_batched tile shape", vcp_amd.BATCH_TILES_NAMES[mode], flush=True)
+    rng = np.random.default_rng(75)
+    IAs, IBs = _set(rng, shapes)
+    ctx.set_param(vcp_amd.PARAM_BATCH_TILES, mode)
+    try:
+        assert ctx.get_param(vcp_amd.PARAM_BATCH_TILES) == mode
+        Cs = ctx.igemm_batched(IAs, IBs)
+    finally:
+        ctx.set_param(vcp_amd.PARAM_BATCH_TILES, 0)
+    for p, (m, n, k) in enumerate(shapes):
+        assert np.array_equal(Cs[p], ctx.igemm(IAs[p], IBs[p], vcp_amd.FORM_PLAIN)), (mode, p)
+        if m * n * k and m * n * k <= 300 * 300 * 300:
+            assert np.array_equal(Cs[p], O.pidblas_mulmm(IAs[p], IBs[p])), (mode, p)
+
+
+def test_batch_tiles_param_rejects_unknown_shapes(ctx):
+    for bad in (-1, 4):
+        with pytest.raises(vcp_amd.VcphipError):
+            ctx.set_param(vcp_amd.PARAM_BATCH_TILES, bad)
+    assert ctx.get_param(vcp_amd.PARAM_BATCH_TILES) == 0
+
+
+def test_tune_batched_shapes_then_replays_bitexact(ctx, tmp_path):
+    """the shape tuner times the set under each shape per size class, keeps
+    the fastest, and the graph captured with its choice gives the single
+    products' bits; the choice survives a save / load"""
+    rng = np.random.default_rng(76)
+    sizes = [65, 70, 90, 97, 130, 150, 200, 260, 300, 33]
+    IAs = [W.uniform_intervals(rng, n, n, -40) for n in sizes]
+    IBs = [W.uniform_intervals(rng, n, n, -40) for n in sizes]
+    dA, dB = [_dev(A) for A in IAs], [_dev(B) for B in IBs]
+    dC = [torch.zeros((n, n, 2), dtype=torch.float64, device="cuda") for n in sizes]
+    args = (sizes, sizes, sizes, [t.data_ptr() for t in dA], sizes, [t.data_ptr() for t in dB], sizes,
+            [t.data_ptr() for t in dC], sizes)
+    torch.cuda.synchronize()
+    ms, padded = ctx.tune_igemm_batched_dev(*args)
+    print("tuned batched shapes: %.3f ms, padded fraction %.4f" % (ms, padded), flush=True)
+    assert ms > 0.0 and 0.0 <= padded < 0.5
+    ref = [ctx.igemm(A, B, vcp_amd.FORM_PLAIN) for A, B in zip(IAs, IBs)]
+    for rep in range(3):   # capture with the tuned shapes, then replays
+        for t in dC:
+            t.zero_()
+        torch.cuda.synchronize()
+        ctx.igemm_batched_dev(*args)
+        ctx.synchronize()
+        for p in range(len(sizes)):
+            assert np.array_equal(dC[p].cpu().numpy().transpose(1, 0, 2), ref[p]), (rep, p)
+    path = tmp_path / "tune.txt"
+    ctx.tune_save(str(path))
+    lines = [ln for ln in path.read_text().splitlines() if ln.startswith("tiles igemm")]
+    assert lines, path.read_text()
+    with vcp_amd.Context(0) as c2:
+        c2.tune_load(str(path))
+        Cs = c2.igemm_batched(IAs, IBs)
+    for p in range(len(sizes)):
+        assert np.array_equal(Cs[p], ref[p]), p
+
+
 def test_batched_rejects_bad_dimensions(ctx):
     rng = np.random.default_rng(74)
     A, B = W.wide_exponent_intervals(rng, 5, 6), W.wide_exponent_intervals(rng, 6, 7)

@@ -28,6 +28,21 @@ def test_resid_bitexact_against_oracle(ctx):
         assert nb == nbo and nb < 1.0
 
 
+@pytest.mark.parametrize("mode", [vcp_amd.BATCH_TILES_64, vcp_amd.BATCH_TILES_EDGE, vcp_amd.BATCH_TILES_32])
+def test_resid_every_tile_shape_bitexact(ctx, mode):
+    print("residual tile shape", vcp_amd.BATCH_TILES_NAMES[mode], flush=True)
+    probs = _problems(SIZES + [31, 32, 33, 96, 97, 160], 4)
+    ctx.set_param(vcp_amd.PARAM_BATCH_TILES, mode)
+    try:
+        Cs, nub = ctx.residual_batched([p[0] for p in probs], [p[1] for p in probs])
+    finally:
+        ctx.set_param(vcp_amd.PARAM_BATCH_TILES, 0)
+    for (R, IA), C, nb in zip(probs, Cs, nub):
+        Co, nbo = O.residual(R, IA)
+        assert np.array_equal(C, Co)
+        assert nb == nbo and nb < 1.0
+
+
 def test_resid_encloses_exact_product(ctx):
     probs = _problems([40, 77], 2)
     Cs, nub = ctx.residual_batched([p[0] for p in probs], [p[1] for p in probs])

@@ -109,6 +109,12 @@ def test_tune_load_rejects_malformed(ctx, tmp_path):
     bad.write_text("vcphip-tune 2\nddgemm 10 10 10 8 32 1\n")   # dd products have no radius scheme
     with pytest.raises(vcp_amd.VcphipError):
         ctx.tune_load(str(bad))
+    bad.write_text("vcphip-tune 2\ntiles residual 9 4\n")   # no tile shape 4
+    with pytest.raises(vcp_amd.VcphipError):
+        ctx.tune_load(str(bad))
+    bad.write_text("vcphip-tune 2\ntiles gemm 9 1\n")
+    with pytest.raises(vcp_amd.VcphipError):
+        ctx.tune_load(str(bad))
     bad.write_text("not a table\n")
     with pytest.raises(vcp_amd.VcphipError):
         ctx.tune_load(str(bad))
@@ -125,8 +131,9 @@ def test_tune_residual_then_replay_bitexact(ctx):
     torch.cuda.synchronize()
     args = (sizes, [t.data_ptr() for t in dR], sizes, [t.data_ptr() for t in dA], sizes,
             [t.data_ptr() for t in dC], sizes, nub.data_ptr())
-    per, ms = vcp_amd.tune_residual_dev(ctx, *args)
-    assert 1 <= per <= 4 and ms > 0.0
+    per, ms, padded = vcp_amd.tune_residual_dev(ctx, *args)
+    print("tuned residual: %d workgroups per CU, %.3f ms, padded fraction %.4f" % (per, ms, padded), flush=True)
+    assert 1 <= per <= 4 and ms > 0.0 and 0.0 <= padded < 0.6
     for _ in range(3):   # first call captures with the tuned grid, then replays
         for t in dC:
             t.fill_(np.nan)

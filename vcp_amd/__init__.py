@@ -29,6 +29,11 @@ PARAM_OZ_VARIANT = 2
 PARAM_STAGE_TIMING = 3
 PARAM_DD_NMOD = 4
 PARAM_OZ_RADIUS = 5
+PARAM_BATCH_TILES = 6
+BATCH_TILES_64 = 1
+BATCH_TILES_EDGE = 2
+BATCH_TILES_32 = 3
+BATCH_TILES_NAMES = {1: "64x64", 2: "64x64+32x64 edge", 3: "32x64"}
 NOALLOC_SCRATCH_BYTES = 100 << 20
 OZ_RADIUS_FULL = 1
 OZ_RADIUS_FOLDED = 2
@@ -38,7 +43,7 @@ OZ_RADIUS_NAMES = {1: "full", 2: "folded", 3: "half", 4: "ceil"}
 
 ERRORS = {
     -1: "dimension", -2: "argument", -3: "device", -4: "oom", -5: "rounding",
-    -6: "verification", -7: "unsupported", -8: "range", -9: "io", -10: "singular",
+    -6: "verification", -7: "unsupported", -8: "range", -9: "io", -10: "singular", -11: "scratch",
 }
 
 _c_int = ctypes.c_int
@@ -107,7 +112,10 @@ SIGNATURES = {
     "vcphip_tune_ddgemm_tile": [_vp, _c_int, _c_int, _c_int, _p_int, _p_dbl],
     "vcphip_tune_igemm_moduli_dev": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, ctypes.c_double, _c_int,
                                      _p_int, _p_int, _p_dbl],
-    "vcphip_tune_residual_dev": [_vp, _c_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _p_dbl],
+    "vcphip_tune_residual_dev": [_vp, _c_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int, _p_dbl,
+                                 _p_dbl],
+    "vcphip_tune_igemm_batched_dev": [_vp, _c_int, _p_int, _p_int, _p_int, _vp, _p_int, _vp, _p_int, _vp, _p_int,
+                                      _p_dbl, _p_dbl],
     "vcphip_ozaki_plan": [_vp, _c_int, _c_int, _c_int, _p_int, _p_int, _p_int, _p_int],
     "vcphip_igemm_ozaki_noalloc_dev": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int],
     "vcphip_tune_save": [_vp, ctypes.c_char_p],
@@ -544,6 +552,19 @@ class Context:
         _check(lib().vcphip_igemm_batched_dev(self._h, cnt, ia(m), ia(n), ia(k), vp(dA), ia(lda), vp(dB), ia(ldb),
                                               vp(dC), ia(ldc)), "vcphip_igemm_batched_dev")
 
+    def tune_igemm_batched_dev(self, m, n, k, dA, lda, dB, ldb, dC, ldc):
+        """tile shape per size class of the batched interval products of this
+        set (arguments as igemm_batched_dev); returns (ms per launch of the
+        set, padded fraction of the tiles' work)"""
+        cnt = len(m)
+        ia = lambda xs: (ctypes.c_int * cnt)(*xs)
+        vp = lambda xs: (_vp * cnt)(*[_vp(x) for x in xs])
+        ms, pad = ctypes.c_double(), ctypes.c_double()
+        _check(lib().vcphip_tune_igemm_batched_dev(self._h, cnt, ia(m), ia(n), ia(k), vp(dA), ia(lda), vp(dB),
+                                                   ia(ldb), vp(dC), ia(ldc), ctypes.byref(ms), ctypes.byref(pad)),
+               "vcphip_tune_igemm_batched_dev")
+        return ms.value, pad.value
+
     def inv_mid(self, IA):
         """R = inv(mid(IA)) formed on the device (imid + pdblas::inv,
         vcp/imats.hpp:298-301, vcp/pdblas.hpp:180-204): kv's midpoints, then
@@ -696,16 +717,18 @@ class Context:
 
 
 def tune_residual_dev(ctx, ns, R_ptrs, ldr, A_ptrs, lda, C_ptrs, ldc, normub_ptr):
-    """autotune the batched residual's persistent grid for this problem set;
-    returns (workgroups per CU, ms per batch)"""
+    """autotune the batched residual for this problem set: the tile shape per
+    size class, then the persistent grid; returns (workgroups per CU, ms of
+    the set's tile launches, padded fraction of the tiles' work)"""
     cnt = len(ns)
     arr = lambda T, xs: (T * cnt)(*xs)  # noqa: E731
-    per, ms = ctypes.c_int(), ctypes.c_double()
+    per, ms, pad = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
     _check(lib().vcphip_tune_residual_dev(ctx.handle, cnt, arr(ctypes.c_int, ns), arr(_vp, R_ptrs),
                                           arr(ctypes.c_int, ldr), arr(_vp, A_ptrs), arr(ctypes.c_int, lda),
                                           arr(_vp, C_ptrs), arr(ctypes.c_int, ldc), _vp(normub_ptr),
-                                          ctypes.byref(per), ctypes.byref(ms)), "vcphip_tune_residual_dev")
-    return per.value, ms.value
+                                          ctypes.byref(per), ctypes.byref(ms), ctypes.byref(pad)),
+           "vcphip_tune_residual_dev")
+    return per.value, ms.value, pad.value
 
 
 def residual_batched_dev(ctx, ns, R_ptrs, ldr, A_ptrs, lda, C_ptrs, ldc, normub_ptr):

@@ -37,11 +37,22 @@ constexpr int LB = BN + PAD;
 constexpr int A_PER_T = BM * BK / NT;   // 4 elements of A per thread per tile
 constexpr int B_PER_T = BK * BN / NT;   // 4 elements of B per thread per tile
 
-template <bool AI, bool BI>
+template <bool AI, bool BI, int APT>
 struct Stage {
   // register staging of one global tile (prefetch)
-  double a0[A_PER_T], a1[A_PER_T];
+  double a0[APT], a1[APT];
   double b0[B_PER_T], b1[B_PER_T];
+};
+
+// LDS of one k-slice: midpoints and radii of A (rows) and B (columns).  One
+// instance per kernel, shared by every tile shape the kernel runs; a tile of
+// 32 rows uses the first 32 (+ padding) doubles of each A row.
+template <bool AI, bool BI>
+struct DrLds {
+  double Am[BK][LA];
+  double Ar[AI ? BK : 1][LA];
+  double Bm[BK][LB];
+  double Br[BI ? BK : 1][LB];
 };
 
 // column of output j (< 4) of thread column tx in a 64-wide tile: pairs
@@ -49,27 +60,34 @@ struct Stage {
 // pieces at consecutive addresses
 __device__ __forceinline__ int dr_col(int tx, int j) { return 2 * tx + (j & 1) + 32 * (j >> 1); }
 
-// One BM x BN output tile at (row0, col0).  RESID: the batched residual
-// form R*A - I, whose diagonal elements get the interval subtraction of 1
-// (imats.hpp:324-326).  Runs in the caller's round-up mode.
-template <bool AI, bool BI, bool RESID>
-__device__ __forceinline__ void dr_tile(int m, int n, int k, const double* __restrict__ A, long lda,
-                                        const double* __restrict__ B, long ldb, double* __restrict__ C,
+// One (16 TMR) x BN output tile at (row0, col0): TMR = 4 gives the 64 x 64
+// tile, TMR = 2 the 32 x 64 tile of the batched launches' short problems and
+// edge rows (each output's chains are the same in both, so are its bits).
+// RESID: the batched residual form R*A - I, whose diagonal elements get the
+// interval subtraction of 1 (imats.hpp:324-326).  Runs in the caller's
+// round-up mode.
+template <bool AI, bool BI, bool RESID, int TMR = TM, class Lds = DrLds<AI, BI>>
+__device__ __forceinline__ void dr_tile(Lds& L, int m, int n, int k, const double* __restrict__ A,
+                                        long lda, const double* __restrict__ B, long ldb, double* __restrict__ C,
                                         long ldc, double* __restrict__ C2, int accumulate, int row0, int col0) {
   constexpr bool II = AI && BI;
-  __shared__ __attribute__((aligned(16))) double sAm[BK][LA];
-  __shared__ __attribute__((aligned(16))) double sAr[AI ? BK : 1][LA];
-  __shared__ __attribute__((aligned(16))) double sBm[BK][LB];
-  __shared__ __attribute__((aligned(16))) double sBr[BI ? BK : 1][LB];
+  constexpr int TBM = 16 * TMR;              // tile rows
+  constexpr int APT = TBM * BK / NT;         // elements of A per thread per slice
+  constexpr int AKS = NT / TBM;              // k stride of a thread's A elements
+  static_assert(TMR == 4 || TMR == 2, "tile rows");
+  auto& sAm = L.Am;
+  auto& sAr = L.Ar;
+  auto& sBm = L.Bm;
+  auto& sBr = L.Br;
 
   const int tid = threadIdx.x;
 
-  // global-load mapping: A column-major, 64 consecutive rows per k column;
+  // global-load mapping: A column-major, TBM consecutive rows per k column;
   // B column-major, 16 consecutive k per column.
-  const int a_row = tid % BM, a_kc = tid / BM;     // k column a_kc + 4q
+  const int a_row = tid % TBM, a_kc = tid / TBM;   // k column a_kc + AKS q
   const int b_k = tid % BK, b_col = tid / BK;      // column b_col + 16q
 
-  Stage<AI, BI> st;
+  Stage<AI, BI, APT> st;
   // global loads of one k-slice; a slice whose rows, columns and k lie
   // inside the operands (every slice of an interior tile but a partial last
   // one) takes the unchecked path (the choice is uniform per workgroup).  The
@@ -77,11 +95,11 @@ __device__ __forceinline__ void dr_tile(int m, int n, int k, const double* __res
   // it spill at three waves per SIMD.
   auto load_tile = [&](int k0) {
     constexpr bool FAST = II || !AI;
-    const bool inside = FAST && row0 + BM <= m && col0 + BN <= n && k0 + BK <= k;
+    const bool inside = FAST && row0 + TBM <= m && col0 + BN <= n && k0 + BK <= k;
     if (inside) {
 #pragma unroll
-      for (int q = 0; q < A_PER_T; ++q) {
-        const int kk = k0 + a_kc + 4 * q, r = row0 + a_row;
+      for (int q = 0; q < APT; ++q) {
+        const int kk = k0 + a_kc + AKS * q, r = row0 + a_row;
         if (AI) {
           const double2 v = *reinterpret_cast<const double2*>(A + 2 * (r + lda * kk));
           st.a0[q] = v.x;
@@ -104,8 +122,8 @@ __device__ __forceinline__ void dr_tile(int m, int n, int k, const double* __res
       return;
     }
 #pragma unroll
-    for (int q = 0; q < A_PER_T; ++q) {
-      const int kk = k0 + a_kc + 4 * q, r = row0 + a_row;
+    for (int q = 0; q < APT; ++q) {
+      const int kk = k0 + a_kc + AKS * q, r = row0 + a_row;
       const bool ok = (r < m) && (kk < k);
       if (AI) {
         double2 v = ok ? *reinterpret_cast<const double2*>(A + 2 * (r + lda * kk)) : make_double2(0.0, 0.0);
@@ -130,14 +148,14 @@ __device__ __forceinline__ void dr_tile(int m, int n, int k, const double* __res
   };
   auto store_tile = [&]() {
 #pragma unroll
-    for (int q = 0; q < A_PER_T; ++q) {
+    for (int q = 0; q < APT; ++q) {
       if (AI) {
         double mm, rr;
         vcphip_midrad_ru(st.a0[q], st.a1[q], mm, rr);
-        sAm[a_kc + 4 * q][a_row] = mm;
-        sAr[a_kc + 4 * q][a_row] = rr;
+        sAm[a_kc + AKS * q][a_row] = mm;
+        sAr[a_kc + AKS * q][a_row] = rr;
       } else {
-        sAm[a_kc + 4 * q][a_row] = st.a0[q];
+        sAm[a_kc + AKS * q][a_row] = st.a0[q];
       }
     }
 #pragma unroll
@@ -154,9 +172,9 @@ __device__ __forceinline__ void dr_tile(int m, int n, int k, const double* __res
   };
 
   const int tx = tid % 16, ty = tid / 16;
-  double up[TM][TN], dn[TM][TN], r1[TM][TN], r2[TM][TN];
+  double up[TMR][TN], dn[TMR][TN], r1[TMR][TN], r2[TMR][TN];
 #pragma unroll
-  for (int i = 0; i < TM; ++i)
+  for (int i = 0; i < TMR; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
       up[i][j] = 0.0;
@@ -168,10 +186,10 @@ __device__ __forceinline__ void dr_tile(int m, int n, int k, const double* __res
   if (!AI && !BI && accumulate) {
     // udmatmul form: the chains continue from the values already in CU / CD
 #pragma unroll
-    for (int i = 0; i < TM; ++i)
+    for (int i = 0; i < TMR; ++i)
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
-        const int r = row0 + ty * TM + i, c = col0 + dr_col(tx, j);
+        const int r = row0 + ty * TMR + i, c = col0 + dr_col(tx, j);
         if (r < m && c < n) {
           up[i][j] = C[r + ldc * c];
           dn[i][j] = -C2[r + ldc * c];
@@ -187,19 +205,25 @@ __device__ __forceinline__ void dr_tile(int m, int n, int k, const double* __res
     if (t + 1 < ntiles) load_tile((t + 1) * BK);
 #pragma unroll 2
     for (int kk = 0; kk < BK; ++kk) {
-      double am[TM], ar[TM], bm[TN], br[TN];
+      double am[TMR], ar[TMR], bm[TN], br[TN];
       {
-        const double2 x0 = *reinterpret_cast<const double2*>(&sAm[kk][ty * TM]);
-        const double2 x1 = *reinterpret_cast<const double2*>(&sAm[kk][ty * TM + 2]);
-        am[0] = x0.x; am[1] = x0.y; am[2] = x1.x; am[3] = x1.y;
+#pragma unroll
+        for (int h = 0; h < TMR / 2; ++h) {
+          const double2 x = *reinterpret_cast<const double2*>(&sAm[kk][ty * TMR + 2 * h]);
+          am[2 * h] = x.x;
+          am[2 * h + 1] = x.y;
+        }
         const double2 y0 = *reinterpret_cast<const double2*>(&sBm[kk][2 * tx]);
         const double2 y1 = *reinterpret_cast<const double2*>(&sBm[kk][2 * tx + 32]);
         bm[0] = y0.x; bm[1] = y0.y; bm[2] = y1.x; bm[3] = y1.y;
       }
       if (AI) {
-        const double2 x0 = *reinterpret_cast<const double2*>(&sAr[kk][ty * TM]);
-        const double2 x1 = *reinterpret_cast<const double2*>(&sAr[kk][ty * TM + 2]);
-        ar[0] = x0.x; ar[1] = x0.y; ar[2] = x1.x; ar[3] = x1.y;
+#pragma unroll
+        for (int h = 0; h < TMR / 2; ++h) {
+          const double2 x = *reinterpret_cast<const double2*>(&sAr[kk][ty * TMR + 2 * h]);
+          ar[2 * h] = x.x;
+          ar[2 * h + 1] = x.y;
+        }
       }
       if (BI) {
         const double2 y0 = *reinterpret_cast<const double2*>(&sBr[kk][2 * tx]);
@@ -207,7 +231,7 @@ __device__ __forceinline__ void dr_tile(int m, int n, int k, const double* __res
         br[0] = y0.x; br[1] = y0.y; br[2] = y1.x; br[3] = y1.y;
       }
 #pragma unroll
-      for (int i = 0; i < TM; ++i) {
+      for (int i = 0; i < TMR; ++i) {
         // (|mA| + rA), RU (:277): an add per k and row, not a third LDS array,
         // since the tile's LDS reads already match its FMA time
         double as = 0.0;
@@ -240,8 +264,8 @@ __device__ __forceinline__ void dr_tile(int m, int n, int k, const double* __res
     const int c = col0 + dr_col(tx, j);
     if (c >= n) continue;
 #pragma unroll
-    for (int i = 0; i < TM; ++i) {
-      const int r = row0 + ty * TM + i;
+    for (int i = 0; i < TMR; ++i) {
+      const int r = row0 + ty * TMR + i;
       if (r >= m) continue;
       if (AI || BI) {
         double t;
@@ -273,38 +297,46 @@ __global__ __launch_bounds__(NT, (AI && BI) ? 2 : 3) void k_drgemm(int m, int n,
   const int tile = vcphip_xcd_remap(blockIdx.x, ntm * ntn);
   int tm, tn;
   vcphip_tile_coords(tile, ntm, ntn, 16, tm, tn);
-  dr_tile<AI, BI, false>(m, n, k, A, lda, B, ldb, C, ldc, C2, accumulate, tm * BM, tn * BN);
+  __shared__ __attribute__((aligned(16))) DrLds<AI, BI> lds;
+  dr_tile<AI, BI, false>(lds, m, n, k, A, lda, B, ldb, C, ldc, C2, accumulate, tm * BM, tn * BN);
 }
 
 // ---------------------------------------------------------------------------
 // Batched verified residual C_p = R_p * IA_p - I (imats::linearsolve,
-// vcp/imats.hpp:318-326) for many problems of uneven size in ONE persistent
-// launch: workgroups stride over a tile list sorted by problem size
-// (largest first), each tile being the PI form above.
+// vcp/imats.hpp:318-326) for many problems of uneven size in one persistent
+// launch per tile shape (TMR = 4: 64 x 64, TMR = 2: 32 x 64): workgroups
+// stride over a tile list sorted by problem size (largest first), each tile
+// being the PI form above.
 // ---------------------------------------------------------------------------
+template <int TMR>
 __global__ __launch_bounds__(NT, 3) void k_resid_tiles(int ntiles, const VcphipResidTile* __restrict__ tiles,
                                                         const VcphipResidProblem* __restrict__ probs) {
   vcphip_mode_round_up();
+  __shared__ __attribute__((aligned(16))) DrLds<false, true> lds;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const VcphipResidTile tl = tiles[t];
     const VcphipResidProblem p = probs[tl.problem];
-    dr_tile<false, true, true>(p.n, p.n, p.n, p.R, p.ldr, p.A, p.lda, p.C, p.ldc, nullptr, 0, tl.row0, tl.col0);
+    dr_tile<false, true, true, TMR>(lds, p.n, p.n, p.n, p.R, p.ldr, p.A, p.lda, p.C, p.ldc, nullptr, 0, tl.row0,
+                                    tl.col0);
     __syncthreads();
   }
 }
 
-// Many interval products C_p = IA_p * IB_p of uneven (m, n, k) in ONE
-// persistent launch (pidblas::mulmm, vcp/pidblas.hpp:239-294, per problem):
-// workgroups stride over a tile list sorted by problem size, each tile the
-// interval x interval form of k_drgemm, so every C_p is bit-identical to the
-// single product's.
+// Many interval products C_p = IA_p * IB_p of uneven (m, n, k) in one
+// persistent launch per tile shape (pidblas::mulmm, vcp/pidblas.hpp:239-294,
+// per problem): workgroups stride over a tile list sorted by problem size,
+// each tile the interval x interval form of k_drgemm, so every C_p is
+// bit-identical to the single product's.
+template <int TMR>
 __global__ __launch_bounds__(NT, 2) void k_igemm_tiles(int ntiles, const VcphipResidTile* __restrict__ tiles,
                                                         const VcphipIgemmProblem* __restrict__ probs) {
   vcphip_mode_round_up();
+  __shared__ __attribute__((aligned(16))) DrLds<true, true> lds;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const VcphipResidTile tl = tiles[t];
     const VcphipIgemmProblem p = probs[tl.problem];
-    dr_tile<true, true, false>(p.m, p.n, p.k, p.A, p.lda, p.B, p.ldb, p.C, p.ldc, nullptr, 0, tl.row0, tl.col0);
+    dr_tile<true, true, false, TMR>(lds, p.m, p.n, p.k, p.A, p.lda, p.B, p.ldb, p.C, p.ldc, nullptr, 0, tl.row0,
+                                    tl.col0);
     __syncthreads();
   }
 }
@@ -648,9 +680,10 @@ int vcphip_launch_drsyrk(int interval, int k, int n, const double* A, long lda, 
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int vcphip_launch_igemm_tiles(int ntiles, const VcphipResidTile* tiles, const VcphipIgemmProblem* probs, int grid,
-                              hipStream_t s) {
-  if (ntiles > 0) hipLaunchKernelGGL(k_igemm_tiles, dim3(grid), dim3(NT), 0, s, ntiles, tiles, probs);
+int vcphip_launch_igemm_tiles(int n64, int n32, const VcphipResidTile* tiles, const VcphipIgemmProblem* probs,
+                              int grid64, int grid32, hipStream_t s) {
+  if (n64 > 0) hipLaunchKernelGGL(k_igemm_tiles<4>, dim3(grid64), dim3(NT), 0, s, n64, tiles, probs);
+  if (n32 > 0) hipLaunchKernelGGL(k_igemm_tiles<2>, dim3(grid32), dim3(NT), 0, s, n32, tiles + n64, probs);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
@@ -660,10 +693,17 @@ int vcphip_fill_f64(double* p, long n, double v, hipStream_t s) {
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int vcphip_launch_resid(int ntiles, const VcphipResidTile* tiles, const VcphipResidProblem* probs, int nrows,
-                        const int* row_problem, double* rowsum, int count, double* normub, int grid,
+int vcphip_launch_resid_tiles(int n64, int n32, const VcphipResidTile* tiles, const VcphipResidProblem* probs,
+                              int grid64, int grid32, hipStream_t s) {
+  if (n64 > 0) hipLaunchKernelGGL(k_resid_tiles<4>, dim3(grid64), dim3(NT), 0, s, n64, tiles, probs);
+  if (n32 > 0) hipLaunchKernelGGL(k_resid_tiles<2>, dim3(grid32), dim3(NT), 0, s, n32, tiles + n64, probs);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_launch_resid(int n64, int n32, const VcphipResidTile* tiles, const VcphipResidProblem* probs, int nrows,
+                        const int* row_problem, double* rowsum, int count, double* normub, int grid64, int grid32,
                         hipStream_t s) {
-  if (ntiles > 0) hipLaunchKernelGGL(k_resid_tiles, dim3(grid), dim3(NT), 0, s, ntiles, tiles, probs);
+  vcphip_launch_resid_tiles(n64, n32, tiles, probs, grid64, grid32, s);
   if (nrows > 0) hipLaunchKernelGGL(k_resid_rowsum, dim3((nrows + 255) / 256), dim3(256), 0, s, nrows, row_problem, probs, rowsum);
   if (count > 0) hipLaunchKernelGGL(k_resid_max, dim3(count), dim3(256), 0, s, count, probs, rowsum, normub);
   return hipGetLastError() == hipSuccess ? 0 : -3;

@@ -32,10 +32,10 @@ struct vcphip_ctx {
   int device = 0;
   hipStream_t own = nullptr;
   hipStream_t stream = nullptr;
-  long long oz_nmod = 0, oz_variant = 0, timing = 0, dd_nmod = 0, oz_radius = 0;
+  long long oz_nmod = 0, oz_variant = 0, timing = 0, dd_nmod = 0, oz_radius = 0, batch_tiles = 0;
   // workspace slots (device memory, grown on demand)
   enum { WS_PA, WS_PB, WS_U, WS_Z, WS_SA, WS_SB, WS_HA, WS_HB, WS_HC, WS_HC2, WS_BATCH, WS_DD, WS_LSS, WS_HR, WS_LSSB,
-         WS_INV, WS_DDS, WS_IGB, WS_N };
+         WS_INV, WS_DDS, WS_IGB, WS_TT, WS_N };
   void* ws[WS_N] = {};
   size_t ws_size[WS_N] = {};
   // stage timing: events recorded by every call since the last synchronize,
@@ -54,7 +54,8 @@ struct vcphip_ctx {
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    int grid = 0;
+    int n64 = 0, n32 = 0;       // tiles of each shape (64 x 64 first in the tile array)
+    int grid = 0, grid32 = 0;   // persistent grids of the two tile launches
     // device descriptors of the cached set (for re-timing by the tuner)
     const VcphipResidTile* dt = nullptr;
     const VcphipResidProblem* dp = nullptr;
@@ -70,6 +71,10 @@ struct vcphip_ctx {
   };
   std::vector<Tune> tune;
   int resid_per_cu = 3;   // persistent workgroups per CU of the batched residual (its occupancy)
+  // tile shape of the batched launches per size class of a problem's rows
+  // (ceil(log2)): VCPHIP_BATCH_TILES_*, 0 = not tuned (64 x 64); [0] the
+  // residual, [1] the interval products
+  unsigned char tshape[2][32] = {};
   // second stream (default priority) of the batched solve: the right-hand
   // side steps run on it beside the residual's persistent launch
   hipStream_t aux = nullptr;
@@ -97,7 +102,8 @@ struct vcphip_ctx {
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    int grid = 0;
+    int n64 = 0, n32 = 0, grid = 0, grid32 = 0;
+    const VcphipIgemmProblem* dp = nullptr;   // device descriptors of the cached set
   } igb;
 };
 
@@ -550,6 +556,14 @@ extern "C" int vcphip_ctx_set_param(vcphip_ctx* c, int key, long long v) {
       if (v != 0 && !oz_radius_valid(v)) return VCPHIP_ERR_ARGUMENT;
       c->oz_radius = v;
       return VCPHIP_OK;
+    case VCPHIP_PARAM_BATCH_TILES:
+      if (v != 0 && (v < VCPHIP_BATCH_TILES_64 || v > VCPHIP_BATCH_TILES_32)) return VCPHIP_ERR_ARGUMENT;
+      if (v != c->batch_tiles) {   // cached tile lists are rebuilt on the next call
+        c->resid.probs.clear();
+        c->igb.probs.clear();
+      }
+      c->batch_tiles = v;
+      return VCPHIP_OK;
     default: return VCPHIP_ERR_ARGUMENT;
   }
 }
@@ -562,6 +576,7 @@ extern "C" int vcphip_ctx_get_param(vcphip_ctx* c, int key, long long* v) {
     case VCPHIP_PARAM_STAGE_TIMING: *v = c->timing; return VCPHIP_OK;
     case VCPHIP_PARAM_DD_NMOD: *v = c->dd_nmod; return VCPHIP_OK;
     case VCPHIP_PARAM_OZ_RADIUS: *v = c->oz_radius; return VCPHIP_OK;
+    case VCPHIP_PARAM_BATCH_TILES: *v = c->batch_tiles; return VCPHIP_OK;
     default: return VCPHIP_ERR_ARGUMENT;
   }
 }
@@ -925,6 +940,152 @@ extern "C" int vcphip_dd_residual_dev(vcphip_ctx* c, int m, int n, int k, const 
 // plain form): one persistent launch over a size-sorted tile list, captured
 // into a hipGraph and replayed while the problem set is unchanged
 // ===========================================================================
+// ---- tile lists of the batched persistent launches ---------------------------
+namespace {
+
+enum { TS_RESID = 0, TS_IGEMM = 1 };
+
+// shape mode of a problem with `rows` output rows: the parameter, else the
+// tuned table, else 64 x 64 tiles
+int batch_mode(const vcphip_ctx* c, int kind, int rows) {
+  if (c->batch_tiles) return (int)c->batch_tiles;
+  const int m = c->tshape[kind][size_class(rows)];
+  return m ? m : VCPHIP_BATCH_TILES_64;
+}
+
+struct TileSet {
+  std::vector<VcphipResidTile> t64, t32;
+  double tile_work = 0.0, useful = 0.0;   // multiply-adds of the tiles (padded) and of the problems
+};
+
+// the tiles of one m x n output (k deep) under a shape mode, in groups of
+// eight row blocks with the columns inside a group, so that a run of
+// consecutive tiles covers a compact block of the output
+void add_tiles(TileSet& s, int pi, int m, int n, int k, int mode) {
+  if (m <= 0 || n <= 0) return;
+  std::vector<int> r0s, hs;
+  for (int r0 = 0; r0 < m;) {
+    const int h = mode == VCPHIP_BATCH_TILES_32 ? 32 : mode == VCPHIP_BATCH_TILES_EDGE && m - r0 <= 32 ? 32 : 64;
+    r0s.push_back(r0);
+    hs.push_back(h);
+    r0 += h;
+  }
+  const int tn = (n + VCPHIP_RESID_TILE - 1) / VCPHIP_RESID_TILE;
+  const size_t GH = 8;
+  for (size_t g = 0; g < r0s.size(); g += GH) {
+    const size_t ge = std::min(r0s.size(), g + GH);
+    for (int b = 0; b < tn; ++b)
+      for (size_t a = g; a < ge; ++a) {
+        (hs[a] == 64 ? s.t64 : s.t32).push_back(VcphipResidTile{pi, r0s[a], b * VCPHIP_RESID_TILE, 0});
+        s.tile_work += (double)hs[a] * VCPHIP_RESID_TILE * k;
+      }
+  }
+  s.useful += (double)m * n * k;
+}
+
+// grid of one shape's persistent launch
+int batch_grid(const vcphip_ctx* c, size_t ntiles, int per_cu) {
+  return (int)std::max<size_t>(1, std::min(ntiles, (size_t)per_cu * (c->num_cu > 0 ? c->num_cu : 256)));
+}
+
+// the launch's tile array: 64 x 64 tiles, then 32 x 64 ones.  (Reordering
+// each part so that every XCD's workgroups take contiguous runs of tiles was
+// measured on 256 problems of n in [65, 1021]: no change in time.)
+std::vector<VcphipResidTile> tile_array(const TileSet& s) {
+  std::vector<VcphipResidTile> all(s.t64);
+  all.insert(all.end(), s.t32.begin(), s.t32.end());
+  return all;
+}
+
+TileSet igemm_tiles(const vcphip_ctx* c, const std::vector<VcphipIgemmProblem>& probs) {
+  // problems with the most work first (longest-processing-time order)
+  std::vector<int> order(probs.size());
+  for (size_t i = 0; i < probs.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return (double)probs[a].m * probs[a].n * probs[a].k > (double)probs[b].m * probs[b].n * probs[b].k;
+  });
+  TileSet s;
+  for (int pi : order)
+    add_tiles(s, pi, probs[pi].m, probs[pi].n, probs[pi].k, batch_mode(c, TS_IGEMM, probs[pi].m));
+  return s;
+}
+
+TileSet resid_tiles(const vcphip_ctx* c, const std::vector<VcphipResidProblem>& probs) {
+  std::vector<int> order(probs.size());
+  for (size_t i = 0; i < probs.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return probs[a].n > probs[b].n; });
+  TileSet s;
+  for (int pi : order) add_tiles(s, pi, probs[pi].n, probs[pi].n, probs[pi].n, batch_mode(c, TS_RESID, probs[pi].n));
+  return s;
+}
+
+// Shape choice per size class of a batched set, by measurement: the classes
+// in order of their work, each tried with the three shapes while the others
+// keep their current choice, the set timed whole (its tile launches only), so
+// that a shape is judged by what it does to the set's time.  time_set(ms)
+// builds and times the set under the current table.
+template <class F>
+int tune_shapes(vcphip_ctx* c, int kind, const std::vector<int>& rows, const std::vector<double>& work, F time_set,
+                double* best_ms) {
+  double cw[32] = {};
+  for (size_t i = 0; i < rows.size(); ++i) cw[size_class(rows[i])] += work[i];
+  std::vector<int> classes;
+  for (int l = 0; l < 32; ++l)
+    if (cw[l] > 0.0) classes.push_back(l);
+  std::stable_sort(classes.begin(), classes.end(), [&](int a, int b) { return cw[a] > cw[b]; });
+  const long long forced = c->batch_tiles;
+  c->batch_tiles = 0;
+  for (int l : classes)
+    if (!c->tshape[kind][l]) c->tshape[kind][l] = VCPHIP_BATCH_TILES_64;
+  double best = 0.0;
+  int rc = time_set(&best);
+  for (size_t ci = 0; ci < classes.size() && rc == VCPHIP_OK; ++ci) {
+    const int l = classes[ci];
+    const int keep = c->tshape[kind][l];
+    int choice = keep;
+    for (int mode = VCPHIP_BATCH_TILES_64; mode <= VCPHIP_BATCH_TILES_32 && rc == VCPHIP_OK; ++mode) {
+      if (mode == keep) continue;
+      c->tshape[kind][l] = (unsigned char)mode;
+      double ms = 0.0;
+      rc = time_set(&ms);
+      if (rc == VCPHIP_OK && ms < best) {
+        best = ms;
+        choice = mode;
+      }
+    }
+    c->tshape[kind][l] = (unsigned char)choice;
+  }
+  c->batch_tiles = forced;
+  if (best_ms) *best_ms = best;
+  return rc;
+}
+
+// time of `reps` back-to-back runs of enqueue() on the context stream, per run
+template <class F>
+int time_runs(vcphip_ctx* c, F enqueue, int reps, double* ms) {
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return VCPHIP_ERR_DEVICE;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    hipEventDestroy(e0);
+    return VCPHIP_ERR_DEVICE;
+  }
+  int rc = enqueue() ? VCPHIP_ERR_DEVICE : VCPHIP_OK;   // warm-up
+  if (rc == VCPHIP_OK && hipEventRecord(e0, c->stream) != hipSuccess) rc = VCPHIP_ERR_DEVICE;
+  for (int r = 0; r < reps && rc == VCPHIP_OK; ++r)
+    if (enqueue()) rc = VCPHIP_ERR_DEVICE;
+  if (rc == VCPHIP_OK && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+    rc = VCPHIP_ERR_DEVICE;
+  float t = 0.f;
+  if (rc == VCPHIP_OK) hipEventElapsedTime(&t, e0, e1);
+  if (rc) hipStreamSynchronize(c->stream);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  *ms = (double)t / reps;
+  return rc;
+}
+
+}  // namespace
+
 // a captured graph is kept only when it holds exactly the `expect` kernel
 // nodes that the capture enqueued: a capture that ends successfully but
 // recorded nothing (as a tool intercepting launches can cause) would replay
@@ -966,19 +1127,13 @@ extern "C" int vcphip_igemm_batched_dev(vcphip_ctx* c, int count, const int* m, 
   bool same = g.exec && g.stream == c->stream && g.probs.size() == probs.size();
   for (size_t i = 0; same && i < probs.size(); ++i) same = same_igemm(g.probs[i], probs[i]);
   if (!same) {
-    // tile list, the most work first (longest-processing-time order)
-    std::vector<int> order((size_t)count);
-    for (int i = 0; i < count; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-      return (double)probs[a].m * probs[a].n * probs[a].k > (double)probs[b].m * probs[b].n * probs[b].k;
-    });
-    g.tiles.clear();
-    for (int pi : order) {
-      const int tm = (probs[pi].m + VCPHIP_RESID_TILE - 1) / VCPHIP_RESID_TILE;
-      const int tn = (probs[pi].n + VCPHIP_RESID_TILE - 1) / VCPHIP_RESID_TILE;
-      for (int a = 0; a < tm; ++a)
-        for (int b = 0; b < tn; ++b) g.tiles.push_back(VcphipResidTile{pi, a * VCPHIP_RESID_TILE, b * VCPHIP_RESID_TILE, 0});
-    }
+    // tile lists per shape, the most work first (longest-processing-time order)
+    TileSet ts = igemm_tiles(c, probs);
+    g.n64 = (int)ts.t64.size();
+    g.n32 = (int)ts.t32.size();
+    g.grid = batch_grid(c, ts.t64.size(), 2);
+    g.grid32 = batch_grid(c, ts.t32.size(), 2);
+    g.tiles = tile_array(ts);
     g.probs = probs;
     const size_t bp = sizeof(VcphipIgemmProblem) * count, bt = sizeof(VcphipResidTile) * g.tiles.size();
     const size_t o_t = (bp + 255) & ~(size_t)255;
@@ -997,16 +1152,16 @@ extern "C" int vcphip_igemm_batched_dev(vcphip_ctx* c, int count, const int* m, 
     if (g.exec) { hipGraphExecDestroy(g.exec); g.exec = nullptr; }
     if (g.graph) { hipGraphDestroy(g.graph); g.graph = nullptr; }
     g.stream = c->stream;
-    const int ntiles = (int)g.tiles.size();
-    g.grid = std::max(1, std::min(ntiles, 2 * (c->num_cu > 0 ? c->num_cu : 256)));
     const VcphipResidTile* dt = (const VcphipResidTile*)(b8 + o_t);
     const VcphipIgemmProblem* dp = (const VcphipIgemmProblem*)b8;
+    g.dp = dp;
+    auto launch = [&]() { return vcphip_launch_igemm_tiles(g.n64, g.n32, dt, dp, g.grid, g.grid32, c->stream); };
     bool captured = false;
     if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-      const int lrc = vcphip_launch_igemm_tiles(ntiles, dt, dp, g.grid, c->stream);
+      const int lrc = launch();
       hipGraph_t gr = nullptr;
       if (hipStreamEndCapture(c->stream, &gr) == hipSuccess && lrc == 0 && gr &&
-          graph_has_kernels(gr, ntiles > 0 ? 1 : 0)) {
+          graph_has_kernels(gr, (size_t)vcphip_igemm_tile_kernels(g.n64, g.n32))) {
         if (hipGraphInstantiate(&g.exec, gr, nullptr, nullptr, 0) == hipSuccess) {
           g.graph = gr;
           captured = true;
@@ -1021,7 +1176,7 @@ extern "C" int vcphip_igemm_batched_dev(vcphip_ctx* c, int count, const int* m, 
       hipGetLastError();
       g.exec = nullptr;
       StageTimer T(c);
-      const int lrc = vcphip_launch_igemm_tiles(ntiles, dt, dp, g.grid, c->stream);
+      const int lrc = launch();
       T.mark("igemm_batched");
       g.probs.clear();
       return check_launch(lrc);
@@ -1059,16 +1214,12 @@ extern "C" int vcphip_residual_batched_dev(vcphip_ctx* c, int count, const int* 
   for (size_t i = 0; same && i < probs.size(); ++i) same = same_problem(rs.probs[i], probs[i]);
   if (!same) {
     // tile list, largest problems first (longest-processing-time order)
-    std::vector<int> order((size_t)count);
-    for (int i = 0; i < count; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return probs[a].n > probs[b].n; });
-    rs.tiles.clear();
-    for (int pi : order) {
-      const int nt = (probs[pi].n + VCPHIP_RESID_TILE - 1) / VCPHIP_RESID_TILE;
-      for (int a = 0; a < nt; ++a)
-        for (int b = 0; b < nt; ++b)
-          rs.tiles.push_back(VcphipResidTile{pi, a * VCPHIP_RESID_TILE, b * VCPHIP_RESID_TILE, 0});
-    }
+    TileSet ts = resid_tiles(c, probs);
+    rs.n64 = (int)ts.t64.size();
+    rs.n32 = (int)ts.t32.size();
+    rs.grid = batch_grid(c, ts.t64.size(), c->resid_per_cu);
+    rs.grid32 = batch_grid(c, ts.t32.size(), c->resid_per_cu);
+    rs.tiles = tile_array(ts);
     rs.row_problem.assign((size_t)nrows, 0);
     for (int i = 0; i < count; ++i)
       for (int r = 0; r < probs[i].n; ++r) rs.row_problem[(size_t)probs[i].row_base + r] = i;
@@ -1089,9 +1240,7 @@ extern "C" int vcphip_residual_batched_dev(vcphip_ctx* c, int count, const int* 
     if (rs.graph) { hipGraphDestroy(rs.graph); rs.graph = nullptr; }
     rs.normub = normub_dev;
     rs.stream = c->stream;
-    const int ntiles = (int)rs.tiles.size();
-    rs.grid = std::max(1, std::min(ntiles, c->resid_per_cu * (c->num_cu > 0 ? c->num_cu : 256)));
-    // capture the persistent launch and the norm kernels once; replays reuse it
+    // capture the persistent launches and the norm kernels once; replays reuse them
     const VcphipResidTile* dt = (const VcphipResidTile*)(b8 + o_t);
     const VcphipResidProblem* dp = (const VcphipResidProblem*)b8;
     const int* drp = (const int*)(b8 + o_r);
@@ -1103,10 +1252,11 @@ extern "C" int vcphip_residual_batched_dev(vcphip_ctx* c, int count, const int* 
     rs.nrows = nrows;
     bool captured = false;
     if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-      const int lrc = vcphip_launch_resid(ntiles, dt, dp, nrows, drp, drs, count, normub_dev, rs.grid, c->stream);
+      const int lrc = vcphip_launch_resid(rs.n64, rs.n32, dt, dp, nrows, drp, drs, count, normub_dev, rs.grid,
+                                          rs.grid32, c->stream);
       hipGraph_t g = nullptr;
       if (hipStreamEndCapture(c->stream, &g) == hipSuccess && lrc == 0 && g &&
-          graph_has_kernels(g, (size_t)vcphip_resid_kernels(ntiles, nrows, count))) {
+          graph_has_kernels(g, (size_t)vcphip_resid_kernels(rs.n64, rs.n32, nrows, count))) {
         if (hipGraphInstantiate(&rs.exec, g, nullptr, nullptr, 0) == hipSuccess) {
           rs.graph = g;
           captured = true;
@@ -1123,7 +1273,8 @@ extern "C" int vcphip_residual_batched_dev(vcphip_ctx* c, int count, const int* 
       // a stream that cannot be captured (e.g. the legacy null stream) runs
       // the same kernels as direct launches
       StageTimer T(c);
-      const int lrc = vcphip_launch_resid(ntiles, dt, dp, nrows, drp, drs, count, normub_dev, rs.grid, c->stream);
+      const int lrc = vcphip_launch_resid(rs.n64, rs.n32, dt, dp, nrows, drp, drs, count, normub_dev, rs.grid,
+                                          rs.grid32, c->stream);
       T.mark("resid");
       rs.probs.clear();   // nothing cached to replay
       return check_launch(lrc);
@@ -2680,55 +2831,65 @@ extern "C" int vcphip_tune_igemm_moduli_dev(vcphip_ctx* c, int m, int n, int k, 
   return VCPHIP_OK;
 }
 
+namespace {
+
+// padded fraction of a tile set's multiply-adds
+double padded_fraction(const TileSet& s) { return s.tile_work > 0.0 ? 1.0 - s.useful / s.tile_work : 0.0; }
+
+}  // namespace
+
 extern "C" int vcphip_tune_residual_dev(vcphip_ctx* c, int count, const int* n, const double* const* R,
                                         const int* ldr, const double* const* IA, const int* lda, double* const* C,
-                                        const int* ldc, double* normub_dev, int* per_cu, double* ms) {
+                                        const int* ldc, double* normub_dev, int* per_cu, double* ms, double* padded) {
   if (!c) return VCPHIP_ERR_ARGUMENT;
   if (count <= 0) return VCPHIP_ERR_ARGUMENT;
   // build (or reuse) the descriptors of this problem set
   int rc = vcphip_residual_batched_dev(c, count, n, R, ldr, IA, lda, C, ldc, normub_dev);
   if (rc) return rc;
   vcphip_ctx::Resid& rs = c->resid;
-  if (rs.probs.empty() || !rs.dt) return VCPHIP_ERR_UNSUPPORTED;
-  const int ntiles = (int)rs.tiles.size();
-  const int cu = c->num_cu > 0 ? c->num_cu : 256;
-  hipEvent_t e0, e1;
-  if (hipEventCreate(&e0) != hipSuccess) return VCPHIP_ERR_DEVICE;
-  if (hipEventCreate(&e1) != hipSuccess) {
-    hipEventDestroy(e0);
-    return VCPHIP_ERR_DEVICE;
+  if (rs.probs.empty() || !rs.dp) return VCPHIP_ERR_UNSUPPORTED;
+  const std::vector<VcphipResidProblem> probs = rs.probs;
+  const VcphipResidProblem* dp = rs.dp;
+  // the tile launches of the set under the current shape table and a grid of
+  // `per` workgroups per CU, from a tile array of the tuner's own
+  int per = c->resid_per_cu;
+  auto time_set = [&](double* t) -> int {
+    TileSet ts = resid_tiles(c, probs);
+    const int n64 = (int)ts.t64.size(), n32 = (int)ts.t32.size();
+    const int g64 = batch_grid(c, ts.t64.size(), per), g32 = batch_grid(c, ts.t32.size(), per);
+    const std::vector<VcphipResidTile> tiles = tile_array(ts);
+    void* base;
+    int wrc = ws_get(c, vcphip_ctx::WS_TT, sizeof(VcphipResidTile) * tiles.size(), &base);
+    if (wrc) return wrc;
+    const VcphipResidTile* dt = (const VcphipResidTile*)base;
+    if (!tiles.empty() && hipMemcpyAsync(base, tiles.data(), sizeof(VcphipResidTile) * tiles.size(),
+                                         hipMemcpyHostToDevice, c->stream) != hipSuccess)
+      return VCPHIP_ERR_DEVICE;
+    // time_runs synchronises the stream before `tiles` goes out of scope
+    return time_runs(c, [&]() { return vcphip_launch_resid_tiles(n64, n32, dt, dp, g64, g32, c->stream); }, 5, t);
+  };
+  std::vector<int> rows((size_t)count);
+  std::vector<double> work((size_t)count);
+  for (int i = 0; i < count; ++i) {
+    rows[i] = n[i];
+    work[i] = (double)n[i] * n[i] * n[i];
   }
-  int best = 0;
   double best_t = 0.0;
-  for (int per = 1; per <= 4 && rc == VCPHIP_OK; ++per) {
-    const int grid = std::max(1, std::min(ntiles, per * cu));
-    auto launch = [&]() {
-      return vcphip_launch_resid(ntiles, rs.dt, rs.dp, rs.nrows, rs.drp, rs.drs, count, normub_dev, grid, c->stream);
-    };
-    if (launch() != 0) {
-      rc = VCPHIP_ERR_DEVICE;
-      break;
-    }
-    hipEventRecord(e0, c->stream);
-    const int reps = 5;
-    for (int r = 0; r < reps; ++r) launch();
-    hipEventRecord(e1, c->stream);
-    if (hipEventSynchronize(e1) != hipSuccess) {
-      rc = VCPHIP_ERR_DEVICE;
-      break;
-    }
-    float t = 0.f;
-    hipEventElapsedTime(&t, e0, e1);
-    if (!best || t / reps < best_t) {
+  rc = tune_shapes(c, TS_RESID, rows, work, time_set, &best_t);
+  // then the persistent grid with the chosen shapes
+  int best = c->resid_per_cu;
+  for (per = 1; per <= 4 && rc == VCPHIP_OK; ++per) {
+    if (per == c->resid_per_cu) continue;
+    double t = 0.0;
+    rc = time_set(&t);
+    if (rc == VCPHIP_OK && t < best_t) {
       best = per;
-      best_t = t / reps;
+      best_t = t;
     }
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
   if (rc) return rc;
   c->resid_per_cu = best;
-  // drop the cached graph: the next call captures the launch with the tuned grid
+  // drop the cached graph: the next call captures the launches with the tuned choices
   if (rs.exec) {
     hipGraphExecDestroy(rs.exec);
     rs.exec = nullptr;
@@ -2740,6 +2901,56 @@ extern "C" int vcphip_tune_residual_dev(vcphip_ctx* c, int count, const int* n, 
   rs.probs.clear();
   if (per_cu) *per_cu = best;
   if (ms) *ms = best_t;
+  if (padded) *padded = padded_fraction(resid_tiles(c, probs));
+  return VCPHIP_OK;
+}
+
+extern "C" int vcphip_tune_igemm_batched_dev(vcphip_ctx* c, int count, const int* m, const int* n, const int* k,
+                                             const double* const* IA, const int* lda, const double* const* IB,
+                                             const int* ldb, double* const* C, const int* ldc, double* ms,
+                                             double* padded) {
+  if (!c) return VCPHIP_ERR_ARGUMENT;
+  if (count <= 0) return VCPHIP_ERR_ARGUMENT;
+  int rc = vcphip_igemm_batched_dev(c, count, m, n, k, IA, lda, IB, ldb, C, ldc);
+  if (rc) return rc;
+  vcphip_ctx::Igb& g = c->igb;
+  if (g.probs.empty() || !g.dp) return VCPHIP_ERR_UNSUPPORTED;
+  const std::vector<VcphipIgemmProblem> probs = g.probs;
+  const VcphipIgemmProblem* dp = g.dp;
+  auto time_set = [&](double* t) -> int {
+    TileSet ts = igemm_tiles(c, probs);
+    const int n64 = (int)ts.t64.size(), n32 = (int)ts.t32.size();
+    const int g64 = batch_grid(c, ts.t64.size(), 2), g32 = batch_grid(c, ts.t32.size(), 2);
+    const std::vector<VcphipResidTile> tiles = tile_array(ts);
+    void* base;
+    int wrc = ws_get(c, vcphip_ctx::WS_TT, sizeof(VcphipResidTile) * tiles.size(), &base);
+    if (wrc) return wrc;
+    const VcphipResidTile* dt = (const VcphipResidTile*)base;
+    if (!tiles.empty() && hipMemcpyAsync(base, tiles.data(), sizeof(VcphipResidTile) * tiles.size(),
+                                         hipMemcpyHostToDevice, c->stream) != hipSuccess)
+      return VCPHIP_ERR_DEVICE;
+    return time_runs(c, [&]() { return vcphip_launch_igemm_tiles(n64, n32, dt, dp, g64, g32, c->stream); }, 5, t);
+  };
+  std::vector<int> rows((size_t)count);
+  std::vector<double> work((size_t)count);
+  for (int i = 0; i < count; ++i) {
+    rows[i] = m[i];
+    work[i] = (double)m[i] * n[i] * k[i];
+  }
+  double best_t = 0.0;
+  rc = tune_shapes(c, TS_IGEMM, rows, work, time_set, &best_t);
+  if (rc) return rc;
+  if (g.exec) {
+    hipGraphExecDestroy(g.exec);
+    g.exec = nullptr;
+  }
+  if (g.graph) {
+    hipGraphDestroy(g.graph);
+    g.graph = nullptr;
+  }
+  g.probs.clear();
+  if (ms) *ms = best_t;
+  if (padded) *padded = padded_fraction(igemm_tiles(c, probs));
   return VCPHIP_OK;
 }
 
@@ -2752,6 +2963,9 @@ extern "C" int vcphip_tune_save(vcphip_ctx* c, const char* path) {
     fprintf(f, "%s %d %d %d %d %d %d\n", t.kind == TUNE_IGEMM ? "igemm" : "ddgemm", t.lm, t.ln, t.lk, t.variant,
             t.nmod, t.radius);
   fprintf(f, "residual %d\n", c->resid_per_cu);
+  for (int kind = 0; kind < 2; ++kind)
+    for (int l = 0; l < 32; ++l)
+      if (c->tshape[kind][l]) fprintf(f, "tiles %s %d %d\n", kind == TS_RESID ? "residual" : "igemm", l, c->tshape[kind][l]);
   const bool ok = fclose(f) == 0;
   return ok ? VCPHIP_OK : VCPHIP_ERR_ARGUMENT;
 }
@@ -2767,6 +2981,10 @@ extern "C" int vcphip_tune_load(vcphip_ctx* c, const char* path) {
     return VCPHIP_ERR_ARGUMENT;
   }
   std::vector<vcphip_ctx::Tune> table;
+  struct Shape {
+    int kind, cls, mode;
+  };
+  std::vector<Shape> shapes;
   int per_cu = c->resid_per_cu;
   int rc = VCPHIP_OK;
   while (fgets(line, sizeof line, f)) {
@@ -2786,6 +3004,13 @@ extern "C" int vcphip_tune_load(vcphip_ctx* c, const char* path) {
       table.push_back(vcphip_ctx::Tune{iv ? TUNE_IGEMM : TUNE_DDGEMM, lm, ln, lk, v, nm, rad});
     } else if (sscanf(line, "residual %d", &p) == 1 && p >= 1 && p <= 4) {
       per_cu = p;
+    } else if (sscanf(line, "tiles %31s %d %d", kind, &lm, &v) == 3) {
+      const int tk = strcmp(kind, "residual") == 0 ? TS_RESID : strcmp(kind, "igemm") == 0 ? TS_IGEMM : -1;
+      if (tk < 0 || lm < 0 || lm > 31 || v < VCPHIP_BATCH_TILES_64 || v > VCPHIP_BATCH_TILES_32) {
+        rc = VCPHIP_ERR_ARGUMENT;
+        break;
+      }
+      shapes.push_back({tk, lm, v});
     } else if (line[0] != '\n' && line[0] != '#') {
       rc = VCPHIP_ERR_ARGUMENT;
       break;
@@ -2795,5 +3020,10 @@ extern "C" int vcphip_tune_load(vcphip_ctx* c, const char* path) {
   if (rc) return rc;
   for (const auto& t : table) tune_put(c, t.kind, t.lm, t.ln, t.lk, t.variant, t.nmod, t.radius);
   c->resid_per_cu = per_cu;
+  for (const Shape& s : shapes) c->tshape[s.kind][s.cls] = (unsigned char)s.mode;
+  if (!shapes.empty()) {   // cached tile lists are rebuilt on the next call
+    c->resid.probs.clear();
+    c->igb.probs.clear();
+  }
   return VCPHIP_OK;
 }

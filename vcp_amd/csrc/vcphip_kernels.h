@@ -28,12 +28,15 @@ struct VcphipResidProblem {
   int n;
   int row_base;      // first global row of this problem in the row-sum buffer
 };
+// one output tile of a batched persistent launch: problem, first row and
+// column.  A launch's tile array holds its 64 x 64 tiles first, then its
+// 32 x 64 ones (rows x columns); each shape runs as a persistent launch of
+// its own, 64 x 64 first.
 struct VcphipResidTile {
   int problem, row0, col0, pad;
 };
 #define VCPHIP_RESID_TILE 64
-// batched interval products C_p = IA_p * IB_p (the plain form), tiles of
-// VCPHIP_RESID_TILE x VCPHIP_RESID_TILE
+// batched interval products C_p = IA_p * IB_p (the plain form)
 struct VcphipIgemmProblem {
   const double* A;   // interval m x k, AoS {inf, sup}, column-major
   const double* B;   // interval k x n
@@ -41,15 +44,22 @@ struct VcphipIgemmProblem {
   long lda, ldb, ldc;
   int m, n, k, pad;
 };
-int vcphip_launch_igemm_tiles(int ntiles, const VcphipResidTile* tiles, const VcphipIgemmProblem* probs, int grid,
-                              hipStream_t s);
-int vcphip_launch_resid(int ntiles, const VcphipResidTile* tiles, const VcphipResidProblem* probs, int nrows,
-                        const int* row_problem, double* rowsum, int count, double* normub, int grid,
+// tiles[0, n64) are 64 x 64 tiles, tiles[n64, n64 + n32) 32 x 64 ones; each
+// shape's persistent grid strides over its own part
+int vcphip_launch_igemm_tiles(int n64, int n32, const VcphipResidTile* tiles, const VcphipIgemmProblem* probs,
+                              int grid64, int grid32, hipStream_t s);
+int vcphip_launch_resid(int n64, int n32, const VcphipResidTile* tiles, const VcphipResidProblem* probs, int nrows,
+                        const int* row_problem, double* rowsum, int count, double* normub, int grid64, int grid32,
                         hipStream_t s);
-// the number of kernels vcphip_launch_resid enqueues for these sizes
-inline int vcphip_resid_kernels(int ntiles, int nrows, int count) {
-  return (ntiles > 0) + (nrows > 0) + (count > 0);
+// the tile launches alone (the tuner times them)
+int vcphip_launch_resid_tiles(int n64, int n32, const VcphipResidTile* tiles, const VcphipResidProblem* probs,
+                              int grid64, int grid32, hipStream_t s);
+// the number of kernels vcphip_launch_resid / vcphip_launch_igemm_tiles
+// enqueue for these sizes
+inline int vcphip_resid_kernels(int n64, int n32, int nrows, int count) {
+  return (n64 > 0) + (n32 > 0) + (nrows > 0) + (count > 0);
 }
+inline int vcphip_igemm_tile_kernels(int n64, int n32) { return (n64 > 0) + (n32 > 0); }
 // p[0..n) = v by a kernel
 int vcphip_fill_f64(double* p, long n, double v, hipStream_t s);
 
