@@ -800,7 +800,12 @@ def bench_dd(ctx, n, steps, with_cpu):
     def step():
         ctx.ddgemm_dev(n, n, n, dA.data_ptr(), n, dB.data_ptr(), n, dC.data_ptr(), n)
 
-    step()
+    # the int8 GEMM tile variant for this shape (autotuner; bit-identical choices)
+    t0 = time.perf_counter()
+    variant, _ = ctx.tune_ddgemm_tile(n, n, n)
+    tune_s = time.perf_counter() - t0
+    for _ in range(2):
+        step()
     ctx.synchronize()
     ctx.set_param(vcp_amd.PARAM_STAGE_TIMING, 1)
     t0 = time.perf_counter()
@@ -813,7 +818,8 @@ def bench_dd(ctx, n, steps, with_cpu):
     nmod = ctx.get_param(vcp_amd.PARAM_DD_NMOD) or None
     out = {"n": n, "ms_per_step": round(t * 1e3, 3), "dd_tflops": round(2.0 * n ** 3 / t / 1e12, 3),
            "stages_ms": {k: round(v, 4) for k, v in stages.items()},
-           "moduli": nmod if nmod else "default: fewest giving beta >= 110"}
+           "moduli": nmod if nmod else "default: fewest giving beta >= 110",
+           "tile_variant": variant, "configuration": "autotuned tile variant (%.2f s)" % tune_s}
     log("dd gemm %s" % out)
     try:
         import oracle_ffi as O

@@ -407,7 +407,7 @@ int vcphip_tune_igemm_moduli_dev(vcphip_ctx* ctx, int m, int n, int k, const dou
  * vcphip_residual_batched_dev, which it also runs): first the tile shape of
  * each size class of the set (VCPHIP_BATCH_TILES_*; classes are ceil(log2 n),
  * taken in order of their work, each tried with the three shapes while the
- * others keep theirs, the whole set's tile launches timed), then 1-4
+ * others keep theirs, the whole set's tile launches timed), then 1-3
  * persistent workgroups per CU.  Keeps the fastest; the next call captures
  * its graph with them.  *per_cu, *ms (the set's tile launches) and *padded
  * (the fraction of the tiles' multiply-adds that is padding, under the

@@ -133,7 +133,7 @@ def test_tune_residual_then_replay_bitexact(ctx):
             [t.data_ptr() for t in dC], sizes, nub.data_ptr())
     per, ms, padded = vcp_amd.tune_residual_dev(ctx, *args)
     print("tuned residual: %d workgroups per CU, %.3f ms, padded fraction %.4f" % (per, ms, padded), flush=True)
-    assert 1 <= per <= 4 and ms > 0.0 and 0.0 <= padded < 0.6
+    assert 1 <= per <= 3 and ms > 0.0 and 0.0 <= padded < 0.6
     for _ in range(3):   # first call captures with the tuned grid, then replays
         for t in dC:
             t.fill_(np.nan)

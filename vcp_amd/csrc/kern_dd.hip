@@ -50,6 +50,39 @@ __global__ __launch_bounds__(256) void k_dd_stats_final(int rows, int nchunk, co
   flags[r] = best == INT32_MIN ? 1 : (best == INT32_MAX ? 2 : 0);
 }
 
+// the same statistics for lines whose elements are contiguous (columns of B,
+// rows of a transposed A): one wave per line, its lanes over the elements
+// (coalesced), the maximum taken across the wave (it does not depend on the
+// order) and written directly.  fexp is monotone, so the exponent of the
+// maximum equals the maximum of the chunk exponents above.
+__global__ __launch_bounds__(256) void k_dd_stats_lines(const double* __restrict__ X, long rs, int rows, int len,
+                                                        int* __restrict__ e, int* __restrict__ flags) {
+  vcphip_mode_round_up();
+  const int lane = threadIdx.x & 63;
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= rows) return;   // uniform per wave
+  const double* x = X + 2 * (long)r * rs;
+  double mu = 0.0;
+  int nonfinite = 0;
+  for (int cc = lane; cc < len; cc += 64) {
+    const double2 v = *reinterpret_cast<const double2*>(x + 2 * (long)cc);
+    nonfinite |= !(__builtin_isfinite(v.x) && __builtin_isfinite(v.y));
+    const double a = __builtin_fabs(v.x) + __builtin_fabs(v.y);
+    mu = a > mu ? a : mu;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(mu, off);
+    mu = o > mu ? o : mu;
+    nonfinite |= __shfl_xor(nonfinite, off);
+  }
+  if (lane == 0) {
+    // flags: 1 = zero line, 2 = a non-finite element
+    e[r] = (nonfinite || !(mu > 0.0)) ? 0 : vcphip_fexp(mu);
+    flags[r] = nonfinite ? 2 : (mu > 0.0 ? 0 : 1);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // split: 128-bit scaled integers -> symmetric int8 residues
 // ---------------------------------------------------------------------------
@@ -716,6 +749,10 @@ int vcphip_dd_add_d(int m, int n, const double* d, double* X, long ldx, double* 
 int vcphip_dd_stats(const double* X, long rs, long cs, int rows, int len, int* e, int* flags, int* part,
                     hipStream_t s) {
   if (rows <= 0) return 0;
+  if (cs == 1) {   // contiguous lines: one wave each
+    hipLaunchKernelGGL(k_dd_stats_lines, dim3((rows + 3) / 4), dim3(256), 0, s, X, rs, rows, len, e, flags);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+  }
   const int nchunk = len > 0 ? (len + VCPHIP_OZ_CHUNK - 1) / VCPHIP_OZ_CHUNK : 0;
   if (nchunk > 0)
     hipLaunchKernelGGL(k_dd_stats_part, dim3((rows + 255) / 256, nchunk), dim3(256), 0, s, X, rs, cs, rows, len,

@@ -158,7 +158,12 @@ int check_launch(int rc) { return rc == 0 ? VCPHIP_OK : (rc == -7 ? VCPHIP_ERR_U
 
 // autotuner table
 enum { TUNE_IGEMM = 1, TUNE_DDGEMM = 2 };
-constexpr int DEFAULT_VARIANT = 8;   // 256 x 256 tile, direct-to-LDS, 4 stages (fastest measured at n = 8192)
+// untuned int8 GEMM tile: 256 x 256, eight waves of 128 x 64, 4 stages.  At
+// n = 8192 it ties the four-wave variant 8 (10.7 ms per 22 plane products);
+// at n = 4096, with one 256 x 256 tile per CU and plane, its second wave per
+// SIMD hides what the four-wave tile exposes (interval 1.38 against 1.67 ms,
+// dd 2.03 against 2.10 ms)
+constexpr int DEFAULT_VARIANT = 15;
 
 int size_class(int x) {
   int l = 0;
@@ -2876,9 +2881,13 @@ extern "C" int vcphip_tune_residual_dev(vcphip_ctx* c, int count, const int* n, 
   }
   double best_t = 0.0;
   rc = tune_shapes(c, TS_RESID, rows, work, time_set, &best_t);
-  // then the persistent grid with the chosen shapes
+  // then the persistent grid with the chosen shapes: 1-3 workgroups per CU,
+  // as many as are co-resident at the tile kernels' three waves per SIMD (a
+  // fourth only queues behind them; alone that measured within noise of
+  // three, and inside the batched solve it took CUs from the right-hand-side
+  // kernels running beside it: 6.3 against 5.55 ms)
   int best = c->resid_per_cu;
-  for (per = 1; per <= 4 && rc == VCPHIP_OK; ++per) {
+  for (per = 1; per <= 3 && rc == VCPHIP_OK; ++per) {
     if (per == c->resid_per_cu) continue;
     double t = 0.0;
     rc = time_set(&t);
