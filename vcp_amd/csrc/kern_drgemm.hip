@@ -14,7 +14,10 @@
 //   r2  = RU chain of rad A*|mid B|                       (:281)
 // and the epilogue applies the reference's fix-ups in its order
 //   t = RU(r1 + r2); upper = RU(up + t); lower = RD(-dn - t) = -RU(dn + t).
-// Interval operands are read as AoS {inf, sup} and converted to
+// Two implementations share these chains: k_drgemm_planes (the single
+// products), fed from midpoint-radius planes made once per operand, and the
+// tile function dr_tile (k_drgemm for the thin products, the batched
+// launches), which reads the AoS {inf, sup} operands and converts them to
 // midpoint-radius on their way into LDS (pidblas_assist::midrad, :43-60).
 //
 // The whole kernel runs in round-toward-+inf (see vcphip_device.h).
@@ -299,6 +302,217 @@ __global__ __launch_bounds__(NT, (AI && BI) ? 2 : 3) void k_drgemm(int m, int n,
   vcphip_tile_coords(tile, ntm, ntn, 16, tm, tn);
   __shared__ __attribute__((aligned(16))) DrLds<AI, BI> lds;
   dr_tile<AI, BI, false>(lds, m, n, k, A, lda, B, ldb, C, ldc, C2, accumulate, tm * BM, tn * BN);
+}
+
+// ---------------------------------------------------------------------------
+// The single products from operand planes.
+//
+// pidblas_assist::midrad (:43-60) is taken once per operand element by two
+// small kernels instead of once per element and tile inside the product, and
+// both operands are laid out for whole, contiguous k-slices:
+//   A (m x k, column-major) -> mid(A), rad(A) (interval) or A (point),
+//                              mpad x kpad, column-major (ld mpad)
+//   B (k x n, column-major) -> mid(B), rad(B) or B, kpad x npad, row-major
+//                              (ld npad)
+// zero beyond m, n and k (mpad, npad multiples of 64, kpad of 16).  Every
+// k-slice of a tile (16 k x 64 of each plane) then lies in 16 runs of 512
+// contiguous bytes, which the product loads straight into LDS with no bounds
+// tests, no conversions and no register staging.  The chains and their order
+// are those of dr_tile, so are the results' bits (padding adds the same zero
+// terms the checked loads of dr_tile add past k).
+// ---------------------------------------------------------------------------
+template <bool AI>
+__global__ __launch_bounds__(256) void k_planes_a(int m, int k, int mpad, const double* __restrict__ A, long lda,
+                                                  double* __restrict__ P0, double* __restrict__ P1) {
+  vcphip_mode_round_up();
+  const int r = blockIdx.x * 256 + threadIdx.x, kk = blockIdx.y;
+  if (r >= mpad) return;
+  const long o = (long)kk * mpad + r;
+  const bool in = r < m && kk < k;
+  if (AI) {
+    double mm = 0.0, rr = 0.0;
+    if (in) {
+      const double2 v = *reinterpret_cast<const double2*>(A + 2 * (r + lda * (long)kk));
+      vcphip_midrad_ru(v.x, v.y, mm, rr);
+    }
+    P0[o] = mm;
+    P1[o] = rr;
+  } else {
+    P0[o] = in ? A[r + lda * (long)kk] : 0.0;
+  }
+}
+
+// B through a 32 x 32 LDS tile: read along k (B's columns), write along n
+template <bool BI>
+__global__ __launch_bounds__(256) void k_planes_bt(int k, int n, int kpad, int npad, const double* __restrict__ B,
+                                                   long ldb, double* __restrict__ P0, double* __restrict__ P1) {
+  vcphip_mode_round_up();
+  __shared__ double2 s[32][33];
+  const int k0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int kk = k0 + tx, j = j0 + ty + 8 * i;
+    double2 v = make_double2(0.0, 0.0);
+    if (kk < k && j < n) {
+      if (BI) v = *reinterpret_cast<const double2*>(B + 2 * (kk + ldb * (long)j));
+      else v.x = B[kk + ldb * (long)j];
+    }
+    s[ty + 8 * i][tx] = v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int kk = k0 + ty + 8 * i, j = j0 + tx;
+    if (kk >= kpad) continue;   // the last block of 32 k may reach past kpad
+    const double2 v = s[tx][ty + 8 * i];
+    const long o = (long)kk * npad + j;
+    if (BI) {
+      double mm, rr;
+      vcphip_midrad_ru(v.x, v.y, mm, rr);   // (0, 0) for the padding
+      P0[o] = mm;
+      P1[o] = rr;
+    } else {
+      P0[o] = v.x;
+    }
+  }
+}
+
+// interval x interval tiles hold four chains per output (2 waves per SIMD),
+// point x point two (4), the mixed forms three (3)
+template <bool AI, bool BI>
+__global__ __launch_bounds__(NT, (AI && BI) ? 2 : (AI || BI) ? 3 : 4) void k_drgemm_planes(
+    int m, int n, int mpad, int npad, int kpad, const double* __restrict__ A0, const double* __restrict__ A1,
+    const double* __restrict__ B0, const double* __restrict__ B1, double* __restrict__ C, long ldc,
+    double* __restrict__ C2, int accumulate) {
+  constexpr bool II = AI && BI;
+  constexpr int NA = AI ? 2 : 1, NP = NA + (BI ? 2 : 1);
+  vcphip_mode_round_up();
+  // [stage][plane: A mid (or A), A rad, B mid (or B), B rad][k][64]
+  __shared__ __attribute__((aligned(16))) double sL[2][NP][BK][BM];
+  static_assert(BM == BN && BM == 64 && BK == 16, "slice layout");
+  const int ntm = mpad / BM, ntn = npad / BN;
+  const int tile = vcphip_xcd_remap(blockIdx.x, ntm * ntn);
+  int tm, tn;
+  vcphip_tile_coords(tile, ntm, ntn, 16, tm, tn);
+  const int row0 = tm * BM, col0 = tn * BN;
+  const int tid = threadIdx.x, lane = tid & 63;
+  // wave w < NP loads plane w of every slice: 8 loads of two k-rows
+  // (2 x 512 B); lane l takes k-row (l >> 5) of the pair, doubles 2 (l & 31)
+  // and + 1
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const bool loader = w < NP;
+  const bool wa = w < NA;
+  const double* plane = wa ? (w == 0 ? A0 : A1) : (w == NA ? B0 : B1);
+  const long ld = wa ? mpad : npad;
+  const double* src = plane + (long)(lane >> 5) * ld + (wa ? row0 : col0) + 2 * (lane & 31);
+  const uint32_t lds0 = vcphip_lds_addr(&sL[0][0][0][0]) + (uint32_t)(loader ? w : 0) * (BK * BM * 8);
+  auto issue = [&](int stage, int t) {
+    if (!loader) return;
+    const double* g = src + (long)t * BK * ld;
+    const uint32_t d = lds0 + (uint32_t)stage * (NP * BK * BM * 8);
+#pragma unroll
+    for (int q = 0; q < BK / 2; ++q) vcphip_lds_dma16(g + (long)(2 * q) * ld, d + (uint32_t)q * (2 * BM * 8));
+  };
+
+  const int tx = tid % 16, ty = tid / 16;
+  double up[TM][TN], dn[TM][TN], r1[TM][TN], r2[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      up[i][j] = 0.0;
+      dn[i][j] = 0.0;
+      r1[i][j] = 0.0;
+      r2[i][j] = 0.0;
+    }
+  if (!AI && !BI && accumulate) {
+    // udmatmul form: the chains continue from the values already in CU / CD
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int r = row0 + ty * TM + i, c = col0 + dr_col(tx, j);
+        if (r < m && c < n) {
+          up[i][j] = C[r + ldc * c];
+          dn[i][j] = -C2[r + ldc * c];
+        }
+      }
+  }
+
+  const int nt = kpad / BK;
+  issue(0, 0);
+  for (int t = 0; t < nt; ++t) {
+    // only slice t's loads are in flight: wait for them, then for every wave
+    // (which also ends every wave's reads of the other stage, slice t - 1)
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    if (t + 1 < nt) issue((t + 1) & 1, t + 1);
+    const double(*sAm)[BM] = sL[t & 1][0];
+    const double(*sAr)[BM] = sL[t & 1][AI ? 1 : 0];
+    const double(*sBm)[BM] = sL[t & 1][NA];
+    const double(*sBr)[BM] = sL[t & 1][BI ? NA + 1 : NA];
+#pragma unroll 2
+    for (int kk = 0; kk < BK; ++kk) {
+      double am[TM], ar[TM], bm[TN], br[TN];
+      {
+        const double2 x0 = *reinterpret_cast<const double2*>(&sAm[kk][ty * TM]);
+        const double2 x1 = *reinterpret_cast<const double2*>(&sAm[kk][ty * TM + 2]);
+        am[0] = x0.x; am[1] = x0.y; am[2] = x1.x; am[3] = x1.y;
+        const double2 y0 = *reinterpret_cast<const double2*>(&sBm[kk][2 * tx]);
+        const double2 y1 = *reinterpret_cast<const double2*>(&sBm[kk][2 * tx + 32]);
+        bm[0] = y0.x; bm[1] = y0.y; bm[2] = y1.x; bm[3] = y1.y;
+      }
+      if (AI) {
+        const double2 z0 = *reinterpret_cast<const double2*>(&sAr[kk][ty * TM]);
+        const double2 z1 = *reinterpret_cast<const double2*>(&sAr[kk][ty * TM + 2]);
+        ar[0] = z0.x; ar[1] = z0.y; ar[2] = z1.x; ar[3] = z1.y;
+      }
+      if (BI) {
+        const double2 u0 = *reinterpret_cast<const double2*>(&sBr[kk][2 * tx]);
+        const double2 u1 = *reinterpret_cast<const double2*>(&sBr[kk][2 * tx + 32]);
+        br[0] = u0.x; br[1] = u0.y; br[2] = u1.x; br[3] = u1.y;
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        double as = 0.0;
+        if (II) as = __builtin_fabs(am[i]) + ar[i];   // (|mA| + rA), RU (:277)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          up[i][j] = __builtin_fma(am[i], bm[j], up[i][j]);
+          dn[i][j] = __builtin_fma(-am[i], bm[j], dn[i][j]);
+          if (II) {
+            r1[i][j] = __builtin_fma(as, br[j], r1[i][j]);                    // (|mA| + rA) rB (:277-279)
+            r2[i][j] = __builtin_fma(ar[i], __builtin_fabs(bm[j]), r2[i][j]);  // rA |mB| (:281)
+          } else if (AI) {
+            r1[i][j] = __builtin_fma(ar[i], __builtin_fabs(bm[j]), r1[i][j]);  // rA*|B|  (:129)
+          } else if (BI) {
+            r1[i][j] = __builtin_fma(__builtin_fabs(am[i]), br[j], r1[i][j]);  // |B|*rA  (:185)
+          }
+        }
+      }
+    }
+  }
+
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int c = col0 + dr_col(tx, j);
+    if (c >= n) continue;
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const int r = row0 + ty * TM + i;
+      if (r >= m) continue;
+      if (AI || BI) {
+        const double tt = II ? r1[i][j] + r2[i][j] : r1[i][j];   // temp (:283, :129, :185)
+        const double hi = up[i][j] + tt;                         // uC += temp, RU (:284)
+        const double nlo = dn[i][j] + tt;                        // lC -= temp, RD = -(RU(-lC + t)) (:285-286)
+        *reinterpret_cast<double2*>(C + 2 * (r + ldc * c)) = make_double2(-nlo, hi);
+      } else {
+        C[r + ldc * c] = up[i][j];
+        C2[r + ldc * c] = -dn[i][j];
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -665,6 +879,35 @@ int vcphip_launch_drgemm(int kind, int m, int n, int k, const double* A, long ld
     case VCPHIP_DR_IP: hipLaunchKernelGGL((k_drgemm<true, false>), grid, block, 0, s, m, n, k, A, lda, B, ldb, C, ldc, C2, accumulate); break;
     case VCPHIP_DR_PI: hipLaunchKernelGGL((k_drgemm<false, true>), grid, block, 0, s, m, n, k, A, lda, B, ldb, C, ldc, C2, accumulate); break;
     case VCPHIP_DR_PP: hipLaunchKernelGGL((k_drgemm<false, false>), grid, block, 0, s, m, n, k, A, lda, B, ldb, C, ldc, C2, accumulate); break;
+    default: return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_launch_planes(int kind, int m, int n, int k, const double* A, long lda, const double* B, long ldb,
+                         double* A0, double* A1, double* B0, double* B1, hipStream_t s) {
+  const int mpad = (m + BM - 1) / BM * BM, npad = (n + BN - 1) / BN * BN, kpad = (k + BK - 1) / BK * BK;
+  if (m <= 0 || n <= 0 || kpad <= 0) return 0;
+  const bool ai = kind == VCPHIP_DR_II || kind == VCPHIP_DR_IP, bi = kind == VCPHIP_DR_II || kind == VCPHIP_DR_PI;
+  const dim3 ga(mpad / 256 + (mpad % 256 != 0), kpad), gb(kpad / 32 + (kpad % 32 != 0), npad / 32);
+  if (ai) hipLaunchKernelGGL(k_planes_a<true>, ga, dim3(256), 0, s, m, k, mpad, A, lda, A0, A1);
+  else hipLaunchKernelGGL(k_planes_a<false>, ga, dim3(256), 0, s, m, k, mpad, A, lda, A0, A1);
+  if (bi) hipLaunchKernelGGL(k_planes_bt<true>, gb, dim3(256), 0, s, k, n, kpad, npad, B, ldb, B0, B1);
+  else hipLaunchKernelGGL(k_planes_bt<false>, gb, dim3(256), 0, s, k, n, kpad, npad, B, ldb, B0, B1);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_launch_drgemm_planes(int kind, int m, int n, int k, const double* A0, const double* A1, const double* B0,
+                                const double* B1, double* C, long ldc, double* C2, int accumulate, hipStream_t s) {
+  const int mpad = (m + BM - 1) / BM * BM, npad = (n + BN - 1) / BN * BN, kpad = (k + BK - 1) / BK * BK;
+  if (m <= 0 || n <= 0) return 0;
+  if (kpad == 0) return -1;   // k = 0 takes the tile path
+  const dim3 grid((mpad / BM) * (npad / BN)), block(NT);
+  switch (kind) {
+    case VCPHIP_DR_II: hipLaunchKernelGGL((k_drgemm_planes<true, true>), grid, block, 0, s, m, n, mpad, npad, kpad, A0, A1, B0, B1, C, ldc, C2, accumulate); break;
+    case VCPHIP_DR_IP: hipLaunchKernelGGL((k_drgemm_planes<true, false>), grid, block, 0, s, m, n, mpad, npad, kpad, A0, A1, B0, B1, C, ldc, C2, accumulate); break;
+    case VCPHIP_DR_PI: hipLaunchKernelGGL((k_drgemm_planes<false, true>), grid, block, 0, s, m, n, mpad, npad, kpad, A0, A1, B0, B1, C, ldc, C2, accumulate); break;
+    case VCPHIP_DR_PP: hipLaunchKernelGGL((k_drgemm_planes<false, false>), grid, block, 0, s, m, n, mpad, npad, kpad, A0, A1, B0, B1, C, ldc, C2, accumulate); break;
     default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -3;

@@ -385,15 +385,6 @@ __global__ __launch_bounds__(256) void k_oz_split_cols(const double* __restrict_
 // which is exactly the v_mfma_i32_16x16x64_i8 accumulator order, so every
 // wave stores whole contiguous blocks.
 // Workgroup: 4 waves as 2 x 2.
-// LDS byte address of a __shared__ object.
-__device__ __forceinline__ uint32_t lds_addr(const void* p) {
-  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
-}
-// One wave-wide 16-byte-per-lane load straight into LDS: lane l's 16 bytes
-// land at lds_base + 16 l (lds_base must be wave-uniform).
-__device__ __forceinline__ void lds_dma16(const int8_t* g, uint32_t lds_base) {
-  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g), "s"(lds_base) : "memory", "m0");
-}
 
 // Variant with direct global->LDS loads (global_load_lds_dwordx4): three LDS
 // stages, two in flight while the MFMAs run on the third.  Rows of a stage are
@@ -427,7 +418,7 @@ __global__ __launch_bounds__(256) void k_oz_gemm3(int mpad, int npad, int kpad, 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w & 1, wn = w >> 1;
   const int wu = __builtin_amdgcn_readfirstlane(w);
-  const uint32_t sA_base = lds_addr(&sA[0][0]), sB_base = lds_addr(&sB[0][0]);
+  const uint32_t sA_base = vcphip_lds_addr(&sA[0][0]), sB_base = vcphip_lds_addr(&sB[0][0]);
   // this lane's source in every 1 KB segment: row lane/4, logical chunk
   const int lrow = lane >> 2, lchunk = (lane & 3) ^ ((lane >> 4) & 3);
   const int src_lane = lrow * 64 + 16 * lchunk;   // inside the 1 KB block of 16 rows x 64 k
@@ -464,12 +455,12 @@ __global__ __launch_bounds__(256) void k_oz_gemm3(int mpad, int npad, int kpad, 
 #pragma unroll
       for (int q = 0; q < SEGA_W; ++q) {
         const int seg = wu * SEGA_W + q;
-        lds_dma16(Ab + ((long)seg * kblk + kt) * 1024, sA_base + stage * (TMB * 64) + seg * 1024);
+        vcphip_lds_dma16(Ab + ((long)seg * kblk + kt) * 1024, sA_base + stage * (TMB * 64) + seg * 1024);
       }
 #pragma unroll
       for (int q = 0; q < SEGB_W; ++q) {
         const int seg = wu * SEGB_W + q;
-        lds_dma16(Bb + ((long)seg * kblk + kt) * 1024, sB_base + stage * (TNB * 64) + seg * 1024);
+        vcphip_lds_dma16(Bb + ((long)seg * kblk + kt) * 1024, sB_base + stage * (TNB * 64) + seg * 1024);
       }
     };
     issue(0, 0);
@@ -584,7 +575,7 @@ __global__ __launch_bounds__(256) void k_oz_gemm4(int mpad, int npad, int kpad, 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w & 1, wn = w >> 1;
   const int wu = __builtin_amdgcn_readfirstlane(w);
-  const uint32_t sA_base = lds_addr(&sA[0][0]), sB_base = lds_addr(&sB[0][0]);
+  const uint32_t sA_base = vcphip_lds_addr(&sA[0][0]), sB_base = vcphip_lds_addr(&sB[0][0]);
   const int lrow = lane >> 2, lchunk = (lane & 3) ^ ((lane >> 4) & 3);
   const int src_lane = lrow * 64 + 16 * lchunk;   // inside the 1 KB block of 16 rows x 64 k
   const long kblk = kpad / 64;
@@ -607,12 +598,12 @@ __global__ __launch_bounds__(256) void k_oz_gemm4(int mpad, int npad, int kpad, 
 #pragma unroll
       for (int q = 0; q < SEGA_W; ++q) {
         const int seg = wu * SEGA_W + q;
-        lds_dma16(Ab + ((long)seg * kblk + kt) * 1024, sA_base + stage * (TMB * 64) + seg * 1024);
+        vcphip_lds_dma16(Ab + ((long)seg * kblk + kt) * 1024, sA_base + stage * (TMB * 64) + seg * 1024);
       }
 #pragma unroll
       for (int q = 0; q < SEGB_W; ++q) {
         const int seg = wu * SEGB_W + q;
-        lds_dma16(Bb + ((long)seg * kblk + kt) * 1024, sB_base + stage * (TNB * 64) + seg * 1024);
+        vcphip_lds_dma16(Bb + ((long)seg * kblk + kt) * 1024, sB_base + stage * (TNB * 64) + seg * 1024);
       }
     };
     // Waits until stage kt+1 (or stage 0 when kt = -1) has landed for every
@@ -626,10 +617,10 @@ __global__ __launch_bounds__(256) void k_oz_gemm4(int mpad, int npad, int kpad, 
     auto issue_one = [&](int stage, int kt, int g) {
       if (g < SEGA_W) {
         const int seg = wu * SEGA_W + g;
-        lds_dma16(Ab + ((long)seg * kblk + kt) * 1024, sA_base + stage * (TMB * 64) + seg * 1024);
+        vcphip_lds_dma16(Ab + ((long)seg * kblk + kt) * 1024, sA_base + stage * (TMB * 64) + seg * 1024);
       } else {
         const int seg = wu * SEGB_W + (g - SEGA_W);
-        lds_dma16(Bb + ((long)seg * kblk + kt) * 1024, sB_base + stage * (TNB * 64) + seg * 1024);
+        vcphip_lds_dma16(Bb + ((long)seg * kblk + kt) * 1024, sB_base + stage * (TNB * 64) + seg * 1024);
       }
     };
     // MFMAs of one stage; with IL the stage's refill loads are spread over
@@ -779,7 +770,7 @@ __global__ __launch_bounds__(64 * NW) void k_oz_gemm5(int mpad, int npad, int kp
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w & 1, wn = w >> 1;
   const int wu = __builtin_amdgcn_readfirstlane(w);
-  const uint32_t sA_base = lds_addr(&sA[0][0]), sB_base = lds_addr(&sB[0][0]);
+  const uint32_t sA_base = vcphip_lds_addr(&sA[0][0]), sB_base = vcphip_lds_addr(&sB[0][0]);
   const int lrow = lane >> 2, lchunk = (lane & 3) ^ ((lane >> 4) & 3);
   const int src_lane = lrow * 64 + 16 * lchunk;   // inside the 1 KB block of 16 rows x 64 k
   const long kblk = kpad / 64;
@@ -804,12 +795,12 @@ __global__ __launch_bounds__(64 * NW) void k_oz_gemm5(int mpad, int npad, int kp
 #pragma unroll
       for (int q = 0; q < SEGA_W; ++q) {
         const int seg = wu * SEGA_W + q;
-        lds_dma16(Ab + ((long)seg * kblk + kt) * 1024, sA_base + stage * (TMB * 64) + seg * 1024);
+        vcphip_lds_dma16(Ab + ((long)seg * kblk + kt) * 1024, sA_base + stage * (TMB * 64) + seg * 1024);
       }
 #pragma unroll
       for (int q = 0; q < SEGB_W; ++q) {
         const int seg = wu * SEGB_W + q;
-        lds_dma16(Bb + ((long)seg * kblk + kt) * 1024, sB_base + stage * (TNB * 64) + seg * 1024);
+        vcphip_lds_dma16(Bb + ((long)seg * kblk + kt) * 1024, sB_base + stage * (TNB * 64) + seg * 1024);
       }
     };
     auto wait_next = [&](int kt) {
@@ -928,7 +919,7 @@ __global__ __launch_bounds__(512) void k_oz_gemm6(int mpad, int npad, int kpad, 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w & 1, wn = w >> 1;
   const int wu = __builtin_amdgcn_readfirstlane(w);
-  const uint32_t sA_base = lds_addr(&sA[0][0]), sB_base = lds_addr(&sB[0][0]);
+  const uint32_t sA_base = vcphip_lds_addr(&sA[0][0]), sB_base = vcphip_lds_addr(&sB[0][0]);
   const int lrow = lane >> 2, lchunk = (lane & 3) ^ ((lane >> 4) & 3);
   const int src_lane = lrow * 64 + 16 * lchunk;   // inside the 1 KB block of 16 rows x 64 k
   const long kblk = kpad / 64;
@@ -960,12 +951,12 @@ __global__ __launch_bounds__(512) void k_oz_gemm6(int mpad, int npad, int kpad, 
 #pragma unroll
       for (int q = 0; q < SEGA_W; ++q) {
         const int seg = wu * SEGA_W + q;
-        lds_dma16(Ab + ((long)seg * kblk + kt) * 1024, sA_base + stage * (TMB * 64) + seg * 1024);
+        vcphip_lds_dma16(Ab + ((long)seg * kblk + kt) * 1024, sA_base + stage * (TMB * 64) + seg * 1024);
       }
 #pragma unroll
       for (int q = 0; q < SEGB_W; ++q) {
         const int seg = wu * SEGB_W + q;
-        lds_dma16(Bb + ((long)bseg[q] * kblk + kt) * 1024, sB_base + stage * (TNB * 64) + seg * 1024);
+        vcphip_lds_dma16(Bb + ((long)bseg[q] * kblk + kt) * 1024, sB_base + stage * (TNB * 64) + seg * 1024);
       }
     };
     auto half = [&](int stage, int h) {

@@ -17,6 +17,19 @@
 
 #define VCPHIP_DEVINL __device__ __forceinline__
 
+// LDS byte address of a __shared__ object.
+VCPHIP_DEVINL uint32_t vcphip_lds_addr(const void* p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
+}
+// One wave-wide 16-byte-per-lane load straight into LDS
+// (global_load_lds_dwordx4): lane l's 16 bytes, from its own address g, land
+// at lds_base + 16 l (lds_base must be wave-uniform).  Neither the load nor
+// its completion is visible to the compiler: callers wait with an explicit
+// s_waitcnt vmcnt before a barrier.
+VCPHIP_DEVINL void vcphip_lds_dma16(const void* g, uint32_t lds_base) {
+  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g), "s"(lds_base) : "memory", "m0");
+}
+
 VCPHIP_DEVINL void vcphip_mode_round_up() {
   asm volatile("s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 4), 5" ::: "memory");
 }

@@ -13,6 +13,15 @@ enum { VCPHIP_DR_II = 0, VCPHIP_DR_IP = 1, VCPHIP_DR_PI = 2, VCPHIP_DR_PP = 3 };
 // plain FP64 directed-rounding GEMM family (kern_drgemm.hip)
 int vcphip_launch_isgemm(int m, int n, int k, const double* A, long lda, const double* B, long ldb, double* C,
                          long ldc, hipStream_t s);
+// the single products in two steps: planes of A (column-major, mpad x kpad;
+// mid and rad of an interval A, or A itself) and of B (row-major, kpad x
+// npad; mid and rad, or B), zero-padded to mpad, npad = ceil64, kpad =
+// ceil16, then the product from them (k > 0).  A1 / B1 are unused for a
+// point operand.
+int vcphip_launch_planes(int kind, int m, int n, int k, const double* A, long lda, const double* B, long ldb,
+                         double* A0, double* A1, double* B0, double* B1, hipStream_t s);
+int vcphip_launch_drgemm_planes(int kind, int m, int n, int k, const double* A0, const double* A1, const double* B0,
+                                const double* B1, double* C, long ldc, double* C2, int accumulate, hipStream_t s);
 int vcphip_launch_drgemm(int kind, int m, int n, int k, const double* A, long lda, const double* B,
                          long ldb, double* C, long ldc, double* C2, int accumulate, hipStream_t s);
 // C = A^T A forms (pidblas::mulltmm / vmulmm), kern_drgemm.hip
