@@ -740,6 +740,120 @@ __global__ __launch_bounds__(NT, 2) void k_drsyrk(int k, int n, const double* __
     }
 }
 
+// The same A^T A forms from the planes of A (k x n) made by k_planes_bt:
+// mid(A), rad(A) (or A) row-major kpad x npad, zero-padded.  Column i of A is
+// then the contiguous run at i of every k-row, so one plane set feeds both
+// operands: wave 0..3 load the slices of the row block (mid, rad) and of the
+// column block (mid, rad) straight into LDS, two stages.  The chains and
+// their order are k_drsyrk's.
+template <bool INTV>
+__global__ __launch_bounds__(NT, 2) void k_drsyrk_planes(int n, int npad, int kpad, const double* __restrict__ P0,
+                                                         const double* __restrict__ P1, double* __restrict__ C,
+                                                         long ldc) {
+  constexpr int NP = INTV ? 4 : 2;
+  vcphip_mode_round_up();
+  // [stage][plane: I mid, (I rad), J mid, (J rad)][k][64]
+  __shared__ __attribute__((aligned(16))) double sL[2][NP][BK][BM];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int nt = npad / BM;
+  int b = blockIdx.x, tm = 0;
+  while (b >= nt - tm) {
+    b -= nt - tm;
+    ++tm;
+  }
+  const int tn = tm + b;
+  const int i0 = tm * BM, j0 = tn * BN;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const bool loader = w < NP;
+  // plane of wave w: I mid, I rad, J mid, J rad (interval); I, J (point)
+  const bool rad = INTV && (w & 1);
+  const bool jside = INTV ? (w >= 2) : (w == 1);
+  const double* src = (rad ? P1 : P0) + (long)(lane >> 5) * npad + (jside ? j0 : i0) + 2 * (lane & 31);
+  const uint32_t lds0 = vcphip_lds_addr(&sL[0][0][0][0]) + (uint32_t)(loader ? w : 0) * (BK * BM * 8);
+  auto issue = [&](int stage, int t) {
+    if (!loader) return;
+    const double* g = src + (long)t * BK * npad;
+    const uint32_t d = lds0 + (uint32_t)stage * (NP * BK * BM * 8);
+#pragma unroll
+    for (int q = 0; q < BK / 2; ++q) vcphip_lds_dma16(g + (long)(2 * q) * npad, d + (uint32_t)q * (2 * BM * 8));
+  };
+
+  const int tx = tid % 16, ty = tid / 16;
+  double up[TM][TN], dn[TM][TN], tij[TM][TN], tji[TM][TN], sij[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      up[i][j] = 0.0;
+      dn[i][j] = 0.0;
+      tij[i][j] = 0.0;
+      tji[i][j] = 0.0;
+      sij[i][j] = 0.0;
+    }
+
+  const int ns = kpad / BK;
+  issue(0, 0);
+  for (int t = 0; t < ns; ++t) {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    if (t + 1 < ns) issue((t + 1) & 1, t + 1);
+    const double(*sIm)[BM] = sL[t & 1][0];
+    const double(*sIr)[BM] = sL[t & 1][INTV ? 1 : 0];
+    const double(*sJm)[BM] = sL[t & 1][INTV ? 2 : 1];
+    const double(*sJr)[BM] = sL[t & 1][INTV ? 3 : 1];
+#pragma unroll(INTV ? 1 : 2)
+    for (int kk = 0; kk < BK; ++kk) {
+      double a[TM], ar[TM], bb[TN], br[TN];
+      {
+        const double2 x0 = *reinterpret_cast<const double2*>(&sIm[kk][ty * TM]);
+        const double2 x1 = *reinterpret_cast<const double2*>(&sIm[kk][ty * TM + 2]);
+        a[0] = x0.x; a[1] = x0.y; a[2] = x1.x; a[3] = x1.y;
+        const double2 y0 = *reinterpret_cast<const double2*>(&sJm[kk][2 * tx]);
+        const double2 y1 = *reinterpret_cast<const double2*>(&sJm[kk][2 * tx + 32]);
+        bb[0] = y0.x; bb[1] = y0.y; bb[2] = y1.x; bb[3] = y1.y;
+      }
+      if (INTV) {
+        const double2 x0 = *reinterpret_cast<const double2*>(&sIr[kk][ty * TM]);
+        const double2 x1 = *reinterpret_cast<const double2*>(&sIr[kk][ty * TM + 2]);
+        ar[0] = x0.x; ar[1] = x0.y; ar[2] = x1.x; ar[3] = x1.y;
+        const double2 y0 = *reinterpret_cast<const double2*>(&sJr[kk][2 * tx]);
+        const double2 y1 = *reinterpret_cast<const double2*>(&sJr[kk][2 * tx + 32]);
+        br[0] = y0.x; br[1] = y0.y; br[2] = y1.x; br[3] = y1.y;
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          up[i][j] = __builtin_fma(a[i], bb[j], up[i][j]);
+          dn[i][j] = __builtin_fma(-a[i], bb[j], dn[i][j]);
+          if (INTV) {
+            tij[i][j] = __builtin_fma(__builtin_fabs(a[i]), br[j], tij[i][j]);
+            tji[i][j] = __builtin_fma(__builtin_fabs(bb[j]), ar[i], tji[i][j]);
+            sij[i][j] = __builtin_fma(ar[i], br[j], sij[i][j]);
+          }
+        }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int r = i0 + ty * TM + i, c = j0 + dr_col(tx, j);
+      if (r >= n || c >= n || r > c) continue;
+      double lo, hi;
+      if (INTV) {
+        const double T = tij[i][j] + (tji[i][j] + sij[i][j]);
+        hi = up[i][j] + T;
+        lo = -(dn[i][j] + T);
+      } else {
+        hi = up[i][j];
+        lo = -dn[i][j];
+      }
+      *reinterpret_cast<double2*>(C + 2 * (r + ldc * c)) = make_double2(lo, hi);
+      *reinterpret_cast<double2*>(C + 2 * (c + ldc * r)) = make_double2(lo, hi);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Inf-sup interval product (pidblas_fma::mulmm, vcp/pidblas_fma.hpp:62-71,
 // with the interval FMA of vcp/ifma.hpp:44-55): per output element the
@@ -909,6 +1023,23 @@ int vcphip_launch_drgemm_planes(int kind, int m, int n, int k, const double* A0,
     case VCPHIP_DR_PI: hipLaunchKernelGGL((k_drgemm_planes<false, true>), grid, block, 0, s, m, n, mpad, npad, kpad, A0, A1, B0, B1, C, ldc, C2, accumulate); break;
     case VCPHIP_DR_PP: hipLaunchKernelGGL((k_drgemm_planes<false, false>), grid, block, 0, s, m, n, mpad, npad, kpad, A0, A1, B0, B1, C, ldc, C2, accumulate); break;
     default: return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vcphip_launch_drsyrk_planes(int interval, int k, int n, const double* A, long lda, double* P0, double* P1,
+                                double* C, long ldc, hipStream_t s) {
+  if (n <= 0) return 0;
+  const int npad = (n + BN - 1) / BN * BN, kpad = (k + BK - 1) / BK * BK;
+  if (kpad == 0) return -1;   // k = 0 takes the tile path
+  const dim3 gb(kpad / 32 + (kpad % 32 != 0), npad / 32);
+  const int nt = npad / BN;
+  if (interval) {
+    hipLaunchKernelGGL(k_planes_bt<true>, gb, dim3(256), 0, s, k, n, kpad, npad, A, lda, P0, P1);
+    hipLaunchKernelGGL(k_drsyrk_planes<true>, dim3(nt * (nt + 1) / 2), dim3(NT), 0, s, n, npad, kpad, P0, P1, C, ldc);
+  } else {
+    hipLaunchKernelGGL(k_planes_bt<false>, gb, dim3(256), 0, s, k, n, kpad, npad, A, lda, P0, P1);
+    hipLaunchKernelGGL(k_drsyrk_planes<false>, dim3(nt * (nt + 1) / 2), dim3(NT), 0, s, n, npad, kpad, P0, P1, C, ldc);
   }
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }

@@ -817,6 +817,25 @@ int planes_product(vcphip_ctx* c, int kind, int m, int n, int k, const double* A
   return rc;
 }
 
+// A^T A (mulltmm / vmulmm) from the planes of A; k = 0 takes the tile kernel
+int syrk_planes(vcphip_ctx* c, int interval, int k, int n, const double* A, int lda, double* C, int ldc) {
+  StageTimer T(c);
+  if (k == 0) {
+    const int rc = check_launch(vcphip_launch_drsyrk(interval, k, n, A, lda, C, ldc, c->stream));
+    T.mark("drsyrk");
+    return rc;
+  }
+  const size_t npad = ((size_t)n + 63) / 64 * 64, kpad = ((size_t)k + 15) / 16 * 16;
+  void* base;
+  int rc = ws_get(c, vcphip_ctx::WS_MR, sizeof(double) * (interval ? 2 : 1) * kpad * npad, &base);
+  if (rc) return rc;
+  double* P0 = (double*)base;
+  rc = check_launch(vcphip_launch_drsyrk_planes(interval, k, n, A, lda, P0, interval ? P0 + kpad * npad : nullptr, C,
+                                                ldc, c->stream));
+  T.mark("drsyrk");
+  return rc;
+}
+
 }  // namespace
 
 extern "C" int vcphip_igemm_dev(vcphip_ctx* c, int form, int m, int n, int k, const double* IA, int lda,
@@ -862,7 +881,7 @@ extern "C" int vcphip_igemm_tn_dev(vcphip_ctx* c, int k, int n, const double* IA
   if (k < 0 || n < 0 || lda < (k > 1 ? k : 1) || ldc < (n > 1 ? n : 1)) return VCPHIP_ERR_DIMENSION;
   if (n == 0) return VCPHIP_OK;
   hipSetDevice(c->device);
-  return check_launch(vcphip_launch_drsyrk(1, k, n, IA, lda, C, ldc, c->stream));
+  return syrk_planes(c, 1, k, n, IA, lda, C, ldc);
 }
 
 extern "C" int vcphip_vgemm_tn_dev(vcphip_ctx* c, int k, int n, const double* A, int lda, double* C, int ldc) {
@@ -870,7 +889,7 @@ extern "C" int vcphip_vgemm_tn_dev(vcphip_ctx* c, int k, int n, const double* A,
   if (k < 0 || n < 0 || lda < (k > 1 ? k : 1) || ldc < (n > 1 ? n : 1)) return VCPHIP_ERR_DIMENSION;
   if (n == 0) return VCPHIP_OK;
   hipSetDevice(c->device);
-  return check_launch(vcphip_launch_drsyrk(0, k, n, A, lda, C, ldc, c->stream));
+  return syrk_planes(c, 0, k, n, A, lda, C, ldc);
 }
 
 extern "C" int vcphip_gemm_rurd_dev(vcphip_ctx* c, int m, int n, int k, const double* A, int lda, const double* B,

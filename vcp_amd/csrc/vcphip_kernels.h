@@ -22,6 +22,10 @@ int vcphip_launch_planes(int kind, int m, int n, int k, const double* A, long ld
                          double* A0, double* A1, double* B0, double* B1, hipStream_t s);
 int vcphip_launch_drgemm_planes(int kind, int m, int n, int k, const double* A0, const double* A1, const double* B0,
                                 const double* B1, double* C, long ldc, double* C2, int accumulate, hipStream_t s);
+// A^T A (interval or point, A k x n) from the planes of A (made here into
+// P0 / P1, each kpad x npad doubles; P1 unused for a point A); k > 0
+int vcphip_launch_drsyrk_planes(int interval, int k, int n, const double* A, long lda, double* P0, double* P1,
+                                double* C, long ldc, hipStream_t s);
 int vcphip_launch_drgemm(int kind, int m, int n, int k, const double* A, long lda, const double* B,
                          long ldb, double* C, long ldc, double* C2, int accumulate, hipStream_t s);
 // C = A^T A forms (pidblas::mulltmm / vmulmm), kern_drgemm.hip
