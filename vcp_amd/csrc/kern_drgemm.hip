@@ -519,39 +519,58 @@ __global__ __launch_bounds__(NT, (AI && BI) ? 2 : (AI || BI) ? 3 : 4) void k_drg
 // Batched verified residual C_p = R_p * IA_p - I (imats::linearsolve,
 // vcp/imats.hpp:318-326) for many problems of uneven size in one persistent
 // launch per tile shape (TMR = 4: 64 x 64, TMR = 2: 32 x 64): workgroups
-// stride over a tile list sorted by problem size (largest first), each tile
-// being the PI form above.
+// take the tiles of a list sorted by problem size (largest first) from a
+// work queue, each tile being the PI form above.
 // ---------------------------------------------------------------------------
+// Work queue of the persistent launches: workgroup thread 0 takes the next
+// tile index from the launch's counter (zeroed by k_queue_reset just before
+// the launch, on the same stream), so tiles are taken in list order as
+// workgroups come free.  (A fixed stride of the grid over the list was
+// 15-20% slower on uneven sets: a workgroup that drew long tiles held up the
+// end of the launch.)
+__device__ __forceinline__ int next_tile(unsigned* __restrict__ queue, int& slot) {
+  __syncthreads();   // every thread has read the previous index (and left the tile's LDS)
+  if (threadIdx.x == 0) slot = (int)atomicAdd(queue, 1u);
+  __syncthreads();
+  return slot;
+}
+
+__global__ __launch_bounds__(64) void k_queue_reset(unsigned* __restrict__ queue) {
+  if (threadIdx.x < 2) queue[threadIdx.x] = 0u;
+}
+
 template <int TMR>
 __global__ __launch_bounds__(NT, 3) void k_resid_tiles(int ntiles, const VcphipResidTile* __restrict__ tiles,
-                                                        const VcphipResidProblem* __restrict__ probs) {
+                                                        const VcphipResidProblem* __restrict__ probs,
+                                                        unsigned* __restrict__ queue) {
   vcphip_mode_round_up();
   __shared__ __attribute__((aligned(16))) DrLds<false, true> lds;
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  __shared__ int slot;
+  for (int t = next_tile(queue, slot); t < ntiles; t = next_tile(queue, slot)) {
     const VcphipResidTile tl = tiles[t];
     const VcphipResidProblem p = probs[tl.problem];
     dr_tile<false, true, true, TMR>(lds, p.n, p.n, p.n, p.R, p.ldr, p.A, p.lda, p.C, p.ldc, nullptr, 0, tl.row0,
                                     tl.col0);
-    __syncthreads();
   }
 }
 
 // Many interval products C_p = IA_p * IB_p of uneven (m, n, k) in one
 // persistent launch per tile shape (pidblas::mulmm, vcp/pidblas.hpp:239-294,
-// per problem): workgroups stride over a tile list sorted by problem size,
-// each tile the interval x interval form of k_drgemm, so every C_p is
-// bit-identical to the single product's.
+// per problem): workgroups take the tiles of a list sorted by problem size
+// from a work queue, each tile the interval x interval form of k_drgemm, so
+// every C_p is bit-identical to the single product's.
 template <int TMR>
 __global__ __launch_bounds__(NT, 2) void k_igemm_tiles(int ntiles, const VcphipResidTile* __restrict__ tiles,
-                                                        const VcphipIgemmProblem* __restrict__ probs) {
+                                                        const VcphipIgemmProblem* __restrict__ probs,
+                                                        unsigned* __restrict__ queue) {
   vcphip_mode_round_up();
   __shared__ __attribute__((aligned(16))) DrLds<true, true> lds;
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  __shared__ int slot;
+  for (int t = next_tile(queue, slot); t < ntiles; t = next_tile(queue, slot)) {
     const VcphipResidTile tl = tiles[t];
     const VcphipIgemmProblem p = probs[tl.problem];
     dr_tile<true, true, false, TMR>(lds, p.m, p.n, p.k, p.A, p.lda, p.B, p.ldb, p.C, p.ldc, nullptr, 0, tl.row0,
                                     tl.col0);
-    __syncthreads();
   }
 }
 
@@ -1055,9 +1074,10 @@ int vcphip_launch_drsyrk(int interval, int k, int n, const double* A, long lda, 
 }
 
 int vcphip_launch_igemm_tiles(int n64, int n32, const VcphipResidTile* tiles, const VcphipIgemmProblem* probs,
-                              int grid64, int grid32, hipStream_t s) {
-  if (n64 > 0) hipLaunchKernelGGL(k_igemm_tiles<4>, dim3(grid64), dim3(NT), 0, s, n64, tiles, probs);
-  if (n32 > 0) hipLaunchKernelGGL(k_igemm_tiles<2>, dim3(grid32), dim3(NT), 0, s, n32, tiles + n64, probs);
+                              int grid64, int grid32, unsigned* queue, hipStream_t s) {
+  if (n64 > 0 || n32 > 0) hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, s, queue);
+  if (n64 > 0) hipLaunchKernelGGL(k_igemm_tiles<4>, dim3(grid64), dim3(NT), 0, s, n64, tiles, probs, queue);
+  if (n32 > 0) hipLaunchKernelGGL(k_igemm_tiles<2>, dim3(grid32), dim3(NT), 0, s, n32, tiles + n64, probs, queue + 1);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
@@ -1068,16 +1088,17 @@ int vcphip_fill_f64(double* p, long n, double v, hipStream_t s) {
 }
 
 int vcphip_launch_resid_tiles(int n64, int n32, const VcphipResidTile* tiles, const VcphipResidProblem* probs,
-                              int grid64, int grid32, hipStream_t s) {
-  if (n64 > 0) hipLaunchKernelGGL(k_resid_tiles<4>, dim3(grid64), dim3(NT), 0, s, n64, tiles, probs);
-  if (n32 > 0) hipLaunchKernelGGL(k_resid_tiles<2>, dim3(grid32), dim3(NT), 0, s, n32, tiles + n64, probs);
+                              int grid64, int grid32, unsigned* queue, hipStream_t s) {
+  if (n64 > 0 || n32 > 0) hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, s, queue);
+  if (n64 > 0) hipLaunchKernelGGL(k_resid_tiles<4>, dim3(grid64), dim3(NT), 0, s, n64, tiles, probs, queue);
+  if (n32 > 0) hipLaunchKernelGGL(k_resid_tiles<2>, dim3(grid32), dim3(NT), 0, s, n32, tiles + n64, probs, queue + 1);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
 int vcphip_launch_resid(int n64, int n32, const VcphipResidTile* tiles, const VcphipResidProblem* probs, int nrows,
                         const int* row_problem, double* rowsum, int count, double* normub, int grid64, int grid32,
-                        hipStream_t s) {
-  vcphip_launch_resid_tiles(n64, n32, tiles, probs, grid64, grid32, s);
+                        unsigned* queue, hipStream_t s) {
+  vcphip_launch_resid_tiles(n64, n32, tiles, probs, grid64, grid32, queue, s);
   if (nrows > 0) hipLaunchKernelGGL(k_resid_rowsum, dim3((nrows + 255) / 256), dim3(256), 0, s, nrows, row_problem, probs, rowsum);
   if (count > 0) hipLaunchKernelGGL(k_resid_max, dim3(count), dim3(256), 0, s, count, probs, rowsum, normub);
   return hipGetLastError() == hipSuccess ? 0 : -3;

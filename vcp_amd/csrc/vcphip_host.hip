@@ -35,7 +35,7 @@ struct vcphip_ctx {
   long long oz_nmod = 0, oz_variant = 0, timing = 0, dd_nmod = 0, oz_radius = 0, batch_tiles = 0;
   // workspace slots (device memory, grown on demand)
   enum { WS_PA, WS_PB, WS_U, WS_Z, WS_SA, WS_SB, WS_HA, WS_HB, WS_HC, WS_HC2, WS_BATCH, WS_DD, WS_LSS, WS_HR, WS_LSSB,
-         WS_INV, WS_DDS, WS_IGB, WS_TT, WS_MR, WS_N };
+         WS_INV, WS_DDS, WS_IGB, WS_TT, WS_MR, WS_Q, WS_N };
   void* ws[WS_N] = {};
   size_t ws_size[WS_N] = {};
   // stage timing: events recorded by every call since the last synchronize,
@@ -1040,6 +1040,13 @@ void add_tiles(TileSet& s, int pi, int m, int n, int k, int mode) {
   s.useful += (double)m * n * k;
 }
 
+// the two work-queue counters of the batched launches on the context stream
+// (a fixed 256-byte slot: it is never regrown, so captured graphs keep it)
+unsigned* batch_queue(vcphip_ctx* c) {
+  void* q = nullptr;
+  return ws_get(c, vcphip_ctx::WS_Q, 256, &q) == VCPHIP_OK ? (unsigned*)q : nullptr;
+}
+
 // grid of one shape's persistent launch
 int batch_grid(const vcphip_ctx* c, size_t ntiles, int per_cu) {
   return (int)std::max<size_t>(1, std::min(ntiles, (size_t)per_cu * (c->num_cu > 0 ? c->num_cu : 256)));
@@ -1231,7 +1238,14 @@ extern "C" int vcphip_igemm_batched_dev(vcphip_ctx* c, int count, const int* m, 
     const VcphipResidTile* dt = (const VcphipResidTile*)(b8 + o_t);
     const VcphipIgemmProblem* dp = (const VcphipIgemmProblem*)b8;
     g.dp = dp;
-    auto launch = [&]() { return vcphip_launch_igemm_tiles(g.n64, g.n32, dt, dp, g.grid, g.grid32, c->stream); };
+    unsigned* queue = batch_queue(c);
+    if (!queue) {
+      g.probs.clear();
+      return VCPHIP_ERR_OOM;
+    }
+    auto launch = [&]() {
+      return vcphip_launch_igemm_tiles(g.n64, g.n32, dt, dp, g.grid, g.grid32, queue, c->stream);
+    };
     bool captured = false;
     if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
       const int lrc = launch();
@@ -1326,10 +1340,15 @@ extern "C" int vcphip_residual_batched_dev(vcphip_ctx* c, int count, const int* 
     rs.drp = drp;
     rs.drs = drs;
     rs.nrows = nrows;
+    unsigned* queue = batch_queue(c);
+    if (!queue) {
+      rs.probs.clear();
+      return VCPHIP_ERR_OOM;
+    }
     bool captured = false;
     if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
       const int lrc = vcphip_launch_resid(rs.n64, rs.n32, dt, dp, nrows, drp, drs, count, normub_dev, rs.grid,
-                                          rs.grid32, c->stream);
+                                          rs.grid32, queue, c->stream);
       hipGraph_t g = nullptr;
       if (hipStreamEndCapture(c->stream, &g) == hipSuccess && lrc == 0 && g &&
           graph_has_kernels(g, (size_t)vcphip_resid_kernels(rs.n64, rs.n32, nrows, count))) {
@@ -1350,7 +1369,7 @@ extern "C" int vcphip_residual_batched_dev(vcphip_ctx* c, int count, const int* 
       // the same kernels as direct launches
       StageTimer T(c);
       const int lrc = vcphip_launch_resid(rs.n64, rs.n32, dt, dp, nrows, drp, drs, count, normub_dev, rs.grid,
-                                          rs.grid32, c->stream);
+                                          rs.grid32, queue, c->stream);
       T.mark("resid");
       rs.probs.clear();   // nothing cached to replay
       return check_launch(lrc);
@@ -2942,7 +2961,10 @@ extern "C" int vcphip_tune_residual_dev(vcphip_ctx* c, int count, const int* n, 
                                          hipMemcpyHostToDevice, c->stream) != hipSuccess)
       return VCPHIP_ERR_DEVICE;
     // time_runs synchronises the stream before `tiles` goes out of scope
-    return time_runs(c, [&]() { return vcphip_launch_resid_tiles(n64, n32, dt, dp, g64, g32, c->stream); }, 5, t);
+    unsigned* queue = batch_queue(c);
+    if (!queue) return VCPHIP_ERR_OOM;
+    return time_runs(c, [&]() { return vcphip_launch_resid_tiles(n64, n32, dt, dp, g64, g32, queue, c->stream); }, 5,
+                     t);
   };
   std::vector<int> rows((size_t)count);
   std::vector<double> work((size_t)count);
@@ -3009,7 +3031,10 @@ extern "C" int vcphip_tune_igemm_batched_dev(vcphip_ctx* c, int count, const int
     if (!tiles.empty() && hipMemcpyAsync(base, tiles.data(), sizeof(VcphipResidTile) * tiles.size(),
                                          hipMemcpyHostToDevice, c->stream) != hipSuccess)
       return VCPHIP_ERR_DEVICE;
-    return time_runs(c, [&]() { return vcphip_launch_igemm_tiles(n64, n32, dt, dp, g64, g32, c->stream); }, 5, t);
+    unsigned* queue = batch_queue(c);
+    if (!queue) return VCPHIP_ERR_OOM;
+    return time_runs(c, [&]() { return vcphip_launch_igemm_tiles(n64, n32, dt, dp, g64, g32, queue, c->stream); }, 5,
+                     t);
   };
   std::vector<int> rows((size_t)count);
   std::vector<double> work((size_t)count);

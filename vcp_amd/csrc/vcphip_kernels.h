@@ -58,21 +58,23 @@ struct VcphipIgemmProblem {
   int m, n, k, pad;
 };
 // tiles[0, n64) are 64 x 64 tiles, tiles[n64, n64 + n32) 32 x 64 ones; each
-// shape's persistent grid strides over its own part
+// shape's persistent grid takes its part's tiles from a work queue: queue[0]
+// and queue[1] (device counters, zeroed by a kernel these launchers enqueue
+// first; one pair per stream of launches)
 int vcphip_launch_igemm_tiles(int n64, int n32, const VcphipResidTile* tiles, const VcphipIgemmProblem* probs,
-                              int grid64, int grid32, hipStream_t s);
+                              int grid64, int grid32, unsigned* queue, hipStream_t s);
 int vcphip_launch_resid(int n64, int n32, const VcphipResidTile* tiles, const VcphipResidProblem* probs, int nrows,
                         const int* row_problem, double* rowsum, int count, double* normub, int grid64, int grid32,
-                        hipStream_t s);
+                        unsigned* queue, hipStream_t s);
 // the tile launches alone (the tuner times them)
 int vcphip_launch_resid_tiles(int n64, int n32, const VcphipResidTile* tiles, const VcphipResidProblem* probs,
-                              int grid64, int grid32, hipStream_t s);
+                              int grid64, int grid32, unsigned* queue, hipStream_t s);
 // the number of kernels vcphip_launch_resid / vcphip_launch_igemm_tiles
 // enqueue for these sizes
+inline int vcphip_igemm_tile_kernels(int n64, int n32) { return (n64 > 0 || n32 > 0) + (n64 > 0) + (n32 > 0); }
 inline int vcphip_resid_kernels(int n64, int n32, int nrows, int count) {
-  return (n64 > 0) + (n32 > 0) + (nrows > 0) + (count > 0);
+  return vcphip_igemm_tile_kernels(n64, n32) + (nrows > 0) + (count > 0);
 }
-inline int vcphip_igemm_tile_kernels(int n64, int n32) { return (n64 > 0) + (n32 > 0); }
 // p[0..n) = v by a kernel
 int vcphip_fill_f64(double* p, long n, double v, hipStream_t s);
 
