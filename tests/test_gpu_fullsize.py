@@ -5,7 +5,8 @@ do not need the whole product on the CPU:
   entries for the Ozaki-II and the plain form; the Ozaki radii within 2x of
   the plain form's over all 67M entries; two rows of each form bit-identical to
   the oracle's restatement (row scaling and the k-chains depend only on the
-  row and on the whole of B, so a row subset is the full product's rows);
+  row and on the whole of B, so a row subset is the full product's rows); the
+  same for the Ozaki configuration the moduli tuner picks;
 - dd GEMM n=4096: error of sampled entries against the exact dot product;
 - RU/RD point GEMM n=4096: two rows bit-identical to the oracle, CD <= CU;
 - batched R*A - I over the 256 problems of the batched workload: every bound
@@ -73,6 +74,44 @@ def test_interval_gemm_n8192(ctx, config3):
     IAr = _rows(IA, rows)
     assert np.array_equal(C[rows], O.ozaki_igemm(IAr, IB, 16))
     assert np.array_equal(P[rows], O.pidblas_mulmm(IAr, IB))
+
+
+def test_interval_gemm_n8192_tuned_configuration(ctx, config3):
+    """the configuration the moduli tuner picks for this workload at a radius
+    ratio target of 2 (the benchmark's headline): every radius within 2x of
+    the plain form's over all 67M entries, 256 sampled entries containing the
+    exact hull, and two rows bit-identical to the oracle's restatement with
+    the same moduli count and radius scheme"""
+    IA, IB = config3
+    n = IA.shape[0]
+    dA, dB = _dev(IA), _dev(IB)
+    dC = torch.empty((n, n, 2), dtype=torch.float64, device="cuda")
+    dP = torch.empty((n, n, 2), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    with vcp_amd.Context(0) as c2:
+        nmod, radius, sampled = c2.tune_igemm_moduli_dev(n, n, n, dA.data_ptr(), n, dB.data_ptr(), n, 2.0, 0)
+        print("tuned: %d moduli, %s radius digits, sampled ratio %.4f"
+              % (nmod, vcp_amd.OZ_RADIUS_NAMES[radius], sampled), flush=True)
+        plan = c2.ozaki_plan(n, n, n)
+        assert plan["moduli"] == nmod and plan["radius"] == radius
+        c2.igemm_dev(vcp_amd.FORM_OZAKI, n, n, n, dA.data_ptr(), n, dB.data_ptr(), n, dC.data_ptr(), n)
+        c2.igemm_dev(vcp_amd.FORM_PLAIN, n, n, n, dA.data_ptr(), n, dB.data_ptr(), n, dP.data_ptr(), n)
+        c2.synchronize()
+    wo = dC[..., 1] - dC[..., 0]
+    wp = dP[..., 1] - dP[..., 0]
+    ratio = wo / torch.clamp(wp, min=torch.finfo(torch.float64).tiny)
+    print("radius ratio max %.4f median %.4f" % (float(ratio.max()), float(ratio.median())), flush=True)
+    assert bool(torch.isfinite(wo).all()) and float(ratio.max()) <= 2.0
+    del wo, wp, ratio, dP
+    C = dC.cpu().numpy().transpose(1, 0, 2)
+    del dA, dB, dC
+    rng = np.random.default_rng(93)
+    ii, jj = rng.integers(0, n, 256), rng.integers(0, n, 256)
+    ok, _, _ = O.check_containment(IA, IB, C, ii, jj)
+    assert ok.all(), "%d of 256 sampled entries not enclosed" % (~ok).sum()
+    rows = [7, 4099]
+    print("oracle rows", rows, flush=True)
+    assert np.array_equal(C[rows], O.ozaki_igemm(_rows(IA, rows), IB, nmod, radius))
 
 
 def test_dd_gemm_n4096(ctx):
