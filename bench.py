@@ -726,6 +726,7 @@ def bench_config2(ctx, n=4096, steps=3):
                                            dD.data_ptr(), n))
     t_ii = timed(lambda: ctx.igemm_dev(vcp_amd.FORM_PLAIN, n, n, n, dIA.data_ptr(), n, dIB.data_ptr(), n,
                                        dC.data_ptr(), n))
+    C_ii = dC.cpu().numpy().transpose(1, 0, 2)   # the other forms below reuse dC
     out = {"n": n,
            "rurd_point": {"ms": round(t_pp * 1e3, 3), "tflops_4n3": round(4.0 * n ** 3 / t_pp / 1e12, 3),
                           "frac_fp64_peak": round(4.0 * n ** 3 / t_pp / 1e12 / PEAK_FP64_TFLOPS, 4)},
@@ -760,13 +761,15 @@ def bench_config2(ctx, n=4096, steps=3):
 
         U = dU.cpu().numpy().T
         D = dD.cpu().numpy().T
-        ii = rng.integers(0, n, 256)
-        jj = rng.integers(0, n, 256)
+        ii = rng.integers(0, n, 4096)
+        jj = rng.integers(0, n, 4096)
         ok = 0
         for i, j in zip(ii, jj):
             lo, hi, _ = O.exact_dot(A[i], B[:, j])
             ok += int(D[i, j] <= lo and hi <= U[i, j])
-        out["rurd_sampled_enclosure"] = {"samples": 256, "ok": ok}
+        out["rurd_sampled_enclosure"] = {"samples": 4096, "ok": ok}
+        okc, _, _ = O.check_containment(IA, IB, C_ii, ii, jj)
+        out["interval_sampled_containment"] = {"samples": 4096, "ok": int(okc.sum())}
     except Exception as e:
         log("config2 check skipped: %r" % (e,))
     log("plain n=4096 %s" % out)
