@@ -314,8 +314,9 @@ __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict_
   int8_t* dst = planes + vcphip_plane_off(row0 + orow, k0 + oc, kpad);
 #pragma unroll 4
   for (int p = 0; p < nplane; ++p)
-    *reinterpret_cast<unsigned*>(dst + p * plane_stride) =
-        *reinterpret_cast<const unsigned*>(s_tile + (p * SR_ROWS + orow) * SR_LD + oc);
+    if (p < N || ((P.dmask_a >> (p - N)) & 1))   // digit planes the radius scheme reads
+      *reinterpret_cast<unsigned*>(dst + p * plane_stride) =
+          *reinterpret_cast<const unsigned*>(s_tile + (p * SR_ROWS + orow) * SR_LD + oc);
 }
 
 // columns of B (k x n, column-major intervals) -> N+4 planes of npad x kpad:
@@ -367,10 +368,12 @@ __global__ __launch_bounds__(256) void k_oz_split_cols(const double* __restrict_
       word |= ((unsigned)sym_residue12(ch[u], z, P) & 0xFFu) << (8 * u);
     *reinterpret_cast<unsigned*>(dst + z * plane_stride) = word;
   }
-  *reinterpret_cast<unsigned*>(dst + N * plane_stride) = drh;
-  *reinterpret_cast<unsigned*>(dst + (N + 1) * plane_stride) = drl;
-  *reinterpret_cast<unsigned*>(dst + (N + 2) * plane_stride) = dmh;
-  *reinterpret_cast<unsigned*>(dst + (N + 3) * plane_stride) = dml;
+  // the digit planes the radius scheme reads (the low digits are unused by
+  // the one-digit schemes)
+  if (P.dmask_b & 1) *reinterpret_cast<unsigned*>(dst + N * plane_stride) = drh;
+  if (P.dmask_b & 2) *reinterpret_cast<unsigned*>(dst + (N + 1) * plane_stride) = drl;
+  if (P.dmask_b & 4) *reinterpret_cast<unsigned*>(dst + (N + 2) * plane_stride) = dmh;
+  if (P.dmask_b & 8) *reinterpret_cast<unsigned*>(dst + (N + 3) * plane_stride) = dml;
 }
 
 // ---------------------------------------------------------------------------

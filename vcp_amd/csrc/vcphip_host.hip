@@ -251,6 +251,12 @@ void oz_set_radius(VcphipOzParams* P, int radius) {
       P->rnz = 2;
       break;
   }
+  P->dmask_a = P->dmask_b = 0;
+  for (int r = 0; r < P->rnz; ++r)
+    for (int q = 0; q < P->rnp[r]; ++q) {
+      P->dmask_a |= 1 << P->rpa[r][q];
+      P->dmask_b |= 1 << P->rpb[r][q];
+    }
 }
 
 int gcd_i(int a, int b) {
@@ -297,6 +303,7 @@ int oz_beta(int k, int nmod, const int* mods) {
 // condition k (2^beta + slack)^2 < M/2, i.e. k 2^(2 beta + extra) < M
 int oz_fill_params(int k, int nmod, int extra, int beta_cap, VcphipOzParams* P) {
   memset(P, 0, sizeof(*P));
+  P->dmask_a = P->dmask_b = ~0;   // every digit plane until a radius scheme is set
   if (nmod < 8 || nmod > VCPHIP_OZ_MAXMOD) return VCPHIP_ERR_ARGUMENT;
   if (oz_moduli(nmod, P->mod) != nmod) return VCPHIP_ERR_ARGUMENT;
   P->nmod = nmod;

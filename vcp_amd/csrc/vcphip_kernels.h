@@ -121,6 +121,9 @@ struct VcphipOzParams {
   // rpb[r][p] (digit planes counted from the first one after the residues)
   int rscheme, rnz;
   int rnp[VCPHIP_OZ_MAXRAD], rpa[VCPHIP_OZ_MAXRAD][2], rpb[VCPHIP_OZ_MAXRAD][2];
+  // the digit planes those passes read (bit d: digit plane d of A / of B);
+  // the split kernels write only these
+  int dmask_a, dmask_b;
 };
 
 // radius-digit planes that follow the N residue planes (7-bit upward digits
