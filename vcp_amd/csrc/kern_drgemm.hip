@@ -586,16 +586,18 @@ __global__ __launch_bounds__(256) void k_resid_rowsum(int nrows, const int* __re
   const int i = g - p.row_base;
   const double* row = p.C + 2 * (long)i;
   const long step = 2 * (long)p.ldc;
-  // the upward sum runs in column order, as the reference's; the loads of
-  // eight columns are issued ahead of their additions
+  // the upward sum runs in column order, as the reference's; a thread per
+  // row is few waves per SIMD, so the loads of 32 columns are issued ahead of
+  // their additions
+  constexpr int U = 32;
   double s = 0.0;
   int j = 0;
-  for (; j + 8 <= p.n; j += 8) {
-    double2 v[8];
+  for (; j + U <= p.n; j += U) {
+    double2 v[U];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = *reinterpret_cast<const double2*>(row + step * (j + u));
+    for (int u = 0; u < U; ++u) v[u] = *reinterpret_cast<const double2*>(row + step * (j + u));
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < U; ++u) {
       const double mag = fmax(__builtin_fabs(v[u].x), __builtin_fabs(v[u].y));
       s = (j + u == 0) ? mag : s + mag;
     }

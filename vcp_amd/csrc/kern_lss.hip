@@ -83,13 +83,19 @@ __global__ __launch_bounds__(NT) void k_lss_gemv_rn(int m, int k, const double* 
   const double* a = A + i;
   const double* bj = B + ldb * (long)j;
   double acc = 0.0;
+  // a thread per row is few waves per SIMD: 32 k of loads in flight per
+  // thread, then the chain in k order
+  constexpr int U = 32;
   int kk = 0;
-  for (; kk + 4 <= k; kk += 4) {
-    const double a0 = a[lda * kk], a1 = a[lda * (kk + 1)], a2 = a[lda * (kk + 2)], a3 = a[lda * (kk + 3)];
-    acc = __builtin_fma(a0, bj[kk], acc);
-    acc = __builtin_fma(a1, bj[kk + 1], acc);
-    acc = __builtin_fma(a2, bj[kk + 2], acc);
-    acc = __builtin_fma(a3, bj[kk + 3], acc);
+  for (; kk + U <= k; kk += U) {
+    double av[U], bv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      av[u] = a[lda * (kk + u)];
+      bv[u] = bj[kk + u];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc = __builtin_fma(av[u], bv[u], acc);
   }
   for (; kk < k; ++kk) acc = __builtin_fma(a[lda * kk], bj[kk], acc);
   C[i + ldc * (long)j] = acc;
@@ -147,12 +153,22 @@ __global__ __launch_bounds__(NT) void k_lss_rowsum(int rows, int cols, const dou
   const int i = blockIdx.x * NT + threadIdx.x;
   if (i >= rows) return;
   double s = 0.0;
-  for (int j = 0; j < cols; ++j) {
-    const double2 v = *reinterpret_cast<const double2*>(X + 2 * (i + ldx * (long)j));
+  auto add = [&](int j, double2 v) {
     const double a = __builtin_fabs(v.x), b = __builtin_fabs(v.y);
     const double mag = a > b ? a : b;
     s = (j == 0) ? mag : s + mag;
+  };
+  // upward, in column order; 32 columns of loads in flight per thread
+  constexpr int U = 32;
+  int j = 0;
+  for (; j + U <= cols; j += U) {
+    double2 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = *reinterpret_cast<const double2*>(X + 2 * (i + ldx * (long)(j + u)));
+#pragma unroll
+    for (int u = 0; u < U; ++u) add(j + u, v[u]);
   }
+  for (; j < cols; ++j) add(j, *reinterpret_cast<const double2*>(X + 2 * (i + ldx * (long)j)));
   out[i] = s;
 }
 
@@ -285,6 +301,30 @@ VCPHIP_DEVINL int find_problem(const long* __restrict__ pb, int count, long item
   return lo;
 }
 
+// one k of the chains of one output of the k_drgemm forms: the operands as
+// loaded (an interval as its AoS pair, a point in .x), midpoint-radius taken
+// here, every chain advanced in the tile kernels' order.  Round-up mode.
+template <bool AI, bool BI>
+VCPHIP_DEVINL void dr_dot_step(double2 av, double2 bv, double& up, double& dn, double& r1, double& r2) {
+  constexpr bool II = AI && BI;
+  double am, ar = 0.0, bm, br = 0.0;
+  if (AI) vcphip_midrad_ru(av.x, av.y, am, ar);
+  else am = av.x;
+  if (BI) vcphip_midrad_ru(bv.x, bv.y, bm, br);
+  else bm = bv.x;
+  up = __builtin_fma(am, bm, up);
+  dn = __builtin_fma(-am, bm, dn);
+  if (II) {
+    const double as = __builtin_fabs(am) + ar;
+    r1 = __builtin_fma(as, br, r1);
+    r2 = __builtin_fma(ar, __builtin_fabs(bm), r2);
+  } else if (AI) {
+    r1 = __builtin_fma(ar, __builtin_fabs(bm), r1);
+  } else if (BI) {
+    r1 = __builtin_fma(__builtin_fabs(am), br, r1);
+  }
+}
+
 // one output of the k_drgemm forms: A row r (stride lda over k), B column c
 // (stride 1 over k); AI / BI: operand is an interval (AoS pairs).
 // Round-up mode.  Returns {lower, upper}.
@@ -292,34 +332,31 @@ template <bool AI, bool BI>
 VCPHIP_DEVINL double2 dr_dot(int k, const double* __restrict__ A, long lda, const double* __restrict__ B) {
   constexpr bool II = AI && BI;
   double up = 0.0, dn = 0.0, r1 = 0.0, r2 = 0.0;
-  // unrolled so that each thread keeps several loads in flight (the chains
-  // themselves stay sequential in k)
-#pragma unroll 8
-  for (int kk = 0; kk < k; ++kk) {
-    double am, ar = 0.0, bm, br = 0.0;
-    if (AI) {
-      const double2 v = *reinterpret_cast<const double2*>(A + 2 * lda * kk);
-      vcphip_midrad_ru(v.x, v.y, am, ar);
-    } else {
-      am = A[lda * kk];
+  // Each thread's chain is sequential in k and these launches hold few waves
+  // per SIMD, so their time is load latency: the operands of U consecutive k
+  // are loaded first (U loads of each operand in flight per thread), then
+  // chained in order.
+  constexpr int U = (AI || BI) ? 16 : 32;
+  int k0 = 0;
+  for (; k0 + U <= k; k0 += U) {
+    double2 av[U], bv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (AI) av[u] = *reinterpret_cast<const double2*>(A + 2 * lda * (k0 + u));
+      else av[u].x = A[lda * (k0 + u)];
+      if (BI) bv[u] = *reinterpret_cast<const double2*>(B + 2 * (k0 + u));
+      else bv[u].x = B[k0 + u];
     }
-    if (BI) {
-      const double2 v = *reinterpret_cast<const double2*>(B + 2 * kk);
-      vcphip_midrad_ru(v.x, v.y, bm, br);
-    } else {
-      bm = B[kk];
-    }
-    up = __builtin_fma(am, bm, up);
-    dn = __builtin_fma(-am, bm, dn);
-    if (II) {
-      const double as = __builtin_fabs(am) + ar;
-      r1 = __builtin_fma(as, br, r1);
-      r2 = __builtin_fma(ar, __builtin_fabs(bm), r2);
-    } else if (AI) {
-      r1 = __builtin_fma(ar, __builtin_fabs(bm), r1);
-    } else if (BI) {
-      r1 = __builtin_fma(__builtin_fabs(am), br, r1);
-    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) dr_dot_step<AI, BI>(av[u], bv[u], up, dn, r1, r2);
+  }
+  for (int kk = k0; kk < k; ++kk) {
+    double2 av, bv;
+    if (AI) av = *reinterpret_cast<const double2*>(A + 2 * lda * kk);
+    else av.x = A[lda * kk];
+    if (BI) bv = *reinterpret_cast<const double2*>(B + 2 * kk);
+    else bv.x = B[kk];
+    dr_dot_step<AI, BI>(av, bv, up, dn, r1, r2);
   }
   const double t = II ? r1 + r2 : r1;
   const double hi = up + t;
@@ -358,8 +395,20 @@ __global__ __launch_bounds__(NT) void k_lssb_mx(int count, const long* __restric
   const double* a = P.R + i;
   const double* bj = P.mb + (long)P.n * j;
   double acc = 0.0;
-#pragma unroll 8
-  for (int kk = 0; kk < P.n; ++kk) acc = __builtin_fma(a[P.ldr * (long)kk], bj[kk], acc);
+  // 32 k of loads in flight per thread, then the chain in k order
+  constexpr int U = 32;
+  int k0 = 0;
+  for (; k0 + U <= P.n; k0 += U) {
+    double av[U], bv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      av[u] = a[P.ldr * (long)(k0 + u)];
+      bv[u] = bj[k0 + u];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc = __builtin_fma(av[u], bv[u], acc);
+  }
+  for (int kk = k0; kk < P.n; ++kk) acc = __builtin_fma(a[P.ldr * (long)kk], bj[kk], acc);
   P.mx[q] = acc;
 }
 
@@ -463,9 +512,7 @@ __global__ __launch_bounds__(NT) void k_lssb_final(int count, const long* __rest
   vcphip_midrad_ru(G.x, G.y, bm, br);
   double up = 0.0, r1 = 0.0, r2 = 0.0;   // the lower chain of W is not used
   const double* crow = P.C + 2 * i;
-#pragma unroll 8
-  for (int kk = 0; kk < P.n; ++kk) {
-    double2 v = *reinterpret_cast<const double2*>(crow + 2 * (long)P.n * kk);
+  auto step = [&](double2 v) {
     if (v.x < 0.0) {
       if (v.y <= 0.0) v = make_double2(-v.y, -v.x);
       else v = make_double2(0.0, -v.x > v.y ? -v.x : v.y);
@@ -476,7 +523,18 @@ __global__ __launch_bounds__(NT) void k_lssb_final(int count, const long* __rest
     const double as = __builtin_fabs(am) + ar;
     r1 = __builtin_fma(as, br, r1);
     r2 = __builtin_fma(ar, __builtin_fabs(bm), r2);
+  };
+  // a thread per row: 32 columns of loads in flight, then the chain in order
+  constexpr int U = 32;
+  int kk = 0;
+  for (; kk + U <= P.n; kk += U) {
+    double2 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = *reinterpret_cast<const double2*>(crow + 2 * (long)P.n * (kk + u));
+#pragma unroll
+    for (int u = 0; u < U; ++u) step(v[u]);
   }
+  for (; kk < P.n; ++kk) step(*reinterpret_cast<const double2*>(crow + 2 * (long)P.n * kk));
   const double wu = up + (r1 + r2);
   const double yu = P.y[2 * q + 1], m = P.mx[q];
   const double u = yu + wu;
