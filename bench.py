@@ -220,7 +220,7 @@ def main():
         ops = 2.0 * n * n * n * nprod
         ach = ops / (t_gemm * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
-                "frac": round(ach / PEAK_INT8_TOPS, 4), "traffic": measured_traffic(n, nmod, nprod),
+                "frac": round(ach / PEAK_INT8_TOPS, 4), "traffic": measured_traffic(n, nmod, nprod, plan["variant"]),
                 "kernel": "k_oz_gemm (int8 MFMA: %d residue + %d radius-digit plane products, %s radius scheme, "
                           "tile variant %d)" % (nmod, nprod - nmod, scheme, plan["variant"]),
                 "ops_per_launch": ops, "ms_per_launch": round(t_gemm, 4)}
@@ -308,11 +308,11 @@ def main():
     ctx.close()
 
 
-def measured_traffic(n, nmod, nprod):
+def measured_traffic(n, nmod, nprod, variant):
     """HBM bytes per launch of the int8 GEMM stage, from the committed
     FETCH_SIZE / WRITE_SIZE counter runs (profiles/ozaki_gemm_traffic.json),
-    when one was taken on this workload (size, moduli and plane products);
-    otherwise None."""
+    when one was taken on this workload (size, moduli, plane products and
+    tile variant); otherwise None."""
     path = os.path.join(ROOT, "profiles", "ozaki_gemm_traffic.json")
     try:
         with open(path) as f:
@@ -320,7 +320,8 @@ def measured_traffic(n, nmod, nprod):
     except (OSError, ValueError, AttributeError):
         return None
     for rec in recs:
-        if rec.get("n") == n and rec.get("moduli") == nmod and rec.get("plane_products") == nprod:
+        if (rec.get("n") == n and rec.get("moduli") == nmod and rec.get("plane_products") == nprod
+                and rec.get("variant") == variant):
             return rec.get("bytes_per_launch")
     return None
 
