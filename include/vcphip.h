@@ -80,9 +80,14 @@ enum {
     VCPHIP_PARAM_OZ_RADIUS = 5,    /* radius-digit scheme of the Ozaki form:
                                       VCPHIP_OZ_RADIUS_* (0 = tuned, else full for
                                       k < 4096 and folded from k = 4096) */
-    VCPHIP_PARAM_BATCH_TILES = 6   /* tile shape of the batched launches for every
+    VCPHIP_PARAM_BATCH_TILES = 6,  /* tile shape of the batched launches for every
                                       problem: VCPHIP_BATCH_TILES_* (0 = tuned per
                                       size class, else 64 x 64) */
+    VCPHIP_PARAM_PLANE_ROWS = 7    /* output rows per tile of the plain single
+                                      products: 64 or 32 (0 = 32 up to 4 64-row
+                                      tiles per CU for interval x interval, 3 for
+                                      the other forms, else 64); the bits of
+                                      every output are the same in both */
 };
 
 /* tile shapes of the batched persistent launches (vcphip_igemm_batched*,

@@ -6,7 +6,10 @@ the rows between, and outputs whose padding rows hold a sentinel: the
 results must be the oracle's restatements bit for bit (pidblas::mulmm,
 mul_im_m, mul_m_im, mulltmm, vmulmm, rmatmul / udmatmul; vcp/pidblas.hpp,
 vblas/rmatmul.hpp:100-127, vblas/udmatmul.hpp:738-760), the NaN never read
-and the sentinel never written.  k = 0 is included."""
+and the sentinel never written.  k = 0 is included, and both tile heights
+(64 and 32 output rows) of the plane-fed kernel."""
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -54,9 +57,19 @@ def _ok(rc, name):
     assert rc == 0, "%s returned %d" % (name, rc)
 
 
+@pytest.fixture(params=[64, 32])
+def plane_rows(ctx, request):
+    """Both tile heights of the plane-fed products (the automatic choice
+    takes 32 rows for grids this small)."""
+    L = vcp_amd.lib()
+    assert L.vcphip_ctx_set_param(ctx._h, vcp_amd.PARAM_PLANE_ROWS, request.param) == 0
+    yield request.param
+    assert L.vcphip_ctx_set_param(ctx._h, vcp_amd.PARAM_PLANE_ROWS, 0) == 0
+
+
 @pytest.mark.parametrize("m,n,k", SHAPES)
-def test_planes_padded_leading_dimensions(ctx, m, n, k):
-    print("planes m=%d n=%d k=%d" % (m, n, k), flush=True)
+def test_planes_padded_leading_dimensions(ctx, plane_rows, m, n, k):
+    print("planes rows=%d m=%d n=%d k=%d" % (plane_rows, m, n, k), flush=True)
     L = vcp_amd.lib()
     rng = np.random.default_rng(m * 7919 + n * 31 + k)
     lda, ldb, ldc = m + PAD, k + PAD, m + PAD
@@ -117,3 +130,12 @@ def test_planes_symmetric_forms_padded(ctx, k, n):
     _ok(L.vcphip_vgemm_tn_dev(ctx._h, k, n, _dev_pt(A, lda).data_ptr(), lda, C.data_ptr(), ldc), "vgemm_tn_dev")
     ctx.synchronize()
     assert np.array_equal(_read_iv(C, n), O.pidblas_vmulmm(A))
+
+
+def test_plane_rows_parameter(ctx):
+    L = vcp_amd.lib()
+    v = ctypes.c_longlong(-1)
+    assert L.vcphip_ctx_get_param(ctx._h, vcp_amd.PARAM_PLANE_ROWS, ctypes.byref(v)) == 0 and v.value == 0
+    for bad in (16, 48, 128, -1):
+        assert L.vcphip_ctx_set_param(ctx._h, vcp_amd.PARAM_PLANE_ROWS, bad) == -2
+    assert L.vcphip_ctx_get_param(ctx._h, vcp_amd.PARAM_PLANE_ROWS, ctypes.byref(v)) == 0 and v.value == 0

@@ -30,6 +30,7 @@ PARAM_STAGE_TIMING = 3
 PARAM_DD_NMOD = 4
 PARAM_OZ_RADIUS = 5
 PARAM_BATCH_TILES = 6
+PARAM_PLANE_ROWS = 7
 BATCH_TILES_64 = 1
 BATCH_TILES_EDGE = 2
 BATCH_TILES_32 = 3

@@ -378,47 +378,56 @@ __global__ __launch_bounds__(256) void k_planes_bt(int k, int n, int kpad, int n
   }
 }
 
-// interval x interval tiles hold four chains per output (2 waves per SIMD),
-// point x point two (4), the mixed forms three (3)
-template <bool AI, bool BI>
-__global__ __launch_bounds__(NT, (AI && BI) ? 2 : (AI || BI) ? 3 : 4) void k_drgemm_planes(
-    int m, int n, int mpad, int npad, int kpad, const double* __restrict__ A0, const double* __restrict__ A1,
-    const double* __restrict__ B0, const double* __restrict__ B1, double* __restrict__ C, long ldc,
-    double* __restrict__ C2, int accumulate) {
+// Tiles of 16 TMR x 64 outputs: TMR = 4 (64 rows) for grids that fill the
+// device, TMR = 2 (32 rows) for small ones, which would leave most CUs one
+// workgroup (one wave per SIMD) or idle.  Interval x interval tiles hold four
+// chains per output, point x point two, the mixed forms three; the waves per
+// SIMD follow (64 rows: 2 / 4 / 3, 32 rows: 3 / 4 / 4).
+template <bool AI, bool BI, int TMR>
+__global__ __launch_bounds__(NT, TMR == 4 ? ((AI && BI) ? 2 : (AI || BI) ? 3 : 4) : ((AI && BI) ? 3 : 4))
+void k_drgemm_planes(int m, int n, int mpad, int npad, int kpad, const double* __restrict__ A0,
+                     const double* __restrict__ A1, const double* __restrict__ B0, const double* __restrict__ B1,
+                     double* __restrict__ C, long ldc, double* __restrict__ C2, int accumulate) {
   constexpr bool II = AI && BI;
-  constexpr int NA = AI ? 2 : 1, NP = NA + (BI ? 2 : 1);
+  constexpr int TBM = 16 * TMR;                  // tile rows
+  constexpr int NA = AI ? 2 : 1, NB = BI ? 2 : 1, NP = NA + NB;
+  constexpr int ASL = BK * TBM, BSL = BK * BN;   // doubles of one plane's k-slice
+  constexpr int STAGE = NA * ASL + NB * BSL;
+  static_assert(TMR == 4 || TMR == 2, "tile rows");
+  static_assert(BN == 64 && BK == 16, "slice layout");
   vcphip_mode_round_up();
-  // [stage][plane: A mid (or A), A rad, B mid (or B), B rad][k][64]
-  __shared__ __attribute__((aligned(16))) double sL[2][NP][BK][BM];
-  static_assert(BM == BN && BM == 64 && BK == 16, "slice layout");
-  const int ntm = mpad / BM, ntn = npad / BN;
+  // [stage][A mid (or A), A rad: k x TBM][B mid (or B), B rad: k x 64]
+  __shared__ __attribute__((aligned(16))) double sL[2 * STAGE];
+  const int ntm = mpad / TBM, ntn = npad / BN;
   const int tile = vcphip_xcd_remap(blockIdx.x, ntm * ntn);
   int tm, tn;
   vcphip_tile_coords(tile, ntm, ntn, 16, tm, tn);
-  const int row0 = tm * BM, col0 = tn * BN;
+  const int row0 = tm * TBM, col0 = tn * BN;
   const int tid = threadIdx.x, lane = tid & 63;
-  // wave w < NP loads plane w of every slice: 8 loads of two k-rows
-  // (2 x 512 B); lane l takes k-row (l >> 5) of the pair, doubles 2 (l & 31)
-  // and + 1
+  // wave w < NP loads plane w of every slice, one k-row of a plane slice
+  // per 2^lh lanes (two doubles a lane): a load instruction moves 1 KB, i.e.
+  // 64 / 2^lh k-rows, and a slice takes BK 2^lh / 64 of them
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const bool loader = w < NP;
   const bool wa = w < NA;
+  const int lh = (wa && TBM == 32) ? 4 : 5;
+  const int nq = BK >> (6 - lh), rpi = 64 >> lh;
   const double* plane = wa ? (w == 0 ? A0 : A1) : (w == NA ? B0 : B1);
   const long ld = wa ? mpad : npad;
-  const double* src = plane + (long)(lane >> 5) * ld + (wa ? row0 : col0) + 2 * (lane & 31);
-  const uint32_t lds0 = vcphip_lds_addr(&sL[0][0][0][0]) + (uint32_t)(loader ? w : 0) * (BK * BM * 8);
+  const double* src = plane + (long)(lane >> lh) * ld + (wa ? row0 : col0) + 2 * (lane & ((1 << lh) - 1));
+  const int poff = !loader ? 0 : wa ? w * ASL : NA * ASL + (w - NA) * BSL;
+  const uint32_t lds0 = vcphip_lds_addr(sL) + (uint32_t)poff * 8;
   auto issue = [&](int stage, int t) {
     if (!loader) return;
     const double* g = src + (long)t * BK * ld;
-    const uint32_t d = lds0 + (uint32_t)stage * (NP * BK * BM * 8);
-#pragma unroll
-    for (int q = 0; q < BK / 2; ++q) vcphip_lds_dma16(g + (long)(2 * q) * ld, d + (uint32_t)q * (2 * BM * 8));
+    const uint32_t d = lds0 + (uint32_t)stage * (STAGE * 8);
+    for (int q = 0; q < nq; ++q) vcphip_lds_dma16(g + (long)(rpi * q) * ld, d + (uint32_t)q * 1024);
   };
 
   const int tx = tid % 16, ty = tid / 16;
-  double up[TM][TN], dn[TM][TN], r1[TM][TN], r2[TM][TN];
+  double up[TMR][TN], dn[TMR][TN], r1[TMR][TN], r2[TMR][TN];
 #pragma unroll
-  for (int i = 0; i < TM; ++i)
+  for (int i = 0; i < TMR; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
       up[i][j] = 0.0;
@@ -429,10 +438,10 @@ __global__ __launch_bounds__(NT, (AI && BI) ? 2 : (AI || BI) ? 3 : 4) void k_drg
   if (!AI && !BI && accumulate) {
     // udmatmul form: the chains continue from the values already in CU / CD
 #pragma unroll
-    for (int i = 0; i < TM; ++i)
+    for (int i = 0; i < TMR; ++i)
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
-        const int r = row0 + ty * TM + i, c = col0 + dr_col(tx, j);
+        const int r = row0 + ty * TMR + i, c = col0 + dr_col(tx, j);
         if (r < m && c < n) {
           up[i][j] = C[r + ldc * c];
           dn[i][j] = -C2[r + ldc * c];
@@ -448,33 +457,39 @@ __global__ __launch_bounds__(NT, (AI && BI) ? 2 : (AI || BI) ? 3 : 4) void k_drg
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
     if (t + 1 < nt) issue((t + 1) & 1, t + 1);
-    const double(*sAm)[BM] = sL[t & 1][0];
-    const double(*sAr)[BM] = sL[t & 1][AI ? 1 : 0];
-    const double(*sBm)[BM] = sL[t & 1][NA];
-    const double(*sBr)[BM] = sL[t & 1][BI ? NA + 1 : NA];
+    const double* sAm = sL + (t & 1) * STAGE;
+    const double* sAr = sAm + (AI ? ASL : 0);
+    const double* sBm = sAm + NA * ASL;
+    const double* sBr = sBm + (BI ? BSL : 0);
 #pragma unroll 2
     for (int kk = 0; kk < BK; ++kk) {
-      double am[TM], ar[TM], bm[TN], br[TN];
+      double am[TMR], ar[TMR], bm[TN], br[TN];
+#pragma unroll
+      for (int h = 0; h < TMR / 2; ++h) {
+        const double2 x = *reinterpret_cast<const double2*>(&sAm[kk * TBM + ty * TMR + 2 * h]);
+        am[2 * h] = x.x;
+        am[2 * h + 1] = x.y;
+      }
       {
-        const double2 x0 = *reinterpret_cast<const double2*>(&sAm[kk][ty * TM]);
-        const double2 x1 = *reinterpret_cast<const double2*>(&sAm[kk][ty * TM + 2]);
-        am[0] = x0.x; am[1] = x0.y; am[2] = x1.x; am[3] = x1.y;
-        const double2 y0 = *reinterpret_cast<const double2*>(&sBm[kk][2 * tx]);
-        const double2 y1 = *reinterpret_cast<const double2*>(&sBm[kk][2 * tx + 32]);
+        const double2 y0 = *reinterpret_cast<const double2*>(&sBm[kk * BN + 2 * tx]);
+        const double2 y1 = *reinterpret_cast<const double2*>(&sBm[kk * BN + 2 * tx + 32]);
         bm[0] = y0.x; bm[1] = y0.y; bm[2] = y1.x; bm[3] = y1.y;
       }
       if (AI) {
-        const double2 z0 = *reinterpret_cast<const double2*>(&sAr[kk][ty * TM]);
-        const double2 z1 = *reinterpret_cast<const double2*>(&sAr[kk][ty * TM + 2]);
-        ar[0] = z0.x; ar[1] = z0.y; ar[2] = z1.x; ar[3] = z1.y;
+#pragma unroll
+        for (int h = 0; h < TMR / 2; ++h) {
+          const double2 z = *reinterpret_cast<const double2*>(&sAr[kk * TBM + ty * TMR + 2 * h]);
+          ar[2 * h] = z.x;
+          ar[2 * h + 1] = z.y;
+        }
       }
       if (BI) {
-        const double2 u0 = *reinterpret_cast<const double2*>(&sBr[kk][2 * tx]);
-        const double2 u1 = *reinterpret_cast<const double2*>(&sBr[kk][2 * tx + 32]);
+        const double2 u0 = *reinterpret_cast<const double2*>(&sBr[kk * BN + 2 * tx]);
+        const double2 u1 = *reinterpret_cast<const double2*>(&sBr[kk * BN + 2 * tx + 32]);
         br[0] = u0.x; br[1] = u0.y; br[2] = u1.x; br[3] = u1.y;
       }
 #pragma unroll
-      for (int i = 0; i < TM; ++i) {
+      for (int i = 0; i < TMR; ++i) {
         double as = 0.0;
         if (II) as = __builtin_fabs(am[i]) + ar[i];   // (|mA| + rA), RU (:277)
 #pragma unroll
@@ -499,8 +514,8 @@ __global__ __launch_bounds__(NT, (AI && BI) ? 2 : (AI || BI) ? 3 : 4) void k_drg
     const int c = col0 + dr_col(tx, j);
     if (c >= n) continue;
 #pragma unroll
-    for (int i = 0; i < TM; ++i) {
-      const int r = row0 + ty * TM + i;
+    for (int i = 0; i < TMR; ++i) {
+      const int r = row0 + ty * TMR + i;
       if (r >= m) continue;
       if (AI || BI) {
         const double tt = II ? r1[i][j] + r2[i][j] : r1[i][j];   // temp (:283, :129, :185)
@@ -1033,18 +1048,29 @@ int vcphip_launch_planes(int kind, int m, int n, int k, const double* A, long ld
 }
 
 int vcphip_launch_drgemm_planes(int kind, int m, int n, int k, const double* A0, const double* A1, const double* B0,
-                                const double* B1, double* C, long ldc, double* C2, int accumulate, hipStream_t s) {
+                                const double* B1, double* C, long ldc, double* C2, int accumulate, int rows,
+                                hipStream_t s) {
   const int mpad = (m + BM - 1) / BM * BM, npad = (n + BN - 1) / BN * BN, kpad = (k + BK - 1) / BK * BK;
   if (m <= 0 || n <= 0) return 0;
-  if (kpad == 0) return -1;   // k = 0 takes the tile path
-  const dim3 grid((mpad / BM) * (npad / BN)), block(NT);
+  if (kpad == 0 || (rows != 64 && rows != 32)) return -1;   // k = 0 takes the tile path
+  const dim3 grid((mpad / rows) * (npad / BN)), block(NT);
+#define VCPHIP_PLANES_CASE(KIND, AI, BI)                                                                          \
+  case KIND:                                                                                                     \
+    if (rows == 64)                                                                                              \
+      hipLaunchKernelGGL((k_drgemm_planes<AI, BI, 4>), grid, block, 0, s, m, n, mpad, npad, kpad, A0, A1, B0, B1, \
+                         C, ldc, C2, accumulate);                                                                \
+    else                                                                                                         \
+      hipLaunchKernelGGL((k_drgemm_planes<AI, BI, 2>), grid, block, 0, s, m, n, mpad, npad, kpad, A0, A1, B0, B1, \
+                         C, ldc, C2, accumulate);                                                                \
+    break;
   switch (kind) {
-    case VCPHIP_DR_II: hipLaunchKernelGGL((k_drgemm_planes<true, true>), grid, block, 0, s, m, n, mpad, npad, kpad, A0, A1, B0, B1, C, ldc, C2, accumulate); break;
-    case VCPHIP_DR_IP: hipLaunchKernelGGL((k_drgemm_planes<true, false>), grid, block, 0, s, m, n, mpad, npad, kpad, A0, A1, B0, B1, C, ldc, C2, accumulate); break;
-    case VCPHIP_DR_PI: hipLaunchKernelGGL((k_drgemm_planes<false, true>), grid, block, 0, s, m, n, mpad, npad, kpad, A0, A1, B0, B1, C, ldc, C2, accumulate); break;
-    case VCPHIP_DR_PP: hipLaunchKernelGGL((k_drgemm_planes<false, false>), grid, block, 0, s, m, n, mpad, npad, kpad, A0, A1, B0, B1, C, ldc, C2, accumulate); break;
+    VCPHIP_PLANES_CASE(VCPHIP_DR_II, true, true)
+    VCPHIP_PLANES_CASE(VCPHIP_DR_IP, true, false)
+    VCPHIP_PLANES_CASE(VCPHIP_DR_PI, false, true)
+    VCPHIP_PLANES_CASE(VCPHIP_DR_PP, false, false)
     default: return -1;
   }
+#undef VCPHIP_PLANES_CASE
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

@@ -32,7 +32,7 @@ struct vcphip_ctx {
   int device = 0;
   hipStream_t own = nullptr;
   hipStream_t stream = nullptr;
-  long long oz_nmod = 0, oz_variant = 0, timing = 0, dd_nmod = 0, oz_radius = 0, batch_tiles = 0;
+  long long oz_nmod = 0, oz_variant = 0, timing = 0, dd_nmod = 0, oz_radius = 0, batch_tiles = 0, plane_rows = 0;
   // workspace slots (device memory, grown on demand)
   enum { WS_PA, WS_PB, WS_U, WS_Z, WS_SA, WS_SB, WS_HA, WS_HB, WS_HC, WS_HC2, WS_BATCH, WS_DD, WS_LSS, WS_HR, WS_LSSB,
          WS_INV, WS_DDS, WS_IGB, WS_TT, WS_MR, WS_Q, WS_N };
@@ -576,6 +576,10 @@ extern "C" int vcphip_ctx_set_param(vcphip_ctx* c, int key, long long v) {
       }
       c->batch_tiles = v;
       return VCPHIP_OK;
+    case VCPHIP_PARAM_PLANE_ROWS:
+      if (v != 0 && v != 32 && v != 64) return VCPHIP_ERR_ARGUMENT;
+      c->plane_rows = v;
+      return VCPHIP_OK;
     default: return VCPHIP_ERR_ARGUMENT;
   }
 }
@@ -589,6 +593,7 @@ extern "C" int vcphip_ctx_get_param(vcphip_ctx* c, int key, long long* v) {
     case VCPHIP_PARAM_DD_NMOD: *v = c->dd_nmod; return VCPHIP_OK;
     case VCPHIP_PARAM_OZ_RADIUS: *v = c->oz_radius; return VCPHIP_OK;
     case VCPHIP_PARAM_BATCH_TILES: *v = c->batch_tiles; return VCPHIP_OK;
+    case VCPHIP_PARAM_PLANE_ROWS: *v = c->plane_rows; return VCPHIP_OK;
     default: return VCPHIP_ERR_ARGUMENT;
   }
 }
@@ -791,6 +796,20 @@ extern "C" int vcphip_igemm_ozaki_noalloc_dev(vcphip_ctx* c, int m, int n, int k
 
 namespace {
 
+// Tile rows of a plane-fed product.  The 32-row tiles give a CU more
+// resident waves (3 against 2 per SIMD for interval x interval, 4-6 against
+// 3-5 for the other forms) and a finer last round of tiles, at 1.5x the LDS
+// reads per FMA; measured on MI355X they are the faster choice up to about 4
+// (interval x interval) or 3 (the other forms) 64-row tiles per CU: n = 2048
+// interval x interval 1.43 against 1.55 ms, n = 1024 0.21 against 0.25 ms,
+// while from n = 3072 the 64-row tiles are 3-8% faster.
+int plane_rows(const vcphip_ctx* c, int kind, size_t mpad, size_t npad) {
+  if (c->plane_rows) return (int)c->plane_rows;
+  const size_t cus = c->num_cu > 0 ? (size_t)c->num_cu : 256;
+  const size_t per_cu = kind == VCPHIP_DR_II ? 4 : 3;
+  return (mpad / 64) * (npad / 64) <= per_cu * cus ? 32 : 64;
+}
+
 // A single product of kind VCPHIP_DR_*: the operand planes (midpoint and
 // radius of an interval operand, the point operand itself), padded and laid
 // out for whole k-slices, then the product from them.  k = 0 (an all-zero or,
@@ -819,7 +838,7 @@ int planes_product(vcphip_ctx* c, int kind, int m, int n, int k, const double* A
   T.mark("planes");
   if (rc) return rc;
   rc = check_launch(vcphip_launch_drgemm_planes(kind, m, n, k, A0, ai ? A1 : nullptr, B0, bi ? B1 : nullptr, C, ldc,
-                                                C2, accumulate, c->stream));
+                                                C2, accumulate, plane_rows(c, kind, mpad, npad), c->stream));
   T.mark("drgemm");
   return rc;
 }

@@ -16,12 +16,13 @@ int vcphip_launch_isgemm(int m, int n, int k, const double* A, long lda, const d
 // the single products in two steps: planes of A (column-major, mpad x kpad;
 // mid and rad of an interval A, or A itself) and of B (row-major, kpad x
 // npad; mid and rad, or B), zero-padded to mpad, npad = ceil64, kpad =
-// ceil16, then the product from them (k > 0).  A1 / B1 are unused for a
-// point operand.
+// ceil16, then the product from them (k > 0) in tiles of `rows` (64 or 32)
+// x 64 outputs.  A1 / B1 are unused for a point operand.
 int vcphip_launch_planes(int kind, int m, int n, int k, const double* A, long lda, const double* B, long ldb,
                          double* A0, double* A1, double* B0, double* B1, hipStream_t s);
 int vcphip_launch_drgemm_planes(int kind, int m, int n, int k, const double* A0, const double* A1, const double* B0,
-                                const double* B1, double* C, long ldc, double* C2, int accumulate, hipStream_t s);
+                                const double* B1, double* C, long ldc, double* C2, int accumulate, int rows,
+                                hipStream_t s);
 // A^T A (interval or point, A k x n) from the planes of A (made here into
 // P0 / P1, each kpad x npad doubles; P1 unused for a point A); k > 0
 int vcphip_launch_drsyrk_planes(int interval, int k, int n, const double* A, long lda, double* P0, double* P1,
