@@ -317,11 +317,12 @@ __global__ __launch_bounds__(256) void k_dd_recon(int m, int n, int mpad, int np
                                                   const uint8_t* __restrict__ U, double* __restrict__ C, long ldc) {
   // threads follow the blocked 16 x 16 layout of the residue planes
   // grid: x = 16-row block, y = 16-column block (the plane's block order)
-  const int o = threadIdx.x;
-  const long boff = ((long)blockIdx.y * (mpad >> 4) + blockIdx.x) * 256 + o;
-  const int i = (int)blockIdx.x * 16 + 4 * (o >> 6) + (o & 3);
-  const int j = (int)blockIdx.y * 16 + ((o >> 2) & 15);
-  if (i >= m || j >= n) return;
+  // the block's planes from a uniform base plus the thread's offset o
+  const unsigned o = threadIdx.x;
+  const long bbase = ((long)blockIdx.y * (mpad >> 4) + blockIdx.x) * 256;
+  const unsigned i = blockIdx.x * 16 + 4 * (o >> 6) + (o & 3);
+  const unsigned j = blockIdx.y * 16 + ((o >> 2) & 15);
+  if (i >= (unsigned)m || j >= (unsigned)n) return;
   if (fa[i] || fb[j]) {   // a zero row / column gives 0, a non-finite one NaN
     const double v = ((fa[i] | fb[j]) & 2) ? __builtin_nan("") : 0.0;
     *reinterpret_cast<double2*>(C + 2 * (i + ldc * (long)j)) = make_double2(v, v);
@@ -329,7 +330,7 @@ __global__ __launch_bounds__(256) void k_dd_recon(int m, int n, int mpad, int np
   }
   const long plane = (long)mpad * npad;
   unsigned mag[NL];
-  const bool neg = vcphip_crt_symmetric<N, NL>(U, plane, boff, P, mag);
+  const bool neg = vcphip_crt_symmetric<N, NL>(U + bbase, plane, o, P, mag);
   const int sc = ea[i] + eb[j] - 2 * P.beta;
   int q;
   bool up;

@@ -1102,15 +1102,18 @@ __global__ __launch_bounds__(256) void k_oz_recon(int m, int n, int k, int mpad,
   // residue reads): offset o in a block holds row 4 (o / 64) + o % 4 and
   // column (o / 4) % 16 of that block
   // grid: x = 16-row block, y = 16-column block (the plane's block order)
-  const int o = threadIdx.x;
-  const long boff = ((long)blockIdx.y * (mpad >> 4) + blockIdx.x) * 256 + o;
-  const int i = (int)blockIdx.x * 16 + 4 * (o >> 6) + (o & 3);
-  const int j = (int)blockIdx.y * 16 + ((o >> 2) & 15);
-  if (i >= m || j >= n) return;
+  // the block's planes are addressed from a uniform base (Ub, Zb) plus the
+  // thread's offset o, and the row / column tables by unsigned indices
+  const unsigned o = threadIdx.x;
+  const long bbase = ((long)blockIdx.y * (mpad >> 4) + blockIdx.x) * 256;
+  const unsigned i = blockIdx.x * 16 + 4 * (o >> 6) + (o & 3);
+  const unsigned j = blockIdx.y * 16 + ((o >> 2) & 15);
+  if (i >= (unsigned)m || j >= (unsigned)n) return;
   const long plane = (long)mpad * npad;
+  const int32_t* __restrict__ Zb = Z + bbase;
 
   unsigned mag[NL];
-  const bool neg = vcphip_crt_symmetric<N, NL>(U, plane, boff, P, mag);
+  const bool neg = vcphip_crt_symmetric<N, NL>(U + bbase, plane, o, P, mag);
   const int fa = sa.flags[i], fb = sb.flags[j];
   const bool zr = (fa & VCPHIP_OZ_MIDZERO) != 0, zc = (fb & VCPHIP_OZ_MIDZERO) != 0;
   const int ei = sa.e[i], fj = sb.e[j];
@@ -1131,12 +1134,15 @@ __global__ __launch_bounds__(256) void k_oz_recon(int m, int n, int k, int mpad,
   // numerator times the RU or, when negative, the RD reciprocal.
   double t1 = 0.0, t2 = 0.0;
   const bool ceil_only = P.rscheme == VCPHIP_OZ_RADIUS_CEIL, full = P.rscheme == VCPHIP_OZ_RADIUS_FULL;
+  auto zload = [&](int z) {   // digit product z of this element
+    return (int)__builtin_amdgcn_raw_buffer_load_b32(vcphip_rsrc(Zb + z * plane, 1024), 4 * o, 0, 0);
+  };
   auto digit_sum = [&](int zh, int zm, int zl) {
-    if (ceil_only) return (double)Z[zh * plane + boff];
-    const int mx = Z[zm * plane + boff];
-    double u = (double)Z[zh * plane + boff] + (double)mx * (mx >= 0 ? P.c1 : P.c1dn);
+    if (ceil_only) return (double)zload(zh);
+    const int mx = zload(zm);
+    double u = (double)zload(zh) + (double)mx * (mx >= 0 ? P.c1 : P.c1dn);
     if (full) {
-      const int ll = Z[zl * plane + boff];
+      const int ll = zload(zl);
       u = u + (double)ll * (ll >= 0 ? P.c127 : P.c127dn);
     }
     return u;

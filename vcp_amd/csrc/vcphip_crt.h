@@ -70,12 +70,15 @@ VCPHIP_DEVINL bool vcphip_crt_from_residues(const unsigned (&u)[N], const Vcphip
   return neg;
 }
 
-// Residues of element `off` from N planes of `plane` bytes
+// Residues of element o (< 256) of a 16 x 16 plane block from N planes of
+// `plane` bytes, U being the block's base in the first plane (workgroup-
+// uniform): one buffer load per plane, from a scalar resource at the lane's
+// offset.
 template <int N, int NL>
-VCPHIP_DEVINL bool vcphip_crt_symmetric(const uint8_t* __restrict__ U, long plane, long off,
+VCPHIP_DEVINL bool vcphip_crt_symmetric(const uint8_t* __restrict__ U, long plane, unsigned o,
                                         const VcphipOzParams& P, unsigned (&mag)[NL]) {
   unsigned u[N];
 #pragma unroll
-  for (int t = 0; t < N; ++t) u[t] = U[(long)t * plane + off];
+  for (int t = 0; t < N; ++t) u[t] = __builtin_amdgcn_raw_buffer_load_b8(vcphip_rsrc(U + (long)t * plane, 256), o, 0, 0);
   return vcphip_crt_from_residues<N, NL>(u, P, mag);
 }

@@ -30,6 +30,14 @@ VCPHIP_DEVINL void vcphip_lds_dma16(const void* g, uint32_t lds_base) {
   asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g), "s"(lds_base) : "memory", "m0");
 }
 
+// Raw buffer resource (stride 0, gfx9 word 3) over `bytes` bytes from a
+// wave-uniform base: loads through it take a 32-bit lane offset in bytes and
+// need no per-lane 64-bit address arithmetic; offsets at or past `bytes`
+// read 0.
+VCPHIP_DEVINL __amdgpu_buffer_rsrc_t vcphip_rsrc(const void* base, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, bytes, 0x00020000);
+}
+
 VCPHIP_DEVINL void vcphip_mode_round_up() {
   asm volatile("s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 4), 5" ::: "memory");
 }
