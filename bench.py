@@ -52,6 +52,9 @@ def main():
     ap.add_argument("--no-dd", action="store_true")
     ap.add_argument("--dd-n", type=int, default=4096)
     ap.add_argument("--dd-steps", type=int, default=5)
+    ap.add_argument("--no-power", action="store_true",
+                    help="skip the sustained runs that read socket power and GFX clock with amd-smi")
+    ap.add_argument("--power-seconds", type=float, default=5.0)
     ap.add_argument("--variant", type=int, default=0,
                     help="int8 GEMM tile variant of the headline Ozaki product (0: the library default); "
                          "variants are bit-identical, so only the time changes")
@@ -225,6 +228,27 @@ def main():
                           "tile variant %d)" % (nmod, nprod - nmod, scheme, plan["variant"]),
                 "ops_per_launch": ops, "ms_per_launch": round(t_gemm, 4)}
 
+    # ---- power and clock under sustained load -------------------------------
+    # The published peaks assume the 2400 MHz maximum GFX clock; under a
+    # sustained product the board holds its socket power near the cap and
+    # the clock drops.  Both readings (amd-smi) are taken while the headline
+    # product, then the plain one, runs back to back; the GEMM stage's
+    # fraction is also given against the int8 peak at the clock it ran at.
+    power = None
+    if not args.no_power:
+        power = {"ozaki_headline": sustained_power(lambda: run(hctx, vcp_amd.FORM_OZAKI, dH), hctx.synchronize,
+                                                   args.power_seconds),
+                 "plain_fp64": sustained_power(lambda: run(ctx, vcp_amd.FORM_PLAIN, dP), ctx.synchronize,
+                                               args.power_seconds)}
+        power.update(power_limits())
+        oz_p = power["ozaki_headline"]
+        if roof and oz_p and oz_p.get("gfx_mhz") and power.get("max_gfx_mhz"):
+            peak_clk = PEAK_INT8_TOPS * oz_p["gfx_mhz"] / power["max_gfx_mhz"]
+            roof["gfx_mhz_sustained"] = oz_p["gfx_mhz"]
+            roof["peak_at_sustained_clock"] = round(peak_clk, 1)
+            roof["frac_at_sustained_clock"] = round(roof["achieved"] / peak_clk, 4)
+        log("power %s" % power)
+
     # ---- CPU baseline: pidblas::mulmm on the reference's own vblas rmatmul
     # (oracle/_ref, built from the reference sources) when present, else the
     # oracle's restatement; a row sample of the same workload -----------------
@@ -303,9 +327,79 @@ def main():
         "plain_n4096": cfg2,
         "galerkin_uhphiphi": galerkin,
         "igemm_batched": batched,
+        "power_clock": power,
     }
     print(json.dumps(line), flush=True)
     ctx.close()
+
+
+def _smi(args, tool="amd-smi"):
+    import shutil
+    import subprocess
+
+    exe = shutil.which(tool)
+    if not exe:
+        return ""
+    try:
+        return subprocess.run([exe] + args, capture_output=True, text=True, timeout=20).stdout
+    except Exception:
+        return ""
+
+
+def sustained_power(step, sync, seconds, period=0.25):
+    """Socket power (W) and mean GFX clock over the XCDs (MHz), read with
+    amd-smi every `period` s while step() runs back to back for `seconds`:
+    medians of the samples after the first second (the clocks settle).  None
+    when amd-smi is absent or reads nothing."""
+    import re
+    import statistics
+    import threading
+
+    samples = []
+    done = threading.Event()
+
+    def poll():
+        while not done.is_set():
+            out = _smi(["metric", "--power", "--clock"])
+            pw = re.search(r"SOCKET_POWER:\s*([0-9.]+)", out)
+            clks = [int(x) for x in re.findall(r"GFX_\d+:\s*\n\s*CLK:\s*(\d+)", out)]
+            if pw and clks:
+                samples.append((time.perf_counter(), float(pw.group(1)), sum(clks) / len(clks)))
+            done.wait(period)
+
+    th = threading.Thread(target=poll, daemon=True)
+    t0 = time.perf_counter()
+    th.start()
+    calls = 0
+    while time.perf_counter() - t0 < seconds:
+        step()
+        calls += 1
+        if calls % 4 == 0:
+            sync()
+    sync()
+    done.set()
+    th.join()
+    use = [s for s in samples if s[0] - t0 > 1.0]
+    if not use:
+        return None
+    return {"socket_power_w": statistics.median(s[1] for s in use),
+            "gfx_mhz": round(statistics.median(s[2] for s in use)), "samples": len(use), "calls": calls,
+            "seconds": round(time.perf_counter() - t0, 2)}
+
+
+def power_limits():
+    """The maximum GFX clock (MHz, amd-smi) and the maximum graphics package
+    power (W, rocm-smi), as the tools report them (empty when they do not)."""
+    import re
+
+    out = {}
+    m = re.search(r"MAX_CLK:\s*(\d+)", _smi(["metric", "--clock"]))
+    if m:
+        out["max_gfx_mhz"] = int(m.group(1))
+    m = re.search(r"Max Graphics Package Power \(W\)\"?:\s*\"?([0-9.]+)", _smi(["-M", "--json"], tool="rocm-smi"))
+    if m:
+        out["power_cap_w"] = float(m.group(1))
+    return out
 
 
 def measured_traffic(n, nmod, nprod, variant):
