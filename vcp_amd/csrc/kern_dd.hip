@@ -120,32 +120,24 @@ VCPHIP_DEVINL void dd_scaled(double hi, double lo, int s, bool& neg, u128& mag) 
   if (mag == 0) neg = false;
 }
 
-typedef unsigned short vcphip_v2u16 __attribute__((ext_vector_type(2)));
-
-// the magnitude (< 2^120) as ten 12-bit chunks, held as five 16-bit pairs
-VCPHIP_DEVINL void chunks12x10(u128 mag, unsigned (&p)[5]) {
+// the magnitude (< 2^120) as its four 32-bit words, low first
+VCPHIP_DEVINL void words4(u128 mag, unsigned (&p)[4]) {
 #pragma unroll
-  for (int i = 0; i < 5; ++i) {
-    const unsigned c0 = (unsigned)(mag >> (24 * i)) & 0xFFFu;
-    const unsigned c1 = (unsigned)(mag >> (24 * i + 12)) & 0xFFFu;
-    p[i] = c0 | (c1 << 16);
-  }
+  for (int i = 0; i < 4; ++i) p[i] = (unsigned)(mag >> (32 * i));
 }
 
 // the residue of (neg ? -mag : mag) in the symmetric int8 range: y = mag +
-// floor(m/2) (mod m) from five 16-bit dot products (y < 2^23.4, exact in
-// FP32); q = trunc(y * r_rcp) is floor(y / m) or one less for m >= 16 in
-// either rounding mode (r_rcp is biased low), so r = y - q m lies in [0, 2m)
-// and one unsigned minimum brings it into [0, m); c = r - floor(m/2) is the
-// centred residue of mag, and -c that of -mag (for m = 256, c = -128 gives
-// 128, the same int8 byte).
-VCPHIP_DEVINL int dd_residue12(const unsigned (&p)[5], bool neg, int t, const VcphipOzParams& P) {
+// floor(m/2) (mod m) from four byte dot products of mag's 16 bytes with the
+// weights 256^i mod m (y < 2^20, exact in FP32); q = trunc(y * r_rcp) is
+// floor(y / m) or one less for m >= 16 in either rounding mode (r_rcp is
+// biased low), so r = y - q m lies in [0, 2m) and one unsigned minimum
+// brings it into [0, m); c = r - floor(m/2) is the centred residue of mag,
+// and -c that of -mag (for m = 256, c = -128 gives 128, the same int8 byte).
+VCPHIP_DEVINL int dd_residue8(const unsigned (&p)[4], bool neg, int t, const VcphipOzParams& P) {
   const unsigned m = (unsigned)P.mod[t];
   unsigned y = m >> 1;
 #pragma unroll
-  for (int i = 0; i < 5; ++i)
-    y = __builtin_amdgcn_udot2(__builtin_bit_cast(vcphip_v2u16, p[i]), __builtin_bit_cast(vcphip_v2u16, P.dd_wp[i][t]),
-                               y, false);
+  for (int i = 0; i < 4; ++i) y = __builtin_amdgcn_udot4(p[i], P.dd_wb[t][i], y, false);
   const unsigned q = (unsigned)((float)y * P.r_rcp[t]);
   unsigned r = y - __umul24(q, m);
   const unsigned r2 = r - m;
@@ -177,19 +169,19 @@ __global__ __launch_bounds__(256) void k_dd_split_rows(const double* __restrict_
     const int kk = k0 + lk0 + u;
     v[u] = (rowok && kk < k) ? *reinterpret_cast<const double2*>(A + 2 * (r + lda * kk)) : make_double2(0.0, 0.0);
   }
-  unsigned c[4][5];
+  unsigned c[4][4];
   bool neg[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     u128 mag;
     dd_scaled(v[u].x, v[u].y, sc, neg[u], mag);
-    chunks12x10(mag, c[u]);
+    words4(mag, c[u]);
   }
 #pragma unroll
   for (int t = 0; t < N; ++t) {
     unsigned word = 0;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) word |= ((unsigned)dd_residue12(c[u], neg[u], t, P) & 0xFFu) << (8 * u);
+    for (int u = 0; u < 4; ++u) word |= ((unsigned)dd_residue8(c[u], neg[u], t, P) & 0xFFu) << (8 * u);
     *reinterpret_cast<unsigned*>(&s_tile[(t * SR_ROWS + lr) * SR_LD + lk0]) = word;
   }
   __syncthreads();
@@ -220,13 +212,13 @@ __global__ __launch_bounds__(256) void k_dd_split_cols(const double* __restrict_
     const int kk = k0 + u;
     v[u] = (colok && kk < k) ? *reinterpret_cast<const double2*>(B + 2 * (kk + ldb * (long)j)) : make_double2(0.0, 0.0);
   }
-  unsigned c[4][5];
+  unsigned c[4][4];
   bool neg[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     u128 mag;
     dd_scaled(v[u].x, v[u].y, sc, neg[u], mag);
-    chunks12x10(mag, c[u]);
+    words4(mag, c[u]);
   }
   const long plane_stride = (long)npad * kpad;
   int8_t* dst = planes + vcphip_plane_off(j, k0, kpad);
@@ -234,7 +226,7 @@ __global__ __launch_bounds__(256) void k_dd_split_cols(const double* __restrict_
   for (int z = 0; z < N; ++z) {
     unsigned word = 0;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) word |= ((unsigned)dd_residue12(c[u], neg[u], z, P) & 0xFFu) << (8 * u);
+    for (int u = 0; u < 4; ++u) word |= ((unsigned)dd_residue8(c[u], neg[u], z, P) & 0xFFu) << (8 * u);
     *reinterpret_cast<unsigned*>(dst + z * plane_stride) = word;
   }
 }

@@ -204,37 +204,27 @@ VCPHIP_DEVINL void oz_pass_planes(const VcphipOzParams& P, int z, int nmod, int&
   pb1 = nmod + P.rpb[r][1];
 }
 
-typedef unsigned short vcphip_v2u16 __attribute__((ext_vector_type(2)));
-
-// x (|x| <= 2^62) as x = d0 + d1 2^12 + d2 2^24 + d3 2^36 + d4 2^48 + (e5 - 4) 2^60
-// with unsigned 12-bit chunks d0..d4 of its two's complement and e5 in [0, 7],
-// held as the 16-bit pairs (d0, d1), (d2, d3), (d4, e5)
-struct Chunks12 {
-  unsigned p01, p23, p45;
+// x (|x| <= 2^62) as the eight bytes of x + 2^63 (its two's complement with
+// the sign bit flipped), held as two byte quads, low byte first
+struct Bytes8 {
+  unsigned lo, hi;
 };
-VCPHIP_DEVINL Chunks12 chunks12(long long x) {
-  const unsigned lo = (unsigned)(unsigned long long)x, hi = (unsigned)((unsigned long long)x >> 32);
-  Chunks12 c;
-  c.p01 = (lo & 0xFFFu) | (((lo >> 12) & 0xFFFu) << 16);
-  c.p23 = ((lo >> 24) | ((hi & 0xFu) << 8)) | (((hi >> 4) & 0xFFFu) << 16);
-  c.p45 = ((hi >> 16) & 0xFFFu) | ((unsigned)((int)(x >> 60) + 4) << 16);
-  return c;
+VCPHIP_DEVINL Bytes8 bytes8(long long x) {
+  const unsigned long long u = (unsigned long long)x;
+  return Bytes8{(unsigned)u, (unsigned)(u >> 32) ^ 0x80000000u};
 }
 
 // the residue of x in [-floor(m/2), m - 1 - floor(m/2)] (the symmetric int8
 // range; for m = 256 that is [-128, 127]).  y = x + floor(m/2) (mod m) from
-// three 16-bit dot products of the chunks with their weights (y < 2^22,
-// exact in FP32); q = trunc(RU(y * r_rcp)) is floor(y / m) or one less
-// (r_rcp is biased low), so r = y - q m lies in [0, 2m) and one unsigned
-// minimum brings it into [0, m).  Round-up mode.
-VCPHIP_DEVINL int sym_residue12(const Chunks12& c, int t, const VcphipOzParams& P) {
+// two byte dot products of x + 2^63 with the weights 256^i mod m, started
+// at the bias (floor(m/2) - 2^63) mod m (y < 2^19, exact in FP32);
+// q = trunc(RU(y * r_rcp)) is floor(y / m) or one less (r_rcp is biased
+// low), so r = y - q m lies in [0, 2m) and one unsigned minimum brings it
+// into [0, m).  Round-up mode.
+VCPHIP_DEVINL int sym_residue8(const Bytes8& b, int t, const VcphipOzParams& P) {
   const unsigned m = (unsigned)P.mod[t];
-  unsigned y = __builtin_amdgcn_udot2(__builtin_bit_cast(vcphip_v2u16, c.p01),
-                                      __builtin_bit_cast(vcphip_v2u16, P.r_wp[0][t]), P.r_hh[t], false);
-  y = __builtin_amdgcn_udot2(__builtin_bit_cast(vcphip_v2u16, c.p23), __builtin_bit_cast(vcphip_v2u16, P.r_wp[1][t]),
-                             y, false);
-  y = __builtin_amdgcn_udot2(__builtin_bit_cast(vcphip_v2u16, c.p45), __builtin_bit_cast(vcphip_v2u16, P.r_wp[2][t]),
-                             y, false);
+  unsigned y = __builtin_amdgcn_udot4(b.lo, P.r_wb[0][t], P.r_bias[t], false);
+  y = __builtin_amdgcn_udot4(b.hi, P.r_wb[1][t], y, false);
   const unsigned q = (unsigned)((float)y * P.r_rcp[t]);
   unsigned r = y - __umul24(q, m);
   const unsigned r2 = r - m;
@@ -273,7 +263,7 @@ __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict_
       const int kk = k0 + lk0 + u;
       v[u] = (rowok && kk < k) ? *reinterpret_cast<const double2*>(A + 2 * (r + lda * kk)) : make_double2(0.0, 0.0);
     }
-    Chunks12 ch[4];
+    Bytes8 ch[4];
     unsigned dsh = 0, dsl = 0, drh = 0, drl = 0, dsc = 0, drc = 0;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -281,7 +271,7 @@ __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict_
       vcphip_midrad_ru(v[u].x, v[u].y, am, ar);
       const double as = __builtin_fabs(am) + ar;
       const long long ap = zr ? 0 : rnint_floor(vcphip_mulpow2(am, P.beta - e));
-      ch[u] = chunks12(ap);
+      ch[u] = bytes8(ap);
       const unsigned ds = digits2_ru(as, g), dr = digits2_ru(ar, h);
       dsh |= (ds & 0xff) << (8 * u);
       dsl |= (ds >> 8) << (8 * u);
@@ -296,7 +286,7 @@ __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict_
       unsigned word = 0;
 #pragma unroll
       for (int u = 0; u < 4; ++u)
-        word |= ((unsigned)sym_residue12(ch[u], t, P) & 0xFFu) << (8 * u);
+        word |= ((unsigned)sym_residue8(ch[u], t, P) & 0xFFu) << (8 * u);
       *reinterpret_cast<unsigned*>(&s_tile[(t * SR_ROWS + lr) * SR_LD + lk0]) = word;
     }
     *reinterpret_cast<unsigned*>(&s_tile[(N * SR_ROWS + lr) * SR_LD + lk0]) = dsh;
@@ -343,14 +333,14 @@ __global__ __launch_bounds__(256) void k_oz_split_cols(const double* __restrict_
     const int kk = k0 + u;
     v[u] = (colok && kk < k) ? *reinterpret_cast<const double2*>(B + 2 * (kk + ldb * (long)j)) : make_double2(0.0, 0.0);
   }
-  Chunks12 ch[4];
+  Bytes8 ch[4];
   unsigned drh = 0, drl = 0, dmh = 0, dml = 0;
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     double bm, br;
     vcphip_midrad_ru(v[u].x, v[u].y, bm, br);
     const long long bp = zc ? 0 : rnint_floor(vcphip_mulpow2(bm, P.beta - f));
-    ch[u] = chunks12(bp);
+    ch[u] = bytes8(bp);
     const unsigned d1 = digits2_ru_neglo(br, xi);
     const unsigned d2 = digits2_ru_neglo(__builtin_fabs(bm), zeta);
     drh |= (d1 & 0xff) << (8 * u);
@@ -365,7 +355,7 @@ __global__ __launch_bounds__(256) void k_oz_split_cols(const double* __restrict_
     unsigned word = 0;
 #pragma unroll
     for (int u = 0; u < 4; ++u)
-      word |= ((unsigned)sym_residue12(ch[u], z, P) & 0xFFu) << (8 * u);
+      word |= ((unsigned)sym_residue8(ch[u], z, P) & 0xFFu) << (8 * u);
     *reinterpret_cast<unsigned*>(dst + z * plane_stride) = word;
   }
   // the digit planes the radius scheme reads (the low digits are unused by

@@ -320,27 +320,23 @@ int oz_fill_params(int k, int nmod, int extra, int beta_cap, VcphipOzParams* P) 
     const int m = P->mod[t];
     if (m < 16) return VCPHIP_ERR_UNSUPPORTED;   // the FP32 quotient estimate of the splits needs m >= 16
     P->rcpd[t] = 1.0 / (double)m;
-    unsigned wc[6];   // 2^(12 c) mod m, c = 0..5
+    unsigned wb[16];   // 256^i mod m, i = 0..15 (each < 256)
     unsigned long long pw = 1 % (unsigned)m;
-    for (int c = 0; c < 6; ++c) {
-      wc[c] = (unsigned)pw;
-      pw = pw * ((1ull << 12) % (unsigned)m) % (unsigned)m;
+    for (int i = 0; i < 16; ++i) {
+      wb[i] = (unsigned)pw;
+      pw = pw * 256ull % (unsigned)m;
     }
-    for (int c = 0; c < 3; ++c) P->r_wp[c][t] = wc[2 * c] | (wc[2 * c + 1] << 16);
-    P->r_hh[t] = (uint32_t)(((unsigned)(m >> 1) + 4ull * ((unsigned)m - wc[5])) % (unsigned)m);
+    for (int q = 0; q < 4; ++q)
+      P->dd_wb[t][q] = wb[4 * q] | (wb[4 * q + 1] << 8) | (wb[4 * q + 2] << 16) | (wb[4 * q + 3] << 24);
+    for (int q = 0; q < 2; ++q) P->r_wb[q][t] = P->dd_wb[t][q];
+    const unsigned two63 = (unsigned)(wb[7] * 128ull % (unsigned)m);   // 2^63 = 256^7 * 128
+    P->r_bias[t] = ((unsigned)(m >> 1) + (unsigned)m - two63) % (unsigned)m;
     {
       const double v = (1.0 / m) * (1.0 - ldexp(1.0, -22));
       float f = (float)v;
       if ((double)f > v) f = nextafterf(f, 0.0f);
       P->r_rcp[t] = f;
     }
-    unsigned long long w = 1 % (unsigned)m;
-    unsigned wd[10];   // 2^(12 c) mod m, c = 0..9
-    for (int c = 0; c < 10; ++c) {
-      wd[c] = (unsigned)w;
-      w = w * ((1ull << 12) % (unsigned)m) % (unsigned)m;
-    }
-    for (int c = 0; c < 5; ++c) P->dd_wp[c][t] = wd[2 * c] | (wd[2 * c + 1] << 16);
   }
   uint32_t M[VCPHIP_OZ_NLIMB] = {1, 0, 0, 0, 0, 0, 0, 0};
   for (int t = 0; t < nmod; ++t) {

@@ -101,15 +101,16 @@ struct VcphipOzParams {
   // w_{4g}, w_{4g+1}, w_{4g+2}, w_{4g+3} (low byte first); and w_t / M
   uint32_t crt_w8[VCPHIP_OZ_MAXMOD][VCPHIP_OZ_MAXMOD / 4];
   double crt_f[VCPHIP_OZ_MAXMOD];
-  // residues of dd magnitudes < 2^120 from ten 12-bit chunks: weights
-  // 2^(12 c) mod m_t as 16-bit pairs (c = 2i, 2i + 1) for v_dot2
-  uint32_t dd_wp[5][VCPHIP_OZ_MAXMOD];
-  // residues of |x| <= 2^62 from 12-bit chunks (interval split): the chunk
-  // weights 1, 2^12, 2^24, 2^36, 2^48, 2^60 mod m_t as 16-bit pairs (low
-  // half first) for v_dot2; r_hh = (floor(m/2) - 4 * 2^60) mod m_t; r_rcp an
-  // FP32 value in [(1 - 2^-21) / m, (1 - 2^-22) / m]
-  uint32_t r_wp[3][VCPHIP_OZ_MAXMOD];
-  uint32_t r_hh[VCPHIP_OZ_MAXMOD];
+  // residues of dd magnitudes < 2^120 from their 16 bytes: the byte
+  // weights 256^i mod m_t, four to a word ([t][q]: i = 4q .. 4q + 3, low
+  // byte first) for v_dot4
+  uint32_t dd_wb[VCPHIP_OZ_MAXMOD][4];
+  // residues of |x| <= 2^62 from the eight bytes of x + 2^63 (interval
+  // split): the byte weights 256^i mod m_t, four to a word; r_bias =
+  // (floor(m/2) - 2^63) mod m_t; r_rcp an FP32 value in
+  // [(1 - 2^-21) / m, (1 - 2^-22) / m]
+  uint32_t r_wb[2][VCPHIP_OZ_MAXMOD];
+  uint32_t r_bias[VCPHIP_OZ_MAXMOD];
   float r_rcp[VCPHIP_OZ_MAXMOD];
   uint32_t Mlimb[VCPHIP_OZ_NLIMB];  // M
   double c1;                   // RU(1/127)
