@@ -73,3 +73,66 @@ def test_uhphiphi_large_tiles_sampled(ctx):
     sub[...] = phi[idx]
     assert np.array_equal(A[np.ix_(idx, idx)], O.uhphiphi(sub, w, u))
     assert np.array_equal(A, A.transpose(1, 0, 2))
+
+
+def _with_zero_crossings(rng, phi, u, points):
+    """Entries of phi at the given points that straddle or touch zero
+    ([-a, b], [0, b], [-b, 0], [-0, 0], [0, 0]) and one solution value
+    u = [0, 0]: the stages holding them take the kernel's general product,
+    the others its two-product form."""
+    m = phi.shape[0]
+    for k in points:
+        sel = rng.random(m) < 0.3
+        n = int(sel.sum())
+        a, b = rng.uniform(0, 1, n), rng.uniform(0, 1, n)
+        kind = rng.integers(0, 5, n)
+        lo = np.choose(kind, [-a, np.zeros(n), -b, np.full(n, -0.0), np.zeros(n)])
+        hi = np.choose(kind, [b, b, np.zeros(n), np.zeros(n), np.zeros(n)])
+        phi[sel, k, 0] = lo
+        phi[sel, k, 1] = hi
+    u[points[0]] = 0.0
+    return phi, u
+
+
+@pytest.mark.parametrize("m,npts,tag", [(130, 70, "32x32 tiles"), (2880, 40, "64x64 tiles")])
+def test_uhphiphi_two_product_and_general_stages(ctx, m, npts, tag):
+    """Stages of 16 points whose operands all lie strictly on one side of
+    zero use two products per interval product, the rest the general eight;
+    both must give the oracle's bits, in one matrix.  Zero crossings sit at
+    points 20-23 and at the last points, so stages 0, 2 and 3 take the
+    two-product form and stages 1 and 4 the general one."""
+    print("uhphiphi mixed stages", m, npts, tag, flush=True)
+    rng = np.random.default_rng(m + npts)
+    phi, w, u = _rand(rng, m, npts)
+    # negative solution values too: every sign combination of s and phi
+    u[::3] = -u[::3, ::-1]
+    phi, u = _with_zero_crossings(rng, phi, u, [20, 21, 22, 23, npts - 2, npts - 1])
+    A = ctx.uhphiphi(phi, w, u)
+    if m <= 256:
+        assert np.array_equal(A, O.uhphiphi(phi, w, u))
+    else:
+        idx = np.linspace(0, m - 1, 24).astype(int)
+        sub = O.pairs(len(idx), npts)
+        sub[...] = phi[idx]
+        assert np.array_equal(A[np.ix_(idx, idx)], O.uhphiphi(sub, w, u))
+        assert np.array_equal(A, A.transpose(1, 0, 2))
+
+
+def test_uhphiphi_two_product_signed_zero_results(ctx):
+    """Products that underflow to zero in the two-product form: tiny operands
+    of every sign combination, whose rounded products are -0, +0 or the
+    smallest subnormal, bit-identical to the oracle (signed zeros included)."""
+    rng = np.random.default_rng(11)
+    m, npts = 40, 33
+    v = rng.uniform(0.5, 1, (m, npts)) * 2.0 ** -540 * rng.choice([-1.0, 1.0], (m, npts))
+    phi = O.pairs(m, npts)
+    phi[..., 0] = np.where(v > 0, v, v * (1 + 2.0 ** -20))
+    phi[..., 1] = np.where(v > 0, v * (1 + 2.0 ** -20), v)
+    wv = rng.uniform(0.5, 1, npts)
+    w = np.stack([wv, wv], axis=-1)
+    uv = rng.uniform(0.5, 1, npts) * rng.choice([-1.0, 1.0], npts)
+    u = np.stack([uv, uv], axis=-1)
+    A = ctx.uhphiphi(phi, w, u)
+    R = O.uhphiphi(phi, w, u)
+    assert np.array_equal(A, R)
+    assert np.array_equal(np.signbit(A), np.signbit(R))
