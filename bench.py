@@ -189,8 +189,16 @@ def main():
         run(ctx, vcp_amd.FORM_INFSUP, dI)
         ctx.synchronize()
         t_is = time.perf_counter() - t0
+        # per (i, j, k): two chains of 4 FMAs and 3 maxima, or of 2 FMAs and
+        # 1 maximum in k-slices whose operands are all strictly sign-definite
+        # (every slice of this workload); counted as 2 ops per instruction
+        definite = bool(((IA[..., 0] > 0) | (IA[..., 1] < 0)).all() and ((IB[..., 0] > 0) | (IB[..., 1] < 0)).all())
+        per = 6 if definite else 14
+        ops = 2.0 * per * n * n * n
         infsup = {"ms_per_step": round(t_is * 1e3, 3), "tflops_8n3": round(flops / t_is / 1e12, 3),
-                  "fp64_ops_per_step": 28.0 * n * n * n, "radius_ratio_over_plain": ratio_of(dI)}
+                  "two_fma_slices": definite, "fp64_instr_per_ijk": per, "fp64_ops_per_step": ops,
+                  "frac_fp64_peak": round(ops / t_is / 1e12 / PEAK_FP64_TFLOPS, 4),
+                  "radius_ratio_over_plain": ratio_of(dI)}
         log("infsup %s" % infsup)
         del dI
 

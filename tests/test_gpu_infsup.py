@@ -92,3 +92,20 @@ def test_infsup_not_wider_than_midrad_on_sign_definite_data(ctx):
     Ci = ctx.igemm(IA, IB, vcp_amd.FORM_INFSUP)
     Cp = ctx.igemm(IA, IB, vcp_amd.FORM_PLAIN)
     assert ((Ci[..., 1] - Ci[..., 0]) <= (Cp[..., 1] - Cp[..., 0])).all()
+
+
+@pytest.mark.parametrize("m,n,k", [(96, 80, 80), (130, 64, 200)])
+def test_infsup_two_fma_and_general_slices(ctx, m, n, k):
+    """k-slices whose operands all lie strictly on one side of zero take two
+    FMAs per chain, the others four: sign-definite data with intervals
+    across or touching zero in k = 16..19 (A) and k = 64..66 (B), so both
+    loops run in one product, against the oracle."""
+    print("infsup mixed slices", m, n, k, flush=True)
+    rng = np.random.default_rng(m + n + k)
+    IA, IB = W.wide_exponent_intervals(rng, m, k), W.wide_exponent_intervals(rng, k, n)
+    IA[:, 16:20, 0] = -np.abs(IA[:, 16:20, 1])          # across zero
+    IB[64:66, :, 0] = 0.0                                # touching zero
+    IB[64:66, :, 1] = np.abs(IB[64:66, :, 1])
+    IB[66, ::2, :] = 0.0                                 # [0, 0]
+    C = ctx.igemm(IA, IB, vcp_amd.FORM_INFSUP)
+    assert np.array_equal(C, O.pidblas_fma_mulmm(IA, IB))
