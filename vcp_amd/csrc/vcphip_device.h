@@ -25,7 +25,10 @@ VCPHIP_DEVINL uint32_t vcphip_lds_addr(const void* p) {
 // (global_load_lds_dwordx4): lane l's 16 bytes, from its own address g, land
 // at lds_base + 16 l (lds_base must be wave-uniform).  Neither the load nor
 // its completion is visible to the compiler: callers wait with an explicit
-// s_waitcnt vmcnt before a barrier.
+// s_waitcnt vmcnt before a barrier.  m0 is listed as clobbered; hipcc warns
+// that a reserved register in a clobber list is not preserved for code that
+// keeps a value in it.  The kernels using this helper keep nothing in m0:
+// in their gfx950 assembly m0 is written only by this sequence.
 VCPHIP_DEVINL void vcphip_lds_dma16(const void* g, uint32_t lds_base) {
   asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g), "s"(lds_base) : "memory", "m0");
 }
