@@ -354,12 +354,23 @@ def _smi(args, tool="amd-smi"):
         return ""
 
 
+def parse_power_clock(text):
+    """(socket power in W, mean GFX clock over the XCDs in MHz) from the text
+    of `amd-smi metric --power --clock`, or None when either is missing."""
+    import re
+
+    pw = re.search(r"SOCKET_POWER:\s*([0-9.]+)", text)
+    clks = [int(x) for x in re.findall(r"GFX_\d+:\s*\n\s*CLK:\s*(\d+)", text)]
+    if not pw or not clks:
+        return None
+    return float(pw.group(1)), sum(clks) / len(clks)
+
+
 def sustained_power(step, sync, seconds, period=0.25):
     """Socket power (W) and mean GFX clock over the XCDs (MHz), read with
     amd-smi every `period` s while step() runs back to back for `seconds`:
     medians of the samples after the first second (the clocks settle).  None
     when amd-smi is absent or reads nothing."""
-    import re
     import statistics
     import threading
 
@@ -368,11 +379,9 @@ def sustained_power(step, sync, seconds, period=0.25):
 
     def poll():
         while not done.is_set():
-            out = _smi(["metric", "--power", "--clock"])
-            pw = re.search(r"SOCKET_POWER:\s*([0-9.]+)", out)
-            clks = [int(x) for x in re.findall(r"GFX_\d+:\s*\n\s*CLK:\s*(\d+)", out)]
-            if pw and clks:
-                samples.append((time.perf_counter(), float(pw.group(1)), sum(clks) / len(clks)))
+            pc = parse_power_clock(_smi(["metric", "--power", "--clock"]))
+            if pc:
+                samples.append((time.perf_counter(),) + pc)
             done.wait(period)
 
     th = threading.Thread(target=poll, daemon=True)
