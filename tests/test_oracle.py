@@ -235,10 +235,15 @@ def test_ozaki_moduli_and_beta():
 
 def test_crt_symmetric_reconstruction_identity():
     """the reconstruction the kernels use: X = sum_t u_t w_t - q M with
-    w_t = (M/m_t)((M/m_t)^-1 mod m_t) and q = floor(0.5 + sum_t u_t fl(w_t/M))
-    in double precision, recovers every X the splitting admits, including the
-    extremes |X| = (M/2) 2^-1e-6 (interval form) and M/4 (dd form); the
-    byte-column sums of the dot-product formulation give the same S"""
+    w_t = (M/m_t)((M/m_t)^-1 mod m_t) and q = floor((sum_t u_t F_t + 2^47) /
+    2^48), F_t = floor(2^48 w_t / M), recovers every X the splitting admits,
+    including the extremes |X| = (M/2) 2^-1e-6 (interval form) and M/4 (dd
+    form); so does the double-precision q = floor(0.5 + sum_t u_t fl(w_t/M)).
+    The byte-column sums give the same S and Q, and each equals the centred
+    int8 product sum_t (u_t - 128)(v_t - 128) plus 128 sum_t u_t plus the
+    column constant 128 sum_t v_t - 128^2 N, with sum_t u_t itself the
+    centred product against a column of ones plus 128 N (the matrix-core
+    form)"""
     rng = np.random.default_rng(10)
     for nmod in (8, 12, 16, 20, 24, 28, 32):
         mods = O.ozaki_moduli(nmod)
@@ -248,6 +253,13 @@ def test_crt_symmetric_reconstruction_identity():
             Mt = M // m
             w.append(Mt * pow(Mt % m, -1, m))
         f = [float(F(wt, M)) for wt in w]
+        assert max(w) < 2 ** (8 * nmod)              # bytes past N are zero: N weight columns
+        Fq = [(wt << 48) // M for wt in w]
+        assert all(F(wt, M) - F(ft, 2 ** 48) < F(1, 2 ** 48) for wt, ft in zip(w, Fq))
+        wb = [[(wt >> (8 * c)) & 255 for wt in w] for c in range(nmod)]
+        fb = [[(ft >> (8 * b)) & 255 for ft in Fq] for b in range(6)]
+        for vb in wb + fb:                           # centred operands are int8
+            assert all(-128 <= v - 128 <= 127 for v in vb)
         edge = int(F(M, 2) * F(2) ** F(-1, 10 ** 6)) if nmod <= 20 else M // 4 - 1
         cases = [0, 1, -1, edge, -edge, edge - 1, -(edge - 1)]
         cases += [int(x) for x in rng.integers(-2 ** 62, 2 ** 62, 100)]
@@ -267,6 +279,20 @@ def test_crt_symmetric_reconstruction_identity():
             cols = [sum(ut * ((wt >> (8 * c)) & 255) for ut, wt in zip(u, w)) for c in range(nb)]
             assert max(cols) < 2 ** 21
             assert sum(col << (8 * c) for c, col in enumerate(cols)) == S
+            # fixed-point quotient from the six bytes of F_t
+            Q = sum(ut * ft for ut, ft in zip(u, Fq)) + 2 ** 47
+            assert Q >> 48 == q and Q < 2 ** 63
+            fcols = [sum(ut * ((ft >> (8 * b)) & 255) for ut, ft in zip(u, Fq)) for b in range(6)]
+            assert max(fcols) < 2 ** 21
+            assert sum(fc << (8 * b) for b, fc in enumerate(fcols)) + 2 ** 47 == Q
+            # centred int8 products (column of ones for sum_t u_t)
+            ssum = sum((ut - 128) * 1 for ut in u) + 128 * nmod
+            assert ssum == sum(u)
+            wcols = [sum(ut * v for ut, v in zip(u, vb)) for vb in wb]   # N columns (the top ones may be 0)
+            assert wcols[:nb] == cols[:nmod] and not any(wcols[nb:])
+            for vb, want in [(wb[c], wcols[c]) for c in range(nmod)] + [(fb[b], fcols[b]) for b in range(6)]:
+                cen = sum((ut - 128) * (v - 128) for ut, v in zip(u, vb))
+                assert cen + 128 * ssum + 128 * sum(vb) - 128 * 128 * nmod == want
 
 
 def test_pidblas_fma_is_the_interval_fma_chain():

@@ -310,19 +310,25 @@ __global__ __launch_bounds__(256) void k_dd_recon(int m, int n, int mpad, int np
   // threads follow the blocked 16 x 16 layout of the residue planes
   // grid: x = 16-row block, y = 16-column block (the plane's block order)
   // the block's planes from a uniform base plus the thread's offset o
+  // every lane runs through the CRT's matrix instructions; lanes past m or n
+  // (inside the padded planes) read row / column 0's tables and store nothing
   const unsigned o = threadIdx.x;
   const long bbase = ((long)blockIdx.y * (mpad >> 4) + blockIdx.x) * 256;
-  const unsigned i = blockIdx.x * 16 + 4 * (o >> 6) + (o & 3);
-  const unsigned j = blockIdx.y * 16 + ((o >> 2) & 15);
-  if (i >= (unsigned)m || j >= (unsigned)n) return;
-  if (fa[i] || fb[j]) {   // a zero row / column gives 0, a non-finite one NaN
-    const double v = ((fa[i] | fb[j]) & 2) ? __builtin_nan("") : 0.0;
-    *reinterpret_cast<double2*>(C + 2 * (i + ldc * (long)j)) = make_double2(v, v);
+  const unsigned i0 = blockIdx.x * 16 + 4 * (o >> 6) + (o & 3);
+  const unsigned j0 = blockIdx.y * 16 + ((o >> 2) & 15);
+  const bool live = i0 < (unsigned)m && j0 < (unsigned)n;
+  const unsigned i = live ? i0 : 0u, j = live ? j0 : 0u;
+  const long plane = (long)mpad * npad;
+  VcphipCrtOps<N> T;
+  vcphip_crt_ops_load<N>(P, T);
+  unsigned mag[NL];
+  const bool neg = vcphip_crt_symmetric<N, NL>(U + bbase, plane, o, T, P, mag);
+  const int flag = fa[i] | fb[j];
+  if (flag) {   // a zero row / column gives 0, a non-finite one NaN
+    const double v = (flag & 2) ? __builtin_nan("") : 0.0;
+    if (live) *reinterpret_cast<double2*>(C + 2 * (i + ldc * (long)j)) = make_double2(v, v);
     return;
   }
-  const long plane = (long)mpad * npad;
-  unsigned mag[NL];
-  const bool neg = vcphip_crt_symmetric<N, NL>(U + bbase, plane, o, P, mag);
   const int sc = ea[i] + eb[j] - 2 * P.beta;
   int q;
   bool up;
@@ -360,7 +366,7 @@ __global__ __launch_bounds__(256) void k_dd_recon(int m, int n, int mpad, int np
   double lo = rn_limbs(rem, sc, q2, up2);
   if (rem_neg) lo = -lo;
   const double sgn = neg ? -1.0 : 1.0;
-  *reinterpret_cast<double2*>(C + 2 * (i + ldc * (long)j)) = make_double2(sgn * hi, sgn * lo);
+  if (live) *reinterpret_cast<double2*>(C + 2 * (i + ldc * (long)j)) = make_double2(sgn * hi, sgn * lo);
 }
 
 // ---------------------------------------------------------------------------

@@ -1094,16 +1094,21 @@ __global__ __launch_bounds__(256) void k_oz_recon(int m, int n, int k, int mpad,
   // grid: x = 16-row block, y = 16-column block (the plane's block order)
   // the block's planes are addressed from a uniform base (Ub, Zb) plus the
   // thread's offset o, and the row / column tables by unsigned indices
+  // every lane runs through the CRT's matrix instructions; lanes past m or n
+  // (inside the padded planes) read row / column 0's tables and store nothing
   const unsigned o = threadIdx.x;
   const long bbase = ((long)blockIdx.y * (mpad >> 4) + blockIdx.x) * 256;
-  const unsigned i = blockIdx.x * 16 + 4 * (o >> 6) + (o & 3);
-  const unsigned j = blockIdx.y * 16 + ((o >> 2) & 15);
-  if (i >= (unsigned)m || j >= (unsigned)n) return;
+  const unsigned i0 = blockIdx.x * 16 + 4 * (o >> 6) + (o & 3);
+  const unsigned j0 = blockIdx.y * 16 + ((o >> 2) & 15);
+  const bool live = i0 < (unsigned)m && j0 < (unsigned)n;
+  const unsigned i = live ? i0 : 0u, j = live ? j0 : 0u;
   const long plane = (long)mpad * npad;
   const int32_t* __restrict__ Zb = Z + bbase;
 
+  VcphipCrtOps<N> T;
+  vcphip_crt_ops_load<N>(P, T);
   unsigned mag[NL];
-  const bool neg = vcphip_crt_symmetric<N, NL>(U + bbase, plane, o, P, mag);
+  const bool neg = vcphip_crt_symmetric<N, NL>(U + bbase, plane, o, T, P, mag);
   const int fa = sa.flags[i], fb = sb.flags[j];
   const bool zr = (fa & VCPHIP_OZ_MIDZERO) != 0, zc = (fb & VCPHIP_OZ_MIDZERO) != 0;
   const int ei = sa.e[i], fj = sb.e[j];
@@ -1158,7 +1163,7 @@ __global__ __launch_bounds__(256) void k_oz_recon(int m, int n, int k, int mpad,
     lo = nan ? __builtin_nan("") : -__builtin_inf();
     hi = nan ? __builtin_nan("") : __builtin_inf();
   }
-  *reinterpret_cast<double2*>(C + 2 * (i + ldc * (long)j)) = make_double2(lo, hi);
+  if (live) *reinterpret_cast<double2*>(C + 2 * (i + ldc * (long)j)) = make_double2(lo, hi);
 }
 
 }  // namespace

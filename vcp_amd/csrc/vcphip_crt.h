@@ -6,56 +6,102 @@
 // |X| < M/4 (dd) - determine X as
 //     S = sum_t u_t w_t,   w_t = (M/m_t) ((M/m_t)^-1 mod m_t),
 //     X = S - q M,         q = round(S / M) = floor(sum_t u_t (w_t / M) + 1/2).
-// q is evaluated in double precision: with u_t < 256, N <= 32 and w_t / M
-// rounded to double, the computed sum is within 2^-34 of S / M, which stays
-// at least 3.4e-7 (interval) or 1/4 (dd) away from a half-integer, so the
-// floor is exact in every rounding mode.  S is accumulated exactly in byte
-// columns with v_dot4_u32_u8 (column c = sum_t u_t * byte c of w_t < 2^21),
-// and X is formed modulo 2^(32 NL), which suffices because |X| < 2^(32 NL - 1).
-// This replaces a mixed-radix (Garner) reconstruction: O(N * NL) byte dot
-// products instead of O(N^2) modular steps and an N-step multiword Horner.
+// Both sums are byte-column sums, formed on the matrix cores:
+//     col_c = sum_t u_t * byte c of w_t             (c < N, w_t < M < 2^(8 N))
+//     fc_b  = sum_t u_t * byte b of F_t,   F_t = floor(2^48 w_t / M)   (b < 6)
+// so S = sum_c col_c 2^(8c) and q = floor((sum_b fc_b 2^(8b) + 2^47) / 2^48).
+// Each F_t is within 2^-48 of w_t / M, so with u_t < 256 and N <= 32 the
+// fixed-point quotient is within 2^-35 of S / M, which stays at least 3.4e-7
+// (interval) or 1/4 (dd) away from a half-integer: q is exact.  X is formed
+// modulo 2^(32 NL), which suffices because |X| < 2^(32 NL - 1).
+//
+// The column sums are integer matrix products of int8 operands:
+// sum_t u_t v_t = sum_t (u_t - 128)(v_t - 128) + 128 sum_t u_t + K, with
+// K = 128 sum_t v_t - 128^2 N a per-column constant.  One
+// v_mfma_i32_32x32x32_i8 handles the 64 elements of a wave (one per lane)
+// against 16 columns and 16 residues: output row r holds column
+// 4 (r / 8) + r % 4 of element group (r / 4) % 2, so a lane's accumulator
+// register v is column v of its own element.  Its B operand is the lane's own
+// 16 centred residues (k-slots of lane half h carry element group h), and the
+// A operand of lane half h is zero in the rows of the other group, which
+// keeps the two groups apart.  A seventh extra column of ones gives
+// sum_t (u_t - 128).  This replaces O(N^2 / 4) byte dot products and N
+// double-precision multiply-adds per element by (N + 7) / 16 x N / 16
+// matrix instructions per 64 elements.  Every lane of the wave must execute
+// it (the kernels keep all lanes through it and mask their stores).
 #pragma once
 
 #include "vcphip_device.h"
 #include "vcphip_kernels.h"
 
-// From the residues u[t] (bytes) of X: returns the sign of X and its
-// magnitude in NL little-endian 32-bit limbs.
+typedef int vcphip_v4i __attribute__((ext_vector_type(4)));
+typedef int vcphip_v16i __attribute__((ext_vector_type(16)));
+
+// The per-lane A operands (weights) of the CRT products.
+template <int N>
+struct VcphipCrtOps {
+  static constexpr int NC = N + 7;           // N weight bytes, 6 quotient bytes, the residue sum
+  static constexpr int CH = (NC + 15) / 16;  // 16-column groups
+  static constexpr int RH = (N + 15) / 16;   // 16-residue groups
+  static_assert(16 * CH <= VCPHIP_OZ_CRTCOL && 4 * RH <= VCPHIP_OZ_MAXMOD / 4, "CRT table");
+  vcphip_v4i a[CH][RH];
+};
+
+template <int N>
+VCPHIP_DEVINL void vcphip_crt_ops_load(const VcphipOzParams& P, VcphipCrtOps<N>& T) {
+  const unsigned lane = threadIdx.x & 63u;
+  const unsigned h = lane >> 5, r = lane & 31u;
+  const bool mine = ((r >> 2) & 1u) == h;
+  const unsigned cl = 4 * (r >> 3) + (r & 3u);
+#pragma unroll
+  for (int ch = 0; ch < T.CH; ++ch)
+#pragma unroll
+    for (int rh = 0; rh < T.RH; ++rh)
+      T.a[ch][rh] = mine ? *reinterpret_cast<const vcphip_v4i*>(&P.crt_a[16 * ch + cl][4 * rh]) : vcphip_v4i{0, 0, 0, 0};
+}
+
+// From the residues u[t] (bytes) of this lane's X: returns the sign of X and
+// its magnitude in NL little-endian 32-bit limbs.
 template <int N, int NL>
-VCPHIP_DEVINL bool vcphip_crt_from_residues(const unsigned (&u)[N], const VcphipOzParams& P, unsigned (&mag)[NL]) {
-  constexpr int G = (N + 3) / 4;
-  constexpr int NB = N < 4 * NL ? N : 4 * NL;   // byte columns below 2^(32 NL); w_t < M < 2^(8 N)
-  unsigned r[G];
+VCPHIP_DEVINL bool vcphip_crt_from_residues(const unsigned (&u)[N], const VcphipCrtOps<N>& T, const VcphipOzParams& P,
+                                            unsigned (&mag)[NL]) {
+  static_assert(N <= 4 * NL, "byte columns beyond the limbs");
+  constexpr int CH = VcphipCrtOps<N>::CH, RH = VcphipCrtOps<N>::RH;
+  vcphip_v4i b[RH];
 #pragma unroll
-  for (int g = 0; g < G; ++g) {
-    unsigned x = 0;
+  for (int rh = 0; rh < RH; ++rh)
 #pragma unroll
-    for (int e = 0; e < 4; ++e)
-      if (4 * g + e < N) x |= u[4 * g + e] << (8 * e);
-    r[g] = x;
+    for (int e = 0; e < 4; ++e) {
+      unsigned x = 0;
+#pragma unroll
+      for (int y = 0; y < 4; ++y)
+        if (16 * rh + 4 * e + y < N) x |= u[16 * rh + 4 * e + y] << (8 * y);
+      b[rh][e] = (int)(x ^ 0x80808080u);   // u - 128 as int8 (bytes past N meet zero weights)
+    }
+  vcphip_v16i acc[CH];
+#pragma unroll
+  for (int ch = 0; ch < CH; ++ch) {
+    acc[ch] = vcphip_v16i{};
+#pragma unroll
+    for (int rh = 0; rh < RH; ++rh) acc[ch] = __builtin_amdgcn_mfma_i32_32x32x32_i8(T.a[ch][rh], b[rh], acc[ch], 0, 0, 0);
   }
-  double qf = 0.5;
+  auto D = [&](int c) { return acc[c >> 4][c & 15]; };
+  const int s128 = 128 * (D(N + 6) + 128 * N);   // 128 sum_t u_t
+  auto col = [&](int c) { return (unsigned)(D(c) + s128 + P.crt_k[c]); };   // < 2^21
+  unsigned long long Q = 1ull << 47;
 #pragma unroll
-  for (int t = 0; t < N; ++t) qf = __builtin_fma((double)u[t], P.crt_f[t], qf);
-  const long long q = (long long)__builtin_floor(qf);
-  unsigned col[NB];
-#pragma unroll
-  for (int c = 0; c < NB; ++c) {
-    unsigned a = 0;
-#pragma unroll
-    for (int g = 0; g < G; ++g) a = __builtin_amdgcn_udot4(r[g], P.crt_w8[c][g], a, false);
-    col[c] = a;
-  }
+  for (int c = 0; c < 6; ++c) Q += (unsigned long long)col(N + c) << (8 * c);
+  const long long q = (long long)(Q >> 48);
   long long carry = 0;
 #pragma unroll
   for (int l = 0; l < NL; ++l) {
-    long long acc = carry;
+    long long acc_l = carry;
 #pragma unroll
     for (int e = 0; e < 4; ++e)
-      if (4 * l + e < NB) acc += (long long)col[4 * l + e] << (8 * e);
-    acc -= q * (long long)P.Mlimb[l];
-    mag[l] = (unsigned)acc;
-    carry = acc >> 32;
+      if (4 * l + e < N) acc_l += (long long)col(4 * l + e) << (8 * e);
+    acc_l -= q * (long long)P.Mlimb[l];
+    mag[l] = (unsigned)acc_l;
+    carry = acc_l >> 32;
   }
   const bool neg = (int)mag[NL - 1] < 0;
   if (neg) {
@@ -76,9 +122,9 @@ VCPHIP_DEVINL bool vcphip_crt_from_residues(const unsigned (&u)[N], const Vcphip
 // offset.
 template <int N, int NL>
 VCPHIP_DEVINL bool vcphip_crt_symmetric(const uint8_t* __restrict__ U, long plane, unsigned o,
-                                        const VcphipOzParams& P, unsigned (&mag)[NL]) {
+                                        const VcphipCrtOps<N>& T, const VcphipOzParams& P, unsigned (&mag)[NL]) {
   unsigned u[N];
 #pragma unroll
   for (int t = 0; t < N; ++t) u[t] = __builtin_amdgcn_raw_buffer_load_b8(vcphip_rsrc(U + (long)t * plane, 256), o, 0, 0);
-  return vcphip_crt_from_residues<N, NL>(u, P, mag);
+  return vcphip_crt_from_residues<N, NL>(u, T, P, mag);
 }

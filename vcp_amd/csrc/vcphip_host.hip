@@ -278,6 +278,42 @@ int modinv(int a, int m) {   // a^{-1} mod m, gcd(a, m) = 1
   return ((t % m) + m) % m;
 }
 
+// floor(2^48 w / M) for w < M, both of VCPHIP_OZ_NLIMB limbs (binary long division)
+unsigned long long frac48(const uint32_t (&w)[VCPHIP_OZ_NLIMB], const uint32_t (&M)[VCPHIP_OZ_NLIMB]) {
+  uint32_t r[VCPHIP_OZ_NLIMB + 1];
+  for (int l = 0; l < VCPHIP_OZ_NLIMB; ++l) r[l] = w[l];
+  r[VCPHIP_OZ_NLIMB] = 0;
+  unsigned long long F = 0;
+  for (int b = 0; b < 48; ++b) {   // invariant r < M
+    uint32_t cin = 0;
+    for (int l = 0; l <= VCPHIP_OZ_NLIMB; ++l) {
+      const uint32_t cout = r[l] >> 31;
+      r[l] = (r[l] << 1) | cin;
+      cin = cout;
+    }
+    bool ge = r[VCPHIP_OZ_NLIMB] != 0;
+    if (!ge) {
+      ge = true;   // equal counts as >=
+      for (int l = VCPHIP_OZ_NLIMB - 1; l >= 0; --l)
+        if (r[l] != M[l]) {
+          ge = r[l] > M[l];
+          break;
+        }
+    }
+    F <<= 1;
+    if (ge) {
+      long long borrow = 0;
+      for (int l = 0; l <= VCPHIP_OZ_NLIMB; ++l) {
+        const long long d = (long long)r[l] - (l < VCPHIP_OZ_NLIMB ? (long long)M[l] : 0) - borrow;
+        borrow = d < 0;
+        r[l] = (uint32_t)(d + (borrow << 32));
+      }
+      F |= 1;
+    }
+  }
+  return F;
+}
+
 // greedy pairwise-coprime moduli from 256 downward
 int oz_moduli(int nmod, int* out) {
   int cnt = 0;
@@ -349,9 +385,8 @@ int oz_fill_params(int k, int nmod, int extra, int beta_cap, VcphipOzParams* P) 
     if (carry) return VCPHIP_ERR_UNSUPPORTED;
   }
   for (int l = 0; l < VCPHIP_OZ_NLIMB; ++l) P->Mlimb[l] = M[l];
-  // CRT weights w_t = (M/m_t) y_t, y_t = (M/m_t)^-1 mod m_t, and w_t / M
-  long double Mval = 0.0L;
-  for (int l = VCPHIP_OZ_NLIMB - 1; l >= 0; --l) Mval = Mval * 4294967296.0L + (long double)M[l];
+  // CRT weights w_t = (M/m_t) y_t, y_t = (M/m_t)^-1 mod m_t, their bytes and
+  // F_t = floor(2^48 w_t / M) as the centred int8 columns of the MFMA CRT
   for (int t = 0; t < nmod; ++t) {
     const unsigned m = (unsigned)P->mod[t];
     uint32_t Mt[VCPHIP_OZ_NLIMB];
@@ -372,14 +407,17 @@ int oz_fill_params(int k, int nmod, int extra, int beta_cap, VcphipOzParams* P) 
       carry = p >> 32;
     }
     if (carry) return VCPHIP_ERR_UNSUPPORTED;
-    long double wval = 0.0L;
-    for (int l = VCPHIP_OZ_NLIMB - 1; l >= 0; --l) wval = wval * 4294967296.0L + (long double)wt[l];
-    P->crt_f[t] = (double)(wval / Mval);
-    for (int c = 0; c < VCPHIP_OZ_MAXMOD; ++c) {
-      const uint32_t byte = (wt[c / 4] >> (8 * (c % 4))) & 0xFFu;
-      P->crt_w8[c][t / 4] |= byte << (8 * (t % 4));
-    }
+    const unsigned long long F = frac48(wt, M);
+    auto put = [&](int c, unsigned byte, bool centred) {
+      const unsigned v = centred ? ((byte - 128u) & 0xFFu) : byte;
+      P->crt_a[c][t / 4] |= v << (8 * (t % 4));
+      if (centred) P->crt_k[c] += 128 * (int32_t)byte;
+    };
+    for (int c = 0; c < nmod; ++c) put(c, (wt[c / 4] >> (8 * (c % 4))) & 0xFFu, true);   // w_t < M < 2^(8 N)
+    for (int b = 0; b < 6; ++b) put(nmod + b, (unsigned)(F >> (8 * b)) & 0xFFu, true);
+    put(nmod + 6, 1u, false);
   }
+  for (int c = 0; c < nmod + 6; ++c) P->crt_k[c] -= 128 * 128 * nmod;
   {
     const int old = fegetround();
     volatile double one = 1.0, d = 16129.0, d1 = 127.0;

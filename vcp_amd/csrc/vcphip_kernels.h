@@ -91,16 +91,21 @@ int vcphip_fill_f64(double* p, long n, double v, hipStream_t s);
 #define VCPHIP_OZ_PADMN 256   // m and n are padded to multiples of this
 #define VCPHIP_OZ_PADK 128    // k is padded to a multiple of this
 #define VCPHIP_OZ_NLIMB 8     // 32-bit limbs of M (M < 2^256)
+#define VCPHIP_OZ_CRTCOL 48   // CRT columns: N weight bytes, 6 quotient bytes, the residue sum (3 x 16)
 
 struct VcphipOzParams {
   int nmod;                    // number of moduli N
   int beta;                    // scaled integers satisfy |A'| <= 2^beta (+1 for dd)
   int mod[VCPHIP_OZ_MAXMOD];   // pairwise coprime moduli (mod[0] = 256)
   double rcpd[VCPHIP_OZ_MAXMOD];
-  // CRT weights w_t = (M/m_t) ((M/m_t)^-1 mod m_t) < M: [c][g] packs byte c of
-  // w_{4g}, w_{4g+1}, w_{4g+2}, w_{4g+3} (low byte first); and w_t / M
-  uint32_t crt_w8[VCPHIP_OZ_MAXMOD][VCPHIP_OZ_MAXMOD / 4];
-  double crt_f[VCPHIP_OZ_MAXMOD];
+  // CRT on the matrix cores, with w_t = (M/m_t) ((M/m_t)^-1 mod m_t) < M and
+  // F_t = floor(2^48 w_t / M): crt_a[c][g] packs the int8 weights of
+  // residues t = 4g .. 4g + 3 (low byte first, 0 for t >= N) of column c:
+  // byte c of w_t minus 128 for c < N, byte c - N of F_t minus 128 for
+  // c = N .. N + 5, and 1 for c = N + 6 (the residue sum); crt_k[c] =
+  // 128 sum_t (byte) - 128^2 N for the byte columns c < N + 6
+  alignas(16) uint32_t crt_a[VCPHIP_OZ_CRTCOL][VCPHIP_OZ_MAXMOD / 4];
+  int32_t crt_k[VCPHIP_OZ_CRTCOL];
   // residues of dd magnitudes < 2^120 from their 16 bytes: the byte
   // weights 256^i mod m_t, four to a word ([t][q]: i = 4q .. 4q + 3, low
   // byte first) for v_dot4
