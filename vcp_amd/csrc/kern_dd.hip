@@ -126,24 +126,88 @@ VCPHIP_DEVINL void words4(u128 mag, unsigned (&p)[4]) {
   for (int i = 0; i < 4; ++i) p[i] = (unsigned)(mag >> (32 * i));
 }
 
-// the residue of (neg ? -mag : mag) in the symmetric int8 range: y = mag +
-// floor(m/2) (mod m) from four byte dot products of mag's 16 bytes with the
-// weights 256^i mod m (y < 2^20, exact in FP32); q = trunc(y * r_rcp) is
-// floor(y / m) or one less for m >= 16 in either rounding mode (r_rcp is
-// biased low), so r = y - q m lies in [0, 2m) and one unsigned minimum
-// brings it into [0, m); c = r - floor(m/2) is the centred residue of mag,
-// and -c that of -mag (for m = 256, c = -128 gives 128, the same int8 byte).
-VCPHIP_DEVINL int dd_residue8(const unsigned (&p)[4], bool neg, int t, const VcphipOzParams& P) {
-  const unsigned m = (unsigned)P.mod[t];
-  unsigned y = m >> 1;
+// The centred residues of the thread's four elements (magnitudes c[u] < 2^120
+// as four words, signs neg[u]) for every modulus: emit(t, word) receives the
+// word whose byte u is the residue of (neg[u] ? -mag : mag) modulo m_t in
+// [-floor(m_t/2), m_t - 1 - floor(m_t/2)] (for m_t = 256 the low byte of
+// +-mag).
+//   y_t = sum_i b_i (256^i mod m_t) over the 16 bytes b_i of mag (< 2^20) is
+// an int8 matrix product of centred operands: sum_i (b_i - 128)(w_ti - 128) +
+// 128 sum_i b_i + dd_kc[t].  One v_mfma_i32_32x32x32_i8 takes 64 elements
+// (one per lane) against 16 moduli: output row r is modulus 4 (r / 8) + r % 4
+// of element group (r / 4) % 2, so lane register v holds modulus 16 ch + v of
+// the lane's own element; the B operand is the lane's own 16 centred bytes
+// (k-slots of lane half h carry element group h) and the A operand of lane
+// half h is zero in the other group's rows.  Every lane must run it (the
+// split kernels have no early exit).
+//   q = floor((y + floor(m/2)) / m) = (8 (y + floor(m/2)) r_magic) >> 32 is
+// exact (8 (y + m/2) < 2^24: v_mul_hi_u32_u24), so c = y - q m lies in
+// [-floor(m/2), m - 1 - floor(m/2)] (v_mad_i32_i24), and -c for a negative
+// element is (c ^ -1) + 1 (v_xad_u32).  Moduli are taken 16 at a time, the
+// four elements' products of a group at once.
+VCPHIP_DEVINL unsigned mad_i24(unsigned a, int b, unsigned c) {   // a b + c, a and b signed 24-bit
+  unsigned d;
+  asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "s"(b), "v"(c));
+  return d;
+}
+VCPHIP_DEVINL unsigned xad(unsigned a, unsigned b, unsigned c) {   // (a ^ b) + c
+  unsigned d;
+  asm("v_xad_u32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+  return d;
+}
+
+template <int N, typename Emit>
+VCPHIP_DEVINL void dd_residue_words(const unsigned (&c)[4][4], const bool (&neg)[4], const VcphipOzParams& P,
+                                    Emit emit) {
+  constexpr int CH = (N + 15) / 16;
+  const unsigned lane = threadIdx.x & 63u, h = lane >> 5, r = lane & 31u;
+  const bool mine = ((r >> 2) & 1u) == h;
+  const unsigned cl = 4 * (r >> 3) + (r & 3u);
+  vcphip_v4i b[4];
+  int sb128[4];
+  unsigned sx[4], s1[4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) y = __builtin_amdgcn_udot4(p[i], P.dd_wb[t][i], y, false);
-  const unsigned q = (unsigned)((float)y * P.r_rcp[t]);
-  unsigned r = y - __umul24(q, m);
-  const unsigned r2 = r - m;
-  r = r < r2 ? r : r2;
-  const int c = (int)r - (int)(m >> 1);
-  return neg ? -c : c;
+  for (int u = 0; u < 4; ++u) {
+    unsigned sb = 0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      b[u][e] = (int)(c[u][e] ^ 0x80808080u);
+      sb = __builtin_amdgcn_udot4(c[u][e], 0x01010101u, sb, false);
+    }
+    sb128[u] = (int)(sb << 7);
+    sx[u] = neg[u] ? ~0u : 0u;
+    s1[u] = neg[u] ? 1u : 0u;
+  }
+#pragma unroll
+  for (int ch = 0; ch < CH; ++ch) {
+    const unsigned tl = 16 * ch + cl;
+    const bool use = mine && tl < (unsigned)N;
+    const vcphip_v4i w = *reinterpret_cast<const vcphip_v4i*>(&P.dd_wb[use ? tl : 0][0]);
+    vcphip_v4i a;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) a[e] = use ? (int)((unsigned)w[e] ^ 0x80808080u) : 0;
+    vcphip_v16i acc[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[u] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b[u], vcphip_v16i{}, 0, 0, 0);
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int t = 16 * ch + v;
+      if (t >= N) break;
+      const unsigned m = (unsigned)P.mod[t] & 0x1FFu;
+      const unsigned magic = P.r_magic[t] & 0xFFFFFFu, h8 = 8u * (m >> 1);
+      unsigned res[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const unsigned y = (unsigned)(acc[u][v] + sb128[u] + P.dd_kc[t]);   // < 2^20
+        const unsigned y8 = (y << 3) + h8;
+        const unsigned q = (unsigned)(((unsigned long long)(y8 & 0xFFFFFFu) * (unsigned long long)magic) >> 32);
+        res[u] = xad(mad_i24(q, -(int)m, y), sx[u], s1[u]);
+      }
+      const unsigned lo = __builtin_amdgcn_perm(res[1], res[0], 0x0c0c0400u);   // bytes: res0, res1, 0, 0
+      const unsigned hi = __builtin_amdgcn_perm(res[3], res[2], 0x04000c0cu);   // bytes: 0, 0, res2, res3
+      emit(t, lo | hi);
+    }
+  }
 }
 
 constexpr int SR_ROWS = 16, SR_K = 64, SR_LD = 68;   // one 1 KB plane block per workgroup and plane
@@ -177,13 +241,9 @@ __global__ __launch_bounds__(256) void k_dd_split_rows(const double* __restrict_
     dd_scaled(v[u].x, v[u].y, sc, neg[u], mag);
     words4(mag, c[u]);
   }
-#pragma unroll
-  for (int t = 0; t < N; ++t) {
-    unsigned word = 0;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) word |= ((unsigned)dd_residue8(c[u], neg[u], t, P) & 0xFFu) << (8 * u);
+  dd_residue_words<N>(c, neg, P, [&](int t, unsigned word) {
     *reinterpret_cast<unsigned*>(&s_tile[(t * SR_ROWS + lr) * SR_LD + lk0]) = word;
-  }
+  });
   __syncthreads();
   const int orow = tid / 16, oc = (tid % 16) * 4;
   const long plane_stride = (long)mpad * kpad;
@@ -222,13 +282,9 @@ __global__ __launch_bounds__(256) void k_dd_split_cols(const double* __restrict_
   }
   const long plane_stride = (long)npad * kpad;
   int8_t* dst = planes + vcphip_plane_off(j, k0, kpad);
-#pragma unroll
-  for (int z = 0; z < N; ++z) {
-    unsigned word = 0;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) word |= ((unsigned)dd_residue8(c[u], neg[u], z, P) & 0xFFu) << (8 * u);
+  dd_residue_words<N>(c, neg, P, [&](int z, unsigned word) {
     *reinterpret_cast<unsigned*>(dst + z * plane_stride) = word;
-  }
+  });
 }
 
 // ---------------------------------------------------------------------------

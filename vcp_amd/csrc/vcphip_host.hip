@@ -354,7 +354,9 @@ int oz_fill_params(int k, int nmod, int extra, int beta_cap, VcphipOzParams* P) 
   if (P->beta < 8) return VCPHIP_ERR_UNSUPPORTED;
   for (int t = 0; t < nmod; ++t) {
     const int m = P->mod[t];
-    if (m < 16) return VCPHIP_ERR_UNSUPPORTED;   // the FP32 quotient estimate of the splits needs m >= 16
+    // the FP32 quotient estimate of the interval split needs m >= 16, the
+    // 24-bit exact quotient of the dd split m > 32 (r_magic < 2^24)
+    if (m <= 32) return VCPHIP_ERR_UNSUPPORTED;
     P->rcpd[t] = 1.0 / (double)m;
     unsigned wb[16];   // 256^i mod m, i = 0..15 (each < 256)
     unsigned long long pw = 1 % (unsigned)m;
@@ -364,6 +366,9 @@ int oz_fill_params(int k, int nmod, int extra, int beta_cap, VcphipOzParams* P) 
     }
     for (int q = 0; q < 4; ++q)
       P->dd_wb[t][q] = wb[4 * q] | (wb[4 * q + 1] << 8) | (wb[4 * q + 2] << 16) | (wb[4 * q + 3] << 24);
+    P->dd_kc[t] = -128 * 128 * 16;
+    for (int i = 0; i < 16; ++i) P->dd_kc[t] += 128 * (int32_t)wb[i];
+    P->r_magic[t] = (uint32_t)(((1u << 29) + (unsigned)m - 1u) / (unsigned)m);
     for (int q = 0; q < 2; ++q) P->r_wb[q][t] = P->dd_wb[t][q];
     const unsigned two63 = (unsigned)(wb[7] * 128ull % (unsigned)m);   // 2^63 = 256^7 * 128
     P->r_bias[t] = ((unsigned)(m >> 1) + (unsigned)m - two63) % (unsigned)m;

@@ -108,8 +108,12 @@ struct VcphipOzParams {
   int32_t crt_k[VCPHIP_OZ_CRTCOL];
   // residues of dd magnitudes < 2^120 from their 16 bytes: the byte
   // weights 256^i mod m_t, four to a word ([t][q]: i = 4q .. 4q + 3, low
-  // byte first) for v_dot4
-  uint32_t dd_wb[VCPHIP_OZ_MAXMOD][4];
+  // byte first); dd_kc[t] = 128 sum_i (256^i mod m_t) - 128^2 16 (the
+  // centred-product constant); r_magic[t] = ceil(2^29 / m_t), with which
+  // floor(y / m_t) = (8 y r_magic[t]) >> 32 exactly for y < 2^21, m_t <= 256
+  alignas(16) uint32_t dd_wb[VCPHIP_OZ_MAXMOD][4];
+  int32_t dd_kc[VCPHIP_OZ_MAXMOD];
+  uint32_t r_magic[VCPHIP_OZ_MAXMOD];
   // residues of |x| <= 2^62 from the eight bytes of x + 2^63 (interval
   // split): the byte weights 256^i mod m_t, four to a word; r_bias =
   // (floor(m/2) - 2^63) mod m_t; r_rcp an FP32 value in
