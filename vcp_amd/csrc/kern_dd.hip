@@ -140,22 +140,10 @@ VCPHIP_DEVINL void words4(u128 mag, unsigned (&p)[4]) {
 // (k-slots of lane half h carry element group h) and the A operand of lane
 // half h is zero in the other group's rows.  Every lane must run it (the
 // split kernels have no early exit).
-//   q = floor((y + floor(m/2)) / m) = (8 (y + floor(m/2)) r_magic) >> 32 is
-// exact (8 (y + m/2) < 2^24: v_mul_hi_u32_u24), so c = y - q m lies in
-// [-floor(m/2), m - 1 - floor(m/2)] (v_mad_i32_i24), and -c for a negative
-// element is (c ^ -1) + 1 (v_xad_u32).  Moduli are taken 16 at a time, the
+//   q = floor((y + floor(m/2)) / m) = vcphip_quot24(8 (y + floor(m/2)), m) is
+// exact, so c = y - q m lies in [-floor(m/2), m - 1 - floor(m/2)]
+// (v_mad_i32_i24), and -c for a negative element is (c ^ -1) + 1 (v_xad_u32).  Moduli are taken 16 at a time, the
 // four elements' products of a group at once.
-VCPHIP_DEVINL unsigned mad_i24(unsigned a, int b, unsigned c) {   // a b + c, a and b signed 24-bit
-  unsigned d;
-  asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "s"(b), "v"(c));
-  return d;
-}
-VCPHIP_DEVINL unsigned xad(unsigned a, unsigned b, unsigned c) {   // (a ^ b) + c
-  unsigned d;
-  asm("v_xad_u32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
-  return d;
-}
-
 template <int N, typename Emit>
 VCPHIP_DEVINL void dd_residue_words(const unsigned (&c)[4][4], const bool (&neg)[4], const VcphipOzParams& P,
                                     Emit emit) {
@@ -199,13 +187,10 @@ VCPHIP_DEVINL void dd_residue_words(const unsigned (&c)[4][4], const bool (&neg)
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const unsigned y = (unsigned)(acc[u][v] + sb128[u] + P.dd_kc[t]);   // < 2^20
-        const unsigned y8 = (y << 3) + h8;
-        const unsigned q = (unsigned)(((unsigned long long)(y8 & 0xFFFFFFu) * (unsigned long long)magic) >> 32);
-        res[u] = xad(mad_i24(q, -(int)m, y), sx[u], s1[u]);
+        const unsigned q = vcphip_quot24((y << 3) + h8, magic);
+        res[u] = vcphip_xad(vcphip_mad_i24(q, -(int)m, y), sx[u], s1[u]);
       }
-      const unsigned lo = __builtin_amdgcn_perm(res[1], res[0], 0x0c0c0400u);   // bytes: res0, res1, 0, 0
-      const unsigned hi = __builtin_amdgcn_perm(res[3], res[2], 0x04000c0cu);   // bytes: 0, 0, res2, res3
-      emit(t, lo | hi);
+      emit(t, vcphip_pack4(res));
     }
   }
 }

@@ -214,22 +214,22 @@ VCPHIP_DEVINL Bytes8 bytes8(long long x) {
   return Bytes8{(unsigned)u, (unsigned)(u >> 32) ^ 0x80000000u};
 }
 
-// the residue of x in [-floor(m/2), m - 1 - floor(m/2)] (the symmetric int8
-// range; for m = 256 that is [-128, 127]).  y = x + floor(m/2) (mod m) from
-// two byte dot products of x + 2^63 with the weights 256^i mod m, started
-// at the bias (floor(m/2) - 2^63) mod m (y < 2^19, exact in FP32);
-// q = trunc(RU(y * r_rcp)) is floor(y / m) or one less (r_rcp is biased
-// low), so r = y - q m lies in [0, 2m) and one unsigned minimum brings it
-// into [0, m).  Round-up mode.
-VCPHIP_DEVINL int sym_residue8(const Bytes8& b, int t, const VcphipOzParams& P) {
-  const unsigned m = (unsigned)P.mod[t];
-  unsigned y = __builtin_amdgcn_udot4(b.lo, P.r_wb[0][t], P.r_bias[t], false);
-  y = __builtin_amdgcn_udot4(b.hi, P.r_wb[1][t], y, false);
-  const unsigned q = (unsigned)((float)y * P.r_rcp[t]);
-  unsigned r = y - __umul24(q, m);
-  const unsigned r2 = r - m;
-  r = r < r2 ? r : r2;
-  return (int)r - (int)(m >> 1);
+// the residues of the four elements' x in [-floor(m/2), m - 1 - floor(m/2)]
+// (the symmetric int8 range; for m = 256 that is [-128, 127]) as the bytes of
+// one word.  y = x (mod m) from two byte dot products of x + 2^63 with the
+// weights 256^i mod m, started at the bias ((floor(m/2) - 2^63) mod m) -
+// floor(m/2), so y + floor(m/2) lies in [0, 2^19); q = floor((y +
+// floor(m/2)) / m) is exact (vcphip_quot24) and c = y - q m is the residue.
+VCPHIP_DEVINL unsigned sym_residue_word(const Bytes8 (&b)[4], int t, const VcphipOzParams& P) {
+  const unsigned m = (unsigned)P.mod[t] & 0x1FFu, h8 = 8u * (m >> 1);
+  unsigned res[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    unsigned y = __builtin_amdgcn_udot4(b[u].lo, P.r_wb[0][t], P.r_bias[t], false);
+    y = __builtin_amdgcn_udot4(b[u].hi, P.r_wb[1][t], y, false);
+    res[u] = vcphip_mad_i24(vcphip_quot24((y << 3) + h8, P.r_magic[t]), -(int)m, y);
+  }
+  return vcphip_pack4(res);
 }
 
 constexpr int SR_ROWS = 16, SR_K = 64, SR_LD = 68;   // LDS byte tile (one plane block), odd-word row stride
@@ -282,13 +282,8 @@ __global__ __launch_bounds__(256) void k_oz_split_rows(const double* __restrict_
     }
     // modulus by modulus: only one modulus' constants are live at a time
 #pragma unroll
-    for (int t = 0; t < N; ++t) {
-      unsigned word = 0;
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        word |= ((unsigned)sym_residue8(ch[u], t, P) & 0xFFu) << (8 * u);
-      *reinterpret_cast<unsigned*>(&s_tile[(t * SR_ROWS + lr) * SR_LD + lk0]) = word;
-    }
+    for (int t = 0; t < N; ++t)
+      *reinterpret_cast<unsigned*>(&s_tile[(t * SR_ROWS + lr) * SR_LD + lk0]) = sym_residue_word(ch, t, P);
     *reinterpret_cast<unsigned*>(&s_tile[(N * SR_ROWS + lr) * SR_LD + lk0]) = dsh;
     *reinterpret_cast<unsigned*>(&s_tile[((N + 1) * SR_ROWS + lr) * SR_LD + lk0]) = dsl;
     *reinterpret_cast<unsigned*>(&s_tile[((N + 2) * SR_ROWS + lr) * SR_LD + lk0]) = drh;
@@ -351,13 +346,7 @@ __global__ __launch_bounds__(256) void k_oz_split_cols(const double* __restrict_
   const long plane_stride = (long)npad * kpad;
   int8_t* dst = planes + vcphip_plane_off(j, k0, kpad);
 #pragma unroll
-  for (int z = 0; z < N; ++z) {
-    unsigned word = 0;
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-      word |= ((unsigned)sym_residue8(ch[u], z, P) & 0xFFu) << (8 * u);
-    *reinterpret_cast<unsigned*>(dst + z * plane_stride) = word;
-  }
+  for (int z = 0; z < N; ++z) *reinterpret_cast<unsigned*>(dst + z * plane_stride) = sym_residue_word(ch, z, P);
   // the digit planes the radius scheme reads (the low digits are unused by
   // the one-digit schemes)
   if (P.dmask_b & 1) *reinterpret_cast<unsigned*>(dst + N * plane_stride) = drh;

@@ -33,6 +33,33 @@ VCPHIP_DEVINL void vcphip_lds_dma16(const void* g, uint32_t lds_base) {
   asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g), "s"(lds_base) : "memory", "m0");
 }
 
+// Small integer helpers of the residue kernels.
+// a b + c with a and b taken as signed 24-bit values (v_mad_i32_i24)
+VCPHIP_DEVINL unsigned vcphip_mad_i24(unsigned a, int b, unsigned c) {
+  unsigned d;
+  asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "s"(b), "v"(c));
+  return d;
+}
+// (a ^ b) + c (v_xad_u32): with b = -1, c = 1 the negation of a
+VCPHIP_DEVINL unsigned vcphip_xad(unsigned a, unsigned b, unsigned c) {
+  unsigned d;
+  asm("v_xad_u32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+  return d;
+}
+// floor(y / m) = (8 y magic) >> 32 for magic = ceil(2^29 / m), y8 = 8 y <
+// 2^24 and m <= 256: with magic = 2^29 / m + d, d < 1, the product is
+// 8 y / m + 8 y d / 2^32 < y / m + 2^-8, and frac(y / m) <= 1 - 1/m leaves
+// room for that for m < 256 (d = 0 for m = 256)  (v_mul_hi_u32_u24)
+VCPHIP_DEVINL unsigned vcphip_quot24(unsigned y8, unsigned magic) {
+  return (unsigned)(((unsigned long long)(y8 & 0xFFFFFFu) * (unsigned long long)(magic & 0xFFFFFFu)) >> 32);
+}
+// the low bytes of r[0..3] as one word, r[0] lowest
+VCPHIP_DEVINL unsigned vcphip_pack4(const unsigned (&r)[4]) {
+  const unsigned lo = __builtin_amdgcn_perm(r[1], r[0], 0x0c0c0400u);   // bytes r0, r1, 0, 0
+  const unsigned hi = __builtin_amdgcn_perm(r[3], r[2], 0x04000c0cu);   // bytes 0, 0, r2, r3
+  return lo | hi;
+}
+
 // Raw buffer resource (stride 0, gfx9 word 3) over `bytes` bytes from a
 // wave-uniform base: loads through it take a 32-bit lane offset in bytes and
 // need no per-lane 64-bit address arithmetic; offsets at or past `bytes`

@@ -354,9 +354,7 @@ int oz_fill_params(int k, int nmod, int extra, int beta_cap, VcphipOzParams* P) 
   if (P->beta < 8) return VCPHIP_ERR_UNSUPPORTED;
   for (int t = 0; t < nmod; ++t) {
     const int m = P->mod[t];
-    // the FP32 quotient estimate of the interval split needs m >= 16, the
-    // 24-bit exact quotient of the dd split m > 32 (r_magic < 2^24)
-    if (m <= 32) return VCPHIP_ERR_UNSUPPORTED;
+    if (m <= 32) return VCPHIP_ERR_UNSUPPORTED;   // the splits' 24-bit quotient needs r_magic < 2^24
     P->rcpd[t] = 1.0 / (double)m;
     unsigned wb[16];   // 256^i mod m, i = 0..15 (each < 256)
     unsigned long long pw = 1 % (unsigned)m;
@@ -371,13 +369,7 @@ int oz_fill_params(int k, int nmod, int extra, int beta_cap, VcphipOzParams* P) 
     P->r_magic[t] = (uint32_t)(((1u << 29) + (unsigned)m - 1u) / (unsigned)m);
     for (int q = 0; q < 2; ++q) P->r_wb[q][t] = P->dd_wb[t][q];
     const unsigned two63 = (unsigned)(wb[7] * 128ull % (unsigned)m);   // 2^63 = 256^7 * 128
-    P->r_bias[t] = ((unsigned)(m >> 1) + (unsigned)m - two63) % (unsigned)m;
-    {
-      const double v = (1.0 / m) * (1.0 - ldexp(1.0, -22));
-      float f = (float)v;
-      if ((double)f > v) f = nextafterf(f, 0.0f);
-      P->r_rcp[t] = f;
-    }
+    P->r_bias[t] = (uint32_t)((int)(((unsigned)(m >> 1) + (unsigned)m - two63) % (unsigned)m) - (m >> 1));
   }
   uint32_t M[VCPHIP_OZ_NLIMB] = {1, 0, 0, 0, 0, 0, 0, 0};
   for (int t = 0; t < nmod; ++t) {

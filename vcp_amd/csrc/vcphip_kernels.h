@@ -116,11 +116,9 @@ struct VcphipOzParams {
   uint32_t r_magic[VCPHIP_OZ_MAXMOD];
   // residues of |x| <= 2^62 from the eight bytes of x + 2^63 (interval
   // split): the byte weights 256^i mod m_t, four to a word; r_bias =
-  // (floor(m/2) - 2^63) mod m_t; r_rcp an FP32 value in
-  // [(1 - 2^-21) / m, (1 - 2^-22) / m]
+  // ((floor(m/2) - 2^63) mod m_t) - floor(m/2) (two's complement)
   uint32_t r_wb[2][VCPHIP_OZ_MAXMOD];
   uint32_t r_bias[VCPHIP_OZ_MAXMOD];
-  float r_rcp[VCPHIP_OZ_MAXMOD];
   uint32_t Mlimb[VCPHIP_OZ_NLIMB];  // M
   double c1;                   // RU(1/127)
   double c1dn;                 // RD(1/127)
