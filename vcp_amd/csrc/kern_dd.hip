@@ -26,15 +26,24 @@ __global__ __launch_bounds__(256) void k_dd_stats_part(const double* __restrict_
   const int r = blockIdx.x * 256 + threadIdx.x;
   const int c = blockIdx.y;
   if (r >= rows) return;
-  const int c0 = c * VCPHIP_OZ_CHUNK, c1 = min(c0 + VCPHIP_OZ_CHUNK, len);
+  const int c0 = c * VCPHIP_DD_CHUNK, c1 = min(c0 + VCPHIP_DD_CHUNK, len);
   double mu = 0.0;
   bool nonfinite = false;
-  for (int cc = c0; cc < c1; ++cc) {
-    const double2 v = *reinterpret_cast<const double2*>(X + 2 * ((long)r * rs + (long)cc * cs));
+  auto take = [&](const double2 v) {
     nonfinite |= !(__builtin_isfinite(v.x) && __builtin_isfinite(v.y));
     const double a = __builtin_fabs(v.x) + __builtin_fabs(v.y);
     mu = a > mu ? a : mu;
+  };
+  const double* x = X + 2 * (long)r * rs;
+  int cc = c0;
+  for (; cc + 3 < c1; cc += 4) {   // four loads in flight per lane
+    double2 v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = *reinterpret_cast<const double2*>(x + 2 * (long)(cc + q) * cs);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) take(v[q]);
   }
+  for (; cc < c1; ++cc) take(*reinterpret_cast<const double2*>(x + 2 * (long)cc * cs));
   // INT32_MAX marks a non-finite element (it wins the maximum over chunks)
   part[(long)c * rows + r] = nonfinite ? INT32_MAX : (mu > 0.0 ? vcphip_fexp(mu) : INT32_MIN);
 }
@@ -64,12 +73,20 @@ __global__ __launch_bounds__(256) void k_dd_stats_lines(const double* __restrict
   const double* x = X + 2 * (long)r * rs;
   double mu = 0.0;
   int nonfinite = 0;
-  for (int cc = lane; cc < len; cc += 64) {
-    const double2 v = *reinterpret_cast<const double2*>(x + 2 * (long)cc);
+  auto take = [&](const double2 v) {
     nonfinite |= !(__builtin_isfinite(v.x) && __builtin_isfinite(v.y));
     const double a = __builtin_fabs(v.x) + __builtin_fabs(v.y);
     mu = a > mu ? a : mu;
+  };
+  int cc = lane;
+  for (; cc + 192 < len; cc += 256) {   // four loads in flight per lane
+    double2 v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = *reinterpret_cast<const double2*>(x + 2 * (long)(cc + 64 * q));
+#pragma unroll
+    for (int q = 0; q < 4; ++q) take(v[q]);
   }
+  for (; cc < len; cc += 64) take(*reinterpret_cast<const double2*>(x + 2 * (long)cc));
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     const double o = __shfl_xor(mu, off);
@@ -793,7 +810,7 @@ int vcphip_dd_stats(const double* X, long rs, long cs, int rows, int len, int* e
     hipLaunchKernelGGL(k_dd_stats_lines, dim3((rows + 3) / 4), dim3(256), 0, s, X, rs, rows, len, e, flags);
     return hipGetLastError() == hipSuccess ? 0 : -3;
   }
-  const int nchunk = len > 0 ? (len + VCPHIP_OZ_CHUNK - 1) / VCPHIP_OZ_CHUNK : 0;
+  const int nchunk = len > 0 ? (len + VCPHIP_DD_CHUNK - 1) / VCPHIP_DD_CHUNK : 0;
   if (nchunk > 0)
     hipLaunchKernelGGL(k_dd_stats_part, dim3((rows + 255) / 256, nchunk), dim3(256), 0, s, X, rs, cs, rows, len,
                        nchunk, part);

@@ -1023,10 +1023,16 @@ __global__ __launch_bounds__(512) void k_oz_gemm6(int mpad, int npad, int kpad, 
 // ---------------------------------------------------------------------------
 // reconstruction
 // ---------------------------------------------------------------------------
-constexpr int NL = 5;   // 32-bit limbs: M < 2^160 for N <= 20
+// 32-bit limbs of X: M <= 256^N, so |X| < M/2 fits ceil(N / 4) limbs with
+// its sign bit to spare
+template <int N>
+constexpr int oz_limbs() {
+  return (N + 3) / 4;
+}
 
 // |X| * 2^s rounded toward zero (tz) and away from zero (aw), exactly: both
 // come from one 64-bit window of the limbs and the sticky bit below it
+template <int NL>
 VCPHIP_DEVINL void conv_limbs2(const unsigned (&mag)[NL], int s, double& tz, double& aw) {
   int top = -1;
 #pragma unroll
@@ -1096,6 +1102,7 @@ __global__ __launch_bounds__(256) void k_oz_recon(int m, int n, int k, int mpad,
 
   VcphipCrtOps<N> T;
   vcphip_crt_ops_load<N>(P, T);
+  constexpr int NL = oz_limbs<N>();
   unsigned mag[NL];
   const bool neg = vcphip_crt_symmetric<N, NL>(U + bbase, plane, o, T, P, mag);
   const int fa = sa.flags[i], fb = sb.flags[j];

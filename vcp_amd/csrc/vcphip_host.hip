@@ -973,7 +973,7 @@ static int dd_gemm_dev(vcphip_ctx* c, bool tn, int m, int n, int k, const double
   const int mpad = (m + VCPHIP_OZ_PADMN - 1) / VCPHIP_OZ_PADMN * VCPHIP_OZ_PADMN;
   const int npad = (n + VCPHIP_OZ_PADMN - 1) / VCPHIP_OZ_PADMN * VCPHIP_OZ_PADMN;
   const int kpad = k > 0 ? (k + VCPHIP_OZ_PADK - 1) / VCPHIP_OZ_PADK * VCPHIP_OZ_PADK : VCPHIP_OZ_PADK;
-  const int nchunk = k > 0 ? (k + VCPHIP_OZ_CHUNK - 1) / VCPHIP_OZ_CHUNK : 1;
+  const int nchunk = k > 0 ? (k + VCPHIP_DD_CHUNK - 1) / VCPHIP_DD_CHUNK : 1;
   void *pa = nullptr, *pb, *pu, *st;
   if (!tn && (rc = ws_get(c, vcphip_ctx::WS_PA, (size_t)nmod * mpad * kpad, &pa))) return rc;
   if ((rc = ws_get(c, vcphip_ctx::WS_PB, (size_t)nmod * npad * kpad, &pb))) return rc;

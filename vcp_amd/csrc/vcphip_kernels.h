@@ -88,6 +88,7 @@ int vcphip_fill_f64(double* p, long n, double v, hipStream_t s);
 // against B's rounded-up digit: 4), ceil 2 (rounded-up digits: 2)
 #define VCPHIP_OZ_MAXRAD 6
 #define VCPHIP_OZ_CHUNK 256   // k-chunk of the upward 1-norm partial sums
+#define VCPHIP_DD_CHUNK 64    // k-chunk of the dd row maxima (a maximum: any split gives the same result)
 #define VCPHIP_OZ_PADMN 256   // m and n are padded to multiples of this
 #define VCPHIP_OZ_PADK 128    // k is padded to a multiple of this
 #define VCPHIP_OZ_NLIMB 8     // 32-bit limbs of M (M < 2^256)
