@@ -21,7 +21,27 @@ def _dd(rng, r, c, kind="uniform"):
     if kind == "zeros":
         x[::4] = 0.0
         x[:, ::5] = 0.0
+    if kind == "integers":   # exact products: sums with zero low limbs, lo = 0
+        x[..., 0] = rng.integers(-8, 9, (r, c)).astype(np.float64)
+        x[..., 1] = 0.0
     return x
+
+
+@pytest.mark.parametrize("kind,sa,sb", [("integers", 0, 0), ("uniform", -540, -540), ("uniform", -600, -470),
+                                        ("uniform", 500, 515), ("wide", -520, -530)])
+def test_dd_bitexact_conversion_edges(ctx, kind, sa, sb):
+    """exact integer sums, gradual underflow of hi and lo, results next to
+    the overflow threshold: the reconstruction's roundings on every path"""
+    rng = np.random.default_rng(hash((kind, sa, sb)) % 2 ** 32)
+    A, B = _dd(rng, 40, 96, kind), _dd(rng, 96, 36, kind)
+    A, B = np.ldexp(A, sa), np.ldexp(B, sb)
+    C = ctx.ddgemm(A, B)
+    beta_ok = [nm for nm in range(24, 33) if O.ozaki_dd_beta(96, nm) >= 110]
+    R = O.ozaki_ddgemm(A, B, beta_ok[0])
+    same = (C == R) | (np.isnan(C) & np.isnan(R))
+    same &= np.signbit(C) == np.signbit(R)
+    bad = np.argwhere(~same.all(axis=2))
+    assert len(bad) == 0, "mismatch at %s: gpu %s oracle %s" % (bad[:3].tolist(), C[tuple(bad[0])], R[tuple(bad[0])])
 
 
 @pytest.mark.parametrize("m,n,k", [(1, 1, 1), (16, 16, 64), (70, 33, 100), (256, 256, 256), (300, 257, 513)])

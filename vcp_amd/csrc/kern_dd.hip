@@ -12,6 +12,7 @@
 #include "vcphip_device.h"
 #include "vcphip_kernels.h"
 #include "vcphip_crt.h"
+#include "vcphip_ddconv.h"
 
 namespace {
 
@@ -293,72 +294,7 @@ __global__ __launch_bounds__(256) void k_dd_split_cols(const double* __restrict_
 // reconstruction: CRT -> 256-bit integer -> nearest (hi, lo)
 // ---------------------------------------------------------------------------
 constexpr int NL = VCPHIP_OZ_NLIMB;
-
-VCPHIP_DEVINL int top_bit(const unsigned (&x)[NL]) {
-  int top = -1;
-#pragma unroll
-  for (int l = NL - 1; l >= 0; --l)
-    if (top < 0 && x[l] != 0) top = 32 * l + 31 - __builtin_clz(x[l]);
-  return top;
-}
-
-// bits [q, q+64) of x (q >= 0)
-VCPHIP_DEVINL unsigned long long bits64(const unsigned (&x)[NL], int q) {
-  const int ls = q >> 5, bs = q & 31;
-  unsigned w0 = 0, w1 = 0, w2 = 0;
-#pragma unroll
-  for (int l = 0; l < NL; ++l) {
-    if (l == ls) w0 = x[l];
-    if (l == ls + 1) w1 = x[l];
-    if (l == ls + 2) w2 = x[l];
-  }
-  const unsigned long long lo = (unsigned long long)w0 | ((unsigned long long)w1 << 32);
-  return bs ? ((lo >> bs) | ((unsigned long long)w2 << (64 - bs))) : lo;
-}
-
-VCPHIP_DEVINL bool bit_at(const unsigned (&x)[NL], int b) {
-  bool v = false;
-  if (b < 0) return false;
-#pragma unroll
-  for (int l = 0; l < NL; ++l)
-    if (l == (b >> 5)) v = (x[l] >> (b & 31)) & 1u;
-  return v;
-}
-
-VCPHIP_DEVINL bool any_below(const unsigned (&x)[NL], int b) {   // any bit < b set
-  bool v = false;
-#pragma unroll
-  for (int l = 0; l < NL; ++l) {
-    const int lo = 32 * l;
-    if (b >= lo + 32) v = v || x[l] != 0;
-    else if (b > lo) v = v || (x[l] & ((1u << (b - lo)) - 1u)) != 0;
-  }
-  return v;
-}
-
-// nearest double of x 2^s (ties to even); q_out = lowest kept bit, up_out = rounded up
-VCPHIP_DEVINL double rn_limbs(const unsigned (&x)[NL], int s, int& q_out, bool& up_out) {
-  const int top = top_bit(x);
-  q_out = 0;
-  up_out = false;
-  if (top < 0) return 0.0;
-  int q = top - 52;
-  if (q < -1074 - s) q = -1074 - s;
-  if (q <= 0) {
-    q_out = 0;
-    return __builtin_ldexp((double)bits64(x, 0), s);
-  }
-  unsigned long long mant = q < 32 * NL ? bits64(x, q) : 0ull;
-  const bool rb = bit_at(x, q - 1);
-  const bool st = any_below(x, q - 1);
-  const bool up = rb && (st || (mant & 1ull));
-  mant += up ? 1ull : 0ull;
-  q_out = q;
-  up_out = up;
-  const int ex = q + s;
-  if (mant != 0 && ex + (64 - __builtin_clzll(mant)) > 1024) return __builtin_inf();
-  return __builtin_ldexp((double)mant, ex);
-}
+static_assert(NL == vcphip_ddconv::NL, "limb count of the conversion");
 
 template <int N>
 __global__ __launch_bounds__(256) void k_dd_recon(int m, int n, int mpad, int npad, VcphipOzParams P,
@@ -388,41 +324,8 @@ __global__ __launch_bounds__(256) void k_dd_recon(int m, int n, int mpad, int np
     return;
   }
   const int sc = ea[i] + eb[j] - 2 * P.beta;
-  int q;
-  bool up;
-  const double hi = rn_limbs(mag, sc, q, up);
-  // remainder |X| - round(|X|) at bit q: low bits, or 2^q - low bits if rounded up
-  unsigned rem[NL];
-  bool rem_neg = false;
-  if (q <= 0) {
-#pragma unroll
-    for (int l = 0; l < NL; ++l) rem[l] = 0;
-  } else {
-#pragma unroll
-    for (int l = 0; l < NL; ++l) {
-      const int lo = 32 * l;
-      unsigned w = mag[l];
-      if (q <= lo) w = 0;
-      else if (q < lo + 32) w &= (1u << (q - lo)) - 1u;
-      rem[l] = w;
-    }
-    if (up) {   // rem = 2^q - rem
-      rem_neg = true;
-      long long borrow = 0;
-#pragma unroll
-      for (int l = 0; l < NL; ++l) {
-        const int lo = 32 * l;
-        const unsigned long long twoq = (q >= lo && q < lo + 32) ? (1ull << (q - lo)) : 0ull;
-        const long long d = (long long)twoq - (long long)rem[l] - borrow;
-        borrow = d < 0 ? 1 : 0;
-        rem[l] = (unsigned)(d + (borrow << 32));
-      }
-    }
-  }
-  int q2;
-  bool up2;
-  double lo = rn_limbs(rem, sc, q2, up2);
-  if (rem_neg) lo = -lo;
+  double hi, lo;
+  vcphip_dd_from_limbs(mag, sc, hi, lo);
   const double sgn = neg ? -1.0 : 1.0;
   if (live) *reinterpret_cast<double2*>(C + 2 * (i + ldc * (long)j)) = make_double2(sgn * hi, sgn * lo);
 }

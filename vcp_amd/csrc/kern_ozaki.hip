@@ -1125,16 +1125,20 @@ __global__ __launch_bounds__(256) void k_oz_recon(int m, int n, int k, int mpad,
   // numerator times the RU or, when negative, the RD reciprocal.
   double t1 = 0.0, t2 = 0.0;
   const bool ceil_only = P.rscheme == VCPHIP_OZ_RADIUS_CEIL, full = P.rscheme == VCPHIP_OZ_RADIUS_FULL;
+  // the directed reciprocals as register values (a per-lane choice between
+  // two parameter fields would otherwise become a per-lane load)
+  const double c1 = vcphip_uniform(P.c1), c1dn = vcphip_uniform(P.c1dn);
+  const double c127 = vcphip_uniform(P.c127), c127dn = vcphip_uniform(P.c127dn);
   auto zload = [&](int z) {   // digit product z of this element
     return (int)__builtin_amdgcn_raw_buffer_load_b32(vcphip_rsrc(Zb + z * plane, 1024), 4 * o, 0, 0);
   };
   auto digit_sum = [&](int zh, int zm, int zl) {
     if (ceil_only) return (double)zload(zh);
     const int mx = zload(zm);
-    double u = (double)zload(zh) + (double)mx * (mx >= 0 ? P.c1 : P.c1dn);
+    double u = (double)zload(zh) + (double)mx * (mx >= 0 ? c1 : c1dn);
     if (full) {
       const int ll = zload(zl);
-      u = u + (double)ll * (ll >= 0 ? P.c127 : P.c127dn);
+      u = u + (double)ll * (ll >= 0 ? c127 : c127dn);
     }
     return u;
   };

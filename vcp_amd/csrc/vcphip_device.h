@@ -33,6 +33,15 @@ VCPHIP_DEVINL void vcphip_lds_dma16(const void* g, uint32_t lds_base) {
   asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g), "s"(lds_base) : "memory", "m0");
 }
 
+// A wave-uniform double as an opaque register value (v_readfirstlane of its
+// halves), so that selects between such values stay selects of values.
+VCPHIP_DEVINL double vcphip_uniform(double x) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 // Small integer helpers of the residue kernels.
 // a b + c with a and b taken as signed 24-bit values (v_mad_i32_i24)
 VCPHIP_DEVINL unsigned vcphip_mad_i24(unsigned a, int b, unsigned c) {
