@@ -233,6 +233,22 @@ def test_ozaki_moduli_and_beta():
             assert k * 2 ** (2 * beta + 5) >= M or beta == 62  # and not wastefully small
 
 
+def test_split_quotient_24bit_exact():
+    """the splits' residue quotient: floor(y / m) = (8 y ceil(2^29 / m)) >> 32
+    (v_mul_hi_u32_u24: both factors below 2^24) for every y < 2^21 and every
+    modulus 33 <= m <= 256; a plausible slip (floor instead of ceil, 2^28)
+    fails it"""
+    y = np.arange(2 ** 21, dtype=np.uint64)
+    for m in range(33, 257):
+        magic = np.uint64(-(-(2 ** 29) // m))
+        assert int(magic) < 2 ** 24
+        q = ((np.uint64(8) * y) * magic) >> np.uint64(32)
+        assert np.array_equal(q, y // np.uint64(m)), m
+    m = 255
+    slip = ((np.uint64(8) * y) * np.uint64(2 ** 29 // m)) >> np.uint64(32)
+    assert not np.array_equal(slip, y // np.uint64(m))
+
+
 def test_crt_symmetric_reconstruction_identity():
     """the reconstruction the kernels use: X = sum_t u_t w_t - q M with
     w_t = (M/m_t)((M/m_t)^-1 mod m_t) and q = floor((sum_t u_t F_t + 2^47) /
