@@ -40,7 +40,7 @@ typedef int vcphip_v16i __attribute__((ext_vector_type(16)));
 // The per-lane A operands (weights) of the CRT products.
 template <int N>
 struct VcphipCrtOps {
-  static constexpr int NC = N + 7;           // N weight bytes, 6 quotient bytes, the residue sum
+  static constexpr int NC = N + 7;           // 6 quotient bytes, the residue sum, N weight bytes
   static constexpr int CH = (NC + 15) / 16;  // 16-column groups
   static constexpr int RH = (N + 15) / 16;   // 16-residue groups
   static_assert(16 * CH <= VCPHIP_OZ_CRTCOL && 4 * RH <= VCPHIP_OZ_MAXMOD / 4, "CRT table");
@@ -78,28 +78,38 @@ VCPHIP_DEVINL bool vcphip_crt_from_residues(const unsigned (&u)[N], const Vcphip
         if (16 * rh + 4 * e + y < N) x |= u[16 * rh + 4 * e + y] << (8 * y);
       b[rh][e] = (int)(x ^ 0x80808080u);   // u - 128 as int8 (bytes past N meet zero weights)
     }
-  vcphip_v16i acc[CH];
+  // column group 0 first: its columns 0..6 are the quotient bytes and the
+  // residue sum, so q is known before the weight columns (7 .. N + 6) of the
+  // later groups are formed, and one group's accumulators are live at a time
+  vcphip_v16i acc = vcphip_v16i{};
 #pragma unroll
-  for (int ch = 0; ch < CH; ++ch) {
-    acc[ch] = vcphip_v16i{};
-#pragma unroll
-    for (int rh = 0; rh < RH; ++rh) acc[ch] = __builtin_amdgcn_mfma_i32_32x32x32_i8(T.a[ch][rh], b[rh], acc[ch], 0, 0, 0);
-  }
-  auto D = [&](int c) { return acc[c >> 4][c & 15]; };
-  const int s128 = 128 * (D(N + 6) + 128 * N);   // 128 sum_t u_t
-  auto col = [&](int c) { return (unsigned)(D(c) + s128 + P.crt_k[c]); };   // < 2^21
+  for (int rh = 0; rh < RH; ++rh) acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(T.a[0][rh], b[rh], acc, 0, 0, 0);
+  const int s128 = 128 * (acc[6] + 128 * N);   // 128 sum_t u_t
   unsigned long long Q = 1ull << 47;
 #pragma unroll
-  for (int c = 0; c < 6; ++c) Q += (unsigned long long)col(N + c) << (8 * c);
+  for (int c = 0; c < 6; ++c) Q += (unsigned long long)(unsigned)(acc[c] + s128 + P.crt_k[c]) << (8 * c);
   const long long q = (long long)(Q >> 48);
+  long long limb[NL];
+#pragma unroll
+  for (int l = 0; l < NL; ++l) limb[l] = -q * (long long)P.Mlimb[l];
+#pragma unroll
+  for (int ch = 0; ch < CH; ++ch) {
+    if (ch > 0) {
+      acc = vcphip_v16i{};
+#pragma unroll
+      for (int rh = 0; rh < RH; ++rh) acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(T.a[ch][rh], b[rh], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int c = 16 * ch + v - 7;   // weight byte c
+      if (c < 0 || c >= N) continue;
+      limb[c >> 2] += (long long)(unsigned)(acc[v] + s128 + P.crt_k[16 * ch + v]) << (8 * (c & 3));   // < 2^21
+    }
+  }
   long long carry = 0;
 #pragma unroll
   for (int l = 0; l < NL; ++l) {
-    long long acc_l = carry;
-#pragma unroll
-    for (int e = 0; e < 4; ++e)
-      if (4 * l + e < N) acc_l += (long long)col(4 * l + e) << (8 * e);
-    acc_l -= q * (long long)P.Mlimb[l];
+    const long long acc_l = limb[l] + carry;
     mag[l] = (unsigned)acc_l;
     carry = acc_l >> 32;
   }

@@ -410,11 +410,12 @@ int oz_fill_params(int k, int nmod, int extra, int beta_cap, VcphipOzParams* P) 
       P->crt_a[c][t / 4] |= v << (8 * (t % 4));
       if (centred) P->crt_k[c] += 128 * (int32_t)byte;
     };
-    for (int c = 0; c < nmod; ++c) put(c, (wt[c / 4] >> (8 * (c % 4))) & 0xFFu, true);   // w_t < M < 2^(8 N)
-    for (int b = 0; b < 6; ++b) put(nmod + b, (unsigned)(F >> (8 * b)) & 0xFFu, true);
-    put(nmod + 6, 1u, false);
+    for (int b = 0; b < 6; ++b) put(b, (unsigned)(F >> (8 * b)) & 0xFFu, true);
+    put(6, 1u, false);
+    for (int c = 0; c < nmod; ++c) put(7 + c, (wt[c / 4] >> (8 * (c % 4))) & 0xFFu, true);   // w_t < M < 2^(8 N)
   }
-  for (int c = 0; c < nmod + 6; ++c) P->crt_k[c] -= 128 * 128 * nmod;
+  for (int c = 0; c < nmod + 7; ++c)
+    if (c != 6) P->crt_k[c] -= 128 * 128 * nmod;
   {
     const int old = fegetround();
     volatile double one = 1.0, d = 16129.0, d1 = 127.0;

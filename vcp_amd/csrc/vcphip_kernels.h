@@ -102,9 +102,9 @@ struct VcphipOzParams {
   // CRT on the matrix cores, with w_t = (M/m_t) ((M/m_t)^-1 mod m_t) < M and
   // F_t = floor(2^48 w_t / M): crt_a[c][g] packs the int8 weights of
   // residues t = 4g .. 4g + 3 (low byte first, 0 for t >= N) of column c:
-  // byte c of w_t minus 128 for c < N, byte c - N of F_t minus 128 for
-  // c = N .. N + 5, and 1 for c = N + 6 (the residue sum); crt_k[c] =
-  // 128 sum_t (byte) - 128^2 N for the byte columns c < N + 6
+  // byte c of F_t minus 128 for c < 6, 1 for c = 6 (the residue sum), and
+  // byte c - 7 of w_t minus 128 for c = 7 .. N + 6; crt_k[c] =
+  // 128 sum_t (byte) - 128^2 N for the byte columns (c != 6)
   alignas(16) uint32_t crt_a[VCPHIP_OZ_CRTCOL][VCPHIP_OZ_MAXMOD / 4];
   int32_t crt_k[VCPHIP_OZ_CRTCOL];
   // residues of dd magnitudes < 2^120 from their 16 bytes: the byte
