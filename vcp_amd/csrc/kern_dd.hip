@@ -192,24 +192,27 @@ VCPHIP_DEVINL void dd_residue_words(const unsigned (&c)[4][4], const bool (&neg)
     vcphip_v4i a;
 #pragma unroll
     for (int e = 0; e < 4; ++e) a[e] = use ? (int)((unsigned)w[e] ^ 0x80808080u) : 0;
-    vcphip_v16i acc[4];
+    // element by element (one accumulator live), each residue inserted as
+    // byte u of its modulus' word
+    unsigned word[16];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) acc[u] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b[u], vcphip_v16i{}, 0, 0, 0);
+    for (int u = 0; u < 4; ++u) {
+      const vcphip_v16i acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b[u], vcphip_v16i{}, 0, 0, 0);
+      const unsigned sel = 0x03020100u | (0x04u << (8 * u));   // word's bytes, byte u from the residue
 #pragma unroll
-    for (int v = 0; v < 16; ++v) {
-      const int t = 16 * ch + v;
-      if (t >= N) break;
-      const unsigned m = (unsigned)P.mod[t] & 0x1FFu;
-      const unsigned magic = P.r_magic[t] & 0xFFFFFFu, h8 = 8u * (m >> 1);
-      unsigned res[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const unsigned y = (unsigned)(acc[u][v] + sb128[u] + P.dd_kc[t]);   // < 2^20
-        const unsigned q = vcphip_quot24((y << 3) + h8, magic);
-        res[u] = vcphip_xad(vcphip_mad_i24(q, -(int)m, y), sx[u], s1[u]);
+      for (int v = 0; v < 16; ++v) {
+        const int t = 16 * ch + v;
+        if (t >= N) break;
+        const unsigned m = (unsigned)P.mod[t] & 0x1FFu, h8 = 8u * (m >> 1);
+        const unsigned y = (unsigned)(acc[v] + sb128[u] + P.dd_kc[t]);   // < 2^20
+        const unsigned q = vcphip_quot24((y << 3) + h8, P.r_magic[t]);
+        const unsigned res = vcphip_xad(vcphip_mad_i24(q, -(int)m, y), sx[u], s1[u]);
+        word[v] = u == 0 ? res : __builtin_amdgcn_perm(res, word[v], (sel & ~(0xFFu << (8 * u))) | (0x04u << (8 * u)));
       }
-      emit(t, vcphip_pack4(res));
     }
+#pragma unroll
+    for (int v = 0; v < 16; ++v)
+      if (16 * ch + v < N) emit(16 * ch + v, word[v]);
   }
 }
 
