@@ -58,7 +58,7 @@ def test_dd_bitexact(ctx, m, n, k, kind):
     del nmod
 
 
-@pytest.mark.parametrize("nmod", [26, 30, 32])
+@pytest.mark.parametrize("nmod", [24, 25, 26, 30, 32])
 def test_dd_nmod_bitexact(ctx, nmod):
     rng = np.random.default_rng(nmod)
     A, B = _dd(rng, 64, 200), _dd(rng, 200, 48)

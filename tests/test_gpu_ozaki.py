@@ -46,7 +46,7 @@ def test_ozaki_bitexact(ctx, m, n, k, kind):
     assert len(bad) == 0, "first mismatches %s gpu %s oracle %s" % (bad[:3].tolist(), C[tuple(bad[0])], R[tuple(bad[0])])
 
 
-@pytest.mark.parametrize("nmod", [12, 14, 18, 20])
+@pytest.mark.parametrize("nmod", [8, 9, 10, 11, 12, 14, 17, 18, 20])   # one to four CRT products per wave
 def test_ozaki_nmod_bitexact(ctx, nmod):
     rng = np.random.default_rng(nmod)
     IA, IB = _iv(rng, 128, 200, "wide"), _iv(rng, 200, 96, "wide")
