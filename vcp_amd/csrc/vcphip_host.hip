@@ -326,15 +326,6 @@ int oz_moduli(int nmod, int* out) {
   return cnt;
 }
 
-int oz_beta(int k, int nmod, const int* mods) {
-  double lm = 0.0;
-  for (int t = 0; t < nmod; ++t) lm += log2((double)mods[t]);
-  int lk = 0;
-  while ((1ll << lk) < (long long)(k > 1 ? k : 1)) ++lk;
-  const int beta = (int)floor((lm - 1.0 - (double)lk - 1e-6) / 2.0);
-  return beta > 62 ? 62 : beta;
-}
-
 // CRT constants shared by the interval and dd forms; beta from the no-wrap
 // condition k (2^beta + slack)^2 < M/2, i.e. k 2^(2 beta + extra) < M
 int oz_fill_params(int k, int nmod, int extra, int beta_cap, VcphipOzParams* P) {
