@@ -369,12 +369,15 @@ __global__ __launch_bounds__(256) void k_dd_sub_from(int m, int n, const double*
 }
 
 // hi parts of a dd matrix: D(i, j) = X(i, j).hi, D with leading dimension m
+// D(i, j) = hi(X(rows[i], j)), rows = identity when null (a row permutation
+// with entries in [0, m), checked by the caller)
 __global__ __launch_bounds__(256) void k_dd_hi(int m, int n, const double* __restrict__ X, long ldx,
-                                               double* __restrict__ D) {
+                                               const int* __restrict__ rows, double* __restrict__ D) {
   const long idx = (long)blockIdx.x * 256 + threadIdx.x;
   if (idx >= (long)m * n) return;
   const int i = (int)(idx % m), j = (int)(idx / m);
-  D[idx] = X[2 * (i + ldx * (long)j)];
+  const int r = rows ? rows[i] : i;
+  D[idx] = X[2 * (r + ldx * (long)j)];
 }
 
 // the refinement step of pddblas::linearsolve (vcp/pddblas.hpp:450-457):
@@ -649,7 +652,12 @@ __global__ __launch_bounds__(256) void k_max_pairs(int nb, const double* __restr
 
 int vcphip_dd_hi(int m, int n, const double* X, long ldx, double* D, hipStream_t s) {
   if ((long)m * n == 0) return 0;
-  hipLaunchKernelGGL(k_dd_hi, dim3((unsigned)(((long)m * n + 255) / 256)), dim3(256), 0, s, m, n, X, ldx, D);
+  return vcphip_dd_hi_rows(m, n, X, ldx, nullptr, D, s);
+}
+
+int vcphip_dd_hi_rows(int m, int n, const double* X, long ldx, const int* rows, double* D, hipStream_t s) {
+  if ((long)m * n <= 0) return 0;
+  hipLaunchKernelGGL(k_dd_hi, dim3((unsigned)(((long)m * n + 255) / 256)), dim3(256), 0, s, m, n, X, ldx, rows, D);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

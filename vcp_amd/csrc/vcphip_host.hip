@@ -1678,6 +1678,29 @@ extern "C" int vcphip_dd_solve_dev(vcphip_ctx* c, int n, int nrhs, const double*
       hipStreamSynchronize(s) != hipSuccess)
     return VCPHIP_ERR_DEVICE;
   if (hinfo != 0) return VCPHIP_ERR_SINGULAR;   // the reference's lapack_error("dgetrf")
+  // the row interchanges as one permutation, applied while taking hi(r):
+  // LAPACK's getrs would apply them with a row-swap pass per call, which
+  // with n right-hand sides (the inverse) took longer than the two
+  // triangular solves
+  {
+    std::vector<rocblas_int> hp(n);
+    if (hipMemcpyAsync(hp.data(), piv, sizeof(rocblas_int) * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return VCPHIP_ERR_DEVICE;
+    std::vector<int> perm(n);
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    for (int i = 0; i < n; ++i) {
+      const int j = hp[i] - 1;
+      if (j < i || j >= n) return VCPHIP_ERR_DEVICE;   // getrf pivots lie in [i + 1, n] (1-based)
+      std::swap(perm[i], perm[j]);
+    }
+    if (hipMemcpyAsync(piv, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return VCPHIP_ERR_DEVICE;
+  }
+  static_assert(sizeof(rocblas_int) == sizeof(int), "the permutation reuses the pivot buffer");
+  const int* rowperm = (const int*)piv;
+  const double one = 1.0;
   T.mark("dd_solve_lu");
   // x = 0, r = b (:438-439)
   if (hipMemset2DAsync(x, 16 * (size_t)ldx, 0, 16 * (size_t)n, nrhs, s) != hipSuccess) return fail(VCPHIP_ERR_DEVICE);
@@ -1687,8 +1710,11 @@ extern "C" int vcphip_dd_solve_dev(vcphip_ctx* c, int n, int nrhs, const double*
   int it = 0;
   while (it < itmax) {
     // d = LU \ hi(r); x += dd(d); the norms (:446-458)
-    if (vcphip_dd_hi(n, nrhs, rc, rld, d, s) ||
-        rocsolver_dgetrs(c->blas, rocblas_operation_none, n, nrhs, LU, n, piv, d, n) != rocblas_status_success ||
+    if (vcphip_dd_hi_rows(n, nrhs, rc, rld, rowperm, d, s) ||
+        rocblas_dtrsm(c->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_unit,
+                      n, nrhs, &one, LU, n, d, n) != rocblas_status_success ||
+        rocblas_dtrsm(c->blas, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                      rocblas_diagonal_non_unit, n, nrhs, &one, LU, n, d, n) != rocblas_status_success ||
         vcphip_dd_add_d(n, nrhs, d, x, ldx, part, norms, s))
       return fail(VCPHIP_ERR_DEVICE);
     double h[2];

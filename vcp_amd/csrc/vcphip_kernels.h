@@ -199,6 +199,8 @@ int vcphip_dd_sub_from(int m, int n, const double* B, long ldb, double* R, long 
 // dd solve (pddblas::linearsolve): hi parts, and x += dd(d) with max |d|, max |x.hi|
 constexpr int VCPHIP_DD_ADD_BLOCKS = 1024;
 int vcphip_dd_hi(int m, int n, const double* X, long ldx, double* D, hipStream_t s);
+// D(i, j) = hi(X(rows[i], j)) (rows: a permutation of 0 .. m-1 on the device)
+int vcphip_dd_hi_rows(int m, int n, const double* X, long ldx, const int* rows, double* D, hipStream_t s);
 int vcphip_dd_add_d(int m, int n, const double* d, double* X, long ldx, double* part, double* out, hipStream_t s);
 // dd Cholesky refinement (pddblas::Cholesky): U from dpotrf's upper part, W
 // from G, U += F (optionally zeroing U's lower part), the symmetry check
