@@ -374,6 +374,48 @@ __global__ __launch_bounds__(256) void k_oz_split_cols(const double* __restrict_
 // the 16-byte fragment reads are bank-conflict free.  Each wave fills whole
 // 1 KB segments (16 rows): lane l writes segment byte 16 l, i.e. row l / 4,
 // physical chunk l % 4, which holds logical chunk (l % 4) ^ ((l / 16) % 4).
+// The residue-plane products reduced mod m into [0, m), four to a packed
+// word.  For kpad <= 8192 (|acc| <= kpad 2^14 <= 2^27) in integer
+// arithmetic: t = acc + K 2^16 >= 0 (K = ceil(2^11 / m) m, so t < 2^29 and
+// t >> 16 < 4352) is congruent to z = t + (t >> 16)(c16 - 2^16) =
+// (t >> 16) c16 + (t mod 2^16) < 2^21 with c16 = 2^16 mod m (one
+// v_mad_i32_i24), and z mod m = z - q m with q the exact 24-bit quotient:
+// six VALU instructions per product.  Larger k: a double-precision floor
+// division (by an approximate reciprocal) with two fix-ups.  `fast` is uniform, so each launch takes one
+// branch.
+struct OzModPlane {
+  bool fast;
+  int m, c16m;
+  unsigned kbias, magic;
+  double rc;
+  VCPHIP_DEVINL OzModPlane(const VcphipOzParams& P, int z, int kpad)
+      : fast(kpad <= 8192), m(P.mod[z]), c16m(P.ep_c16[z]), kbias(P.ep_bias[z]), magic(P.r_magic[z]),
+        rc(fast ? 0.0 : 1.0 / (double)P.mod[z]) {}
+  VCPHIP_DEVINL unsigned pack(int a0, int a1, int a2, int a3) const {
+    const int a[4] = {a0, a1, a2, a3};
+    unsigned r[4];
+    if (fast) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const unsigned t = (unsigned)a[v] + kbias;
+        const unsigned zz = vcphip_mad_i24(t >> 16, c16m, t);
+        r[v] = vcphip_mad_i24(vcphip_quot24(zz << 3, magic), -m, zz);
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double y = (double)a[v];
+        const double qd = __builtin_floor(y * rc);
+        int rr = (int)(y - qd * (double)m);
+        rr += (rr < 0) ? m : 0;
+        rr -= (rr >= m) ? m : 0;
+        r[v] = (unsigned)rr;
+      }
+    }
+    return vcphip_pack4(r);
+  }
+};
+
 template <int FM, int FN>
 __global__ __launch_bounds__(256) void k_oz_gemm3(int mpad, int npad, int kpad, int nmod, int nz,
                                                   const int8_t* __restrict__ Aplanes,
@@ -470,24 +512,14 @@ __global__ __launch_bounds__(256) void k_oz_gemm3(int mpad, int npad, int kpad, 
 
   const long blocks_m = mpad / 16;
   if (z < nmod) {
-    const int md = P.mod[z];
-    const double rc = P.rcpd[z];
+    const OzModPlane mod(P, z, kpad);
     uint8_t* Uz = U + (long)z * mpad * npad;
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
       for (int j = 0; j < FN; ++j) {
         const long bi = (row0 + wm * WM + 16 * i) / 16, bj = (col0 + wn * WN + 16 * j) / 16;
-        unsigned packed = 0;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const double y = (double)acc[i][j][v];
-          const double qd = __builtin_floor(y * rc);
-          int rr = (int)(y - qd * (double)md);
-          rr += (rr < 0) ? md : 0;
-          rr -= (rr >= md) ? md : 0;
-          packed |= (unsigned)rr << (8 * v);
-        }
+        const unsigned packed = mod.pack(acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]);
         *reinterpret_cast<unsigned*>(Uz + (bj * blocks_m + bi) * 256 + lane * 4) = packed;
       }
   } else {
@@ -650,24 +682,14 @@ __global__ __launch_bounds__(256) void k_oz_gemm4(int mpad, int npad, int kpad, 
 
   const long blocks_m = mpad / 16;
   if (z < nmod) {
-    const int md = P.mod[z];
-    const double rc = P.rcpd[z];
+    const OzModPlane mod(P, z, kpad);
     uint8_t* Uz = U + (long)z * mpad * npad;
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
       for (int j = 0; j < FN; ++j) {
         const long bi = (row0 + wm * WM + 16 * i) / 16, bj = (col0 + wn * WN + 16 * j) / 16;
-        unsigned packed = 0;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const double y = (double)acc[i][j][v];
-          const double qd = __builtin_floor(y * rc);
-          int rr = (int)(y - qd * (double)md);
-          rr += (rr < 0) ? md : 0;
-          rr -= (rr >= md) ? md : 0;
-          packed |= (unsigned)rr << (8 * v);
-        }
+        const unsigned packed = mod.pack(acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]);
         *reinterpret_cast<unsigned*>(Uz + (bj * blocks_m + bi) * 256 + lane * 4) = packed;
       }
   } else {
@@ -823,8 +845,7 @@ __global__ __launch_bounds__(64 * NW) void k_oz_gemm5(int mpad, int npad, int kp
   const long blocks_m = mpad / 16;
   const int l2 = 16 * (kq) + (lane & 15), chalf = (lane & 31) >> 4;
   if (z < nmod) {
-    const int md = P.mod[z];
-    const double rc = P.rcpd[z];
+    const OzModPlane mod(P, z, kpad);
     uint8_t* Uz = U + (long)z * mpad * npad;
 #pragma unroll
     for (int i = 0; i < FM; ++i)
@@ -834,16 +855,8 @@ __global__ __launch_bounds__(64 * NW) void k_oz_gemm5(int mpad, int npad, int kp
         for (int q = 0; q < 4; ++q) {
           const long bi = (row0 + wm * WM + 32 * i) / 16 + (q >> 1);
           const long bj = (col0 + wn * WN + 32 * j) / 16 + chalf;
-          unsigned packed = 0;
-#pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            const double y = (double)acc[i][j][4 * q + t];
-            const double qd = __builtin_floor(y * rc);
-            int rr = (int)(y - qd * (double)md);
-            rr += (rr < 0) ? md : 0;
-            rr -= (rr >= md) ? md : 0;
-            packed |= (unsigned)rr << (8 * t);
-          }
+          const unsigned packed =
+              mod.pack(acc[i][j][4 * q], acc[i][j][4 * q + 1], acc[i][j][4 * q + 2], acc[i][j][4 * q + 3]);
           *reinterpret_cast<unsigned*>(Uz + (bj * blocks_m + bi) * 256 + (l2 + 32 * (q & 1)) * 4) = packed;
         }
   } else {
@@ -977,8 +990,7 @@ __global__ __launch_bounds__(512) void k_oz_gemm6(int mpad, int npad, int kpad, 
   const long blocks_m = mpad / 16;
   const int l2 = 16 * (kq) + (lane & 15), chalf = (lane & 31) >> 4;
   if (z < nmod) {
-    const int md = P.mod[z];
-    const double rc = P.rcpd[z];
+    const OzModPlane mod(P, z, kpad);
     uint8_t* Uz = U + (long)z * mpad * npad;
 #pragma unroll
     for (int i = 0; i < FM; ++i)
@@ -989,16 +1001,8 @@ __global__ __launch_bounds__(512) void k_oz_gemm6(int mpad, int npad, int kpad, 
         for (int q = 0; q < 4; ++q) {
           const long bi = (row0 + wm * WM + 32 * i) / 16 + (q >> 1);
           const long bj = (col0 + wn * WN + 32 * j) / 16 + chalf;
-          unsigned packed = 0;
-#pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            const double y = (double)acc[i][j][4 * q + t];
-            const double qd = __builtin_floor(y * rc);
-            int rr = (int)(y - qd * (double)md);
-            rr += (rr < 0) ? md : 0;
-            rr -= (rr >= md) ? md : 0;
-            packed |= (unsigned)rr << (8 * t);
-          }
+          const unsigned packed =
+              mod.pack(acc[i][j][4 * q], acc[i][j][4 * q + 1], acc[i][j][4 * q + 2], acc[i][j][4 * q + 3]);
           *reinterpret_cast<unsigned*>(Uz + (bj * blocks_m + bi) * 256 + (l2 + 32 * (q & 1)) * 4) = packed;
         }
       }

@@ -346,7 +346,6 @@ int oz_fill_params(int k, int nmod, int extra, int beta_cap, VcphipOzParams* P) 
   for (int t = 0; t < nmod; ++t) {
     const int m = P->mod[t];
     if (m <= 32) return VCPHIP_ERR_UNSUPPORTED;   // the splits' 24-bit quotient needs r_magic < 2^24
-    P->rcpd[t] = 1.0 / (double)m;
     unsigned wb[16];   // 256^i mod m, i = 0..15 (each < 256)
     unsigned long long pw = 1 % (unsigned)m;
     for (int i = 0; i < 16; ++i) {
@@ -358,6 +357,8 @@ int oz_fill_params(int k, int nmod, int extra, int beta_cap, VcphipOzParams* P) 
     P->dd_kc[t] = -128 * 128 * 16;
     for (int i = 0; i < 16; ++i) P->dd_kc[t] += 128 * (int32_t)wb[i];
     P->r_magic[t] = (uint32_t)(((1u << 29) + (unsigned)m - 1u) / (unsigned)m);
+    P->ep_bias[t] = (uint32_t)((2048 + m - 1) / m * m) << 16;
+    P->ep_c16[t] = (int32_t)(65536 % m) - 65536;
     for (int q = 0; q < 2; ++q) P->r_wb[q][t] = P->dd_wb[t][q];
     const unsigned two63 = (unsigned)(wb[7] * 128ull % (unsigned)m);   // 2^63 = 256^7 * 128
     P->r_bias[t] = (uint32_t)((int)(((unsigned)(m >> 1) + (unsigned)m - two63) % (unsigned)m) - (m >> 1));

@@ -98,7 +98,6 @@ struct VcphipOzParams {
   int nmod;                    // number of moduli N
   int beta;                    // scaled integers satisfy |A'| <= 2^beta (+1 for dd)
   int mod[VCPHIP_OZ_MAXMOD];   // pairwise coprime moduli (mod[0] = 256)
-  double rcpd[VCPHIP_OZ_MAXMOD];
   // CRT on the matrix cores, with w_t = (M/m_t) ((M/m_t)^-1 mod m_t) < M and
   // F_t = floor(2^48 w_t / M): crt_a[c][g] packs the int8 weights of
   // residues t = 4g .. 4g + 3 (low byte first, 0 for t >= N) of column c:
@@ -115,6 +114,10 @@ struct VcphipOzParams {
   alignas(16) uint32_t dd_wb[VCPHIP_OZ_MAXMOD][4];
   int32_t dd_kc[VCPHIP_OZ_MAXMOD];
   uint32_t r_magic[VCPHIP_OZ_MAXMOD];
+  // the GEMM epilogue's reduction mod m_t (k <= 8192): ep_bias[t] =
+  // ceil(2^11 / m_t) m_t 2^16 and ep_c16[t] = (2^16 mod m_t) - 2^16
+  uint32_t ep_bias[VCPHIP_OZ_MAXMOD];
+  int32_t ep_c16[VCPHIP_OZ_MAXMOD];
   // residues of |x| <= 2^62 from the eight bytes of x + 2^63 (interval
   // split): the byte weights 256^i mod m_t, four to a word; r_bias =
   // ((floor(m/2) - 2^63) mod m_t) - floor(m/2) (two's complement)
