@@ -249,6 +249,33 @@ def test_split_quotient_24bit_exact():
     assert not np.array_equal(slip, y // np.uint64(m))
 
 
+def test_gemm_epilogue_mod_reduction():
+    """the int8 GEMM epilogue's reduction of |acc| <= 2^27 modulo m: t = acc +
+    ceil(2^11/m) m 2^16, z = t + (t >> 16)((2^16 mod m) - 2^16), q from the
+    24-bit quotient of 8z, r = z - q m equals acc mod m, with t < 2^29 and
+    z < 2^21 (the operand ranges of v_mad_i32_i24 and v_mul_hi_u32_u24), for
+    every modulus 33 <= m <= 256 on the extremes, the multiples of m around
+    them and random sums"""
+    rng = np.random.default_rng(27)
+    base = np.concatenate([np.array([0, 1, -1, 2 ** 27, -2 ** 27, 2 ** 27 - 1, -2 ** 27 + 1], dtype=np.int64),
+                           rng.integers(-2 ** 27, 2 ** 27 + 1, 200000)])
+    for m in range(33, 257):
+        near = np.array([k * m + d for k in (2 ** 27 // m, -(2 ** 27 // m), 0, 1, -1) for d in (-1, 0, 1)], dtype=np.int64)
+        acc = np.concatenate([base, near[np.abs(near) <= 2 ** 27]])
+        kbias = ((2048 + m - 1) // m * m) << 16
+        c16m = (65536 % m) - 65536
+        magic = -(-(2 ** 29) // m)
+        t = acc + kbias
+        assert t.min() >= 0 and t.max() < 2 ** 29
+        th = t >> 16
+        assert th.max() < 2 ** 23
+        z = t + th * c16m
+        assert z.min() >= 0 and z.max() < 2 ** 21
+        q = ((8 * z) * magic) >> 32
+        r = z - q * m
+        assert np.array_equal(r, np.mod(acc, m)), m
+
+
 def test_crt_symmetric_reconstruction_identity():
     """the reconstruction the kernels use: X = sum_t u_t w_t - q M with
     w_t = (M/m_t)((M/m_t)^-1 mod m_t) and q = floor((sum_t u_t F_t + 2^47) /
