@@ -43,7 +43,8 @@ VCPHIP_DEVINL double vcphip_uniform(double x) {
 }
 
 // Small integer helpers of the residue kernels.
-// a b + c with a and b taken as signed 24-bit values (v_mad_i32_i24)
+// a b + c with a and b taken as signed 24-bit values (v_mad_i32_i24); b is
+// an SGPR operand and must be wave-uniform (the callers pass modulus constants)
 VCPHIP_DEVINL unsigned vcphip_mad_i24(unsigned a, int b, unsigned c) {
   unsigned d;
   asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "s"(b), "v"(c));
