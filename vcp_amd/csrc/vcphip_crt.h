@@ -24,11 +24,13 @@
 // register v is column v of its own element.  Its B operand is the lane's own
 // 16 centred residues (k-slots of lane half h carry element group h), and the
 // A operand of lane half h is zero in the rows of the other group, which
-// keeps the two groups apart.  A seventh extra column of ones gives
-// sum_t (u_t - 128).  This replaces O(N^2 / 4) byte dot products and N
-// double-precision multiply-adds per element by (N + 7) / 16 x N / 16
-// matrix instructions per 64 elements.  Every lane of the wave must execute
-// it (the kernels keep all lanes through it and mask their stores).
+// keeps the two groups apart.  The table's columns are the six quotient
+// bytes (0 .. 5), a column of ones (6), which gives sum_t (u_t - 128), and
+// the N weight bytes (7 .. N + 6).  This replaces O(N^2 / 4) byte dot
+// products and N double-precision multiply-adds per element by
+// ceil((N + 7) / 16) x ceil(N / 16) matrix instructions per 64 elements.
+// Every lane of the wave must execute them (the kernels keep all lanes
+// through them and mask their stores).
 #pragma once
 
 #include "vcphip_device.h"
